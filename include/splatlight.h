@@ -1,0 +1,118 @@
+/* splatlight_hip: the C ABI of the MI355X splat frame path.
+ *
+ * The reference reaches the GPU through WebGPU from its JavaScript host; this ABI is what a thin N-API addon
+ * (or ctypes) binds in place of those WebGPU calls for the splat frame path. All entry points are extern "C",
+ * take plain pointers and sizes, and return an sl_status. Host byte buffers are copied before the call returns
+ * (the caller keeps ownership, as queue.writeBuffer does). Device work is enqueued on the context's HIP stream.
+ *
+ * Errors: a call that fails returns a non-zero status and sets sl_last_error(ctx). A HIP (device) error
+ * poisons the context: every later call on it returns SL_ERR_POISONED, with no retry (the containment of
+ * src/main.js:74-77).
+ */
+#ifndef SPLATLIGHT_H
+#define SPLATLIGHT_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int sl_status;
+#define SL_OK 0
+#define SL_ERR_INVALID 1       /* bad argument */
+#define SL_ERR_LAYOUT 2        /* record stride / offset / count / byte-size mismatch (src/asset.js:18-38) */
+#define SL_ERR_DEVICE 3        /* a HIP call failed; the context is now poisoned */
+#define SL_ERR_POISONED 4      /* an earlier device error poisoned this context */
+#define SL_ERR_UNSUPPORTED 5   /* a frame feature outside this path (motion blur, LOD, refraction, ...) */
+#define SL_ERR_BUSY 6          /* the GPU session is held elsewhere */
+#define SL_ERR_CAPACITY 7      /* a budget or size limit would be exceeded */
+
+typedef struct sl_context sl_context;
+typedef struct sl_scene sl_scene;
+
+/* ---- GPU session: replaces acquireGpuSession (src/gpu-session.js:3-14) ----------------------------------------
+ * An exclusive, non-blocking lock named `name` (NULL: "gs-game-engine-gpu"). Fails at once with SL_ERR_BUSY and
+ * the reference's message when another holder has it. */
+sl_status sl_session_acquire(const char *name, int64_t *token);
+sl_status sl_session_release(int64_t token);
+const char *sl_session_error(void);
+
+/* ---- context: one device, one HIP stream (the reference's device + queue, src/main.js:56-62) ----------------- */
+sl_status sl_ctx_create(int device_ordinal, sl_context **out);
+void sl_ctx_destroy(sl_context *ctx);
+const char *sl_last_error(const sl_context *ctx);
+sl_status sl_ctx_status(const sl_context *ctx);
+sl_status sl_sync(sl_context *ctx);      /* wait for the context's stream (onSubmittedWorkDone) */
+
+/* ---- device buffers (createBuffer / writeBuffer / mapAsync read) ------------------------------------------- */
+sl_status sl_buf_alloc(sl_context *ctx, size_t bytes, void **dev);
+sl_status sl_buf_free(sl_context *ctx, void *dev);
+sl_status sl_buf_write(sl_context *ctx, void *dev, size_t offset, const void *host, size_t bytes);
+sl_status sl_buf_read(sl_context *ctx, const void *dev, size_t offset, void *host, size_t bytes);
+
+/* ---- scene: the asset records the renderer and lighting upload (src/renderer.js:54-62, src/asset.js:8-16) --- */
+#define SL_STORAGE_F32 0u
+#define SL_STORAGE_F16 1u
+#define SL_STORAGE_BF16 2u
+typedef struct sl_scene_desc {
+  const void *gaussians; uint32_t gaussian_count; uint32_t gaussian_stride;   /* 64 B records, <= 2^24 */
+  const void *parts; uint32_t part_count; uint32_t part_stride;               /* 128 B records, <= 32 */
+  const void *materials; uint32_t material_count; uint32_t material_stride;   /* 32 B records (may be 0) */
+  const void *nodes; uint32_t node_count; uint32_t node_stride;               /* 96 B leaf nodes (NULL: derived) */
+  const void *vgrid; uint32_t vgrid_count; uint32_t vgrid_stride;             /* 128 B grids (NULL: V = 1) */
+  const void *emitters; uint32_t emitter_count;                               /* world-space emissive records */
+  uint32_t storage;                                                           /* SL_STORAGE_* */
+} sl_scene_desc;
+sl_status sl_scene_create(sl_context *ctx, const sl_scene_desc *desc, sl_scene **out);
+void sl_scene_destroy(sl_context *ctx, sl_scene *scene);
+/* SplatRenderer.writeParts (src/renderer.js:407-409): the whole part table, part_count * 128 B */
+sl_status sl_scene_write_parts(sl_context *ctx, sl_scene *scene, const void *part_bytes, size_t bytes);
+/* the lit colour per Gaussian (vec4) the renderer reads (bindColour, src/renderer.js:264-268); host copy */
+sl_status sl_scene_write_colour(sl_context *ctx, sl_scene *scene, const float *rgba, size_t count);
+/* the bounce per Gaussian (3 vec4 each, lo_b); NULL clears it (bounce gain then adds nothing) */
+sl_status sl_scene_write_bounce(sl_context *ctx, sl_scene *scene, const float *lo_b, size_t count);
+size_t sl_scene_device_bytes(const sl_scene *scene);
+
+/* ---- frame: SplatRenderer.encode (src/renderer.js:411-524) up to the linear HDR target ---------------------- */
+#define SL_OUT_RGBA32F 0u
+#define SL_OUT_RGBA16F 1u      /* the reference's rgba16float target (src/renderer.js:13) */
+#define SL_FRAME_TIMERS 1u     /* record hipEvent pairs per pass (sl_timers_read) */
+typedef struct sl_frame_opts {
+  uint32_t width, height;      /* must equal the RenderFrame viewport */
+  uint32_t out_format;         /* SL_OUT_* */
+  uint32_t flags;              /* SL_FRAME_* */
+} sl_frame_opts;
+/* Enqueue one frame: project, depth order, pair emission, tile sort, tile ranges, composite. out_device holds
+ * width * height pixels of out_format, rows top-down. frame_bytes: the 1200 B RenderFrame. */
+sl_status sl_frame_encode(sl_context *ctx, sl_scene *scene, const void *frame_bytes, size_t frame_len,
+                          const sl_frame_opts *opts, void *out_device);
+typedef struct sl_frame_stats {
+  uint32_t slots;        /* splat slots projected: count + 1 + emitters */
+  uint32_t visible;      /* valid depth keys (draw_args[1], src/main.js:723) */
+  uint32_t pairs;        /* (splat, tile) pairs */
+  uint32_t overflow;     /* pairs that did not fit (the frame was re-rendered with larger buffers if > 0) */
+  uint32_t tiles;        /* 16x16 tiles */
+  uint32_t pad[3];
+} sl_frame_stats;
+/* Wait for the last frame and report its counts. Re-runs the frame once with grown pair buffers when the
+ * pair count exceeded them. */
+sl_status sl_frame_finish(sl_context *ctx, sl_frame_stats *stats);
+
+/* Artefacts of the last frame, for parity checks: */
+#define SL_ART_KEYS 1u         /* u32 per slot: 16-bit depth key or 0xFFFFFFFF (project.wgsl:366-368) */
+#define SL_ART_ORDER 2u        /* u32 per slot: stable order by key, INVALID last (radix.wgsl) */
+#define SL_ART_SPLATS 3u       /* 48 B per slot (shaders/common.wgsl:29-35); invalid slots undefined */
+#define SL_ART_PAIR_TILES 4u   /* u32 per pair: tile id, ascending */
+#define SL_ART_PAIR_SLOTS 5u   /* u32 per pair: splat slot, each tile's list in back-to-front order */
+#define SL_ART_RANGES 6u       /* 2 u32 per tile: [start, end) into the pair lists */
+sl_status sl_frame_read(sl_context *ctx, uint32_t which, void *host, size_t bytes, size_t *written);
+
+/* Per-pass GPU milliseconds of the last frame encoded with SL_FRAME_TIMERS; names as in the reference's PASSES
+ * (src/renderer.js:7-8) where one exists. Fills up to `cap` entries; returns the count in *n. */
+sl_status sl_timers_read(sl_context *ctx, const char **names, float *ms, uint32_t cap, uint32_t *n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

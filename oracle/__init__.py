@@ -1,0 +1,124 @@
+"""CPU oracle for the splat frame path: TEST INFRASTRUCTURE ONLY.
+
+A ctypes wrapper around liboracle.so (plain C restatement of the reference's WGSL/JS, see oracle.h). Only
+tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this, as the checker; the product
+path never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+_lib = None
+
+
+def build():
+    r = subprocess.run(["make", "-s", "-C", HERE], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + r.stdout + r.stderr)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        vp, u32 = C.c_void_p, C.c_uint32
+        L.or_depth_key.restype = u32
+        L.or_depth_key.argtypes = [C.c_float, C.c_float, C.c_float]
+        L.or_project_f32.restype = u32
+        L.or_project_f32.argtypes = [vp, vp, vp, vp, vp, vp, u32, vp, vp]
+        L.or_project_f64.restype = None
+        L.or_project_f64.argtypes = [vp, vp, vp, vp, vp, vp, u32, vp, vp]
+        L.or_sort_keys.restype = u32
+        L.or_sort_keys.argtypes = [vp, u32, vp, vp, vp]
+        L.or_tile_box.restype = u32
+        L.or_tile_box.argtypes = [vp, u32, u32, u32, vp]
+        L.or_tile_lists.restype = C.c_uint64
+        L.or_tile_lists.argtypes = [vp, vp, u32, u32, u32, u32, vp, vp]
+        L.or_composite.restype = None
+        L.or_composite.argtypes = [vp, vp, vp, u32, u32, u32, C.c_int, vp]
+        L.or_composite_f64.restype = None
+        L.or_composite_f64.argtypes = [vp, vp, vp, u32, u32, u32, vp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def depth_key(depth: float, near: float, far: float) -> int:
+    return int(lib().or_depth_key(depth, near, far))
+
+
+def project(frame: bytes, scene, colour=None, lo_b=None, f64=False):
+    """Returns (splats u32 [slots, 12], keys u32 [slots], visible) or, f64, (splats f64 [slots, 12], valid)."""
+    fb = np.frombuffer(frame, np.uint8).copy()
+    g = np.ascontiguousarray(scene.gaussians)
+    p = np.ascontiguousarray(scene.parts)
+    e = np.ascontiguousarray(scene.emitters) if scene.emitters is not None else np.zeros(0, g.dtype)
+    n_emit = e.shape[0]
+    nslots = g.shape[0] + 1 + n_emit
+    col = None if colour is None else np.ascontiguousarray(colour, np.float32).reshape(-1, 4)
+    lb = None if lo_b is None else np.ascontiguousarray(lo_b, np.float32).reshape(-1, 12)
+    if f64:
+        out = np.zeros((nslots, 12), np.float64)
+        valid = np.zeros(nslots, np.uint8)
+        lib().or_project_f64(_p(fb), _p(g), _p(p), _p(col), _p(lb), _p(e), n_emit, _p(out), _p(valid))
+        return out, valid.astype(bool)
+    splats = np.zeros((nslots, 12), np.uint32)
+    keys = np.zeros(nslots, np.uint32)
+    vis = lib().or_project_f32(_p(fb), _p(g), _p(p), _p(col), _p(lb), _p(e), n_emit, _p(splats), _p(keys))
+    return splats, keys, int(vis)
+
+
+def sort_keys(keys: np.ndarray):
+    """(hist[65536], offsets[65536], order[n], visible)."""
+    k = np.ascontiguousarray(keys, np.uint32)
+    hist = np.zeros(65536, np.uint32)
+    off = np.zeros(65536, np.uint32)
+    order = np.zeros(k.shape[0], np.uint32)
+    vis = lib().or_sort_keys(_p(k), k.shape[0], _p(hist), _p(off), _p(order))
+    return hist, off, order, int(vis)
+
+
+def tile_box(splat_row: np.ndarray, W: int, H: int, tile_px: int = 16):
+    box = np.zeros(4, np.uint32)
+    n = lib().or_tile_box(_p(np.ascontiguousarray(splat_row, np.uint32)), W, H, tile_px, _p(box))
+    return int(n), box
+
+
+def tile_lists(splats: np.ndarray, order: np.ndarray, visible: int, W: int, H: int, tile_px: int = 16):
+    """(tile_start[ntiles + 1] u64, vals[P] u32)."""
+    s = np.ascontiguousarray(splats, np.uint32)
+    o = np.ascontiguousarray(order, np.uint32)
+    nt = ((W + tile_px - 1) // tile_px) * ((H + tile_px - 1) // tile_px)
+    P = int(lib().or_tile_lists(_p(s), _p(o), visible, W, H, tile_px, None, None))
+    start = np.zeros(nt + 1, np.uint64)
+    vals = np.zeros(max(P, 1), np.uint32)
+    lib().or_tile_lists(_p(s), _p(o), visible, W, H, tile_px, _p(start), _p(vals))
+    return start, vals[:P]
+
+
+def composite(frame: bytes, splats, order, visible, W, H, mode=0):
+    """mode 0: float32 back-to-front; 2: float16 target after every blend. Returns [H, W, 4] float32."""
+    out = np.zeros((H, W, 4), np.float32)
+    fb = np.frombuffer(frame, np.uint8).copy()
+    lib().or_composite(_p(fb), _p(np.ascontiguousarray(splats, np.uint32)), _p(np.ascontiguousarray(order, np.uint32)),
+                       visible, W, H, mode, _p(out))
+    return out
+
+
+def composite_f64(frame: bytes, splats64, order, visible, W, H):
+    out = np.zeros((H, W, 4), np.float64)
+    fb = np.frombuffer(frame, np.uint8).copy()
+    lib().or_composite_f64(_p(fb), _p(np.ascontiguousarray(splats64, np.float64)),
+                           _p(np.ascontiguousarray(order, np.uint32)), visible, W, H, _p(out))
+    return out

@@ -1,0 +1,59 @@
+/* CPU oracle for the splat frame path: TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C restatement of the reference's WGSL/JS for the path, used by tests/, the smoke check and the
+ * CPU-baseline leg of bench.py as the checker. The shipped library never links, loads or calls it.
+ * Each function cites the reference file:line it follows; DESIGN.md lists how each one is pinned.
+ */
+#ifndef SL_ORACLE_H
+#define SL_ORACLE_H
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* 16-bit log-depth key (shaders/project.wgsl:366-367) with log fixed as the correctly rounded float log. */
+uint32_t or_depth_key(float depth, float near_, float far_);
+
+/* project (shaders/project.wgsl:84-370), float32 evaluated in the WGSL expression order, no contraction.
+ * frame: 1200 B RenderFrame; gaussians/emitters: 64 B records; parts: 128 B records; colour: vec4 per Gaussian
+ * (may be NULL: zero); lo_b: 3 vec4 per Gaussian (may be NULL: zero). Writes splats[12*nslots] (the 48 B Splat
+ * words, untouched where the key is invalid) and keys[nslots]. nslots = count + 1 + n_emit. Returns visible. */
+uint32_t or_project_f32(const void *frame, const void *gaussians, const void *parts, const float *colour,
+                        const float *lo_b, const void *emitters, uint32_t n_emit, uint32_t *splats,
+                        uint32_t *keys);
+
+/* The same in float64: out[12*slot] = centre.xy, axis0.xy, axis1.xy, r, tau0, rgb, depth; valid[slot]. */
+void or_project_f64(const void *frame, const void *gaussians, const void *parts, const float *colour,
+                    const float *lo_b, const void *emitters, uint32_t n_emit, double *out, uint8_t *valid);
+
+/* Counting-sort artefacts of the 16-bit keys: histogram (shaders/project.wgsl:369), exclusive offsets and the
+ * visible count (shaders/scan.wgsl:16-43), and the stable order (shaders/radix.wgsl:1-139 == a stable argsort,
+ * INVALID last). hist/offsets: 65536 entries each (may be NULL); order: n entries. Returns visible. */
+uint32_t or_sort_keys(const uint32_t *keys, uint32_t n, uint32_t *hist, uint32_t *offsets, uint32_t *order);
+
+/* Tile overlap of one splat's extent box (shaders/project.wgsl:314) in 16x16 (tile_px) tiles: returns the
+ * number of tiles and writes the inclusive tile box (tx0, ty0, tx1, ty1). */
+uint32_t or_tile_box(const uint32_t *splat, uint32_t W, uint32_t H, uint32_t tile_px, uint32_t box[4]);
+
+/* Per-tile lists in draw order: tile_start[ntiles + 1] (CSR), vals[P] = Gaussian slot index ascending in
+ * (key, index) within each tile. vals may be NULL to count only. Returns P. */
+uint64_t or_tile_lists(const uint32_t *splats, const uint32_t *order, uint32_t visible, uint32_t W, uint32_t H,
+                       uint32_t tile_px, uint64_t *tile_start, uint32_t *vals);
+
+/* Composite (shaders/splat.wgsl:22-78, src/renderer.js:133-142): back-to-front premultiplied "over" of the
+ * quads in `order`, rows top-down. mode 0: float32 accumulation; 2: float16 stored after every blend (the
+ * rgba16float target, src/renderer.js:13). view_mode from the frame (0 flat, 1 depth, 2 overdraw, 3 lit).
+ * out: W*H*4 floats. */
+void or_composite(const void *frame, const uint32_t *splats, const uint32_t *order, uint32_t visible,
+                  uint32_t W, uint32_t H, int mode, float *out);
+
+/* The same blend in float64 over or_project_f64's splats (out: W*H*4 doubles). */
+void or_composite_f64(const void *frame, const double *splats, const uint32_t *order, uint32_t visible,
+                      uint32_t W, uint32_t H, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,36 @@
+"""Build the HIP library (gfx950) in-tree: splatlight_amd/libsplatlight_hip.so."""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB = os.path.join(HERE, "libsplatlight_hip.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("SL_OFFLOAD_ARCH", "gfx950")
+
+
+def hip_flags() -> list[str]:
+    return ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-I" + os.path.join(ROOT, "include")]
+
+
+def build_lib(verbose: bool = False) -> str:
+    src = os.path.join(HERE, "csrc", "splatlight.hip")
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC, *hip_flags(), "-shared", "-o", tmp, src]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc failed for libsplatlight_hip.so")
+    if verbose:
+        sys.stderr.write(r.stderr)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build_lib(verbose=True))

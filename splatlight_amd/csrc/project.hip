@@ -1,0 +1,226 @@
+// project: pose, cull and project every splat slot (shaders/project.wgsl:84-370) on gfx950.
+//
+// One lane per slot. Gaussian attributes are SoA float4 streams (mu|kappa, quat, scale|part_surface,
+// albedo|flags), so each of the four loads of a wavefront reads one contiguous 1 KiB run. Outputs per slot: the
+// 48 B Splat in the reference's layout, the 16-bit depth key (or INVALID), and the splat's inclusive 16x16 tile
+// box packed as (tx0 | ty0 << 16, tx1 | ty1 << 16) for the pair emission.
+//
+// Contraction is off in this file: every float expression rounds as written, in the WGSL evaluation order
+// (dot products left to right, m * v as m[0] v.x + m[1] v.y + m[2] v.z). Division and sqrt are IEEE correctly
+// rounded and the two logs (depth key, cut radius) are the correctly rounded float log, so every splat word,
+// the 16-bit key and the tile box are bit-identical to the CPU restatement's.
+#pragma clang fp contract(off)
+#include "sl_device.h"
+
+namespace {
+
+__device__ __forceinline__ float clampf(float v, float lo, float hi) { return fminf(fmaxf(v, lo), hi); }
+
+// quat_to_mat (shaders/common.wgsl:83-89): m[col][row]
+__device__ __forceinline__ void quat_to_mat(float4 q, float m[3][3]) {
+  const float x = q.x, y = q.y, z = q.z, w = q.w;
+  m[0][0] = 1.0f - 2.0f * (y * y + z * z); m[0][1] = 2.0f * (x * y + w * z); m[0][2] = 2.0f * (x * z - w * y);
+  m[1][0] = 2.0f * (x * y - w * z); m[1][1] = 1.0f - 2.0f * (x * x + z * z); m[1][2] = 2.0f * (y * z + w * x);
+  m[2][0] = 2.0f * (x * z + w * y); m[2][1] = 2.0f * (y * z - w * x); m[2][2] = 1.0f - 2.0f * (x * x + y * y);
+}
+
+// The 16-bit back-to-front depth key (shaders/project.wgsl:366-367) with log taken as the correctly rounded
+// float log (double log, rounded once).
+__device__ __forceinline__ uint32_t depth_key(float depth, float near_, float log_far_near) {
+  const float num = (float)log((double)(depth / near_));
+  float t = num / log_far_near;
+  t = fminf(fmaxf(t, 0.0f), 1.0f);
+  const float s = t * 65535.0f + 0.5f;
+  const uint32_t k = 65535u - (uint32_t)s;
+  return k < 0xfffeu ? k : 0xfffeu;
+}
+
+}  // namespace
+
+extern "C" __global__ void __launch_bounds__(256)
+sl_project_kernel(FrameDev f, const float4 *__restrict__ g0, const float4 *__restrict__ g1,
+                  const float4 *__restrict__ g2, const float4 *__restrict__ g3, const float4 *__restrict__ e0,
+                  const float4 *__restrict__ e1, const float4 *__restrict__ e2, const float4 *__restrict__ e3,
+                  const float4 *__restrict__ parts, const float4 *__restrict__ colour,
+                  const float4 *__restrict__ lo_b, Splat *__restrict__ splats, uint32_t *__restrict__ keys,
+                  uint2 *__restrict__ tbox, FrameCounters *__restrict__ counters) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool ok = i < f.nslots && i != f.count;
+  float4 A, Q4, S4, C4;
+  float mu[3], rot[3][3];
+  uint32_t material = SL_EMITTER_MATERIAL;
+  float keep = 1.0f;
+  if (ok) {
+    if (i > f.count) {                                                // emitters (project.wgsl:123-127)
+      const uint32_t k = i - f.count - 1u;
+      A = e0[k]; Q4 = e1[k]; S4 = e2[k]; C4 = e3[k];
+      mu[0] = A.x; mu[1] = A.y; mu[2] = A.z;
+      quat_to_mat(Q4, rot);
+    } else {                                                          // pose (project.wgsl:128-148)
+      A = g0[i]; Q4 = g1[i]; S4 = g2[i]; C4 = g3[i];
+      uint32_t part = __float_as_uint(S4.w) & 0xffffu;
+      part = part < SL_MAX_PARTS - 1u ? part : SL_MAX_PARTS - 1u;
+      const float4 x0 = parts[part * 8 + 0], x1 = parts[part * 8 + 1], x2 = parts[part * 8 + 2];
+      const uint32_t pflags = __float_as_uint(parts[part * 8 + 7].w);
+      keep = f.part_keep[part];
+      ok = (pflags & SL_PART_HIDDEN) == 0u && keep > 0.0f;
+      mu[0] = ((x0.x * A.x + x0.y * A.y) + x0.z * A.z) + x0.w * 1.0f;
+      mu[1] = ((x1.x * A.x + x1.y * A.y) + x1.z * A.z) + x1.w * 1.0f;
+      mu[2] = ((x2.x * A.x + x2.y * A.y) + x2.z * A.z) + x2.w * 1.0f;
+      float q[3][3];
+      quat_to_mat(Q4, q);
+      const float xr[3][3] = {{x0.x, x0.y, x0.z}, {x1.x, x1.y, x1.z}, {x2.x, x2.y, x2.z}};   // xr[row][col]
+#pragma unroll
+      for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int r = 0; r < 3; r++) rot[j][r] = (xr[r][0] * q[j][0] + xr[r][1] * q[j][1]) + xr[r][2] * q[j][2];
+      material = f.part_material[part];
+    }
+  }
+  float tv[3], depth = 0.0f;
+  if (ok) {
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+      tv[r] = ((f.view[r] * mu[0] + f.view[4 + r] * mu[1]) + f.view[8 + r] * mu[2]) + f.view[12 + r] * 1.0f;
+    depth = -tv[2];
+    ok = !(depth < f.near_ || depth > f.far_);                       // project.wgsl:199-200
+  }
+  Splat out;
+  uint32_t key = SL_INVALID_KEY;
+  uint2 box = make_uint2(0xffffffffu, 0u);
+  float cmax = 0.0f;
+  if (ok) {
+    // 3D covariance in view space W R S^2 R^T W^T (project.wgsl:202-206)
+    float wr[3][3], m[3][3], cov[3][3];
+    const float sc[3] = {S4.x, S4.y, S4.z};
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+#pragma unroll
+      for (int r = 0; r < 3; r++)
+        wr[j][r] = (f.view[r] * rot[j][0] + f.view[4 + r] * rot[j][1]) + f.view[8 + r] * rot[j][2];
+    const float s2[3] = {sc[0] * sc[0], sc[1] * sc[1], sc[2] * sc[2]};
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+#pragma unroll
+      for (int r = 0; r < 3; r++) m[j][r] = wr[j][r] * sc[j];
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+#pragma unroll
+      for (int r = 0; r < 3; r++) cov[j][r] = (m[0][r] * m[0][j] + m[1][r] * m[1][j]) + m[2][r] * m[2][j];
+
+    // EWA Jacobian with the 1.3x tangent clamp (project.wgsl:212-231)
+    const float iz = 1.0f / depth;
+    const float px = f.fx * tv[0] * iz;
+    const float py = f.fy * tv[1] * iz;
+    const float limx = 1.3f * f.vw * 0.5f, limy = 1.3f * f.vh * 0.5f;
+    const float cx = clampf(tv[0] * iz, -limx / f.fx, limx / f.fx) * depth;
+    const float cy = clampf(tv[1] * iz, -limy / f.fy, limy / f.fy) * depth;
+    const float j0[3] = {f.fx * iz, 0.0f, f.fx * cx * iz * iz};
+    const float j1[3] = {0.0f, f.fy * iz, f.fy * cy * iz * iz};
+    float cj0[3], cj1[3];
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      cj0[r] = (cov[0][r] * j0[0] + cov[1][r] * j0[1]) + cov[2][r] * j0[2];
+      cj1[r] = (cov[0][r] * j1[0] + cov[1][r] * j1[1]) + cov[2][r] * j1[2];
+    }
+    float a = (j0[0] * cj0[0] + j0[1] * cj0[1]) + j0[2] * cj0[2];
+    const float b = (j0[0] * cj1[0] + j0[1] * cj1[1]) + j0[2] * cj1[2];
+    float c = (j1[0] * cj1[0] + j1[1] * cj1[1]) + j1[2] * cj1[2];
+    const float det0 = a * c - b * b;
+    a += f.lowpass;
+    c += f.lowpass;
+    const float det = a * c - b * b;
+    ok = !(det <= 0.0f);                                              // project.wgsl:232
+    // peak optical depth tau0 = kappa sqrt(2 pi / q) sqrt(det0 / det) (project.wgsl:237-261)
+    const float len = sqrtf((tv[0] * tv[0] + tv[1] * tv[1]) + tv[2] * tv[2]);
+    const float d[3] = {tv[0] / len, tv[1] / len, tv[2] / len};
+    float dl[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) dl[k] = (wr[k][0] * d[0] + wr[k][1] * d[1]) + wr[k][2] * d[2];
+    const float q = ((dl[0] * dl[0]) * (1.0f / s2[0]) + (dl[1] * dl[1]) * (1.0f / s2[1])) +
+                    (dl[2] * dl[2]) * (1.0f / s2[2]);
+    float tau0 = A.w * sqrtf(6.283185307f / q) * sqrtf(fmaxf(det0, 0.0f) / det);
+    tau0 *= keep;
+    // cut radius r: alpha >= 1/255 (project.wgsl:290-295); tau_min = -log(1 - 1/255) correctly rounded
+    const float tau_min = 0x1.018258p-8f;
+    ok = ok && !(tau0 <= tau_min);
+    const float r = sqrtf(2.0f * (float)log((double)(tau0 / tau_min)));   // correctly rounded float log
+    // stable eigen axes of the 2x2 covariance (project.wgsl:297-311)
+    const float mid = 0.5f * (a + c);
+    const float disc = sqrtf(0.25f * (a - c) * (a - c) + b * b);
+    const float l0 = mid + disc;
+    const float l1 = fmaxf(det / l0, 1e-6f);
+    float e0x = 1.0f, e0y = 0.0f;
+    if (a >= c) {
+      if (fabsf(b) > 0.0f || l0 > a) {
+        const float vx = l0 - c, vy = b, n = sqrtf(vx * vx + vy * vy);
+        e0x = vx / n; e0y = vy / n;
+      }
+    } else {
+      const float vx = b, vy = l0 - a, n = sqrtf(vx * vx + vy * vy);
+      e0x = vx / n; e0y = vy / n;
+    }
+    const float e1x = -e0y, e1y = e0x;
+    const float tnx = 2.0f / f.vw, tny = 2.0f / f.vh;
+    const float sl0 = sqrtf(l0), sl1 = sqrtf(l1);
+    const float a0x = e0x * sl0 * r * tnx, a0y = e0y * sl0 * r * tny;
+    const float a1x = e1x * sl1 * r * tnx, a1y = e1y * sl1 * r * tny;
+    ok = ok && !(sl0 * r < 0.1f);
+    const float cenx = px * tnx, ceny = py * tny;                     // project.wgsl:313-315
+    const float ex = fabsf(a0x) + fabsf(a1x), ey = fabsf(a0y) + fabsf(a1y);
+    ok = ok && !(fabsf(cenx) - ex > 1.0f || fabsf(ceny) - ey > 1.0f);
+    if (ok) {
+      // colour (project.wgsl:317-350)
+      float col[3] = {C4.x, C4.y, C4.z};
+      if (material != SL_EMITTER_MATERIAL) {
+        if (f.view_mode == 3u) {
+          const float4 cv = colour ? colour[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+          const float4 lb = lo_b ? lo_b[3 * (size_t)i] : make_float4(0.f, 0.f, 0.f, 0.f);
+          col[0] = cv.x + f.gain * lb.x;
+          col[1] = cv.y + f.gain * lb.y;
+          col[2] = cv.z + f.gain * lb.z;
+          if (material < 8u) { col[0] *= f.palette[material][0]; col[1] *= f.palette[material][1]; col[2] *= f.palette[material][2]; }
+        } else if (material < 8u && f.palette[material][3] > 0.5f) {
+          col[0] = f.palette[material][0]; col[1] = f.palette[material][1]; col[2] = f.palette[material][2];
+        }
+        col[0] *= f.albedo_scale; col[1] *= f.albedo_scale; col[2] *= f.albedo_scale;
+      }
+      out.center_axis0 = make_float4(cenx, ceny, a0x, a0y);
+      out.axis1_r_tau = make_float4(a1x, a1y, r, tau0);
+      // SPLAT_GLASS (shaders/lens.wgsl:21) in the depth word's low bits (project.wgsl:167-168)
+      const uint32_t mat_idx = min(__float_as_uint(C4.w) & 0xffffu, 31u);
+      const uint32_t glass = material != SL_EMITTER_MATERIAL ? (f.glass_mask >> mat_idx) & 1u : 0u;
+      out.color_depth = make_uint4(__float_as_uint(col[0]), __float_as_uint(col[1]), __float_as_uint(col[2]),
+                                   (__float_as_uint(depth) & ~3u) | glass);
+      key = depth_key(depth, f.near_, f.log_far_near);
+      cmax = fmaxf(fmaxf(fabsf(col[0]), fabsf(col[1])), fabsf(col[2]));
+      // tile box of the extent box in pixel centres (the pair definition)
+      const float hw = 0.5f * (float)f.width, hh = 0.5f * (float)f.height;
+      float fx0 = ceilf((cenx - ex + 1.0f) * hw - 0.5f), fx1 = floorf((cenx + ex + 1.0f) * hw - 0.5f);
+      float fy0 = ceilf((1.0f - (ceny + ey)) * hh - 0.5f), fy1 = floorf((1.0f - (ceny - ey)) * hh - 0.5f);
+      fx0 = fx0 < 0.0f ? 0.0f : fx0;
+      fy0 = fy0 < 0.0f ? 0.0f : fy0;
+      fx1 = fx1 > (float)(f.width - 1) ? (float)(f.width - 1) : fx1;
+      fy1 = fy1 > (float)(f.height - 1) ? (float)(f.height - 1) : fy1;
+      if (fx0 <= fx1 && fy0 <= fy1) {
+        const uint32_t tx0 = (uint32_t)fx0 >> 4, ty0 = (uint32_t)fy0 >> 4;
+        const uint32_t tx1 = (uint32_t)fx1 >> 4, ty1 = (uint32_t)fy1 >> 4;
+        box = make_uint2(tx0 | (ty0 << 16), tx1 | (ty1 << 16));
+      }
+    }
+  }
+  if (i < f.nslots) {
+    if (ok) splats[i] = out;
+    keys[i] = key;
+    tbox[i] = box;
+  }
+  // per-block visible count and colour max, one atomic each
+  const uint64_t vis = __ballot(ok);
+  float wmax = cmax;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) wmax = fmaxf(wmax, __shfl_xor(wmax, o));
+  if (lane_id() == 0) {
+    if (vis) atomicAdd(&counters->visible, (uint32_t)__popcll(vis));
+    if (wmax > 0.0f) atomicMax(&counters->color_max_bits, __float_as_uint(wmax));
+  }
+}

@@ -1,0 +1,52 @@
+// Device-side declarations shared by the splat-path kernels (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define SL_INVALID_KEY 0xffffffffu
+#define SL_MAX_PARTS 32u
+#define SL_PART_HIDDEN 4u
+#define SL_EMITTER_MATERIAL 254u
+#define SL_WAVE 64
+
+// Per-frame parameters decoded from the 1200 B RenderFrame (shaders/common.wgsl:44-77) on the host and passed
+// to the kernels by value.
+struct FrameDev {
+  float view[16];            // column-major
+  float palette[8][4];       // palette slots 0-7 (slot 3: bounce gain, emitter count, shutter, blur coverage)
+  uint32_t part_material[SL_MAX_PARTS];
+  float part_keep[SL_MAX_PARTS];   // 1 - clamp(part_fade)
+  float vw, vh, fx, fy;
+  float near_, far_;
+  float log_far_near;        // (float)log((double)(far / near)): the depth key's denominator
+  float albedo_scale, lowpass, gain;
+  uint32_t count;            // scene Gaussians
+  uint32_t n_emit;           // emitter slots after count + 1
+  uint32_t nslots;           // count + 1 + n_emit
+  uint32_t view_mode;
+  uint32_t glass_mask;       // bit m: material m's mat_view flags bit 0 (coverage set: SPLAT_GLASS)
+  uint32_t width, height, tiles_x, tiles_y;
+};
+
+// Splat record, 48 B: the layout project.wgsl writes (shaders/common.wgsl:29-35).
+struct __align__(16) Splat {
+  float4 center_axis0;   // centre.xy (NDC), axis0.xy (NDC)
+  float4 axis1_r_tau;    // axis1.xy (NDC), r, tau0
+  uint4 color_depth;     // rgb bits, depth bits & ~3
+};
+
+// Per-frame device counters, one 256 B block.
+struct FrameCounters {
+  uint32_t visible;        // valid depth keys
+  uint32_t pairs;          // (splat, tile) pairs emitted
+  uint32_t overflow;       // pairs that did not fit the pair buffers
+  uint32_t color_max_bits; // max |rgb| over visible splats (float bits, non-negative)
+  uint32_t pad[60];
+};
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  const uint32_t l = __lane_id();
+  return l == 0 ? 0ull : (~0ull >> (64 - l));
+}

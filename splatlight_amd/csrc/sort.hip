@@ -1,0 +1,206 @@
+// Stable LSD radix sort of (u32 key, u32 value) pairs, onesweep form, for gfx950 (wave64).
+//
+// Used twice per frame: the depth order (16-bit keys of every splat slot, values = slot index; the result is
+// the reference's stable order, shaders/radix.wgsl:1-139) and the tile order (tile ids of the (splat, tile)
+// pairs emitted in depth order; a stable sort keeps every tile's list in depth order).
+//
+// One read of the keys builds every pass's 256-digit histogram (sl_radix_hist + sl_radix_bases). Each pass is
+// then one kernel: persistent workgroups take 4096-element tiles in ticket order; inside a tile each wave ranks
+// its 1024 elements by 8 ballots per item (peers with the same digit) and a per-wave LDS digit counter, so
+// the rank follows element order; the tile's digit counts are published with a decoupled look-back
+// (aggregate / inclusive flags packed with the count in one word) and the tile is reordered by digit in LDS
+// so the scatter writes runs of consecutive addresses.
+#include "sl_device.h"
+
+#define RS_THREADS 256
+#define RS_WAVES (RS_THREADS / SL_WAVE)
+#define RS_ITEMS 16
+#define RS_TILE (RS_THREADS * RS_ITEMS)
+#define RS_WAVE_SPAN (SL_WAVE * RS_ITEMS)
+#define RS_FLAG_AGG (1u << 30)
+#define RS_FLAG_INC (2u << 30)
+#define RS_FLAG_MASK (3u << 30)
+#define RS_VALUE_MASK ((1u << 30) - 1u)
+
+namespace {
+
+__device__ __forceinline__ uint32_t load_n(const uint32_t *n_ptr, uint32_t n_static) {
+  return n_ptr ? __hip_atomic_load(n_ptr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : n_static;
+}
+
+// exclusive scan of one value per thread over a 256-thread block; returns the block total in *total
+__device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t *s_tmp, uint32_t *total) {
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o);
+    if (lane >= (uint32_t)o) x += y;
+  }
+  if (lane == 63u) s_tmp[w] = x;
+  __syncthreads();
+  uint32_t before = 0, all = 0;
+#pragma unroll
+  for (int k = 0; k < RS_WAVES; k++) {
+    const uint32_t t = s_tmp[k];
+    before += (uint32_t)k < w ? t : 0u;
+    all += t;
+  }
+  __syncthreads();
+  *total = all;
+  return before + x - v;
+}
+
+}  // namespace
+
+// Digit histograms of every pass in one read: hist[p * 256 + digit], digits at shift0 + 8 p.
+extern "C" __global__ void __launch_bounds__(RS_THREADS)
+sl_radix_hist(const uint32_t *__restrict__ keys, const uint32_t *n_ptr, uint32_t n_static, uint32_t shift0,
+              uint32_t npasses, uint32_t *__restrict__ hist) {
+  __shared__ uint32_t s_hist[4 * 256];
+  for (uint32_t k = threadIdx.x; k < 4u * 256u; k += RS_THREADS) s_hist[k] = 0u;
+  __syncthreads();
+  const uint32_t n = load_n(n_ptr, n_static);
+  for (uint32_t i = blockIdx.x * RS_THREADS + threadIdx.x; i < n; i += gridDim.x * RS_THREADS) {
+    const uint32_t key = keys[i];
+    for (uint32_t p = 0; p < npasses; p++) atomicAdd(&s_hist[p * 256u + ((key >> (shift0 + 8u * p)) & 255u)], 1u);
+  }
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < npasses * 256u; k += RS_THREADS)
+    if (s_hist[k]) atomicAdd(&hist[k], s_hist[k]);
+}
+
+// Exclusive digit offsets per pass, in place: one workgroup per pass.
+extern "C" __global__ void __launch_bounds__(RS_THREADS) sl_radix_bases(uint32_t *__restrict__ hist) {
+  __shared__ uint32_t s_tmp[RS_WAVES];
+  uint32_t *h = hist + blockIdx.x * 256u;
+  const uint32_t v = h[threadIdx.x];
+  uint32_t total;
+  const uint32_t ex = block_excl_scan256(v, s_tmp, &total);
+  h[threadIdx.x] = ex;
+}
+
+// One stable pass on digit (key >> shift) & 255. vals_in == nullptr: values are the element index.
+// keys_out == nullptr: keys are not written (the last pass when only the order is wanted).
+// status: >= ceil(n / 4096) * 256 words, zero on entry; ticket: one word, zero on entry.
+extern "C" __global__ void __launch_bounds__(RS_THREADS)
+sl_radix_pass(const uint32_t *__restrict__ keys_in, const uint32_t *__restrict__ vals_in,
+              uint32_t *__restrict__ keys_out, uint32_t *__restrict__ vals_out, const uint32_t *n_ptr,
+              uint32_t n_static, uint32_t shift, const uint32_t *__restrict__ bases, uint32_t *status,
+              uint32_t *ticket) {
+  __shared__ uint32_t s_keys[RS_TILE];
+  __shared__ uint32_t s_vals[RS_TILE];
+  __shared__ uint32_t s_whist[RS_WAVES * 256];
+  __shared__ uint32_t s_dexcl[256];
+  __shared__ uint32_t s_gbase[256];
+  __shared__ uint32_t s_tmp[RS_WAVES];
+  __shared__ uint32_t s_tile;
+
+  const uint32_t n = load_n(n_ptr, n_static);
+  const uint32_t num_tiles = (n + RS_TILE - 1u) / RS_TILE;
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+  const uint32_t digit_base = bases[tid];
+
+  for (;;) {
+    if (tid == 0) s_tile = atomicAdd(ticket, 1u);
+#pragma unroll
+    for (int k = 0; k < RS_WAVES; k++) s_whist[k * 256 + tid] = 0u;
+    __syncthreads();
+    const uint32_t t = s_tile;
+    if (t >= num_tiles) break;
+    const uint32_t tile_base = t * RS_TILE;
+    const uint32_t tile_n = min(RS_TILE, n - tile_base);
+    const uint32_t wbase = tile_base + w * RS_WAVE_SPAN;
+
+    uint32_t key[RS_ITEMS], val[RS_ITEMS], rank[RS_ITEMS];
+#pragma unroll
+    for (int k = 0; k < RS_ITEMS; k++) {
+      const uint32_t idx = wbase + (uint32_t)k * SL_WAVE + lane;
+      const bool v = idx < n;
+      key[k] = v ? keys_in[idx] : 0u;
+      val[k] = v ? (vals_in ? vals_in[idx] : idx) : 0u;
+    }
+    // rank within the wave, element order (k, lane)
+    uint32_t *whist = s_whist + w * 256u;
+#pragma unroll
+    for (int k = 0; k < RS_ITEMS; k++) {
+      const uint32_t idx = wbase + (uint32_t)k * SL_WAVE + lane;
+      const bool v = idx < n;
+      const uint32_t d = (key[k] >> shift) & 255u;
+      uint64_t peers = __ballot(v);
+#pragma unroll
+      for (int b = 0; b < 8; b++) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t bal = __ballot(bit);
+        peers &= bit ? bal : ~bal;
+      }
+      uint32_t r = 0;
+      if (v) {
+        const uint32_t below = (uint32_t)__popcll(peers & lanemask_lt());
+        const uint32_t old = whist[d];
+        r = old + below;
+        if (below == 0u) whist[d] = old + (uint32_t)__popcll(peers);
+      }
+      rank[k] = r;
+    }
+    __syncthreads();
+    // digit tid: per-wave exclusive offsets and the tile's count
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int k = 0; k < RS_WAVES; k++) {
+      const uint32_t c = s_whist[k * 256 + tid];
+      s_whist[k * 256 + tid] = cnt;
+      cnt += c;
+    }
+    // decoupled look-back for digit tid
+    uint32_t *st = status + (size_t)t * 256u + tid;
+    uint32_t excl = 0;
+    if (t == 0) {
+      __hip_atomic_store(st, RS_FLAG_INC | cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      __hip_atomic_store(st, RS_FLAG_AGG | cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      int32_t j = (int32_t)t - 1;
+      for (;;) {
+        const uint32_t s = __hip_atomic_load(status + (size_t)j * 256u + tid, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t fl = s & RS_FLAG_MASK;
+        if (fl == 0u) { __builtin_amdgcn_s_sleep(1); continue; }
+        excl += s & RS_VALUE_MASK;
+        if (fl == RS_FLAG_INC) break;
+        --j;
+      }
+      __hip_atomic_store(st, RS_FLAG_INC | (excl + cnt), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    s_gbase[tid] = digit_base + excl;
+    uint32_t total;
+    s_dexcl[tid] = block_excl_scan256(cnt, s_tmp, &total);
+    __syncthreads();
+    // reorder the tile by digit in LDS
+#pragma unroll
+    for (int k = 0; k < RS_ITEMS; k++) {
+      const uint32_t idx = wbase + (uint32_t)k * SL_WAVE + lane;
+      if (idx < n) {
+        const uint32_t d = (key[k] >> shift) & 255u;
+        const uint32_t pos = s_dexcl[d] + s_whist[w * 256u + d] + rank[k];
+        s_keys[pos] = key[k];
+        s_vals[pos] = val[k];
+      }
+    }
+    __syncthreads();
+    for (uint32_t j = tid; j < tile_n; j += RS_THREADS) {
+      const uint32_t kk = s_keys[j];
+      const uint32_t d = (kk >> shift) & 255u;
+      const uint32_t dst = s_gbase[d] + (j - s_dexcl[d]);
+      if (keys_out) keys_out[dst] = kk;
+      vals_out[dst] = s_vals[j];
+    }
+    __syncthreads();
+  }
+}
+
+// Fill n words with a value (clears look-back state and counters; a kernel rather than a memset so the frame
+// can be captured in a graph).
+extern "C" __global__ void __launch_bounds__(256)
+sl_fill_u32(uint32_t *__restrict__ p, uint32_t value, uint32_t n) {
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) p[i] = value;
+}

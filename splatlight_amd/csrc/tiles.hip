@@ -1,0 +1,151 @@
+// (splat, tile) pair emission and per-tile ranges.
+//
+// A pair is a visible splat and a 16x16 tile whose pixel-centre range meets the splat's extent box
+// (shaders/project.wgsl:314). Pairs are emitted in the depth order produced by the depth sort, so after a
+// stable sort by tile id every tile's list is that tile's slice of the reference's global back-to-front order.
+//
+// sl_pairs_emit is one pass over the visible ranks: persistent workgroups take 1024-rank chunks in ticket
+// order, scan the chunk's pair counts, get the chunk's global offset by a decoupled look-back (64-bit status
+// words), then emit the chunk's pairs cooperatively: lane j of the workgroup writes pair j, finding its rank by a
+// binary search over the chunk's offsets in LDS, so the key and value stores are contiguous. The tile-id digit
+// histograms for the tile sort are accumulated on the way.
+#include "sl_device.h"
+
+#define PE_THREADS 256
+#define PE_RPT 4
+#define PE_CHUNK (PE_THREADS * PE_RPT)
+#define PE_FLAG_AGG (1ull << 62)
+#define PE_FLAG_INC (2ull << 62)
+#define PE_FLAG_MASK (3ull << 62)
+#define PE_VALUE_MASK ((1ull << 62) - 1ull)
+
+extern "C" __global__ void __launch_bounds__(PE_THREADS)
+sl_pairs_emit(const uint32_t *__restrict__ order, const uint2 *__restrict__ tbox, uint32_t tiles_x,
+              uint32_t capacity, FrameCounters *counters, uint32_t *__restrict__ pair_keys,
+              uint32_t *__restrict__ pair_vals, uint64_t *chunk_status, uint32_t *ticket,
+              uint32_t *__restrict__ tile_hist) {
+  __shared__ uint32_t s_start[PE_CHUNK + 1];
+  __shared__ uint2 s_box[PE_CHUNK];
+  __shared__ uint32_t s_g[PE_CHUNK];
+  __shared__ uint32_t s_hist[2 * 256];
+  __shared__ uint32_t s_wsum[PE_THREADS / SL_WAVE];
+  __shared__ uint32_t s_chunk;
+  __shared__ uint64_t s_excl;
+
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+  const uint32_t V = __hip_atomic_load(&counters->visible, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const uint32_t num_chunks = (V + PE_CHUNK - 1u) / PE_CHUNK;
+  for (uint32_t k = tid; k < 512u; k += PE_THREADS) s_hist[k] = 0u;
+
+  for (;;) {
+    if (tid == 0) s_chunk = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t c = s_chunk;
+    if (c >= num_chunks) break;
+    const uint32_t r0 = c * PE_CHUNK;
+    // per-thread counts of PE_RPT consecutive ranks
+    uint32_t cnt[PE_RPT], sum = 0;
+#pragma unroll
+    for (int k = 0; k < PE_RPT; k++) {
+      const uint32_t lr = tid * PE_RPT + k;
+      const uint32_t r = r0 + lr;
+      uint32_t n = 0;
+      uint2 b = make_uint2(0xffffffffu, 0u);
+      uint32_t g = 0;
+      if (r < V) {
+        g = order[r];
+        b = tbox[g];
+        if (b.x != 0xffffffffu)
+          n = ((b.y & 0xffffu) - (b.x & 0xffffu) + 1u) * ((b.y >> 16) - (b.x >> 16) + 1u);
+      }
+      s_box[lr] = b;
+      s_g[lr] = g;
+      cnt[k] = n;
+      sum += n;
+    }
+    // block exclusive scan of the per-thread sums
+    uint32_t x = sum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o);
+      if (lane >= (uint32_t)o) x += y;
+    }
+    if (lane == 63u) s_wsum[w] = x;
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+#pragma unroll
+    for (int k = 0; k < PE_THREADS / SL_WAVE; k++) {
+      before += (uint32_t)k < w ? s_wsum[k] : 0u;
+      total += s_wsum[k];
+    }
+    uint32_t run = before + x - sum;
+#pragma unroll
+    for (int k = 0; k < PE_RPT; k++) { s_start[tid * PE_RPT + k] = run; run += cnt[k]; }
+    if (tid == 0) {
+      s_start[PE_CHUNK] = total;
+      // decoupled look-back over chunks
+      uint64_t *st = chunk_status + c;
+      uint64_t excl = 0;
+      if (c == 0) {
+        __hip_atomic_store(st, PE_FLAG_INC | (uint64_t)total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        __hip_atomic_store(st, PE_FLAG_AGG | (uint64_t)total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int64_t j = (int64_t)c - 1;
+        for (;;) {
+          const uint64_t s = __hip_atomic_load(chunk_status + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          const uint64_t fl = s & PE_FLAG_MASK;
+          if (fl == 0ull) { __builtin_amdgcn_s_sleep(1); continue; }
+          excl += s & PE_VALUE_MASK;
+          if (fl == PE_FLAG_INC) break;
+          --j;
+        }
+        __hip_atomic_store(st, PE_FLAG_INC | (excl + total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      s_excl = excl;
+      if (c == num_chunks - 1u) {
+        const uint64_t all = excl + total;
+        const uint32_t kept = all > capacity ? capacity : (uint32_t)all;
+        __hip_atomic_store(&counters->pairs, kept, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&counters->overflow, (uint32_t)(all - kept), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    __syncthreads();
+    const uint64_t base = s_excl;
+    // cooperative emission: lane j writes pair j of the chunk
+    for (uint32_t j = tid; j < total; j += PE_THREADS) {
+      uint32_t lo = 0, hi = PE_CHUNK;                   // last rank with start <= j
+      while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s_start[mid] <= j) lo = mid; else hi = mid;
+      }
+      const uint2 b = s_box[lo];
+      const uint32_t local = j - s_start[lo];
+      const uint32_t tx0 = b.x & 0xffffu, ty0 = b.x >> 16;
+      const uint32_t bw = (b.y & 0xffffu) - tx0 + 1u;
+      const uint32_t ty = ty0 + local / bw, tx = tx0 + local % bw;
+      const uint32_t tile = ty * tiles_x + tx;
+      const uint64_t dst = base + j;
+      if (dst < capacity) {
+        pair_keys[dst] = tile;
+        pair_vals[dst] = s_g[lo];
+        atomicAdd(&s_hist[tile & 255u], 1u);
+        atomicAdd(&s_hist[256u + ((tile >> 8) & 255u)], 1u);
+      }
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+  for (uint32_t k = tid; k < 512u; k += PE_THREADS)
+    if (s_hist[k]) atomicAdd(&tile_hist[k], s_hist[k]);
+}
+
+// Per-tile [start, end) in the tile-sorted pair list; ranges are zero on entry (empty tiles stay [0, 0)).
+extern "C" __global__ void __launch_bounds__(256)
+sl_tile_ranges(const uint32_t *__restrict__ keys, const FrameCounters *counters, uint2 *__restrict__ ranges) {
+  const uint32_t n = __hip_atomic_load(&counters->pairs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
+    const uint32_t t = keys[i];
+    if (i == 0u || keys[i - 1u] != t) ranges[t].x = i;
+    if (i == n - 1u || keys[i + 1u] != t) ranges[t].y = i + 1u;
+  }
+}

@@ -1,0 +1,246 @@
+"""ctypes binding of libsplatlight_hip.so (include/splatlight.h).
+
+This is the binding a Python host uses; INTEGRATION.md shows the N-API one. The library is the only compute
+path: if it is missing or fails to load this module raises, there is no fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsplatlight_hip.so")
+
+SL_OK = 0
+STATUS_NAMES = {0: "SL_OK", 1: "SL_ERR_INVALID", 2: "SL_ERR_LAYOUT", 3: "SL_ERR_DEVICE", 4: "SL_ERR_POISONED",
+                5: "SL_ERR_UNSUPPORTED", 6: "SL_ERR_BUSY", 7: "SL_ERR_CAPACITY"}
+SL_STORAGE_F32, SL_STORAGE_F16, SL_STORAGE_BF16 = 0, 1, 2
+SL_OUT_RGBA32F, SL_OUT_RGBA16F = 0, 1
+SL_FRAME_TIMERS = 1
+ART_KEYS, ART_ORDER, ART_SPLATS, ART_PAIR_TILES, ART_PAIR_SLOTS, ART_RANGES = 1, 2, 3, 4, 5, 6
+
+# Every symbol include/splatlight.h declares (checked by the CPU tests).
+EXPORTS = [
+    "sl_session_acquire", "sl_session_release", "sl_session_error",
+    "sl_ctx_create", "sl_ctx_destroy", "sl_last_error", "sl_ctx_status", "sl_sync",
+    "sl_buf_alloc", "sl_buf_free", "sl_buf_write", "sl_buf_read",
+    "sl_scene_create", "sl_scene_destroy", "sl_scene_write_parts", "sl_scene_write_colour",
+    "sl_scene_write_bounce", "sl_scene_device_bytes",
+    "sl_frame_encode", "sl_frame_finish", "sl_frame_read", "sl_timers_read",
+]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [
+        ("gaussians", C.c_void_p), ("gaussian_count", C.c_uint32), ("gaussian_stride", C.c_uint32),
+        ("parts", C.c_void_p), ("part_count", C.c_uint32), ("part_stride", C.c_uint32),
+        ("materials", C.c_void_p), ("material_count", C.c_uint32), ("material_stride", C.c_uint32),
+        ("nodes", C.c_void_p), ("node_count", C.c_uint32), ("node_stride", C.c_uint32),
+        ("vgrid", C.c_void_p), ("vgrid_count", C.c_uint32), ("vgrid_stride", C.c_uint32),
+        ("emitters", C.c_void_p), ("emitter_count", C.c_uint32),
+        ("storage", C.c_uint32),
+    ]
+
+
+class FrameOpts(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("out_format", C.c_uint32), ("flags", C.c_uint32)]
+
+
+class FrameStats(C.Structure):
+    _fields_ = [("slots", C.c_uint32), ("visible", C.c_uint32), ("pairs", C.c_uint32), ("overflow", C.c_uint32),
+                ("tiles", C.c_uint32), ("pad", C.c_uint32 * 3)]
+
+
+class SplatlightError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load the HIP library (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; "
+                          f"g.build()'` (hipcc, gfx950)")
+    L = C.CDLL(LIB_PATH)
+    vp, u32, i32, sz = C.c_void_p, C.c_uint32, C.c_int, C.c_size_t
+    sig = {
+        "sl_session_acquire": (i32, [C.c_char_p, C.POINTER(C.c_int64)]),
+        "sl_session_release": (i32, [C.c_int64]),
+        "sl_session_error": (C.c_char_p, []),
+        "sl_ctx_create": (i32, [i32, C.POINTER(vp)]),
+        "sl_ctx_destroy": (None, [vp]),
+        "sl_last_error": (C.c_char_p, [vp]),
+        "sl_ctx_status": (i32, [vp]),
+        "sl_sync": (i32, [vp]),
+        "sl_buf_alloc": (i32, [vp, sz, C.POINTER(vp)]),
+        "sl_buf_free": (i32, [vp, vp]),
+        "sl_buf_write": (i32, [vp, vp, sz, vp, sz]),
+        "sl_buf_read": (i32, [vp, vp, sz, vp, sz]),
+        "sl_scene_create": (i32, [vp, C.POINTER(SceneDesc), C.POINTER(vp)]),
+        "sl_scene_destroy": (None, [vp, vp]),
+        "sl_scene_write_parts": (i32, [vp, vp, vp, sz]),
+        "sl_scene_write_colour": (i32, [vp, vp, vp, sz]),
+        "sl_scene_write_bounce": (i32, [vp, vp, vp, sz]),
+        "sl_scene_device_bytes": (sz, [vp]),
+        "sl_frame_encode": (i32, [vp, vp, vp, sz, C.POINTER(FrameOpts), vp]),
+        "sl_frame_finish": (i32, [vp, C.POINTER(FrameStats)]),
+        "sl_frame_read": (i32, [vp, u32, vp, sz, C.POINTER(sz)]),
+        "sl_timers_read": (i32, [vp, C.POINTER(C.c_char_p), C.POINTER(C.c_float), u32, C.POINTER(u32)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Context:
+    """One device and one HIP stream (sl_context)."""
+
+    def __init__(self, device: int = 0):
+        self.L = lib()
+        h = C.c_void_p()
+        st = self.L.sl_ctx_create(device, C.byref(h))
+        if st != SL_OK:
+            raise SplatlightError(st, f"sl_ctx_create({device}) failed (no usable HIP device?)")
+        self.h = h
+
+    def check(self, st: int):
+        if st != SL_OK:
+            raise SplatlightError(st, (self.L.sl_last_error(self.h) or b"").decode())
+
+    def close(self):
+        if self.h:
+            self.L.sl_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        self.check(self.L.sl_sync(self.h))
+
+    # buffers
+    def alloc(self, nbytes: int) -> int:
+        p = C.c_void_p()
+        self.check(self.L.sl_buf_alloc(self.h, nbytes, C.byref(p)))
+        return p.value
+
+    def free(self, dev: int):
+        self.check(self.L.sl_buf_free(self.h, C.c_void_p(dev)))
+
+    def write(self, dev: int, host: np.ndarray, offset: int = 0):
+        host = np.ascontiguousarray(host)
+        self.check(self.L.sl_buf_write(self.h, C.c_void_p(dev), offset, _ptr(host), host.nbytes))
+
+    def read(self, dev: int, nbytes: int, dtype=np.uint8, offset: int = 0) -> np.ndarray:
+        out = np.empty(nbytes // np.dtype(dtype).itemsize, dtype)
+        self.check(self.L.sl_buf_read(self.h, C.c_void_p(dev), offset, _ptr(out), out.nbytes))
+        return out
+
+
+class Scene:
+    """Device-resident scene records (sl_scene)."""
+
+    def __init__(self, ctx: Context, gaussians: np.ndarray, parts: np.ndarray, emitters: np.ndarray | None = None,
+                 materials: np.ndarray | None = None, nodes: np.ndarray | None = None,
+                 vgrid: np.ndarray | None = None, storage: int = SL_STORAGE_F32):
+        self.ctx = ctx
+        self._keep = [np.ascontiguousarray(a) for a in (gaussians, parts)]
+        g, p = self._keep
+        d = SceneDesc()
+        d.gaussians, d.gaussian_count, d.gaussian_stride = _ptr(g), g.shape[0], g.dtype.itemsize
+        d.parts, d.part_count, d.part_stride = _ptr(p), p.shape[0], p.dtype.itemsize
+        if materials is not None:
+            m = np.ascontiguousarray(materials)
+            self._keep.append(m)
+            d.materials, d.material_count, d.material_stride = _ptr(m), m.shape[0], m.dtype.itemsize
+        if nodes is not None:
+            n = np.ascontiguousarray(nodes)
+            self._keep.append(n)
+            d.nodes, d.node_count, d.node_stride = _ptr(n), n.shape[0], n.dtype.itemsize
+        if vgrid is not None:
+            v = np.ascontiguousarray(vgrid)
+            self._keep.append(v)
+            d.vgrid, d.vgrid_count, d.vgrid_stride = _ptr(v), v.shape[0], v.reshape(v.shape[0], -1).shape[1] * v.itemsize
+        if emitters is not None and emitters.shape[0] > 0:
+            e = np.ascontiguousarray(emitters)
+            self._keep.append(e)
+            d.emitters, d.emitter_count = _ptr(e), e.shape[0]
+        d.storage = storage
+        h = C.c_void_p()
+        ctx.check(ctx.L.sl_scene_create(ctx.h, C.byref(d), C.byref(h)))
+        self.h = h
+        self.count = g.shape[0]
+        self.n_emit = 0 if emitters is None else emitters.shape[0]
+        self._keep = None
+
+    def destroy(self):
+        if self.h:
+            self.ctx.L.sl_scene_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def write_parts(self, parts: np.ndarray):
+        p = np.ascontiguousarray(parts)
+        self.ctx.check(self.ctx.L.sl_scene_write_parts(self.ctx.h, self.h, _ptr(p), p.nbytes))
+
+    def write_colour(self, rgba: np.ndarray):
+        c = np.ascontiguousarray(rgba, dtype=np.float32).reshape(-1, 4)
+        self.ctx.check(self.ctx.L.sl_scene_write_colour(self.ctx.h, self.h, _ptr(c), c.shape[0]))
+
+    def write_bounce(self, lo_b: np.ndarray | None):
+        if lo_b is None:
+            self.ctx.check(self.ctx.L.sl_scene_write_bounce(self.ctx.h, self.h, None, 0))
+            return
+        b = np.ascontiguousarray(lo_b, dtype=np.float32).reshape(-1, 12)
+        self.ctx.check(self.ctx.L.sl_scene_write_bounce(self.ctx.h, self.h, _ptr(b), b.shape[0]))
+
+    @property
+    def device_bytes(self) -> int:
+        return int(self.ctx.L.sl_scene_device_bytes(self.h))
+
+
+def frame_encode(ctx: Context, scene: Scene, frame: bytes, width: int, height: int, out_dev: int,
+                 out_format: int = SL_OUT_RGBA32F, timers: bool = False):
+    fb = (C.c_uint8 * len(frame)).from_buffer_copy(frame)
+    o = FrameOpts(width, height, out_format, SL_FRAME_TIMERS if timers else 0)
+    ctx.check(ctx.L.sl_frame_encode(ctx.h, scene.h, fb, len(frame), C.byref(o), C.c_void_p(out_dev)))
+
+
+def frame_finish(ctx: Context) -> FrameStats:
+    s = FrameStats()
+    ctx.check(ctx.L.sl_frame_finish(ctx.h, C.byref(s)))
+    return s
+
+
+def frame_read(ctx: Context, which: int, dtype=np.uint32) -> np.ndarray:
+    n = C.c_size_t()
+    ctx.check(ctx.L.sl_frame_read(ctx.h, which, None, 0, C.byref(n)))
+    out = np.empty(n.value // np.dtype(dtype).itemsize, dtype)
+    ctx.check(ctx.L.sl_frame_read(ctx.h, which, _ptr(out), out.nbytes, C.byref(n)))
+    return out
+
+
+def timers(ctx: Context) -> dict:
+    names = (C.c_char_p * 16)()
+    ms = (C.c_float * 16)()
+    n = C.c_uint32()
+    ctx.check(ctx.L.sl_timers_read(ctx.h, names, ms, 16, C.byref(n)))
+    return {names[k].decode(): float(ms[k]) for k in range(n.value)}
