@@ -53,6 +53,37 @@ void or_composite(const void *frame, const uint32_t *splats, const uint32_t *ord
 void or_composite_f64(const void *frame, const double *splats, const uint32_t *order, uint32_t visible,
                       uint32_t W, uint32_t H, double *out);
 
+/* ---- relighting (lighting.c): float32 restatements of runtime/wgsl for the default M39 matte path ---------- */
+
+float or_erfc_pos(float z);                                                      /* common.wgsl:113-118 */
+float or_ein(float x);                                                           /* common.wgsl:226-239 */
+float or_footprint(const void *node, float detC, float wx, float wy, float wz);  /* common.wgsl:242-246 */
+float or_vgrid_lookup(const uint8_t *vgrid128, float wx, float wy, float wz);    /* common.wgsl:162-177 */
+/* iso_norm per leaf (iso_init.wgsl:29-56) */
+void or_iso_init(const void *gaussians, const void *nodes, const uint8_t *vgrid, uint32_t n, float *iso_norm);
+/* colour per Gaussian, vec4 (emissive.wgsl:853-980); lamp_T: emitter-light transmittance per Gaussian or NULL */
+void or_shade_emit(const void *U, const void *EP, const void *emit_nodes, const void *gaussians, const void *nodes,
+                   const uint8_t *vgrid, const void *parts, const float *lamp_T, const void *materials,
+                   const float *iso_norm, float *colour);
+/* emitter-light opacity map A (light_shadow.wgsl:191-392): acc [parts][res^2][3] u32 (zeroed by the caller),
+ * mip [parts][layer_texels] vec4, T per Gaussian */
+void or_lsm_splat(const void *U, const void *LS, const void *gaussians, const void *nodes, const void *parts,
+                  uint32_t *acc);
+void or_lsm_resolve(const void *LS, const uint32_t *acc, float *mip);
+void or_lsm_lookup(const void *U, const void *LS, const void *gaussians, const void *parts, const float *mip,
+                   float *T_out);
+/* sun maps (sun_map.wgsl:91-324): layers 12 floats each; acc [layers][res^2][3]; env_T vec4 per Gaussian */
+void or_sun_frame(const void *SM, const void *desc, const void *parts, float *layers);
+void or_sun_splat(const void *U, const void *SM, const void *desc, const float *layers, const void *gaussians,
+                  const void *nodes, const void *parts, uint32_t *acc);
+void or_sun_resolve(const void *SM, const void *desc, const uint32_t *acc, float *mip);
+void or_sun_lookup(const void *U, const void *SM, const void *desc, const float *layers, const void *gaussians,
+                   const void *parts, const float *mip, float *env_T);
+/* colour += the distant lights' direct light (environment.wgsl:207-229, 266-277) */
+void or_env_direct(const void *U, const void *ENV, const void *gaussians, const void *nodes, const uint8_t *vgrid,
+                   const void *parts, const void *materials, const float *iso_norm, const float *env_T,
+                   float *colour);
+
 #ifdef __cplusplus
 }
 #endif

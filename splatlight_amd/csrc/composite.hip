@@ -1,20 +1,22 @@
 // Front-to-back compositing of 16x16 tiles (the blend of shaders/splat.wgsl:22-78 with the "over" state of
 // src/renderer.js:140-142, evaluated per tile instead of by the rasteriser).
 //
-// One 256-lane workgroup per tile, one lane per pixel. The tile's pair range is walked from its end, which is
-// the reference's back-to-front sequence for every pixel reversed, in batches of 256 splats: each lane stages
-// one splat into LDS already reduced to what a pixel needs (centre, the inverse of the quad's axes, the Gaussian
-// falloff and optical depth folded into exp2 arguments, colour), then every lane sweeps the batch. A pixel stops
-// once its transmittance is below t_eps = 2^-12 / max(1, max|rgb|) (so the colour it no longer adds is below
-// 2^-12 absolute), a wave skips batches when all its pixels stopped, and the workgroup leaves when all did.
+// One 256-lane workgroup per tile, launched heaviest tile first; each wave owns an 8x8 pixel block of it, one lane per pixel. The tile's pair
+// range is walked from its end, which is the reference's back-to-front sequence for every pixel reversed, in
+// batches of 256 splats: each lane stages one splat into LDS already reduced to what a pixel needs, then each
+// wave culls the batch against its 8x8 block (64 splats per ballot, tight ellipse bounds) and sweeps only the
+// survivors, in order. A pixel stops once its transmittance is below t_eps = 2^-12 / max(1, max|rgb|) (the
+// colour it no longer adds is below 2^-12 absolute), a wave stops when all its pixels stopped, and the
+// workgroup leaves when all waves did.
 //
-// Pixel (px, py) has its centre at NDC ((px + 0.5) 2 / W - 1, 1 - (py + 0.5) 2 / H), rows top-down; a pixel is
-// covered by a splat when its centre lies in the quad c + u a0 + v a1, |u|, |v| <= 1, and its alpha
-// 1 - exp(-tau0 exp(-r^2 (u^2 + v^2) / 2)) reaches 1/255. The fast path evaluates u, v and alpha with folded
-// constants and exp2; a fragment within a hair of the quad edge or of the 1/255 threshold (|u| or |v| within
-// 1e-4 of 1, alpha within 0.1% of 1/255) is decided again with the exact expression: u, v by the 2x2 solve,
-// exp as the correctly rounded float exp, no contraction. Coverage is then decided exactly as the CPU
-// restatement decides it, so a fragment never lands on the other side of the discard.
+// Coverage. Pixel (px, py) has its centre at NDC ((px + 0.5) 2 / W - 1, 1 - (py + 0.5) 2 / H), rows top-down; it
+// is covered by a splat when its centre lies in the quad c + u a0 + v a1, |u|, |v| <= 1, and its alpha
+// 1 - exp(-tau0 exp(-r^2 (u^2 + v^2) / 2)) reaches 1/255. Since r^2 = 2 ln(tau0 / tau_min), alpha >= 1/255 is
+// q = u^2 + v^2 <= 1 in exact arithmetic, and q <= 1 puts the centre inside the quad. The fast path therefore
+// evaluates u and v with the exp2 scale folded into the inverse axes (its rounding error grows only linearly
+// with the splat's aspect ratio), and a fragment with q within max(1e-4, 1e-6 aspect) of 1 is decided again with
+// the exact expression of the CPU restatement (u, v by the 2x2 solve, exp as the correctly rounded float exp,
+// no contraction), so no fragment lands on the other side of the discard.
 #include "sl_device.h"
 #include <hip/hip_fp16.h>
 
@@ -23,12 +25,15 @@
 #define LOG2E 1.4426950408889634f
 
 struct __align__(16) StagedSplat {
-  float cx, cy, ux, uy;      // centre (NDC); u = dx ux + dy uy
-  float vx, vy, kq, kt;      // v = dx vx + dy vy; e = exp2(kq (u^2 + v^2)); alpha = 1 - exp2(kt e)
-  float r, g, b, pad;
+  float cx, cy, ux, uy;      // centre (NDC); u' = dx ux + dy uy = u sqrt(r^2 log2(e) / 2)
+  float vx, vy, lo, hi;      // v' likewise; s = u'^2 + v'^2: s > hi skip, s >= lo exact, else exp2(-s)
 };
 
-struct __align__(16) RawSplat {  // the splat's own words, for the exact re-evaluation
+struct __align__(16) SplatBounds {   // tight ellipse bounds (NDC), widened by a hair
+  float x0, x1, y0, y1;
+};
+
+struct __align__(16) RawSplat {      // the splat's own words, for the exact re-evaluation
   float a0x, a0y, a1x, a1y;
   float det, rr, tau0, pad;
 };
@@ -47,22 +52,26 @@ __device__ __noinline__ float exact_alpha(float dx, float dy, const RawSplat rs)
 
 extern "C" __global__ void __launch_bounds__(CP_THREADS)
 sl_composite(FrameDev f, const Splat *__restrict__ splats, const uint32_t *__restrict__ vals,
-             const uint2 *__restrict__ ranges, const FrameCounters *__restrict__ counters, void *__restrict__ out,
-             uint32_t out_format) {
+             const uint2 *__restrict__ ranges, const uint32_t *__restrict__ tile_order,
+             const FrameCounters *__restrict__ counters, void *__restrict__ out, uint32_t out_format) {
   __shared__ StagedSplat s_sp[CP_BATCH];
+  __shared__ float4 s_col[CP_BATCH];
+  __shared__ SplatBounds s_bd[CP_BATCH];
   __shared__ RawSplat s_raw[CP_BATCH];
 
-  // XCD-aware order: consecutive workgroups go round-robin over the 8 XCDs, so give each XCD a contiguous band
-  // of tiles (neighbouring tiles share splats, and each XCD has its own L2).
-  const uint32_t ntiles = f.tiles_x * f.tiles_y;
-  const uint32_t per = (ntiles + 7u) / 8u;
-  const uint32_t tile = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
-  if (tile >= ntiles) return;
+  // workgroup b takes the b-th heaviest tile (sl_tile_order): longest-first over the XCDs
+  const uint32_t tile = tile_order[blockIdx.x];
   const uint32_t tyi = tile / f.tiles_x, txi = tile - tyi * f.tiles_x;
-  const uint32_t px = txi * 16u + (threadIdx.x & 15u), py = tyi * 16u + (threadIdx.x >> 4);
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  const uint32_t bx = txi * 16u + (w & 1u) * 8u, by = tyi * 16u + (w >> 1) * 8u;   // this wave's 8x8 block
+  const uint32_t px = bx + (lane & 7u), py = by + (lane >> 3);
   const bool inside = px < f.width && py < f.height;
-  const float nx = __fsub_rn(__fdiv_rn(__fmul_rn(__fadd_rn((float)px, 0.5f), 2.0f), (float)f.width), 1.0f);
-  const float ny = __fsub_rn(1.0f, __fdiv_rn(__fmul_rn(__fadd_rn((float)py, 0.5f), 2.0f), (float)f.height));
+  const float fw = (float)f.width, fh = (float)f.height;
+  const float nx = __fsub_rn(__fdiv_rn(__fmul_rn(__fadd_rn((float)px, 0.5f), 2.0f), fw), 1.0f);
+  const float ny = __fsub_rn(1.0f, __fdiv_rn(__fmul_rn(__fadd_rn((float)py, 0.5f), 2.0f), fh));
+  // the wave's block in NDC (pixel centres)
+  const float wx0 = ((float)bx + 0.5f) * 2.0f / fw - 1.0f, wx1 = ((float)bx + 7.5f) * 2.0f / fw - 1.0f;
+  const float wy1 = 1.0f - ((float)by + 0.5f) * 2.0f / fh, wy0 = 1.0f - ((float)by + 7.5f) * 2.0f / fh;
   const float alpha_min = 1.0f / 255.0f;
   const float cmax = __uint_as_float(counters->color_max_bits);
   const float t_eps = 0x1p-12f / fmaxf(1.0f, cmax);
@@ -82,13 +91,20 @@ sl_composite(FrameDev f, const Splat *__restrict__ splats, const uint32_t *__res
       const float a1x = s.axis1_r_tau.x, a1y = s.axis1_r_tau.y;
       const float r = s.axis1_r_tau.z, tau0 = s.axis1_r_tau.w;
       const float det = a0x * a1y - a1x * a0y;
-      const float inv = det != 0.0f ? 1.0f / det : 0.0f;
+      const bool ok = det != 0.0f;                       // a degenerate quad covers nothing
+      const float inv = ok ? 1.0f / det : 0.0f;
+      const float kq = 0.5f * r * r * LOG2E;            // s = kq q
+      const float sk = sqrtf(kq) * inv;
+      const float la0 = sqrtf(a0x * a0x + a0y * a0y), la1 = sqrtf(a1x * a1x + a1y * a1y);
+      const float aspect = fmaxf(la0, la1) / fmaxf(fminf(la0, la1), 1e-30f);
+      const float m = fmaxf(1e-4f, 1e-6f * aspect);     // the band decided exactly
       StagedSplat o;
       o.cx = cx; o.cy = cy;
-      o.ux = a1y * inv; o.uy = -a1x * inv;
-      o.vx = -a0y * inv; o.vy = a0x * inv;
-      o.kq = -0.5f * r * r * LOG2E;
-      o.kt = det != 0.0f ? -tau0 * LOG2E : 0.0f;   // a degenerate quad covers nothing
+      o.ux = a1y * sk; o.uy = -a1x * sk;
+      o.vx = -a0y * sk; o.vy = a0x * sk;
+      o.lo = kq * (1.0f - m);
+      o.hi = kq * (1.0f + m);
+      s_sp[threadIdx.x] = o;
       float cr_ = __uint_as_float(s.color_depth.x), cg_ = __uint_as_float(s.color_depth.y),
             cb_ = __uint_as_float(s.color_depth.z);
       if (depth_view) {
@@ -97,44 +113,60 @@ sl_composite(FrameDev f, const Splat *__restrict__ splats, const uint32_t *__res
         t = fminf(fmaxf(t, 0.0f), 1.0f);
         cr_ = cg_ = cb_ = 1.0f - t;
       }
-      o.r = cr_; o.g = cg_; o.b = cb_; o.pad = 0.0f;
-      s_sp[threadIdx.x] = o;
+      s_col[threadIdx.x] = make_float4(cr_, cg_, cb_, -tau0 * LOG2E);
+      const float ex = sqrtf(a0x * a0x + a1x * a1x) * (1.0f + 2.0f * m) + 1e-7f;
+      const float ey = sqrtf(a0y * a0y + a1y * a1y) * (1.0f + 2.0f * m) + 1e-7f;
+      SplatBounds b;
+      b.x0 = cx - ex; b.x1 = cx + ex; b.y0 = cy - ey; b.y1 = cy + ey;
+      if (!ok) { b.x0 = 2.0f; b.x1 = -2.0f; }            // culled by every wave
+      s_bd[threadIdx.x] = b;
       RawSplat rs;
       rs.a0x = a0x; rs.a0y = a0y; rs.a1x = a1x; rs.a1y = a1y;
       rs.det = det; rs.rr = r; rs.tau0 = tau0; rs.pad = 0.0f;
       s_raw[threadIdx.x] = rs;
     }
     __syncthreads();
-    // wave vote: skip the sweep when every pixel of this wave has stopped
-    if (__ballot(!done) != 0ull) {
-      for (uint32_t k = 0; k < nb; k++) {
-        if (done) break;
+    for (uint32_t k0 = 0; k0 < nb; k0 += 64u) {
+      if (__ballot(!done) == 0ull) break;               // wave vote: every pixel of this wave has stopped
+      // cull 64 splats at once against this wave's block
+      const uint32_t kk = k0 + lane;
+      bool hit = false;
+      if (kk < nb) {
+        const SplatBounds b = s_bd[kk];
+        hit = b.x0 <= wx1 && b.x1 >= wx0 && b.y0 <= wy1 && b.y1 >= wy0;
+      }
+      uint64_t mask = __ballot(hit);
+      while (mask) {
+        const uint32_t k = k0 + (uint32_t)__ffsll((unsigned long long)mask) - 1u;
+        mask &= mask - 1ull;
+        if (done) continue;
         const StagedSplat sp = s_sp[k];
         const float dx = __fsub_rn(nx, sp.cx), dy = __fsub_rn(ny, sp.cy);
         const float u = dx * sp.ux + dy * sp.uy;
         const float v = dx * sp.vx + dy * sp.vy;
-        const float au = fabsf(u), av = fabsf(v);
-        if (au > 1.0001f || av > 1.0001f) continue;
-        const float e = __builtin_amdgcn_exp2f(sp.kq * (u * u + v * v));
-        float alpha = 1.0f - __builtin_amdgcn_exp2f(sp.kt * e);
-        if (au >= 0.9999f || av >= 0.9999f || fabsf(alpha - alpha_min) <= 1e-3f * alpha_min) {
+        const float s = u * u + v * v;
+        if (s > sp.hi) continue;                          // q beyond the band: alpha < 1/255
+        const float4 c = s_col[k];
+        float alpha;
+        if (s >= sp.lo) {                                 // q in the band around 1: decide exactly
           alpha = exact_alpha(dx, dy, s_raw[k]);
-          if (alpha < 0.0f) continue;
+          if (alpha < alpha_min) continue;
+        } else {
+          alpha = 1.0f - __builtin_amdgcn_exp2f(c.w * __builtin_amdgcn_exp2f(-s));
         }
-        if (alpha < alpha_min) continue;
         if (overdraw) { nfrag += 1.0f; continue; }
         const float wgt = T * alpha;
-        cr += wgt * sp.r;
-        cg += wgt * sp.g;
-        cb += wgt * sp.b;
-        T *= 1.0f - alpha;
+        cr += wgt * c.x;
+        cg += wgt * c.y;
+        cb += wgt * c.z;
+        T -= wgt;
         done = T < t_eps;
       }
     }
     if (!__syncthreads_or(!done)) break;
   }
   if (!inside) return;
-  float4 px4 = overdraw ? make_float4(nfrag, 0.0f, 0.0f, 0.0f) : make_float4(cr, cg, cb, 1.0f - T);
+  const float4 px4 = overdraw ? make_float4(nfrag, 0.0f, 0.0f, 0.0f) : make_float4(cr, cg, cb, 1.0f - T);
   const size_t o = (size_t)py * f.width + px;
   if (out_format == 0u) {
     reinterpret_cast<float4 *>(out)[o] = px4;

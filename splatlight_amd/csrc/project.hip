@@ -43,7 +43,8 @@ sl_project_kernel(FrameDev f, const float4 *__restrict__ g0, const float4 *__res
                   const float4 *__restrict__ e1, const float4 *__restrict__ e2, const float4 *__restrict__ e3,
                   const float4 *__restrict__ parts, const float4 *__restrict__ colour,
                   const float4 *__restrict__ lo_b, Splat *__restrict__ splats, uint32_t *__restrict__ keys,
-                  uint2 *__restrict__ tbox, FrameCounters *__restrict__ counters) {
+                  uint2 *__restrict__ tbox, uint2 *__restrict__ partials) {
+  __shared__ uint2 s_part[256 / SL_WAVE];
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   bool ok = i < f.nslots && i != f.count;
   float4 A, Q4, S4, C4;
@@ -214,13 +215,48 @@ sl_project_kernel(FrameDev f, const float4 *__restrict__ g0, const float4 *__res
     keys[i] = key;
     tbox[i] = box;
   }
-  // per-block visible count and colour max, one atomic each
+  // per-block visible count and colour max into partials[block] (no atomics: sl_frame_totals reduces them)
   const uint64_t vis = __ballot(ok);
   float wmax = cmax;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) wmax = fmaxf(wmax, __shfl_xor(wmax, o));
-  if (lane_id() == 0) {
-    if (vis) atomicAdd(&counters->visible, (uint32_t)__popcll(vis));
-    if (wmax > 0.0f) atomicMax(&counters->color_max_bits, __float_as_uint(wmax));
+  if (lane_id() == 0) s_part[threadIdx.x >> 6] = make_uint2((uint32_t)__popcll(vis), __float_as_uint(wmax));
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint2 t = s_part[0];
+#pragma unroll
+    for (int k = 1; k < 256 / SL_WAVE; k++) {
+      t.x += s_part[k].x;
+      t.y = max(t.y, s_part[k].y);   // non-negative floats order as their bits
+    }
+    partials[blockIdx.x] = t;
+  }
+}
+
+// Frame totals from project's per-block partials: visible count and colour max (one workgroup).
+extern "C" __global__ void __launch_bounds__(1024)
+sl_frame_totals(const uint2 *__restrict__ partials, uint32_t nblocks, FrameCounters *__restrict__ counters) {
+  __shared__ uint2 s_w[1024 / SL_WAVE];
+  uint2 t = make_uint2(0u, 0u);
+  for (uint32_t i = threadIdx.x; i < nblocks; i += 1024u) {
+    const uint2 p = partials[i];
+    t.x += p.x;
+    t.y = max(t.y, p.y);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    t.x += __shfl_xor(t.x, o);
+    t.y = max(t.y, (uint32_t)__shfl_xor(t.y, o));
+  }
+  if (lane_id() == 0) s_w[threadIdx.x >> 6] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint2 a = s_w[0];
+    for (int k = 1; k < 1024 / SL_WAVE; k++) {
+      a.x += s_w[k].x;
+      a.y = max(a.y, s_w[k].y);
+    }
+    counters->visible = a.x;
+    counters->color_max_bits = a.y;
   }
 }

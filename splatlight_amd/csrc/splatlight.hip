@@ -70,6 +70,7 @@ struct Workspace {
   Splat *splats = nullptr;
   uint32_t *keys = nullptr, *keys_alt = nullptr, *vals_alt = nullptr, *order = nullptr;
   uint2 *tbox = nullptr;
+  uint2 *partials = nullptr;      // project's per-block (visible, colour max)
   uint32_t *pk0 = nullptr, *pv0 = nullptr, *pk1 = nullptr, *pv1 = nullptr;
   // zeroed at the start of every frame:
   uint8_t *scratch = nullptr;
@@ -79,6 +80,7 @@ struct Workspace {
   uint32_t *dstatus[2] = {nullptr, nullptr}, *tstatus[2] = {nullptr, nullptr};
   uint64_t *chunk_status = nullptr;
   uint2 *ranges = nullptr;
+  uint32_t *tile_order = nullptr;  // composite launch order (written every frame)
 };
 
 struct sl_context {
@@ -154,6 +156,7 @@ sl_status ensure_workspace(sl_context *ctx, uint32_t slots, uint32_t pairs, uint
   const size_t o_keys = take((size_t)s_cap * 4), o_keys_alt = take((size_t)s_cap * 4);
   const size_t o_vals_alt = take((size_t)s_cap * 4), o_order = take((size_t)s_cap * 4);
   const size_t o_tbox = take((size_t)s_cap * 8);
+  const size_t o_partials = take((size_t)((s_cap + 255) / 256) * 8);
   const size_t o_pk0 = take((size_t)p_cap * 4), o_pv0 = take((size_t)p_cap * 4);
   const size_t o_pk1 = take((size_t)p_cap * 4), o_pv1 = take((size_t)p_cap * 4);
   const size_t o_scratch = off;
@@ -164,6 +167,7 @@ sl_status ensure_workspace(sl_context *ctx, uint32_t slots, uint32_t pairs, uint
   const size_t o_ts0 = take((size_t)p_tiles * 256 * 4), o_ts1 = take((size_t)p_tiles * 256 * 4);
   const size_t o_chunk = take((size_t)chunks * 8);
   const size_t o_ranges = take((size_t)t_cap * 8);
+  const size_t o_torder = take((size_t)t_cap * 4);
   const size_t total = off;
   void *base = nullptr;
   if (w.base) {
@@ -185,6 +189,7 @@ sl_status ensure_workspace(sl_context *ctx, uint32_t slots, uint32_t pairs, uint
   w.vals_alt = (uint32_t *)(b + o_vals_alt);
   w.order = (uint32_t *)(b + o_order);
   w.tbox = (uint2 *)(b + o_tbox);
+  w.partials = (uint2 *)(b + o_partials);
   w.pk0 = (uint32_t *)(b + o_pk0);
   w.pv0 = (uint32_t *)(b + o_pv0);
   w.pk1 = (uint32_t *)(b + o_pk1);
@@ -201,6 +206,7 @@ sl_status ensure_workspace(sl_context *ctx, uint32_t slots, uint32_t pairs, uint
   w.tstatus[1] = (uint32_t *)(b + o_ts1);
   w.chunk_status = (uint64_t *)(b + o_chunk);
   w.ranges = (uint2 *)(b + o_ranges);
+  w.tile_order = (uint32_t *)(b + o_torder);
   return SL_OK;
 }
 
@@ -287,7 +293,8 @@ sl_status launch_frame(sl_context *ctx, const FrameDev &f, sl_scene *sc, void *o
   if ((s = mark(1)) != SL_OK) return s;
   sl_project_kernel<<<grid_for(f.nslots, 256, 1u << 30), 256, 0, st>>>(
       f, sc->g[0], sc->g[1], sc->g[2], sc->g[3], sc->e[0], sc->e[1], sc->e[2], sc->e[3], sc->parts, sc->colour,
-      sc->lo_b, w.splats, w.keys, w.tbox, w.counters);
+      sc->lo_b, w.splats, w.keys, w.tbox, w.partials);
+  sl_frame_totals<<<1, 1024, 0, st>>>(w.partials, (f.nslots + 255u) / 256u, w.counters);
   if ((s = mark(2)) != SL_OK) return s;
   // depth order: stable sort of the 16-bit keys, values = slot
   const uint32_t d_tiles = (f.nslots + RS_TILE - 1) / RS_TILE;
@@ -312,9 +319,9 @@ sl_status launch_frame(sl_context *ctx, const FrameDev &f, sl_scene *sc, void *o
                                                              8u, w.tile_hist + 256, w.tstatus[1], w.tickets + 4);
   if ((s = mark(5)) != SL_OK) return s;
   sl_tile_ranges<<<grid_for(w.pairs_cap, 256, 4096), 256, 0, st>>>(w.pk0, w.counters, w.ranges);
+  sl_tile_order<<<1, 1024, 0, st>>>(w.ranges, ntiles, w.tile_order);
   if ((s = mark(6)) != SL_OK) return s;
-  const uint32_t per = (ntiles + 7u) / 8u;
-  sl_composite<<<per * 8u, CP_THREADS, 0, st>>>(f, w.splats, w.pv0, w.ranges, w.counters, out, fmt);
+  sl_composite<<<ntiles, CP_THREADS, 0, st>>>(f, w.splats, w.pv0, w.ranges, w.tile_order, w.counters, out, fmt);
   if ((s = mark(7)) != SL_OK) return s;
   if ((s = check_launch(ctx, "frame kernels")) != SL_OK) return s;
   SL_HIP(ctx, hipMemcpyAsync(ctx->counters_host, w.counters, sizeof(FrameCounters), hipMemcpyDeviceToHost, st));

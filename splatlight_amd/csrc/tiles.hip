@@ -125,11 +125,24 @@ sl_pairs_emit(const uint32_t *__restrict__ order, const uint2 *__restrict__ tbox
       const uint32_t ty = ty0 + local / bw, tx = tx0 + local % bw;
       const uint32_t tile = ty * tiles_x + tx;
       const uint64_t dst = base + j;
-      if (dst < capacity) {
+      const bool keep = dst < capacity;
+      if (keep) {
         pair_keys[dst] = tile;
         pair_vals[dst] = s_g[lo];
-        atomicAdd(&s_hist[tile & 255u], 1u);
-        atomicAdd(&s_hist[256u + ((tile >> 8) & 255u)], 1u);
+      }
+      // tile-id digit counts for the tile sort, one LDS add per distinct digit of the wave (neighbouring pairs
+      // share digits, so per-lane atomics would serialise on one address)
+#pragma unroll
+      for (int pass = 0; pass < 2; pass++) {
+        const uint32_t d = (tile >> (8 * pass)) & 255u;
+        uint64_t peers = __ballot(keep);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+          const bool bit = (d >> b) & 1u;
+          const uint64_t bal = __ballot(bit);
+          peers &= bit ? bal : ~bal;
+        }
+        if (keep && (peers & lanemask_lt()) == 0ull) atomicAdd(&s_hist[pass * 256 + d], (uint32_t)__popcll(peers));
       }
     }
     __syncthreads();
@@ -147,5 +160,38 @@ sl_tile_ranges(const uint32_t *__restrict__ keys, const FrameCounters *counters,
     const uint32_t t = keys[i];
     if (i == 0u || keys[i - 1u] != t) ranges[t].x = i;
     if (i == n - 1u || keys[i + 1u] != t) ranges[t].y = i + 1u;
+  }
+}
+
+// Composite launch order: tiles by pair count, heaviest first, in half-octave buckets (one workgroup). The
+// hardware hands out workgroups in launch order as earlier ones retire, so heavy tiles start first and the
+// light ones fill in behind them (greedy longest-first), and consecutive workgroups go to different XCDs, so
+// each XCD gets an even share of every cost class. Order within a bucket is arbitrary: a tile's pixels do not
+// depend on when it runs.
+extern "C" __global__ void __launch_bounds__(1024)
+sl_tile_order(const uint2 *__restrict__ ranges, uint32_t ntiles, uint32_t *__restrict__ order) {
+  __shared__ uint32_t s_cnt[64];
+  __shared__ uint32_t s_next[64];
+  if (threadIdx.x < 64u) s_cnt[threadIdx.x] = 0u;
+  __syncthreads();
+  auto bucket = [](uint32_t c) -> uint32_t {
+    if (c == 0u) return 63u;
+    const uint32_t msb = 31u - (uint32_t)__clz(c);
+    const uint32_t half = msb > 0u ? (c >> (msb - 1u)) & 1u : 0u;
+    return 62u - min(61u, 2u * msb + half);         // heavier -> smaller bucket index
+  };
+  for (uint32_t t = threadIdx.x; t < ntiles; t += 1024u) {
+    const uint2 r = ranges[t];
+    atomicAdd(&s_cnt[bucket(r.y - r.x)], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (int b = 0; b < 64; b++) { s_next[b] = run; run += s_cnt[b]; }
+  }
+  __syncthreads();
+  for (uint32_t t = threadIdx.x; t < ntiles; t += 1024u) {
+    const uint2 r = ranges[t];
+    order[atomicAdd(&s_next[bucket(r.y - r.x)], 1u)] = t;
   }
 }

@@ -17,7 +17,10 @@ this module writes:
   (runtime/wgsl/emissive.wgsl:898).
 * Lights: emissive Gaussians in the `emitters` buffer (world space, albedo = Le), drawn in the splat slots after
   `count + 1` (shaders/project.wgsl:123-127) and lighting the scene through the emitter tree
-  (src/lighting.js:99-169) with EP.scale 1 and two-sided emission.
+  (src/lighting.js:99-169) with EP.scale 1; the Cornell lamp emits one-sided toward its short axis (down), the
+  hill's lanterns two-sided. The hill's sun: 35 degrees up, azimuth 30 degrees from +x, angular std 0.27 degrees.
+* Parts: the Cornell room (walls), short box and tall box; the hill's ground, trunk and canopy. Shadows between
+  parts come from the opacity maps (a part's own occlusion is its visibility grid's).
 * Leaf nodes are derived from the Gaussian: mean mu, radius 3 max(s), cinv = R diag(1/s^2) R^T, kappa, smax =
   max(s); the baked SGGX S_hat is taken as the identity and the visibility grid as 255 (V = 1) because their
   bakers are not part of the reference.
@@ -45,7 +48,8 @@ class Scene:
     width: int
     height: int
     far: float
-    sun: dict | None = None          # {"dir": unit vector toward the sun, "E": rgb, "res": int}
+    sun: dict | None = None          # {"dir": unit vector toward the sun, "E": rgb, "std": rad, "res": int}
+    one_sided: bool = False          # emitters emit only toward +short axis (EmitParams.one_sided)
 
     @property
     def count(self) -> int:
@@ -121,12 +125,18 @@ def _emitter_records(mu, le, scale, quat=None):
     return e
 
 
-def _bound_part(g):
-    mu = g["mu"].astype(np.float64)
-    lo, hi = mu.min(0), mu.max(0)
-    c = 0.5 * (lo + hi)
-    r = float(np.linalg.norm(hi - lo) * 0.5 + 3 * g["scale"].max())
-    return R.identity_part(0, g.shape[0], bound=(*c, r))
+def _parts(g, part_of):
+    """Parts table: identity poses, rest bounds (centre, radius) and Gaussian ranges; part_of must be sorted."""
+    g["part_surface"] = part_of.astype(np.uint32)
+    out = []
+    for p in range(int(part_of.max()) + 1):
+        idx = np.nonzero(part_of == p)[0]
+        mu = g["mu"][idx].astype(np.float64)
+        lo, hi = mu.min(0), mu.max(0)
+        c = 0.5 * (lo + hi)
+        r = float(np.linalg.norm(mu - c, axis=1).max() + 3 * g["scale"][idx].max())
+        out.append(R.identity_part(int(idx[0]), int(idx[-1]) + 1, bound=(*c, r)))
+    return np.concatenate(out)
 
 
 def _far(g):
@@ -165,13 +175,14 @@ def cornell(n: int = 200_000, seed: int = 0, width: int = 640, height: int = 480
         if col is not None:
             alb[m] = col
     g = _pack(rng, mu, scale_mu, 0.4, alb)
+    parts = _parts(g, np.select([which < 5, which < 10], [0, 1], 2))      # room, short box, tall box
     # 16 emitters: a 4 x 4 grid under the ceiling, short axis down (x -> -z)
     gx, gy = np.meshgrid(np.linspace(-0.3, 0.3, 4), np.linspace(-0.3, 0.3, 4))
     emu = np.stack([gx.ravel(), gy.ravel(), np.full(16, 1.97)], 1)
     s45 = math.sqrt(0.5)
     em = _emitter_records(emu, np.full((16, 3), 12.0), (0.005, 0.04, 0.04), quat=(0.0, s45, 0.0, s45))
-    return Scene("cornell", g, _bound_part(g), _materials(), em, (0.0, -3.5, 1.0), (0.0, 0.0, 1.0), 40.0,
-                 width, height, _far(g))
+    return Scene("cornell", g, parts, _materials(), em, (0.0, -3.5, 1.0), (0.0, 0.0, 1.0), 40.0,
+                 width, height, _far(g), one_sided=True)
 
 
 def hill(n: int = 1_000_000, seed: int = 1, width: int = 1920, height: int = 1080, n_emit: int = 64,
@@ -192,14 +203,15 @@ def hill(n: int = 1_000_000, seed: int = 1, width: int = 1920, height: int = 108
     mu = np.concatenate([ground, trunk, canopy])
     alb = rng.uniform(0.0, 1.0, (n, 3))
     g = _pack(rng, mu, -3.5, 0.5, alb)
+    parts = _parts(g, np.repeat(np.arange(3), [n_ground, n_trunk, n_canopy]))   # ground, trunk, canopy
     exy = rng.uniform(-8.0, 8.0, (n_emit, 2))
     ez = 0.3 * np.sin(exy[:, 0]) * np.cos(exy[:, 1]) + rng.uniform(0.3, 2.0, n_emit)
     le = rng.uniform(1.0, 20.0, n_emit)
     em = _emitter_records(np.column_stack([exy, ez]), np.repeat(le[:, None], 3, 1), (0.03, 0.03, 0.03))
     el, az = math.radians(35.0), math.radians(30.0)
     sun = {"dir": (math.cos(el) * math.cos(az), math.cos(el) * math.sin(az), math.sin(el)),
-           "E": (3.0, 2.9, 2.7), "res": shadow_res}
-    return Scene("hill", g, _bound_part(g), _materials(), em, (0.0, -15.0, 4.0), (0.0, 0.0, 1.5), 40.0,
+           "E": (3.0, 2.9, 2.7), "std": math.radians(0.27), "res": shadow_res}
+    return Scene("hill", g, parts, _materials(), em, (0.0, -15.0, 4.0), (0.0, 0.0, 1.5), 40.0,
                  width, height, _far(g), sun)
 
 
