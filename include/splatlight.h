@@ -59,8 +59,8 @@ typedef struct sl_scene_desc {
   const void *gaussians; uint32_t gaussian_count; uint32_t gaussian_stride;   /* 64 B records, <= 2^24 */
   const void *parts; uint32_t part_count; uint32_t part_stride;               /* 128 B records, <= 32 */
   const void *materials; uint32_t material_count; uint32_t material_stride;   /* 32 B records (may be 0) */
-  const void *nodes; uint32_t node_count; uint32_t node_stride;               /* 96 B leaf nodes (NULL: derived) */
-  const void *vgrid; uint32_t vgrid_count; uint32_t vgrid_stride;             /* 128 B grids (NULL: V = 1) */
+  const void *nodes; uint32_t node_count; uint32_t node_stride;               /* 96 B, leaves first (relighting) */
+  const void *vgrid; uint32_t vgrid_count; uint32_t vgrid_stride;             /* 128 B grids (NULL: all 255) */
   const void *emitters; uint32_t emitter_count;                               /* world-space emissive records */
   uint32_t storage;                                                           /* SL_STORAGE_* */
 } sl_scene_desc;
@@ -107,6 +107,39 @@ sl_status sl_frame_finish(sl_context *ctx, sl_frame_stats *stats);
 #define SL_ART_PAIR_SLOTS 5u   /* u32 per pair: splat slot, each tile's list in back-to-front order */
 #define SL_ART_RANGES 6u       /* 2 u32 per tile: [start, end) into the pair lists */
 sl_status sl_frame_read(sl_context *ctx, uint32_t which, void *host, size_t bytes, size_t *written);
+
+/* ---- relighting: Lighting.encode (src/lighting.js:1258-1274) for the default static M39 matte path ----------------
+ * The host keeps the reference's parameter logic and hands over the uniform bytes its writers produce. The scene
+ * must carry nodes (leaf Node records), materials and, optionally, vgrid (absent: every cell 255, V = 1). */
+typedef struct sl_lighting_desc {
+  const void *frame_uniforms;   /* 64 B FrameUniforms (runtime/wgsl/common.wgsl:39-47, src/lighting.js:1229-1238) */
+  const void *emit_params;      /* 64 B EmitParams (runtime/wgsl/emissive.wgsl:34-53, src/lighting.js:657-673) */
+  const void *emit_nodes;       /* EmitNode records, 96 B (src/lighting.js:61-65) */
+  uint32_t emit_node_count;
+  const void *ls_params;        /* 336 B LSParams (src/light_shadow.js:113-140); NULL: no emitter shadows (T = 1) */
+  const void *env;              /* 416 B Env (src/environment.js:111-153); NULL: no distant lights */
+  const void *sun_params;       /* 432 B SunParams (src/sun_map.js:133-150); NULL: distant lights unshadowed */
+  const void *sun_layers;       /* LayerDesc records, 32 B (src/sun_map.js:127-132) */
+  uint32_t sun_layer_count;
+  uint32_t flags;               /* SL_LIGHT_TIMERS */
+} sl_lighting_desc;
+#define SL_LIGHT_TIMERS 1u
+/* iso_norm of every leaf from its V grid (runIsoInit, src/lighting.js:546-555); once per asset */
+sl_status sl_lighting_iso_init(sl_context *ctx, sl_scene *scene);
+/* the emitter and sun opacity maps of dirty parts / layers (LSParams.dirty, LayerDesc.dirty), their lookups,
+ * shade_emit and env_direct, leaving the lit colour the next sl_frame_encode draws */
+sl_status sl_lighting_encode(sl_context *ctx, sl_scene *scene, const sl_lighting_desc *desc);
+
+/* Scene-resident lighting state, for parity checks: */
+#define SL_SCENE_COLOUR 1u     /* vec4 per Gaussian */
+#define SL_SCENE_LAMP_T 2u     /* float per Gaussian: emitter-light transmittance (lamp_state slot 0) */
+#define SL_SCENE_ENV_T 3u      /* vec4 per Gaussian: per distant light transmittance */
+#define SL_SCENE_ISO_NORM 4u   /* float per Gaussian (Node.pad0) */
+#define SL_SCENE_LSM_ACC 5u    /* u32 [parts][res^2][3]: the emitter map's fixed-point sums */
+#define SL_SCENE_LSM_MIP 6u    /* vec4 [parts][layer texels] */
+#define SL_SCENE_SUN_ACC 7u    /* u32 [layers][res^2][3] */
+#define SL_SCENE_SUN_MIP 8u    /* vec4 [layers][layer texels] */
+sl_status sl_scene_read(sl_context *ctx, sl_scene *scene, uint32_t which, void *host, size_t bytes, size_t *written);
 
 /* Per-pass GPU milliseconds of the last frame encoded with SL_FRAME_TIMERS; names as in the reference's PASSES
  * (src/renderer.js:7-8) where one exists. Fills up to `cap` entries; returns the count in *n. */

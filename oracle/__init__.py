@@ -46,6 +46,24 @@ def lib():
         L.or_composite.argtypes = [vp, vp, vp, u32, u32, u32, C.c_int, vp]
         L.or_composite_f64.restype = None
         L.or_composite_f64.argtypes = [vp, vp, vp, u32, u32, u32, vp]
+        f32 = C.c_float
+        L.or_erfc_pos.restype = f32
+        L.or_erfc_pos.argtypes = [f32]
+        L.or_ein.restype = f32
+        L.or_ein.argtypes = [f32]
+        L.or_footprint.restype = f32
+        L.or_footprint.argtypes = [vp, f32, f32, f32, f32]
+        L.or_vgrid_lookup.restype = f32
+        L.or_vgrid_lookup.argtypes = [vp, f32, f32, f32]
+        L.or_iso_init.restype = None
+        L.or_iso_init.argtypes = [vp, vp, vp, u32, vp]
+        L.or_shade_emit.restype = None
+        L.or_shade_emit.argtypes = [vp] * 11
+        for name, n in (("or_lsm_splat", 6), ("or_lsm_resolve", 3), ("or_lsm_lookup", 6), ("or_sun_frame", 4),
+                        ("or_sun_splat", 8), ("or_sun_resolve", 4), ("or_sun_lookup", 8), ("or_env_direct", 10)):
+            fn = getattr(L, name)
+            fn.restype = None
+            fn.argtypes = [vp] * n
         _lib = L
     return _lib
 
@@ -122,3 +140,81 @@ def composite_f64(frame: bytes, splats64, order, visible, W, H):
     lib().or_composite_f64(_p(fb), _p(np.ascontiguousarray(splats64, np.float64)),
                            _p(np.ascontiguousarray(order, np.uint32)), visible, W, H, _p(out))
     return out
+
+
+# ---- relighting -------------------------------------------------------------------------------------------------
+
+def _b(x):
+    return np.frombuffer(x, np.uint8).copy() if isinstance(x, (bytes, bytearray)) else np.ascontiguousarray(x)
+
+
+def erfc_pos(z):
+    return float(lib().or_erfc_pos(z))
+
+
+def ein(x):
+    return float(lib().or_ein(x))
+
+
+def footprint(node_row, detC, w):
+    return float(lib().or_footprint(_p(np.ascontiguousarray(node_row)), detC, *[float(v) for v in w]))
+
+
+def vgrid_lookup(vg128, w):
+    return float(lib().or_vgrid_lookup(_p(np.ascontiguousarray(vg128, np.uint8)), *[float(v) for v in w]))
+
+
+def iso_init(gaussians, nodes, vgrid):
+    n = gaussians.shape[0]
+    out = np.zeros(n, np.float32)
+    lib().or_iso_init(_p(_b(gaussians)), _p(_b(nodes)), _p(np.ascontiguousarray(vgrid, np.uint8)), n, _p(out))
+    return out
+
+
+def shade_emit(U, EP, emit_nodes, gaussians, nodes, vgrid, parts, lamp_T, materials, iso_norm):
+    n = gaussians.shape[0]
+    col = np.zeros((n, 4), np.float32)
+    lt = None if lamp_T is None else np.ascontiguousarray(lamp_T, np.float32)
+    lib().or_shade_emit(_p(_b(U)), _p(_b(EP)), _p(np.ascontiguousarray(emit_nodes, np.float32)), _p(_b(gaussians)),
+                        _p(_b(nodes)), _p(np.ascontiguousarray(vgrid, np.uint8)), _p(_b(parts)), _p(lt),
+                        _p(_b(materials)), _p(np.ascontiguousarray(iso_norm, np.float32)), _p(col))
+    return col
+
+
+def lsm(U, LS, gaussians, nodes, parts):
+    """(acc u32 [P, res^2, 3], mip f32 [P, layer_texels, 4], T f32 [G])."""
+    w = np.frombuffer(LS, np.uint32)
+    res, lt, pc = int(w[3]), int(w[11]), int(w[19])
+    acc = np.zeros((pc, res * res, 3), np.uint32)
+    mip = np.zeros((pc, lt, 4), np.float32)
+    T = np.zeros(gaussians.shape[0], np.float32)
+    L = lib()
+    L.or_lsm_splat(_p(_b(U)), _p(_b(LS)), _p(_b(gaussians)), _p(_b(nodes)), _p(_b(parts)), _p(acc))
+    L.or_lsm_resolve(_p(_b(LS)), _p(acc), _p(mip))
+    L.or_lsm_lookup(_p(_b(U)), _p(_b(LS)), _p(_b(gaussians)), _p(_b(parts)), _p(mip), _p(T))
+    return acc, mip, T
+
+
+def sun(U, SM, desc, gaussians, nodes, parts):
+    """(acc u32 [L, res^2, 3], mip f32 [L, layer_texels, 4], env_T f32 [G, 4])."""
+    w = np.frombuffer(SM, np.uint32)
+    nl, res, lt = int(w[0]), int(w[1]), int(w[3])
+    layers = np.zeros((nl, 12), np.float32)
+    acc = np.zeros((nl, res * res, 3), np.uint32)
+    mip = np.zeros((nl, lt, 4), np.float32)
+    T = np.zeros((gaussians.shape[0], 4), np.float32)
+    L = lib()
+    L.or_sun_frame(_p(_b(SM)), _p(_b(desc)), _p(_b(parts)), _p(layers))
+    L.or_sun_splat(_p(_b(U)), _p(_b(SM)), _p(_b(desc)), _p(layers), _p(_b(gaussians)), _p(_b(nodes)), _p(_b(parts)),
+                   _p(acc))
+    L.or_sun_resolve(_p(_b(SM)), _p(_b(desc)), _p(acc), _p(mip))
+    L.or_sun_lookup(_p(_b(U)), _p(_b(SM)), _p(_b(desc)), _p(layers), _p(_b(gaussians)), _p(_b(parts)), _p(mip), _p(T))
+    return acc, mip, T
+
+
+def env_direct(U, ENV, gaussians, nodes, vgrid, parts, materials, iso_norm, env_T, colour):
+    col = np.ascontiguousarray(colour, np.float32).copy()
+    lib().or_env_direct(_p(_b(U)), _p(_b(ENV)), _p(_b(gaussians)), _p(_b(nodes)), _p(np.ascontiguousarray(vgrid, np.uint8)),
+                        _p(_b(parts)), _p(_b(materials)), _p(np.ascontiguousarray(iso_norm, np.float32)),
+                        _p(np.ascontiguousarray(env_T, np.float32)), _p(col))
+    return col

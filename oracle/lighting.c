@@ -32,6 +32,10 @@ static v3 cross3(v3 a, v3 b) { return V3(a.y * b.z - a.z * b.y, a.z * b.x - a.x 
 static float clampf_(float v, float lo, float hi) { return v < lo ? lo : (v > hi ? hi : v); }
 static float mixf(float a, float b, float t) { return a * (1.0f - t) + b * t; }
 static float exp_cr(float x) { return (float)exp((double)x); }
+/* picks of a cell, mip level or cull decision: double, rounded once (as the device does) */
+static float log2_cr(float x) { return (float)log2((double)x); }
+static float asin_cr(float x) { return (float)asin((double)x); }
+static float acos_cr(float x) { return (float)acos((double)x); }
 static uint32_t u32_sat(float f) { /* WGSL u32(f32): truncate, saturate */
   if (!(f > 0.0f)) return 0u;
   if (f >= 4294967296.0f) return 0xFFFFFFFFu;
@@ -122,7 +126,7 @@ static float vgrid_cell(const uint8_t *v, uint32_t i, uint32_t j) { return (floa
 
 /* bilinear on the 8x16 cell centres (runtime/wgsl/common.wgsl:162-177) */
 float or_vgrid_lookup(const uint8_t *v, float wx, float wy, float wz) {
-  float el = asinf(clampf_(wz, -1.0f, 1.0f));
+  float el = asin_cr(clampf_(wz, -1.0f, 1.0f));
   float fi = clampf_((0.5f * PI_F - el) / (PI_F / 8.0f) - 0.5f, 0.0f, 7.0f);
   uint32_t i0 = (uint32_t)fi < 6u ? (uint32_t)fi : 6u;
   float ti = fi - (float)i0;
@@ -284,7 +288,7 @@ static void resolve_layer(const uint32_t *acc, float fix, uint32_t n, float *mip
     float s0 = (float)acc[3 * (size_t)i] / fix;
     float o[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     if (s0 > 1e-5f) {
-      float a = 1.0f - expf(-s0);
+      float a = (float)(-expm1(-(double)s0));   /* 1 - e^-s0 rounded once (the f32 form cancels for small s0) */
       o[0] = a;
       o[1] = a * (float)acc[3 * (size_t)i + 1] / (fix * s0);
       o[2] = a * (float)acc[3 * (size_t)i + 2] / (fix * s0);
@@ -488,11 +492,11 @@ void or_lsm_lookup(const void *U, const void *LS, const void *gaussians, const v
       if (z <= zn) continue;
       float search = 2.0f * sigma * (z - zn) / (zn * z);
       if (dc > bw) {
-        float ang = acosf(clampf_(dot3(V3(bc.x / dc, bc.y / dc, bc.z / dc), w), -1.0f, 1.0f));
-        if (ang > asinf(bw / dc) + search + 2.0f * tex_ang) continue;
+        float ang = acos_cr(clampf_(dot3(V3(bc.x / dc, bc.y / dc, bc.z / dc), w), -1.0f, 1.0f));
+        if (ang > asin_cr(bw / dc) + search + 2.0f * tex_ang) continue;
       }
       float x = 2.0f * search / tex_ang;
-      float ls = log2f(x > 1.0f ? x : 1.0f);
+      float ls = log2_cr(x > 1.0f ? x : 1.0f);
       float a0[4], s[4];
       const float *mp = mip + 4 * (size_t)p * lt;
       mip_tri(mp, level_off, res, levels, ceilf(ls), tc, a0);
@@ -501,7 +505,7 @@ void or_lsm_lookup(const void *U, const void *LS, const void *gaussians, const v
       zb = zb > 0.02f ? zb : 0.02f;
       float pen = sigma * (z - zb > 0.0f ? z - zb : 0.0f) / (zb * z);
       x = 2.0f * pen / tex_ang;
-      mip_tri(mp, level_off, res, levels, log2f(x > 1.0f ? x : 1.0f), tc, s);
+      mip_tri(mp, level_off, res, levels, log2_cr(x > 1.0f ? x : 1.0f), tc, s);
       if (s[0] < 1e-5f) continue;
       float mu = s[1] / s[0];
       float var = s[2] / s[0] - mu * mu;
@@ -647,12 +651,12 @@ void or_sun_lookup(const void *U, const void *SM, const void *desc, const float 
       float x = 2.0f * search / texel;
       float a0[4], s[4];
       const float *mp = mip + 4 * (size_t)l * lt;
-      mip_tri(mp, level_off, res, levels, log2f(x > 1.0f ? x : 1.0f), tc, a0);
+      mip_tri(mp, level_off, res, levels, log2_cr(x > 1.0f ? x : 1.0f), tc, a0);
       if (a0[0] < 1e-4f) continue;
       float zb = a0[1] / a0[0];
       float pen = sig * (z - zb > 0.0f ? z - zb : 0.0f);
       x = 2.0f * pen / texel;
-      mip_tri(mp, level_off, res, levels, log2f(x > 1.0f ? x : 1.0f), tc, s);
+      mip_tri(mp, level_off, res, levels, log2_cr(x > 1.0f ? x : 1.0f), tc, s);
       if (s[0] < 1e-5f) continue;
       float mu = s[1] / s[0];
       float var = s[2] / s[0] - mu * mu;

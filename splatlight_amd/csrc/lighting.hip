@@ -1,0 +1,843 @@
+// Relighting on gfx950: iso_norm per leaf, the emitter-light cut walk, the emitter and sun opacity maps, and the
+// distant lights' direct light (runtime/wgsl/{iso_init,emissive,light_shadow,sun_map,environment}.wgsl, default
+// static M39 matte path).
+//
+// One lane per Gaussian everywhere except the map resolve/down passes (one lane per texel). Contraction is off in
+// this file so every float expression rounds as written in WGSL order; the map splats take exp as the correctly
+// rounded float exp, so their fixed-point sums (integer atomics, order-independent) are bit-identical to any
+// implementation using the same definition.
+#pragma clang fp contract(off)
+#include "sl_device.h"
+#include <hip/hip_fp16.h>
+
+namespace lt {
+
+constexpr float PI_F = 3.14159265358979f;
+constexpr uint32_t NO_CHILD = 0xFFFFFFFFu;
+
+struct v3 { float x, y, z; };
+struct v2 { float x, y; };
+struct m2 { v2 c0, c1; };        // column-major
+struct m3 { v3 c0, c1, c2; };
+
+__device__ __forceinline__ v3 V3(float x, float y, float z) { return {x, y, z}; }
+__device__ __forceinline__ v2 V2(float x, float y) { return {x, y}; }
+__device__ __forceinline__ v3 add3(v3 a, v3 b) { return V3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ v3 sub3(v3 a, v3 b) { return V3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ v3 mul3s(v3 a, float s) { return V3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ float dot3(v3 a, v3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ __forceinline__ float dot2(v2 a, v2 b) { return a.x * b.x + a.y * b.y; }
+__device__ __forceinline__ float len3(v3 a) { return sqrtf(dot3(a, a)); }
+__device__ __forceinline__ v3 norm3(v3 a) { const float l = len3(a); return V3(a.x / l, a.y / l, a.z / l); }
+__device__ __forceinline__ v3 cross3(v3 a, v3 b) {
+  return V3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float clampf_(float v, float lo, float hi) { return v < lo ? lo : (v > hi ? hi : v); }
+__device__ __forceinline__ float mixf(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+__device__ __forceinline__ float exp_cr(float x) { return (float)exp((double)x); }
+// transcendentals whose result picks a cell, a mip level or a cull decision: double, rounded once, so the
+// choice does not hang on a library's last-ulp behaviour
+__device__ __forceinline__ float log2_cr(float x) { return (float)log2((double)x); }
+__device__ __forceinline__ float asin_cr(float x) { return (float)asin((double)x); }
+__device__ __forceinline__ float acos_cr(float x) { return (float)acos((double)x); }
+__device__ __forceinline__ uint32_t u32_sat(float f) {   // WGSL u32(f32): truncate, saturate
+  if (!(f > 0.0f)) return 0u;
+  if (f >= 4294967296.0f) return 0xFFFFFFFFu;
+  return (uint32_t)f;
+}
+__device__ __forceinline__ v3 m3v(m3 m, v3 v) { return add3(add3(mul3s(m.c0, v.x), mul3s(m.c1, v.y)), mul3s(m.c2, v.z)); }
+__device__ __forceinline__ m3 m3mul(m3 a, m3 b) { return {m3v(a, b.c0), m3v(a, b.c1), m3v(a, b.c2)}; }
+__device__ __forceinline__ m3 m3t(m3 m) {
+  return {V3(m.c0.x, m.c1.x, m.c2.x), V3(m.c0.y, m.c1.y, m.c2.y), V3(m.c0.z, m.c1.z, m.c2.z)};
+}
+__device__ __forceinline__ v2 m2v(m2 m, v2 v) { return V2(m.c0.x * v.x + m.c1.x * v.y, m.c0.y * v.x + m.c1.y * v.y); }
+__device__ __forceinline__ m2 m2mul(m2 a, m2 b) { return {m2v(a, b.c0), m2v(a, b.c1)}; }
+__device__ __forceinline__ m2 m2t(m2 m) { return {V2(m.c0.x, m.c1.x), V2(m.c0.y, m.c1.y)}; }
+__device__ __forceinline__ m2 m2s(m2 m, float s) { return {V2(m.c0.x * s, m.c0.y * s), V2(m.c1.x * s, m.c1.y * s)}; }
+__device__ __forceinline__ float m2det(m2 m) { return m.c0.x * m.c1.y - m.c1.x * m.c0.y; }
+__device__ __forceinline__ m2 inv2(m2 m) {   // light_shadow.wgsl:112-115
+  const float d = m2det(m);
+  return m2s({V2(m.c1.y, -m.c0.y), V2(-m.c1.x, m.c0.x)}, 1.0f / d);
+}
+__device__ __forceinline__ m3 quat_matrix(float4 q) {   // runtime/wgsl/common.wgsl:218-223
+  const float x = q.x, y = q.y, z = q.z, w = q.w;
+  return {V3(1.0f - 2.0f * (y * y + z * z), 2.0f * (x * y + z * w), 2.0f * (x * z - y * w)),
+          V3(2.0f * (x * y - z * w), 1.0f - 2.0f * (x * x + z * z), 2.0f * (y * z + x * w)),
+          V3(2.0f * (x * z + y * w), 2.0f * (y * z - x * w), 1.0f - 2.0f * (x * x + y * y))};
+}
+
+struct Xf { v3 r0, r1, r2; v3 t; };   // a part's world_from_rest rows [R row | t]
+__device__ __forceinline__ Xf load_xf(const float4 *parts, uint32_t p) {
+  const float4 a = parts[p * 8 + 0], b = parts[p * 8 + 1], c = parts[p * 8 + 2];
+  return {V3(a.x, a.y, a.z), V3(b.x, b.y, b.z), V3(c.x, c.y, c.z), V3(a.w, b.w, c.w)};
+}
+__device__ __forceinline__ v3 xf_point(const Xf &x, v3 p) {
+  return V3(dot3(x.r0, p) + x.t.x, dot3(x.r1, p) + x.t.y, dot3(x.r2, p) + x.t.z);
+}
+__device__ __forceinline__ v3 xf_dir_inv(const Xf &x, v3 d) {
+  return add3(add3(mul3s(x.r0, d.x), mul3s(x.r1, d.y)), mul3s(x.r2, d.z));
+}
+__device__ __forceinline__ m3 part_rot(const Xf &x) { return m3t({x.r0, x.r1, x.r2}); }
+
+__device__ __forceinline__ float erfc_pos(float z) {   // runtime/wgsl/common.wgsl:113-118
+  const float t = 1.0f / (1.0f + 0.5f * z);
+  const float p = -z * z - 1.26551223f + t * (1.00002368f + t * (0.37409196f + t * (0.09678418f + t * (-0.18628806f
+                  + t * (0.27886807f + t * (-1.13520398f + t * (1.48851587f + t * (-0.82215223f + t * 0.17087277f))))))));
+  return t * expf(p);
+}
+__device__ __forceinline__ float phi_(float x) {
+  const float a = fabsf(x) * 0.70710678f;
+  const float h = 0.5f * erfc_pos(a);
+  return x >= 0.0f ? 1.0f - h : h;
+}
+__device__ __forceinline__ v3 sym_mul(v3 dg, v3 off, v3 v) {
+  return V3((dg.x * v.x + off.x * v.y) + off.y * v.z, (off.x * v.x + dg.y * v.y) + off.z * v.z,
+            (off.y * v.x + off.z * v.y) + dg.z * v.z);
+}
+
+// Node (96 B): mean 0, radius 3, cinv_diag 4, kappa 7, cinv_off 8, smax 11, sn_diag 12, part 15, sn_off 16
+struct NodeL { v3 cinv_d, cinv_o, sn_d, sn_o; float kappa; };
+__device__ __forceinline__ NodeL load_node(const float4 *nodes, uint32_t g) {
+  const float4 *n = nodes + 6 * (size_t)g;
+  const float4 a = n[1], b = n[2], c = n[3], d = n[4];
+  return {V3(a.x, a.y, a.z), V3(b.x, b.y, b.z), V3(c.x, c.y, c.z), V3(d.x, d.y, d.z), a.w};
+}
+__device__ __forceinline__ float dirm(const NodeL &nd, v3 w) {
+  const float q = dot3(w, sym_mul(nd.sn_d, nd.sn_o, w));
+  return sqrtf(q > 0.0f ? q : 0.0f);
+}
+__device__ __forceinline__ float ein(float x) {   // runtime/wgsl/common.wgsl:226-239
+  if (x < 1.0f) {
+    float term = x, s = x;
+#pragma unroll
+    for (int k = 2; k <= 12; k++) {
+      term *= -x / (float)k;
+      s += term / (float)k;
+    }
+    return s;
+  }
+  const float p = x * x * x * x + 8.5733287401f * x * x * x + 18.0590169730f * x * x + 8.6347608925f * x + 0.2677737343f;
+  const float q = x * x * x * x + 9.5733223454f * x * x * x + 25.6329561486f * x * x + 21.0996530827f * x + 3.9584969228f;
+  return logf(x) + 0.5772156649f + p / q / (x * expf(x));
+}
+__device__ __forceinline__ float footprint(const NodeL &nd, float detC, v3 w) {   // common.wgsl:242-246
+  const float a = dot3(w, sym_mul(nd.cinv_d, nd.cinv_o, w));
+  const float K = nd.kappa * dirm(nd, w) * sqrtf(2.0f * PI_F / a);
+  return 2.0f * PI_F * sqrtf(detC * a) * ein(K);
+}
+__device__ __forceinline__ float vgrid_cell(const uint8_t *v, uint32_t i, uint32_t j) {
+  return (float)v[i * 16u + j] * (1.0f / 255.0f);
+}
+__device__ __forceinline__ float vgrid_lookup(const uint8_t *v, v3 w) {   // common.wgsl:162-177
+  const float el = asin_cr(clampf_(w.z, -1.0f, 1.0f));
+  const float fi = clampf_((0.5f * PI_F - el) / (PI_F / 8.0f) - 0.5f, 0.0f, 7.0f);
+  const uint32_t i0 = min((uint32_t)fi, 6u);
+  const float ti = fi - (float)i0;
+  const float az = (fabsf(w.x) + fabsf(w.y) < 1e-30f) ? 0.0f : atan2f(w.y, w.x);
+  const float fj = (az + PI_F) / (2.0f * PI_F) * 16.0f - 0.5f;
+  const float j0f = floorf(fj);
+  const float tj = fj - j0f;
+  const uint32_t j0 = (uint32_t)((int32_t)j0f + 16) % 16u;
+  const uint32_t j1 = (j0 + 1u) % 16u;
+  const float a = mixf(vgrid_cell(v, i0, j0), vgrid_cell(v, i0, j1), tj);
+  const float b = mixf(vgrid_cell(v, i0 + 1u, j0), vgrid_cell(v, i0 + 1u, j1), tj);
+  return mixf(a, b, ti);
+}
+__device__ __forceinline__ v3 basis_k(v3 w) {
+  v3 a = V3(0.0f, 0.0f, 1.0f);
+  if (fabsf(w.z) >= 0.999f) a = V3(1.0f, 0.0f, 0.0f);
+  return norm3(cross3(a, w));
+}
+__device__ __forceinline__ m3 leaf_cov(float4 q, float4 s, m3 R) {
+  const m3 Rq = quat_matrix(q);
+  const m3 S = {V3(s.x * s.x, 0.0f, 0.0f), V3(0.0f, s.y * s.y, 0.0f), V3(0.0f, 0.0f, s.z * s.z)};
+  const m3 C = m3mul(m3mul(Rq, S), m3t(Rq));
+  return m3mul(m3mul(R, C), m3t(R));
+}
+__device__ __forceinline__ float unpack_half(uint32_t h) { return __half2float(__ushort_as_half((unsigned short)h)); }
+
+}  // namespace lt
+
+// Uniform layouts (the bytes the host writes)
+struct FrameUniformsDev {   // runtime/wgsl/common.wgsl:39-47
+  uint32_t frame, gaussian_count, part_count, lamp_count;
+  float theta, alpha_min, alpha_part;
+  uint32_t slot_count;
+  float eye[3];
+  uint32_t stratum_step, light_samples_axis, material_count, history, iso;
+};
+struct EmitParamsDev {      // runtime/wgsl/emissive.wgsl:34-53
+  uint32_t count, root;
+  float scale, theta;
+  uint32_t mode, node_count, skip_part, one_sided;
+  float normal[3];
+  uint32_t n5;
+  float edge_soft;
+  uint32_t hier, stride, lights;
+};
+struct LSParamsDev {        // runtime/wgsl/light_shadow.wgsl:26-38
+  float center[3]; uint32_t res;
+  float normal[3]; uint32_t levels;
+  float t1[3]; uint32_t layer_texels;
+  float t2[3]; uint32_t dirty;
+  float sigma_l; uint32_t skip_part, hemi, part_count;
+  float cut_theta; uint32_t tiles, tile_cap; float refine;
+  float lc_diag[3]; uint32_t max_lobes;
+  float lc_off[3]; uint32_t budget;
+  uint32_t slices; float d0, dlog; uint32_t combine;
+  uint32_t level_off[16];
+  float layers[32];
+};
+struct SunParamsDev {       // runtime/wgsl/sun_map.wgsl:24-32
+  uint32_t n_layers, res, levels, layer_texels, n_lights, n_movable, dirty, range_count;
+  float margin, shutter, cut_k;
+  uint32_t node_count;
+  float lights[4][4];
+  uint32_t part_layers[64];
+  uint32_t level_off[16];
+};
+struct EnvDev {             // runtime/wgsl/environment.wgsl:21-30 (416 B buffer)
+  float sh[16][4];
+  float lights[4][8];
+  uint32_t info[4];
+  uint32_t pad[4];
+};
+struct SunLayerDesc { uint32_t part, light, kind, dirty; float c[2], half, pad; };   // sun_map.wgsl:36
+struct SunLayer { float c[2]; float half, zref; float t1[3]; uint32_t part; float t2[3]; uint32_t light; };
+static_assert(sizeof(FrameUniformsDev) == 64, "FrameUniforms");
+static_assert(sizeof(EmitParamsDev) == 64, "EmitParams");
+static_assert(sizeof(LSParamsDev) == 336, "LSParams");
+static_assert(sizeof(SunParamsDev) == 432, "SunParams");
+static_assert(sizeof(EnvDev) == 416, "Env");
+static_assert(sizeof(SunLayerDesc) == 32, "LayerDesc");
+static_assert(sizeof(SunLayer) == 48, "SunLayer");
+
+using namespace lt;
+
+// ---- iso_init (runtime/wgsl/iso_init.wgsl:29-56) ---------------------------------------------------------------
+
+extern "C" __global__ void __launch_bounds__(64)
+sl_iso_init(uint32_t n, const float4 *__restrict__ g2, const float4 *__restrict__ nodes,
+            const uint8_t *__restrict__ vgrid, float *__restrict__ iso_norm) {
+  const uint32_t g = blockIdx.x * 64u + threadIdx.x;
+  if (g >= n) return;
+  const float4 s = g2[g];
+  const NodeL nd = load_node(nodes, g);
+  const float s3 = s.x * s.y * s.z;
+  const uint8_t *vg = vgrid + 128 * (size_t)g;
+  float S = 0.0f, S0 = 0.0f;
+  for (uint32_t c = 0; c < 128u; c++) {
+    const uint32_t i = c / 16u, j = c % 16u;
+    const float el = 0.5f * PI_F - ((float)i + 0.5f) * PI_F / 8.0f;
+    const float az = -PI_F + ((float)j + 0.5f) * 2.0f * PI_F / 16.0f;
+    const v3 w = V3(cosf(el) * cosf(az), cosf(el) * sinf(az), sinf(el));
+    const float top = 0.5f * PI_F - (float)i * PI_F / 8.0f;
+    const float om = (2.0f * PI_F / 16.0f) * (sinf(top) - sinf(top - PI_F / 8.0f));
+    const float v = (float)vg[c] * (1.0f / 255.0f);
+    const float a = footprint(nd, s3 * s3, w) * om;
+    S += a * v;
+    S0 += a;
+  }
+  const float m = S > 0.02f * S0 ? S : 0.02f * S0;
+  iso_norm[g] = 1.0f / (m > 1e-30f ? m : 1e-30f);
+}
+
+// ---- shade_emit (runtime/wgsl/emissive.wgsl:123-150, 853-980), M39 matte ----------------------------------------
+
+extern "C" __global__ void __launch_bounds__(64)
+sl_shade_emit(FrameUniformsDev U, EmitParamsDev EP, const float4 *__restrict__ emit_nodes, uint32_t n_nodes,
+              const float4 *__restrict__ g0, const float4 *__restrict__ g1, const float4 *__restrict__ g2,
+              const float4 *__restrict__ g3, const float4 *__restrict__ nodes, const uint8_t *__restrict__ vgrid,
+              const float4 *__restrict__ parts, const float *__restrict__ lamp_T, const uint32_t *__restrict__ materials,
+              const float *__restrict__ iso_norm, float4 *__restrict__ colour) {
+  const uint32_t g = blockIdx.x * 64u + threadIdx.x;
+  if (g >= U.gaussian_count) return;
+  const float4 A = g0[g], Q = g1[g], S = g2[g], C = g3[g];
+  const uint32_t mi = __float_as_uint(C.w) & 0xFFFFu;
+  const uint32_t mmax = max(U.material_count, 1u) - 1u;
+  const uint32_t *mat = materials + 8 * (size_t)min(mi, mmax);
+  const uint32_t mflags = U.material_count ? mat[5] : 0u;
+  if (EP.count == 0u && (mflags & 1u) == 0u) {
+    colour[g] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    return;
+  }
+  const Xf xf = load_xf(parts, __float_as_uint(S.w) & 0xFFFFu);
+  const v3 P = xf_point(xf, V3(A.x, A.y, A.z));
+  const m3 Rq = quat_matrix(Q);
+  const float s3 = S.x * S.y * S.z;
+  const float detC = s3 * s3;
+  const v3 wo = norm3(sub3(V3(U.eye[0], U.eye[1], U.eye[2]), P));
+  const NodeL nd = load_node(nodes, g);
+  const uint8_t *vg = vgrid + 128 * (size_t)g;
+  const v3 en = V3(EP.normal[0], EP.normal[1], EP.normal[2]);
+  v3 acc = V3(0.0f, 0.0f, 0.0f);
+  uint32_t stack[48];
+  stack[0] = EP.root;
+  uint32_t sp = 1;
+  while (sp > 0) {
+    sp -= 1;
+    const uint32_t ni = stack[sp];
+    if (ni >= n_nodes) continue;                      // a malformed tree: the node does not exist
+    const float4 *e = emit_nodes + 6 * (size_t)ni;
+    const float4 e0 = e[0], e1 = e[1], e2 = e[2], e3 = e[3];
+    const uint32_t clo = __float_as_uint(e2.w);
+    bool use;
+    if (clo == NO_CHILD || EP.mode == 0u) {
+      use = clo == NO_CHILD;
+    } else {
+      const v3 d = sub3(P, V3(e0.x, e0.y, e0.z));
+      use = dot3(d, d) > EP.theta * EP.theta * e0.w * e0.w;
+    }
+    if (!use) {
+      if (sp + 2u <= 48u) { stack[sp] = clo; stack[sp + 1] = __float_as_uint(e3.w); sp += 2; }
+      continue;
+    }
+    const v3 d = sub3(V3(e0.x, e0.y, e0.z), P);
+    const float d2 = dot3(d, d);
+    if (d2 < 1e-8f) continue;
+    const float sd = sqrtf(d2);
+    const v3 ww = V3(d.x / sd, d.y / sd, d.z / sd);
+    const v3 wi = xf_dir_inv(xf, ww);
+    const float vi = vgrid_lookup(vg, wi);
+    if (vi <= 1e-7f) continue;
+    const float fp = footprint(nd, detC, wi);
+    // intensity(e, -ww) (emissive.wgsl:123-130)
+    const v3 w = V3(-ww.x, -ww.y, -ww.z);
+    float s = 1.0f;
+    if (EP.one_sided != 0u || (__float_as_uint(e1.w) & 2u) != 0u) {
+      const float c = dot3(w, en);
+      if (EP.edge_soft > 0.0f) {
+        const float t = clampf_((c - -EP.edge_soft) / (EP.edge_soft - -EP.edge_soft), 0.0f, 1.0f);
+        s = t * t * (3.0f - 2.0f * t);
+      } else {
+        s = c > 0.0f ? 1.0f : 0.0f;
+      }
+    }
+    const float q = dot3(w, sym_mul(V3(e1.x, e1.y, e1.z), V3(e2.x, e2.y, e2.z), w));
+    const float ik = s * EP.scale * sqrtf(q > 0.0f ? q : 0.0f);
+    const float k = fp / d2 * 1.0f;
+    acc = add3(acc, V3(e3.x * ik * vi * k, e3.y * ik * vi * k, e3.z * ik * vi * k));
+  }
+  const float T = lamp_T ? lamp_T[g] : 1.0f;
+  const float tk = T * iso_norm[g];
+  v3 c = V3(C.x * acc.x * tk, C.y * acc.y * tk, C.z * acc.z * tk);
+  if ((mflags & 1u) != 0u && ((mflags & 4u) == 0u || dot3(xf_dir_inv(xf, wo), Rq.c0) > 0.0f)) {
+    c = add3(c, V3(unpack_half(mat[6] & 0xFFFFu), unpack_half(mat[6] >> 16), unpack_half(mat[7] & 0xFFFFu)));
+  }
+  colour[g] = make_float4(c.x, c.y, c.z, 1.0f);
+}
+
+// ---- opacity maps, shared: resolve, mips (light_shadow.wgsl:290-321, sun_map.wgsl:228-261) ----------------------
+
+// layer l's texel i -> (alpha, alpha mu, alpha m2, 0); layers with dirty_mask bit 0 (bit l) skipped
+extern "C" __global__ void __launch_bounds__(256)
+sl_opacity_resolve(const uint32_t *__restrict__ acc, float4 *__restrict__ mip, uint32_t n_layers, uint32_t texels,
+                   uint32_t layer_texels, float fix, const uint32_t *__restrict__ dirty) {
+  const size_t i = (size_t)blockIdx.x * 256u + threadIdx.x;
+  if (i >= (size_t)texels * n_layers) return;
+  const uint32_t l = (uint32_t)(i / texels);
+  if (!dirty[l]) return;
+  const float s0 = (float)acc[i * 3] / fix;
+  float4 o = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (s0 > 1e-5f) {
+    // 1 - e^-s0 rounded once from double: the f32 form cancels for small s0 (a 1-ulp exp difference becomes
+    // 1e-3 relative at s0 = 1.5e-5, and the moments divide by s0)
+    const float a = (float)(-expm1(-(double)s0));
+    o = make_float4(a, a * (float)acc[i * 3 + 1] / (fix * s0), a * (float)acc[i * 3 + 2] / (fix * s0), 0.0f);
+  }
+  mip[(size_t)l * layer_texels + (i - (size_t)l * texels)] = o;
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+sl_opacity_down(float4 *__restrict__ mip, uint32_t n_layers, uint32_t rk, uint32_t off_src, uint32_t off_dst,
+                uint32_t layer_texels, const uint32_t *__restrict__ dirty) {
+  const size_t i = (size_t)blockIdx.x * 256u + threadIdx.x;
+  const size_t n = (size_t)rk * rk;
+  if (i >= n * n_layers) return;
+  const uint32_t l = (uint32_t)(i / n);
+  if (!dirty[l]) return;
+  const uint32_t j = (uint32_t)(i - (size_t)l * n);
+  const uint32_t y = j / rk, x = j - y * rk, pr = rk * 2u;
+  const size_t src = (size_t)l * layer_texels + off_src + (size_t)(2u * y) * pr + 2u * x;
+  const float4 a = mip[src], b = mip[src + 1], c = mip[src + pr], d = mip[src + pr + 1];
+  mip[(size_t)l * layer_texels + off_dst + j] =
+      make_float4(0.25f * (((a.x + b.x) + c.x) + d.x), 0.25f * (((a.y + b.y) + c.y) + d.y),
+                  0.25f * (((a.z + b.z) + c.z) + d.z), 0.25f * (((a.w + b.w) + c.w) + d.w));
+}
+
+// ---- opacity-map splats as tiled sums -------------------------------------------------------------------------
+//
+// The reference splats each Gaussian with one thread looping over its texel box and three u32 atomics per texel
+// (light_shadow.wgsl:191-286, sun_map.wgsl:127-219). Here a prep pass turns each (Gaussian, layer) into a 64 B
+// record, the records are binned by 16x16 map tile (count, scan, fill), and one workgroup per tile sums its
+// records' fixed-point moments in registers, one lane per texel, and writes the tile once. The per-texel
+// arithmetic is the reference's, unchanged; u32 sums are order-independent, so the maps are bit-identical to
+// the atomic form whatever order the records arrive in.
+
+constexpr int MT = 16;   // map tile side, texels
+
+// One Gaussian's splat onto one map layer: everything the per-texel sum needs.
+struct MapRec {
+  float ctx, cty, s00, s01;   // centre in texels; inverse footprint, column 0
+  float s10, s11, tau0, zc;   // column 1; peak optical depth; depth at the centre
+  float gzx, gzy, sz2;        // depth change per texel; depth variance along the light
+  uint32_t layer;
+  int32_t x0, x1, y0, y1;     // texel box clamped to the map; x0 > x1 when the record adds nothing
+};
+static_assert(sizeof(MapRec) == 64, "MapRec");
+
+__device__ __forceinline__ MapRec empty_rec() {
+  MapRec r = {};
+  r.x0 = 1; r.x1 = 0; r.y0 = 1; r.y1 = 0;
+  return r;
+}
+
+__device__ __forceinline__ void count_tiles(const MapRec &r, uint32_t tiles_x, uint32_t tpl,
+                                            uint32_t *__restrict__ tile_count) {
+  if (r.x0 > r.x1 || r.y0 > r.y1) return;
+  const uint32_t base = r.layer * tpl;
+  for (int ty = r.y0 / MT; ty <= r.y1 / MT; ty++)
+    for (int tx = r.x0 / MT; tx <= r.x1 / MT; tx++) atomicAdd(tile_count + base + (uint32_t)ty * tiles_x + tx, 1u);
+}
+
+// tile_start[i] = sum of tile_count[0, i), over n tiles in blocks of 4096; block_sums[b] = the sum of block b.
+extern "C" __global__ void __launch_bounds__(256)
+sl_map_reduce(const uint32_t *__restrict__ tile_count, uint32_t n, uint32_t *__restrict__ block_sums) {
+  __shared__ uint32_t s_tmp[RS_WAVES];
+  const uint32_t b0 = blockIdx.x * 4096u + threadIdx.x * 16u;
+  uint32_t v = 0;
+  for (uint32_t k = 0; k < 16u; k++) v += b0 + k < n ? tile_count[b0 + k] : 0u;
+  uint32_t total;
+  block_excl_scan256(v, s_tmp, &total);
+  if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+sl_map_scan(const uint32_t *__restrict__ tile_count, uint32_t n, const uint32_t *__restrict__ block_sums,
+            uint32_t *__restrict__ tile_start) {
+  __shared__ uint32_t s_tmp[RS_WAVES];
+  uint32_t before = 0;
+  for (uint32_t b = threadIdx.x; b < blockIdx.x; b += 256u) before += block_sums[b];
+  uint32_t base;
+  block_excl_scan256(before, s_tmp, &base);   // base = sum over the block of 'before' = all earlier blocks
+  const uint32_t b0 = blockIdx.x * 4096u + threadIdx.x * 16u;
+  uint32_t c[16], v = 0;
+  for (uint32_t k = 0; k < 16u; k++) {
+    c[k] = b0 + k < n ? tile_count[b0 + k] : 0u;
+    v += c[k];
+  }
+  uint32_t total;
+  uint32_t run = base + block_excl_scan256(v, s_tmp, &total);
+  for (uint32_t k = 0; k < 16u; k++) {
+    if (b0 + k < n) tile_start[b0 + k] = run;
+    run += c[k];
+  }
+  if (blockIdx.x == gridDim.x - 1u && threadIdx.x == 0) tile_start[n] = base + total;
+}
+
+// Each record's slot into the lists of the tiles its box touches; tile_count returns to zero.
+extern "C" __global__ void __launch_bounds__(256)
+sl_map_fill(uint32_t n_rec, const MapRec *__restrict__ recs, uint32_t tiles_x, uint32_t tpl,
+            const uint32_t *__restrict__ tile_start, uint32_t *__restrict__ tile_count, uint32_t *__restrict__ list) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= n_rec) return;
+  const float4 w2 = reinterpret_cast<const float4 *>(recs)[4 * (size_t)i + 2];
+  const int4 box = reinterpret_cast<const int4 *>(recs)[4 * (size_t)i + 3];
+  if (box.x > box.y || box.z > box.w) return;
+  const uint32_t base = __float_as_uint(w2.w) * tpl;
+  for (int ty = box.z / MT; ty <= box.w / MT; ty++)
+    for (int tx = box.x / MT; tx <= box.y / MT; tx++) {
+      const uint32_t t = base + (uint32_t)ty * tiles_x + tx;
+      const uint32_t pos = atomicSub(tile_count + t, 1u) - 1u;
+      list[tile_start[t] + pos] = i;
+    }
+}
+
+// One workgroup per 16x16 tile of every layer: acc[layer][texel][3] = the sums of the tile's records.
+// Layers whose dirty flag is 0 keep their sums.
+extern "C" __global__ void __launch_bounds__(256)
+sl_map_accumulate(const MapRec *__restrict__ recs, const uint32_t *__restrict__ list,
+                  const uint32_t *__restrict__ tile_start, uint32_t res, uint32_t tiles_x, uint32_t tpl, float fix,
+                  const uint32_t *__restrict__ dirty, uint32_t *__restrict__ acc) {
+  __shared__ MapRec s_rec[256];
+  const uint32_t tile = blockIdx.x, layer = tile / tpl;
+  if (!dirty[layer]) return;
+  const uint32_t rem = tile - layer * tpl, ty = rem / tiles_x, tx = rem - ty * tiles_x;
+  const int x = (int)(tx * MT + (threadIdx.x & 15u)), y = (int)(ty * MT + (threadIdx.x >> 4));
+  const int wy0 = (int)(ty * MT + (threadIdx.x >> 6) * 4u), wx0 = (int)(tx * MT);
+  const uint32_t begin = tile_start[tile], end = tile_start[tile + 1];
+  uint32_t a0 = 0, a1 = 0, a2 = 0;
+  for (uint32_t b = begin; b < end; b += 256u) {
+    const uint32_t nb = min(256u, end - b);
+    __syncthreads();
+    if (threadIdx.x < nb) s_rec[threadIdx.x] = recs[list[b + threadIdx.x]];
+    __syncthreads();
+    for (uint32_t j = 0; j < nb; j++) {
+      const MapRec r = s_rec[j];
+      if (r.x1 < wx0 || r.x0 > wx0 + 15 || r.y1 < wy0 || r.y0 > wy0 + 3) continue;   // the wave's 16x4 texels
+      if (x < r.x0 || x > r.x1 || y < r.y0 || y > r.y1) continue;
+      const v2 dt = V2(((float)x + 0.5f) - r.ctx, ((float)y + 0.5f) - r.cty);
+      const m2 Si = {V2(r.s00, r.s01), V2(r.s10, r.s11)};
+      const float q = dot2(dt, m2v(Si, dt));
+      if (q > 9.0f) continue;
+      float t = r.tau0 * exp_cr(-0.5f * q);
+      t = t < 64.0f ? t : 64.0f;
+      const float t_fix = t * fix;
+      if (t_fix < 0.5f) continue;
+      float z = r.zc + dot2(V2(r.gzx, r.gzy), dt);
+      z = z > 0.0f ? z : 0.0f;
+      a0 += u32_sat(t_fix + 0.5f);
+      a1 += u32_sat(t_fix * z + 0.5f);
+      a2 += u32_sat(t_fix * (z * z + r.sz2) + 0.5f);
+    }
+  }
+  if ((uint32_t)x < res && (uint32_t)y < res) {
+    uint32_t *k = acc + ((size_t)layer * res * res + (size_t)y * res + (size_t)x) * 3;
+    k[0] = a0;
+    k[1] = a1;
+    k[2] = a2;
+  }
+}
+
+namespace lt {
+__device__ __forceinline__ float4 mip_at(const float4 *mip, const uint32_t *level_off, uint32_t res, uint32_t k, v2 tc) {
+  const uint32_t rk = max(res >> k, 1u);
+  const float sc = (float)(1u << k);
+  const float cx = clampf_(tc.x / sc - 0.5f, 0.0f, (float)(rk - 1u)), cy = clampf_(tc.y / sc - 0.5f, 0.0f, (float)(rk - 1u));
+  const uint32_t i0x = (uint32_t)floorf(cx), i0y = (uint32_t)floorf(cy);
+  const uint32_t i1x = min(i0x + 1u, rk - 1u), i1y = min(i0y + 1u, rk - 1u);
+  const float fx = cx - (float)i0x, fy = cy - (float)i0y;
+  const float4 *b = mip + level_off[k];
+  const float4 p00 = b[i0y * rk + i0x], p01 = b[i0y * rk + i1x], p10 = b[i1y * rk + i0x], p11 = b[i1y * rk + i1x];
+  float4 o;
+  o.x = mixf(mixf(p00.x, p01.x, fx), mixf(p10.x, p11.x, fx), fy);
+  o.y = mixf(mixf(p00.y, p01.y, fx), mixf(p10.y, p11.y, fx), fy);
+  o.z = mixf(mixf(p00.z, p01.z, fx), mixf(p10.z, p11.z, fx), fy);
+  o.w = mixf(mixf(p00.w, p01.w, fx), mixf(p10.w, p11.w, fx), fy);
+  return o;
+}
+__device__ __forceinline__ float4 mip_tri(const float4 *mip, const uint32_t *level_off, uint32_t res, uint32_t levels,
+                                          float l, v2 tc) {
+  const float lc = clampf_(l, 0.0f, (float)(levels - 1u));
+  const uint32_t k0 = (uint32_t)floorf(lc);
+  const uint32_t k1 = min(k0 + 1u, levels - 1u);
+  const float4 a = mip_at(mip, level_off, res, k0, tc), b = mip_at(mip, level_off, res, k1, tc);
+  const float t = lc - (float)k0;
+  return make_float4(mixf(a.x, b.x, t), mixf(a.y, b.y, t), mixf(a.z, b.z, t), mixf(a.w, b.w, t));
+}
+
+// octahedral map coordinates in [-1, 1]^2 around the light's normal (light_shadow.wgsl:64-76)
+__device__ __forceinline__ v2 oct_uv(const LSParamsDev &LS, v3 w) {
+  const v3 l = V3(dot3(w, V3(LS.t1[0], LS.t1[1], LS.t1[2])), dot3(w, V3(LS.t2[0], LS.t2[1], LS.t2[2])),
+                  dot3(w, V3(LS.normal[0], LS.normal[1], LS.normal[2])));
+  if (LS.hemi != 0u) {
+    const float z = l.z > 0.0f ? l.z : 0.0f;
+    float den = (fabsf(l.x) + fabsf(l.y)) + z;
+    den = den > 1e-12f ? den : 1e-12f;
+    const v2 p = V2(l.x / den, l.y / den);
+    return V2(p.x + p.y, p.x - p.y);
+  }
+  float den = (fabsf(l.x) + fabsf(l.y)) + fabsf(l.z);
+  den = den > 1e-12f ? den : 1e-12f;
+  v2 p = V2(l.x / den, l.y / den);
+  if (l.z < 0.0f)
+    p = V2((1.0f - fabsf(p.y)) * (p.x >= 0.0f ? 1.0f : -1.0f), (1.0f - fabsf(p.x)) * (p.y >= 0.0f ? 1.0f : -1.0f));
+  return p;
+}
+__device__ __forceinline__ m2 oct_jac(const LSParamsDev &LS, v3 w, v3 b0, v3 b1) {
+  const float e = 1e-3f;
+  const v2 u0 = oct_uv(LS, w);
+  const v2 ua = oct_uv(LS, norm3(add3(w, mul3s(b0, e))));
+  const v2 ub = oct_uv(LS, norm3(add3(w, mul3s(b1, e))));
+  return {V2((ua.x - u0.x) / e, (ua.y - u0.y) / e), V2((ub.x - u0.x) / e, (ub.y - u0.y) / e)};
+}
+}  // namespace lt
+
+// ---- A: emitter light opacity map (light_shadow.wgsl:191-286, 347-392) -------------------------------------------
+
+// the emitter map record of Gaussian g (its own part's layer); z = r + gz . dt
+__device__ MapRec lsm_rec(uint32_t g, const LSParamsDev &LS, const float4 *__restrict__ g0,
+                          const float4 *__restrict__ g1, const float4 *__restrict__ g2,
+                          const float4 *__restrict__ nodes, const float4 *__restrict__ parts) {
+  const float4 A = g0[g], Q = g1[g], S4 = g2[g];
+  const uint32_t p = __float_as_uint(S4.w) & 0xFFFFu;
+  if (p >= LS.part_count || p == LS.skip_part || (LS.dirty & (1u << p)) == 0u) return empty_rec();
+  const NodeL nd = load_node(nodes, g);
+  const Xf xf = load_xf(parts, p);
+  const m3 R = part_rot(xf);
+  const v3 m = xf_point(xf, V3(A.x, A.y, A.z));
+  const v3 center = V3(LS.center[0], LS.center[1], LS.center[2]), nrm = V3(LS.normal[0], LS.normal[1], LS.normal[2]);
+  const v3 dv = sub3(m, center);
+  const float r = len3(dv);
+  if (r < 1e-4f) return empty_rec();
+  const v3 w = V3(dv.x / r, dv.y / r, dv.z / r);
+  const m3 C = leaf_cov(Q, S4, R);
+  if (LS.hemi != 0u && dot3(dv, nrm) < -3.0f * sqrtf(dot3(nrm, m3v(C, nrm)))) return empty_rec();
+  const v3 b0 = basis_k(w), b1 = cross3(w, b0);
+  const v3 c0 = m3v(C, b0), c1 = m3v(C, b1), cw = m3v(C, w);
+  const m2 Cbb = {V2(dot3(b0, c0), dot3(b0, c1)), V2(dot3(b0, c1), dot3(b1, c1))};
+  const v2 cbz = V2(dot3(b0, cw), dot3(b1, cw));
+  const float czz = dot3(w, cw);
+  const m2 Cbb_i = inv2(Cbb);
+  float sz2 = czz - dot2(cbz, m2v(Cbb_i, cbz));
+  sz2 = sz2 > 1e-12f ? sz2 : 1e-12f;
+  float tau0 = nd.kappa * dirm(nd, xf_dir_inv(xf, w)) * sqrtf(2.0f * PI_F * sz2);
+  const float half_res = 0.5f * (float)LS.res;
+  const m2 J = m2s(oct_jac(LS, w, b0, b1), half_res);
+  const m2 Sang = m2s(Cbb, 1.0f / (r * r));
+  m2 S = m2mul(m2mul(J, Sang), m2t(J));
+  const float det0 = m2det(S);
+  S.c0.x += 0.3f;
+  S.c1.y += 0.3f;
+  const float det1 = m2det(S);
+  tau0 *= sqrtf((det0 > 0.0f ? det0 : 0.0f) / det1);
+  const m2 Si = inv2(S);
+  const v2 cz = m2v(Cbb_i, cbz);
+  const v2 gz = m2v(m2t(inv2(J)), V2(cz.x * r, cz.y * r));
+  const v2 uv = oct_uv(LS, w);
+  const v2 ct = V2((uv.x * 0.5f + 0.5f) * (float)LS.res, (uv.y * 0.5f + 0.5f) * (float)LS.res);
+  const int hx = min((int)ceilf(3.0f * sqrtf(S.c0.x)), 48), hy = min((int)ceilf(3.0f * sqrtf(S.c1.y)), 48);
+  const int cx = (int)floorf(ct.x), cy = (int)floorf(ct.y);
+  if (tau0 * 65536.0f < 0.5f) return empty_rec();   // no texel reaches half a fixed-point unit
+  const int res = (int)LS.res;
+  MapRec o;
+  o.ctx = ct.x; o.cty = ct.y;
+  o.s00 = Si.c0.x; o.s01 = Si.c0.y; o.s10 = Si.c1.x; o.s11 = Si.c1.y;
+  o.tau0 = tau0; o.zc = r; o.gzx = gz.x; o.gzy = gz.y; o.sz2 = sz2;
+  o.layer = p;
+  o.x0 = max(cx - hx, 0); o.x1 = min(cx + hx, res - 1);
+  o.y0 = max(cy - hy, 0); o.y1 = min(cy + hy, res - 1);
+  return o;
+}
+
+extern "C" __global__ void __launch_bounds__(64)
+sl_lsm_prep(uint32_t n, LSParamsDev LS, const float4 *__restrict__ g0, const float4 *__restrict__ g1,
+            const float4 *__restrict__ g2, const float4 *__restrict__ nodes, const float4 *__restrict__ parts,
+            MapRec *__restrict__ recs, uint32_t *__restrict__ tile_count) {
+  const uint32_t g = blockIdx.x * 64u + threadIdx.x;
+  if (g >= n) return;
+  const MapRec r = lsm_rec(g, LS, g0, g1, g2, nodes, parts);
+  recs[g] = r;
+  const uint32_t tiles_x = (LS.res + MT - 1) / MT;
+  count_tiles(r, tiles_x, tiles_x * tiles_x, tile_count);
+}
+
+extern "C" __global__ void __launch_bounds__(64)
+sl_lsm_lookup(uint32_t n, LSParamsDev LS, const float4 *__restrict__ g0, const float4 *__restrict__ g2,
+              const float4 *__restrict__ parts, const float4 *__restrict__ mip, float *__restrict__ T_out) {
+  const uint32_t g = blockIdx.x * 64u + threadIdx.x;
+  if (g >= n) return;
+  const float4 A = g0[g], S4 = g2[g];
+  const uint32_t own = __float_as_uint(S4.w) & 0xFFFFu;
+  const v3 P = xf_point(load_xf(parts, own), V3(A.x, A.y, A.z));
+  const v3 center = V3(LS.center[0], LS.center[1], LS.center[2]);
+  const v3 dv = sub3(P, center);
+  const float z = len3(dv);
+  if (z < 1e-4f) { T_out[g] = 1.0f; return; }
+  const v3 w = V3(dv.x / z, dv.y / z, dv.z / z);
+  const v3 b0 = basis_k(w), b1 = cross3(w, b0);
+  const m2 J = oct_jac(LS, w, b0, b1);
+  const float dj = fabsf(m2det(J));
+  const float tex_ang = (2.0f / (float)LS.res) / sqrtf(dj > 1e-12f ? dj : 1e-12f);
+  const v2 uv = oct_uv(LS, w);
+  const v2 tc = V2((uv.x * 0.5f + 0.5f) * (float)LS.res, (uv.y * 0.5f + 0.5f) * (float)LS.res);
+  float T = 1.0f;
+  for (uint32_t p = 0; p < LS.part_count; p++) {
+    if (p == own || p == LS.skip_part) continue;
+    const float4 bs = parts[p * 8 + 6];
+    const v3 bc = sub3(xf_point(load_xf(parts, p), V3(bs.x, bs.y, bs.z)), center);
+    const float dc = len3(bc);
+    const float zn = dc - bs.w > 0.02f ? dc - bs.w : 0.02f;
+    if (z <= zn) continue;
+    const float search = 2.0f * LS.sigma_l * (z - zn) / (zn * z);
+    if (dc > bs.w) {
+      const float ang = acos_cr(clampf_(dot3(V3(bc.x / dc, bc.y / dc, bc.z / dc), w), -1.0f, 1.0f));
+      if (ang > asin_cr(bs.w / dc) + search + 2.0f * tex_ang) continue;
+    }
+    float x = 2.0f * search / tex_ang;
+    const float ls = log2_cr(x > 1.0f ? x : 1.0f);
+    const float4 *mp = mip + (size_t)p * LS.layer_texels;
+    const float4 a0 = mip_tri(mp, LS.level_off, LS.res, LS.levels, ceilf(ls), tc);
+    if (a0.x < 1e-4f) continue;
+    float zb = a0.y / a0.x;
+    zb = zb > 0.02f ? zb : 0.02f;
+    const float pen = LS.sigma_l * (z - zb > 0.0f ? z - zb : 0.0f) / (zb * z);
+    x = 2.0f * pen / tex_ang;
+    const float4 s = mip_tri(mp, LS.level_off, LS.res, LS.levels, log2_cr(x > 1.0f ? x : 1.0f), tc);
+    if (s.x < 1e-5f) continue;
+    const float mu = s.y / s.x;
+    const float var = s.z / s.x - mu * mu;
+    const float sd = sqrtf(var > 1e-8f ? var : 1e-8f);
+    T *= 1.0f - s.x * phi_((z - mu) / sd);
+  }
+  T_out[g] = T;
+}
+
+// ---- sun maps (sun_map.wgsl:91-219, 263-324) ----------------------------------------------------------------------
+
+extern "C" __global__ void __launch_bounds__(64)
+sl_sun_frame(SunParamsDev SM, const SunLayerDesc *__restrict__ desc, const float4 *__restrict__ parts,
+             SunLayer *__restrict__ layers) {
+  const uint32_t l = blockIdx.x * 64u + threadIdx.x;
+  if (l >= SM.n_layers) return;
+  const SunLayerDesc d = desc[l];
+  const v3 L = V3(SM.lights[d.light][0], SM.lights[d.light][1], SM.lights[d.light][2]);
+  const v3 t1 = basis_k(L), t2 = cross3(L, t1);
+  const float4 b = parts[d.part * 8 + 6];
+  const v3 c = xf_point(load_xf(parts, d.part), V3(b.x, b.y, b.z));
+  SunLayer o;
+  o.c[0] = dot3(c, t1);
+  o.c[1] = dot3(c, t2);
+  o.half = b.w + SM.margin;
+  if (d.kind == 1u) { o.c[0] = d.c[0]; o.c[1] = d.c[1]; o.half = d.half; }
+  o.zref = dot3(c, L) + b.w;
+  o.t1[0] = t1.x; o.t1[1] = t1.y; o.t1[2] = t1.z; o.part = d.part;
+  o.t2[0] = t2.x; o.t2[1] = t2.y; o.t2[2] = t2.z; o.light = d.light;
+  layers[l] = o;
+}
+
+// Gaussian g's records on the layers of its part: slot g * max_layers + j for the j-th layer, empty when the layer
+// is clean, its light is off or nothing reaches half a fixed-point unit; z = z0 - gz . dt, stored as z0 + (-gz) . dt
+// (the same float result: negation is exact).
+extern "C" __global__ void __launch_bounds__(64)
+sl_sun_prep(uint32_t n, SunParamsDev SM, const SunLayerDesc *__restrict__ desc, const SunLayer *__restrict__ layers,
+            const float4 *__restrict__ g0, const float4 *__restrict__ g1, const float4 *__restrict__ g2,
+            const float4 *__restrict__ nodes, const float4 *__restrict__ parts, uint32_t max_layers,
+            MapRec *__restrict__ recs, uint32_t *__restrict__ tile_count) {
+  const uint32_t g = blockIdx.x * 64u + threadIdx.x;
+  if (g >= n) return;
+  MapRec *out = recs + (size_t)g * max_layers;
+  const float4 A = g0[g], Q = g1[g], S4 = g2[g];
+  const uint32_t p = __float_as_uint(S4.w) & 0xFFFFu;
+  const uint32_t first = p < 32u ? SM.part_layers[2 * p] : 0u, cnt = p < 32u ? min(SM.part_layers[2 * p + 1], max_layers) : 0u;
+  for (uint32_t j = cnt; j < max_layers; j++) out[j] = empty_rec();
+  if (cnt == 0u) return;
+  const Xf xf = load_xf(parts, p);
+  const m3 R = part_rot(xf);
+  const NodeL nd = load_node(nodes, g);
+  const v3 x = xf_point(xf, V3(A.x, A.y, A.z));
+  const m3 C = leaf_cov(Q, S4, R);
+  const int res = (int)SM.res;
+  const uint32_t tiles_x = (SM.res + MT - 1) / MT;
+  for (uint32_t j = 0; j < cnt; j++) {
+    const uint32_t l = first + j;
+    const SunLayerDesc d = desc[l];
+    if (d.dirty == 0u || SM.lights[d.light][3] < 0.0f) {
+      out[j] = empty_rec();
+      continue;
+    }
+    const SunLayer ly = layers[l];
+    const v3 w = V3(SM.lights[d.light][0], SM.lights[d.light][1], SM.lights[d.light][2]);
+    const v3 b0 = V3(ly.t1[0], ly.t1[1], ly.t1[2]), b1 = V3(ly.t2[0], ly.t2[1], ly.t2[2]);
+    const v3 c0 = m3v(C, b0), c1 = m3v(C, b1), cw = m3v(C, w);
+    const m2 Cbb = {V2(dot3(b0, c0), dot3(b1, c0)), V2(dot3(b0, c1), dot3(b1, c1))};
+    const v2 cbz = V2(dot3(b0, cw), dot3(b1, cw));
+    const m2 Cbb_i = inv2(Cbb);
+    float sz2 = dot3(w, cw) - dot2(cbz, m2v(Cbb_i, cbz));
+    sz2 = sz2 > 1e-12f ? sz2 : 1e-12f;
+    const float tex = (float)SM.res / (2.0f * ly.half);
+    m2 S = m2s(Cbb, tex * tex);
+    const float det0 = m2det(S);
+    S.c0.x += 0.3f;
+    S.c1.y += 0.3f;
+    const float det1 = m2det(S);
+    const float tau0 = nd.kappa * dirm(nd, xf_dir_inv(xf, w)) * sqrtf(2.0f * PI_F * sz2) *
+                       sqrtf((det0 > 0.0f ? det0 : 0.0f) / det1);
+    const m2 Si = inv2(S);
+    const float z0 = ly.zref - dot3(x, w);
+    const v2 cz = m2v(Cbb_i, cbz);
+    const v2 gz = V2(cz.x / tex, cz.y / tex);
+    const v2 ct = V2(((dot3(x, b0) - ly.c[0]) + ly.half) * tex, ((dot3(x, b1) - ly.c[1]) + ly.half) * tex);
+    const int hx = min((int)ceilf(3.0f * sqrtf(S.c0.x)), 32), hy = min((int)ceilf(3.0f * sqrtf(S.c1.y)), 32);
+    const int cx = (int)floorf(ct.x), cy = (int)floorf(ct.y);
+    if (tau0 * 4096.0f < 0.5f) {   // no texel reaches half a fixed-point unit
+      out[j] = empty_rec();
+      continue;
+    }
+    MapRec o;
+    o.ctx = ct.x; o.cty = ct.y;
+    o.s00 = Si.c0.x; o.s01 = Si.c0.y; o.s10 = Si.c1.x; o.s11 = Si.c1.y;
+    o.tau0 = tau0; o.zc = z0; o.gzx = -gz.x; o.gzy = -gz.y; o.sz2 = sz2;
+    o.layer = l;
+    o.x0 = max(cx - hx, 0); o.x1 = min(cx + hx, res - 1);
+    o.y0 = max(cy - hy, 0); o.y1 = min(cy + hy, res - 1);
+    out[j] = o;
+    count_tiles(o, tiles_x, tiles_x * tiles_x, tile_count);
+  }
+}
+
+extern "C" __global__ void __launch_bounds__(64)
+sl_sun_lookup(uint32_t n, SunParamsDev SM, const SunLayerDesc *__restrict__ desc, const SunLayer *__restrict__ layers,
+              const float4 *__restrict__ g0, const float4 *__restrict__ g2, const float4 *__restrict__ parts,
+              const float4 *__restrict__ mip, float4 *__restrict__ env_T) {
+  const uint32_t g = blockIdx.x * 64u + threadIdx.x;
+  if (g >= n) return;
+  const float4 A = g0[g], S4 = g2[g];
+  const uint32_t own = __float_as_uint(S4.w) & 0xFFFFu;
+  const v3 P = xf_point(load_xf(parts, own), V3(A.x, A.y, A.z));
+  float T[4] = {1.0f, 1.0f, 1.0f, 1.0f};
+  uint32_t used = 0xFFFFFFFFu;
+  for (uint32_t l = 0; l < SM.n_layers; l++) {
+    const SunLayer ly = layers[l];
+    const uint32_t key = ly.part * 8u + ly.light;
+    if (ly.part == own || key == used || SM.lights[ly.light][3] < 0.0f) continue;
+    const v3 w = V3(SM.lights[ly.light][0], SM.lights[ly.light][1], SM.lights[ly.light][2]);
+    const float ux = dot3(P, V3(ly.t1[0], ly.t1[1], ly.t1[2])) - ly.c[0];
+    const float uy = dot3(P, V3(ly.t2[0], ly.t2[1], ly.t2[2])) - ly.c[1];
+    const float inner = (desc[l].kind == 1u ? 0.9f : 1.0f) * ly.half;
+    if (fabsf(ux) >= inner || fabsf(uy) >= inner) continue;
+    used = key;
+    const float z = ly.zref - dot3(P, w);
+    if (z <= 0.0f) continue;
+    const float texel = 2.0f * ly.half / (float)SM.res;
+    const v2 tc = V2((ux + ly.half) / texel, (uy + ly.half) / texel);
+    const float sig = SM.lights[ly.light][3];
+    const float search = 2.0f * sig * z;
+    float xv = 2.0f * search / texel;
+    const float4 *mp = mip + (size_t)l * SM.layer_texels;
+    const float4 a0 = mip_tri(mp, SM.level_off, SM.res, SM.levels, log2_cr(xv > 1.0f ? xv : 1.0f), tc);
+    if (a0.x < 1e-4f) continue;
+    const float zb = a0.y / a0.x;
+    const float pen = sig * (z - zb > 0.0f ? z - zb : 0.0f);
+    xv = 2.0f * pen / texel;
+    const float4 s = mip_tri(mp, SM.level_off, SM.res, SM.levels, log2_cr(xv > 1.0f ? xv : 1.0f), tc);
+    if (s.x < 1e-5f) continue;
+    const float mu = s.y / s.x;
+    const float var = s.z / s.x - mu * mu;
+    const float sd = sqrtf(var > 1e-8f ? var : 1e-8f);
+    T[ly.light] *= 1.0f - s.x * phi_((z - mu) / sd);
+  }
+  env_T[g] = make_float4(T[0], T[1], T[2], T[3]);
+}
+
+// ---- env_direct (environment.wgsl:207-229, 266-277), matte -------------------------------------------------------
+
+extern "C" __global__ void __launch_bounds__(64)
+sl_env_direct(FrameUniformsDev U, EnvDev ENV, const float4 *__restrict__ g2, const float4 *__restrict__ g3,
+              const float4 *__restrict__ nodes, const uint8_t *__restrict__ vgrid, const float4 *__restrict__ parts,
+              const uint32_t *__restrict__ materials, const float *__restrict__ iso_norm,
+              const float4 *__restrict__ env_T, float4 *__restrict__ colour) {
+  const uint32_t g = blockIdx.x * 64u + threadIdx.x;
+  if (g >= U.gaussian_count || ENV.info[0] == 0u || (U.iso & 1u) == 0u) return;
+  const float4 S4 = g2[g], C = g3[g];
+  const uint32_t mi = __float_as_uint(C.w) & 0xFFFFu;
+  const uint32_t mmax = max(U.material_count, 1u) - 1u;
+  if (U.material_count && (materials[8 * (size_t)min(mi, mmax) + 5] & 1u)) return;
+  const NodeL nd = load_node(nodes, g);
+  const Xf xf = load_xf(parts, __float_as_uint(S4.w) & 0xFFFFu);
+  const float s3 = S4.x * S4.y * S4.z;
+  const float4 Tg = env_T ? env_T[g] : make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+  const float Tk[4] = {Tg.x, Tg.y, Tg.z, Tg.w};
+  v3 acc = V3(0.0f, 0.0f, 0.0f);
+  const uint32_t nl = min(ENV.info[0], 4u);
+  for (uint32_t k = 0; k < nl; k++) {
+    const v3 wi = xf_dir_inv(xf, V3(ENV.lights[k][0], ENV.lights[k][1], ENV.lights[k][2]));
+    const float v = vgrid_lookup(vgrid + 128 * (size_t)g, wi);
+    if (v <= 1e-7f) continue;
+    const float f = Tk[k] * v * footprint(nd, s3 * s3, wi);
+    acc = add3(acc, mul3s(V3(ENV.lights[k][4], ENV.lights[k][5], ENV.lights[k][6]), f));
+  }
+  const float inorm = iso_norm[g];
+  const float4 c = colour[g];
+  colour[g] = make_float4(c.x + C.x * acc.x * inorm, c.y + C.y * acc.y * inorm, c.z + C.z * acc.z * inorm, c.w);
+}

@@ -23,6 +23,8 @@
 #include "composite.hip"
 #include "sort.hip"
 #include "tiles.hip"
+#include "lighting.hip"
+#pragma clang fp contract(fast)
 
 namespace {
 
@@ -51,6 +53,12 @@ size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 }  // namespace
 
+// A device buffer that grows on demand and is owned by a scene.
+struct Grow {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+
 struct sl_scene {
   uint32_t count = 0, part_count = 0, n_emit = 0, material_count = 0, storage = SL_STORAGE_F32;
   float4 *g[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -59,8 +67,22 @@ struct sl_scene {
   float4 *colour = nullptr;
   float4 *lo_b = nullptr;
   std::vector<uint8_t> parts_host;
+  std::vector<uint8_t> materials_host;
   std::vector<void *> allocs;
   size_t bytes = 0;
+  // relighting state
+  float4 *nodes = nullptr;          // leaf Node records (AoS, 96 B)
+  uint8_t *vgrid = nullptr;         // 128 B per Gaussian
+  uint32_t *materials = nullptr;    // 32 B records
+  float *iso_norm = nullptr;
+  bool iso_done = false;
+  float *lamp_T = nullptr;
+  float4 *env_T = nullptr;
+  Grow emit_nodes, lsm_acc, lsm_mip, lsm_dirty, sun_acc, sun_mip, sun_desc, sun_layers, sun_dirty;
+  // opacity-map splat binning, shared by both maps: records, per-tile counts (zero between uses), starts, lists
+  Grow map_rec, map_cnt, map_start, map_bsum, map_list;
+  uint32_t lsm_parts = 0, lsm_res = 0, lsm_layer_texels = 0;
+  uint32_t sun_n = 0, sun_res = 0, sun_layer_texels = 0;
 };
 
 struct Workspace {
@@ -92,7 +114,9 @@ struct sl_context {
   Workspace ws;
   FrameCounters *counters_host = nullptr;   // pinned
   hipEvent_t ev[kPasses + 1] = {};
+  hipEvent_t lev[5] = {};         // lighting: shadow, shade, sun, env
   bool timers_valid = false;
+  bool light_timers_valid = false;
   float ms[kPasses] = {};
   // the last frame, for sl_frame_finish (re-run on overflow) and sl_frame_read
   bool has_last = false;
@@ -388,6 +412,8 @@ extern "C" sl_status sl_ctx_create(int device_ordinal, sl_context **out) {
   memset(ctx->counters_host, 0, sizeof(FrameCounters));
   for (int k = 0; k <= kPasses; k++)
     if ((e = hipEventCreate(&ctx->ev[k])) != hipSuccess) return bail(e);
+  for (int k = 0; k < 5; k++)
+    if ((e = hipEventCreate(&ctx->lev[k])) != hipSuccess) return bail(e);
   *out = ctx;
   return SL_OK;
 }
@@ -399,6 +425,8 @@ extern "C" void sl_ctx_destroy(sl_context *ctx) {
   if (ctx->ws.base) (void)hipFree(ctx->ws.base);
   for (int k = 0; k <= kPasses; k++)
     if (ctx->ev[k]) (void)hipEventDestroy(ctx->ev[k]);
+  for (int k = 0; k < 5; k++)
+    if (ctx->lev[k]) (void)hipEventDestroy(ctx->lev[k]);
   if (ctx->counters_host) (void)hipHostFree(ctx->counters_host);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -484,7 +512,27 @@ sl_status scene_alloc(sl_context *ctx, sl_scene *sc, size_t bytes, void **p) {
 
 void scene_free(sl_scene *sc) {
   for (void *p : sc->allocs) (void)hipFree(p);
+  for (Grow *gb : {&sc->emit_nodes, &sc->lsm_acc, &sc->lsm_mip, &sc->lsm_dirty, &sc->sun_acc, &sc->sun_mip,
+                   &sc->sun_desc, &sc->sun_layers, &sc->sun_dirty, &sc->map_rec, &sc->map_cnt, &sc->map_start,
+                   &sc->map_bsum, &sc->map_list})
+    if (gb->p) (void)hipFree(gb->p);
   delete sc;
+}
+
+sl_status grow(sl_context *ctx, sl_scene *sc, Grow *gb, size_t bytes) {
+  if (gb->p && gb->bytes >= bytes) return SL_OK;
+  if (gb->p) {
+    SL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    SL_HIP(ctx, hipFree(gb->p));
+    sc->bytes -= gb->bytes;
+    gb->p = nullptr;
+    gb->bytes = 0;
+  }
+  sl_status s = dev_alloc(ctx, bytes, &gb->p);
+  if (s != SL_OK) return s;
+  gb->bytes = bytes;
+  sc->bytes += bytes;
+  return SL_OK;
 }
 
 }  // namespace
@@ -513,6 +561,12 @@ extern "C" sl_status sl_scene_create(sl_context *ctx, const sl_scene_desc *d, sl
   if (d->emitter_count > 16384u || (d->emitter_count && !d->emitters))
     return fail(ctx, SL_ERR_LAYOUT, "Bad emitter count " + std::to_string(d->emitter_count));
   if (d->storage > SL_STORAGE_BF16) return fail(ctx, SL_ERR_INVALID, "unknown storage format");
+  for (uint32_t i = 0; i < d->gaussian_count; i++) {
+    const uint32_t part = rd_u32((const uint8_t *)d->gaussians + 64 * (size_t)i, 11) & 0xffffu;
+    if (part >= d->part_count)
+      return fail(ctx, SL_ERR_LAYOUT, "Gaussian " + std::to_string(i) + " names part " + std::to_string(part) +
+                                          " of " + std::to_string(d->part_count));
+  }
   SL_HIP(ctx, hipSetDevice(ctx->device));
   sl_scene *sc = new sl_scene();
   sc->count = d->gaussian_count;
@@ -532,6 +586,38 @@ extern "C" sl_status sl_scene_create(sl_context *ctx, const sl_scene_desc *d, sl
                                   ctx->stream);
     if (e == hipSuccess) e = hipMemsetAsync(sc->colour, 0, (size_t)sc->count * 16, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) s = hip_fail(ctx, e, "scene upload");
+  }
+  // relighting records: nodes (leaves), vgrid (all 255 when absent), materials
+  if (s == SL_OK && d->nodes) {
+    if ((s = scene_alloc(ctx, sc, (size_t)sc->count * 96, &p)) == SL_OK) {
+      sc->nodes = (float4 *)p;
+      hipError_t e = hipMemcpyAsync(sc->nodes, d->nodes, (size_t)sc->count * 96, hipMemcpyHostToDevice, ctx->stream);
+      if (e != hipSuccess) s = hip_fail(ctx, e, "node upload");
+    }
+    if (s == SL_OK && (s = scene_alloc(ctx, sc, (size_t)sc->count * 128, &p)) == SL_OK) {
+      sc->vgrid = (uint8_t *)p;
+      hipError_t e = d->vgrid ? hipMemcpyAsync(sc->vgrid, d->vgrid, (size_t)sc->count * 128, hipMemcpyHostToDevice,
+                                               ctx->stream)
+                              : hipMemsetAsync(sc->vgrid, 0xFF, (size_t)sc->count * 128, ctx->stream);
+      if (e != hipSuccess) s = hip_fail(ctx, e, "vgrid upload");
+    }
+    if (s == SL_OK && (s = scene_alloc(ctx, sc, (size_t)sc->count * 4, &p)) == SL_OK) sc->iso_norm = (float *)p;
+    if (s == SL_OK && (s = scene_alloc(ctx, sc, (size_t)sc->count * 4, &p)) == SL_OK) sc->lamp_T = (float *)p;
+    if (s == SL_OK && (s = scene_alloc(ctx, sc, (size_t)sc->count * 16, &p)) == SL_OK) sc->env_T = (float4 *)p;
+  }
+  if (s == SL_OK && d->materials && d->material_count) {
+    sc->materials_host.assign((const uint8_t *)d->materials, (const uint8_t *)d->materials + 32 * (size_t)d->material_count);
+    if ((s = scene_alloc(ctx, sc, 32 * (size_t)d->material_count, &p)) == SL_OK) {
+      sc->materials = (uint32_t *)p;
+      hipError_t e = hipMemcpyAsync(sc->materials, sc->materials_host.data(), sc->materials_host.size(),
+                                    hipMemcpyHostToDevice, ctx->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+      if (e != hipSuccess) s = hip_fail(ctx, e, "material upload");
+    }
+  }
+  if (s == SL_OK) {
+    hipError_t e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) s = hip_fail(ctx, e, "scene upload");
   }
   if (s != SL_OK) {
@@ -677,6 +763,263 @@ extern "C" sl_status sl_frame_read(sl_context *ctx, uint32_t which, void *host, 
   return SL_OK;
 }
 
+// ---- relighting ---------------------------------------------------------------------------------------------
+
+extern "C" sl_status sl_lighting_iso_init(sl_context *ctx, sl_scene *sc) {
+  SL_LIVE(ctx);
+  if (!sc) return SL_ERR_INVALID;
+  if (!sc->nodes) return fail(ctx, SL_ERR_INVALID, "relighting needs the scene's leaf nodes");
+  SL_HIP(ctx, hipSetDevice(ctx->device));
+  sl_iso_init<<<(sc->count + 63) / 64, 64, 0, ctx->stream>>>(sc->count, sc->g[2], sc->nodes, sc->vgrid, sc->iso_norm);
+  sl_status s = check_launch(ctx, "iso_init");
+  if (s != SL_OK) return s;
+  SL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  sc->iso_done = true;
+  return SL_OK;
+}
+
+namespace {
+
+template <class T>
+void read_struct(const void *src, T *dst) { memcpy(dst, src, sizeof(T)); }
+
+// every mip level of a layer lies inside it
+bool levels_fit(uint32_t res, uint32_t levels, const uint32_t *level_off, uint32_t layer_texels) {
+  if (res < 1 || res > 16384 || (res & (res - 1)) != 0 || levels < 1 || levels > 16) return false;
+  for (uint32_t k = 0; k < levels; k++) {
+    const uint32_t rk = res >> k;
+    if (rk < 1) return false;
+    if ((uint64_t)level_off[k] + (uint64_t)rk * rk > layer_texels) return false;
+  }
+  return true;
+}
+
+// Buffers for n_rec records over n_tiles map tiles, each record touching at most span x span tiles.
+sl_status ensure_map(sl_context *ctx, sl_scene *sc, size_t n_rec, uint32_t n_tiles, uint32_t span) {
+  sl_status s;
+  const bool fresh_cnt = !sc->map_cnt.p || sc->map_cnt.bytes < 4 * (size_t)n_tiles;
+  if ((s = grow(ctx, sc, &sc->map_rec, std::max<size_t>(n_rec, 1) * sizeof(MapRec))) != SL_OK) return s;
+  if ((s = grow(ctx, sc, &sc->map_cnt, 4 * (size_t)n_tiles)) != SL_OK) return s;
+  if ((s = grow(ctx, sc, &sc->map_start, 4 * ((size_t)n_tiles + 1))) != SL_OK) return s;
+  if ((s = grow(ctx, sc, &sc->map_bsum, 4 * (((size_t)n_tiles + 4095) / 4096))) != SL_OK) return s;
+  if ((s = grow(ctx, sc, &sc->map_list, 4 * std::max<size_t>(n_rec * span * span, 1))) != SL_OK) return s;
+  if (fresh_cnt)
+    sl_fill_u32<<<grid_for(n_tiles, 256, 4096), 256, 0, ctx->stream>>>((uint32_t *)sc->map_cnt.p, 0u, n_tiles);
+  return SL_OK;
+}
+
+// The records written by a prep pass -> per-tile lists -> per-tile sums into acc for the layers flagged dirty.
+void run_map_splat(sl_context *ctx, sl_scene *sc, size_t n_rec, uint32_t n_layers, uint32_t res, float fix,
+                   const uint32_t *dirty_dev, uint32_t *acc) {
+  hipStream_t st = ctx->stream;
+  const uint32_t tiles_x = (res + MT - 1) / MT, tpl = tiles_x * tiles_x, n_tiles = n_layers * tpl;
+  const uint32_t nb = (n_tiles + 4095) / 4096;
+  uint32_t *cnt = (uint32_t *)sc->map_cnt.p, *start = (uint32_t *)sc->map_start.p;
+  sl_map_reduce<<<nb, 256, 0, st>>>(cnt, n_tiles, (uint32_t *)sc->map_bsum.p);
+  sl_map_scan<<<nb, 256, 0, st>>>(cnt, n_tiles, (const uint32_t *)sc->map_bsum.p, start);
+  if (n_rec)
+    sl_map_fill<<<(uint32_t)((n_rec + 255) / 256), 256, 0, st>>>((uint32_t)n_rec, (const MapRec *)sc->map_rec.p,
+                                                                  tiles_x, tpl, start, cnt, (uint32_t *)sc->map_list.p);
+  sl_map_accumulate<<<n_tiles, 256, 0, st>>>((const MapRec *)sc->map_rec.p, (const uint32_t *)sc->map_list.p, start,
+                                             res, tiles_x, tpl, fix, dirty_dev, acc);
+}
+
+// tiles a box of at most 2 * max_half + 1 texels can span per axis
+uint32_t map_span(uint32_t max_half, uint32_t res) {
+  return std::min<uint32_t>((2u * max_half) / MT + 2u, (res + MT - 1) / MT);
+}
+
+sl_status run_opacity_maps(sl_context *ctx, sl_scene *sc, float4 *mip, uint32_t n_layers, uint32_t res,
+                           uint32_t levels, const uint32_t *level_off, uint32_t layer_texels, float fix,
+                           const uint32_t *acc, const uint32_t *dirty_dev) {
+  hipStream_t st = ctx->stream;
+  const size_t texels = (size_t)res * res;
+  sl_opacity_resolve<<<(uint32_t)((texels * n_layers + 255) / 256), 256, 0, st>>>(acc, mip, n_layers, (uint32_t)texels,
+                                                                                 layer_texels, fix, dirty_dev);
+  for (uint32_t k = 1; k < levels; k++) {
+    const uint32_t rk = res >> k;
+    sl_opacity_down<<<(uint32_t)(((size_t)rk * rk * n_layers + 255) / 256), 256, 0, st>>>(
+        mip, n_layers, rk, level_off[k - 1], level_off[k], layer_texels, dirty_dev);
+  }
+  return check_launch(ctx, "opacity map resolve");
+}
+
+}  // namespace
+
+extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_lighting_desc *d) {
+  SL_LIVE(ctx);
+  if (!sc || !d || !d->frame_uniforms || !d->emit_params || !d->emit_nodes)
+    return fail(ctx, SL_ERR_INVALID, "null argument");
+  if (!sc->nodes || !sc->materials) return fail(ctx, SL_ERR_INVALID, "relighting needs the scene's nodes and materials");
+  FrameUniformsDev U;
+  EmitParamsDev EP;
+  read_struct(d->frame_uniforms, &U);
+  read_struct(d->emit_params, &EP);
+  if (U.gaussian_count != sc->count) return fail(ctx, SL_ERR_INVALID, "FrameUniforms.gaussian_count != scene count");
+  if (U.material_count == 0 || U.material_count > sc->materials_host.size() / 32)
+    return fail(ctx, SL_ERR_INVALID, "FrameUniforms.material_count does not match the scene's materials");
+  if ((U.iso & 1u) == 0u || (U.iso & (8u | 16u)) != 0u)
+    return fail(ctx, SL_ERR_UNSUPPORTED, "only the M39 transfer without transmission or reflection split is on this path");
+  for (uint32_t m = 0; m < U.material_count; m++) {
+    const uint32_t fl = rd_u32(sc->materials_host.data() + 32 * (size_t)m, 5);
+    if ((fl & 1u) == 0u && (fl & 8u) != 0u)
+      return fail(ctx, SL_ERR_UNSUPPORTED, "shiny materials (flags bit 3) are not on this path");
+  }
+  if (EP.lights != 0u) return fail(ctx, SL_ERR_UNSUPPORTED, "unified lights are not on this path");
+  if (d->emit_node_count == 0 || EP.root >= d->emit_node_count || (EP.count && d->emit_node_count < 2 * EP.count - 1))
+    return fail(ctx, SL_ERR_INVALID, "emitter tree does not match EmitParams");
+  SL_HIP(ctx, hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  sl_status s;
+  if (!sc->iso_done && (s = sl_lighting_iso_init(ctx, sc)) != SL_OK) return s;
+  if ((s = grow(ctx, sc, &sc->emit_nodes, (size_t)d->emit_node_count * 96)) != SL_OK) return s;
+  SL_HIP(ctx, hipMemcpyAsync(sc->emit_nodes.p, d->emit_nodes, (size_t)d->emit_node_count * 96, hipMemcpyHostToDevice, st));
+  const uint32_t gblocks = (sc->count + 63) / 64;
+  const bool timers = (d->flags & SL_LIGHT_TIMERS) != 0;
+  if (timers) SL_HIP(ctx, hipEventRecord(ctx->lev[0], st));
+  // A: the emitter light's opacity map and its lookup -> lamp_T
+  const float *lampT = nullptr;
+  if (d->ls_params) {
+    LSParamsDev LS;
+    read_struct(d->ls_params, &LS);
+    if (LS.part_count != sc->part_count || !levels_fit(LS.res, LS.levels, LS.level_off, LS.layer_texels))
+      return fail(ctx, SL_ERR_INVALID, "LSParams do not match the scene");
+    const size_t texels = (size_t)LS.res * LS.res;
+    if ((s = grow(ctx, sc, &sc->lsm_acc, texels * 12 * LS.part_count)) != SL_OK) return s;
+    if ((s = grow(ctx, sc, &sc->lsm_mip, (size_t)LS.layer_texels * 16 * LS.part_count)) != SL_OK) return s;
+    if ((s = grow(ctx, sc, &sc->lsm_dirty, 4 * 32)) != SL_OK) return s;
+    if (sc->lsm_res != LS.res || sc->lsm_parts != LS.part_count) LS.dirty = (1u << LS.part_count) - 1u;
+    sc->lsm_res = LS.res;
+    sc->lsm_parts = LS.part_count;
+    sc->lsm_layer_texels = LS.layer_texels;
+    uint32_t flags[32];
+    for (uint32_t p = 0; p < 32; p++) flags[p] = (LS.dirty >> p) & 1u;
+    SL_HIP(ctx, hipMemcpyAsync(sc->lsm_dirty.p, flags, sizeof(flags), hipMemcpyHostToDevice, st));
+    if (LS.dirty) {
+      const uint32_t tiles_x = (LS.res + MT - 1) / MT;
+      if ((s = ensure_map(ctx, sc, sc->count, LS.part_count * tiles_x * tiles_x, map_span(48, LS.res))) != SL_OK)
+        return s;
+      if (sc->count)
+        sl_lsm_prep<<<gblocks, 64, 0, st>>>(sc->count, LS, sc->g[0], sc->g[1], sc->g[2], sc->nodes, sc->parts,
+                                            (MapRec *)sc->map_rec.p, (uint32_t *)sc->map_cnt.p);
+      run_map_splat(ctx, sc, sc->count, LS.part_count, LS.res, 65536.0f, (const uint32_t *)sc->lsm_dirty.p,
+                    (uint32_t *)sc->lsm_acc.p);
+      if ((s = run_opacity_maps(ctx, sc, (float4 *)sc->lsm_mip.p, LS.part_count, LS.res, LS.levels, LS.level_off,
+                                LS.layer_texels, 65536.0f, (const uint32_t *)sc->lsm_acc.p,
+                                (const uint32_t *)sc->lsm_dirty.p)) != SL_OK)
+        return s;
+    }
+    sl_lsm_lookup<<<gblocks, 64, 0, st>>>(sc->count, LS, sc->g[0], sc->g[2], sc->parts, (const float4 *)sc->lsm_mip.p,
+                                          sc->lamp_T);
+    lampT = sc->lamp_T;
+  }
+  if (timers) SL_HIP(ctx, hipEventRecord(ctx->lev[1], st));
+  sl_shade_emit<<<gblocks, 64, 0, st>>>(U, EP, (const float4 *)sc->emit_nodes.p, d->emit_node_count, sc->g[0], sc->g[1], sc->g[2], sc->g[3],
+                                        sc->nodes, sc->vgrid, sc->parts, lampT, sc->materials, sc->iso_norm, sc->colour);
+  if (timers) SL_HIP(ctx, hipEventRecord(ctx->lev[2], st));
+  // distant lights: sun maps -> env_T, then env_direct
+  if (d->env) {
+    EnvDev ENV;
+    read_struct(d->env, &ENV);
+    const float4 *envT = nullptr;
+    if (d->sun_params && ENV.info[0] > 0u) {
+      SunParamsDev SM;
+      read_struct(d->sun_params, &SM);
+      if (SM.n_layers != d->sun_layer_count || !d->sun_layers || SM.n_layers == 0 || SM.n_lights > 4 ||
+          !levels_fit(SM.res, SM.levels, SM.level_off, SM.layer_texels))
+        return fail(ctx, SL_ERR_INVALID, "SunParams / layer plan do not match");
+      for (uint32_t p = 0; p < 32; p++)
+        if ((uint64_t)SM.part_layers[2 * p] + SM.part_layers[2 * p + 1] > SM.n_layers)
+          return fail(ctx, SL_ERR_INVALID, "SunParams part layer range outside the plan");
+      std::vector<SunLayerDesc> desc(SM.n_layers);
+      memcpy(desc.data(), d->sun_layers, sizeof(SunLayerDesc) * SM.n_layers);
+      for (const SunLayerDesc &ld : desc)
+        if (ld.part >= sc->part_count || ld.light >= 4u) return fail(ctx, SL_ERR_INVALID, "bad sun layer");
+      const size_t texels = (size_t)SM.res * SM.res;
+      if ((s = grow(ctx, sc, &sc->sun_acc, texels * 12 * SM.n_layers)) != SL_OK) return s;
+      if ((s = grow(ctx, sc, &sc->sun_mip, (size_t)SM.layer_texels * 16 * SM.n_layers)) != SL_OK) return s;
+      if ((s = grow(ctx, sc, &sc->sun_desc, sizeof(SunLayerDesc) * SM.n_layers)) != SL_OK) return s;
+      if ((s = grow(ctx, sc, &sc->sun_layers, sizeof(SunLayer) * SM.n_layers)) != SL_OK) return s;
+      if ((s = grow(ctx, sc, &sc->sun_dirty, 4 * (size_t)SM.n_layers)) != SL_OK) return s;
+      const bool fresh = sc->sun_res != SM.res || sc->sun_n != SM.n_layers;
+      std::vector<uint32_t> flags(SM.n_layers);
+      for (uint32_t l = 0; l < SM.n_layers; l++) {
+        if (fresh) desc[l].dirty = 1u;
+        flags[l] = desc[l].dirty ? 1u : 0u;
+      }
+      sc->sun_res = SM.res;
+      sc->sun_n = SM.n_layers;
+      sc->sun_layer_texels = SM.layer_texels;
+      SL_HIP(ctx, hipMemcpyAsync(sc->sun_desc.p, desc.data(), sizeof(SunLayerDesc) * SM.n_layers, hipMemcpyHostToDevice, st));
+      SL_HIP(ctx, hipMemcpyAsync(sc->sun_dirty.p, flags.data(), 4 * (size_t)SM.n_layers, hipMemcpyHostToDevice, st));
+      sl_sun_frame<<<(SM.n_layers + 63) / 64, 64, 0, st>>>(SM, (const SunLayerDesc *)sc->sun_desc.p, sc->parts,
+                                                           (SunLayer *)sc->sun_layers.p);
+      bool any = false;
+      for (uint32_t l = 0; l < SM.n_layers; l++) any = any || flags[l] != 0u;
+      if (any) {
+        uint32_t max_layers = 0;
+        for (uint32_t p = 0; p < sc->part_count; p++) max_layers = std::max(max_layers, SM.part_layers[2 * p + 1]);
+        const uint32_t tiles_x = (SM.res + MT - 1) / MT;
+        const size_t n_rec = (size_t)sc->count * max_layers;
+        if ((s = ensure_map(ctx, sc, n_rec, SM.n_layers * tiles_x * tiles_x, map_span(32, SM.res))) != SL_OK) return s;
+        if (n_rec)
+          sl_sun_prep<<<gblocks, 64, 0, st>>>(sc->count, SM, (const SunLayerDesc *)sc->sun_desc.p,
+                                              (const SunLayer *)sc->sun_layers.p, sc->g[0], sc->g[1], sc->g[2],
+                                              sc->nodes, sc->parts, max_layers, (MapRec *)sc->map_rec.p,
+                                              (uint32_t *)sc->map_cnt.p);
+        run_map_splat(ctx, sc, n_rec, SM.n_layers, SM.res, 4096.0f, (const uint32_t *)sc->sun_dirty.p,
+                      (uint32_t *)sc->sun_acc.p);
+        if ((s = run_opacity_maps(ctx, sc, (float4 *)sc->sun_mip.p, SM.n_layers, SM.res, SM.levels, SM.level_off,
+                                  SM.layer_texels, 4096.0f, (const uint32_t *)sc->sun_acc.p,
+                                  (const uint32_t *)sc->sun_dirty.p)) != SL_OK)
+          return s;
+      }
+      sl_sun_lookup<<<gblocks, 64, 0, st>>>(sc->count, SM, (const SunLayerDesc *)sc->sun_desc.p,
+                                            (const SunLayer *)sc->sun_layers.p, sc->g[0], sc->g[2], sc->parts,
+                                            (const float4 *)sc->sun_mip.p, sc->env_T);
+      envT = sc->env_T;
+    } else {
+      sl_fill_u32<<<grid_for((size_t)sc->count * 4, 256, 4096), 256, 0, st>>>((uint32_t *)sc->env_T, 0x3f800000u,
+                                                                               sc->count * 4);
+      envT = sc->env_T;
+    }
+    if (timers) SL_HIP(ctx, hipEventRecord(ctx->lev[3], st));
+    sl_env_direct<<<gblocks, 64, 0, st>>>(U, ENV, sc->g[2], sc->g[3], sc->nodes, sc->vgrid, sc->parts, sc->materials,
+                                          sc->iso_norm, envT, sc->colour);
+  } else if (timers) {
+    SL_HIP(ctx, hipEventRecord(ctx->lev[3], st));
+  }
+  if (timers) SL_HIP(ctx, hipEventRecord(ctx->lev[4], st));
+  ctx->light_timers_valid = timers;
+  return check_launch(ctx, "lighting kernels");
+}
+
+extern "C" sl_status sl_scene_read(sl_context *ctx, sl_scene *sc, uint32_t which, void *host, size_t bytes,
+                                   size_t *written) {
+  SL_LIVE(ctx);
+  if (!sc) return SL_ERR_INVALID;
+  SL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  const void *src = nullptr;
+  size_t n = 0;
+  switch (which) {
+    case SL_SCENE_COLOUR: src = sc->colour; n = (size_t)sc->count * 16; break;
+    case SL_SCENE_LAMP_T: src = sc->lamp_T; n = sc->lamp_T ? (size_t)sc->count * 4 : 0; break;
+    case SL_SCENE_ENV_T: src = sc->env_T; n = sc->env_T ? (size_t)sc->count * 16 : 0; break;
+    case SL_SCENE_ISO_NORM: src = sc->iso_norm; n = sc->iso_norm ? (size_t)sc->count * 4 : 0; break;
+    case SL_SCENE_LSM_ACC: src = sc->lsm_acc.p; n = (size_t)sc->lsm_res * sc->lsm_res * 12 * sc->lsm_parts; break;
+    case SL_SCENE_LSM_MIP: src = sc->lsm_mip.p; n = (size_t)sc->lsm_layer_texels * 16 * sc->lsm_parts; break;
+    case SL_SCENE_SUN_ACC: src = sc->sun_acc.p; n = (size_t)sc->sun_res * sc->sun_res * 12 * sc->sun_n; break;
+    case SL_SCENE_SUN_MIP: src = sc->sun_mip.p; n = (size_t)sc->sun_layer_texels * 16 * sc->sun_n; break;
+    default: return fail(ctx, SL_ERR_INVALID, "unknown scene state");
+  }
+  if (written) *written = n;
+  if (!host || n == 0) return SL_OK;
+  if (bytes < n) return fail(ctx, SL_ERR_INVALID, "host buffer too small");
+  SL_HIP(ctx, hipMemcpyAsync(host, src, n, hipMemcpyDeviceToHost, ctx->stream));
+  SL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return SL_OK;
+}
+
 extern "C" sl_status sl_timers_read(sl_context *ctx, const char **names, float *ms, uint32_t cap, uint32_t *n) {
   SL_LIVE(ctx);
   uint32_t k = 0;
@@ -685,6 +1028,17 @@ extern "C" sl_status sl_timers_read(sl_context *ctx, const char **names, float *
       if (names) names[k] = kPassNames[k];
       if (ms) ms[k] = ctx->ms[k];
     }
+  if (ctx->light_timers_valid) {
+    // the lighting passes of the last sl_lighting_encode with SL_LIGHT_TIMERS (PASSES names: shadow, shade)
+    static const char *const lnames[] = {"shadow", "shade", "sunmap", "env_direct"};
+    SL_HIP(ctx, hipEventSynchronize(ctx->lev[4]));
+    for (int j = 0; j < 4 && k < cap; j++, k++) {
+      float v = 0.0f;
+      SL_HIP(ctx, hipEventElapsedTime(&v, ctx->lev[j], ctx->lev[j + 1]));
+      if (names) names[k] = lnames[j];
+      if (ms) ms[k] = v;
+    }
+  }
   if (n) *n = k;
   return SL_OK;
 }

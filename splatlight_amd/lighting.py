@@ -339,3 +339,60 @@ def sun_plan(parts: np.ndarray, lights, eye, fwd, res=512, cap_bytes=None, stati
     for k, (o, _) in enumerate(levels):
         sp[92 + k] = o
     return sp.tobytes(), du.tobytes(), n_layers, res
+
+
+# ---- the per-scene lighting setup the host hands to sl_lighting_encode ----------------------------------------
+
+def lighting_inputs(sc, eye=None, fwd=None, dirty_all=True, sun_res=None, frame=0):
+    """All uniform bytes for one lit frame of a synthetic scene (scenes.Scene): FrameUniforms, EmitParams, the
+    emitter tree, LSParams, Env, SunParams and the sun layer plan. Maps are marked dirty when dirty_all."""
+    eye = tuple(eye or sc.eye)
+    if fwd is None:
+        d = np.asarray(sc.target, np.float64) - np.asarray(eye, np.float64)
+        fwd = tuple(d / np.linalg.norm(d))
+    U = R.frame_uniforms(frame=frame, gaussian_count=sc.count, part_count=sc.parts.shape[0],
+                         theta=8.0, alpha_min=1 / 16, alpha_part=0.5, eye=eye,
+                         material_count=sc.materials.shape[0], iso=1 | 2)
+    nodes, root, cov = build_emitter_tree(sc.emitters)
+    n = sc.emitters.shape[0]
+    normal = one_sided_normal(sc.emitters[0]) if sc.one_sided else [0.0, 0.0, 1.0]
+    EP = R.emit_params(count=n, root=root, scale=1.0, theta=EMIT_THETA, mode=1, node_count=nodes.shape[0],
+                       skip_part=0xFFFFFFFF, one_sided=1 if sc.one_sided else 0, normal=normal, n5=1 << 3,
+                       edge_soft=EMIT_EDGE_SOFT)
+    P = sc.parts.shape[0]
+    LS = ls_params(nodes, root, cov, normal, sc.one_sided, P, ((1 << P) - 1) if dirty_all else 0)
+    out = {"U": U, "EP": EP, "emit_nodes": pack_emit_nodes(nodes, cov), "LS": LS, "ENV": None, "SM": None,
+           "desc": None}
+    if sc.sun is not None:
+        d = sc.sun["dir"]
+        out["ENV"] = env_bytes([(d, sc.sun["std"], sc.sun["E"], True)])
+        SM, desc, nl, res = sun_plan(sc.parts, [(d, sc.sun["std"], True)], eye, fwd,
+                                     res=sun_res or sc.sun.get("res", 512))
+        if not dirty_all:
+            du = np.frombuffer(desc, np.uint32).reshape(-1, 8).copy()
+            du[:, 3] = 0
+            desc = du.tobytes()
+        out["SM"], out["desc"] = SM, desc
+    return out
+
+
+class Lighting:
+    """Lighting (src/lighting.js:318-1275) over the HIP library, for the default static M39 matte path: the host
+    keeps the reference's parameter logic; encode() runs the opacity maps of what moved, shade_emit and the
+    distant lights' direct light on the device."""
+
+    def __init__(self, device, scene_native, sc, sun_res=None):
+        self.device = device
+        self.scene = scene_native
+        self.sc = sc
+        self.sun_res = sun_res
+        self.frame = 0
+        self.first = True
+
+    def encode(self, eye=None, changed=True, timers=False):
+        inp = lighting_inputs(self.sc, eye=eye, dirty_all=self.first or changed, sun_res=self.sun_res,
+                              frame=self.frame)
+        self.frame += 1
+        self.scene.light(inp["U"], inp["EP"], inp["emit_nodes"], inp["LS"], inp["ENV"], inp["SM"], inp["desc"],
+                         timers=timers)
+        self.first = False

@@ -29,7 +29,11 @@ EXPORTS = [
     "sl_scene_create", "sl_scene_destroy", "sl_scene_write_parts", "sl_scene_write_colour",
     "sl_scene_write_bounce", "sl_scene_device_bytes",
     "sl_frame_encode", "sl_frame_finish", "sl_frame_read", "sl_timers_read",
+    "sl_lighting_iso_init", "sl_lighting_encode", "sl_scene_read",
 ]
+SCENE_COLOUR, SCENE_LAMP_T, SCENE_ENV_T, SCENE_ISO_NORM = 1, 2, 3, 4
+SCENE_LSM_ACC, SCENE_LSM_MIP, SCENE_SUN_ACC, SCENE_SUN_MIP = 5, 6, 7, 8
+SL_LIGHT_TIMERS = 1
 
 
 class SceneDesc(C.Structure):
@@ -51,6 +55,13 @@ class FrameOpts(C.Structure):
 class FrameStats(C.Structure):
     _fields_ = [("slots", C.c_uint32), ("visible", C.c_uint32), ("pairs", C.c_uint32), ("overflow", C.c_uint32),
                 ("tiles", C.c_uint32), ("pad", C.c_uint32 * 3)]
+
+
+class LightingDesc(C.Structure):
+    _fields_ = [("frame_uniforms", C.c_void_p), ("emit_params", C.c_void_p), ("emit_nodes", C.c_void_p),
+                ("emit_node_count", C.c_uint32), ("ls_params", C.c_void_p), ("env", C.c_void_p),
+                ("sun_params", C.c_void_p), ("sun_layers", C.c_void_p), ("sun_layer_count", C.c_uint32),
+                ("flags", C.c_uint32)]
 
 
 class SplatlightError(RuntimeError):
@@ -95,6 +106,9 @@ def lib() -> C.CDLL:
         "sl_frame_finish": (i32, [vp, C.POINTER(FrameStats)]),
         "sl_frame_read": (i32, [vp, u32, vp, sz, C.POINTER(sz)]),
         "sl_timers_read": (i32, [vp, C.POINTER(C.c_char_p), C.POINTER(C.c_float), u32, C.POINTER(u32)]),
+        "sl_lighting_iso_init": (i32, [vp, vp]),
+        "sl_lighting_encode": (i32, [vp, vp, C.POINTER(LightingDesc)]),
+        "sl_scene_read": (i32, [vp, vp, u32, vp, sz, C.POINTER(sz)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -215,6 +229,42 @@ class Scene:
     @property
     def device_bytes(self) -> int:
         return int(self.ctx.L.sl_scene_device_bytes(self.h))
+
+    def iso_init(self):
+        self.ctx.check(self.ctx.L.sl_lighting_iso_init(self.ctx.h, self.h))
+
+    def light(self, frame_uniforms: bytes, emit_params: bytes, emit_nodes: np.ndarray, ls_params: bytes | None = None,
+              env: bytes | None = None, sun_params: bytes | None = None, sun_layers: bytes | None = None,
+              timers: bool = False):
+        """sl_lighting_encode: the lighting passes for this scene, leaving the lit colour on the device."""
+        keep = []
+
+        def buf(b):
+            if b is None:
+                return None
+            a = (C.c_uint8 * len(b)).from_buffer_copy(b)
+            keep.append(a)
+            return C.cast(a, C.c_void_p)
+        en = np.ascontiguousarray(emit_nodes, np.float32)
+        d = LightingDesc()
+        d.frame_uniforms = buf(frame_uniforms)
+        d.emit_params = buf(emit_params)
+        d.emit_nodes = en.ctypes.data_as(C.c_void_p)
+        d.emit_node_count = en.nbytes // 96
+        d.ls_params = buf(ls_params)
+        d.env = buf(env)
+        d.sun_params = buf(sun_params)
+        d.sun_layers = buf(sun_layers)
+        d.sun_layer_count = 0 if sun_layers is None else len(sun_layers) // 32
+        d.flags = SL_LIGHT_TIMERS if timers else 0
+        self.ctx.check(self.ctx.L.sl_lighting_encode(self.ctx.h, self.h, C.byref(d)))
+
+    def read(self, which: int, dtype=np.float32) -> np.ndarray:
+        n = C.c_size_t()
+        self.ctx.check(self.ctx.L.sl_scene_read(self.ctx.h, self.h, which, None, 0, C.byref(n)))
+        out = np.empty(n.value // np.dtype(dtype).itemsize, dtype)
+        self.ctx.check(self.ctx.L.sl_scene_read(self.ctx.h, self.h, which, _ptr(out), out.nbytes, C.byref(n)))
+        return out
 
 
 def frame_encode(ctx: Context, scene: Scene, frame: bytes, width: int, height: int, out_dev: int,
