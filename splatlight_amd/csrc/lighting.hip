@@ -453,22 +453,12 @@ sl_map_fill(uint32_t n_rec, const MapRec *__restrict__ recs, uint32_t tiles_x, u
     }
 }
 
-// One workgroup per 16x16 tile of every layer: acc[layer][texel][3] = the sums of the tile's records.
-// Layers whose dirty flag is 0 keep their sums.
-extern "C" __global__ void __launch_bounds__(256)
-sl_map_accumulate(const MapRec *__restrict__ recs, const uint32_t *__restrict__ list,
-                  const uint32_t *__restrict__ tile_start, uint32_t res, uint32_t tiles_x, uint32_t tpl, float fix,
-                  const uint32_t *__restrict__ dirty, uint32_t *__restrict__ acc) {
-  __shared__ MapRec s_rec[256];
-  const uint32_t tile = blockIdx.x, layer = tile / tpl;
-  if (!dirty[layer]) return;
-  const uint32_t rem = tile - layer * tpl, ty = rem / tiles_x, tx = rem - ty * tiles_x;
-  const int x = (int)(tx * MT + (threadIdx.x & 15u)), y = (int)(ty * MT + (threadIdx.x >> 4));
-  const int wy0 = (int)(ty * MT + (threadIdx.x >> 6) * 4u), wx0 = (int)(tx * MT);
-  const uint32_t begin = tile_start[tile], end = tile_start[tile + 1];
-  uint32_t a0 = 0, a1 = 0, a2 = 0;
-  for (uint32_t b = begin; b < end; b += 256u) {
-    const uint32_t nb = min(256u, end - b);
+// Per-tile sums of the records list[s0, s1) of tile `tile`, one lane per texel of the 16x16 tile.
+__device__ __forceinline__ void map_tile_sums(const MapRec *__restrict__ recs, const uint32_t *__restrict__ list,
+                                              uint32_t s0, uint32_t s1, int x, int y, int wx0, int wy0, float fix,
+                                              MapRec *s_rec, uint32_t &a0, uint32_t &a1, uint32_t &a2) {
+  for (uint32_t b = s0; b < s1; b += 256u) {
+    const uint32_t nb = min(256u, s1 - b);
     __syncthreads();
     if (threadIdx.x < nb) s_rec[threadIdx.x] = recs[list[b + threadIdx.x]];
     __syncthreads();
@@ -491,11 +481,60 @@ sl_map_accumulate(const MapRec *__restrict__ recs, const uint32_t *__restrict__ 
       a2 += u32_sat(t_fix * (z * z + r.sz2) + 0.5f);
     }
   }
-  if ((uint32_t)x < res && (uint32_t)y < res) {
-    uint32_t *k = acc + ((size_t)layer * res * res + (size_t)y * res + (size_t)x) * 3;
-    k[0] = a0;
-    k[1] = a1;
-    k[2] = a2;
+}
+
+// last index t in [0, n] with start[t] <= v (start ascending)
+__device__ __forceinline__ uint32_t last_le(const uint32_t *__restrict__ start, uint32_t n, uint32_t v) {
+  uint32_t lo = 0, hi = n;   // start[0] = 0 <= v
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1u) >> 1;
+    if (start[mid] <= v) lo = mid; else hi = mid - 1u;
+  }
+  return lo;
+}
+
+// Persistent: workgroups take chunks of MAP_CHUNK consecutive list entries (the list is grouped by tile), so a
+// crowded tile is shared by several workgroups. A tile inside one chunk is stored; a tile split across chunks
+// is merged with integer atomic adds (order-independent). acc of the dirty layers is zero on entry; tiles
+// with no records are not touched.
+constexpr uint32_t MAP_CHUNK = 1024;
+extern "C" __global__ void __launch_bounds__(256)
+sl_map_accumulate(const MapRec *__restrict__ recs, const uint32_t *__restrict__ list,
+                  const uint32_t *__restrict__ tile_start, uint32_t n_tiles, uint32_t res, uint32_t tiles_x,
+                  uint32_t tpl, float fix, uint32_t *__restrict__ acc) {
+  __shared__ MapRec s_rec[256];
+  __shared__ uint32_t s_tile;
+  const uint32_t total = tile_start[n_tiles];
+  const uint32_t n_chunks = (total + MAP_CHUNK - 1) / MAP_CHUNK;
+  for (uint32_t c = blockIdx.x; c < n_chunks; c += gridDim.x) {
+    const uint32_t lo = c * MAP_CHUNK, hi = min(lo + MAP_CHUNK, total);
+    __syncthreads();
+    if (threadIdx.x == 0) s_tile = last_le(tile_start, n_tiles, lo);
+    __syncthreads();
+    uint32_t tile = s_tile;
+    while (tile < n_tiles) {
+      const uint32_t t0 = tile_start[tile], t1 = tile_start[tile + 1];
+      if (t0 >= hi) break;
+      const uint32_t layer = tile / tpl, rem = tile - layer * tpl, ty = rem / tiles_x, tx = rem - ty * tiles_x;
+      const int x = (int)(tx * MT + (threadIdx.x & 15u)), y = (int)(ty * MT + (threadIdx.x >> 4));
+      const int wy0 = (int)(ty * MT + (threadIdx.x >> 6) * 4u), wx0 = (int)(tx * MT);
+      uint32_t a0 = 0, a1 = 0, a2 = 0;
+      map_tile_sums(recs, list, max(t0, lo), min(t1, hi), x, y, wx0, wy0, fix, s_rec, a0, a1, a2);
+      if ((uint32_t)x < res && (uint32_t)y < res) {
+        uint32_t *k = acc + ((size_t)layer * res * res + (size_t)y * res + (size_t)x) * 3;
+        if (t0 >= lo && t1 <= hi) {
+          k[0] = a0;
+          k[1] = a1;
+          k[2] = a2;
+        } else {
+          if (a0) atomicAdd(k, a0);
+          if (a1) atomicAdd(k + 1, a1);
+          if (a2) atomicAdd(k + 2, a2);
+        }
+      }
+      // next tile with records: the last one whose start is t1
+      tile = t1 >= hi ? n_tiles : last_le(tile_start, n_tiles, t1);
+    }
   }
 }
 

@@ -808,9 +808,9 @@ sl_status ensure_map(sl_context *ctx, sl_scene *sc, size_t n_rec, uint32_t n_til
   return SL_OK;
 }
 
-// The records written by a prep pass -> per-tile lists -> per-tile sums into acc for the layers flagged dirty.
+// The records written by a prep pass -> per-tile lists -> per-tile sums into acc, whose dirty layers are zero.
 void run_map_splat(sl_context *ctx, sl_scene *sc, size_t n_rec, uint32_t n_layers, uint32_t res, float fix,
-                   const uint32_t *dirty_dev, uint32_t *acc) {
+                   uint32_t *acc) {
   hipStream_t st = ctx->stream;
   const uint32_t tiles_x = (res + MT - 1) / MT, tpl = tiles_x * tiles_x, n_tiles = n_layers * tpl;
   const uint32_t nb = (n_tiles + 4095) / 4096;
@@ -820,8 +820,8 @@ void run_map_splat(sl_context *ctx, sl_scene *sc, size_t n_rec, uint32_t n_layer
   if (n_rec)
     sl_map_fill<<<(uint32_t)((n_rec + 255) / 256), 256, 0, st>>>((uint32_t)n_rec, (const MapRec *)sc->map_rec.p,
                                                                   tiles_x, tpl, start, cnt, (uint32_t *)sc->map_list.p);
-  sl_map_accumulate<<<n_tiles, 256, 0, st>>>((const MapRec *)sc->map_rec.p, (const uint32_t *)sc->map_list.p, start,
-                                             res, tiles_x, tpl, fix, dirty_dev, acc);
+  sl_map_accumulate<<<ctx->cus * 8, 256, 0, st>>>((const MapRec *)sc->map_rec.p, (const uint32_t *)sc->map_list.p,
+                                                  start, n_tiles, res, tiles_x, tpl, fix, acc);
 }
 
 // tiles a box of at most 2 * max_half + 1 texels can span per axis
@@ -899,11 +899,14 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
       const uint32_t tiles_x = (LS.res + MT - 1) / MT;
       if ((s = ensure_map(ctx, sc, sc->count, LS.part_count * tiles_x * tiles_x, map_span(48, LS.res))) != SL_OK)
         return s;
+      for (uint32_t p = 0; p < LS.part_count; p++)
+        if ((LS.dirty >> p) & 1u)
+          sl_fill_u32<<<grid_for(texels * 3, 256, 4096), 256, 0, st>>>((uint32_t *)sc->lsm_acc.p + texels * 3 * p, 0u,
+                                                                       (uint32_t)(texels * 3));
       if (sc->count)
         sl_lsm_prep<<<gblocks, 64, 0, st>>>(sc->count, LS, sc->g[0], sc->g[1], sc->g[2], sc->nodes, sc->parts,
                                             (MapRec *)sc->map_rec.p, (uint32_t *)sc->map_cnt.p);
-      run_map_splat(ctx, sc, sc->count, LS.part_count, LS.res, 65536.0f, (const uint32_t *)sc->lsm_dirty.p,
-                    (uint32_t *)sc->lsm_acc.p);
+      run_map_splat(ctx, sc, sc->count, LS.part_count, LS.res, 65536.0f, (uint32_t *)sc->lsm_acc.p);
       if ((s = run_opacity_maps(ctx, sc, (float4 *)sc->lsm_mip.p, LS.part_count, LS.res, LS.levels, LS.level_off,
                                 LS.layer_texels, 65536.0f, (const uint32_t *)sc->lsm_acc.p,
                                 (const uint32_t *)sc->lsm_dirty.p)) != SL_OK)
@@ -955,7 +958,12 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
       sl_sun_frame<<<(SM.n_layers + 63) / 64, 64, 0, st>>>(SM, (const SunLayerDesc *)sc->sun_desc.p, sc->parts,
                                                            (SunLayer *)sc->sun_layers.p);
       bool any = false;
-      for (uint32_t l = 0; l < SM.n_layers; l++) any = any || flags[l] != 0u;
+      for (uint32_t l = 0; l < SM.n_layers; l++)
+        if (flags[l]) {
+          any = true;
+          sl_fill_u32<<<grid_for(texels * 3, 256, 4096), 256, 0, st>>>((uint32_t *)sc->sun_acc.p + texels * 3 * l, 0u,
+                                                                       (uint32_t)(texels * 3));
+        }
       if (any) {
         uint32_t max_layers = 0;
         for (uint32_t p = 0; p < sc->part_count; p++) max_layers = std::max(max_layers, SM.part_layers[2 * p + 1]);
@@ -967,8 +975,7 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
                                               (const SunLayer *)sc->sun_layers.p, sc->g[0], sc->g[1], sc->g[2],
                                               sc->nodes, sc->parts, max_layers, (MapRec *)sc->map_rec.p,
                                               (uint32_t *)sc->map_cnt.p);
-        run_map_splat(ctx, sc, n_rec, SM.n_layers, SM.res, 4096.0f, (const uint32_t *)sc->sun_dirty.p,
-                      (uint32_t *)sc->sun_acc.p);
+        run_map_splat(ctx, sc, n_rec, SM.n_layers, SM.res, 4096.0f, (uint32_t *)sc->sun_acc.p);
         if ((s = run_opacity_maps(ctx, sc, (float4 *)sc->sun_mip.p, SM.n_layers, SM.res, SM.levels, SM.level_off,
                                   SM.layer_texels, 4096.0f, (const uint32_t *)sc->sun_acc.p,
                                   (const uint32_t *)sc->sun_dirty.p)) != SL_OK)
