@@ -36,35 +36,80 @@ def parse():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--half", type=int, default=1, help="1: rgba16f target (the reference's), 0: rgba32f")
     ap.add_argument("--cpu-baseline", type=int, default=1)
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="only run a plain device-to-device copy of this many MiB, 5 times (HBM counter calibration)")
     return ap.parse_args()
 
 
-def cpu_baseline(sc, fb):
-    """The CPU restatement (oracle, one core) on one frame of the same workload, colour from shade_emit."""
+def copy_calibration(mib):
+    """A plain copy kernel of known size: what FETCH_SIZE / WRITE_SIZE read for it scales the other kernels'."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / 5
+    print(json.dumps({"copy_mib": mib, "bytes_read": n * 4, "bytes_written": n * 4, "ms": round(dt * 1e3, 4),
+                      "GB_s": round(2 * n * 4 / dt / 1e9, 1)}))
+
+
+def cpu_baseline(sc, fb, k=8):
+    """The CPU restatement (oracle, one core) on a bounded sample of one relit frame, scaled to the frame.
+
+    Per-Gaussian relighting (emitter map splat + lookup, shade_emit, sun map splat + lookup, env_direct) runs on
+    every k-th Gaussian and is scaled by k; the per-texel map resolves are timed alone and not scaled; project,
+    depth order and tile lists run on the whole frame; the fp16-target blend runs on every k-th pixel row and is
+    scaled by k. iso_norm (once per asset) is not timed, as on the device.
+    """
+    import dataclasses
+
     import oracle
     from splatlight_amd import lighting as L
     from splatlight_amd import scenes
     W, H = sc.width, sc.height
-    inp = L.lighting_inputs(sc, dirty_all=True)
-    nd = scenes.leaf_nodes(sc.gaussians)
-    vg = scenes.vgrid_open(sc.count)
+    sub = dataclasses.replace(sc, gaussians=np.ascontiguousarray(sc.gaussians[::k]))
+    inp = L.lighting_inputs(sub, dirty_all=True)
+    nd = scenes.leaf_nodes(sub.gaussians)
+    vg = scenes.vgrid_open(sub.count)
+    iso = oracle.iso_init(sub.gaussians, nd, vg)
     t0 = time.perf_counter()
-    iso = oracle.iso_init(sc.gaussians, nd, vg)
-    col = oracle.shade_emit(inp["U"], inp["EP"], inp["emit_nodes"], sc.gaussians, nd, vg, sc.parts, None,
-                            sc.materials, iso)
-    sp, keys, vis = oracle.project(fb, sc, colour=col)
+    acc, _, T = oracle.lsm(inp["U"], inp["LS"], sub.gaussians, nd, sub.parts)
+    col = oracle.shade_emit(inp["U"], inp["EP"], inp["emit_nodes"], sub.gaussians, nd, vg, sub.parts, T,
+                            sub.materials, iso)
+    sacc, _, envT = oracle.sun(inp["U"], inp["SM"], inp["desc"], sub.gaussians, nd, sub.parts)
+    oracle.env_direct(inp["U"], inp["ENV"], sub.gaussians, nd, vg, sub.parts, sub.materials, iso, envT, col)
+    t_light = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    oracle.lsm_resolve(inp["LS"], acc)
+    oracle.sun_resolve(inp["SM"], inp["desc"], sacc)
+    t_resolve = time.perf_counter() - t0
+    colour = np.concatenate([sc.gaussians["albedo"], np.ones((sc.count, 1))], 1).astype(np.float32)
+    t0 = time.perf_counter()
+    sp, keys, vis = oracle.project(fb, sc, colour=colour)
     _, _, order, _ = oracle.sort_keys(keys)
-    start, vals = oracle.tile_lists(sp, order, vis, W, H)
-    oracle.composite(fb, sp, order, vis, W, H, mode=2)
-    dt = time.perf_counter() - t0
-    return {"value": round(W * H / dt / 1e6, 4), "unit": "Mpix/s", "cores": 1, "kind": "port",
-            "sample": f"one frame ({sc.count} Gaussians, {W}x{H}): iso_norm, unshadowed shade_emit, project, "
-                      f"stable order, tile lists and the fp16-target back-to-front blend in the C restatement "
-                      f"(opacity maps not included), {dt:.1f} s"}
+    oracle.tile_lists(sp, order, vis, W, H)
+    t_frame = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    oracle.composite(fb, sp, order, vis, W, H, mode=2, row_step=k)
+    t_blend = time.perf_counter() - t0
+    est = k * (t_light - t_resolve) + t_resolve + t_frame + k * t_blend
+    return {"value": round(W * H / est / 1e6, 4), "unit": "Mpix/s", "cores": 1, "kind": "port",
+            "sample": f"one relit frame ({sc.count} Gaussians, {W}x{H}, emitter map + {2048}^2 sun maps) on one "
+                      f"core: per-Gaussian relighting of every {k}th Gaussian ({t_light:.1f} s, x{k}), map resolves "
+                      f"({t_resolve:.1f} s), project/order/tile lists of the whole frame ({t_frame:.1f} s), the "
+                      f"fp16-target blend of every {k}th pixel row ({t_blend:.1f} s, x{k}); estimated frame "
+                      f"{est:.1f} s"}
 
 
 def main():
     a = parse()
+    if a.copy_mib:
+        copy_calibration(a.copy_mib)
+        return
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (this build has no multi-GPU path)",
               file=sys.stderr)

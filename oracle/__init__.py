@@ -44,9 +44,13 @@ def lib():
         L.or_tile_lists.argtypes = [vp, vp, u32, u32, u32, u32, vp, vp]
         L.or_composite.restype = None
         L.or_composite.argtypes = [vp, vp, vp, u32, u32, u32, C.c_int, vp]
+        L.or_composite_rows.restype = None
+        L.or_composite_rows.argtypes = [vp, vp, vp, u32, u32, u32, C.c_int, u32, vp]
         L.or_composite_f64.restype = None
         L.or_composite_f64.argtypes = [vp, vp, vp, u32, u32, u32, vp]
         f32 = C.c_float
+        L.or_exp_map_n.restype = None
+        L.or_exp_map_n.argtypes = [vp, vp, u32]
         L.or_erfc_pos.restype = f32
         L.or_erfc_pos.argtypes = [f32]
         L.or_ein.restype = f32
@@ -125,12 +129,13 @@ def tile_lists(splats: np.ndarray, order: np.ndarray, visible: int, W: int, H: i
     return start, vals[:P]
 
 
-def composite(frame: bytes, splats, order, visible, W, H, mode=0):
-    """mode 0: float32 back-to-front; 2: float16 target after every blend. Returns [H, W, 4] float32."""
+def composite(frame: bytes, splats, order, visible, W, H, mode=0, row_step=1):
+    """mode 0: float32 back-to-front; 2: float16 target after every blend. Returns [H, W, 4] float32.
+    row_step > 1 blends only the rows py % row_step == 0 (a timing sample)."""
     out = np.zeros((H, W, 4), np.float32)
     fb = np.frombuffer(frame, np.uint8).copy()
-    lib().or_composite(_p(fb), _p(np.ascontiguousarray(splats, np.uint32)), _p(np.ascontiguousarray(order, np.uint32)),
-                       visible, W, H, mode, _p(out))
+    lib().or_composite_rows(_p(fb), _p(np.ascontiguousarray(splats, np.uint32)),
+                            _p(np.ascontiguousarray(order, np.uint32)), visible, W, H, mode, row_step, _p(out))
     return out
 
 
@@ -146,6 +151,14 @@ def composite_f64(frame: bytes, splats64, order, visible, W, H):
 
 def _b(x):
     return np.frombuffer(x, np.uint8).copy() if isinstance(x, (bytes, bytearray)) else np.ascontiguousarray(x)
+
+
+def exp_map(x):
+    """The opacity-map splats' e^x (float32 in, float32 out)."""
+    xs = np.ascontiguousarray(x, np.float32).ravel()
+    y = np.zeros_like(xs)
+    lib().or_exp_map_n(_p(xs), _p(y), xs.size)
+    return y
 
 
 def erfc_pos(z):
@@ -210,6 +223,22 @@ def sun(U, SM, desc, gaussians, nodes, parts):
     L.or_sun_resolve(_p(_b(SM)), _p(_b(desc)), _p(acc), _p(mip))
     L.or_sun_lookup(_p(_b(U)), _p(_b(SM)), _p(_b(desc)), _p(layers), _p(_b(gaussians)), _p(_b(parts)), _p(mip), _p(T))
     return acc, mip, T
+
+
+def lsm_resolve(LS, acc):
+    """The emitter map's resolve and mips alone (per texel), from fixed-point sums."""
+    w = np.frombuffer(LS, np.uint32)
+    mip = np.zeros((int(w[19]), int(w[11]), 4), np.float32)
+    lib().or_lsm_resolve(_p(_b(LS)), _p(np.ascontiguousarray(acc, np.uint32)), _p(mip))
+    return mip
+
+
+def sun_resolve(SM, desc, acc):
+    """The sun maps' resolve and mips alone (per texel), from fixed-point sums."""
+    w = np.frombuffer(SM, np.uint32)
+    mip = np.zeros((int(w[0]), int(w[3]), 4), np.float32)
+    lib().or_sun_resolve(_p(_b(SM)), _p(_b(desc)), _p(np.ascontiguousarray(acc, np.uint32)), _p(mip))
+    return mip
 
 
 def env_direct(U, ENV, gaussians, nodes, vgrid, parts, materials, iso_norm, env_T, colour):

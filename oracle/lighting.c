@@ -31,7 +31,28 @@ static v3 norm3(v3 a) { float l = len3(a); return V3(a.x / l, a.y / l, a.z / l);
 static v3 cross3(v3 a, v3 b) { return V3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
 static float clampf_(float v, float lo, float hi) { return v < lo ? lo : (v > hi ? hi : v); }
 static float mixf(float a, float b, float t) { return a * (1.0f - t) + b * t; }
-static float exp_cr(float x) { return (float)exp((double)x); }
+/* e^x for the opacity-map splats (as the device defines it): Cody-Waite reduction by ln 2 split in two, fused,
+   and the degree-7 Taylor polynomial on [-ln2/2, ln2/2] in Horner form with explicit fmaf; x clamped to
+   [-86, 88]. Every operation is a correctly rounded IEEE one, so host and device agree bit for bit. */
+static float exp_map(float x) {
+  if (!(x == x)) return x;
+  x = fminf(fmaxf(x, -86.0f), 88.0f);
+  const float k = rintf(x * 1.44269504f);
+  float r = fmaf(k, -0.693145751953125f, x);
+  r = fmaf(k, -1.42860677e-6f, r);
+  float p = 1.98412698e-4f;
+  p = fmaf(p, r, 1.38888889e-3f);
+  p = fmaf(p, r, 8.33333333e-3f);
+  p = fmaf(p, r, 4.16666667e-2f);
+  p = fmaf(p, r, 1.66666667e-1f);
+  p = fmaf(p, r, 0.5f);
+  p = fmaf(p, r, 1.0f);
+  p = fmaf(p, r, 1.0f);
+  return ldexpf(p, (int)k);
+}
+void or_exp_map_n(const float *x, float *y, uint32_t n) {
+  for (uint32_t i = 0; i < n; i++) y[i] = exp_map(x[i]);
+}
 /* picks of a cell, mip level or cull decision: double, rounded once (as the device does) */
 static float log2_cr(float x) { return (float)log2((double)x); }
 static float asin_cr(float x) { return (float)asin((double)x); }
@@ -433,7 +454,7 @@ void or_lsm_splat(const void *U, const void *LS, const void *gaussians, const vo
         v2 dt = V2(((float)x + 0.5f) - ct.x, ((float)y + 0.5f) - ct.y);
         float q = dot2(dt, m2v(Si, dt));
         if (q > 9.0f) continue;
-        float t = tau0 * exp_cr(-0.5f * q);
+        float t = tau0 * exp_map(-0.5f * q);
         t = t < 64.0f ? t : 64.0f;
         float t_fix = t * FIX;
         if (t_fix < 0.5f) continue;
@@ -593,7 +614,7 @@ void or_sun_splat(const void *U, const void *SM, const void *desc, const float *
           v2 dt = V2(((float)xx + 0.5f) - ct.x, ((float)y + 0.5f) - ct.y);
           float q = dot2(dt, m2v(Si, dt));
           if (q > 9.0f) continue;
-          float t = tau0 * exp_cr(-0.5f * q);
+          float t = tau0 * exp_map(-0.5f * q);
           t = t < 64.0f ? t : 64.0f;
           float t_fix = t * FIX;
           if (t_fix < 0.5f) continue;

@@ -226,6 +226,12 @@ static float to_half_rtne(float x) {
 
 void or_composite(const void *frame, const uint32_t *splats, const uint32_t *order, uint32_t visible,
                   uint32_t W, uint32_t H, int mode, float *out) {
+  or_composite_rows(frame, splats, order, visible, W, H, mode, 1u, out);
+}
+
+/* The same blend on the rows py % row_step == 0 only (a bounded sample for timing); other rows stay 0. */
+void or_composite_rows(const void *frame, const uint32_t *splats, const uint32_t *order, uint32_t visible,
+                       uint32_t W, uint32_t H, int mode, uint32_t row_step, float *out) {
   const float ALPHA_MIN = 1.0f / 255.0f;
   uint32_t view_mode = uw(frame, FR_VIEW_MODE);
   float near_ = fw(frame, FR_NEAR), far_ = fw(frame, FR_FAR);
@@ -248,7 +254,7 @@ void or_composite(const void *frame, const uint32_t *splats, const uint32_t *ord
     if (!or_tile_box(s, W, H, 1, box)) continue;
     float det = a0x * a1y - a1x * a0y;
     if (det == 0.0f) continue;
-    for (uint32_t py = box[1]; py <= box[3]; py++) {
+    for (uint32_t py = (box[1] + row_step - 1u) / row_step * row_step; py <= box[3]; py += row_step) {
       float ny = 1.0f - ((float)py + 0.5f) * 2.0f / (float)H;
       for (uint32_t px = box[0]; px <= box[2]; px++) {
         float nx = ((float)px + 0.5f) * 2.0f / (float)W - 1.0f;

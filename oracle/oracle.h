@@ -48,6 +48,8 @@ uint64_t or_tile_lists(const uint32_t *splats, const uint32_t *order, uint32_t v
  * out: W*H*4 floats. */
 void or_composite(const void *frame, const uint32_t *splats, const uint32_t *order, uint32_t visible,
                   uint32_t W, uint32_t H, int mode, float *out);
+void or_composite_rows(const void *frame, const uint32_t *splats, const uint32_t *order, uint32_t visible,
+                       uint32_t W, uint32_t H, int mode, uint32_t row_step, float *out);
 
 /* The same blend in float64 over or_project_f64's splats (out: W*H*4 doubles). */
 void or_composite_f64(const void *frame, const double *splats, const uint32_t *order, uint32_t visible,
@@ -55,6 +57,7 @@ void or_composite_f64(const void *frame, const double *splats, const uint32_t *o
 
 /* ---- relighting (lighting.c): float32 restatements of runtime/wgsl for the default M39 matte path ---------- */
 
+void or_exp_map_n(const float *x, float *y, uint32_t n);                        /* the map splats' e^x */
 float or_erfc_pos(float z);                                                      /* common.wgsl:113-118 */
 float or_ein(float x);                                                           /* common.wgsl:226-239 */
 float or_footprint(const void *node, float detC, float wx, float wy, float wz);  /* common.wgsl:242-246 */

@@ -3,9 +3,9 @@
 // static M39 matte path).
 //
 // One lane per Gaussian everywhere except the map resolve/down passes (one lane per texel). Contraction is off in
-// this file so every float expression rounds as written in WGSL order; the map splats take exp as the correctly
-// rounded float exp, so their fixed-point sums (integer atomics, order-independent) are bit-identical to any
-// implementation using the same definition.
+// this file so every float expression rounds as written in WGSL order; the map splats take exp from exp_map (fused
+// operations only), so their fixed-point sums (order-independent u32 adds) are bit-identical to any implementation
+// using the same definition.
 #pragma clang fp contract(off)
 #include "sl_device.h"
 #include <hip/hip_fp16.h>
@@ -34,16 +34,33 @@ __device__ __forceinline__ v3 cross3(v3 a, v3 b) {
 }
 __device__ __forceinline__ float clampf_(float v, float lo, float hi) { return v < lo ? lo : (v > hi ? hi : v); }
 __device__ __forceinline__ float mixf(float a, float b, float t) { return a * (1.0f - t) + b * t; }
-__device__ __forceinline__ float exp_cr(float x) { return (float)exp((double)x); }
 // transcendentals whose result picks a cell, a mip level or a cull decision: double, rounded once, so the
 // choice does not hang on a library's last-ulp behaviour
 __device__ __forceinline__ float log2_cr(float x) { return (float)log2((double)x); }
 __device__ __forceinline__ float asin_cr(float x) { return (float)asin((double)x); }
 __device__ __forceinline__ float acos_cr(float x) { return (float)acos((double)x); }
-__device__ __forceinline__ uint32_t u32_sat(float f) {   // WGSL u32(f32): truncate, saturate
-  if (!(f > 0.0f)) return 0u;
-  if (f >= 4294967296.0f) return 0xFFFFFFFFu;
-  return (uint32_t)f;
+__device__ __forceinline__ uint32_t u32_sat(float f) {   // WGSL u32(f32): truncate, saturate; NaN -> 0
+  const float c = fminf(fmaxf(f, 0.0f), 4294967040.0f);   // the largest float below 2^32
+  return f >= 4294967296.0f ? 0xFFFFFFFFu : (uint32_t)c;
+}
+// e^x for the opacity-map splats: Cody-Waite reduction by ln 2 (split in two, fused) and the degree-7 Taylor
+// polynomial on [-ln2/2, ln2/2] in Horner form with explicit fused multiply-adds, so every IEEE operation is
+// the same on any host or device that has fma; within 2 ulp of e^x. x is clamped to [-86, 88] (normal results).
+__device__ __forceinline__ float exp_map(float x) {
+  if (!(x == x)) return x;
+  x = fminf(fmaxf(x, -86.0f), 88.0f);
+  const float k = rintf(x * 1.44269504f);
+  float r = fmaf(k, -0.693145751953125f, x);
+  r = fmaf(k, -1.42860677e-6f, r);
+  float p = 1.98412698e-4f;                  // 1/7!
+  p = fmaf(p, r, 1.38888889e-3f);            // 1/6!
+  p = fmaf(p, r, 8.33333333e-3f);            // 1/5!
+  p = fmaf(p, r, 4.16666667e-2f);            // 1/4!
+  p = fmaf(p, r, 1.66666667e-1f);            // 1/3!
+  p = fmaf(p, r, 0.5f);
+  p = fmaf(p, r, 1.0f);
+  p = fmaf(p, r, 1.0f);
+  return ldexpf(p, (int)k);
 }
 __device__ __forceinline__ v3 m3v(m3 m, v3 v) { return add3(add3(mul3s(m.c0, v.x), mul3s(m.c1, v.y)), mul3s(m.c2, v.z)); }
 __device__ __forceinline__ m3 m3mul(m3 a, m3 b) { return {m3v(a, b.c0), m3v(a, b.c1), m3v(a, b.c2)}; }
@@ -453,24 +470,29 @@ sl_map_fill(uint32_t n_rec, const MapRec *__restrict__ recs, uint32_t tiles_x, u
     }
 }
 
-// Per-tile sums of the records list[s0, s1) of tile `tile`, one lane per texel of the 16x16 tile.
+// Per-tile sums of the records list[s0, s1) of one tile, one lane per texel of the 16x16 tile. Records are staged
+// 256 at a time in LDS with one coalesced load per lane; each wave reads them back as broadcasts, one record ahead
+// of the one it evaluates.
 __device__ __forceinline__ void map_tile_sums(const MapRec *__restrict__ recs, const uint32_t *__restrict__ list,
                                               uint32_t s0, uint32_t s1, int x, int y, int wx0, int wy0, float fix,
                                               MapRec *s_rec, uint32_t &a0, uint32_t &a1, uint32_t &a2) {
+  const float px = (float)x + 0.5f, py = (float)y + 0.5f;
   for (uint32_t b = s0; b < s1; b += 256u) {
     const uint32_t nb = min(256u, s1 - b);
     __syncthreads();
     if (threadIdx.x < nb) s_rec[threadIdx.x] = recs[list[b + threadIdx.x]];
     __syncthreads();
+    MapRec nx = s_rec[0];
     for (uint32_t j = 0; j < nb; j++) {
-      const MapRec r = s_rec[j];
+      const MapRec r = nx;
+      if (j + 1 < nb) nx = s_rec[j + 1];
       if (r.x1 < wx0 || r.x0 > wx0 + 15 || r.y1 < wy0 || r.y0 > wy0 + 3) continue;   // the wave's 16x4 texels
       if (x < r.x0 || x > r.x1 || y < r.y0 || y > r.y1) continue;
-      const v2 dt = V2(((float)x + 0.5f) - r.ctx, ((float)y + 0.5f) - r.cty);
+      const v2 dt = V2(px - r.ctx, py - r.cty);
       const m2 Si = {V2(r.s00, r.s01), V2(r.s10, r.s11)};
       const float q = dot2(dt, m2v(Si, dt));
       if (q > 9.0f) continue;
-      float t = r.tau0 * exp_cr(-0.5f * q);
+      float t = r.tau0 * exp_map(-0.5f * q);
       t = t < 64.0f ? t : 64.0f;
       const float t_fix = t * fix;
       if (t_fix < 0.5f) continue;

@@ -235,3 +235,19 @@ def test_sun_map_single_occluder_and_env_direct():
     want0 = g["albedo"] * iso[0] * 2.0 * envT[0, 0] * 1.0 * A
     np.testing.assert_allclose(col[0, :3], want0, rtol=1e-5)
     assert np.all(col[centre, :3] < 1e-3 * want0)
+
+
+def test_exp_map_within_two_ulp():
+    """The map splats' e^x against float64 exp rounded to float32: <= 2 ulp over the clamp range, exact at 0,
+    monotone on the splat range [-4.5, 0], NaN kept."""
+    rng = np.random.default_rng(7)
+    x = np.concatenate([rng.uniform(-4.5, 0.0, 400_000), rng.uniform(-86.0, 88.0, 200_000),
+                        np.linspace(-4.5, 0.0, 100_001)]).astype(np.float32)
+    y = oracle.exp_map(x)
+    ref = np.exp(x.astype(np.float64)).astype(np.float32)
+    ulp = np.abs(y.view(np.int32).astype(np.int64) - ref.view(np.int32).astype(np.int64))
+    assert ulp.max() <= 2, (ulp.max(), x[np.argmax(ulp)])
+    assert oracle.exp_map(np.zeros(1))[0] == 1.0
+    grid = np.sort(x[:400_000])
+    assert np.all(np.diff(oracle.exp_map(grid)) >= 0)
+    assert np.isnan(oracle.exp_map(np.array([np.nan]))[0])
