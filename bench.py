@@ -36,6 +36,11 @@ def parse():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--half", type=int, default=1, help="1: rgba16f target (the reference's), 0: rgba32f")
     ap.add_argument("--cpu-baseline", type=int, default=1)
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_hill1m_1080p.txt"),
+                    help="rocprof traffic summary written by --pmc-summary; its figure fills roofline.traffic")
+    ap.add_argument("--pmc-summary", nargs=4, metavar=("FETCH_DIR", "WRITE_DIR", "COPY_FETCH_DIR", "COPY_WRITE_DIR"),
+                    help="summarise rocprofv3 --pmc output directories into --traffic and exit")
+    ap.add_argument("--pmc-command", default="", help="the profiling command, recorded in the summary")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB, 5 times (HBM counter calibration)")
     return ap.parse_args()
@@ -105,8 +110,103 @@ def cpu_baseline(sc, fb, k=8):
                       f"{est:.1f} s"}
 
 
+def kernel_bytes(passes, P, T, target_bytes, ls):
+    """Per-launch algorithmic HBM bytes and HIP-event milliseconds of the two heaviest kernels of a relit frame.
+
+    sl_composite (one launch): a 48 B splat fetch and a 4 B pair value per pair, 8 B per tile range, the target.
+    sl_map_accumulate (two launches, emitter map then sun maps): a 64 B record and a 4 B list entry per
+    (record, map tile) pair, 4 B per tile start, 12 B per texel of every map tile that has records.
+    """
+    comp = {"kernel": "sl_composite", "launches": 1, "ms": float(np.mean(passes["splat"])),
+            "bytes": 48 * P + 4 * P + 8 * T + target_bytes}
+    if "lsm_sum" not in passes and "sun_sum" not in passes:
+        return comp, None
+    b, ms, n = 0, 0.0, 0
+    for m, key in (("lsm", "lsm_sum"), ("sun", "sun_sum")):
+        if key in passes:
+            b += 68 * ls[f"{m}_pairs"] + 4 * (ls[f"{m}_tiles"] + 1) + 256 * 12 * ls[f"{m}_tiles_used"]
+            ms += float(np.mean(passes[key]))
+            n += 1
+    return comp, {"kernel": "sl_map_accumulate", "launches": n, "ms": ms / n, "bytes": b // n}
+
+
+def roofline(passes, P, T, target_bytes, ls, traffic_file):
+    """The roofline object of the kernel that takes the most time per frame (HIP events on the library's stream)."""
+    comp, msum = kernel_bytes(passes, P, T, target_bytes, ls)
+    k = comp if msum is None or comp["ms"] * comp["launches"] >= msum["ms"] * msum["launches"] else msum
+    achieved = k["bytes"] / (k["ms"] * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": k["kernel"], "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(traffic_file,
+                                                                                                 k["kernel"]),
+            "algorithmic_bytes": int(k["bytes"]), "avg_ms": round(k["ms"], 5), "launches_per_frame": k["launches"]}
+
+
+def measured_traffic(path, kernel):
+    """HBM bytes per launch of `kernel` from a rocprof FETCH_SIZE / WRITE_SIZE summary (see --pmc-summary), or
+    None when no summary for it is at hand."""
+    if not path or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("TRAFFIC "):
+                d = json.loads(line[len("TRAFFIC "):])
+                if d.get("kernel") == kernel:
+                    return d["bytes_per_launch"]
+    return None
+
+
+def pmc_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, out, command):
+    """Per-kernel HBM bytes from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs (KiB per dispatch), each scaled by
+    what the counter reads for a plain copy kernel of known size; writes a short text summary."""
+    import csv
+    from collections import defaultdict
+
+    def per_kernel(d, counter):
+        import glob
+        vals = defaultdict(list)
+        for p in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            for r in csv.DictReader(open(p)):
+                if r["Counter_Name"] == counter:
+                    vals[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+        return vals
+
+    def copy_scale(d, counter, nbytes):
+        v = per_kernel(d, counter)
+        k = max(v, key=lambda name: np.mean(v[name]))
+        return nbytes / (np.mean(v[k]) * 1024.0), k, float(np.mean(v[k]))
+
+    copy_bytes = 1024 << 20
+    fs, fk, fv = copy_scale(copy_fetch_dir, "FETCH_SIZE", copy_bytes)
+    ws, wk, wv = copy_scale(copy_write_dir, "WRITE_SIZE", copy_bytes)
+    fetch, write = per_kernel(fetch_dir, "FETCH_SIZE"), per_kernel(write_dir, "WRITE_SIZE")
+    lines = ["# HBM traffic per launch, rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE), hill 1M Gaussians 1920x1080", "",
+             "Command (each counter in its own run):", "", "    " + command, "",
+             f"Calibration: a 1 GiB device copy ({fk}) reads FETCH_SIZE {fv:.0f} KiB and WRITE_SIZE {wv:.0f} KiB per "
+             f"dispatch, so FETCH_SIZE is scaled by {fs:.3f} and WRITE_SIZE by {ws:.3f}.", "",
+             "| kernel | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB | HBM bytes per launch (scaled) |",
+             "|---|---|---|---|---|"]
+    rows = []
+    for name in sorted(set(fetch) | set(write), key=lambda n: -np.mean(fetch.get(n, [0]))):
+        f_kib, w_kib = float(np.mean(fetch.get(name, [0]))), float(np.mean(write.get(name, [0])))
+        b = f_kib * 1024 * fs + w_kib * 1024 * ws
+        rows.append((name, len(fetch.get(name, [])), f_kib, w_kib, b))
+        lines.append(f"| {name[:48]} | {len(fetch.get(name, []))} | {f_kib:.0f} | {w_kib:.0f} | {b:.4g} |")
+    lines.append("")
+    for name, _, f_kib, w_kib, b in rows:
+        if name in ("sl_composite", "sl_map_accumulate"):
+            lines.append("TRAFFIC " + json.dumps({"kernel": name, "bytes_per_launch": int(b),
+                                                  "fetch_kib": round(f_kib, 1), "write_kib": round(w_kib, 1),
+                                                  "fetch_scale": round(fs, 4), "write_scale": round(ws, 4)}))
+    with open(out, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
 def main():
     a = parse()
+    if a.pmc_summary:
+        pmc_summary(*a.pmc_summary, a.traffic, a.pmc_command)
+        return
     if a.copy_mib:
         copy_calibration(a.copy_mib)
         return
@@ -159,10 +259,9 @@ def main():
         N.frame_finish(ctx)
     view_only = (time.perf_counter() - t1) / a.steps
     P, V, T = st.pairs, st.visible, st.tiles
+    ls = scene.lighting_stats()
     mpix = W * H * a.steps / total / 1e6
-    comp = float(np.mean(comp_ms))
-    comp_bytes = 48 * P + 4 * P + 8 * T + W * H * bpx        # splat fetch + pair value per pair, ranges, target
-    achieved = comp_bytes / (comp * 1e-3) / 1e9
+    roof = roofline(passes, P, T, W * H * bpx, ls, a.traffic)
     res = {
         "metric": "Mpix/s + Gaussian-tile pairs/s, 1080p/1M Gaussians; p50/p99 latency at fixed req/s",
         "value": round(mpix, 3),
@@ -187,9 +286,10 @@ def main():
         "frame_ms_p50": round(float(np.percentile(frame_ms, 50)), 4),
         "frame_ms_p99": round(float(np.percentile(frame_ms, 99)), 4),
         "pass_ms": {k: round(float(np.mean(v)), 4) for k, v in passes.items()},
-        "roofline": {"bound": "hbm", "kernel": "sl_composite", "achieved": round(achieved, 2),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                     "traffic": None, "algorithmic_bytes": comp_bytes, "avg_ms": round(comp, 5)},
+        "roofline": roof,
+        "map_splat": {"emitter_pairs": ls["lsm_pairs"], "sun_pairs": ls["sun_pairs"],
+                      "sum_ms": {"emitter": round(float(np.mean(passes.get("lsm_sum", [0]))), 4),
+                                 "sun": round(float(np.mean(passes.get("sun_sum", [0]))), 4)}},
     }
     ctx.free(out)
     scene.destroy()

@@ -141,8 +141,18 @@ sl_status sl_lighting_encode(sl_context *ctx, sl_scene *scene, const sl_lighting
 #define SL_SCENE_SUN_MIP 8u    /* vec4 [layers][layer texels] */
 sl_status sl_scene_read(sl_context *ctx, sl_scene *scene, uint32_t which, void *host, size_t bytes, size_t *written);
 
+/* Sizes of the last relighting's opacity-map splats (binned records and pairs), for throughput figures. No
+ * counterpart in the reference (its splats are one atomic loop per Gaussian). */
+typedef struct sl_lighting_stats {
+  uint32_t lsm_records, lsm_pairs, lsm_tiles, lsm_tiles_used;   /* emitter map: record slots, (record, tile) */
+  uint32_t sun_records, sun_pairs, sun_tiles, sun_tiles_used;   /* sun maps; tiles: all, with records */
+} sl_lighting_stats;
+sl_status sl_lighting_stats_read(sl_context *ctx, sl_scene *scene, sl_lighting_stats *out);
+
 /* Per-pass GPU milliseconds of the last frame encoded with SL_FRAME_TIMERS; names as in the reference's PASSES
- * (src/renderer.js:7-8) where one exists. Fills up to `cap` entries; returns the count in *n. */
+ * (src/renderer.js:7-8) where one exists, then, after sl_lighting_encode with SL_LIGHT_TIMERS, its passes
+ * "shadow", "shade", "sunmap", "env_direct" and the map-sum kernels alone, "lsm_sum" and "sun_sum". Fills up to
+ * `cap` entries; returns the count in *n. */
 sl_status sl_timers_read(sl_context *ctx, const char **names, float *ms, uint32_t cap, uint32_t *n);
 
 #ifdef __cplusplus

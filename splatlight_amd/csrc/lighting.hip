@@ -288,12 +288,15 @@ sl_shade_emit(FrameUniformsDev U, EmitParamsDev EP, const float4 *__restrict__ e
   const uint8_t *vg = vgrid + 128 * (size_t)g;
   const v3 en = V3(EP.normal[0], EP.normal[1], EP.normal[2]);
   v3 acc = V3(0.0f, 0.0f, 0.0f);
-  uint32_t stack[48];
-  stack[0] = EP.root;
+  // the cut walk's stack (48 entries, as the reference's) in LDS, one column per lane, node indices as u16
+  // (the host caps the tree at 65535 nodes; a child index beyond that becomes 0xFFFF, which the walk skips)
+  __shared__ uint16_t s_stack[48][64];
+  uint16_t *stack = &s_stack[0][threadIdx.x];
+  stack[0] = (uint16_t)min(EP.root, 0xFFFFu);
   uint32_t sp = 1;
   while (sp > 0) {
     sp -= 1;
-    const uint32_t ni = stack[sp];
+    const uint32_t ni = stack[64 * sp];
     if (ni >= n_nodes) continue;                      // a malformed tree: the node does not exist
     const float4 *e = emit_nodes + 6 * (size_t)ni;
     const float4 e0 = e[0], e1 = e[1], e2 = e[2], e3 = e[3];
@@ -306,7 +309,11 @@ sl_shade_emit(FrameUniformsDev U, EmitParamsDev EP, const float4 *__restrict__ e
       use = dot3(d, d) > EP.theta * EP.theta * e0.w * e0.w;
     }
     if (!use) {
-      if (sp + 2u <= 48u) { stack[sp] = clo; stack[sp + 1] = __float_as_uint(e3.w); sp += 2; }
+      if (sp + 2u <= 48u) {
+        stack[64 * sp] = (uint16_t)min(clo, 0xFFFFu);
+        stack[64 * (sp + 1)] = (uint16_t)min(__float_as_uint(e3.w), 0xFFFFu);
+        sp += 2;
+      }
       continue;
     }
     const v3 d = sub3(V3(e0.x, e0.y, e0.z), P);
@@ -419,8 +426,10 @@ __device__ __forceinline__ void count_tiles(const MapRec &r, uint32_t tiles_x, u
 
 // tile_start[i] = sum of tile_count[0, i), over n tiles in blocks of 4096; block_sums[b] = the sum of block b.
 extern "C" __global__ void __launch_bounds__(256)
-sl_map_reduce(const uint32_t *__restrict__ tile_count, uint32_t n, uint32_t *__restrict__ block_sums) {
+sl_map_reduce(const uint32_t *__restrict__ tile_count, uint32_t n, uint32_t *__restrict__ block_sums,
+              uint32_t *__restrict__ stats) {
   __shared__ uint32_t s_tmp[RS_WAVES];
+  if (blockIdx.x == 0 && threadIdx.x == 0) stats[1] = 0u;   // tiles with records, counted by sl_map_scan
   const uint32_t b0 = blockIdx.x * 4096u + threadIdx.x * 16u;
   uint32_t v = 0;
   for (uint32_t k = 0; k < 16u; k++) v += b0 + k < n ? tile_count[b0 + k] : 0u;
@@ -431,25 +440,31 @@ sl_map_reduce(const uint32_t *__restrict__ tile_count, uint32_t n, uint32_t *__r
 
 extern "C" __global__ void __launch_bounds__(256)
 sl_map_scan(const uint32_t *__restrict__ tile_count, uint32_t n, const uint32_t *__restrict__ block_sums,
-            uint32_t *__restrict__ tile_start) {
+            uint32_t *__restrict__ tile_start, uint32_t *__restrict__ stats) {
   __shared__ uint32_t s_tmp[RS_WAVES];
   uint32_t before = 0;
   for (uint32_t b = threadIdx.x; b < blockIdx.x; b += 256u) before += block_sums[b];
   uint32_t base;
   block_excl_scan256(before, s_tmp, &base);   // base = sum over the block of 'before' = all earlier blocks
   const uint32_t b0 = blockIdx.x * 4096u + threadIdx.x * 16u;
-  uint32_t c[16], v = 0;
+  uint32_t c[16], v = 0, used = 0;
   for (uint32_t k = 0; k < 16u; k++) {
     c[k] = b0 + k < n ? tile_count[b0 + k] : 0u;
     v += c[k];
+    used += c[k] != 0u;
   }
-  uint32_t total;
+  uint32_t total, used_total;
   uint32_t run = base + block_excl_scan256(v, s_tmp, &total);
+  block_excl_scan256(used, s_tmp, &used_total);
   for (uint32_t k = 0; k < 16u; k++) {
     if (b0 + k < n) tile_start[b0 + k] = run;
     run += c[k];
   }
-  if (blockIdx.x == gridDim.x - 1u && threadIdx.x == 0) tile_start[n] = base + total;
+  if (threadIdx.x == 0 && used_total) atomicAdd(stats + 1, used_total);
+  if (blockIdx.x == gridDim.x - 1u && threadIdx.x == 0) {
+    tile_start[n] = base + total;
+    stats[0] = base + total;
+  }
 }
 
 // Each record's slot into the lists of the tiles its box touches; tile_count returns to zero.

@@ -81,6 +81,8 @@ struct sl_scene {
   Grow emit_nodes, lsm_acc, lsm_mip, lsm_dirty, sun_acc, sun_mip, sun_desc, sun_layers, sun_dirty;
   // opacity-map splat binning, shared by both maps: records, per-tile counts (zero between uses), starts, lists
   Grow map_rec, map_cnt, map_start, map_bsum, map_list;
+  Grow map_stats;                 // per map (emitter, sun): pairs, tiles with records
+  uint32_t map_records[2] = {0, 0}, map_tiles[2] = {0, 0};
   uint32_t lsm_parts = 0, lsm_res = 0, lsm_layer_texels = 0;
   uint32_t sun_n = 0, sun_res = 0, sun_layer_texels = 0;
 };
@@ -115,6 +117,8 @@ struct sl_context {
   FrameCounters *counters_host = nullptr;   // pinned
   hipEvent_t ev[kPasses + 1] = {};
   hipEvent_t lev[5] = {};         // lighting: shadow, shade, sun, env
+  hipEvent_t mev[4] = {};         // the two map-sum launches (emitter map, sun maps): begin, end
+  bool map_timed[2] = {false, false};
   bool timers_valid = false;
   bool light_timers_valid = false;
   float ms[kPasses] = {};
@@ -414,6 +418,8 @@ extern "C" sl_status sl_ctx_create(int device_ordinal, sl_context **out) {
     if ((e = hipEventCreate(&ctx->ev[k])) != hipSuccess) return bail(e);
   for (int k = 0; k < 5; k++)
     if ((e = hipEventCreate(&ctx->lev[k])) != hipSuccess) return bail(e);
+  for (int k = 0; k < 4; k++)
+    if ((e = hipEventCreate(&ctx->mev[k])) != hipSuccess) return bail(e);
   *out = ctx;
   return SL_OK;
 }
@@ -427,6 +433,8 @@ extern "C" void sl_ctx_destroy(sl_context *ctx) {
     if (ctx->ev[k]) (void)hipEventDestroy(ctx->ev[k]);
   for (int k = 0; k < 5; k++)
     if (ctx->lev[k]) (void)hipEventDestroy(ctx->lev[k]);
+  for (int k = 0; k < 4; k++)
+    if (ctx->mev[k]) (void)hipEventDestroy(ctx->mev[k]);
   if (ctx->counters_host) (void)hipHostFree(ctx->counters_host);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -514,7 +522,7 @@ void scene_free(sl_scene *sc) {
   for (void *p : sc->allocs) (void)hipFree(p);
   for (Grow *gb : {&sc->emit_nodes, &sc->lsm_acc, &sc->lsm_mip, &sc->lsm_dirty, &sc->sun_acc, &sc->sun_mip,
                    &sc->sun_desc, &sc->sun_layers, &sc->sun_dirty, &sc->map_rec, &sc->map_cnt, &sc->map_start,
-                   &sc->map_bsum, &sc->map_list})
+                   &sc->map_bsum, &sc->map_list, &sc->map_stats})
     if (gb->p) (void)hipFree(gb->p);
   delete sc;
 }
@@ -803,25 +811,36 @@ sl_status ensure_map(sl_context *ctx, sl_scene *sc, size_t n_rec, uint32_t n_til
   if ((s = grow(ctx, sc, &sc->map_start, 4 * ((size_t)n_tiles + 1))) != SL_OK) return s;
   if ((s = grow(ctx, sc, &sc->map_bsum, 4 * (((size_t)n_tiles + 4095) / 4096))) != SL_OK) return s;
   if ((s = grow(ctx, sc, &sc->map_list, 4 * std::max<size_t>(n_rec * span * span, 1))) != SL_OK) return s;
+  if (!sc->map_stats.p) {
+    if ((s = grow(ctx, sc, &sc->map_stats, 16)) != SL_OK) return s;
+    sl_fill_u32<<<1, 256, 0, ctx->stream>>>((uint32_t *)sc->map_stats.p, 0u, 4u);
+  }
   if (fresh_cnt)
     sl_fill_u32<<<grid_for(n_tiles, 256, 4096), 256, 0, ctx->stream>>>((uint32_t *)sc->map_cnt.p, 0u, n_tiles);
   return SL_OK;
 }
 
 // The records written by a prep pass -> per-tile lists -> per-tile sums into acc, whose dirty layers are zero.
-void run_map_splat(sl_context *ctx, sl_scene *sc, size_t n_rec, uint32_t n_layers, uint32_t res, float fix,
-                   uint32_t *acc) {
+// which: 0 emitter map, 1 sun maps (stats slot and timer events).
+void run_map_splat(sl_context *ctx, sl_scene *sc, uint32_t which, size_t n_rec, uint32_t n_layers, uint32_t res,
+                   float fix, uint32_t *acc, bool timers) {
   hipStream_t st = ctx->stream;
   const uint32_t tiles_x = (res + MT - 1) / MT, tpl = tiles_x * tiles_x, n_tiles = n_layers * tpl;
   const uint32_t nb = (n_tiles + 4095) / 4096;
   uint32_t *cnt = (uint32_t *)sc->map_cnt.p, *start = (uint32_t *)sc->map_start.p;
-  sl_map_reduce<<<nb, 256, 0, st>>>(cnt, n_tiles, (uint32_t *)sc->map_bsum.p);
-  sl_map_scan<<<nb, 256, 0, st>>>(cnt, n_tiles, (const uint32_t *)sc->map_bsum.p, start);
+  uint32_t *stats = (uint32_t *)sc->map_stats.p + 2 * which;
+  sl_map_reduce<<<nb, 256, 0, st>>>(cnt, n_tiles, (uint32_t *)sc->map_bsum.p, stats);
+  sl_map_scan<<<nb, 256, 0, st>>>(cnt, n_tiles, (const uint32_t *)sc->map_bsum.p, start, stats);
   if (n_rec)
     sl_map_fill<<<(uint32_t)((n_rec + 255) / 256), 256, 0, st>>>((uint32_t)n_rec, (const MapRec *)sc->map_rec.p,
                                                                   tiles_x, tpl, start, cnt, (uint32_t *)sc->map_list.p);
+  if (timers) (void)hipEventRecord(ctx->mev[2 * which], st);
   sl_map_accumulate<<<ctx->cus * 8, 256, 0, st>>>((const MapRec *)sc->map_rec.p, (const uint32_t *)sc->map_list.p,
                                                   start, n_tiles, res, tiles_x, tpl, fix, acc);
+  if (timers) (void)hipEventRecord(ctx->mev[2 * which + 1], st);
+  ctx->map_timed[which] = timers;
+  sc->map_records[which] = (uint32_t)n_rec;
+  sc->map_tiles[which] = n_tiles;
 }
 
 // tiles a box of at most 2 * max_half + 1 texels can span per axis
@@ -868,6 +887,7 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
   if (EP.lights != 0u) return fail(ctx, SL_ERR_UNSUPPORTED, "unified lights are not on this path");
   if (d->emit_node_count == 0 || EP.root >= d->emit_node_count || (EP.count && d->emit_node_count < 2 * EP.count - 1))
     return fail(ctx, SL_ERR_INVALID, "emitter tree does not match EmitParams");
+  if (d->emit_node_count > 65535u) return fail(ctx, SL_ERR_CAPACITY, "emitter tree over 65535 nodes");
   SL_HIP(ctx, hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
   sl_status s;
@@ -877,6 +897,7 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
   const uint32_t gblocks = (sc->count + 63) / 64;
   const bool timers = (d->flags & SL_LIGHT_TIMERS) != 0;
   if (timers) SL_HIP(ctx, hipEventRecord(ctx->lev[0], st));
+  ctx->map_timed[0] = ctx->map_timed[1] = false;
   // A: the emitter light's opacity map and its lookup -> lamp_T
   const float *lampT = nullptr;
   if (d->ls_params) {
@@ -906,7 +927,7 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
       if (sc->count)
         sl_lsm_prep<<<gblocks, 64, 0, st>>>(sc->count, LS, sc->g[0], sc->g[1], sc->g[2], sc->nodes, sc->parts,
                                             (MapRec *)sc->map_rec.p, (uint32_t *)sc->map_cnt.p);
-      run_map_splat(ctx, sc, sc->count, LS.part_count, LS.res, 65536.0f, (uint32_t *)sc->lsm_acc.p);
+      run_map_splat(ctx, sc, 0, sc->count, LS.part_count, LS.res, 65536.0f, (uint32_t *)sc->lsm_acc.p, timers);
       if ((s = run_opacity_maps(ctx, sc, (float4 *)sc->lsm_mip.p, LS.part_count, LS.res, LS.levels, LS.level_off,
                                 LS.layer_texels, 65536.0f, (const uint32_t *)sc->lsm_acc.p,
                                 (const uint32_t *)sc->lsm_dirty.p)) != SL_OK)
@@ -975,7 +996,7 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
                                               (const SunLayer *)sc->sun_layers.p, sc->g[0], sc->g[1], sc->g[2],
                                               sc->nodes, sc->parts, max_layers, (MapRec *)sc->map_rec.p,
                                               (uint32_t *)sc->map_cnt.p);
-        run_map_splat(ctx, sc, n_rec, SM.n_layers, SM.res, 4096.0f, (uint32_t *)sc->sun_acc.p);
+        run_map_splat(ctx, sc, 1, n_rec, SM.n_layers, SM.res, 4096.0f, (uint32_t *)sc->sun_acc.p, timers);
         if ((s = run_opacity_maps(ctx, sc, (float4 *)sc->sun_mip.p, SM.n_layers, SM.res, SM.levels, SM.level_off,
                                   SM.layer_texels, 4096.0f, (const uint32_t *)sc->sun_acc.p,
                                   (const uint32_t *)sc->sun_dirty.p)) != SL_OK)
@@ -1027,6 +1048,26 @@ extern "C" sl_status sl_scene_read(sl_context *ctx, sl_scene *sc, uint32_t which
   return SL_OK;
 }
 
+extern "C" sl_status sl_lighting_stats_read(sl_context *ctx, sl_scene *sc, sl_lighting_stats *out) {
+  SL_LIVE(ctx);
+  if (!sc || !out) return fail(ctx, SL_ERR_INVALID, "null argument");
+  memset(out, 0, sizeof(*out));
+  uint32_t dev[4] = {0, 0, 0, 0};
+  if (sc->map_stats.p) {
+    SL_HIP(ctx, hipMemcpyAsync(dev, sc->map_stats.p, sizeof(dev), hipMemcpyDeviceToHost, ctx->stream));
+    SL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  out->lsm_records = sc->map_records[0];
+  out->lsm_pairs = dev[0];
+  out->lsm_tiles = sc->map_tiles[0];
+  out->lsm_tiles_used = dev[1];
+  out->sun_records = sc->map_records[1];
+  out->sun_pairs = dev[2];
+  out->sun_tiles = sc->map_tiles[1];
+  out->sun_tiles_used = dev[3];
+  return SL_OK;
+}
+
 extern "C" sl_status sl_timers_read(sl_context *ctx, const char **names, float *ms, uint32_t cap, uint32_t *n) {
   SL_LIVE(ctx);
   uint32_t k = 0;
@@ -1045,6 +1086,16 @@ extern "C" sl_status sl_timers_read(sl_context *ctx, const char **names, float *
       if (names) names[k] = lnames[j];
       if (ms) ms[k] = v;
     }
+    // the map-sum kernel alone (sl_map_accumulate), inside "shadow" and "sunmap"
+    static const char *const mnames[] = {"lsm_sum", "sun_sum"};
+    for (int j = 0; j < 2 && k < cap; j++)
+      if (ctx->map_timed[j]) {
+        float v = 0.0f;
+        SL_HIP(ctx, hipEventElapsedTime(&v, ctx->mev[2 * j], ctx->mev[2 * j + 1]));
+        if (names) names[k] = mnames[j];
+        if (ms) ms[k] = v;
+        k++;
+      }
   }
   if (n) *n = k;
   return SL_OK;

@@ -29,11 +29,15 @@ EXPORTS = [
     "sl_scene_create", "sl_scene_destroy", "sl_scene_write_parts", "sl_scene_write_colour",
     "sl_scene_write_bounce", "sl_scene_device_bytes",
     "sl_frame_encode", "sl_frame_finish", "sl_frame_read", "sl_timers_read",
-    "sl_lighting_iso_init", "sl_lighting_encode", "sl_scene_read",
+    "sl_lighting_iso_init", "sl_lighting_encode", "sl_scene_read", "sl_lighting_stats_read",
 ]
 SCENE_COLOUR, SCENE_LAMP_T, SCENE_ENV_T, SCENE_ISO_NORM = 1, 2, 3, 4
 SCENE_LSM_ACC, SCENE_LSM_MIP, SCENE_SUN_ACC, SCENE_SUN_MIP = 5, 6, 7, 8
 SL_LIGHT_TIMERS = 1
+
+
+class LightingStats(C.Structure):
+    _fields_ = [(f"{m}_{k}", C.c_uint32) for m in ("lsm", "sun") for k in ("records", "pairs", "tiles", "tiles_used")]
 
 
 class SceneDesc(C.Structure):
@@ -109,6 +113,7 @@ def lib() -> C.CDLL:
         "sl_lighting_iso_init": (i32, [vp, vp]),
         "sl_lighting_encode": (i32, [vp, vp, C.POINTER(LightingDesc)]),
         "sl_scene_read": (i32, [vp, vp, u32, vp, sz, C.POINTER(sz)]),
+        "sl_lighting_stats_read": (i32, [vp, vp, C.POINTER(LightingStats)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -259,6 +264,12 @@ class Scene:
         d.flags = SL_LIGHT_TIMERS if timers else 0
         self.ctx.check(self.ctx.L.sl_lighting_encode(self.ctx.h, self.h, C.byref(d)))
 
+    def lighting_stats(self) -> dict:
+        """Records and (record, tile) pairs of the last relighting's opacity-map splats."""
+        st = LightingStats()
+        self.ctx.check(self.ctx.L.sl_lighting_stats_read(self.ctx.h, self.h, C.byref(st)))
+        return {name: int(getattr(st, name)) for name, _ in LightingStats._fields_}
+
     def read(self, which: int, dtype=np.float32) -> np.ndarray:
         n = C.c_size_t()
         self.ctx.check(self.ctx.L.sl_scene_read(self.ctx.h, self.h, which, None, 0, C.byref(n)))
@@ -289,8 +300,8 @@ def frame_read(ctx: Context, which: int, dtype=np.uint32) -> np.ndarray:
 
 
 def timers(ctx: Context) -> dict:
-    names = (C.c_char_p * 16)()
-    ms = (C.c_float * 16)()
+    names = (C.c_char_p * 32)()
+    ms = (C.c_float * 32)()
     n = C.c_uint32()
-    ctx.check(ctx.L.sl_timers_read(ctx.h, names, ms, 16, C.byref(n)))
+    ctx.check(ctx.L.sl_timers_read(ctx.h, names, ms, 32, C.byref(n)))
     return {names[k].decode(): float(ms[k]) for k in range(n.value)}
