@@ -99,7 +99,30 @@ typedef struct sl_frame_stats {
  * pair count exceeded them. */
 sl_status sl_frame_finish(sl_context *ctx, sl_frame_stats *stats);
 
-/* Artefacts of the last frame, for parity checks: */
+/* ---- batches: many views over resident scenes in one launch per stage -------------------------------------
+ * The views (each: a scene, its 1200 B RenderFrame, its own target size and format) are laid out back to back in
+ * one slot space and one tile space; project, the depth sort (keyed by view, then depth), pair emission, the tile
+ * sort, the ranges, the heaviest-first order and the composite each run once over all of them. Every view's
+ * image, keys order and pairs are those it gets alone. With SL_BATCH_GRAPH the stage sequence is captured as a
+ * hipGraph the first time and replayed while the layout (view count, sizes, scenes' slot counts) is unchanged;
+ * camera and frame changes only re-upload the view table. No counterpart in the reference (one frame per
+ * requestAnimationFrame). */
+typedef struct sl_view {
+  sl_scene *scene;
+  const void *frame_bytes;     /* 1200 B RenderFrame */
+  size_t frame_len;
+  uint32_t width, height;      /* must equal the RenderFrame viewport */
+  uint32_t out_format;         /* SL_OUT_* */
+  void *out_device;            /* width * height pixels, rows top-down */
+} sl_view;
+#define SL_BATCH_GRAPH 2u      /* capture / replay the batch as a hipGraph (timers are not recorded) */
+sl_status sl_batch_encode(sl_context *ctx, const sl_view *views, uint32_t n_views, uint32_t flags);
+/* wait for the last batch (re-running it once with grown pair buffers on overflow); per-view stats */
+sl_status sl_batch_finish(sl_context *ctx, sl_frame_stats *stats, uint32_t n_stats);
+
+/* Artefacts of the last frame, for parity checks (a batch: the global slot space; view v's slots start at the
+ * sum of the earlier views' slot counts rounded up to 256, its keys carry v in bits 16-23, its tile ids start at
+ * the sum of the earlier views' tile counts): */
 #define SL_ART_KEYS 1u         /* u32 per slot: 16-bit depth key or 0xFFFFFFFF (project.wgsl:366-368) */
 #define SL_ART_ORDER 2u        /* u32 per slot: stable order by key, INVALID last (radix.wgsl) */
 #define SL_ART_SPLATS 3u       /* 48 B per slot (shaders/common.wgsl:29-35); invalid slots undefined */

@@ -51,16 +51,22 @@ __device__ __noinline__ float exact_alpha(float dx, float dy, const RawSplat rs)
 }
 
 extern "C" __global__ void __launch_bounds__(CP_THREADS)
-sl_composite(FrameDev f, const Splat *__restrict__ splats, const uint32_t *__restrict__ vals,
-             const uint2 *__restrict__ ranges, const uint32_t *__restrict__ tile_order,
-             const FrameCounters *__restrict__ counters, void *__restrict__ out, uint32_t out_format) {
+sl_composite(const ViewDev *__restrict__ views, const uint32_t *__restrict__ tile_base, uint32_t n_views,
+             const ViewCounters *__restrict__ view_counters, const Splat *__restrict__ splats,
+             const uint32_t *__restrict__ vals, const uint2 *__restrict__ ranges,
+             const uint32_t *__restrict__ tile_order) {
   __shared__ StagedSplat s_sp[CP_BATCH];
   __shared__ float4 s_col[CP_BATCH];
   __shared__ SplatBounds s_bd[CP_BATCH];
   __shared__ RawSplat s_raw[CP_BATCH];
 
-  // workgroup b takes the b-th heaviest tile (sl_tile_order): longest-first over the XCDs
-  const uint32_t tile = tile_order[blockIdx.x];
+  // workgroup b takes the b-th heaviest tile of the batch (sl_tile_order): longest-first over the XCDs
+  const uint32_t gtile = tile_order[blockIdx.x];
+  const uint32_t v = n_views > 1u ? view_of(tile_base, n_views, gtile) : 0u;
+  const FrameDev &f = views[v].f;
+  void *__restrict__ out = views[v].out;
+  const uint32_t out_format = views[v].out_format;
+  const uint32_t tile = gtile - tile_base[v];
   const uint32_t tyi = tile / f.tiles_x, txi = tile - tyi * f.tiles_x;
   const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
   const uint32_t bx = txi * 16u + (w & 1u) * 8u, by = tyi * 16u + (w >> 1) * 8u;   // this wave's 8x8 block
@@ -73,12 +79,12 @@ sl_composite(FrameDev f, const Splat *__restrict__ splats, const uint32_t *__res
   const float wx0 = ((float)bx + 0.5f) * 2.0f / fw - 1.0f, wx1 = ((float)bx + 7.5f) * 2.0f / fw - 1.0f;
   const float wy1 = 1.0f - ((float)by + 0.5f) * 2.0f / fh, wy0 = 1.0f - ((float)by + 7.5f) * 2.0f / fh;
   const float alpha_min = 1.0f / 255.0f;
-  const float cmax = __uint_as_float(counters->color_max_bits);
+  const float cmax = __uint_as_float(view_counters[v].color_max_bits);
   const float t_eps = 0x1p-12f / fmaxf(1.0f, cmax);
   const bool overdraw = f.view_mode == 2u;
   const bool depth_view = f.view_mode == 1u;
 
-  const uint2 rg = ranges[tile];
+  const uint2 rg = ranges[gtile];
   float T = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f, nfrag = 0.0f;
   bool done = !inside;
   for (uint32_t end = rg.y; end > rg.x; end -= min(end - rg.x, (uint32_t)CP_BATCH)) {

@@ -37,15 +37,22 @@ __device__ __forceinline__ uint32_t depth_key(float depth, float near_, float lo
 
 }  // namespace
 
+// One lane per global slot of a batch; the 256-slot block belongs to one view. A valid slot's key is
+// (view << 16) | the 16-bit depth key, so one stable sort orders views, then depth, then slot; slots past a view's
+// count (padding to 256) get INVALID.
 extern "C" __global__ void __launch_bounds__(256)
-sl_project_kernel(FrameDev f, const float4 *__restrict__ g0, const float4 *__restrict__ g1,
-                  const float4 *__restrict__ g2, const float4 *__restrict__ g3, const float4 *__restrict__ e0,
-                  const float4 *__restrict__ e1, const float4 *__restrict__ e2, const float4 *__restrict__ e3,
-                  const float4 *__restrict__ parts, const float4 *__restrict__ colour,
-                  const float4 *__restrict__ lo_b, Splat *__restrict__ splats, uint32_t *__restrict__ keys,
-                  uint2 *__restrict__ tbox, uint2 *__restrict__ partials) {
+sl_project_kernel(const ViewDev *__restrict__ views, const uint32_t *__restrict__ slot_base, uint32_t n_views,
+                  Splat *__restrict__ splats, uint32_t *__restrict__ keys, uint2 *__restrict__ tbox,
+                  uint2 *__restrict__ partials) {
   __shared__ uint2 s_part[256 / SL_WAVE];
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t v = n_views > 1u ? view_of(slot_base, n_views, blockIdx.x * blockDim.x) : 0u;
+  const ViewDev &vd = views[v];
+  const FrameDev &f = vd.f;
+  const float4 *__restrict__ g0 = vd.g0, *__restrict__ g1 = vd.g1, *__restrict__ g2 = vd.g2, *__restrict__ g3 = vd.g3;
+  const float4 *__restrict__ e0 = vd.e0, *__restrict__ e1 = vd.e1, *__restrict__ e2 = vd.e2, *__restrict__ e3 = vd.e3;
+  const float4 *__restrict__ parts = vd.parts, *__restrict__ colour = vd.colour, *__restrict__ lo_b = vd.lo_b;
+  const uint32_t i = gi - vd.slot_base;
   bool ok = i < f.nslots && i != f.count;
   float4 A, Q4, S4, C4;
   float mu[3], rot[3][3];
@@ -193,7 +200,7 @@ sl_project_kernel(FrameDev f, const float4 *__restrict__ g0, const float4 *__res
       const uint32_t glass = material != SL_EMITTER_MATERIAL ? (f.glass_mask >> mat_idx) & 1u : 0u;
       out.color_depth = make_uint4(__float_as_uint(col[0]), __float_as_uint(col[1]), __float_as_uint(col[2]),
                                    (__float_as_uint(depth) & ~3u) | glass);
-      key = depth_key(depth, f.near_, f.log_far_near);
+      key = (v << 16) | depth_key(depth, f.near_, f.log_far_near);
       cmax = fmaxf(fmaxf(fabsf(col[0]), fabsf(col[1])), fabsf(col[2]));
       // tile box of the extent box in pixel centres (the pair definition)
       const float hw = 0.5f * (float)f.width, hh = 0.5f * (float)f.height;
@@ -210,11 +217,9 @@ sl_project_kernel(FrameDev f, const float4 *__restrict__ g0, const float4 *__res
       }
     }
   }
-  if (i < f.nslots) {
-    if (ok) splats[i] = out;
-    keys[i] = key;
-    tbox[i] = box;
-  }
+  if (ok) splats[gi] = out;
+  keys[gi] = key;
+  tbox[gi] = box;
   // per-block visible count and colour max into partials[block] (no atomics: sl_frame_totals reduces them)
   const uint64_t vis = __ballot(ok);
   float wmax = cmax;
@@ -233,12 +238,16 @@ sl_project_kernel(FrameDev f, const float4 *__restrict__ g0, const float4 *__res
   }
 }
 
-// Frame totals from project's per-block partials: visible count and colour max (one workgroup).
+// Per-view totals from project's per-block partials (one workgroup per view): visible count and colour max into
+// view_counters[v]; the batch's visible total into counters.
 extern "C" __global__ void __launch_bounds__(1024)
-sl_frame_totals(const uint2 *__restrict__ partials, uint32_t nblocks, FrameCounters *__restrict__ counters) {
+sl_frame_totals(const uint2 *__restrict__ partials, const uint32_t *__restrict__ slot_base,
+                ViewCounters *__restrict__ view_counters, FrameCounters *__restrict__ counters) {
   __shared__ uint2 s_w[1024 / SL_WAVE];
+  const uint32_t v = blockIdx.x;
+  const uint32_t b0 = slot_base[v] / 256u, b1 = slot_base[v + 1] / 256u;
   uint2 t = make_uint2(0u, 0u);
-  for (uint32_t i = threadIdx.x; i < nblocks; i += 1024u) {
+  for (uint32_t i = b0 + threadIdx.x; i < b1; i += 1024u) {
     const uint2 p = partials[i];
     t.x += p.x;
     t.y = max(t.y, p.y);
@@ -256,7 +265,9 @@ sl_frame_totals(const uint2 *__restrict__ partials, uint32_t nblocks, FrameCount
       a.x += s_w[k].x;
       a.y = max(a.y, s_w[k].y);
     }
-    counters->visible = a.x;
-    counters->color_max_bits = a.y;
+    view_counters[v].visible = a.x;
+    view_counters[v].color_max_bits = a.y;
+    if (a.x) atomicAdd(&counters->visible, a.x);
+    atomicMax(&counters->color_max_bits, a.y);
   }
 }

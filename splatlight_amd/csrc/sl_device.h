@@ -28,6 +28,37 @@ struct FrameDev {
   uint32_t width, height, tiles_x, tiles_y;
 };
 
+// One view of a batch: its frame, its scene's streams and its target. Views lie back to back in the global slot
+// space (each view's slots start at a multiple of 256, so a 256-slot block belongs to one view) and in the global
+// tile space; a single frame is a batch of one view.
+struct ViewDev {
+  FrameDev f;
+  const float4 *g0, *g1, *g2, *g3;          // the scene's SoA Gaussian streams
+  const float4 *e0, *e1, *e2, *e3;          // its emitter streams
+  const float4 *parts, *colour, *lo_b;
+  void *out;                                // width * height pixels of out_format
+  uint32_t out_format;
+  uint32_t slot_base, tile_base, pad;
+};
+
+// Per-view results of a batch.
+struct ViewCounters {
+  uint32_t visible;        // valid depth keys of the view
+  uint32_t pairs;          // its (splat, tile) pairs
+  uint32_t color_max_bits; // max |rgb| over its visible splats (float bits)
+  uint32_t pad;
+};
+
+// the view owning global index x: the last v with base[v] <= x (base ascending, base[0] = 0)
+__device__ __forceinline__ uint32_t view_of(const uint32_t *__restrict__ base, uint32_t n_views, uint32_t x) {
+  uint32_t lo = 0, hi = n_views - 1u;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1u) >> 1;
+    if (base[mid] <= x) lo = mid; else hi = mid - 1u;
+  }
+  return lo;
+}
+
 // Splat record, 48 B: the layout project.wgsl writes (shaders/common.wgsl:29-35).
 struct __align__(16) Splat {
   float4 center_axis0;   // centre.xy (NDC), axis0.xy (NDC)

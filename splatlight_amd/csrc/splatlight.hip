@@ -45,6 +45,19 @@ constexpr uint32_t kGaussianStride = 64, kPartStride = 128, kMaterialStride = 32
 
 const char *const kPassNames[] = {"clear", "project", "radix", "pairs", "tilesort", "ranges", "splat"};
 constexpr int kPasses = 7;
+constexpr uint32_t kMaxViews = 256;   // the view id sits in bits 16-23 of the depth sort key
+constexpr size_t kViewBlock = sizeof(ViewDev) * kMaxViews + 4 * (3 * kMaxViews + 2);
+
+// A batch as encoded: the views' device descriptors and the global slot / tile layout.
+struct Batch {
+  std::vector<ViewDev> views;
+  std::vector<sl_scene *> scenes;
+  std::vector<uint32_t> slot_base, tile_base, tiles_x;   // V + 1, V + 1, V
+  uint32_t total_slots = 0, total_tiles = 0;
+  uint32_t depth_passes = 2, tile_passes = 2;
+  uint32_t flags = 0;
+  bool graph = false;
+};
 
 uint32_t rd_u32(const void *p, int word) { uint32_t v; memcpy(&v, (const char *)p + 4 * word, 4); return v; }
 float rd_f32(const void *p, int word) { float v; memcpy(&v, (const char *)p + 4 * word, 4); return v; }
@@ -96,12 +109,17 @@ struct Workspace {
   uint2 *tbox = nullptr;
   uint2 *partials = nullptr;      // project's per-block (visible, colour max)
   uint32_t *pk0 = nullptr, *pv0 = nullptr, *pk1 = nullptr, *pv1 = nullptr;
+  // the batch's view table, uploaded before every batch: ViewDev[kMaxViews], then slot_base, tile_base (kMaxViews + 1
+  // each) and tiles_x (kMaxViews)
+  ViewDev *views = nullptr;
+  uint32_t *slot_base = nullptr, *tile_base = nullptr, *tiles_x = nullptr;
+  ViewCounters *view_counters = nullptr;
   // zeroed at the start of every frame:
   uint8_t *scratch = nullptr;
   size_t scratch_words = 0;
   FrameCounters *counters = nullptr;
   uint32_t *tickets = nullptr, *depth_hist = nullptr, *tile_hist = nullptr;
-  uint32_t *dstatus[2] = {nullptr, nullptr}, *tstatus[2] = {nullptr, nullptr};
+  uint32_t *dstatus[3] = {nullptr, nullptr, nullptr}, *tstatus[3] = {nullptr, nullptr, nullptr};
   uint64_t *chunk_status = nullptr;
   uint2 *ranges = nullptr;
   uint32_t *tile_order = nullptr;  // composite launch order (written every frame)
@@ -122,12 +140,14 @@ struct sl_context {
   bool timers_valid = false;
   bool light_timers_valid = false;
   float ms[kPasses] = {};
-  // the last frame, for sl_frame_finish (re-run on overflow) and sl_frame_read
+  // the last batch (a frame is a batch of one), for sl_*_finish (re-run on overflow) and sl_frame_read
   bool has_last = false;
-  FrameDev last_f{};
-  sl_scene *last_scene = nullptr;
-  void *last_out = nullptr;
-  uint32_t last_fmt = 0, last_flags = 0;
+  Batch last;
+  uint32_t *final_keys = nullptr, *final_vals = nullptr;   // the tile-sorted pair lists of the last batch
+  ViewCounters *view_counters_host = nullptr;               // pinned, kMaxViews
+  // the last captured batch graph and what it was captured for
+  hipGraphExec_t graph = nullptr;
+  std::vector<uint64_t> graph_key;
 };
 
 namespace {
@@ -187,12 +207,16 @@ sl_status ensure_workspace(sl_context *ctx, uint32_t slots, uint32_t pairs, uint
   const size_t o_partials = take((size_t)((s_cap + 255) / 256) * 8);
   const size_t o_pk0 = take((size_t)p_cap * 4), o_pv0 = take((size_t)p_cap * 4);
   const size_t o_pk1 = take((size_t)p_cap * 4), o_pv1 = take((size_t)p_cap * 4);
+  const size_t o_views = take(kViewBlock);
+  const size_t o_vcount = take(sizeof(ViewCounters) * kMaxViews);
   const size_t o_scratch = off;
   const size_t o_counters = take(sizeof(FrameCounters));
   const size_t o_tickets = take(16 * 4);
-  const size_t o_dhist = take(2 * 256 * 4), o_thist = take(2 * 256 * 4);
+  const size_t o_dhist = take(3 * 256 * 4), o_thist = take(3 * 256 * 4);
   const size_t o_ds0 = take((size_t)d_tiles * 256 * 4), o_ds1 = take((size_t)d_tiles * 256 * 4);
+  const size_t o_ds2 = take((size_t)d_tiles * 256 * 4);
   const size_t o_ts0 = take((size_t)p_tiles * 256 * 4), o_ts1 = take((size_t)p_tiles * 256 * 4);
+  const size_t o_ts2 = take((size_t)p_tiles * 256 * 4);
   const size_t o_chunk = take((size_t)chunks * 8);
   const size_t o_ranges = take((size_t)t_cap * 8);
   const size_t o_torder = take((size_t)t_cap * 4);
@@ -202,6 +226,11 @@ sl_status ensure_workspace(sl_context *ctx, uint32_t slots, uint32_t pairs, uint
     SL_HIP(ctx, hipStreamSynchronize(ctx->stream));
     SL_HIP(ctx, hipFree(w.base));
     w.base = nullptr;
+  }
+  if (ctx->graph) {   // captured with the old buffers
+    (void)hipGraphExecDestroy(ctx->graph);
+    ctx->graph = nullptr;
+    ctx->graph_key.clear();
   }
   sl_status s = dev_alloc(ctx, total, &base);
   if (s != SL_OK) return s;
@@ -222,6 +251,11 @@ sl_status ensure_workspace(sl_context *ctx, uint32_t slots, uint32_t pairs, uint
   w.pv0 = (uint32_t *)(b + o_pv0);
   w.pk1 = (uint32_t *)(b + o_pk1);
   w.pv1 = (uint32_t *)(b + o_pv1);
+  w.views = (ViewDev *)(b + o_views);
+  w.slot_base = (uint32_t *)(b + o_views + sizeof(ViewDev) * kMaxViews);
+  w.tile_base = w.slot_base + kMaxViews + 1;
+  w.tiles_x = w.tile_base + kMaxViews + 1;
+  w.view_counters = (ViewCounters *)(b + o_vcount);
   w.scratch = b + o_scratch;
   w.scratch_words = (total - o_scratch) / 4;
   w.counters = (FrameCounters *)(b + o_counters);
@@ -230,8 +264,10 @@ sl_status ensure_workspace(sl_context *ctx, uint32_t slots, uint32_t pairs, uint
   w.tile_hist = (uint32_t *)(b + o_thist);
   w.dstatus[0] = (uint32_t *)(b + o_ds0);
   w.dstatus[1] = (uint32_t *)(b + o_ds1);
+  w.dstatus[2] = (uint32_t *)(b + o_ds2);
   w.tstatus[0] = (uint32_t *)(b + o_ts0);
   w.tstatus[1] = (uint32_t *)(b + o_ts1);
+  w.tstatus[2] = (uint32_t *)(b + o_ts2);
   w.chunk_status = (uint64_t *)(b + o_chunk);
   w.ranges = (uint2 *)(b + o_ranges);
   w.tile_order = (uint32_t *)(b + o_torder);
@@ -306,10 +342,11 @@ uint32_t grid_for(uint64_t n, uint32_t per_block, uint32_t cap) {
   return (uint32_t)std::min<uint64_t>(g, cap);
 }
 
-sl_status launch_frame(sl_context *ctx, const FrameDev &f, sl_scene *sc, void *out, uint32_t fmt, bool timers) {
+// The stage sequence of a batch on the context's stream: one launch per stage over every view.
+sl_status enqueue_batch(sl_context *ctx, const Batch &b, bool timers) {
   Workspace &w = ctx->ws;
   hipStream_t st = ctx->stream;
-  const uint32_t ntiles = f.tiles_x * f.tiles_y;
+  const uint32_t V = (uint32_t)b.views.size();
   auto mark = [&](int k) -> sl_status {
     if (timers) SL_HIP(ctx, hipEventRecord(ctx->ev[k], st));
     return SL_OK;
@@ -319,40 +356,104 @@ sl_status launch_frame(sl_context *ctx, const FrameDev &f, sl_scene *sc, void *o
   sl_fill_u32<<<grid_for(w.scratch_words, 256, 2048), 256, 0, st>>>((uint32_t *)w.scratch, 0u,
                                                                     (uint32_t)w.scratch_words);
   if ((s = mark(1)) != SL_OK) return s;
-  sl_project_kernel<<<grid_for(f.nslots, 256, 1u << 30), 256, 0, st>>>(
-      f, sc->g[0], sc->g[1], sc->g[2], sc->g[3], sc->e[0], sc->e[1], sc->e[2], sc->e[3], sc->parts, sc->colour,
-      sc->lo_b, w.splats, w.keys, w.tbox, w.partials);
-  sl_frame_totals<<<1, 1024, 0, st>>>(w.partials, (f.nslots + 255u) / 256u, w.counters);
+  sl_project_kernel<<<b.total_slots / 256u, 256, 0, st>>>(w.views, w.slot_base, V, w.splats, w.keys, w.tbox,
+                                                          w.partials);
+  sl_frame_totals<<<V, 1024, 0, st>>>(w.partials, w.slot_base, w.view_counters, w.counters);
   if ((s = mark(2)) != SL_OK) return s;
-  // depth order: stable sort of the 16-bit keys, values = slot
-  const uint32_t d_tiles = (f.nslots + RS_TILE - 1) / RS_TILE;
-  const uint32_t d_grid = std::min<uint32_t>(d_tiles, 4u * ctx->cus);
-  sl_radix_hist<<<grid_for(f.nslots, 256 * 16, 1024), RS_THREADS, 0, st>>>(w.keys, nullptr, f.nslots, 0u, 2u,
-                                                                          w.depth_hist);
-  sl_radix_bases<<<2, RS_THREADS, 0, st>>>(w.depth_hist);
-  sl_radix_pass<<<d_grid, RS_THREADS, 0, st>>>(w.keys, nullptr, w.keys_alt, w.vals_alt, nullptr, f.nslots, 0u,
+  // depth order: stable sort of (view, 16-bit depth) keys, values = global slot
+  const uint32_t n = b.total_slots;
+  const uint32_t d_grid = std::min<uint32_t>((n + RS_TILE - 1) / RS_TILE, 4u * ctx->cus);
+  sl_radix_hist<<<grid_for(n, 256 * 16, 1024), RS_THREADS, 0, st>>>(w.keys, nullptr, n, 0u, b.depth_passes,
+                                                                   w.depth_hist);
+  sl_radix_bases<<<b.depth_passes, RS_THREADS, 0, st>>>(w.depth_hist);
+  sl_radix_pass<<<d_grid, RS_THREADS, 0, st>>>(w.keys, nullptr, w.keys_alt, w.vals_alt, nullptr, n, 0u,
                                                w.depth_hist, w.dstatus[0], w.tickets + 0);
-  sl_radix_pass<<<d_grid, RS_THREADS, 0, st>>>(w.keys_alt, w.vals_alt, nullptr, w.order, nullptr, f.nslots, 8u,
-                                               w.depth_hist + 256, w.dstatus[1], w.tickets + 1);
+  if (b.depth_passes == 2) {
+    sl_radix_pass<<<d_grid, RS_THREADS, 0, st>>>(w.keys_alt, w.vals_alt, nullptr, w.order, nullptr, n, 8u,
+                                                 w.depth_hist + 256, w.dstatus[1], w.tickets + 1);
+  } else {
+    sl_radix_pass<<<d_grid, RS_THREADS, 0, st>>>(w.keys_alt, w.vals_alt, w.pk1, w.pv1, nullptr, n, 8u,
+                                                 w.depth_hist + 256, w.dstatus[1], w.tickets + 1);
+    sl_radix_pass<<<d_grid, RS_THREADS, 0, st>>>(w.pk1, w.pv1, nullptr, w.order, nullptr, n, 16u,
+                                                 w.depth_hist + 512, w.dstatus[2], w.tickets + 2);
+  }
   if ((s = mark(3)) != SL_OK) return s;
-  sl_pairs_emit<<<grid_for(f.nslots, PE_CHUNK, 2u * ctx->cus), PE_THREADS, 0, st>>>(
-      w.order, w.tbox, f.tiles_x, w.pairs_cap, w.counters, w.pk0, w.pv0, w.chunk_status, w.tickets + 2,
-      w.tile_hist);
+  sl_pairs_emit<<<grid_for(n, PE_CHUNK, 2u * ctx->cus), PE_THREADS, 0, st>>>(
+      w.order, w.tbox, w.slot_base, w.tile_base, w.tiles_x, V, b.tile_passes, w.pairs_cap, w.counters, w.pk0, w.pv0,
+      w.chunk_status, w.tickets + 3, w.tile_hist);
   if ((s = mark(4)) != SL_OK) return s;
-  const uint32_t p_grid = std::min<uint32_t>((w.pairs_cap + RS_TILE - 1) / RS_TILE, 4u * ctx->cus);
-  sl_radix_bases<<<2, RS_THREADS, 0, st>>>(w.tile_hist);
-  sl_radix_pass<<<std::max(p_grid, 1u), RS_THREADS, 0, st>>>(w.pk0, w.pv0, w.pk1, w.pv1, &w.counters->pairs, 0u,
-                                                             0u, w.tile_hist, w.tstatus[0], w.tickets + 3);
-  sl_radix_pass<<<std::max(p_grid, 1u), RS_THREADS, 0, st>>>(w.pk1, w.pv1, w.pk0, w.pv0, &w.counters->pairs, 0u,
-                                                             8u, w.tile_hist + 256, w.tstatus[1], w.tickets + 4);
+  const uint32_t p_grid = std::max(std::min<uint32_t>((w.pairs_cap + RS_TILE - 1) / RS_TILE, 4u * ctx->cus), 1u);
+  sl_radix_bases<<<b.tile_passes, RS_THREADS, 0, st>>>(w.tile_hist);
+  uint32_t *kin = w.pk0, *vin = w.pv0, *kout = w.pk1, *vout = w.pv1;
+  for (uint32_t pass = 0; pass < b.tile_passes; pass++) {
+    sl_radix_pass<<<p_grid, RS_THREADS, 0, st>>>(kin, vin, kout, vout, &w.counters->pairs, 0u, 8u * pass,
+                                                 w.tile_hist + 256 * pass, w.tstatus[pass], w.tickets + 4 + pass);
+    std::swap(kin, kout);
+    std::swap(vin, vout);
+  }
+  ctx->final_keys = kin;
+  ctx->final_vals = vin;
   if ((s = mark(5)) != SL_OK) return s;
-  sl_tile_ranges<<<grid_for(w.pairs_cap, 256, 4096), 256, 0, st>>>(w.pk0, w.counters, w.ranges);
-  sl_tile_order<<<1, 1024, 0, st>>>(w.ranges, ntiles, w.tile_order);
+  sl_tile_ranges<<<grid_for(w.pairs_cap, 256, 4096), 256, 0, st>>>(kin, w.counters, w.ranges);
+  sl_tile_order<<<1, 1024, 0, st>>>(w.ranges, b.total_tiles, w.tile_order);
+  sl_view_pairs<<<V, 256, 0, st>>>(w.ranges, w.tile_base, w.view_counters);
   if ((s = mark(6)) != SL_OK) return s;
-  sl_composite<<<ntiles, CP_THREADS, 0, st>>>(f, w.splats, w.pv0, w.ranges, w.tile_order, w.counters, out, fmt);
+  sl_composite<<<b.total_tiles, CP_THREADS, 0, st>>>(w.views, w.tile_base, V, w.view_counters, w.splats, vin,
+                                                     w.ranges, w.tile_order);
   if ((s = mark(7)) != SL_OK) return s;
-  if ((s = check_launch(ctx, "frame kernels")) != SL_OK) return s;
+  return check_launch(ctx, "frame kernels");
+}
+
+// Upload the view table, run the batch (replaying its graph when one was captured for the same layout), and
+// queue the counters' read-back.
+sl_status launch_batch(sl_context *ctx, const Batch &b) {
+  Workspace &w = ctx->ws;
+  hipStream_t st = ctx->stream;
+  const uint32_t V = (uint32_t)b.views.size();
+  std::vector<uint8_t> blk(kViewBlock, 0);
+  memcpy(blk.data(), b.views.data(), sizeof(ViewDev) * V);
+  uint32_t *sb = (uint32_t *)(blk.data() + sizeof(ViewDev) * kMaxViews);
+  memcpy(sb, b.slot_base.data(), 4 * (V + 1));
+  memcpy(sb + kMaxViews + 1, b.tile_base.data(), 4 * (V + 1));
+  memcpy(sb + 2 * (kMaxViews + 1), b.tiles_x.data(), 4 * V);
+  SL_HIP(ctx, hipMemcpyAsync(w.views, blk.data(), blk.size(), hipMemcpyHostToDevice, st));
+  const bool timers = (b.flags & SL_FRAME_TIMERS) != 0 && !b.graph;
+  sl_status s;
+  if (b.graph) {
+    // what the captured launches depend on: the layout and the workspace (the view table is read at run time)
+    std::vector<uint64_t> key = {(uint64_t)(uintptr_t)w.base, V, b.total_slots, b.total_tiles, w.pairs_cap,
+                                 b.depth_passes, b.tile_passes};
+    if (!ctx->graph || key != ctx->graph_key) {
+      if (ctx->graph) {
+        (void)hipGraphExecDestroy(ctx->graph);
+        ctx->graph = nullptr;
+      }
+      hipGraph_t g = nullptr;
+      SL_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      s = enqueue_batch(ctx, b, false);
+      hipError_t e = hipStreamEndCapture(st, &g);
+      if (s != SL_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return s;
+      }
+      if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamEndCapture");
+      e = hipGraphInstantiate(&ctx->graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) return hip_fail(ctx, e, "hipGraphInstantiate");
+      ctx->graph_key = key;
+    } else {
+      // the same launches: the pair lists end in the buffers the capture left them in
+      Workspace &ww = ctx->ws;
+      ctx->final_keys = b.tile_passes % 2 ? ww.pk1 : ww.pk0;
+      ctx->final_vals = b.tile_passes % 2 ? ww.pv1 : ww.pv0;
+    }
+    SL_HIP(ctx, hipGraphLaunch(ctx->graph, st));
+  } else if ((s = enqueue_batch(ctx, b, timers)) != SL_OK) {
+    return s;
+  }
   SL_HIP(ctx, hipMemcpyAsync(ctx->counters_host, w.counters, sizeof(FrameCounters), hipMemcpyDeviceToHost, st));
+  SL_HIP(ctx, hipMemcpyAsync(ctx->view_counters_host, w.view_counters, sizeof(ViewCounters) * V,
+                             hipMemcpyDeviceToHost, st));
   ctx->timers_valid = timers;
   return SL_OK;
 }
@@ -414,6 +515,9 @@ extern "C" sl_status sl_ctx_create(int device_ordinal, sl_context **out) {
   if ((e = hipHostMalloc((void **)&ctx->counters_host, sizeof(FrameCounters), hipHostMallocDefault)) != hipSuccess)
     return bail(e);
   memset(ctx->counters_host, 0, sizeof(FrameCounters));
+  if ((e = hipHostMalloc((void **)&ctx->view_counters_host, sizeof(ViewCounters) * kMaxViews, hipHostMallocDefault)) !=
+      hipSuccess)
+    return bail(e);
   for (int k = 0; k <= kPasses; k++)
     if ((e = hipEventCreate(&ctx->ev[k])) != hipSuccess) return bail(e);
   for (int k = 0; k < 5; k++)
@@ -436,6 +540,8 @@ extern "C" void sl_ctx_destroy(sl_context *ctx) {
   for (int k = 0; k < 4; k++)
     if (ctx->mev[k]) (void)hipEventDestroy(ctx->mev[k]);
   if (ctx->counters_host) (void)hipHostFree(ctx->counters_host);
+  if (ctx->view_counters_host) (void)hipHostFree(ctx->view_counters_host);
+  if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -639,7 +745,8 @@ extern "C" sl_status sl_scene_create(sl_context *ctx, const sl_scene_desc *d, sl
 extern "C" void sl_scene_destroy(sl_context *ctx, sl_scene *scene) {
   if (!scene) return;
   if (ctx && ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  if (ctx && ctx->last_scene == scene) { ctx->last_scene = nullptr; ctx->has_last = false; }
+  if (ctx && ctx->has_last && std::find(ctx->last.scenes.begin(), ctx->last.scenes.end(), scene) != ctx->last.scenes.end())
+    ctx->has_last = false;
   scene_free(scene);
 }
 
@@ -684,56 +791,94 @@ extern "C" sl_status sl_scene_write_bounce(sl_context *ctx, sl_scene *sc, const 
 
 // ---- frame --------------------------------------------------------------------------------------------------
 
-extern "C" sl_status sl_frame_encode(sl_context *ctx, sl_scene *sc, const void *frame_bytes, size_t frame_len,
-                                     const sl_frame_opts *opts, void *out_device) {
-  SL_LIVE(ctx);
-  if (!sc || !frame_bytes || !opts || !out_device) return fail(ctx, SL_ERR_INVALID, "null argument");
-  if (frame_len != kFrameBytes) return fail(ctx, SL_ERR_INVALID, "RenderFrame must be 1200 B");
-  if (opts->out_format > SL_OUT_RGBA16F) return fail(ctx, SL_ERR_INVALID, "unknown output format");
-  FrameDev f;
-  sl_status s = decode_frame(ctx, sc, frame_bytes, opts, &f);
-  if (s != SL_OK) return s;
-  const uint32_t ntiles = f.tiles_x * f.tiles_y;
-  if (ntiles > 65536u) return fail(ctx, SL_ERR_CAPACITY, "more than 65536 16x16 tiles");
+namespace {
+
+// Decode, validate and lay out a batch; grow the workspace; record it as the context's last batch and launch it.
+sl_status encode_batch(sl_context *ctx, const sl_view *views, uint32_t n_views, uint32_t flags) {
+  if (!views || n_views == 0) return fail(ctx, SL_ERR_INVALID, "no views");
+  if (n_views > kMaxViews) return fail(ctx, SL_ERR_CAPACITY, "more than 256 views in one batch");
+  Batch b;
+  b.flags = flags;
+  b.graph = (flags & SL_BATCH_GRAPH) != 0;
+  b.views.resize(n_views);
+  b.scenes.resize(n_views);
+  b.slot_base.assign(n_views + 1, 0);
+  b.tile_base.assign(n_views + 1, 0);
+  b.tiles_x.assign(n_views, 0);
+  uint64_t slots = 0, tiles = 0, want_pairs = 0;
+  for (uint32_t v = 0; v < n_views; v++) {
+    const sl_view &in = views[v];
+    sl_scene *sc = in.scene;
+    if (!sc || !in.frame_bytes || !in.out_device) return fail(ctx, SL_ERR_INVALID, "null argument");
+    if (in.frame_len != kFrameBytes) return fail(ctx, SL_ERR_INVALID, "RenderFrame must be 1200 B");
+    if (in.out_format > SL_OUT_RGBA16F) return fail(ctx, SL_ERR_INVALID, "unknown output format");
+    const sl_frame_opts o = {in.width, in.height, in.out_format, 0u};
+    ViewDev &vd = b.views[v];
+    memset(&vd, 0, sizeof(vd));
+    sl_status s = decode_frame(ctx, sc, in.frame_bytes, &o, &vd.f);
+    if (s != SL_OK) return s;
+    const uint32_t nt = vd.f.tiles_x * vd.f.tiles_y;
+    if (nt > 65536u) return fail(ctx, SL_ERR_CAPACITY, "more than 65536 16x16 tiles in a view");
+    vd.g0 = sc->g[0]; vd.g1 = sc->g[1]; vd.g2 = sc->g[2]; vd.g3 = sc->g[3];
+    vd.e0 = sc->e[0]; vd.e1 = sc->e[1]; vd.e2 = sc->e[2]; vd.e3 = sc->e[3];
+    vd.parts = sc->parts; vd.colour = sc->colour; vd.lo_b = sc->lo_b;
+    vd.out = in.out_device;
+    vd.out_format = in.out_format;
+    vd.slot_base = (uint32_t)slots;
+    vd.tile_base = (uint32_t)tiles;
+    b.scenes[v] = sc;
+    b.slot_base[v] = (uint32_t)slots;
+    b.tile_base[v] = (uint32_t)tiles;
+    b.tiles_x[v] = vd.f.tiles_x;
+    slots += ((uint64_t)vd.f.nslots + 255u) / 256u * 256u;
+    tiles += nt;
+    want_pairs += 8ull * vd.f.nslots;
+  }
+  if (slots >= (1ull << 31)) return fail(ctx, SL_ERR_CAPACITY, "more than 2^31 splat slots in one batch");
+  if (tiles > (1ull << 24)) return fail(ctx, SL_ERR_CAPACITY, "more than 2^24 tiles in one batch");
+  b.slot_base[n_views] = (uint32_t)slots;
+  b.tile_base[n_views] = (uint32_t)tiles;
+  b.total_slots = (uint32_t)slots;
+  b.total_tiles = (uint32_t)tiles;
+  b.depth_passes = n_views > 1 ? 3u : 2u;
+  b.tile_passes = tiles > 65536u ? 3u : 2u;
   SL_HIP(ctx, hipSetDevice(ctx->device));
-  const uint32_t want_pairs = std::max<uint32_t>(ctx->ws.pairs_cap, std::min<uint64_t>(8ull * f.nslots + 4096, 1u << 29));
-  if ((s = ensure_workspace(ctx, f.nslots, want_pairs, ntiles)) != SL_OK) return s;
+  const uint32_t pairs = std::max<uint32_t>(ctx->ws.pairs_cap, (uint32_t)std::min<uint64_t>(want_pairs + 4096, 1u << 29));
+  sl_status s;
+  if ((s = ensure_workspace(ctx, b.total_slots, std::max(pairs, b.total_slots), b.total_tiles)) != SL_OK) return s;
   ctx->has_last = true;
-  ctx->last_f = f;
-  ctx->last_scene = sc;
-  ctx->last_out = out_device;
-  ctx->last_fmt = opts->out_format;
-  ctx->last_flags = opts->flags;
-  return launch_frame(ctx, f, sc, out_device, opts->out_format, (opts->flags & SL_FRAME_TIMERS) != 0);
+  ctx->last = std::move(b);
+  return launch_batch(ctx, ctx->last);
 }
 
-extern "C" sl_status sl_frame_finish(sl_context *ctx, sl_frame_stats *stats) {
-  SL_LIVE(ctx);
+// Wait for the last batch; on pair overflow grow the pair buffers and run it again; per-view stats.
+sl_status finish_batch(sl_context *ctx, sl_frame_stats *stats, uint32_t n_stats) {
   if (!ctx->has_last) return fail(ctx, SL_ERR_INVALID, "no frame encoded");
   SL_HIP(ctx, hipStreamSynchronize(ctx->stream));
   FrameCounters c = *ctx->counters_host;
+  const Batch &b = ctx->last;
   if (c.overflow > 0) {
-    // the pair buffers were too small: grow them and render the frame again
     const uint64_t need = (uint64_t)c.pairs + c.overflow;
     const uint64_t grown = std::min<uint64_t>((need * 5) / 4 + 4096, 1u << 30);
-    if (need > grown) return fail(ctx, SL_ERR_CAPACITY, "frame needs more than 2^30 pairs");
-    sl_status s = ensure_workspace(ctx, ctx->last_f.nslots, (uint32_t)grown, ctx->last_f.tiles_x * ctx->last_f.tiles_y);
+    if (need > grown) return fail(ctx, SL_ERR_CAPACITY, "batch needs more than 2^30 pairs");
+    sl_status s = ensure_workspace(ctx, b.total_slots, (uint32_t)grown, b.total_tiles);
     if (s != SL_OK) return s;
-    s = launch_frame(ctx, ctx->last_f, ctx->last_scene, ctx->last_out, ctx->last_fmt,
-                     (ctx->last_flags & SL_FRAME_TIMERS) != 0);
-    if (s != SL_OK) return s;
+    if ((s = launch_batch(ctx, b)) != SL_OK) return s;
     SL_HIP(ctx, hipStreamSynchronize(ctx->stream));
     c = *ctx->counters_host;
     if (c.overflow > 0) return fail(ctx, SL_ERR_CAPACITY, "pair buffers still too small");
   }
-  if (stats) {
-    memset(stats, 0, sizeof(*stats));
-    stats->slots = ctx->last_f.nslots;
-    stats->visible = c.visible;
-    stats->pairs = c.pairs;
-    stats->overflow = c.overflow;
-    stats->tiles = ctx->last_f.tiles_x * ctx->last_f.tiles_y;
-  }
+  const uint32_t V = (uint32_t)b.views.size();
+  if (stats)
+    for (uint32_t v = 0; v < std::min(V, n_stats); v++) {
+      const ViewCounters &vc = ctx->view_counters_host[v];
+      memset(&stats[v], 0, sizeof(stats[v]));
+      stats[v].slots = b.views[v].f.nslots;
+      stats[v].visible = vc.visible;
+      stats[v].pairs = vc.pairs;
+      stats[v].overflow = c.overflow;
+      stats[v].tiles = b.views[v].f.tiles_x * b.views[v].f.tiles_y;
+    }
   if (ctx->timers_valid) {
     for (int k = 0; k < kPasses; k++) {
       float ms = 0.0f;
@@ -744,6 +889,31 @@ extern "C" sl_status sl_frame_finish(sl_context *ctx, sl_frame_stats *stats) {
   return SL_OK;
 }
 
+}  // namespace
+
+extern "C" sl_status sl_frame_encode(sl_context *ctx, sl_scene *sc, const void *frame_bytes, size_t frame_len,
+                                     const sl_frame_opts *opts, void *out_device) {
+  SL_LIVE(ctx);
+  if (!sc || !frame_bytes || !opts || !out_device) return fail(ctx, SL_ERR_INVALID, "null argument");
+  const sl_view v = {sc, frame_bytes, frame_len, opts->width, opts->height, opts->out_format, out_device};
+  return encode_batch(ctx, &v, 1, opts->flags & SL_FRAME_TIMERS);
+}
+
+extern "C" sl_status sl_frame_finish(sl_context *ctx, sl_frame_stats *stats) {
+  SL_LIVE(ctx);
+  return finish_batch(ctx, stats, 1);
+}
+
+extern "C" sl_status sl_batch_encode(sl_context *ctx, const sl_view *views, uint32_t n_views, uint32_t flags) {
+  SL_LIVE(ctx);
+  return encode_batch(ctx, views, n_views, flags);
+}
+
+extern "C" sl_status sl_batch_finish(sl_context *ctx, sl_frame_stats *stats, uint32_t n_stats) {
+  SL_LIVE(ctx);
+  return finish_batch(ctx, stats, n_stats);
+}
+
 extern "C" sl_status sl_frame_read(sl_context *ctx, uint32_t which, void *host, size_t bytes, size_t *written) {
   SL_LIVE(ctx);
   if (!ctx->has_last) return fail(ctx, SL_ERR_INVALID, "no frame encoded");
@@ -752,13 +922,16 @@ extern "C" sl_status sl_frame_read(sl_context *ctx, uint32_t which, void *host, 
   const Workspace &w = ctx->ws;
   const void *src = nullptr;
   size_t n = 0;
+  // a single frame: its own slots; a batch: the global slot space (views at their slot_base)
+  const Batch &b = ctx->last;
+  const size_t slots = b.views.size() == 1 ? b.views[0].f.nslots : b.total_slots;
   switch (which) {
-    case SL_ART_KEYS: src = w.keys; n = (size_t)ctx->last_f.nslots * 4; break;
-    case SL_ART_ORDER: src = w.order; n = (size_t)ctx->last_f.nslots * 4; break;
-    case SL_ART_SPLATS: src = w.splats; n = (size_t)ctx->last_f.nslots * sizeof(Splat); break;
-    case SL_ART_PAIR_TILES: src = w.pk0; n = (size_t)c.pairs * 4; break;
-    case SL_ART_PAIR_SLOTS: src = w.pv0; n = (size_t)c.pairs * 4; break;
-    case SL_ART_RANGES: src = w.ranges; n = (size_t)ctx->last_f.tiles_x * ctx->last_f.tiles_y * 8; break;
+    case SL_ART_KEYS: src = w.keys; n = slots * 4; break;
+    case SL_ART_ORDER: src = w.order; n = slots * 4; break;
+    case SL_ART_SPLATS: src = w.splats; n = slots * sizeof(Splat); break;
+    case SL_ART_PAIR_TILES: src = ctx->final_keys; n = (size_t)c.pairs * 4; break;
+    case SL_ART_PAIR_SLOTS: src = ctx->final_vals; n = (size_t)c.pairs * 4; break;
+    case SL_ART_RANGES: src = w.ranges; n = (size_t)b.total_tiles * 8; break;
     default: return fail(ctx, SL_ERR_INVALID, "unknown artefact");
   }
   if (written) *written = n;

@@ -19,15 +19,19 @@
 #define PE_FLAG_MASK (3ull << 62)
 #define PE_VALUE_MASK ((1ull << 62) - 1ull)
 
+// Batches: a rank's slot belongs to the view v with slot_base[v] <= slot < slot_base[v + 1]; its pairs' tile ids
+// are global, tile_base[v] + ty * tiles_x[v] + tx. tile_passes (2 or 3) digit histograms are built.
 extern "C" __global__ void __launch_bounds__(PE_THREADS)
-sl_pairs_emit(const uint32_t *__restrict__ order, const uint2 *__restrict__ tbox, uint32_t tiles_x,
-              uint32_t capacity, FrameCounters *counters, uint32_t *__restrict__ pair_keys,
+sl_pairs_emit(const uint32_t *__restrict__ order, const uint2 *__restrict__ tbox, const uint32_t *__restrict__ slot_base,
+              const uint32_t *__restrict__ tile_base, const uint32_t *__restrict__ tiles_x, uint32_t n_views,
+              uint32_t tile_passes, uint32_t capacity, FrameCounters *counters, uint32_t *__restrict__ pair_keys,
               uint32_t *__restrict__ pair_vals, uint64_t *chunk_status, uint32_t *ticket,
               uint32_t *__restrict__ tile_hist) {
   __shared__ uint32_t s_start[PE_CHUNK + 1];
   __shared__ uint2 s_box[PE_CHUNK];
   __shared__ uint32_t s_g[PE_CHUNK];
-  __shared__ uint32_t s_hist[2 * 256];
+  __shared__ uint32_t s_view[PE_CHUNK];
+  __shared__ uint32_t s_hist[3 * 256];
   __shared__ uint32_t s_wsum[PE_THREADS / SL_WAVE];
   __shared__ uint32_t s_chunk;
   __shared__ uint64_t s_excl;
@@ -35,7 +39,7 @@ sl_pairs_emit(const uint32_t *__restrict__ order, const uint2 *__restrict__ tbox
   const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
   const uint32_t V = __hip_atomic_load(&counters->visible, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   const uint32_t num_chunks = (V + PE_CHUNK - 1u) / PE_CHUNK;
-  for (uint32_t k = tid; k < 512u; k += PE_THREADS) s_hist[k] = 0u;
+  for (uint32_t k = tid; k < 768u; k += PE_THREADS) s_hist[k] = 0u;
 
   for (;;) {
     if (tid == 0) s_chunk = atomicAdd(ticket, 1u);
@@ -51,15 +55,17 @@ sl_pairs_emit(const uint32_t *__restrict__ order, const uint2 *__restrict__ tbox
       const uint32_t r = r0 + lr;
       uint32_t n = 0;
       uint2 b = make_uint2(0xffffffffu, 0u);
-      uint32_t g = 0;
+      uint32_t g = 0, vw = 0;
       if (r < V) {
         g = order[r];
         b = tbox[g];
         if (b.x != 0xffffffffu)
           n = ((b.y & 0xffffu) - (b.x & 0xffffu) + 1u) * ((b.y >> 16) - (b.x >> 16) + 1u);
+        if (n_views > 1u) vw = view_of(slot_base, n_views, g);
       }
       s_box[lr] = b;
       s_g[lr] = g;
+      s_view[lr] = vw;
       cnt[k] = n;
       sum += n;
     }
@@ -123,7 +129,8 @@ sl_pairs_emit(const uint32_t *__restrict__ order, const uint2 *__restrict__ tbox
       const uint32_t tx0 = b.x & 0xffffu, ty0 = b.x >> 16;
       const uint32_t bw = (b.y & 0xffffu) - tx0 + 1u;
       const uint32_t ty = ty0 + local / bw, tx = tx0 + local % bw;
-      const uint32_t tile = ty * tiles_x + tx;
+      const uint32_t vw = s_view[lo];
+      const uint32_t tile = tile_base[vw] + ty * tiles_x[vw] + tx;
       const uint64_t dst = base + j;
       const bool keep = dst < capacity;
       if (keep) {
@@ -132,8 +139,7 @@ sl_pairs_emit(const uint32_t *__restrict__ order, const uint2 *__restrict__ tbox
       }
       // tile-id digit counts for the tile sort, one LDS add per distinct digit of the wave (neighbouring pairs
       // share digits, so per-lane atomics would serialise on one address)
-#pragma unroll
-      for (int pass = 0; pass < 2; pass++) {
+      for (uint32_t pass = 0; pass < tile_passes; pass++) {
         const uint32_t d = (tile >> (8 * pass)) & 255u;
         uint64_t peers = __ballot(keep);
 #pragma unroll
@@ -148,8 +154,23 @@ sl_pairs_emit(const uint32_t *__restrict__ order, const uint2 *__restrict__ tbox
     __syncthreads();
   }
   __syncthreads();
-  for (uint32_t k = tid; k < 512u; k += PE_THREADS)
+  for (uint32_t k = tid; k < 256u * tile_passes; k += PE_THREADS)
     if (s_hist[k]) atomicAdd(&tile_hist[k], s_hist[k]);
+}
+
+// Pairs per view of a batch: the sum of its tiles' range lengths (one workgroup per view).
+extern "C" __global__ void __launch_bounds__(256)
+sl_view_pairs(const uint2 *__restrict__ ranges, const uint32_t *__restrict__ tile_base,
+              ViewCounters *__restrict__ view_counters) {
+  __shared__ uint32_t s_tmp[256 / SL_WAVE];
+  const uint32_t v = blockIdx.x;
+  uint32_t n = 0;
+  for (uint32_t t = tile_base[v] + threadIdx.x; t < tile_base[v + 1]; t += 256u) n += ranges[t].y - ranges[t].x;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) n += __shfl_xor(n, o);
+  if (lane_id() == 0) s_tmp[threadIdx.x >> 6] = n;
+  __syncthreads();
+  if (threadIdx.x == 0) view_counters[v].pairs = s_tmp[0] + s_tmp[1] + s_tmp[2] + s_tmp[3];
 }
 
 // Per-tile [start, end) in the tile-sorted pair list; ranges are zero on entry (empty tiles stay [0, 0)).

@@ -30,7 +30,9 @@ EXPORTS = [
     "sl_scene_write_bounce", "sl_scene_device_bytes",
     "sl_frame_encode", "sl_frame_finish", "sl_frame_read", "sl_timers_read",
     "sl_lighting_iso_init", "sl_lighting_encode", "sl_scene_read", "sl_lighting_stats_read",
+    "sl_batch_encode", "sl_batch_finish",
 ]
+SL_BATCH_GRAPH = 2
 SCENE_COLOUR, SCENE_LAMP_T, SCENE_ENV_T, SCENE_ISO_NORM = 1, 2, 3, 4
 SCENE_LSM_ACC, SCENE_LSM_MIP, SCENE_SUN_ACC, SCENE_SUN_MIP = 5, 6, 7, 8
 SL_LIGHT_TIMERS = 1
@@ -59,6 +61,11 @@ class FrameOpts(C.Structure):
 class FrameStats(C.Structure):
     _fields_ = [("slots", C.c_uint32), ("visible", C.c_uint32), ("pairs", C.c_uint32), ("overflow", C.c_uint32),
                 ("tiles", C.c_uint32), ("pad", C.c_uint32 * 3)]
+
+
+class View(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("frame_bytes", C.c_void_p), ("frame_len", C.c_size_t), ("width", C.c_uint32),
+                ("height", C.c_uint32), ("out_format", C.c_uint32), ("out_device", C.c_void_p)]
 
 
 class LightingDesc(C.Structure):
@@ -114,6 +121,8 @@ def lib() -> C.CDLL:
         "sl_lighting_encode": (i32, [vp, vp, C.POINTER(LightingDesc)]),
         "sl_scene_read": (i32, [vp, vp, u32, vp, sz, C.POINTER(sz)]),
         "sl_lighting_stats_read": (i32, [vp, vp, C.POINTER(LightingStats)]),
+        "sl_batch_encode": (i32, [vp, C.POINTER(View), u32, u32]),
+        "sl_batch_finish": (i32, [vp, C.POINTER(FrameStats), u32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -283,6 +292,24 @@ def frame_encode(ctx: Context, scene: Scene, frame: bytes, width: int, height: i
     fb = (C.c_uint8 * len(frame)).from_buffer_copy(frame)
     o = FrameOpts(width, height, out_format, SL_FRAME_TIMERS if timers else 0)
     ctx.check(ctx.L.sl_frame_encode(ctx.h, scene.h, fb, len(frame), C.byref(o), C.c_void_p(out_dev)))
+
+
+def batch_encode(ctx: Context, views, graph: bool = False, timers: bool = False):
+    """sl_batch_encode: views = [(scene, frame_bytes, width, height, out_dev, out_format)], one launch per stage."""
+    arr = (View * len(views))()
+    keep = []
+    for k, (scene, frame, w, h, out_dev, fmt) in enumerate(views):
+        fb = (C.c_uint8 * len(frame)).from_buffer_copy(frame)
+        keep.append(fb)
+        arr[k] = View(scene.h, C.cast(fb, C.c_void_p), len(frame), w, h, fmt, out_dev)
+    flags = (SL_BATCH_GRAPH if graph else 0) | (SL_FRAME_TIMERS if timers else 0)
+    ctx.check(ctx.L.sl_batch_encode(ctx.h, arr, len(views), flags))
+
+
+def batch_finish(ctx: Context, n: int) -> list:
+    st = (FrameStats * n)()
+    ctx.check(ctx.L.sl_batch_finish(ctx.h, st, n))
+    return [st[k] for k in range(n)]
 
 
 def frame_finish(ctx: Context) -> FrameStats:

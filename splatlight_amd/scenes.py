@@ -253,3 +253,35 @@ def tiny(n: int = 64, seed: int = 7, width: int = 64, height: int = 48, scale_mu
     s = cornell(n=n, seed=seed, width=width, height=height, scale_mu=scale_mu)
     s.name = "tiny"
     return s
+
+
+def orbit_eye(sc: Scene, radius: float, azimuth: float, elevation: float) -> tuple:
+    """A camera position on an orbit of the given radius around the scene's target (z-up)."""
+    t = np.asarray(sc.target, np.float64)
+    d = np.array([math.cos(elevation) * math.cos(azimuth), math.cos(elevation) * math.sin(azimuth),
+                  math.sin(elevation)])
+    return tuple(float(x) for x in t + radius * d)
+
+
+BATCH_SIZES = ((256, 256), (640, 480), (1280, 720), (1920, 1080))
+
+
+def batch_workload(n_scenes: int = 8, n_views: int = 64, lo: int = 50_000, hi: int = 2_000_000, seed: int = 0,
+                   sizes=BATCH_SIZES):
+    """Scenes of seeds 1..n_scenes with Gaussian counts log-uniform in [lo, hi], alternately Cornell-like (odd
+    seeds) and hill (even seeds), and n_views views: a scene, a size from `sizes` and a camera on a random orbit
+    of radius U[2, 6] m around the scene's target. Returns (scenes, [(scene index, width, height, eye)])."""
+    rng = np.random.default_rng(seed)
+    counts = np.exp(rng.uniform(math.log(lo), math.log(hi), n_scenes)).astype(int)
+    scenes = []
+    for k in range(n_scenes):
+        sd = k + 1
+        scenes.append(cornell(n=int(counts[k]), seed=sd) if sd % 2 else hill(n=int(counts[k]), seed=sd))
+    views = []
+    for _ in range(n_views):
+        si = int(rng.integers(n_scenes))
+        w, h = sizes[int(rng.integers(len(sizes)))]
+        eye = orbit_eye(scenes[si], rng.uniform(2.0, 6.0), rng.uniform(0.0, 2 * math.pi),
+                        rng.uniform(math.radians(10.0), math.radians(40.0)))
+        views.append((si, int(w), int(h), eye))
+    return scenes, views
