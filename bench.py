@@ -41,6 +41,7 @@ def parse():
     ap.add_argument("--pmc-summary", nargs=4, metavar=("FETCH_DIR", "WRITE_DIR", "COPY_FETCH_DIR", "COPY_WRITE_DIR"),
                     help="summarise rocprofv3 --pmc output directories into --traffic and exit")
     ap.add_argument("--pmc-command", default="", help="the profiling command, recorded in the summary")
+    ap.add_argument("--batched", type=int, default=1, help="also measure the batched form (64 views, 8 scenes)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB, 5 times (HBM counter calibration)")
     return ap.parse_args()
@@ -108,6 +109,63 @@ def cpu_baseline(sc, fb, k=8):
                       f"({t_resolve:.1f} s), project/order/tile lists of the whole frame ({t_frame:.1f} s), the "
                       f"fp16-target blend of every {k}th pixel row ({t_blend:.1f} s, x{k}); estimated frame "
                       f"{est:.1f} s"}
+
+
+def batched_leg(N, L, scenes, rounds=5):
+    """The batched form on 64 views over 8 resident scenes (Gaussian counts log-uniform in [50k, 2M], Cornell-like
+    and hill alternately, seeds 1-8; sizes from 256^2 to 1920x1080; cameras on orbits of radius U[2, 6] m), each
+    scene relit once (the lighting is view-independent), against the plain per-frame path over the same views."""
+    scs, views = scenes.batch_workload(seed=0)
+    ctx = N.Context(0)
+    handles = []
+    for sc in scs:
+        h = N.Scene(ctx, sc.gaussians, sc.parts, emitters=sc.emitters, materials=sc.materials,
+                    nodes=scenes.leaf_nodes(sc.gaussians), vgrid=scenes.vgrid_open(sc.count))
+        inp = L.lighting_inputs(sc, dirty_all=True)
+        h.light(inp["U"], inp["EP"], inp["emit_nodes"], inp["LS"], inp["ENV"], inp["SM"], inp["desc"])
+        handles.append(h)
+    frames = [scs[si].frame_bytes(width=w, height=h, eye=eye) for si, w, h, eye in views]
+    outs = [ctx.alloc(w * h * 8) for _, w, h, _ in views]
+    batch = [(handles[si], fb, w, h, o, N.SL_OUT_RGBA16F) for (si, w, h, _), fb, o in zip(views, frames, outs)]
+    mpix = sum(w * h for _, w, h, _ in views) / 1e6
+
+    def plain():
+        for (sc, fb, w, h, o, fmt) in batch:
+            N.frame_encode(ctx, sc, fb, w, h, o, fmt)
+            N.frame_finish(ctx)
+
+    def batched():
+        N.batch_encode(ctx, batch, graph=True)
+        return N.batch_finish(ctx, len(batch))
+
+    plain()
+    batched()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(rounds):
+        plain()
+    t_plain = (time.perf_counter() - t0) / rounds
+    lat = []
+    for _ in range(rounds * 4):
+        t = time.perf_counter()
+        st = batched()
+        lat.append(time.perf_counter() - t)
+    t_batch = float(np.mean(lat))
+    pairs = sum(x.pairs for x in st)
+    for o in outs:
+        ctx.free(o)
+    for h in handles:
+        h.destroy()
+    ctx.close()
+    return {"workload": f"{len(views)} views over {len(scs)} scenes "
+                        f"({min(sc.count for sc in scs)}-{max(sc.count for sc in scs)} Gaussians), sizes 256^2 to "
+                        f"1920x1080, rgba16f, hipGraph replay",
+            "mpix_per_batch": round(mpix, 3), "pairs_per_batch": int(pairs),
+            "batched_mpix_s": round(mpix / t_batch, 1), "plain_mpix_s": round(mpix / t_plain, 1),
+            "speedup_vs_plain": round(t_plain / t_batch, 3),
+            "batch_ms_p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
+            "batch_ms_p99": round(float(np.percentile(lat, 99)) * 1e3, 3),
+            "plain_ms_per_batch": round(t_plain * 1e3, 3)}
 
 
 def kernel_bytes(passes, P, T, target_bytes, ls):
@@ -294,6 +352,8 @@ def main():
     ctx.free(out)
     scene.destroy()
     ctx.close()
+    if a.batched:
+        res["batched"] = batched_leg(N, L, scenes)
     if a.cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(sc, fb)
     print(json.dumps(res))
