@@ -139,6 +139,8 @@ def batched_leg(N, L, scenes, rounds=5):
         return N.batch_finish(ctx, len(batch))
 
     plain()
+    from splatlight_amd import autotune
+    tuning, table = autotune.tune(ctx, batch, repeats=3, cache=None)
     batched()
     ctx.sync()
     t0 = time.perf_counter()
@@ -165,7 +167,9 @@ def batched_leg(N, L, scenes, rounds=5):
             "speedup_vs_plain": round(t_plain / t_batch, 3),
             "batch_ms_p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
             "batch_ms_p99": round(float(np.percentile(lat, 99)) * 1e3, 3),
-            "plain_ms_per_batch": round(t_plain * 1e3, 3)}
+            "plain_ms_per_batch": round(t_plain * 1e3, 3), "tuning": tuning,
+            "tuning_ms": {f"{c['composite_batch']}/{c['radix_items']}/{c['grid_mul']}": round(ms, 3)
+                          for c, ms in table}}
 
 
 def kernel_bytes(passes, P, T, target_bytes, ls):

@@ -115,6 +115,15 @@ typedef struct sl_view {
   uint32_t out_format;         /* SL_OUT_* */
   void *out_device;            /* width * height pixels, rows top-down */
 } sl_view;
+/* Launch shape of the frame kernels, picked per workload by an autotuner (splatlight_amd/autotune.py); every
+ * setting gives the same results. composite_batch: splats staged in LDS per round (64, 128, 256); radix_items: keys
+ * per lane of an onesweep tile (8: 2048-key tiles, 16: 4096); grid_mul: persistent radix workgroups per CU (1-16).
+ * Defaults 256, 16, 4. No counterpart in the reference. */
+typedef struct sl_tuning {
+  uint32_t composite_batch, radix_items, grid_mul, pad;
+} sl_tuning;
+sl_status sl_ctx_set_tuning(sl_context *ctx, const sl_tuning *tuning);
+sl_status sl_ctx_get_tuning(sl_context *ctx, sl_tuning *tuning);
 #define SL_BATCH_GRAPH 2u      /* capture / replay the batch as a hipGraph (timers are not recorded) */
 sl_status sl_batch_encode(sl_context *ctx, const sl_view *views, uint32_t n_views, uint32_t flags);
 /* wait for the last batch (re-running it once with grown pair buffers on overflow); per-view stats */

@@ -1,0 +1,70 @@
+"""Autotuner for the frame kernels' launch shape (sl_tuning): LDS batch depth of the composite, keys per lane of
+the onesweep radix passes, persistent radix workgroups per CU.
+
+Every candidate gives bit-identical results, so the tuner only times: it replays the caller's own batch as a
+hipGraph under each candidate and keeps the fastest. Choices are cached per device and workload signature in a JSON
+file (default ~/.cache/splatlight_amd/tuning.json) so a serving process tunes once per workload shape.
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import time
+
+import numpy as np
+
+from . import native as N
+
+CANDIDATES = [{"composite_batch": b, "radix_items": i, "grid_mul": g}
+              for b in (64, 128, 256) for i in (8, 16) for g in (2, 4, 8)]
+DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "splatlight_amd", "tuning.json")
+
+
+def signature(views) -> str:
+    """A workload's shape: the views' sizes and scenes' Gaussian counts (not the cameras)."""
+    parts = sorted(f"{w}x{h}:{scene.count}" for scene, _, w, h, _, _ in views)
+    return hashlib.sha1(";".join(parts).encode()).hexdigest()[:16]
+
+
+def _load(path):
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def tune(ctx: N.Context, views, repeats: int = 5, cache: str | None = DEFAULT_CACHE, device: str = "gfx950",
+         candidates=CANDIDATES):
+    """Time `views` (batch_encode tuples) under every candidate, set the fastest on ctx and return
+    (best, [(candidate, median ms)]). A cached choice for the same device and signature is applied without timing."""
+    key = f"{device}:{signature(views)}"
+    if cache:
+        hit = _load(cache).get(key)
+        if hit:
+            N.set_tuning(ctx, **hit["best"])
+            return hit["best"], hit.get("table", [])
+    table = []
+    for cand in candidates:
+        N.set_tuning(ctx, **cand)
+        N.batch_encode(ctx, views, graph=True)      # capture
+        N.batch_finish(ctx, len(views))
+        ts = []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            N.batch_encode(ctx, views, graph=True)
+            N.batch_finish(ctx, len(views))
+            ts.append(time.perf_counter() - t0)
+        table.append((cand, float(np.median(ts)) * 1e3))
+    best = min(table, key=lambda r: r[1])[0]
+    N.set_tuning(ctx, **best)
+    if cache:
+        data = _load(cache)
+        data[key] = {"best": best, "table": [[c, round(ms, 4)] for c, ms in table]}
+        os.makedirs(os.path.dirname(cache), exist_ok=True)
+        tmp = cache + f".{os.getpid()}"
+        with open(tmp, "w") as f:
+            json.dump(data, f, indent=1)
+        os.replace(tmp, cache)
+    return best, table

@@ -1,10 +1,10 @@
 // Front-to-back compositing of 16x16 tiles (the blend of shaders/splat.wgsl:22-78 with the "over" state of
 // src/renderer.js:140-142, evaluated per tile instead of by the rasteriser).
 //
-// One 256-lane workgroup per tile, launched heaviest tile first; each wave owns an 8x8 pixel block of it, one lane per pixel. The tile's pair
-// range is walked from its end, which is the reference's back-to-front sequence for every pixel reversed, in
-// batches of 256 splats: each lane stages one splat into LDS already reduced to what a pixel needs, then each
-// wave culls the batch against its 8x8 block (64 splats per ballot, tight ellipse bounds) and sweeps only the
+// One 256-lane workgroup per tile, launched heaviest tile first; each wave owns an 8x8 pixel block of it, one lane
+// per pixel. The tile's pair range is walked from its end, which is the reference's back-to-front sequence for
+// every pixel reversed, in batches of CP_BATCH_T splats (64, 128 or 256, tuned): each lane stages one splat into
+// LDS already reduced to what a pixel needs, then each wave culls the batch against its 8x8 block (64 splats per ballot, tight ellipse bounds) and sweeps only the
 // survivors, in order. A pixel stops once its transmittance is below t_eps = 2^-12 / max(1, max|rgb|) (the
 // colour it no longer adds is below 2^-12 absolute), a wave stops when all its pixels stopped, and the
 // workgroup leaves when all waves did.
@@ -21,7 +21,6 @@
 #include <hip/hip_fp16.h>
 
 #define CP_THREADS 256
-#define CP_BATCH 256
 #define LOG2E 1.4426950408889634f
 
 struct __align__(16) StagedSplat {
@@ -50,15 +49,16 @@ __device__ __noinline__ float exact_alpha(float dx, float dy, const RawSplat rs)
   return 1.0f - (float)exp((double)(-rs.tau0 * e));
 }
 
-extern "C" __global__ void __launch_bounds__(CP_THREADS)
+template <uint32_t CP_BATCH_T>
+__global__ void __launch_bounds__(CP_THREADS)
 sl_composite(const ViewDev *__restrict__ views, const uint32_t *__restrict__ tile_base, uint32_t n_views,
              const ViewCounters *__restrict__ view_counters, const Splat *__restrict__ splats,
              const uint32_t *__restrict__ vals, const uint2 *__restrict__ ranges,
              const uint32_t *__restrict__ tile_order) {
-  __shared__ StagedSplat s_sp[CP_BATCH];
-  __shared__ float4 s_col[CP_BATCH];
-  __shared__ SplatBounds s_bd[CP_BATCH];
-  __shared__ RawSplat s_raw[CP_BATCH];
+  __shared__ StagedSplat s_sp[CP_BATCH_T];
+  __shared__ float4 s_col[CP_BATCH_T];
+  __shared__ SplatBounds s_bd[CP_BATCH_T];
+  __shared__ RawSplat s_raw[CP_BATCH_T];
 
   // workgroup b takes the b-th heaviest tile of the batch (sl_tile_order): longest-first over the XCDs
   const uint32_t gtile = tile_order[blockIdx.x];
@@ -87,8 +87,8 @@ sl_composite(const ViewDev *__restrict__ views, const uint32_t *__restrict__ til
   const uint2 rg = ranges[gtile];
   float T = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f, nfrag = 0.0f;
   bool done = !inside;
-  for (uint32_t end = rg.y; end > rg.x; end -= min(end - rg.x, (uint32_t)CP_BATCH)) {
-    const uint32_t nb = min(end - rg.x, (uint32_t)CP_BATCH);
+  for (uint32_t end = rg.y; end > rg.x; end -= min(end - rg.x, CP_BATCH_T)) {
+    const uint32_t nb = min(end - rg.x, CP_BATCH_T);
     if (threadIdx.x < nb) {
       const uint32_t g = vals[end - 1u - threadIdx.x];
       const Splat s = splats[g];

@@ -5,7 +5,7 @@
 // pairs emitted in depth order; a stable sort keeps every tile's list in depth order).
 //
 // One read of the keys builds every pass's 256-digit histogram (sl_radix_hist + sl_radix_bases). Each pass is
-// then one kernel: persistent workgroups take 4096-element tiles in ticket order; inside a tile each wave ranks
+// then one kernel: persistent workgroups take 2048- or 4096-key tiles in ticket order; inside a tile each wave ranks
 // its 1024 elements by 8 ballots per item (peers with the same digit) and a per-wave LDS digit counter, so
 // the rank follows element order; the tile's digit counts are published with a decoupled look-back
 // (aggregate / inclusive flags packed with the count in one word) and the tile is reordered by digit in LDS
@@ -14,9 +14,7 @@
 
 #define RS_THREADS 256
 #define RS_WAVES (RS_THREADS / SL_WAVE)
-#define RS_ITEMS 16
-#define RS_TILE (RS_THREADS * RS_ITEMS)
-#define RS_WAVE_SPAN (SL_WAVE * RS_ITEMS)
+#define RS_TILE_MIN (RS_THREADS * 8)   // the smallest onesweep tile (8 keys per lane): sizes the look-back state
 #define RS_FLAG_AGG (1u << 30)
 #define RS_FLAG_INC (2u << 30)
 #define RS_FLAG_MASK (3u << 30)
@@ -82,12 +80,15 @@ extern "C" __global__ void __launch_bounds__(RS_THREADS) sl_radix_bases(uint32_t
 
 // One stable pass on digit (key >> shift) & 255. vals_in == nullptr: values are the element index.
 // keys_out == nullptr: keys are not written (the last pass when only the order is wanted).
-// status: >= ceil(n / 4096) * 256 words, zero on entry; ticket: one word, zero on entry.
-extern "C" __global__ void __launch_bounds__(RS_THREADS)
+// status: >= ceil(n / tile) * 256 words, zero on entry; ticket: one word, zero on entry.
+// RS_ITEMS keys per lane (8 or 16, tuned): a tile of 256 * RS_ITEMS keys.
+template <uint32_t RS_ITEMS>
+__global__ void __launch_bounds__(RS_THREADS)
 sl_radix_pass(const uint32_t *__restrict__ keys_in, const uint32_t *__restrict__ vals_in,
               uint32_t *__restrict__ keys_out, uint32_t *__restrict__ vals_out, const uint32_t *n_ptr,
               uint32_t n_static, uint32_t shift, const uint32_t *__restrict__ bases, uint32_t *status,
               uint32_t *ticket) {
+  constexpr uint32_t RS_TILE = RS_THREADS * RS_ITEMS, RS_WAVE_SPAN = SL_WAVE * RS_ITEMS;
   __shared__ uint32_t s_keys[RS_TILE];
   __shared__ uint32_t s_vals[RS_TILE];
   __shared__ uint32_t s_whist[RS_WAVES * 256];

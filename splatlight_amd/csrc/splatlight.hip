@@ -148,6 +148,7 @@ struct sl_context {
   // the last captured batch graph and what it was captured for
   hipGraphExec_t graph = nullptr;
   std::vector<uint64_t> graph_key;
+  sl_tuning tuning = {256u, 16u, 4u, 0u};
 };
 
 namespace {
@@ -195,8 +196,8 @@ sl_status ensure_workspace(sl_context *ctx, uint32_t slots, uint32_t pairs, uint
   const uint32_t s_cap = std::max(slots, w.slots_cap);
   const uint32_t p_cap = std::max(pairs, w.pairs_cap);
   const uint32_t t_cap = std::max(tiles, w.tiles_cap);
-  const uint32_t d_tiles = (s_cap + RS_TILE - 1) / RS_TILE;
-  const uint32_t p_tiles = (p_cap + RS_TILE - 1) / RS_TILE;
+  const uint32_t d_tiles = (s_cap + RS_TILE_MIN - 1) / RS_TILE_MIN;
+  const uint32_t p_tiles = (p_cap + RS_TILE_MIN - 1) / RS_TILE_MIN;
   const uint32_t chunks = (s_cap + PE_CHUNK - 1) / PE_CHUNK;
   size_t off = 0;
   auto take = [&](size_t bytes) { const size_t o = off; off = align_up(off + bytes, 256); return o; };
@@ -362,32 +363,41 @@ sl_status enqueue_batch(sl_context *ctx, const Batch &b, bool timers) {
   if ((s = mark(2)) != SL_OK) return s;
   // depth order: stable sort of (view, 16-bit depth) keys, values = global slot
   const uint32_t n = b.total_slots;
-  const uint32_t d_grid = std::min<uint32_t>((n + RS_TILE - 1) / RS_TILE, 4u * ctx->cus);
+  const sl_tuning &tu = ctx->tuning;
+  const uint32_t rs_tile = RS_THREADS * tu.radix_items;
+  const uint32_t d_grid = std::min<uint32_t>((n + rs_tile - 1) / rs_tile, tu.grid_mul * ctx->cus);
+  auto radix_pass = [&](const uint32_t *ki, const uint32_t *vi, uint32_t *ko, uint32_t *vo, const uint32_t *n_ptr,
+                        uint32_t n_static, uint32_t shift, const uint32_t *bases, uint32_t *status, uint32_t *ticket,
+                        uint32_t grid) {
+    if (tu.radix_items == 8u)
+      sl_radix_pass<8><<<grid, RS_THREADS, 0, st>>>(ki, vi, ko, vo, n_ptr, n_static, shift, bases, status, ticket);
+    else
+      sl_radix_pass<16><<<grid, RS_THREADS, 0, st>>>(ki, vi, ko, vo, n_ptr, n_static, shift, bases, status, ticket);
+  };
   sl_radix_hist<<<grid_for(n, 256 * 16, 1024), RS_THREADS, 0, st>>>(w.keys, nullptr, n, 0u, b.depth_passes,
                                                                    w.depth_hist);
   sl_radix_bases<<<b.depth_passes, RS_THREADS, 0, st>>>(w.depth_hist);
-  sl_radix_pass<<<d_grid, RS_THREADS, 0, st>>>(w.keys, nullptr, w.keys_alt, w.vals_alt, nullptr, n, 0u,
-                                               w.depth_hist, w.dstatus[0], w.tickets + 0);
+  radix_pass(w.keys, nullptr, w.keys_alt, w.vals_alt, nullptr, n, 0u, w.depth_hist, w.dstatus[0], w.tickets + 0, d_grid);
   if (b.depth_passes == 2) {
-    sl_radix_pass<<<d_grid, RS_THREADS, 0, st>>>(w.keys_alt, w.vals_alt, nullptr, w.order, nullptr, n, 8u,
-                                                 w.depth_hist + 256, w.dstatus[1], w.tickets + 1);
+    radix_pass(w.keys_alt, w.vals_alt, nullptr, w.order, nullptr, n, 8u, w.depth_hist + 256, w.dstatus[1],
+               w.tickets + 1, d_grid);
   } else {
-    sl_radix_pass<<<d_grid, RS_THREADS, 0, st>>>(w.keys_alt, w.vals_alt, w.pk1, w.pv1, nullptr, n, 8u,
-                                                 w.depth_hist + 256, w.dstatus[1], w.tickets + 1);
-    sl_radix_pass<<<d_grid, RS_THREADS, 0, st>>>(w.pk1, w.pv1, nullptr, w.order, nullptr, n, 16u,
-                                                 w.depth_hist + 512, w.dstatus[2], w.tickets + 2);
+    radix_pass(w.keys_alt, w.vals_alt, w.pk1, w.pv1, nullptr, n, 8u, w.depth_hist + 256, w.dstatus[1], w.tickets + 1,
+               d_grid);
+    radix_pass(w.pk1, w.pv1, nullptr, w.order, nullptr, n, 16u, w.depth_hist + 512, w.dstatus[2], w.tickets + 2,
+               d_grid);
   }
   if ((s = mark(3)) != SL_OK) return s;
   sl_pairs_emit<<<grid_for(n, PE_CHUNK, 2u * ctx->cus), PE_THREADS, 0, st>>>(
       w.order, w.tbox, w.slot_base, w.tile_base, w.tiles_x, V, b.tile_passes, w.pairs_cap, w.counters, w.pk0, w.pv0,
       w.chunk_status, w.tickets + 3, w.tile_hist);
   if ((s = mark(4)) != SL_OK) return s;
-  const uint32_t p_grid = std::max(std::min<uint32_t>((w.pairs_cap + RS_TILE - 1) / RS_TILE, 4u * ctx->cus), 1u);
+  const uint32_t p_grid = std::max(std::min<uint32_t>((w.pairs_cap + rs_tile - 1) / rs_tile, tu.grid_mul * ctx->cus), 1u);
   sl_radix_bases<<<b.tile_passes, RS_THREADS, 0, st>>>(w.tile_hist);
   uint32_t *kin = w.pk0, *vin = w.pv0, *kout = w.pk1, *vout = w.pv1;
   for (uint32_t pass = 0; pass < b.tile_passes; pass++) {
-    sl_radix_pass<<<p_grid, RS_THREADS, 0, st>>>(kin, vin, kout, vout, &w.counters->pairs, 0u, 8u * pass,
-                                                 w.tile_hist + 256 * pass, w.tstatus[pass], w.tickets + 4 + pass);
+    radix_pass(kin, vin, kout, vout, &w.counters->pairs, 0u, 8u * pass, w.tile_hist + 256 * pass, w.tstatus[pass],
+               w.tickets + 4 + pass, p_grid);
     std::swap(kin, kout);
     std::swap(vin, vout);
   }
@@ -398,8 +408,19 @@ sl_status enqueue_batch(sl_context *ctx, const Batch &b, bool timers) {
   sl_tile_order<<<1, 1024, 0, st>>>(w.ranges, b.total_tiles, w.tile_order);
   sl_view_pairs<<<V, 256, 0, st>>>(w.ranges, w.tile_base, w.view_counters);
   if ((s = mark(6)) != SL_OK) return s;
-  sl_composite<<<b.total_tiles, CP_THREADS, 0, st>>>(w.views, w.tile_base, V, w.view_counters, w.splats, vin,
-                                                     w.ranges, w.tile_order);
+  switch (tu.composite_batch) {
+    case 64u:
+      sl_composite<64><<<b.total_tiles, CP_THREADS, 0, st>>>(w.views, w.tile_base, V, w.view_counters, w.splats, vin,
+                                                             w.ranges, w.tile_order);
+      break;
+    case 128u:
+      sl_composite<128><<<b.total_tiles, CP_THREADS, 0, st>>>(w.views, w.tile_base, V, w.view_counters, w.splats, vin,
+                                                              w.ranges, w.tile_order);
+      break;
+    default:
+      sl_composite<256><<<b.total_tiles, CP_THREADS, 0, st>>>(w.views, w.tile_base, V, w.view_counters, w.splats, vin,
+                                                              w.ranges, w.tile_order);
+  }
   if ((s = mark(7)) != SL_OK) return s;
   return check_launch(ctx, "frame kernels");
 }
@@ -422,7 +443,8 @@ sl_status launch_batch(sl_context *ctx, const Batch &b) {
   if (b.graph) {
     // what the captured launches depend on: the layout and the workspace (the view table is read at run time)
     std::vector<uint64_t> key = {(uint64_t)(uintptr_t)w.base, V, b.total_slots, b.total_tiles, w.pairs_cap,
-                                 b.depth_passes, b.tile_passes};
+                                 b.depth_passes, b.tile_passes, ctx->tuning.composite_batch,
+                                 ctx->tuning.radix_items, ctx->tuning.grid_mul};
     if (!ctx->graph || key != ctx->graph_key) {
       if (ctx->graph) {
         (void)hipGraphExecDestroy(ctx->graph);
@@ -902,6 +924,24 @@ extern "C" sl_status sl_frame_encode(sl_context *ctx, sl_scene *sc, const void *
 extern "C" sl_status sl_frame_finish(sl_context *ctx, sl_frame_stats *stats) {
   SL_LIVE(ctx);
   return finish_batch(ctx, stats, 1);
+}
+
+extern "C" sl_status sl_ctx_set_tuning(sl_context *ctx, const sl_tuning *t) {
+  SL_LIVE(ctx);
+  if (!t) return fail(ctx, SL_ERR_INVALID, "null argument");
+  if ((t->composite_batch != 64u && t->composite_batch != 128u && t->composite_batch != 256u) ||
+      (t->radix_items != 8u && t->radix_items != 16u) || t->grid_mul < 1u || t->grid_mul > 16u)
+    return fail(ctx, SL_ERR_INVALID, "tuning: composite_batch 64/128/256, radix_items 8/16, grid_mul 1-16");
+  ctx->tuning = *t;
+  ctx->tuning.pad = 0u;
+  return SL_OK;
+}
+
+extern "C" sl_status sl_ctx_get_tuning(sl_context *ctx, sl_tuning *t) {
+  SL_LIVE(ctx);
+  if (!t) return fail(ctx, SL_ERR_INVALID, "null argument");
+  *t = ctx->tuning;
+  return SL_OK;
 }
 
 extern "C" sl_status sl_batch_encode(sl_context *ctx, const sl_view *views, uint32_t n_views, uint32_t flags) {

@@ -30,7 +30,7 @@ EXPORTS = [
     "sl_scene_write_bounce", "sl_scene_device_bytes",
     "sl_frame_encode", "sl_frame_finish", "sl_frame_read", "sl_timers_read",
     "sl_lighting_iso_init", "sl_lighting_encode", "sl_scene_read", "sl_lighting_stats_read",
-    "sl_batch_encode", "sl_batch_finish",
+    "sl_batch_encode", "sl_batch_finish", "sl_ctx_set_tuning", "sl_ctx_get_tuning",
 ]
 SL_BATCH_GRAPH = 2
 SCENE_COLOUR, SCENE_LAMP_T, SCENE_ENV_T, SCENE_ISO_NORM = 1, 2, 3, 4
@@ -66,6 +66,11 @@ class FrameStats(C.Structure):
 class View(C.Structure):
     _fields_ = [("scene", C.c_void_p), ("frame_bytes", C.c_void_p), ("frame_len", C.c_size_t), ("width", C.c_uint32),
                 ("height", C.c_uint32), ("out_format", C.c_uint32), ("out_device", C.c_void_p)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("composite_batch", C.c_uint32), ("radix_items", C.c_uint32), ("grid_mul", C.c_uint32),
+                ("pad", C.c_uint32)]
 
 
 class LightingDesc(C.Structure):
@@ -122,6 +127,8 @@ def lib() -> C.CDLL:
         "sl_scene_read": (i32, [vp, vp, u32, vp, sz, C.POINTER(sz)]),
         "sl_lighting_stats_read": (i32, [vp, vp, C.POINTER(LightingStats)]),
         "sl_batch_encode": (i32, [vp, C.POINTER(View), u32, u32]),
+        "sl_ctx_set_tuning": (i32, [vp, C.POINTER(Tuning)]),
+        "sl_ctx_get_tuning": (i32, [vp, C.POINTER(Tuning)]),
         "sl_batch_finish": (i32, [vp, C.POINTER(FrameStats), u32]),
     }
     for name, (res, args) in sig.items():
@@ -310,6 +317,16 @@ def batch_finish(ctx: Context, n: int) -> list:
     st = (FrameStats * n)()
     ctx.check(ctx.L.sl_batch_finish(ctx.h, st, n))
     return [st[k] for k in range(n)]
+
+
+def set_tuning(ctx: Context, composite_batch: int, radix_items: int, grid_mul: int):
+    ctx.check(ctx.L.sl_ctx_set_tuning(ctx.h, C.byref(Tuning(composite_batch, radix_items, grid_mul, 0))))
+
+
+def get_tuning(ctx: Context) -> dict:
+    t = Tuning()
+    ctx.check(ctx.L.sl_ctx_get_tuning(ctx.h, C.byref(t)))
+    return {"composite_batch": t.composite_batch, "radix_items": t.radix_items, "grid_mul": t.grid_mul}
 
 
 def frame_finish(ctx: Context) -> FrameStats:

@@ -143,3 +143,43 @@ def test_batch_refuses_bad_views(ctx, world):
         N.batch_finish(ctx, 1)
     finally:
         ctx.free(out)
+
+
+def test_every_tuning_gives_the_same_results(ctx, world):
+    """The launch-shape candidates of the autotuner change no result: images and pair lists bit-identical."""
+    from splatlight_amd import autotune
+    scs, handles = world
+    frames = frames_for(scs, VIEWS, 0.2)
+    ref_imgs, ref_stats, ref_arts = run_batch(ctx, handles, VIEWS, frames, graph=False)
+    try:
+        for cand in autotune.CANDIDATES:
+            N.set_tuning(ctx, **cand)
+            imgs, stats, arts = run_batch(ctx, handles, VIEWS, frames, graph=True)
+            assert all(np.array_equal(a, b) for a, b in zip(imgs, ref_imgs)), cand
+            assert [s.pairs for s in stats] == [s.pairs for s in ref_stats], cand
+            for k in ref_arts:
+                assert np.array_equal(arts[k], ref_arts[k]), (cand, k)
+        log(f"  {len(autotune.CANDIDATES)} tunings: identical images and artefacts")
+    finally:
+        N.set_tuning(ctx, 256, 16, 4)
+
+
+def test_autotune_picks_and_caches(ctx, world, tmp_path):
+    from splatlight_amd import autotune
+    scs, handles = world
+    frames = frames_for(scs, VIEWS, 0.4)
+    outs = [ctx.alloc(w * h * 16) for _, w, h, *_ in VIEWS]
+    try:
+        views = [(handles[si], fb, w, h, o, fmt) for (si, w, h, r, az, el, fmt), fb, o in zip(VIEWS, frames, outs)]
+        cache = str(tmp_path / "tuning.json")
+        best, table = autotune.tune(ctx, views, repeats=2, cache=cache)
+        assert len(table) == len(autotune.CANDIDATES) and best in autotune.CANDIDATES
+        assert N.get_tuning(ctx) == best
+        N.set_tuning(ctx, 256, 16, 4)
+        again, _ = autotune.tune(ctx, views, cache=cache)               # from the cache, no timing
+        assert again == best and N.get_tuning(ctx) == best
+        log(f"  autotune: best {best}, {min(ms for _, ms in table):.3f} ms .. {max(ms for _, ms in table):.3f} ms")
+    finally:
+        N.set_tuning(ctx, 256, 16, 4)
+        for o in outs:
+            ctx.free(o)
