@@ -42,6 +42,8 @@ def parse():
                     help="summarise rocprofv3 --pmc output directories into --traffic and exit")
     ap.add_argument("--pmc-command", default="", help="the profiling command, recorded in the summary")
     ap.add_argument("--batched", type=int, default=1, help="also measure the batched form (64 views, 8 scenes)")
+    ap.add_argument("--serving", type=int, default=1, help="also measure the served form (Poisson arrivals)")
+    ap.add_argument("--rate", type=float, default=1000.0, help="serving arrival rate, requests per second")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB, 5 times (HBM counter calibration)")
     return ap.parse_args()
@@ -170,6 +172,78 @@ def batched_leg(N, L, scenes, rounds=5):
             "plain_ms_per_batch": round(t_plain * 1e3, 3), "tuning": tuning,
             "tuning_ms": {f"{c['composite_batch']}/{c['radix_items']}/{c['grid_mul']}": round(ms, 3)
                           for c, ms in table}}
+
+
+def serving_leg(N, scenes, rate=1000.0, seconds=2.0, budget=4 << 30, lanes=2, max_batch=32):
+    """The served form: Poisson arrivals at `rate` over 32 resident scenes (100k-3M Gaussians, seeds 10-41, colour
+    precomputed), 4 sizes from 256^2 to 1920x1080, random orbit poses; admission by a fixed HBM budget, continuous
+    batching over `lanes` lanes. Against the plain per-frame path fed the same arrivals one frame at a time. Also
+    the saturated rate of both (every request queued at once)."""
+    scs = scenes.serving_workload()
+    ctx = N.Context(0)
+    handles = []
+    for sc in scs:
+        h = N.Scene(ctx, sc.gaussians, sc.parts, emitters=sc.emitters)
+        rng = np.random.default_rng(sc.count)
+        h.write_colour(np.concatenate([sc.gaussians["albedo"] * rng.uniform(0.5, 3.0, (sc.count, 1)),
+                                       np.ones((sc.count, 1))], 1).astype(np.float32))
+        handles.append(h)
+    reqs = scenes.request_stream(scs, rate, int(rate * seconds))
+    frames = [scs[si].frame_bytes(width=w, height=h, eye=eye) for _, si, w, h, eye in reqs]
+    ring = lanes * max_batch                      # targets per size: enough for everything in flight
+    targets = {(w, h): [ctx.alloc(w * h * 8) for _ in range(ring)] for w, h in scenes.BATCH_SIZES}
+    mpix = sum(w * h for _, _, w, h, _ in reqs) / 1e6
+
+    def served(arrivals):
+        srv = N.Server(ctx, hbm_budget=budget, lanes=lanes, max_batch_views=max_batch)
+        done, sizes = [], []
+        t0 = time.perf_counter()
+        for k, ((t_arr, si, w, h, _), fb) in enumerate(zip(reqs, frames)):
+            if arrivals:
+                while time.perf_counter() - t0 < t_arr:
+                    done += srv.poll()
+            srv.submit(handles[si], fb, w, h, targets[(w, h)][k % ring], N.SL_OUT_RGBA16F)
+        while len(done) < len(reqs):
+            done += srv.poll(wait_us=2000)
+        srv.close()
+        lat = np.array([r.done_s - r.submit_s for r in done])
+        span = max(r.done_s for r in done) - min(r.submit_s for r in done)
+        return lat, span, np.mean([r.batch_views for r in done])
+
+    def plain(arrivals):
+        lat = []
+        t0 = time.perf_counter()
+        for k, ((t_arr, si, w, h, _), fb) in enumerate(zip(reqs, frames)):
+            if arrivals:
+                while time.perf_counter() - t0 < t_arr:
+                    pass
+            ta = t0 + t_arr if arrivals else t0
+            N.frame_encode(ctx, handles[si], fb, w, h, targets[(w, h)][k % ring], N.SL_OUT_RGBA16F)
+            N.frame_finish(ctx)
+            lat.append(time.perf_counter() - ta)
+        return np.array(lat), time.perf_counter() - t0
+
+    served(False)                                  # warm-up: workspaces of every lane grown
+    lat_s, _, views_s = served(True)
+    lat_p, _ = plain(True)
+    _, span_sat, views_sat = served(False)
+    _, span_plain = plain(False)
+    for lst in targets.values():
+        for o in lst:
+            ctx.free(o)
+    for h in handles:
+        h.destroy()
+    ctx.close()
+    pc = lambda a, q: round(float(np.percentile(a, q)) * 1e3, 3)
+    return {"workload": f"Poisson {rate:.0f} req/s for {seconds:.0f} s over {len(scs)} resident scenes "
+                        f"({min(sc.count for sc in scs)}-{max(sc.count for sc in scs)} Gaussians), sizes 256^2 to "
+                        f"1920x1080, rgba16f; HBM budget {budget >> 20} MiB, {lanes} lanes, <= {max_batch} views/batch",
+            "requests": len(reqs), "mpix": round(mpix, 2),
+            "served_p50_ms": pc(lat_s, 50), "served_p99_ms": pc(lat_s, 99), "served_mean_batch": round(views_s, 2),
+            "plain_p50_ms": pc(lat_p, 50), "plain_p99_ms": pc(lat_p, 99),
+            "saturated_served_mpix_s": round(mpix / span_sat, 1), "saturated_mean_batch": round(views_sat, 2),
+            "saturated_plain_mpix_s": round(mpix / span_plain, 1),
+            "saturated_speedup": round(span_plain / span_sat, 3)}
 
 
 def kernel_bytes(passes, P, T, target_bytes, ls):
@@ -358,6 +432,8 @@ def main():
     ctx.close()
     if a.batched:
         res["batched"] = batched_leg(N, L, scenes)
+    if a.serving:
+        res["serving"] = serving_leg(N, scenes, rate=a.rate)
     if a.cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(sc, fb)
     print(json.dumps(res))

@@ -129,6 +129,36 @@ sl_status sl_batch_encode(sl_context *ctx, const sl_view *views, uint32_t n_view
 /* wait for the last batch (re-running it once with grown pair buffers on overflow); per-view stats */
 sl_status sl_batch_finish(sl_context *ctx, sl_frame_stats *stats, uint32_t n_stats);
 
+/* ---- serving: continuous batching of render requests over resident scenes --------------------------------
+ * A worker thread owns `lanes` execution lanes (each its own stream and workspace on the context's device). Whenever
+ * a lane is free it admits waiting requests, in arrival order, into one batch while the batch has at most
+ * max_batch_views views and the HBM estimate of all in-flight work stays within hbm_budget, and launches it; up to
+ * `lanes` batches run at once, so requests arriving while others render are admitted as lanes and memory free up.
+ * Each result is the view rendered alone (same kernels, per-view state). The scenes and targets must stay alive
+ * until their results are polled. No counterpart in the reference (one frame per requestAnimationFrame). */
+typedef struct sl_server sl_server;
+typedef struct sl_server_opts {
+  uint64_t hbm_budget;         /* bytes of in-flight work (targets, slots, pair lists) */
+  uint32_t lanes;              /* 1-8 concurrent batches */
+  uint32_t max_batch_views;    /* 1-256 */
+} sl_server_opts;
+typedef struct sl_request_result {
+  uint64_t id;
+  sl_status status;
+  uint32_t batch_views;        /* size of the batch it ran in */
+  sl_frame_stats stats;
+  double submit_s, start_s, done_s;   /* steady clock, seconds since sl_server_create */
+} sl_request_result;
+sl_status sl_server_create(sl_context *ctx, const sl_server_opts *opts, sl_server **out);
+/* copies the frame bytes; the view's target receives the image */
+sl_status sl_server_submit(sl_server *server, const sl_view *view, uint64_t *request_id);
+/* completed requests (up to cap), waiting up to wait_us for the first when none is ready */
+sl_status sl_server_poll(sl_server *server, sl_request_result *out, uint32_t cap, uint32_t *n, uint32_t wait_us);
+sl_status sl_server_info(sl_server *server, uint32_t *pending, uint32_t *running, uint64_t *in_flight_bytes);
+const char *sl_server_error(sl_server *server);
+/* finishes every submitted request, then stops the worker */
+void sl_server_destroy(sl_server *server);
+
 /* Artefacts of the last frame, for parity checks (a batch: the global slot space; view v's slots start at the
  * sum of the earlier views' slot counts rounded up to 256, its keys carry v in bits 16-23, its tile ids start at
  * the sum of the earlier views' tile counts): */

@@ -1313,3 +1313,5 @@ extern "C" sl_status sl_timers_read(sl_context *ctx, const char **names, float *
   if (n) *n = k;
   return SL_OK;
 }
+
+#include "server.hip"

@@ -31,6 +31,8 @@ EXPORTS = [
     "sl_frame_encode", "sl_frame_finish", "sl_frame_read", "sl_timers_read",
     "sl_lighting_iso_init", "sl_lighting_encode", "sl_scene_read", "sl_lighting_stats_read",
     "sl_batch_encode", "sl_batch_finish", "sl_ctx_set_tuning", "sl_ctx_get_tuning",
+    "sl_server_create", "sl_server_submit", "sl_server_poll", "sl_server_info", "sl_server_error",
+    "sl_server_destroy",
 ]
 SL_BATCH_GRAPH = 2
 SCENE_COLOUR, SCENE_LAMP_T, SCENE_ENV_T, SCENE_ISO_NORM = 1, 2, 3, 4
@@ -71,6 +73,15 @@ class View(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("composite_batch", C.c_uint32), ("radix_items", C.c_uint32), ("grid_mul", C.c_uint32),
                 ("pad", C.c_uint32)]
+
+
+class ServerOpts(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("lanes", C.c_uint32), ("max_batch_views", C.c_uint32)]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("status", C.c_int32), ("batch_views", C.c_uint32), ("stats", FrameStats),
+                ("submit_s", C.c_double), ("start_s", C.c_double), ("done_s", C.c_double)]
 
 
 class LightingDesc(C.Structure):
@@ -128,6 +139,12 @@ def lib() -> C.CDLL:
         "sl_lighting_stats_read": (i32, [vp, vp, C.POINTER(LightingStats)]),
         "sl_batch_encode": (i32, [vp, C.POINTER(View), u32, u32]),
         "sl_ctx_set_tuning": (i32, [vp, C.POINTER(Tuning)]),
+        "sl_server_create": (i32, [vp, C.POINTER(ServerOpts), C.POINTER(vp)]),
+        "sl_server_submit": (i32, [vp, C.POINTER(View), C.POINTER(C.c_uint64)]),
+        "sl_server_poll": (i32, [vp, C.POINTER(RequestResult), u32, C.POINTER(u32), u32]),
+        "sl_server_info": (i32, [vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(C.c_uint64)]),
+        "sl_server_error": (C.c_char_p, [vp]),
+        "sl_server_destroy": (None, [vp]),
         "sl_ctx_get_tuning": (i32, [vp, C.POINTER(Tuning)]),
         "sl_batch_finish": (i32, [vp, C.POINTER(FrameStats), u32]),
     }
@@ -317,6 +334,42 @@ def batch_finish(ctx: Context, n: int) -> list:
     st = (FrameStats * n)()
     ctx.check(ctx.L.sl_batch_finish(ctx.h, st, n))
     return [st[k] for k in range(n)]
+
+
+class Server:
+    """sl_server_*: continuous batching of render requests over resident scenes."""
+
+    def __init__(self, ctx: Context, hbm_budget: int, lanes: int = 2, max_batch_views: int = 32):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        ctx.check(ctx.L.sl_server_create(ctx.h, C.byref(ServerOpts(hbm_budget, lanes, max_batch_views)),
+                                         C.byref(self.h)))
+
+    def submit(self, scene: "Scene", frame: bytes, width: int, height: int, out_dev: int,
+               out_format: int = SL_OUT_RGBA32F) -> int:
+        fb = (C.c_uint8 * len(frame)).from_buffer_copy(frame)
+        v = View(scene.h, C.cast(fb, C.c_void_p), len(frame), width, height, out_format, out_dev)
+        rid = C.c_uint64()
+        st = self.ctx.L.sl_server_submit(self.h, C.byref(v), C.byref(rid))
+        if st != 0:
+            raise SplatlightError(st, "sl_server_submit refused the request")
+        return int(rid.value)
+
+    def poll(self, wait_us: int = 0, cap: int = 256) -> list:
+        out = (RequestResult * cap)()
+        n = C.c_uint32()
+        self.ctx.L.sl_server_poll(self.h, out, cap, C.byref(n), wait_us)
+        return [out[k] for k in range(n.value)]
+
+    def info(self):
+        p, r, b = C.c_uint32(), C.c_uint32(), C.c_uint64()
+        self.ctx.L.sl_server_info(self.h, C.byref(p), C.byref(r), C.byref(b))
+        return int(p.value), int(r.value), int(b.value)
+
+    def close(self):
+        if self.h:
+            self.ctx.L.sl_server_destroy(self.h)
+            self.h = C.c_void_p()
 
 
 def set_tuning(ctx: Context, composite_batch: int, radix_items: int, grid_mul: int):

@@ -285,3 +285,30 @@ def batch_workload(n_scenes: int = 8, n_views: int = 64, lo: int = 50_000, hi: i
                         rng.uniform(math.radians(10.0), math.radians(40.0)))
         views.append((si, int(w), int(h), eye))
     return scenes, views
+
+
+def serving_workload(n_scenes: int = 32, lo: int = 100_000, hi: int = 3_000_000, first_seed: int = 10,
+                     seed: int = 0):
+    """Resident scenes of seeds first_seed.. with Gaussian counts log-uniform in [lo, hi], Cornell-like (even
+    seeds) and hill (odd seeds) alternately."""
+    rng = np.random.default_rng(seed)
+    counts = np.exp(rng.uniform(math.log(lo), math.log(hi), n_scenes)).astype(int)
+    out = []
+    for k in range(n_scenes):
+        sd = first_seed + k
+        out.append(cornell(n=int(counts[k]), seed=sd) if sd % 2 == 0 else hill(n=int(counts[k]), seed=sd))
+    return out
+
+
+def request_stream(scs, rate: float, count: int, seed: int = 1, sizes=BATCH_SIZES):
+    """Poisson arrivals at `rate` per second: [(arrival s, scene index, width, height, eye)]."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, count))
+    out = []
+    for k in range(count):
+        si = int(rng.integers(len(scs)))
+        w, h = sizes[int(rng.integers(len(sizes)))]
+        eye = orbit_eye(scs[si], rng.uniform(2.0, 6.0), rng.uniform(0.0, 2 * math.pi),
+                        rng.uniform(math.radians(10.0), math.radians(40.0)))
+        out.append((float(t[k]), si, int(w), int(h), eye))
+    return out

@@ -1,0 +1,99 @@
+"""Serving on the GPU: continuous batching of render requests over resident scenes.
+
+Bar: every request's image is bit-identical to the same view rendered alone; requests that arrive while others
+render join later batches (more than one batch, batches of more than one view, and lanes overlapping in time);
+the HBM budget bounds what is in flight.
+"""
+import math
+import sys
+import time
+
+import numpy as np
+import pytest
+
+from splatlight_amd import native as N
+from splatlight_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def test_served_results_equal_views_alone(ctx):
+    scs = [scenes.cornell(n=15_000, seed=11), scenes.hill(n=25_000, seed=12), scenes.cornell(n=4_000, seed=13)]
+    handles = []
+    for sc in scs:
+        h = N.Scene(ctx, sc.gaussians, sc.parts, emitters=sc.emitters)
+        h.write_colour(np.concatenate([sc.gaussians["albedo"], np.ones((sc.count, 1))], 1).astype(np.float32))
+        handles.append(h)
+    rng = np.random.default_rng(5)
+    sizes = [(64, 64), (160, 120), (320, 180), (256, 256)]
+    reqs = []
+    for k in range(24):
+        si = int(rng.integers(len(scs)))
+        w, h = sizes[int(rng.integers(len(sizes)))]
+        eye = scenes.orbit_eye(scs[si], rng.uniform(2.0, 6.0), rng.uniform(0, 2 * math.pi), rng.uniform(0.1, 0.6))
+        reqs.append((si, w, h, scs[si].frame_bytes(width=w, height=h, eye=eye)))
+    outs = [ctx.alloc(w * h * 16) for _, w, h, _ in reqs]
+    srv = N.Server(ctx, hbm_budget=64 << 20, lanes=2, max_batch_views=6)
+    try:
+        ids = {}
+        t0 = time.perf_counter()
+        for k, (si, w, h, fb) in enumerate(reqs):
+            ids[srv.submit(handles[si], fb, w, h, outs[k])] = k
+            time.sleep(rng.exponential(0.0005))              # arrivals while earlier requests render
+        done = []
+        while len(done) < len(reqs) and time.perf_counter() - t0 < 60:
+            done += srv.poll(wait_us=20_000)
+        assert len(done) == len(reqs)
+        assert all(r.status == 0 for r in done)
+        sizes_run = [r.batch_views for r in done]
+        starts = sorted({(r.start_s, r.done_s) for r in done})
+        overlap = any(b[0] < a[1] for a, b in zip(starts, starts[1:]))
+        log(f"  {len(starts)} batches, views per batch {sorted(set(sizes_run))}, overlapping lanes: {overlap}")
+        assert len(starts) > 1 and max(sizes_run) > 1
+        lat = sorted(r.done_s - r.submit_s for r in done)
+        log(f"  latency p50 {lat[len(lat) // 2] * 1e3:.2f} ms, max {lat[-1] * 1e3:.2f} ms")
+        imgs = [ctx.read(o, w * h * 16, np.uint8) for o, (_, w, h, _) in zip(outs, reqs)]
+    finally:
+        srv.close()
+    try:
+        for r in done:
+            k = ids[r.id]
+            si, w, h, fb = reqs[k]
+            N.frame_encode(ctx, handles[si], fb, w, h, outs[k])
+            st = N.frame_finish(ctx)
+            alone = ctx.read(outs[k], w * h * 16, np.uint8)
+            assert np.array_equal(alone, imgs[k]), f"request {k}: served image differs from the view alone"
+            assert (r.stats.visible, r.stats.pairs) == (st.visible, st.pairs)
+        log(f"  {len(done)} served results identical to the views rendered alone")
+    finally:
+        for o in outs:
+            ctx.free(o)
+        for hd in handles:
+            hd.destroy()
+
+
+def test_server_rejects_bad_requests(ctx):
+    sc = scenes.cornell(n=1000, seed=1)
+    h = N.Scene(ctx, sc.gaussians, sc.parts, emitters=sc.emitters)
+    out = ctx.alloc(64 * 48 * 16)
+    srv = N.Server(ctx, hbm_budget=1 << 30, lanes=1, max_batch_views=4)
+    try:
+        with pytest.raises(N.SplatlightError):
+            srv.submit(h, bytes(100), 64, 48, out)                              # not a RenderFrame
+        with pytest.raises(N.SplatlightError):
+            srv.submit(h, sc.frame_bytes(width=64, height=48), 32, 48, out)    # target size != viewport
+        good = srv.submit(h, sc.frame_bytes(width=64, height=48), 64, 48, out)  # the server goes on
+        got = {}
+        t0 = time.perf_counter()
+        while good not in got and time.perf_counter() - t0 < 30:
+            for r in srv.poll(wait_us=20_000):
+                got[r.id] = r.status
+        assert got.get(good) == 0
+    finally:
+        srv.close()
+        ctx.free(out)
+        h.destroy()
