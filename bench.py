@@ -44,6 +44,7 @@ def parse():
     ap.add_argument("--batched", type=int, default=1, help="also measure the batched form (64 views, 8 scenes)")
     ap.add_argument("--serving", type=int, default=1, help="also measure the served form (Poisson arrivals)")
     ap.add_argument("--rate", type=float, default=1000.0, help="serving arrival rate, requests per second")
+    ap.add_argument("--storage", type=int, default=1, help="also measure f16 / bf16 storage at 4K over 4M Gaussians")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB, 5 times (HBM counter calibration)")
     return ap.parse_args()
@@ -246,6 +247,49 @@ def serving_leg(N, scenes, rate=1000.0, seconds=2.0, budget=4 << 30, lanes=2, ma
             "saturated_speedup": round(span_plain / span_sat, 3)}
 
 
+def storage_leg(N, L, scenes, steps=5):
+    """Reduced-precision storage at 4K: 4M Gaussians of the hill (seed 9), 3840x2160 rgba16f, quat / scale /
+    albedo stored f32, f16 or bf16 (means and opacity f32; all arithmetic f32): relit and cached-lighting frame
+    rates and device bytes per storage."""
+    sc = scenes.hill(n=4_000_000, seed=9, width=3840, height=2160)
+    fb = sc.frame_bytes()
+    W, H = sc.width, sc.height
+    inp = L.lighting_inputs(sc, dirty_all=True)
+    nodes = scenes.leaf_nodes(sc.gaussians)
+    vgrid = scenes.vgrid_open(sc.count)
+    out = {"workload": f"hill {sc.count} Gaussians, {W}x{H} rgba16f, 64 emitters + sun (2048^2 map)"}
+    ctx = N.Context(0)
+    target = ctx.alloc(W * H * 8)
+    for storage, name in ((0, "f32"), (1, "f16"), (2, "bf16")):
+        scene = N.Scene(ctx, sc.gaussians, sc.parts, emitters=sc.emitters, materials=sc.materials, nodes=nodes,
+                        vgrid=vgrid, storage=storage)
+        scene.iso_init()
+
+        def lit():
+            scene.light(inp["U"], inp["EP"], inp["emit_nodes"], inp["LS"], inp["ENV"], inp["SM"], inp["desc"])
+            N.frame_encode(ctx, scene, fb, W, H, target, N.SL_OUT_RGBA16F)
+            return N.frame_finish(ctx)
+
+        st = lit()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            lit()
+        t_lit = (time.perf_counter() - t0) / steps
+        t0 = time.perf_counter()
+        for _ in range(2 * steps):
+            N.frame_encode(ctx, scene, fb, W, H, target, N.SL_OUT_RGBA16F)
+            N.frame_finish(ctx)
+        t_view = (time.perf_counter() - t0) / (2 * steps)
+        out[name] = {"relit_mpix_s": round(W * H / t_lit / 1e6, 1), "relit_ms": round(t_lit * 1e3, 3),
+                     "view_only_mpix_s": round(W * H / t_view / 1e6, 1), "view_only_ms": round(t_view * 1e3, 3),
+                     "pairs": int(st.pairs), "attribute_bytes_per_gaussian": 64 if storage == 0 else 48,
+                     "scene_device_mib": round(scene.device_bytes / 2**20, 1)}
+        scene.destroy()
+    ctx.free(target)
+    ctx.close()
+    return out
+
+
 def kernel_bytes(passes, P, T, target_bytes, ls):
     """Per-launch algorithmic HBM bytes and HIP-event milliseconds of the two heaviest kernels of a relit frame.
 
@@ -434,6 +478,8 @@ def main():
         res["batched"] = batched_leg(N, L, scenes)
     if a.serving:
         res["serving"] = serving_leg(N, scenes, rate=a.rate)
+    if a.storage:
+        res["storage_4k"] = storage_leg(N, L, scenes)
     if a.cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(sc, fb)
     print(json.dumps(res))

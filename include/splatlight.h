@@ -52,6 +52,10 @@ sl_status sl_buf_write(sl_context *ctx, void *dev, size_t offset, const void *ho
 sl_status sl_buf_read(sl_context *ctx, const void *dev, size_t offset, void *host, size_t bytes);
 
 /* ---- scene: the asset records the renderer and lighting upload (src/renderer.js:54-62, src/asset.js:8-16) --- */
+/* Device storage of the Gaussian attributes. F32: the records' values. F16 / BF16: quaternion, scale and albedo
+ * rounded to half / bfloat16 (nearest even) at sl_scene_create, means, opacity (kappa), part and flags kept exact:
+ * 48 B per Gaussian on the device instead of 64. All arithmetic stays f32 on the stored values, and depth keys
+ * depend on the means only, so they are the f32 scene's. */
 #define SL_STORAGE_F32 0u
 #define SL_STORAGE_F16 1u
 #define SL_STORAGE_BF16 2u

@@ -234,11 +234,11 @@ using namespace lt;
 // ---- iso_init (runtime/wgsl/iso_init.wgsl:29-56) ---------------------------------------------------------------
 
 extern "C" __global__ void __launch_bounds__(64)
-sl_iso_init(uint32_t n, const float4 *__restrict__ g2, const float4 *__restrict__ nodes,
+sl_iso_init(uint32_t n, const float4 *__restrict__ g2, uint32_t st, const float4 *__restrict__ nodes,
             const uint8_t *__restrict__ vgrid, float *__restrict__ iso_norm) {
   const uint32_t g = blockIdx.x * 64u + threadIdx.x;
   if (g >= n) return;
-  const float4 s = g2[g];
+  const float4 s = sl_scale(g2, st, g);
   const NodeL nd = load_node(nodes, g);
   const float s3 = s.x * s.y * s.z;
   const uint8_t *vg = vgrid + 128 * (size_t)g;
@@ -264,12 +264,13 @@ sl_iso_init(uint32_t n, const float4 *__restrict__ g2, const float4 *__restrict_
 extern "C" __global__ void __launch_bounds__(64)
 sl_shade_emit(FrameUniformsDev U, EmitParamsDev EP, const float4 *__restrict__ emit_nodes, uint32_t n_nodes,
               const float4 *__restrict__ g0, const float4 *__restrict__ g1, const float4 *__restrict__ g2,
-              const float4 *__restrict__ g3, const float4 *__restrict__ nodes, const uint8_t *__restrict__ vgrid,
-              const float4 *__restrict__ parts, const float *__restrict__ lamp_T, const uint32_t *__restrict__ materials,
-              const float *__restrict__ iso_norm, float4 *__restrict__ colour) {
+              const float4 *__restrict__ g3, uint32_t st, const float4 *__restrict__ nodes,
+              const uint8_t *__restrict__ vgrid, const float4 *__restrict__ parts, const float *__restrict__ lamp_T,
+              const uint32_t *__restrict__ materials, const float *__restrict__ iso_norm,
+              float4 *__restrict__ colour) {
   const uint32_t g = blockIdx.x * 64u + threadIdx.x;
   if (g >= U.gaussian_count) return;
-  const float4 A = g0[g], Q = g1[g], S = g2[g], C = g3[g];
+  const float4 A = g0[g], Q = sl_quat(g1, st, g), S = sl_scale(g2, st, g), C = sl_albedo(g1, g2, g3, st, g);
   const uint32_t mi = __float_as_uint(C.w) & 0xFFFFu;
   const uint32_t mmax = max(U.material_count, 1u) - 1u;
   const uint32_t *mat = materials + 8 * (size_t)min(mi, mmax);
@@ -633,9 +634,9 @@ __device__ __forceinline__ m2 oct_jac(const LSParamsDev &LS, v3 w, v3 b0, v3 b1)
 
 // the emitter map record of Gaussian g (its own part's layer); z = r + gz . dt
 __device__ MapRec lsm_rec(uint32_t g, const LSParamsDev &LS, const float4 *__restrict__ g0,
-                          const float4 *__restrict__ g1, const float4 *__restrict__ g2,
+                          const float4 *__restrict__ g1, const float4 *__restrict__ g2, uint32_t st,
                           const float4 *__restrict__ nodes, const float4 *__restrict__ parts) {
-  const float4 A = g0[g], Q = g1[g], S4 = g2[g];
+  const float4 A = g0[g], Q = sl_quat(g1, st, g), S4 = sl_scale(g2, st, g);
   const uint32_t p = __float_as_uint(S4.w) & 0xFFFFu;
   if (p >= LS.part_count || p == LS.skip_part || (LS.dirty & (1u << p)) == 0u) return empty_rec();
   const NodeL nd = load_node(nodes, g);
@@ -688,11 +689,11 @@ __device__ MapRec lsm_rec(uint32_t g, const LSParamsDev &LS, const float4 *__res
 
 extern "C" __global__ void __launch_bounds__(64)
 sl_lsm_prep(uint32_t n, LSParamsDev LS, const float4 *__restrict__ g0, const float4 *__restrict__ g1,
-            const float4 *__restrict__ g2, const float4 *__restrict__ nodes, const float4 *__restrict__ parts,
-            MapRec *__restrict__ recs, uint32_t *__restrict__ tile_count) {
+            const float4 *__restrict__ g2, uint32_t st, const float4 *__restrict__ nodes,
+            const float4 *__restrict__ parts, MapRec *__restrict__ recs, uint32_t *__restrict__ tile_count) {
   const uint32_t g = blockIdx.x * 64u + threadIdx.x;
   if (g >= n) return;
-  const MapRec r = lsm_rec(g, LS, g0, g1, g2, nodes, parts);
+  const MapRec r = lsm_rec(g, LS, g0, g1, g2, st, nodes, parts);
   recs[g] = r;
   const uint32_t tiles_x = (LS.res + MT - 1) / MT;
   count_tiles(r, tiles_x, tiles_x * tiles_x, tile_count);
@@ -700,10 +701,11 @@ sl_lsm_prep(uint32_t n, LSParamsDev LS, const float4 *__restrict__ g0, const flo
 
 extern "C" __global__ void __launch_bounds__(64)
 sl_lsm_lookup(uint32_t n, LSParamsDev LS, const float4 *__restrict__ g0, const float4 *__restrict__ g2,
-              const float4 *__restrict__ parts, const float4 *__restrict__ mip, float *__restrict__ T_out) {
+              uint32_t st, const float4 *__restrict__ parts, const float4 *__restrict__ mip,
+              float *__restrict__ T_out) {
   const uint32_t g = blockIdx.x * 64u + threadIdx.x;
   if (g >= n) return;
-  const float4 A = g0[g], S4 = g2[g];
+  const float4 A = g0[g], S4 = sl_scale(g2, st, g);
   const uint32_t own = __float_as_uint(S4.w) & 0xFFFFu;
   const v3 P = xf_point(load_xf(parts, own), V3(A.x, A.y, A.z));
   const v3 center = V3(LS.center[0], LS.center[1], LS.center[2]);
@@ -778,12 +780,12 @@ sl_sun_frame(SunParamsDev SM, const SunLayerDesc *__restrict__ desc, const float
 extern "C" __global__ void __launch_bounds__(64)
 sl_sun_prep(uint32_t n, SunParamsDev SM, const SunLayerDesc *__restrict__ desc, const SunLayer *__restrict__ layers,
             const float4 *__restrict__ g0, const float4 *__restrict__ g1, const float4 *__restrict__ g2,
-            const float4 *__restrict__ nodes, const float4 *__restrict__ parts, uint32_t max_layers,
+            uint32_t st, const float4 *__restrict__ nodes, const float4 *__restrict__ parts, uint32_t max_layers,
             MapRec *__restrict__ recs, uint32_t *__restrict__ tile_count) {
   const uint32_t g = blockIdx.x * 64u + threadIdx.x;
   if (g >= n) return;
   MapRec *out = recs + (size_t)g * max_layers;
-  const float4 A = g0[g], Q = g1[g], S4 = g2[g];
+  const float4 A = g0[g], Q = sl_quat(g1, st, g), S4 = sl_scale(g2, st, g);
   const uint32_t p = __float_as_uint(S4.w) & 0xFFFFu;
   const uint32_t first = p < 32u ? SM.part_layers[2 * p] : 0u, cnt = p < 32u ? min(SM.part_layers[2 * p + 1], max_layers) : 0u;
   for (uint32_t j = cnt; j < max_layers; j++) out[j] = empty_rec();
@@ -844,11 +846,11 @@ sl_sun_prep(uint32_t n, SunParamsDev SM, const SunLayerDesc *__restrict__ desc, 
 
 extern "C" __global__ void __launch_bounds__(64)
 sl_sun_lookup(uint32_t n, SunParamsDev SM, const SunLayerDesc *__restrict__ desc, const SunLayer *__restrict__ layers,
-              const float4 *__restrict__ g0, const float4 *__restrict__ g2, const float4 *__restrict__ parts,
-              const float4 *__restrict__ mip, float4 *__restrict__ env_T) {
+              const float4 *__restrict__ g0, const float4 *__restrict__ g2, uint32_t st,
+              const float4 *__restrict__ parts, const float4 *__restrict__ mip, float4 *__restrict__ env_T) {
   const uint32_t g = blockIdx.x * 64u + threadIdx.x;
   if (g >= n) return;
-  const float4 A = g0[g], S4 = g2[g];
+  const float4 A = g0[g], S4 = sl_scale(g2, st, g);
   const uint32_t own = __float_as_uint(S4.w) & 0xFFFFu;
   const v3 P = xf_point(load_xf(parts, own), V3(A.x, A.y, A.z));
   float T[4] = {1.0f, 1.0f, 1.0f, 1.0f};
@@ -889,13 +891,13 @@ sl_sun_lookup(uint32_t n, SunParamsDev SM, const SunLayerDesc *__restrict__ desc
 // ---- env_direct (environment.wgsl:207-229, 266-277), matte -------------------------------------------------------
 
 extern "C" __global__ void __launch_bounds__(64)
-sl_env_direct(FrameUniformsDev U, EnvDev ENV, const float4 *__restrict__ g2, const float4 *__restrict__ g3,
-              const float4 *__restrict__ nodes, const uint8_t *__restrict__ vgrid, const float4 *__restrict__ parts,
+sl_env_direct(FrameUniformsDev U, EnvDev ENV, const float4 *__restrict__ g1, const float4 *__restrict__ g2,
+              const float4 *__restrict__ g3, uint32_t st, const float4 *__restrict__ nodes, const uint8_t *__restrict__ vgrid, const float4 *__restrict__ parts,
               const uint32_t *__restrict__ materials, const float *__restrict__ iso_norm,
               const float4 *__restrict__ env_T, float4 *__restrict__ colour) {
   const uint32_t g = blockIdx.x * 64u + threadIdx.x;
   if (g >= U.gaussian_count || ENV.info[0] == 0u || (U.iso & 1u) == 0u) return;
-  const float4 S4 = g2[g], C = g3[g];
+  const float4 S4 = sl_scale(g2, st, g), C = sl_albedo(g1, g2, g3, st, g);
   const uint32_t mi = __float_as_uint(C.w) & 0xFFFFu;
   const uint32_t mmax = max(U.material_count, 1u) - 1u;
   if (U.material_count && (materials[8 * (size_t)min(mi, mmax) + 5] & 1u)) return;

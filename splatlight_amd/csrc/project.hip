@@ -65,7 +65,8 @@ sl_project_kernel(const ViewDev *__restrict__ views, const uint32_t *__restrict_
       mu[0] = A.x; mu[1] = A.y; mu[2] = A.z;
       quat_to_mat(Q4, rot);
     } else {                                                          // pose (project.wgsl:128-148)
-      A = g0[i]; Q4 = g1[i]; S4 = g2[i]; C4 = g3[i];
+      A = g0[i]; Q4 = sl_quat(g1, vd.storage, i); S4 = sl_scale(g2, vd.storage, i);
+      C4 = sl_albedo(g1, g2, g3, vd.storage, i);
       uint32_t part = __float_as_uint(S4.w) & 0xffffu;
       part = part < SL_MAX_PARTS - 1u ? part : SL_MAX_PARTS - 1u;
       const float4 x0 = parts[part * 8 + 0], x1 = parts[part * 8 + 1], x2 = parts[part * 8 + 2];

@@ -1,6 +1,7 @@
 // Device-side declarations shared by the splat-path kernels (gfx950, wave64).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
 #include <stdint.h>
 
 #define SL_INVALID_KEY 0xffffffffu
@@ -28,6 +29,35 @@ struct FrameDev {
   uint32_t width, height, tiles_x, tiles_y;
 };
 
+// Gaussian attribute storage (SL_STORAGE_*). F32: four float4 streams g0 (mu, kappa), g1 quat, g2 (scale,
+// part_surface bits), g3 (albedo, flags bits). F16 / BF16: g0 unchanged (means and opacity stay f32), g1 = uint4
+// {quat.x | quat.y << 16, quat.z | quat.w << 16, albedo.r | albedo.g << 16, albedo.b}, g2 = uint4 {scale.x |
+// scale.y << 16, scale.z, part_surface, flags}, halves rounded to nearest even on the host; g3 unused. Decoding is
+// exact, so every kernel computes in f32 on the stored values.
+#define SL_ST_F32 0u
+#define SL_ST_BF16 2u
+__device__ __forceinline__ float sl_h2f(uint32_t b, uint32_t st) {
+  return st == SL_ST_BF16 ? __uint_as_float((b & 0xffffu) << 16)
+                          : __half2float(__ushort_as_half((unsigned short)(b & 0xffffu)));
+}
+__device__ __forceinline__ float4 sl_quat(const float4 *g1, uint32_t st, uint32_t i) {
+  if (st == SL_ST_F32) return g1[i];
+  const uint4 h = reinterpret_cast<const uint4 *>(g1)[i];
+  return make_float4(sl_h2f(h.x, st), sl_h2f(h.x >> 16, st), sl_h2f(h.y, st), sl_h2f(h.y >> 16, st));
+}
+__device__ __forceinline__ float4 sl_scale(const float4 *g2, uint32_t st, uint32_t i) {   // .w: part_surface bits
+  if (st == SL_ST_F32) return g2[i];
+  const uint4 h = reinterpret_cast<const uint4 *>(g2)[i];
+  return make_float4(sl_h2f(h.x, st), sl_h2f(h.x >> 16, st), sl_h2f(h.y, st), __uint_as_float(h.z));
+}
+__device__ __forceinline__ float4 sl_albedo(const float4 *g1, const float4 *g2, const float4 *g3, uint32_t st,
+                                            uint32_t i) {                                // .w: flags bits
+  if (st == SL_ST_F32) return g3[i];
+  const uint4 h1 = reinterpret_cast<const uint4 *>(g1)[i];
+  const uint32_t fl = reinterpret_cast<const uint4 *>(g2)[i].w;
+  return make_float4(sl_h2f(h1.z, st), sl_h2f(h1.z >> 16, st), sl_h2f(h1.w, st), __uint_as_float(fl));
+}
+
 // One view of a batch: its frame, its scene's streams and its target. Views lie back to back in the global slot
 // space (each view's slots start at a multiple of 256, so a 256-slot block belongs to one view) and in the global
 // tile space; a single frame is a batch of one view.
@@ -38,7 +68,8 @@ struct ViewDev {
   const float4 *parts, *colour, *lo_b;
   void *out;                                // width * height pixels of out_format
   uint32_t out_format;
-  uint32_t slot_base, tile_base, pad;
+  uint32_t slot_base, tile_base;
+  uint32_t storage;                         // SL_STORAGE_* of the scene's attribute streams
 };
 
 // Per-view results of a batch.

@@ -613,6 +613,50 @@ extern "C" sl_status sl_buf_read(sl_context *ctx, const void *dev, size_t offset
 
 namespace {
 
+// f32 -> 16-bit storage, round to nearest even (NaN stays NaN, overflow goes to infinity)
+uint16_t to_half_bits(float f) {
+  const _Float16 h = (_Float16)f;
+  uint16_t b;
+  memcpy(&b, &h, 2);
+  return b;
+}
+uint16_t to_bf16_bits(float f) {
+  uint32_t x;
+  memcpy(&x, &f, 4);
+  if ((x & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((x >> 16) | 0x40u);
+  return (uint16_t)((x + 0x7fffu + ((x >> 16) & 1u)) >> 16);
+}
+
+// F16 / BF16 scenes: g0 = (mu, kappa) f32; g1 = {quat halves, albedo halves}; g2 = {scale halves, part_surface,
+// flags}; packed on the host from the 64 B records (words: mu 0-2, kappa 3, quat 4-7, scale 8-10, part_surface 11,
+// albedo 12-14, flags 15).
+sl_status upload_packed(sl_context *ctx, sl_scene *sc, const void *aos, uint32_t n, uint32_t storage, float4 *dst[4]) {
+  auto cv = [&](const uint8_t *r, int word) -> uint32_t {
+    const float f = rd_f32(r, word);
+    return storage == SL_STORAGE_BF16 ? to_bf16_bits(f) : to_half_bits(f);
+  };
+  std::vector<float4> g0(n);
+  std::vector<uint4> h1(n), h2(n);
+  for (uint32_t i = 0; i < n; i++) {
+    const uint8_t *r = (const uint8_t *)aos + 64 * (size_t)i;
+    g0[i] = make_float4(rd_f32(r, 0), rd_f32(r, 1), rd_f32(r, 2), rd_f32(r, 3));
+    h1[i] = make_uint4(cv(r, 4) | cv(r, 5) << 16, cv(r, 6) | cv(r, 7) << 16, cv(r, 12) | cv(r, 13) << 16, cv(r, 14));
+    h2[i] = make_uint4(cv(r, 8) | cv(r, 9) << 16, cv(r, 10), rd_u32(r, 11), rd_u32(r, 15));
+  }
+  const void *src[3] = {g0.data(), h1.data(), h2.data()};
+  for (int k = 0; k < 3; k++) {
+    void *p = nullptr;
+    sl_status s = dev_alloc(ctx, (size_t)std::max(n, 1u) * 16, &p);
+    if (s != SL_OK) return s;
+    sc->allocs.push_back(p);
+    sc->bytes += (size_t)std::max(n, 1u) * 16;
+    dst[k] = (float4 *)p;
+    if (n) SL_HIP(ctx, hipMemcpy(p, src[k], (size_t)n * 16, hipMemcpyHostToDevice));
+  }
+  dst[3] = nullptr;
+  return SL_OK;
+}
+
 sl_status upload_soa(sl_context *ctx, sl_scene *sc, const void *aos, uint32_t n, float4 *dst[4]) {
   for (int k = 0; k < 4; k++) {
     void *p = nullptr;
@@ -710,7 +754,8 @@ extern "C" sl_status sl_scene_create(sl_context *ctx, const sl_scene_desc *d, sl
   sc->n_emit = d->emitter_count;
   sc->material_count = d->materials ? d->material_count : 0;
   sc->storage = d->storage;
-  sl_status s = upload_soa(ctx, sc, d->gaussians, sc->count, sc->g);
+  sl_status s = sc->storage == SL_STORAGE_F32 ? upload_soa(ctx, sc, d->gaussians, sc->count, sc->g)
+                                              : upload_packed(ctx, sc, d->gaussians, sc->count, sc->storage, sc->g);
   if (s == SL_OK) s = upload_soa(ctx, sc, d->emitters, sc->n_emit, sc->e);
   void *p = nullptr;
   if (s == SL_OK && (s = scene_alloc(ctx, sc, SL_MAX_PARTS * 128, &p)) == SL_OK) sc->parts = (float4 *)p;
@@ -844,6 +889,7 @@ sl_status encode_batch(sl_context *ctx, const sl_view *views, uint32_t n_views, 
     vd.g0 = sc->g[0]; vd.g1 = sc->g[1]; vd.g2 = sc->g[2]; vd.g3 = sc->g[3];
     vd.e0 = sc->e[0]; vd.e1 = sc->e[1]; vd.e2 = sc->e[2]; vd.e3 = sc->e[3];
     vd.parts = sc->parts; vd.colour = sc->colour; vd.lo_b = sc->lo_b;
+    vd.storage = sc->storage;
     vd.out = in.out_device;
     vd.out_format = in.out_format;
     vd.slot_base = (uint32_t)slots;
@@ -991,7 +1037,8 @@ extern "C" sl_status sl_lighting_iso_init(sl_context *ctx, sl_scene *sc) {
   if (!sc) return SL_ERR_INVALID;
   if (!sc->nodes) return fail(ctx, SL_ERR_INVALID, "relighting needs the scene's leaf nodes");
   SL_HIP(ctx, hipSetDevice(ctx->device));
-  sl_iso_init<<<(sc->count + 63) / 64, 64, 0, ctx->stream>>>(sc->count, sc->g[2], sc->nodes, sc->vgrid, sc->iso_norm);
+  sl_iso_init<<<(sc->count + 63) / 64, 64, 0, ctx->stream>>>(sc->count, sc->g[2], sc->storage, sc->nodes, sc->vgrid,
+                                                               sc->iso_norm);
   sl_status s = check_launch(ctx, "iso_init");
   if (s != SL_OK) return s;
   SL_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -1138,21 +1185,22 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
           sl_fill_u32<<<grid_for(texels * 3, 256, 4096), 256, 0, st>>>((uint32_t *)sc->lsm_acc.p + texels * 3 * p, 0u,
                                                                        (uint32_t)(texels * 3));
       if (sc->count)
-        sl_lsm_prep<<<gblocks, 64, 0, st>>>(sc->count, LS, sc->g[0], sc->g[1], sc->g[2], sc->nodes, sc->parts,
-                                            (MapRec *)sc->map_rec.p, (uint32_t *)sc->map_cnt.p);
+        sl_lsm_prep<<<gblocks, 64, 0, st>>>(sc->count, LS, sc->g[0], sc->g[1], sc->g[2], sc->storage, sc->nodes,
+                                            sc->parts, (MapRec *)sc->map_rec.p, (uint32_t *)sc->map_cnt.p);
       run_map_splat(ctx, sc, 0, sc->count, LS.part_count, LS.res, 65536.0f, (uint32_t *)sc->lsm_acc.p, timers);
       if ((s = run_opacity_maps(ctx, sc, (float4 *)sc->lsm_mip.p, LS.part_count, LS.res, LS.levels, LS.level_off,
                                 LS.layer_texels, 65536.0f, (const uint32_t *)sc->lsm_acc.p,
                                 (const uint32_t *)sc->lsm_dirty.p)) != SL_OK)
         return s;
     }
-    sl_lsm_lookup<<<gblocks, 64, 0, st>>>(sc->count, LS, sc->g[0], sc->g[2], sc->parts, (const float4 *)sc->lsm_mip.p,
-                                          sc->lamp_T);
+    sl_lsm_lookup<<<gblocks, 64, 0, st>>>(sc->count, LS, sc->g[0], sc->g[2], sc->storage, sc->parts,
+                                          (const float4 *)sc->lsm_mip.p, sc->lamp_T);
     lampT = sc->lamp_T;
   }
   if (timers) SL_HIP(ctx, hipEventRecord(ctx->lev[1], st));
-  sl_shade_emit<<<gblocks, 64, 0, st>>>(U, EP, (const float4 *)sc->emit_nodes.p, d->emit_node_count, sc->g[0], sc->g[1], sc->g[2], sc->g[3],
-                                        sc->nodes, sc->vgrid, sc->parts, lampT, sc->materials, sc->iso_norm, sc->colour);
+  sl_shade_emit<<<gblocks, 64, 0, st>>>(U, EP, (const float4 *)sc->emit_nodes.p, d->emit_node_count, sc->g[0],
+                                        sc->g[1], sc->g[2], sc->g[3], sc->storage, sc->nodes, sc->vgrid, sc->parts,
+                                        lampT, sc->materials, sc->iso_norm, sc->colour);
   if (timers) SL_HIP(ctx, hipEventRecord(ctx->lev[2], st));
   // distant lights: sun maps -> env_T, then env_direct
   if (d->env) {
@@ -1207,7 +1255,7 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
         if (n_rec)
           sl_sun_prep<<<gblocks, 64, 0, st>>>(sc->count, SM, (const SunLayerDesc *)sc->sun_desc.p,
                                               (const SunLayer *)sc->sun_layers.p, sc->g[0], sc->g[1], sc->g[2],
-                                              sc->nodes, sc->parts, max_layers, (MapRec *)sc->map_rec.p,
+                                              sc->storage, sc->nodes, sc->parts, max_layers, (MapRec *)sc->map_rec.p,
                                               (uint32_t *)sc->map_cnt.p);
         run_map_splat(ctx, sc, 1, n_rec, SM.n_layers, SM.res, 4096.0f, (uint32_t *)sc->sun_acc.p, timers);
         if ((s = run_opacity_maps(ctx, sc, (float4 *)sc->sun_mip.p, SM.n_layers, SM.res, SM.levels, SM.level_off,
@@ -1216,8 +1264,8 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
           return s;
       }
       sl_sun_lookup<<<gblocks, 64, 0, st>>>(sc->count, SM, (const SunLayerDesc *)sc->sun_desc.p,
-                                            (const SunLayer *)sc->sun_layers.p, sc->g[0], sc->g[2], sc->parts,
-                                            (const float4 *)sc->sun_mip.p, sc->env_T);
+                                            (const SunLayer *)sc->sun_layers.p, sc->g[0], sc->g[2], sc->storage,
+                                            sc->parts, (const float4 *)sc->sun_mip.p, sc->env_T);
       envT = sc->env_T;
     } else {
       sl_fill_u32<<<grid_for((size_t)sc->count * 4, 256, 4096), 256, 0, st>>>((uint32_t *)sc->env_T, 0x3f800000u,
@@ -1225,8 +1273,8 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
       envT = sc->env_T;
     }
     if (timers) SL_HIP(ctx, hipEventRecord(ctx->lev[3], st));
-    sl_env_direct<<<gblocks, 64, 0, st>>>(U, ENV, sc->g[2], sc->g[3], sc->nodes, sc->vgrid, sc->parts, sc->materials,
-                                          sc->iso_norm, envT, sc->colour);
+    sl_env_direct<<<gblocks, 64, 0, st>>>(U, ENV, sc->g[1], sc->g[2], sc->g[3], sc->storage, sc->nodes, sc->vgrid,
+                                          sc->parts, sc->materials, sc->iso_norm, envT, sc->colour);
   } else if (timers) {
     SL_HIP(ctx, hipEventRecord(ctx->lev[3], st));
   }

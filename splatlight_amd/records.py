@@ -204,3 +204,27 @@ def identity_part(g_lo: int, g_hi: int, bound=(0.0, 0.0, 0.0, 1.0), flags=0) -> 
     p["g_hi"][0] = g_hi
     p["flags"][0] = flags
     return p
+
+
+STORAGE_F32, STORAGE_F16, STORAGE_BF16 = 0, 1, 2
+
+
+def _bf16(x: np.ndarray) -> np.ndarray:
+    """float32 -> bfloat16 -> float32, rounding to nearest even (NaN kept)."""
+    b = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    r = ((b + 0x7FFF + ((b >> 16) & 1)) >> 16) << 16
+    nan = (b & 0x7FFFFFFF) > 0x7F800000
+    r = np.where(nan, ((b >> 16) | 0x40) << 16, r)
+    return r.astype(np.uint32).view(np.float32)
+
+
+def dequantized(g: np.ndarray, storage: int) -> np.ndarray:
+    """The Gaussian records as a scene of the given storage holds them: quat, scale and albedo rounded to half or
+    bfloat16 (nearest even) and back; mu, kappa, part_surface and flags unchanged."""
+    if storage == STORAGE_F32:
+        return g
+    out = g.copy()
+    for f in ("quat", "scale", "albedo"):
+        v = np.ascontiguousarray(g[f], np.float32)
+        out[f] = v.astype(np.float16).astype(np.float32) if storage == STORAGE_F16 else _bf16(v)
+    return out
