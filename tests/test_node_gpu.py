@@ -1,0 +1,58 @@
+"""A lit frame through the N-API addon and its JS shim (SplatRenderer / Lighting as the reference's host calls them)
+on the GPU: the bytes of its rgba16f target equal those of the same frame rendered through ctypes, and its
+counts are the same."""
+import json
+import os
+import shutil
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from splatlight_amd import build as B
+from splatlight_amd import lighting as L
+from splatlight_amd import native as N
+from splatlight_amd import scenes
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+NODE = shutil.which("node")
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(NODE is None or not os.path.exists(B.NODE_ADDON), reason="no node / addon")]
+
+
+def test_js_host_frame_equals_python_frame(ctx, tmp_path):
+    sc = scenes.cornell(n=8_000, seed=21, width=200, height=150)
+    nd = scenes.leaf_nodes(sc.gaussians)
+    inp = L.lighting_inputs(sc)
+    fb = sc.frame_bytes()
+    files = {"gaussians.bin": sc.gaussians, "parts.bin": sc.parts, "materials.bin": sc.materials, "nodes.bin": nd,
+             "emitters.bin": sc.emitters, "frame.bin": fb, "U.bin": inp["U"], "EP.bin": inp["EP"],
+             "emit_nodes.bin": np.ascontiguousarray(inp["emit_nodes"], np.float32), "LS.bin": inp["LS"]}
+    for k in ("ENV", "SM", "desc"):
+        if inp[k] is not None:
+            files[f"{k}.bin"] = inp[k]
+    for name, v in files.items():
+        (tmp_path / name).write_bytes(v if isinstance(v, (bytes, bytearray)) else np.ascontiguousarray(v).tobytes())
+    (tmp_path / "meta.json").write_text(json.dumps({"width": sc.width, "height": sc.height}))
+    print(f"node frame: {sc.count} Gaussians {sc.width}x{sc.height}", file=sys.stderr, flush=True)
+    r = subprocess.run([NODE, os.path.join(ROOT, "tests", "node_frame.js"), str(tmp_path)], cwd=ROOT,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    js_img = (tmp_path / "hdr.bin").read_bytes()
+    js = json.loads((tmp_path / "stats.json").read_text())
+    assert set(js["timers"]) >= {"project", "radix", "pairs", "tilesort", "ranges", "splat"}
+    scene = N.Scene(ctx, sc.gaussians, sc.parts, emitters=sc.emitters, materials=sc.materials, nodes=nd)
+    out = ctx.alloc(sc.width * sc.height * 8)
+    try:
+        scene.light(inp["U"], inp["EP"], inp["emit_nodes"], inp["LS"], inp["ENV"], inp["SM"], inp["desc"])
+        N.frame_encode(ctx, scene, fb, sc.width, sc.height, out, N.SL_OUT_RGBA16F)
+        st = N.frame_finish(ctx)
+        py_img = ctx.read(out, sc.width * sc.height * 8, np.uint8).tobytes()
+    finally:
+        ctx.free(out)
+        scene.destroy()
+    assert (js["stats"]["visible"], js["stats"]["pairs"]) == (st.visible, st.pairs)
+    assert js_img == py_img
+    print(f"  JS host: visible={st.visible} pairs={st.pairs}, target bytes identical to the ctypes path",
+          file=sys.stderr, flush=True)
