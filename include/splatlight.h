@@ -207,8 +207,8 @@ sl_status sl_lighting_encode(sl_context *ctx, sl_scene *scene, const sl_lighting
 #define SL_SCENE_SUN_MIP 8u    /* vec4 [layers][layer texels] */
 sl_status sl_scene_read(sl_context *ctx, sl_scene *scene, uint32_t which, void *host, size_t bytes, size_t *written);
 
-/* Sizes of the last relighting's opacity-map splats (binned records and pairs), for throughput figures. No
- * counterpart in the reference (its splats are one atomic loop per Gaussian). */
+/* Sizes of the last relighting's opacity-map splats on this context (binned records and pairs), for throughput
+ * figures. No counterpart in the reference (its splats are one atomic loop per Gaussian). */
 typedef struct sl_lighting_stats {
   uint32_t lsm_records, lsm_pairs, lsm_tiles, lsm_tiles_used;   /* emitter map: record slots, (record, tile) */
   uint32_t sun_records, sun_pairs, sun_tiles, sun_tiles_used;   /* sun maps; tiles: all, with records */

@@ -453,7 +453,7 @@ void or_lsm_splat(const void *U, const void *LS, const void *gaussians, const vo
       for (int x = (cx - hx > 0 ? cx - hx : 0); x <= (cx + hx < ires - 1 ? cx + hx : ires - 1); x++) {
         v2 dt = V2(((float)x + 0.5f) - ct.x, ((float)y + 0.5f) - ct.y);
         float q = dot2(dt, m2v(Si, dt));
-        if (q > 9.0f) continue;
+        if (!(q <= 9.0f)) continue;   /* outside 3 sigma, or a NaN footprint: adds nothing */
         float t = tau0 * exp_map(-0.5f * q);
         t = t < 64.0f ? t : 64.0f;
         float t_fix = t * FIX;
@@ -613,7 +613,7 @@ void or_sun_splat(const void *U, const void *SM, const void *desc, const float *
         for (int xx = (cx - hx > 0 ? cx - hx : 0); xx <= (cx + hx < ires - 1 ? cx + hx : ires - 1); xx++) {
           v2 dt = V2(((float)xx + 0.5f) - ct.x, ((float)y + 0.5f) - ct.y);
           float q = dot2(dt, m2v(Si, dt));
-          if (q > 9.0f) continue;
+          if (!(q <= 9.0f)) continue;   /* outside 3 sigma, or a NaN footprint: adds nothing */
           float t = tau0 * exp_map(-0.5f * q);
           t = t < 64.0f ? t : 64.0f;
           float t_fix = t * FIX;

@@ -39,15 +39,18 @@ __device__ __forceinline__ float mixf(float a, float b, float t) { return a * (1
 __device__ __forceinline__ float log2_cr(float x) { return (float)log2((double)x); }
 __device__ __forceinline__ float asin_cr(float x) { return (float)asin((double)x); }
 __device__ __forceinline__ float acos_cr(float x) { return (float)acos((double)x); }
-__device__ __forceinline__ uint32_t u32_sat(float f) {   // WGSL u32(f32): truncate, saturate; NaN -> 0
-  const float c = fminf(fmaxf(f, 0.0f), 4294967040.0f);   // the largest float below 2^32
-  return f >= 4294967296.0f ? 0xFFFFFFFFu : (uint32_t)c;
+// WGSL u32(f32): truncate toward zero, saturate to [0, 2^32 - 1], NaN -> 0, which is exactly what v_cvt_u32_f32
+// does (C++'s conversion leaves out-of-range values undefined, so the compiler would add range checks)
+__device__ __forceinline__ uint32_t u32_sat(float f) {
+  uint32_t r;
+  asm("v_cvt_u32_f32 %0, %1" : "=v"(r) : "v"(f));
+  return r;
 }
 // e^x for the opacity-map splats: Cody-Waite reduction by ln 2 (split in two, fused) and the degree-7 Taylor
 // polynomial on [-ln2/2, ln2/2] in Horner form with explicit fused multiply-adds, so every IEEE operation is
-// the same on any host or device that has fma; within 2 ulp of e^x. x is clamped to [-86, 88] (normal results).
+// the same on any host or device that has fma; within 2 ulp of e^x. x is clamped to [-86, 88] (normal results);
+// the splats never pass NaN (a NaN footprint value is skipped before).
 __device__ __forceinline__ float exp_map(float x) {
-  if (!(x == x)) return x;
   x = fminf(fmaxf(x, -86.0f), 88.0f);
   const float k = rintf(x * 1.44269504f);
   float r = fmaf(k, -0.693145751953125f, x);
@@ -145,8 +148,10 @@ __device__ __forceinline__ float footprint(const NodeL &nd, float detC, v3 w) { 
 __device__ __forceinline__ float vgrid_cell(const uint8_t *v, uint32_t i, uint32_t j) {
   return (float)v[i * 16u + j] * (1.0f / 255.0f);
 }
+// bilinear between cell centres, so continuous across cell boundaries: the float asin's last-ulp behaviour moves V
+// by rounding only (the host restatement rounds asin once from double)
 __device__ __forceinline__ float vgrid_lookup(const uint8_t *v, v3 w) {   // common.wgsl:162-177
-  const float el = asin_cr(clampf_(w.z, -1.0f, 1.0f));
+  const float el = asinf(clampf_(w.z, -1.0f, 1.0f));
   const float fi = clampf_((0.5f * PI_F - el) / (PI_F / 8.0f) - 0.5f, 0.0f, 7.0f);
   const uint32_t i0 = min((uint32_t)fi, 6u);
   const float ti = fi - (float)i0;
@@ -394,8 +399,9 @@ sl_opacity_down(float4 *__restrict__ mip, uint32_t n_layers, uint32_t rk, uint32
 //
 // The reference splats each Gaussian with one thread looping over its texel box and three u32 atomics per texel
 // (light_shadow.wgsl:191-286, sun_map.wgsl:127-219). Here a prep pass turns each (Gaussian, layer) into a 64 B
-// record, the records are binned by 16x16 map tile (count, scan, fill), and one workgroup per tile sums its
-// records' fixed-point moments in registers, one lane per texel, and writes the tile once. The per-texel
+// record and its box of 16x16 map tiles, the frame path's pair emission and onesweep sort bin the (tile, record)
+// pairs by tile without contended atomics, and workgroups sum each tile's records' fixed-point moments in
+// registers, one lane per texel, writing the tile once. The per-texel
 // arithmetic is the reference's, unchanged; u32 sums are order-independent, so the maps are bit-identical to
 // the atomic form whatever order the records arrive in.
 
@@ -417,73 +423,14 @@ __device__ __forceinline__ MapRec empty_rec() {
   return r;
 }
 
-__device__ __forceinline__ void count_tiles(const MapRec &r, uint32_t tiles_x, uint32_t tpl,
-                                            uint32_t *__restrict__ tile_count) {
-  if (r.x0 > r.x1 || r.y0 > r.y1) return;
-  const uint32_t base = r.layer * tpl;
-  for (int ty = r.y0 / MT; ty <= r.y1 / MT; ty++)
-    for (int tx = r.x0 / MT; tx <= r.x1 / MT; tx++) atomicAdd(tile_count + base + (uint32_t)ty * tiles_x + tx, 1u);
-}
-
-// tile_start[i] = sum of tile_count[0, i), over n tiles in blocks of 4096; block_sums[b] = the sum of block b.
-extern "C" __global__ void __launch_bounds__(256)
-sl_map_reduce(const uint32_t *__restrict__ tile_count, uint32_t n, uint32_t *__restrict__ block_sums,
-              uint32_t *__restrict__ stats) {
-  __shared__ uint32_t s_tmp[RS_WAVES];
-  if (blockIdx.x == 0 && threadIdx.x == 0) stats[1] = 0u;   // tiles with records, counted by sl_map_scan
-  const uint32_t b0 = blockIdx.x * 4096u + threadIdx.x * 16u;
-  uint32_t v = 0;
-  for (uint32_t k = 0; k < 16u; k++) v += b0 + k < n ? tile_count[b0 + k] : 0u;
-  uint32_t total;
-  block_excl_scan256(v, s_tmp, &total);
-  if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
-}
-
-extern "C" __global__ void __launch_bounds__(256)
-sl_map_scan(const uint32_t *__restrict__ tile_count, uint32_t n, const uint32_t *__restrict__ block_sums,
-            uint32_t *__restrict__ tile_start, uint32_t *__restrict__ stats) {
-  __shared__ uint32_t s_tmp[RS_WAVES];
-  uint32_t before = 0;
-  for (uint32_t b = threadIdx.x; b < blockIdx.x; b += 256u) before += block_sums[b];
-  uint32_t base;
-  block_excl_scan256(before, s_tmp, &base);   // base = sum over the block of 'before' = all earlier blocks
-  const uint32_t b0 = blockIdx.x * 4096u + threadIdx.x * 16u;
-  uint32_t c[16], v = 0, used = 0;
-  for (uint32_t k = 0; k < 16u; k++) {
-    c[k] = b0 + k < n ? tile_count[b0 + k] : 0u;
-    v += c[k];
-    used += c[k] != 0u;
-  }
-  uint32_t total, used_total;
-  uint32_t run = base + block_excl_scan256(v, s_tmp, &total);
-  block_excl_scan256(used, s_tmp, &used_total);
-  for (uint32_t k = 0; k < 16u; k++) {
-    if (b0 + k < n) tile_start[b0 + k] = run;
-    run += c[k];
-  }
-  if (threadIdx.x == 0 && used_total) atomicAdd(stats + 1, used_total);
-  if (blockIdx.x == gridDim.x - 1u && threadIdx.x == 0) {
-    tile_start[n] = base + total;
-    stats[0] = base + total;
-  }
-}
-
-// Each record's slot into the lists of the tiles its box touches; tile_count returns to zero.
-extern "C" __global__ void __launch_bounds__(256)
-sl_map_fill(uint32_t n_rec, const MapRec *__restrict__ recs, uint32_t tiles_x, uint32_t tpl,
-            const uint32_t *__restrict__ tile_start, uint32_t *__restrict__ tile_count, uint32_t *__restrict__ list) {
-  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-  if (i >= n_rec) return;
-  const float4 w2 = reinterpret_cast<const float4 *>(recs)[4 * (size_t)i + 2];
-  const int4 box = reinterpret_cast<const int4 *>(recs)[4 * (size_t)i + 3];
-  if (box.x > box.y || box.z > box.w) return;
-  const uint32_t base = __float_as_uint(w2.w) * tpl;
-  for (int ty = box.z / MT; ty <= box.w / MT; ty++)
-    for (int tx = box.x / MT; tx <= box.y / MT; tx++) {
-      const uint32_t t = base + (uint32_t)ty * tiles_x + tx;
-      const uint32_t pos = atomicSub(tile_count + t, 1u) - 1u;
-      list[tile_start[t] + pos] = i;
-    }
+// A record's tile box in the layers' stacked tile grid (layer l's rows follow layer l - 1's), packed as the frame
+// path's pair emission reads it: (tx0 | ty0 << 16, tx1 | ty1 << 16); the grid's tile id ty * tiles_x + tx is then
+// l * tiles_x * tiles_y + (local row) * tiles_x + tx, the map tile index. Empty records: (0xffffffff, 0).
+__device__ __forceinline__ uint2 map_tbox(const MapRec &r, uint32_t tiles_y) {
+  if (r.x0 > r.x1 || r.y0 > r.y1) return make_uint2(0xffffffffu, 0u);
+  const uint32_t yb = r.layer * tiles_y;
+  return make_uint2((uint32_t)(r.x0 / MT) | ((yb + (uint32_t)(r.y0 / MT)) << 16),
+                    (uint32_t)(r.x1 / MT) | ((yb + (uint32_t)(r.y1 / MT)) << 16));
 }
 
 // Per-tile sums of the records list[s0, s1) of one tile, one lane per texel of the 16x16 tile. Records are staged
@@ -507,7 +454,7 @@ __device__ __forceinline__ void map_tile_sums(const MapRec *__restrict__ recs, c
       const v2 dt = V2(px - r.ctx, py - r.cty);
       const m2 Si = {V2(r.s00, r.s01), V2(r.s10, r.s11)};
       const float q = dot2(dt, m2v(Si, dt));
-      if (q > 9.0f) continue;
+      if (!(q <= 9.0f)) continue;                       // outside 3 sigma, or a NaN footprint: adds nothing
       float t = r.tau0 * exp_map(-0.5f * q);
       t = t < 64.0f ? t : 64.0f;
       const float t_fix = t * fix;
@@ -521,46 +468,49 @@ __device__ __forceinline__ void map_tile_sums(const MapRec *__restrict__ recs, c
   }
 }
 
-// last index t in [0, n] with start[t] <= v (start ascending)
-__device__ __forceinline__ uint32_t last_le(const uint32_t *__restrict__ start, uint32_t n, uint32_t v) {
-  uint32_t lo = 0, hi = n;   // start[0] = 0 <= v
-  while (lo < hi) {
-    const uint32_t mid = (lo + hi + 1u) >> 1;
-    if (start[mid] <= v) lo = mid; else hi = mid - 1u;
-  }
-  return lo;
-}
-
-// Persistent: workgroups take chunks of MAP_CHUNK consecutive list entries (the list is grouped by tile), so a
-// crowded tile is shared by several workgroups. A tile inside one chunk is stored; a tile split across chunks
-// is merged with integer atomic adds (order-independent). acc of the dirty layers is zero on entry; tiles
-// with no records are not touched.
+// Persistent: workgroups take chunks of MAP_CHUNK consecutive entries of the tile-sorted pair list, so a crowded
+// tile is shared by several workgroups. Inside a chunk, each run of equal tile ids is one tile's records; a tile
+// whose run lies inside one chunk is stored, a tile split across chunks is merged with integer atomic adds
+// (order-independent). acc of the dirty layers is zero on entry; tiles with no records are not touched.
+// stats[0] = pairs, stats[1] += tiles with records (zero on entry).
 constexpr uint32_t MAP_CHUNK = 1024;
 extern "C" __global__ void __launch_bounds__(256)
-sl_map_accumulate(const MapRec *__restrict__ recs, const uint32_t *__restrict__ list,
-                  const uint32_t *__restrict__ tile_start, uint32_t n_tiles, uint32_t res, uint32_t tiles_x,
-                  uint32_t tpl, float fix, uint32_t *__restrict__ acc) {
+sl_map_accumulate(const MapRec *__restrict__ recs, const uint32_t *__restrict__ keys,
+                  const uint32_t *__restrict__ list, const FrameCounters *counters, uint32_t res, uint32_t tiles_x,
+                  uint32_t tpl, float fix, uint32_t *__restrict__ acc, uint32_t *__restrict__ stats) {
   __shared__ MapRec s_rec[256];
-  __shared__ uint32_t s_tile;
-  const uint32_t total = tile_start[n_tiles];
+  __shared__ uint32_t s_keys[MAP_CHUNK];
+  __shared__ uint32_t s_prev, s_next;
+  const uint32_t total = __hip_atomic_load(&counters->pairs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   const uint32_t n_chunks = (total + MAP_CHUNK - 1) / MAP_CHUNK;
+  uint32_t used = 0;
   for (uint32_t c = blockIdx.x; c < n_chunks; c += gridDim.x) {
-    const uint32_t lo = c * MAP_CHUNK, hi = min(lo + MAP_CHUNK, total);
+    const uint32_t lo = c * MAP_CHUNK, hi = min(lo + MAP_CHUNK, total), n = hi - lo;
     __syncthreads();
-    if (threadIdx.x == 0) s_tile = last_le(tile_start, n_tiles, lo);
+    for (uint32_t j = threadIdx.x; j < n; j += 256u) s_keys[j] = keys[lo + j];
+    if (threadIdx.x == 0) {
+      s_prev = lo > 0u ? keys[lo - 1u] : 0xffffffffu;
+      s_next = hi < total ? keys[hi] : 0xffffffffu;
+    }
     __syncthreads();
-    uint32_t tile = s_tile;
-    while (tile < n_tiles) {
-      const uint32_t t0 = tile_start[tile], t1 = tile_start[tile + 1];
-      if (t0 >= hi) break;
+    for (uint32_t s0 = 0; s0 < n;) {
+      const uint32_t tile = s_keys[s0];
+      uint32_t lo2 = s0 + 1u, hi2 = n;                 // the run's end: the first key past `tile`
+      while (lo2 < hi2) {
+        const uint32_t mid = (lo2 + hi2) >> 1;
+        if (s_keys[mid] <= tile) lo2 = mid + 1u; else hi2 = mid;
+      }
+      const uint32_t s1 = lo2;
+      const bool starts = s0 > 0u || s_prev != tile, ends = s1 < n || s_next != tile;
+      used += starts;
       const uint32_t layer = tile / tpl, rem = tile - layer * tpl, ty = rem / tiles_x, tx = rem - ty * tiles_x;
       const int x = (int)(tx * MT + (threadIdx.x & 15u)), y = (int)(ty * MT + (threadIdx.x >> 4));
       const int wy0 = (int)(ty * MT + (threadIdx.x >> 6) * 4u), wx0 = (int)(tx * MT);
       uint32_t a0 = 0, a1 = 0, a2 = 0;
-      map_tile_sums(recs, list, max(t0, lo), min(t1, hi), x, y, wx0, wy0, fix, s_rec, a0, a1, a2);
+      map_tile_sums(recs, list, lo + s0, lo + s1, x, y, wx0, wy0, fix, s_rec, a0, a1, a2);
       if ((uint32_t)x < res && (uint32_t)y < res) {
         uint32_t *k = acc + ((size_t)layer * res * res + (size_t)y * res + (size_t)x) * 3;
-        if (t0 >= lo && t1 <= hi) {
+        if (starts && ends) {
           k[0] = a0;
           k[1] = a1;
           k[2] = a2;
@@ -570,9 +520,12 @@ sl_map_accumulate(const MapRec *__restrict__ recs, const uint32_t *__restrict__ 
           if (a2) atomicAdd(k + 2, a2);
         }
       }
-      // next tile with records: the last one whose start is t1
-      tile = t1 >= hi ? n_tiles : last_le(tile_start, n_tiles, t1);
+      s0 = s1;
     }
+  }
+  if (threadIdx.x == 0) {
+    if (used) atomicAdd(stats + 1, used);
+    if (blockIdx.x == 0) stats[0] = total;
   }
 }
 
@@ -690,13 +643,12 @@ __device__ MapRec lsm_rec(uint32_t g, const LSParamsDev &LS, const float4 *__res
 extern "C" __global__ void __launch_bounds__(64)
 sl_lsm_prep(uint32_t n, LSParamsDev LS, const float4 *__restrict__ g0, const float4 *__restrict__ g1,
             const float4 *__restrict__ g2, uint32_t st, const float4 *__restrict__ nodes,
-            const float4 *__restrict__ parts, MapRec *__restrict__ recs, uint32_t *__restrict__ tile_count) {
+            const float4 *__restrict__ parts, MapRec *__restrict__ recs, uint2 *__restrict__ tbox) {
   const uint32_t g = blockIdx.x * 64u + threadIdx.x;
   if (g >= n) return;
   const MapRec r = lsm_rec(g, LS, g0, g1, g2, st, nodes, parts);
   recs[g] = r;
-  const uint32_t tiles_x = (LS.res + MT - 1) / MT;
-  count_tiles(r, tiles_x, tiles_x * tiles_x, tile_count);
+  tbox[g] = map_tbox(r, (LS.res + MT - 1) / MT);
 }
 
 extern "C" __global__ void __launch_bounds__(64)
@@ -781,14 +733,18 @@ extern "C" __global__ void __launch_bounds__(64)
 sl_sun_prep(uint32_t n, SunParamsDev SM, const SunLayerDesc *__restrict__ desc, const SunLayer *__restrict__ layers,
             const float4 *__restrict__ g0, const float4 *__restrict__ g1, const float4 *__restrict__ g2,
             uint32_t st, const float4 *__restrict__ nodes, const float4 *__restrict__ parts, uint32_t max_layers,
-            MapRec *__restrict__ recs, uint32_t *__restrict__ tile_count) {
+            MapRec *__restrict__ recs, uint2 *__restrict__ tbox) {
   const uint32_t g = blockIdx.x * 64u + threadIdx.x;
   if (g >= n) return;
   MapRec *out = recs + (size_t)g * max_layers;
+  uint2 *obox = tbox + (size_t)g * max_layers;
   const float4 A = g0[g], Q = sl_quat(g1, st, g), S4 = sl_scale(g2, st, g);
   const uint32_t p = __float_as_uint(S4.w) & 0xFFFFu;
   const uint32_t first = p < 32u ? SM.part_layers[2 * p] : 0u, cnt = p < 32u ? min(SM.part_layers[2 * p + 1], max_layers) : 0u;
-  for (uint32_t j = cnt; j < max_layers; j++) out[j] = empty_rec();
+  for (uint32_t j = cnt; j < max_layers; j++) {
+    out[j] = empty_rec();
+    obox[j] = make_uint2(0xffffffffu, 0u);
+  }
   if (cnt == 0u) return;
   const Xf xf = load_xf(parts, p);
   const m3 R = part_rot(xf);
@@ -796,12 +752,13 @@ sl_sun_prep(uint32_t n, SunParamsDev SM, const SunLayerDesc *__restrict__ desc, 
   const v3 x = xf_point(xf, V3(A.x, A.y, A.z));
   const m3 C = leaf_cov(Q, S4, R);
   const int res = (int)SM.res;
-  const uint32_t tiles_x = (SM.res + MT - 1) / MT;
+  const uint32_t tiles_y = (SM.res + MT - 1) / MT;
   for (uint32_t j = 0; j < cnt; j++) {
     const uint32_t l = first + j;
     const SunLayerDesc d = desc[l];
     if (d.dirty == 0u || SM.lights[d.light][3] < 0.0f) {
       out[j] = empty_rec();
+      obox[j] = make_uint2(0xffffffffu, 0u);
       continue;
     }
     const SunLayer ly = layers[l];
@@ -830,6 +787,7 @@ sl_sun_prep(uint32_t n, SunParamsDev SM, const SunLayerDesc *__restrict__ desc, 
     const int cx = (int)floorf(ct.x), cy = (int)floorf(ct.y);
     if (tau0 * 4096.0f < 0.5f) {   // no texel reaches half a fixed-point unit
       out[j] = empty_rec();
+      obox[j] = make_uint2(0xffffffffu, 0u);
       continue;
     }
     MapRec o;
@@ -840,7 +798,7 @@ sl_sun_prep(uint32_t n, SunParamsDev SM, const SunLayerDesc *__restrict__ desc, 
     o.x0 = max(cx - hx, 0); o.x1 = min(cx + hx, res - 1);
     o.y0 = max(cy - hy, 0); o.y1 = min(cy + hy, res - 1);
     out[j] = o;
-    count_tiles(o, tiles_x, tiles_x * tiles_x, tile_count);
+    obox[j] = map_tbox(o, tiles_y);
   }
 }
 

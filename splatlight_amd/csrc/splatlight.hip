@@ -92,10 +92,7 @@ struct sl_scene {
   float *lamp_T = nullptr;
   float4 *env_T = nullptr;
   Grow emit_nodes, lsm_acc, lsm_mip, lsm_dirty, sun_acc, sun_mip, sun_desc, sun_layers, sun_dirty;
-  // opacity-map splat binning, shared by both maps: records, per-tile counts (zero between uses), starts, lists
-  Grow map_rec, map_cnt, map_start, map_bsum, map_list;
-  Grow map_stats;                 // per map (emitter, sun): pairs, tiles with records
-  uint32_t map_records[2] = {0, 0}, map_tiles[2] = {0, 0};
+
   uint32_t lsm_parts = 0, lsm_res = 0, lsm_layer_texels = 0;
   uint32_t sun_n = 0, sun_res = 0, sun_layer_texels = 0;
 };
@@ -149,6 +146,10 @@ struct sl_context {
   hipGraphExec_t graph = nullptr;
   std::vector<uint64_t> graph_key;
   sl_tuning tuning = {256u, 16u, 4u, 0u};
+  // opacity-map splat binning, shared by both maps and every scene lit on this context: records, their tile boxes,
+  // the (tile, record) pairs and their sorted copies, a cleared scratch block, per-map stats
+  Grow map_rec, map_tbox, map_k0, map_v0, map_k1, map_v1, map_scratch, map_stats;
+  uint32_t map_records[2] = {0, 0}, map_tiles[2] = {0, 0};
 };
 
 namespace {
@@ -564,6 +565,9 @@ extern "C" void sl_ctx_destroy(sl_context *ctx) {
   if (ctx->counters_host) (void)hipHostFree(ctx->counters_host);
   if (ctx->view_counters_host) (void)hipHostFree(ctx->view_counters_host);
   if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
+  for (Grow *gb : {&ctx->map_rec, &ctx->map_tbox, &ctx->map_k0, &ctx->map_v0, &ctx->map_k1, &ctx->map_v1,
+                   &ctx->map_scratch, &ctx->map_stats})
+    if (gb->p) (void)hipFree(gb->p);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -693,8 +697,7 @@ sl_status scene_alloc(sl_context *ctx, sl_scene *sc, size_t bytes, void **p) {
 void scene_free(sl_scene *sc) {
   for (void *p : sc->allocs) (void)hipFree(p);
   for (Grow *gb : {&sc->emit_nodes, &sc->lsm_acc, &sc->lsm_mip, &sc->lsm_dirty, &sc->sun_acc, &sc->sun_mip,
-                   &sc->sun_desc, &sc->sun_layers, &sc->sun_dirty, &sc->map_rec, &sc->map_cnt, &sc->map_start,
-                   &sc->map_bsum, &sc->map_list, &sc->map_stats})
+                   &sc->sun_desc, &sc->sun_layers, &sc->sun_dirty})
     if (gb->p) (void)hipFree(gb->p);
   delete sc;
 }
@@ -1062,45 +1065,90 @@ bool levels_fit(uint32_t res, uint32_t levels, const uint32_t *level_off, uint32
   return true;
 }
 
-// Buffers for n_rec records over n_tiles map tiles, each record touching at most span x span tiles.
-sl_status ensure_map(sl_context *ctx, sl_scene *sc, size_t n_rec, uint32_t n_tiles, uint32_t span) {
-  sl_status s;
-  const bool fresh_cnt = !sc->map_cnt.p || sc->map_cnt.bytes < 4 * (size_t)n_tiles;
-  if ((s = grow(ctx, sc, &sc->map_rec, std::max<size_t>(n_rec, 1) * sizeof(MapRec))) != SL_OK) return s;
-  if ((s = grow(ctx, sc, &sc->map_cnt, 4 * (size_t)n_tiles)) != SL_OK) return s;
-  if ((s = grow(ctx, sc, &sc->map_start, 4 * ((size_t)n_tiles + 1))) != SL_OK) return s;
-  if ((s = grow(ctx, sc, &sc->map_bsum, 4 * (((size_t)n_tiles + 4095) / 4096))) != SL_OK) return s;
-  if ((s = grow(ctx, sc, &sc->map_list, 4 * std::max<size_t>(n_rec * span * span, 1))) != SL_OK) return s;
-  if (!sc->map_stats.p) {
-    if ((s = grow(ctx, sc, &sc->map_stats, 16)) != SL_OK) return s;
-    sl_fill_u32<<<1, 256, 0, ctx->stream>>>((uint32_t *)sc->map_stats.p, 0u, 4u);
+sl_status grow_ctx(sl_context *ctx, Grow *gb, size_t bytes) {
+  if (gb->p && gb->bytes >= bytes) return SL_OK;
+  if (gb->p) {
+    SL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    SL_HIP(ctx, hipFree(gb->p));
+    gb->p = nullptr;
+    gb->bytes = 0;
   }
-  if (fresh_cnt)
-    sl_fill_u32<<<grid_for(n_tiles, 256, 4096), 256, 0, ctx->stream>>>((uint32_t *)sc->map_cnt.p, 0u, n_tiles);
+  sl_status s = dev_alloc(ctx, bytes, &gb->p);
+  if (s == SL_OK) gb->bytes = bytes;
+  return s;
+}
+
+// The map scratch block (u32 words), cleared before every map splat: the pair emission's counters, tickets and
+// tile-digit histograms, the one-view layout it reads (tiles_x, zero slot and tile bases), its chunk look-back
+// words, and the onesweep look-back words of up to 3 passes.
+struct MapScratch {
+  size_t counters = 0, tickets = 64, hist = 80, tiles_x = 848, zeros = 849, chunk = 856, tstat[3] = {0, 0, 0};
+  size_t words = 0, pass_words = 0;
+  MapScratch(size_t n_rec, size_t cap) {
+    const size_t chunks = (n_rec + PE_CHUNK - 1) / PE_CHUNK;
+    pass_words = (cap + 4095) / 4096 * 256;
+    tstat[0] = align_up(chunk + 2 * std::max<size_t>(chunks, 1), 64);
+    tstat[1] = tstat[0] + pass_words;
+    tstat[2] = tstat[1] + pass_words;
+    words = tstat[2] + pass_words;
+  }
+};
+
+// Buffers for n_rec records over n_tiles map tiles, each record touching at most span x span tiles.
+sl_status ensure_map(sl_context *ctx, size_t n_rec, uint32_t n_tiles, uint32_t span) {
+  sl_status s;
+  const size_t cap = std::max<size_t>(n_rec * span * span, 1);
+  if (cap >= (1ull << 32)) return fail(ctx, SL_ERR_CAPACITY, "opacity-map splat needs more than 2^32 pairs");
+  const MapScratch ms(n_rec, cap);
+  if ((s = grow_ctx(ctx, &ctx->map_rec, std::max<size_t>(n_rec, 1) * sizeof(MapRec))) != SL_OK) return s;
+  if ((s = grow_ctx(ctx, &ctx->map_tbox, std::max<size_t>(n_rec, 1) * 8)) != SL_OK) return s;
+  for (Grow *gb : {&ctx->map_k0, &ctx->map_v0, &ctx->map_k1, &ctx->map_v1})
+    if ((s = grow_ctx(ctx, gb, 4 * cap)) != SL_OK) return s;
+  if ((s = grow_ctx(ctx, &ctx->map_scratch, 4 * ms.words)) != SL_OK) return s;
+  if (!ctx->map_stats.p) {
+    if ((s = grow_ctx(ctx, &ctx->map_stats, 16)) != SL_OK) return s;
+    sl_fill_u32<<<1, 256, 0, ctx->stream>>>((uint32_t *)ctx->map_stats.p, 0u, 4u);
+  }
   return SL_OK;
 }
 
-// The records written by a prep pass -> per-tile lists -> per-tile sums into acc, whose dirty layers are zero.
-// which: 0 emitter map, 1 sun maps (stats slot and timer events).
-void run_map_splat(sl_context *ctx, sl_scene *sc, uint32_t which, size_t n_rec, uint32_t n_layers, uint32_t res,
+// The records and tile boxes written by a prep pass -> (tile, record) pairs (sl_pairs_emit over the layers'
+// stacked tile grid) -> sorted by tile (onesweep) -> per-tile sums into acc, whose dirty layers are zero. which: 0 emitter map, 1 sun maps (stats slot and timer events).
+void run_map_splat(sl_context *ctx, uint32_t which, size_t n_rec, uint32_t n_layers, uint32_t res, uint32_t span,
                    float fix, uint32_t *acc, bool timers) {
   hipStream_t st = ctx->stream;
   const uint32_t tiles_x = (res + MT - 1) / MT, tpl = tiles_x * tiles_x, n_tiles = n_layers * tpl;
-  const uint32_t nb = (n_tiles + 4095) / 4096;
-  uint32_t *cnt = (uint32_t *)sc->map_cnt.p, *start = (uint32_t *)sc->map_start.p;
-  uint32_t *stats = (uint32_t *)sc->map_stats.p + 2 * which;
-  sl_map_reduce<<<nb, 256, 0, st>>>(cnt, n_tiles, (uint32_t *)sc->map_bsum.p, stats);
-  sl_map_scan<<<nb, 256, 0, st>>>(cnt, n_tiles, (const uint32_t *)sc->map_bsum.p, start, stats);
-  if (n_rec)
-    sl_map_fill<<<(uint32_t)((n_rec + 255) / 256), 256, 0, st>>>((uint32_t)n_rec, (const MapRec *)sc->map_rec.p,
-                                                                  tiles_x, tpl, start, cnt, (uint32_t *)sc->map_list.p);
+  const uint32_t tile_passes = n_tiles > 65536u ? 3u : 2u;
+  const size_t cap = std::max<size_t>(n_rec * span * span, 1);
+  const MapScratch ms(n_rec, cap);
+  uint32_t *w = (uint32_t *)ctx->map_scratch.p;
+  FrameCounters *counters = (FrameCounters *)(w + ms.counters);
+  uint32_t *stats = (uint32_t *)ctx->map_stats.p + 2 * which;
+  sl_fill_u32<<<grid_for(ms.words, 256, 2048), 256, 0, st>>>(w, 0u, (uint32_t)ms.words);
+  sl_fill_u32<<<1, 256, 0, st>>>(&counters->visible, (uint32_t)n_rec, 1u);   // the records are the "ranks"
+  sl_fill_u32<<<1, 256, 0, st>>>(w + ms.tiles_x, tiles_x, 1u);
+  sl_fill_u32<<<1, 256, 0, st>>>(stats + 1, 0u, 1u);
+  uint32_t *k0 = (uint32_t *)ctx->map_k0.p, *v0 = (uint32_t *)ctx->map_v0.p;
+  uint32_t *k1 = (uint32_t *)ctx->map_k1.p, *v1 = (uint32_t *)ctx->map_v1.p;
+  sl_pairs_emit<<<grid_for(n_rec, PE_CHUNK, 2u * ctx->cus), PE_THREADS, 0, st>>>(
+      nullptr, (const uint2 *)ctx->map_tbox.p, w + ms.zeros, w + ms.zeros, w + ms.tiles_x, 1u, tile_passes,
+      (uint32_t)cap, counters, k0, v0, (uint64_t *)(w + ms.chunk), w + ms.tickets, w + ms.hist);
+  sl_radix_bases<<<tile_passes, RS_THREADS, 0, st>>>(w + ms.hist);
+  const uint32_t grid = std::max(std::min<uint32_t>((uint32_t)((cap + 4095) / 4096), 4u * ctx->cus), 1u);
+  uint32_t *kin = k0, *vin = v0, *kout = k1, *vout = v1;
+  for (uint32_t p = 0; p < tile_passes; p++) {
+    sl_radix_pass<16><<<grid, RS_THREADS, 0, st>>>(kin, vin, kout, vout, &counters->pairs, 0u, 8u * p,
+                                                   w + ms.hist + 256 * p, w + ms.tstat[p], w + ms.tickets + 1 + p);
+    std::swap(kin, kout);
+    std::swap(vin, vout);
+  }
   if (timers) (void)hipEventRecord(ctx->mev[2 * which], st);
-  sl_map_accumulate<<<ctx->cus * 8, 256, 0, st>>>((const MapRec *)sc->map_rec.p, (const uint32_t *)sc->map_list.p,
-                                                  start, n_tiles, res, tiles_x, tpl, fix, acc);
+  sl_map_accumulate<<<ctx->cus * 8, 256, 0, st>>>((const MapRec *)ctx->map_rec.p, kin, vin, counters, res, tiles_x,
+                                                  tpl, fix, acc, stats);
   if (timers) (void)hipEventRecord(ctx->mev[2 * which + 1], st);
   ctx->map_timed[which] = timers;
-  sc->map_records[which] = (uint32_t)n_rec;
-  sc->map_tiles[which] = n_tiles;
+  ctx->map_records[which] = (uint32_t)n_rec;
+  ctx->map_tiles[which] = n_tiles;
 }
 
 // tiles a box of at most 2 * max_half + 1 texels can span per axis
@@ -1177,17 +1225,18 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
     for (uint32_t p = 0; p < 32; p++) flags[p] = (LS.dirty >> p) & 1u;
     SL_HIP(ctx, hipMemcpyAsync(sc->lsm_dirty.p, flags, sizeof(flags), hipMemcpyHostToDevice, st));
     if (LS.dirty) {
-      const uint32_t tiles_x = (LS.res + MT - 1) / MT;
-      if ((s = ensure_map(ctx, sc, sc->count, LS.part_count * tiles_x * tiles_x, map_span(48, LS.res))) != SL_OK)
-        return s;
+      const uint32_t tiles_x = (LS.res + MT - 1) / MT, span = map_span(48, LS.res);
+      if ((uint64_t)LS.part_count * tiles_x > 65535u)
+        return fail(ctx, SL_ERR_CAPACITY, "emitter map: parts x tile rows over 65535");
+      if ((s = ensure_map(ctx, sc->count, LS.part_count * tiles_x * tiles_x, span)) != SL_OK) return s;
       for (uint32_t p = 0; p < LS.part_count; p++)
         if ((LS.dirty >> p) & 1u)
           sl_fill_u32<<<grid_for(texels * 3, 256, 4096), 256, 0, st>>>((uint32_t *)sc->lsm_acc.p + texels * 3 * p, 0u,
                                                                        (uint32_t)(texels * 3));
       if (sc->count)
         sl_lsm_prep<<<gblocks, 64, 0, st>>>(sc->count, LS, sc->g[0], sc->g[1], sc->g[2], sc->storage, sc->nodes,
-                                            sc->parts, (MapRec *)sc->map_rec.p, (uint32_t *)sc->map_cnt.p);
-      run_map_splat(ctx, sc, 0, sc->count, LS.part_count, LS.res, 65536.0f, (uint32_t *)sc->lsm_acc.p, timers);
+                                            sc->parts, (MapRec *)ctx->map_rec.p, (uint2 *)ctx->map_tbox.p);
+      run_map_splat(ctx, 0, sc->count, LS.part_count, LS.res, span, 65536.0f, (uint32_t *)sc->lsm_acc.p, timers);
       if ((s = run_opacity_maps(ctx, sc, (float4 *)sc->lsm_mip.p, LS.part_count, LS.res, LS.levels, LS.level_off,
                                 LS.layer_texels, 65536.0f, (const uint32_t *)sc->lsm_acc.p,
                                 (const uint32_t *)sc->lsm_dirty.p)) != SL_OK)
@@ -1249,15 +1298,17 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
       if (any) {
         uint32_t max_layers = 0;
         for (uint32_t p = 0; p < sc->part_count; p++) max_layers = std::max(max_layers, SM.part_layers[2 * p + 1]);
-        const uint32_t tiles_x = (SM.res + MT - 1) / MT;
+        const uint32_t tiles_x = (SM.res + MT - 1) / MT, span = map_span(32, SM.res);
         const size_t n_rec = (size_t)sc->count * max_layers;
-        if ((s = ensure_map(ctx, sc, n_rec, SM.n_layers * tiles_x * tiles_x, map_span(32, SM.res))) != SL_OK) return s;
+        if ((uint64_t)SM.n_layers * tiles_x > 65535u)
+          return fail(ctx, SL_ERR_CAPACITY, "sun maps: layers x tile rows over 65535");
+        if ((s = ensure_map(ctx, n_rec, SM.n_layers * tiles_x * tiles_x, span)) != SL_OK) return s;
         if (n_rec)
           sl_sun_prep<<<gblocks, 64, 0, st>>>(sc->count, SM, (const SunLayerDesc *)sc->sun_desc.p,
                                               (const SunLayer *)sc->sun_layers.p, sc->g[0], sc->g[1], sc->g[2],
-                                              sc->storage, sc->nodes, sc->parts, max_layers, (MapRec *)sc->map_rec.p,
-                                              (uint32_t *)sc->map_cnt.p);
-        run_map_splat(ctx, sc, 1, n_rec, SM.n_layers, SM.res, 4096.0f, (uint32_t *)sc->sun_acc.p, timers);
+                                              sc->storage, sc->nodes, sc->parts, max_layers, (MapRec *)ctx->map_rec.p,
+                                              (uint2 *)ctx->map_tbox.p);
+        run_map_splat(ctx, 1, n_rec, SM.n_layers, SM.res, span, 4096.0f, (uint32_t *)sc->sun_acc.p, timers);
         if ((s = run_opacity_maps(ctx, sc, (float4 *)sc->sun_mip.p, SM.n_layers, SM.res, SM.levels, SM.level_off,
                                   SM.layer_texels, 4096.0f, (const uint32_t *)sc->sun_acc.p,
                                   (const uint32_t *)sc->sun_dirty.p)) != SL_OK)
@@ -1314,17 +1365,17 @@ extern "C" sl_status sl_lighting_stats_read(sl_context *ctx, sl_scene *sc, sl_li
   if (!sc || !out) return fail(ctx, SL_ERR_INVALID, "null argument");
   memset(out, 0, sizeof(*out));
   uint32_t dev[4] = {0, 0, 0, 0};
-  if (sc->map_stats.p) {
-    SL_HIP(ctx, hipMemcpyAsync(dev, sc->map_stats.p, sizeof(dev), hipMemcpyDeviceToHost, ctx->stream));
+  if (ctx->map_stats.p) {
+    SL_HIP(ctx, hipMemcpyAsync(dev, ctx->map_stats.p, sizeof(dev), hipMemcpyDeviceToHost, ctx->stream));
     SL_HIP(ctx, hipStreamSynchronize(ctx->stream));
   }
-  out->lsm_records = sc->map_records[0];
+  out->lsm_records = ctx->map_records[0];
   out->lsm_pairs = dev[0];
-  out->lsm_tiles = sc->map_tiles[0];
+  out->lsm_tiles = ctx->map_tiles[0];
   out->lsm_tiles_used = dev[1];
-  out->sun_records = sc->map_records[1];
+  out->sun_records = ctx->map_records[1];
   out->sun_pairs = dev[2];
-  out->sun_tiles = sc->map_tiles[1];
+  out->sun_tiles = ctx->map_tiles[1];
   out->sun_tiles_used = dev[3];
   return SL_OK;
 }

@@ -19,7 +19,7 @@
 #define PE_FLAG_MASK (3ull << 62)
 #define PE_VALUE_MASK ((1ull << 62) - 1ull)
 
-// Batches: a rank's slot belongs to the view v with slot_base[v] <= slot < slot_base[v + 1]; its pairs' tile ids
+// order == nullptr: rank r is slot r. Batches: a rank's slot belongs to the view v with slot_base[v] <= slot < slot_base[v + 1]; its pairs' tile ids
 // are global, tile_base[v] + ty * tiles_x[v] + tx. tile_passes (2 or 3) digit histograms are built.
 extern "C" __global__ void __launch_bounds__(PE_THREADS)
 sl_pairs_emit(const uint32_t *__restrict__ order, const uint2 *__restrict__ tbox, const uint32_t *__restrict__ slot_base,
@@ -57,7 +57,7 @@ sl_pairs_emit(const uint32_t *__restrict__ order, const uint2 *__restrict__ tbox
       uint2 b = make_uint2(0xffffffffu, 0u);
       uint32_t g = 0, vw = 0;
       if (r < V) {
-        g = order[r];
+        g = order ? order[r] : r;
         b = tbox[g];
         if (b.x != 0xffffffffu)
           n = ((b.y & 0xffffu) - (b.x & 0xffffu) + 1u) * ((b.y >> 16) - (b.x >> 16) + 1u);
