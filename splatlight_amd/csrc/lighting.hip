@@ -267,14 +267,16 @@ sl_iso_init(uint32_t n, const float4 *__restrict__ g2, uint32_t st, const float4
 // ---- shade_emit (runtime/wgsl/emissive.wgsl:123-150, 853-980), M39 matte ----------------------------------------
 
 extern "C" __global__ void __launch_bounds__(64)
-sl_shade_emit(FrameUniformsDev U, EmitParamsDev EP, const float4 *__restrict__ emit_nodes, uint32_t n_nodes,
+sl_shade_emit(FrameUniformsDev U, EmitParamsDev EP, const uint32_t *__restrict__ order,
+              const float4 *__restrict__ emit_nodes, uint32_t n_nodes,
               const float4 *__restrict__ g0, const float4 *__restrict__ g1, const float4 *__restrict__ g2,
               const float4 *__restrict__ g3, uint32_t st, const float4 *__restrict__ nodes,
               const uint8_t *__restrict__ vgrid, const float4 *__restrict__ parts, const float *__restrict__ lamp_T,
               const uint32_t *__restrict__ materials, const float *__restrict__ iso_norm,
               float4 *__restrict__ colour) {
-  const uint32_t g = blockIdx.x * 64u + threadIdx.x;
-  if (g >= U.gaussian_count) return;
+  const uint32_t t = blockIdx.x * 64u + threadIdx.x;
+  if (t >= U.gaussian_count) return;
+  const uint32_t g = order ? order[t] : t;
   const float4 A = g0[g], Q = sl_quat(g1, st, g), S = sl_scale(g2, st, g), C = sl_albedo(g1, g2, g3, st, g);
   const uint32_t mi = __float_as_uint(C.w) & 0xFFFFu;
   const uint32_t mmax = max(U.material_count, 1u) - 1u;

@@ -89,6 +89,7 @@ struct sl_scene {
   uint32_t *materials = nullptr;    // 32 B records
   float *iso_norm = nullptr;
   bool iso_done = false;
+  uint32_t *order = nullptr;      // per-Gaussian kernels visit the Gaussians in this order (spatially coherent waves)
   float *lamp_T = nullptr;
   float4 *env_T = nullptr;
   Grow emit_nodes, lsm_acc, lsm_mip, lsm_dirty, sun_acc, sun_mip, sun_desc, sun_layers, sun_dirty;
@@ -407,7 +408,7 @@ sl_status enqueue_batch(sl_context *ctx, const Batch &b, bool timers) {
   if ((s = mark(5)) != SL_OK) return s;
   sl_tile_ranges<<<grid_for(w.pairs_cap, 256, 4096), 256, 0, st>>>(kin, w.counters, w.ranges);
   sl_tile_order<<<1, 1024, 0, st>>>(w.ranges, b.total_tiles, w.tile_order);
-  sl_view_pairs<<<V, 256, 0, st>>>(w.ranges, w.tile_base, w.view_counters);
+  if (V > 1) sl_view_pairs<<<V, 256, 0, st>>>(w.ranges, w.tile_base, w.view_counters);   // one view: the total
   if ((s = mark(6)) != SL_OK) return s;
   switch (tu.composite_batch) {
     case 64u:
@@ -617,6 +618,49 @@ extern "C" sl_status sl_buf_read(sl_context *ctx, const void *dev, size_t offset
 
 namespace {
 
+// A Morton order of the Gaussians within each part (rest-frame means quantised to 16 bits per axis over the
+// part's box, part index above them), for the per-Gaussian lighting kernels: neighbouring lanes then light
+// neighbouring Gaussians and walk similar emitter cuts. Only the order of the work changes, never a result.
+std::vector<uint32_t> spatial_order(const void *gaussians, uint32_t n) {
+  float lo[SL_MAX_PARTS][3], hi[SL_MAX_PARTS][3];
+  for (uint32_t p = 0; p < SL_MAX_PARTS; p++)
+    for (int a = 0; a < 3; a++) { lo[p][a] = 3.4e38f; hi[p][a] = -3.4e38f; }
+  auto part_of = [&](const uint8_t *r) { return std::min(rd_u32(r, 11) & 0xffffu, SL_MAX_PARTS - 1u); };
+  for (uint32_t i = 0; i < n; i++) {
+    const uint8_t *r = (const uint8_t *)gaussians + 64 * (size_t)i;
+    const uint32_t p = part_of(r);
+    for (int a = 0; a < 3; a++) {
+      const float v = rd_f32(r, a);
+      if (v == v) { lo[p][a] = std::min(lo[p][a], v); hi[p][a] = std::max(hi[p][a], v); }
+    }
+  }
+  auto spread = [](uint64_t x) {   // 16 bits -> every third of 48
+    x &= 0xffffull;
+    x = (x | (x << 16)) & 0x0000ff0000ffull;
+    x = (x | (x << 8)) & 0x00f00f00f00full;
+    x = (x | (x << 4)) & 0x0c30c30c30c3ull;
+    x = (x | (x << 2)) & 0x249249249249ull;
+    return x;
+  };
+  std::vector<std::pair<uint64_t, uint32_t>> key(n);
+  for (uint32_t i = 0; i < n; i++) {
+    const uint8_t *r = (const uint8_t *)gaussians + 64 * (size_t)i;
+    const uint32_t p = part_of(r);
+    uint64_t m = 0;
+    for (int a = 0; a < 3; a++) {
+      const float ext = hi[p][a] - lo[p][a];
+      float t = ext > 0.0f ? (rd_f32(r, a) - lo[p][a]) / ext : 0.0f;
+      t = t == t ? std::min(std::max(t, 0.0f), 1.0f) : 0.0f;
+      m |= spread((uint64_t)(t * 65535.0f)) << a;
+    }
+    key[i] = {((uint64_t)p << 48) | m, i};
+  }
+  std::sort(key.begin(), key.end());
+  std::vector<uint32_t> order(n);
+  for (uint32_t i = 0; i < n; i++) order[i] = key[i].second;
+  return order;
+}
+
 // f32 -> 16-bit storage, round to nearest even (NaN stays NaN, overflow goes to infinity)
 uint16_t to_half_bits(float f) {
   const _Float16 h = (_Float16)f;
@@ -786,6 +830,12 @@ extern "C" sl_status sl_scene_create(sl_context *ctx, const sl_scene_desc *d, sl
                               : hipMemsetAsync(sc->vgrid, 0xFF, (size_t)sc->count * 128, ctx->stream);
       if (e != hipSuccess) s = hip_fail(ctx, e, "vgrid upload");
     }
+    if (s == SL_OK && (s = scene_alloc(ctx, sc, (size_t)sc->count * 4, &p)) == SL_OK) {
+      sc->order = (uint32_t *)p;
+      const std::vector<uint32_t> ord = spatial_order(d->gaussians, sc->count);
+      hipError_t e = hipMemcpy(sc->order, ord.data(), (size_t)sc->count * 4, hipMemcpyHostToDevice);
+      if (e != hipSuccess) s = hip_fail(ctx, e, "order upload");
+    }
     if (s == SL_OK && (s = scene_alloc(ctx, sc, (size_t)sc->count * 4, &p)) == SL_OK) sc->iso_norm = (float *)p;
     if (s == SL_OK && (s = scene_alloc(ctx, sc, (size_t)sc->count * 4, &p)) == SL_OK) sc->lamp_T = (float *)p;
     if (s == SL_OK && (s = scene_alloc(ctx, sc, (size_t)sc->count * 16, &p)) == SL_OK) sc->env_T = (float4 *)p;
@@ -946,7 +996,7 @@ sl_status finish_batch(sl_context *ctx, sl_frame_stats *stats, uint32_t n_stats)
       memset(&stats[v], 0, sizeof(stats[v]));
       stats[v].slots = b.views[v].f.nslots;
       stats[v].visible = vc.visible;
-      stats[v].pairs = vc.pairs;
+      stats[v].pairs = V == 1 ? c.pairs : vc.pairs;
       stats[v].overflow = c.overflow;
       stats[v].tiles = b.views[v].f.tiles_x * b.views[v].f.tiles_y;
     }
@@ -1247,7 +1297,7 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
     lampT = sc->lamp_T;
   }
   if (timers) SL_HIP(ctx, hipEventRecord(ctx->lev[1], st));
-  sl_shade_emit<<<gblocks, 64, 0, st>>>(U, EP, (const float4 *)sc->emit_nodes.p, d->emit_node_count, sc->g[0],
+  sl_shade_emit<<<gblocks, 64, 0, st>>>(U, EP, sc->order, (const float4 *)sc->emit_nodes.p, d->emit_node_count, sc->g[0],
                                         sc->g[1], sc->g[2], sc->g[3], sc->storage, sc->nodes, sc->vgrid, sc->parts,
                                         lampT, sc->materials, sc->iso_norm, sc->colour);
   if (timers) SL_HIP(ctx, hipEventRecord(ctx->lev[2], st));

@@ -30,13 +30,13 @@ sl_pairs_emit(const uint32_t *__restrict__ order, const uint2 *__restrict__ tbox
   __shared__ uint32_t s_start[PE_CHUNK + 1];
   __shared__ uint2 s_box[PE_CHUNK];
   __shared__ uint32_t s_g[PE_CHUNK];
-  __shared__ uint32_t s_view[PE_CHUNK];
   __shared__ uint32_t s_hist[3 * 256];
   __shared__ uint32_t s_wsum[PE_THREADS / SL_WAVE];
   __shared__ uint32_t s_chunk;
   __shared__ uint64_t s_excl;
 
   const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+  const uint32_t tb0 = tile_base[0], tx0v = tiles_x[0];   // the only view of a single frame
   const uint32_t V = __hip_atomic_load(&counters->visible, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   const uint32_t num_chunks = (V + PE_CHUNK - 1u) / PE_CHUNK;
   for (uint32_t k = tid; k < 768u; k += PE_THREADS) s_hist[k] = 0u;
@@ -55,17 +55,15 @@ sl_pairs_emit(const uint32_t *__restrict__ order, const uint2 *__restrict__ tbox
       const uint32_t r = r0 + lr;
       uint32_t n = 0;
       uint2 b = make_uint2(0xffffffffu, 0u);
-      uint32_t g = 0, vw = 0;
+      uint32_t g = 0;
       if (r < V) {
         g = order ? order[r] : r;
         b = tbox[g];
         if (b.x != 0xffffffffu)
           n = ((b.y & 0xffffu) - (b.x & 0xffffu) + 1u) * ((b.y >> 16) - (b.x >> 16) + 1u);
-        if (n_views > 1u) vw = view_of(slot_base, n_views, g);
       }
       s_box[lr] = b;
       s_g[lr] = g;
-      s_view[lr] = vw;
       cnt[k] = n;
       sum += n;
     }
@@ -129,8 +127,13 @@ sl_pairs_emit(const uint32_t *__restrict__ order, const uint2 *__restrict__ tbox
       const uint32_t tx0 = b.x & 0xffffu, ty0 = b.x >> 16;
       const uint32_t bw = (b.y & 0xffffu) - tx0 + 1u;
       const uint32_t ty = ty0 + local / bw, tx = tx0 + local % bw;
-      const uint32_t vw = s_view[lo];
-      const uint32_t tile = tile_base[vw] + ty * tiles_x[vw] + tx;
+      uint32_t tile;
+      if (n_views > 1u) {
+        const uint32_t vw = view_of(slot_base, n_views, s_g[lo]);
+        tile = tile_base[vw] + ty * tiles_x[vw] + tx;
+      } else {
+        tile = tb0 + ty * tx0v + tx;
+      }
       const uint64_t dst = base + j;
       const bool keep = dst < capacity;
       if (keep) {
@@ -139,7 +142,9 @@ sl_pairs_emit(const uint32_t *__restrict__ order, const uint2 *__restrict__ tbox
       }
       // tile-id digit counts for the tile sort, one LDS add per distinct digit of the wave (neighbouring pairs
       // share digits, so per-lane atomics would serialise on one address)
-      for (uint32_t pass = 0; pass < tile_passes; pass++) {
+#pragma unroll
+      for (uint32_t pass = 0; pass < 3u; pass++) {
+        if (pass >= tile_passes) break;
         const uint32_t d = (tile >> (8 * pass)) & 255u;
         uint64_t peers = __ballot(keep);
 #pragma unroll
