@@ -217,8 +217,9 @@ sl_status sl_lighting_stats_read(sl_context *ctx, sl_scene *scene, sl_lighting_s
 
 /* Per-pass GPU milliseconds of the last frame encoded with SL_FRAME_TIMERS; names as in the reference's PASSES
  * (src/renderer.js:7-8) where one exists, then, after sl_lighting_encode with SL_LIGHT_TIMERS, its passes
- * "shadow", "shade", "sunmap", "env_direct" and the map-sum kernels alone, "lsm_sum" and "sun_sum". Fills up to
- * `cap` entries; returns the count in *n. */
+ * "shadow", "shade", "sun_wait", "env_direct", "sunmap" (the sun-map chain, which runs on a second stream during
+ * shadow and shade; sun_wait is what remains of it after shade), "lighting" (the whole encode) and the map-sum
+ * kernels alone, "lsm_sum" and "sun_sum". Fills up to `cap` entries; returns the count in *n. */
 sl_status sl_timers_read(sl_context *ctx, const char **names, float *ms, uint32_t cap, uint32_t *n);
 
 #ifdef __cplusplus

@@ -147,10 +147,19 @@ struct sl_context {
   hipGraphExec_t graph = nullptr;
   std::vector<uint64_t> graph_key;
   sl_tuning tuning = {256u, 16u, 4u, 0u};
-  // opacity-map splat binning, shared by both maps and every scene lit on this context: records, their tile boxes,
-  // the (tile, record) pairs and their sorted copies, a cleared scratch block, per-map stats
-  Grow map_rec, map_tbox, map_k0, map_v0, map_k1, map_v1, map_scratch, map_stats;
+  // opacity-map splat binning, one set per map kind (0 emitter map, 1 sun maps) so the two chains can run at once,
+  // shared by every scene lit on this context: records, their tile boxes, the (tile, record) pairs and their sorted
+  // copies, a cleared scratch block; per-map stats (two words per kind)
+  struct MapBufs {
+    Grow rec, tbox, k0, v0, k1, v1, scratch;
+  } map[2];
+  Grow map_stats;
   uint32_t map_records[2] = {0, 0}, map_tiles[2] = {0, 0};
+  // the sun-map chain's stream (created on first use, default priority), the fork/join events that order it with
+  // the main stream, and its begin/end timer events
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr, sev[2] = {};
+  bool sun_timed = false;
 };
 
 namespace {
@@ -566,9 +575,14 @@ extern "C" void sl_ctx_destroy(sl_context *ctx) {
   if (ctx->counters_host) (void)hipHostFree(ctx->counters_host);
   if (ctx->view_counters_host) (void)hipHostFree(ctx->view_counters_host);
   if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
-  for (Grow *gb : {&ctx->map_rec, &ctx->map_tbox, &ctx->map_k0, &ctx->map_v0, &ctx->map_k1, &ctx->map_v1,
-                   &ctx->map_scratch, &ctx->map_stats})
-    if (gb->p) (void)hipFree(gb->p);
+  if (ctx->side) (void)hipStreamSynchronize(ctx->side);
+  for (sl_context::MapBufs &m : ctx->map)
+    for (Grow *gb : {&m.rec, &m.tbox, &m.k0, &m.v0, &m.k1, &m.v1, &m.scratch})
+      if (gb->p) (void)hipFree(gb->p);
+  if (ctx->map_stats.p) (void)hipFree(ctx->map_stats.p);
+  for (hipEvent_t e : {ctx->fork, ctx->join, ctx->sev[0], ctx->sev[1]})
+    if (e) (void)hipEventDestroy(e);
+  if (ctx->side) (void)hipStreamDestroy(ctx->side);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -865,6 +879,7 @@ extern "C" sl_status sl_scene_create(sl_context *ctx, const sl_scene_desc *d, sl
 extern "C" void sl_scene_destroy(sl_context *ctx, sl_scene *scene) {
   if (!scene) return;
   if (ctx && ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx && ctx->side) (void)hipStreamSynchronize(ctx->side);
   if (ctx && ctx->has_last && std::find(ctx->last.scenes.begin(), ctx->last.scenes.end(), scene) != ctx->last.scenes.end())
     ctx->has_last = false;
   scene_free(scene);
@@ -1119,6 +1134,7 @@ sl_status grow_ctx(sl_context *ctx, Grow *gb, size_t bytes) {
   if (gb->p && gb->bytes >= bytes) return SL_OK;
   if (gb->p) {
     SL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->side) SL_HIP(ctx, hipStreamSynchronize(ctx->side));
     SL_HIP(ctx, hipFree(gb->p));
     gb->p = nullptr;
     gb->bytes = 0;
@@ -1144,44 +1160,42 @@ struct MapScratch {
   }
 };
 
-// Buffers for n_rec records over n_tiles map tiles, each record touching at most span x span tiles.
-sl_status ensure_map(sl_context *ctx, size_t n_rec, uint32_t n_tiles, uint32_t span) {
+// Buffers of map kind `which` for n_rec records, each touching at most span x span tiles. Called before any launch
+// of the frame's lighting: growing synchronises both streams.
+sl_status ensure_map(sl_context *ctx, uint32_t which, size_t n_rec, uint32_t span) {
   sl_status s;
   const size_t cap = std::max<size_t>(n_rec * span * span, 1);
   if (cap >= (1ull << 32)) return fail(ctx, SL_ERR_CAPACITY, "opacity-map splat needs more than 2^32 pairs");
   const MapScratch ms(n_rec, cap);
-  if ((s = grow_ctx(ctx, &ctx->map_rec, std::max<size_t>(n_rec, 1) * sizeof(MapRec))) != SL_OK) return s;
-  if ((s = grow_ctx(ctx, &ctx->map_tbox, std::max<size_t>(n_rec, 1) * 8)) != SL_OK) return s;
-  for (Grow *gb : {&ctx->map_k0, &ctx->map_v0, &ctx->map_k1, &ctx->map_v1})
+  sl_context::MapBufs &m = ctx->map[which];
+  if ((s = grow_ctx(ctx, &m.rec, std::max<size_t>(n_rec, 1) * sizeof(MapRec))) != SL_OK) return s;
+  if ((s = grow_ctx(ctx, &m.tbox, std::max<size_t>(n_rec, 1) * 8)) != SL_OK) return s;
+  for (Grow *gb : {&m.k0, &m.v0, &m.k1, &m.v1})
     if ((s = grow_ctx(ctx, gb, 4 * cap)) != SL_OK) return s;
-  if ((s = grow_ctx(ctx, &ctx->map_scratch, 4 * ms.words)) != SL_OK) return s;
-  if (!ctx->map_stats.p) {
-    if ((s = grow_ctx(ctx, &ctx->map_stats, 16)) != SL_OK) return s;
-    sl_fill_u32<<<1, 256, 0, ctx->stream>>>((uint32_t *)ctx->map_stats.p, 0u, 4u);
-  }
-  return SL_OK;
+  return grow_ctx(ctx, &m.scratch, 4 * ms.words);
 }
 
 // The records and tile boxes written by a prep pass -> (tile, record) pairs (sl_pairs_emit over the layers'
-// stacked tile grid) -> sorted by tile (onesweep) -> per-tile sums into acc, whose dirty layers are zero. which: 0 emitter map, 1 sun maps (stats slot and timer events).
-void run_map_splat(sl_context *ctx, uint32_t which, size_t n_rec, uint32_t n_layers, uint32_t res, uint32_t span,
-                   float fix, uint32_t *acc, bool timers) {
-  hipStream_t st = ctx->stream;
+// stacked tile grid) -> sorted by tile (onesweep) -> per-tile sums into acc, whose dirty layers are zero, on stream
+// st. which: 0 emitter map, 1 sun maps (buffers, stats slot and timer events).
+void run_map_splat(sl_context *ctx, hipStream_t st, uint32_t which, size_t n_rec, uint32_t n_layers, uint32_t res,
+                   uint32_t span, float fix, uint32_t *acc, bool timers) {
+  sl_context::MapBufs &m = ctx->map[which];
   const uint32_t tiles_x = (res + MT - 1) / MT, tpl = tiles_x * tiles_x, n_tiles = n_layers * tpl;
   const uint32_t tile_passes = n_tiles > 65536u ? 3u : 2u;
   const size_t cap = std::max<size_t>(n_rec * span * span, 1);
   const MapScratch ms(n_rec, cap);
-  uint32_t *w = (uint32_t *)ctx->map_scratch.p;
+  uint32_t *w = (uint32_t *)m.scratch.p;
   FrameCounters *counters = (FrameCounters *)(w + ms.counters);
   uint32_t *stats = (uint32_t *)ctx->map_stats.p + 2 * which;
   sl_fill_u32<<<grid_for(ms.words, 256, 2048), 256, 0, st>>>(w, 0u, (uint32_t)ms.words);
   sl_fill_u32<<<1, 256, 0, st>>>(&counters->visible, (uint32_t)n_rec, 1u);   // the records are the "ranks"
   sl_fill_u32<<<1, 256, 0, st>>>(w + ms.tiles_x, tiles_x, 1u);
   sl_fill_u32<<<1, 256, 0, st>>>(stats + 1, 0u, 1u);
-  uint32_t *k0 = (uint32_t *)ctx->map_k0.p, *v0 = (uint32_t *)ctx->map_v0.p;
-  uint32_t *k1 = (uint32_t *)ctx->map_k1.p, *v1 = (uint32_t *)ctx->map_v1.p;
+  uint32_t *k0 = (uint32_t *)m.k0.p, *v0 = (uint32_t *)m.v0.p;
+  uint32_t *k1 = (uint32_t *)m.k1.p, *v1 = (uint32_t *)m.v1.p;
   sl_pairs_emit<<<grid_for(n_rec, PE_CHUNK, 2u * ctx->cus), PE_THREADS, 0, st>>>(
-      nullptr, (const uint2 *)ctx->map_tbox.p, w + ms.zeros, w + ms.zeros, w + ms.tiles_x, 1u, tile_passes,
+      nullptr, (const uint2 *)m.tbox.p, w + ms.zeros, w + ms.zeros, w + ms.tiles_x, 1u, tile_passes,
       (uint32_t)cap, counters, k0, v0, (uint64_t *)(w + ms.chunk), w + ms.tickets, w + ms.hist);
   sl_radix_bases<<<tile_passes, RS_THREADS, 0, st>>>(w + ms.hist);
   const uint32_t grid = std::max(std::min<uint32_t>((uint32_t)((cap + 4095) / 4096), 4u * ctx->cus), 1u);
@@ -1193,7 +1207,7 @@ void run_map_splat(sl_context *ctx, uint32_t which, size_t n_rec, uint32_t n_lay
     std::swap(vin, vout);
   }
   if (timers) (void)hipEventRecord(ctx->mev[2 * which], st);
-  sl_map_accumulate<<<ctx->cus * 8, 256, 0, st>>>((const MapRec *)ctx->map_rec.p, kin, vin, counters, res, tiles_x,
+  sl_map_accumulate<<<ctx->cus * 8, 256, 0, st>>>((const MapRec *)m.rec.p, kin, vin, counters, res, tiles_x,
                                                   tpl, fix, acc, stats);
   if (timers) (void)hipEventRecord(ctx->mev[2 * which + 1], st);
   ctx->map_timed[which] = timers;
@@ -1206,10 +1220,9 @@ uint32_t map_span(uint32_t max_half, uint32_t res) {
   return std::min<uint32_t>((2u * max_half) / MT + 2u, (res + MT - 1) / MT);
 }
 
-sl_status run_opacity_maps(sl_context *ctx, sl_scene *sc, float4 *mip, uint32_t n_layers, uint32_t res,
-                           uint32_t levels, const uint32_t *level_off, uint32_t layer_texels, float fix,
-                           const uint32_t *acc, const uint32_t *dirty_dev) {
-  hipStream_t st = ctx->stream;
+void run_opacity_maps(hipStream_t st, float4 *mip, uint32_t n_layers, uint32_t res, uint32_t levels,
+                      const uint32_t *level_off, uint32_t layer_texels, float fix, const uint32_t *acc,
+                      const uint32_t *dirty_dev) {
   const size_t texels = (size_t)res * res;
   sl_opacity_resolve<<<(uint32_t)((texels * n_layers + 255) / 256), 256, 0, st>>>(acc, mip, n_layers, (uint32_t)texels,
                                                                                  layer_texels, fix, dirty_dev);
@@ -1218,7 +1231,6 @@ sl_status run_opacity_maps(sl_context *ctx, sl_scene *sc, float4 *mip, uint32_t 
     sl_opacity_down<<<(uint32_t)(((size_t)rk * rk * n_layers + 255) / 256), 256, 0, st>>>(
         mip, n_layers, rk, level_off[k - 1], level_off[k], layer_texels, dirty_dev);
   }
-  return check_launch(ctx, "opacity map resolve");
 }
 
 }  // namespace
@@ -1250,16 +1262,17 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
   hipStream_t st = ctx->stream;
   sl_status s;
   if (!sc->iso_done && (s = sl_lighting_iso_init(ctx, sc)) != SL_OK) return s;
+  // Validate and size both shadow chains before the first launch (a grow synchronises and frees). The chains then
+  // run at once: emitter map -> lamp_T -> shade on the context's stream, sun maps -> env_T on the side stream; they
+  // meet before env_direct, the first kernel that reads both.
   if ((s = grow(ctx, sc, &sc->emit_nodes, (size_t)d->emit_node_count * 96)) != SL_OK) return s;
-  SL_HIP(ctx, hipMemcpyAsync(sc->emit_nodes.p, d->emit_nodes, (size_t)d->emit_node_count * 96, hipMemcpyHostToDevice, st));
-  const uint32_t gblocks = (sc->count + 63) / 64;
-  const bool timers = (d->flags & SL_LIGHT_TIMERS) != 0;
-  if (timers) SL_HIP(ctx, hipEventRecord(ctx->lev[0], st));
-  ctx->map_timed[0] = ctx->map_timed[1] = false;
-  // A: the emitter light's opacity map and its lookup -> lamp_T
-  const float *lampT = nullptr;
+  const bool fresh_stats = !ctx->map_stats.p;
+  if (fresh_stats && (s = grow_ctx(ctx, &ctx->map_stats, 16)) != SL_OK) return s;
+  // emitter map plan
+  LSParamsDev LS{};
+  uint32_t ls_flags[32] = {};
+  uint32_t ls_span = 0;
   if (d->ls_params) {
-    LSParamsDev LS;
     read_struct(d->ls_params, &LS);
     if (LS.part_count != sc->part_count || !levels_fit(LS.res, LS.levels, LS.level_off, LS.layer_texels))
       return fail(ctx, SL_ERR_INVALID, "LSParams do not match the scene");
@@ -1268,46 +1281,26 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
     if ((s = grow(ctx, sc, &sc->lsm_mip, (size_t)LS.layer_texels * 16 * LS.part_count)) != SL_OK) return s;
     if ((s = grow(ctx, sc, &sc->lsm_dirty, 4 * 32)) != SL_OK) return s;
     if (sc->lsm_res != LS.res || sc->lsm_parts != LS.part_count) LS.dirty = (1u << LS.part_count) - 1u;
-    sc->lsm_res = LS.res;
-    sc->lsm_parts = LS.part_count;
-    sc->lsm_layer_texels = LS.layer_texels;
-    uint32_t flags[32];
-    for (uint32_t p = 0; p < 32; p++) flags[p] = (LS.dirty >> p) & 1u;
-    SL_HIP(ctx, hipMemcpyAsync(sc->lsm_dirty.p, flags, sizeof(flags), hipMemcpyHostToDevice, st));
+    for (uint32_t p = 0; p < 32; p++) ls_flags[p] = (LS.dirty >> p) & 1u;
     if (LS.dirty) {
-      const uint32_t tiles_x = (LS.res + MT - 1) / MT, span = map_span(48, LS.res);
+      const uint32_t tiles_x = (LS.res + MT - 1) / MT;
+      ls_span = map_span(48, LS.res);
       if ((uint64_t)LS.part_count * tiles_x > 65535u)
         return fail(ctx, SL_ERR_CAPACITY, "emitter map: parts x tile rows over 65535");
-      if ((s = ensure_map(ctx, sc->count, LS.part_count * tiles_x * tiles_x, span)) != SL_OK) return s;
-      for (uint32_t p = 0; p < LS.part_count; p++)
-        if ((LS.dirty >> p) & 1u)
-          sl_fill_u32<<<grid_for(texels * 3, 256, 4096), 256, 0, st>>>((uint32_t *)sc->lsm_acc.p + texels * 3 * p, 0u,
-                                                                       (uint32_t)(texels * 3));
-      if (sc->count)
-        sl_lsm_prep<<<gblocks, 64, 0, st>>>(sc->count, LS, sc->g[0], sc->g[1], sc->g[2], sc->storage, sc->nodes,
-                                            sc->parts, (MapRec *)ctx->map_rec.p, (uint2 *)ctx->map_tbox.p);
-      run_map_splat(ctx, 0, sc->count, LS.part_count, LS.res, span, 65536.0f, (uint32_t *)sc->lsm_acc.p, timers);
-      if ((s = run_opacity_maps(ctx, sc, (float4 *)sc->lsm_mip.p, LS.part_count, LS.res, LS.levels, LS.level_off,
-                                LS.layer_texels, 65536.0f, (const uint32_t *)sc->lsm_acc.p,
-                                (const uint32_t *)sc->lsm_dirty.p)) != SL_OK)
-        return s;
+      if ((s = ensure_map(ctx, 0, sc->count, ls_span)) != SL_OK) return s;
     }
-    sl_lsm_lookup<<<gblocks, 64, 0, st>>>(sc->count, LS, sc->g[0], sc->g[2], sc->storage, sc->parts,
-                                          (const float4 *)sc->lsm_mip.p, sc->lamp_T);
-    lampT = sc->lamp_T;
   }
-  if (timers) SL_HIP(ctx, hipEventRecord(ctx->lev[1], st));
-  sl_shade_emit<<<gblocks, 64, 0, st>>>(U, EP, sc->order, (const float4 *)sc->emit_nodes.p, d->emit_node_count, sc->g[0],
-                                        sc->g[1], sc->g[2], sc->g[3], sc->storage, sc->nodes, sc->vgrid, sc->parts,
-                                        lampT, sc->materials, sc->iso_norm, sc->colour);
-  if (timers) SL_HIP(ctx, hipEventRecord(ctx->lev[2], st));
-  // distant lights: sun maps -> env_T, then env_direct
+  // sun map plan
+  EnvDev ENV{};
+  SunParamsDev SM{};
+  std::vector<SunLayerDesc> desc;
+  std::vector<uint32_t> sun_flags;
+  bool have_sun = false, sun_any = false;
+  uint32_t max_layers = 0, sun_span = 0;
+  size_t sun_rec = 0;
   if (d->env) {
-    EnvDev ENV;
     read_struct(d->env, &ENV);
-    const float4 *envT = nullptr;
     if (d->sun_params && ENV.info[0] > 0u) {
-      SunParamsDev SM;
       read_struct(d->sun_params, &SM);
       if (SM.n_layers != d->sun_layer_count || !d->sun_layers || SM.n_layers == 0 || SM.n_lights > 4 ||
           !levels_fit(SM.res, SM.levels, SM.level_off, SM.layer_texels))
@@ -1315,7 +1308,7 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
       for (uint32_t p = 0; p < 32; p++)
         if ((uint64_t)SM.part_layers[2 * p] + SM.part_layers[2 * p + 1] > SM.n_layers)
           return fail(ctx, SL_ERR_INVALID, "SunParams part layer range outside the plan");
-      std::vector<SunLayerDesc> desc(SM.n_layers);
+      desc.resize(SM.n_layers);
       memcpy(desc.data(), d->sun_layers, sizeof(SunLayerDesc) * SM.n_layers);
       for (const SunLayerDesc &ld : desc)
         if (ld.part >= sc->part_count || ld.light >= 4u) return fail(ctx, SL_ERR_INVALID, "bad sun layer");
@@ -1326,56 +1319,112 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
       if ((s = grow(ctx, sc, &sc->sun_layers, sizeof(SunLayer) * SM.n_layers)) != SL_OK) return s;
       if ((s = grow(ctx, sc, &sc->sun_dirty, 4 * (size_t)SM.n_layers)) != SL_OK) return s;
       const bool fresh = sc->sun_res != SM.res || sc->sun_n != SM.n_layers;
-      std::vector<uint32_t> flags(SM.n_layers);
+      sun_flags.resize(SM.n_layers);
       for (uint32_t l = 0; l < SM.n_layers; l++) {
         if (fresh) desc[l].dirty = 1u;
-        flags[l] = desc[l].dirty ? 1u : 0u;
+        sun_flags[l] = desc[l].dirty ? 1u : 0u;
+        sun_any = sun_any || sun_flags[l];
       }
-      sc->sun_res = SM.res;
-      sc->sun_n = SM.n_layers;
-      sc->sun_layer_texels = SM.layer_texels;
-      SL_HIP(ctx, hipMemcpyAsync(sc->sun_desc.p, desc.data(), sizeof(SunLayerDesc) * SM.n_layers, hipMemcpyHostToDevice, st));
-      SL_HIP(ctx, hipMemcpyAsync(sc->sun_dirty.p, flags.data(), 4 * (size_t)SM.n_layers, hipMemcpyHostToDevice, st));
-      sl_sun_frame<<<(SM.n_layers + 63) / 64, 64, 0, st>>>(SM, (const SunLayerDesc *)sc->sun_desc.p, sc->parts,
-                                                           (SunLayer *)sc->sun_layers.p);
-      bool any = false;
-      for (uint32_t l = 0; l < SM.n_layers; l++)
-        if (flags[l]) {
-          any = true;
-          sl_fill_u32<<<grid_for(texels * 3, 256, 4096), 256, 0, st>>>((uint32_t *)sc->sun_acc.p + texels * 3 * l, 0u,
-                                                                       (uint32_t)(texels * 3));
-        }
-      if (any) {
-        uint32_t max_layers = 0;
+      if (sun_any) {
         for (uint32_t p = 0; p < sc->part_count; p++) max_layers = std::max(max_layers, SM.part_layers[2 * p + 1]);
-        const uint32_t tiles_x = (SM.res + MT - 1) / MT, span = map_span(32, SM.res);
-        const size_t n_rec = (size_t)sc->count * max_layers;
+        const uint32_t tiles_x = (SM.res + MT - 1) / MT;
+        sun_span = map_span(32, SM.res);
+        sun_rec = (size_t)sc->count * max_layers;
         if ((uint64_t)SM.n_layers * tiles_x > 65535u)
           return fail(ctx, SL_ERR_CAPACITY, "sun maps: layers x tile rows over 65535");
-        if ((s = ensure_map(ctx, n_rec, SM.n_layers * tiles_x * tiles_x, span)) != SL_OK) return s;
-        if (n_rec)
-          sl_sun_prep<<<gblocks, 64, 0, st>>>(sc->count, SM, (const SunLayerDesc *)sc->sun_desc.p,
-                                              (const SunLayer *)sc->sun_layers.p, sc->g[0], sc->g[1], sc->g[2],
-                                              sc->storage, sc->nodes, sc->parts, max_layers, (MapRec *)ctx->map_rec.p,
-                                              (uint2 *)ctx->map_tbox.p);
-        run_map_splat(ctx, 1, n_rec, SM.n_layers, SM.res, span, 4096.0f, (uint32_t *)sc->sun_acc.p, timers);
-        if ((s = run_opacity_maps(ctx, sc, (float4 *)sc->sun_mip.p, SM.n_layers, SM.res, SM.levels, SM.level_off,
-                                  SM.layer_texels, 4096.0f, (const uint32_t *)sc->sun_acc.p,
-                                  (const uint32_t *)sc->sun_dirty.p)) != SL_OK)
-          return s;
+        if ((s = ensure_map(ctx, 1, sun_rec, sun_span)) != SL_OK) return s;
       }
-      sl_sun_lookup<<<gblocks, 64, 0, st>>>(sc->count, SM, (const SunLayerDesc *)sc->sun_desc.p,
-                                            (const SunLayer *)sc->sun_layers.p, sc->g[0], sc->g[2], sc->storage,
-                                            sc->parts, (const float4 *)sc->sun_mip.p, sc->env_T);
-      envT = sc->env_T;
-    } else {
+      have_sun = true;
+    }
+  }
+  if (have_sun && !ctx->side) {
+    SL_HIP(ctx, hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
+    SL_HIP(ctx, hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming));
+    SL_HIP(ctx, hipEventCreateWithFlags(&ctx->join, hipEventDisableTiming));
+    for (hipEvent_t &e : ctx->sev) SL_HIP(ctx, hipEventCreate(&e));
+  }
+
+  const uint32_t gblocks = (sc->count + 63) / 64;
+  const bool timers = (d->flags & SL_LIGHT_TIMERS) != 0;
+  ctx->map_timed[0] = ctx->map_timed[1] = false;
+  ctx->sun_timed = false;
+  SL_HIP(ctx, hipMemcpyAsync(sc->emit_nodes.p, d->emit_nodes, (size_t)d->emit_node_count * 96, hipMemcpyHostToDevice, st));
+  if (fresh_stats) sl_fill_u32<<<1, 256, 0, st>>>((uint32_t *)ctx->map_stats.p, 0u, 4u);
+  if (timers) SL_HIP(ctx, hipEventRecord(ctx->lev[0], st));
+  // side stream: sun maps -> env_T (after everything already on the context's stream)
+  if (have_sun) {
+    hipStream_t ss = ctx->side;
+    SL_HIP(ctx, hipEventRecord(ctx->fork, st));
+    SL_HIP(ctx, hipStreamWaitEvent(ss, ctx->fork, 0));
+    if (timers) SL_HIP(ctx, hipEventRecord(ctx->sev[0], ss));
+    const size_t texels = (size_t)SM.res * SM.res;
+    sc->sun_res = SM.res;
+    sc->sun_n = SM.n_layers;
+    sc->sun_layer_texels = SM.layer_texels;
+    SL_HIP(ctx, hipMemcpyAsync(sc->sun_desc.p, desc.data(), sizeof(SunLayerDesc) * SM.n_layers, hipMemcpyHostToDevice, ss));
+    SL_HIP(ctx, hipMemcpyAsync(sc->sun_dirty.p, sun_flags.data(), 4 * (size_t)SM.n_layers, hipMemcpyHostToDevice, ss));
+    sl_sun_frame<<<(SM.n_layers + 63) / 64, 64, 0, ss>>>(SM, (const SunLayerDesc *)sc->sun_desc.p, sc->parts,
+                                                         (SunLayer *)sc->sun_layers.p);
+    for (uint32_t l = 0; l < SM.n_layers; l++)
+      if (sun_flags[l])
+        sl_fill_u32<<<grid_for(texels * 3, 256, 4096), 256, 0, ss>>>((uint32_t *)sc->sun_acc.p + texels * 3 * l, 0u,
+                                                                     (uint32_t)(texels * 3));
+    if (sun_any) {
+      if (sun_rec)
+        sl_sun_prep<<<gblocks, 64, 0, ss>>>(sc->count, SM, (const SunLayerDesc *)sc->sun_desc.p,
+                                            (const SunLayer *)sc->sun_layers.p, sc->g[0], sc->g[1], sc->g[2],
+                                            sc->storage, sc->nodes, sc->parts, max_layers,
+                                            (MapRec *)ctx->map[1].rec.p, (uint2 *)ctx->map[1].tbox.p);
+      run_map_splat(ctx, ss, 1, sun_rec, SM.n_layers, SM.res, sun_span, 4096.0f, (uint32_t *)sc->sun_acc.p, timers);
+      run_opacity_maps(ss, (float4 *)sc->sun_mip.p, SM.n_layers, SM.res, SM.levels, SM.level_off, SM.layer_texels,
+                       4096.0f, (const uint32_t *)sc->sun_acc.p, (const uint32_t *)sc->sun_dirty.p);
+    }
+    sl_sun_lookup<<<gblocks, 64, 0, ss>>>(sc->count, SM, (const SunLayerDesc *)sc->sun_desc.p,
+                                          (const SunLayer *)sc->sun_layers.p, sc->g[0], sc->g[2], sc->storage,
+                                          sc->parts, (const float4 *)sc->sun_mip.p, sc->env_T);
+    if (timers) SL_HIP(ctx, hipEventRecord(ctx->sev[1], ss));
+    SL_HIP(ctx, hipEventRecord(ctx->join, ss));
+    ctx->sun_timed = timers;
+  }
+  // context's stream: the emitter light's opacity map and its lookup -> lamp_T, then the emitter shading
+  const float *lampT = nullptr;
+  if (d->ls_params) {
+    const size_t texels = (size_t)LS.res * LS.res;
+    sc->lsm_res = LS.res;
+    sc->lsm_parts = LS.part_count;
+    sc->lsm_layer_texels = LS.layer_texels;
+    SL_HIP(ctx, hipMemcpyAsync(sc->lsm_dirty.p, ls_flags, sizeof(ls_flags), hipMemcpyHostToDevice, st));
+    if (LS.dirty) {
+      for (uint32_t p = 0; p < LS.part_count; p++)
+        if (ls_flags[p])
+          sl_fill_u32<<<grid_for(texels * 3, 256, 4096), 256, 0, st>>>((uint32_t *)sc->lsm_acc.p + texels * 3 * p, 0u,
+                                                                       (uint32_t)(texels * 3));
+      if (sc->count)
+        sl_lsm_prep<<<gblocks, 64, 0, st>>>(sc->count, LS, sc->g[0], sc->g[1], sc->g[2], sc->storage, sc->nodes,
+                                            sc->parts, (MapRec *)ctx->map[0].rec.p, (uint2 *)ctx->map[0].tbox.p);
+      run_map_splat(ctx, st, 0, sc->count, LS.part_count, LS.res, ls_span, 65536.0f, (uint32_t *)sc->lsm_acc.p, timers);
+      run_opacity_maps(st, (float4 *)sc->lsm_mip.p, LS.part_count, LS.res, LS.levels, LS.level_off, LS.layer_texels,
+                       65536.0f, (const uint32_t *)sc->lsm_acc.p, (const uint32_t *)sc->lsm_dirty.p);
+    }
+    sl_lsm_lookup<<<gblocks, 64, 0, st>>>(sc->count, LS, sc->g[0], sc->g[2], sc->storage, sc->parts,
+                                          (const float4 *)sc->lsm_mip.p, sc->lamp_T);
+    lampT = sc->lamp_T;
+  }
+  if (timers) SL_HIP(ctx, hipEventRecord(ctx->lev[1], st));
+  sl_shade_emit<<<gblocks, 64, 0, st>>>(U, EP, sc->order, (const float4 *)sc->emit_nodes.p, d->emit_node_count, sc->g[0],
+                                        sc->g[1], sc->g[2], sc->g[3], sc->storage, sc->nodes, sc->vgrid, sc->parts,
+                                        lampT, sc->materials, sc->iso_norm, sc->colour);
+  if (timers) SL_HIP(ctx, hipEventRecord(ctx->lev[2], st));
+  // distant lights: env_T (the side stream's, or 1 without sun maps), then env_direct
+  if (d->env) {
+    if (have_sun)
+      SL_HIP(ctx, hipStreamWaitEvent(st, ctx->join, 0));
+    else
       sl_fill_u32<<<grid_for((size_t)sc->count * 4, 256, 4096), 256, 0, st>>>((uint32_t *)sc->env_T, 0x3f800000u,
                                                                                sc->count * 4);
-      envT = sc->env_T;
-    }
     if (timers) SL_HIP(ctx, hipEventRecord(ctx->lev[3], st));
     sl_env_direct<<<gblocks, 64, 0, st>>>(U, ENV, sc->g[1], sc->g[2], sc->g[3], sc->storage, sc->nodes, sc->vgrid,
-                                          sc->parts, sc->materials, sc->iso_norm, envT, sc->colour);
+                                          sc->parts, sc->materials, sc->iso_norm, sc->env_T, sc->colour);
   } else if (timers) {
     SL_HIP(ctx, hipEventRecord(ctx->lev[3], st));
   }
@@ -1439,14 +1488,30 @@ extern "C" sl_status sl_timers_read(sl_context *ctx, const char **names, float *
       if (ms) ms[k] = ctx->ms[k];
     }
   if (ctx->light_timers_valid) {
-    // the lighting passes of the last sl_lighting_encode with SL_LIGHT_TIMERS (PASSES names: shadow, shade)
-    static const char *const lnames[] = {"shadow", "shade", "sunmap", "env_direct"};
+    // the lighting passes of the last sl_lighting_encode with SL_LIGHT_TIMERS (PASSES names: shadow, shade).
+    // "sunmap" runs on the side stream alongside shadow and shade; "sun_wait" is the context stream's wait for it
+    // before env_direct, and "lighting" the whole encode.
+    static const char *const lnames[] = {"shadow", "shade", "sun_wait", "env_direct"};
     SL_HIP(ctx, hipEventSynchronize(ctx->lev[4]));
     for (int j = 0; j < 4 && k < cap; j++, k++) {
       float v = 0.0f;
       SL_HIP(ctx, hipEventElapsedTime(&v, ctx->lev[j], ctx->lev[j + 1]));
       if (names) names[k] = lnames[j];
       if (ms) ms[k] = v;
+    }
+    if (ctx->sun_timed && k < cap) {
+      float v = 0.0f;
+      SL_HIP(ctx, hipEventElapsedTime(&v, ctx->sev[0], ctx->sev[1]));
+      if (names) names[k] = "sunmap";
+      if (ms) ms[k] = v;
+      k++;
+    }
+    if (k < cap) {
+      float v = 0.0f;
+      SL_HIP(ctx, hipEventElapsedTime(&v, ctx->lev[0], ctx->lev[4]));
+      if (names) names[k] = "lighting";
+      if (ms) ms[k] = v;
+      k++;
     }
     // the map-sum kernel alone (sl_map_accumulate), inside "shadow" and "sunmap"
     static const char *const mnames[] = {"lsm_sum", "sun_sum"};
