@@ -435,65 +435,98 @@ __device__ __forceinline__ uint2 map_tbox(const MapRec &r, uint32_t tiles_y) {
                     (uint32_t)(r.x1 / MT) | ((yb + (uint32_t)(r.y1 / MT)) << 16));
 }
 
-// Per-tile sums of the records list[s0, s1) of one tile, one lane per texel of the 16x16 tile. Records are staged
-// 256 at a time in LDS with one coalesced load per lane; each wave reads them back as broadcasts, one record ahead
-// of the one it evaluates.
+constexpr uint32_t MAP_STAGE = 192;   // records staged in LDS per round
+constexpr uint32_t MAP_CHUNK = 1024;  // tile-sorted pairs per ticket
+
+// One texel's fixed-point moments of record r, added to (a0, a1, a2) (light_shadow.wgsl:237-284,
+// sun_map.wgsl:171-217: the same per-texel arithmetic).
+__device__ __forceinline__ void map_texel(const MapRec &r, float dx, float py, float fix, uint32_t &a0, uint32_t &a1,
+                                          uint32_t &a2) {
+  const v2 dt = V2(dx, py - r.cty);
+  const m2 Si = {V2(r.s00, r.s01), V2(r.s10, r.s11)};
+  const float q = dot2(dt, m2v(Si, dt));
+  if (!(q <= 9.0f)) return;                         // outside 3 sigma, or a NaN footprint: adds nothing
+  float t = r.tau0 * exp_map(-0.5f * q);
+  t = t < 64.0f ? t : 64.0f;
+  const float t_fix = t * fix;
+  if (t_fix < 0.5f) return;
+  float z = r.zc + dot2(V2(r.gzx, r.gzy), dt);
+  z = z > 0.0f ? z : 0.0f;
+  a0 += u32_sat(t_fix + 0.5f);
+  a1 += u32_sat(t_fix * z + 0.5f);
+  a2 += u32_sat(t_fix * (z * z + r.sz2) + 0.5f);
+}
+
+// Per-tile sums of the records list[s0, s1) of the 16x16 tile at texel (tx0, ty0). Records are staged MAP_STAGE
+// at a time in LDS with one coalesced load per lane; wave w evaluates staged records w, w + 4, ... over the whole
+// tile, as four bands of 16x4 texels (lane: column lane & 15, row lane >> 4 of the band), skipping the bands a
+// record's box misses. Each record is read back from LDS by one wave only. The four waves' sums meet in s_acc
+// (integer adds, order-independent); on return s_acc[c][t] holds texel t's sum c (t = row * 16 + column) and the
+// caller clears it after reading.
 __device__ __forceinline__ void map_tile_sums(const MapRec *__restrict__ recs, const uint32_t *__restrict__ list,
-                                              uint32_t s0, uint32_t s1, int x, int y, int wx0, int wy0, float fix,
-                                              MapRec *s_rec, uint32_t &a0, uint32_t &a1, uint32_t &a2) {
-  const float px = (float)x + 0.5f, py = (float)y + 0.5f;
-  for (uint32_t b = s0; b < s1; b += 256u) {
-    const uint32_t nb = min(256u, s1 - b);
+                                              uint32_t s0, uint32_t s1, int tx0, int ty0, float fix, MapRec *s_rec,
+                                              uint32_t *s_acc) {
+  const uint32_t lane = threadIdx.x & 63u, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int x = tx0 + (int)(lane & 15u), yl = (int)(lane >> 4);
+  const float px = (float)x + 0.5f;
+  uint32_t a[4][3] = {};
+  for (uint32_t b = s0; b < s1; b += MAP_STAGE) {
+    const uint32_t nb = min(MAP_STAGE, s1 - b);
     __syncthreads();
     if (threadIdx.x < nb) s_rec[threadIdx.x] = recs[list[b + threadIdx.x]];
     __syncthreads();
-    MapRec nx = s_rec[0];
-    for (uint32_t j = 0; j < nb; j++) {
-      const MapRec r = nx;
-      if (j + 1 < nb) nx = s_rec[j + 1];
-      if (r.x1 < wx0 || r.x0 > wx0 + 15 || r.y1 < wy0 || r.y0 > wy0 + 3) continue;   // the wave's 16x4 texels
-      if (x < r.x0 || x > r.x1 || y < r.y0 || y > r.y1) continue;
-      const v2 dt = V2(px - r.ctx, py - r.cty);
-      const m2 Si = {V2(r.s00, r.s01), V2(r.s10, r.s11)};
-      const float q = dot2(dt, m2v(Si, dt));
-      if (!(q <= 9.0f)) continue;                       // outside 3 sigma, or a NaN footprint: adds nothing
-      float t = r.tau0 * exp_map(-0.5f * q);
-      t = t < 64.0f ? t : 64.0f;
-      const float t_fix = t * fix;
-      if (t_fix < 0.5f) continue;
-      float z = r.zc + dot2(V2(r.gzx, r.gzy), dt);
-      z = z > 0.0f ? z : 0.0f;
-      a0 += u32_sat(t_fix + 0.5f);
-      a1 += u32_sat(t_fix * z + 0.5f);
-      a2 += u32_sat(t_fix * (z * z + r.sz2) + 0.5f);
+    for (uint32_t j = wave; j < nb; j += 4u) {
+      const MapRec r = s_rec[j];
+      const bool xin = x >= r.x0 && x <= r.x1;
+      const float dx = px - r.ctx;
+      // bands [k0, k1] of this tile meet the record's rows (the same for every lane)
+      const int k0 = max(r.y0 - ty0, 0) >> 2, k1 = min(r.y1 - ty0, 15) >> 2;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        if (k < k0 || k > k1) continue;
+        const int y = ty0 + 4 * k + yl;
+        if (xin && y >= r.y0 && y <= r.y1) map_texel(r, dx, (float)y + 0.5f, fix, a[k][0], a[k][1], a[k][2]);
+      }
     }
   }
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+      if (a[k][c]) atomicAdd(&s_acc[c * 256 + 64 * k + lane], a[k][c]);
+  __syncthreads();
 }
 
-// Persistent: workgroups take chunks of MAP_CHUNK consecutive entries of the tile-sorted pair list, so a crowded
-// tile is shared by several workgroups. Inside a chunk, each run of equal tile ids is one tile's records; a tile
-// whose run lies inside one chunk is stored, a tile split across chunks is merged with integer atomic adds
-// (order-independent). acc of the dirty layers is zero on entry; tiles with no records are not touched.
-// stats[0] = pairs, stats[1] += tiles with records (zero on entry).
-constexpr uint32_t MAP_CHUNK = 1024;
+// Persistent: workgroups take chunks of MAP_CHUNK consecutive entries of the tile-sorted pair list by ticket (a
+// crowded tile is shared by several workgroups, and a workgroup that drew cheap chunks takes more). Inside a chunk,
+// each run of equal tile ids is one tile's records; a tile whose run lies inside one chunk is stored, a tile split
+// across chunks is merged with integer atomic adds (order-independent). acc of the dirty layers is zero on entry;
+// tiles with no records are not touched. ticket is zero on entry. stats[0] = pairs, stats[1] += tiles with records
+// (zero on entry). LDS is under 20 KiB per workgroup, so 8 workgroups (32 waves) fit a CU's 160 KiB.
 extern "C" __global__ void __launch_bounds__(256)
 sl_map_accumulate(const MapRec *__restrict__ recs, const uint32_t *__restrict__ keys,
                   const uint32_t *__restrict__ list, const FrameCounters *counters, uint32_t res, uint32_t tiles_x,
-                  uint32_t tpl, float fix, uint32_t *__restrict__ acc, uint32_t *__restrict__ stats) {
-  __shared__ MapRec s_rec[256];
+                  uint32_t tpl, float fix, uint32_t *__restrict__ acc, uint32_t *__restrict__ ticket,
+                  uint32_t *__restrict__ stats) {
+  __shared__ MapRec s_rec[MAP_STAGE];
   __shared__ uint32_t s_keys[MAP_CHUNK];
-  __shared__ uint32_t s_prev, s_next;
+  __shared__ uint32_t s_acc[3 * 256];
+  // the drawn chunk, broadcast through a word of s_rec: every wave reads it before the barrier that precedes the
+  // next record staging
+  uint32_t *s_ticket = &s_rec[0].layer;
   const uint32_t total = __hip_atomic_load(&counters->pairs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   const uint32_t n_chunks = (total + MAP_CHUNK - 1) / MAP_CHUNK;
+  for (uint32_t c = 0; c < 3; c++) s_acc[c * 256 + threadIdx.x] = 0u;
   uint32_t used = 0;
-  for (uint32_t c = blockIdx.x; c < n_chunks; c += gridDim.x) {
-    const uint32_t lo = c * MAP_CHUNK, hi = min(lo + MAP_CHUNK, total), n = hi - lo;
+  for (;;) {
+    __syncthreads();                                   // every wave is done with s_rec and s_keys
+    if (threadIdx.x == 0) *s_ticket = atomicAdd(ticket, 1u);
     __syncthreads();
+    const uint32_t c = *s_ticket;
+    if (c >= n_chunks) break;
+    const uint32_t lo = c * MAP_CHUNK, hi = min(lo + MAP_CHUNK, total), n = hi - lo;
+    const uint32_t prev = lo > 0u ? keys[lo - 1u] : 0xffffffffu, next = hi < total ? keys[hi] : 0xffffffffu;
     for (uint32_t j = threadIdx.x; j < n; j += 256u) s_keys[j] = keys[lo + j];
-    if (threadIdx.x == 0) {
-      s_prev = lo > 0u ? keys[lo - 1u] : 0xffffffffu;
-      s_next = hi < total ? keys[hi] : 0xffffffffu;
-    }
     __syncthreads();
     for (uint32_t s0 = 0; s0 < n;) {
       const uint32_t tile = s_keys[s0];
@@ -503,14 +536,18 @@ sl_map_accumulate(const MapRec *__restrict__ recs, const uint32_t *__restrict__ 
         if (s_keys[mid] <= tile) lo2 = mid + 1u; else hi2 = mid;
       }
       const uint32_t s1 = lo2;
-      const bool starts = s0 > 0u || s_prev != tile, ends = s1 < n || s_next != tile;
+      const bool starts = s0 > 0u || prev != tile, ends = s1 < n || next != tile;
       used += starts;
       const uint32_t layer = tile / tpl, rem = tile - layer * tpl, ty = rem / tiles_x, tx = rem - ty * tiles_x;
-      const int x = (int)(tx * MT + (threadIdx.x & 15u)), y = (int)(ty * MT + (threadIdx.x >> 4));
-      const int wy0 = (int)(ty * MT + (threadIdx.x >> 6) * 4u), wx0 = (int)(tx * MT);
-      uint32_t a0 = 0, a1 = 0, a2 = 0;
-      map_tile_sums(recs, list, lo + s0, lo + s1, x, y, wx0, wy0, fix, s_rec, a0, a1, a2);
-      if ((uint32_t)x < res && (uint32_t)y < res) {
+      map_tile_sums(recs, list, lo + s0, lo + s1, (int)(tx * MT), (int)(ty * MT), fix, s_rec, s_acc);
+      // texel threadIdx.x of the tile: its three sums, then cleared for the next tile (read and cleared by this
+      // thread only; the next adds come after at least one more barrier)
+      const uint32_t x = tx * MT + (threadIdx.x & 15u), y = ty * MT + (threadIdx.x >> 4);
+      const uint32_t a0 = s_acc[threadIdx.x], a1 = s_acc[256 + threadIdx.x], a2 = s_acc[512 + threadIdx.x];
+      s_acc[threadIdx.x] = 0u;
+      s_acc[256 + threadIdx.x] = 0u;
+      s_acc[512 + threadIdx.x] = 0u;
+      if (x < res && y < res) {
         uint32_t *k = acc + ((size_t)layer * res * res + (size_t)y * res + (size_t)x) * 3;
         if (starts && ends) {
           k[0] = a0;
