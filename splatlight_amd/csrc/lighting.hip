@@ -457,12 +457,51 @@ __device__ __forceinline__ void map_texel(const MapRec &r, float dx, float py, f
   a2 += u32_sat(t_fix * (z * z + r.sz2) + 0.5f);
 }
 
+// Which of the 4 bands (16x4 texels, rows ty0 + 4k ..) of the 16x16 tile at (tx0, ty0) may hold a texel where
+// record r adds something (q <= 9): bit k set unless the minimum of the record's quadratic form over the band's
+// texel-centre rectangle (clipped to the record's box) exceeds 9 by a margin. The rectangle's corners are the same
+// float differences px - ctx the sums compute, and rounding is monotonic, so every texel's offsets lie inside it;
+// the margin (1e-4 of the largest term magnitude on the rectangle, plus 1e-3) is some 100x the float rounding of
+// both the per-texel q and this minimum. Every band is kept when the form is not positive definite or not finite,
+// so a band with a contributing texel is never dropped.
+__device__ __forceinline__ uint32_t map_band_mask(const MapRec &r, int tx0, int ty0) {
+  const float a = r.s00, b = 0.5f * (r.s01 + r.s10), c = r.s11;
+  const bool pd = a > 0.0f && c > 0.0f && a * c - b * b > 0.0f && a < 1e30f && c < 1e30f;
+  const int xa = max(r.x0, tx0), xb = min(r.x1, tx0 + 15);
+  const float u0 = ((float)xa + 0.5f) - r.ctx, u1 = ((float)xb + 0.5f) - r.ctx, um = fmaxf(fabsf(u0), fabsf(u1));
+  uint32_t mask = 0;
+#pragma unroll 1
+  for (int k = 0; k < 4; k++) {
+    const int ya = max(r.y0, ty0 + 4 * k), yb = min(r.y1, ty0 + 4 * k + 3);
+    if (xa > xb || ya > yb) continue;
+    if (!pd) {
+      mask |= 1u << k;
+      continue;
+    }
+    const float v0 = ((float)ya + 0.5f) - r.cty, v1 = ((float)yb + 0.5f) - r.cty;
+    auto q = [&](float u, float v) { return (a * u * u + 2.0f * b * u * v) + c * v * v; };
+    float m;
+    if (u0 <= 0.0f && u1 >= 0.0f && v0 <= 0.0f && v1 >= 0.0f) {
+      m = 0.0f;                                          // the centre lies inside
+    } else {
+      // the minimum lies on an edge: on u = const the best v is -b u / c, on v = const the best u is -b v / a
+      m = fminf(fminf(q(u0, fminf(fmaxf(-b * u0 / c, v0), v1)), q(u1, fminf(fmaxf(-b * u1 / c, v0), v1))),
+                fminf(q(fminf(fmaxf(-b * v0 / a, u0), u1), v0), q(fminf(fmaxf(-b * v1 / a, u0), u1), v1)));
+    }
+    const float vm = fmaxf(fabsf(v0), fabsf(v1));
+    const float big = a * um * um + (fabsf(r.s01) + fabsf(r.s10)) * um * vm + c * vm * vm;
+    if (m <= 9.0f + 1e-4f * big + 1e-3f) mask |= 1u << k;
+  }
+  return mask;
+}
+
 // Per-tile sums of the records list[s0, s1) of the 16x16 tile at texel (tx0, ty0). Records are staged MAP_STAGE
-// at a time in LDS with one coalesced load per lane; wave w evaluates staged records w, w + 4, ... over the whole
-// tile, as four bands of 16x4 texels (lane: column lane & 15, row lane >> 4 of the band), skipping the bands a
-// record's box misses. Each record is read back from LDS by one wave only. The four waves' sums meet in s_acc
-// (integer adds, order-independent); on return s_acc[c][t] holds texel t's sum c (t = row * 16 + column) and the
-// caller clears it after reading.
+// at a time in LDS with one coalesced load per lane, each with its band mask for this tile (in place of its layer,
+// which the sums do not read); wave w evaluates staged records w, w + 4, ... over the whole tile, as four bands of
+// 16x4 texels (lane: column lane & 15, row lane >> 4 of the band), skipping the bands the mask rules out. Each
+// record is read back from LDS by one wave only. The four waves' sums meet in s_acc (integer adds,
+// order-independent); on return s_acc[c][t] holds texel t's sum c (t = row * 16 + column) and the caller clears it
+// after reading.
 __device__ __forceinline__ void map_tile_sums(const MapRec *__restrict__ recs, const uint32_t *__restrict__ list,
                                               uint32_t s0, uint32_t s1, int tx0, int ty0, float fix, MapRec *s_rec,
                                               uint32_t *s_acc) {
@@ -473,17 +512,20 @@ __device__ __forceinline__ void map_tile_sums(const MapRec *__restrict__ recs, c
   for (uint32_t b = s0; b < s1; b += MAP_STAGE) {
     const uint32_t nb = min(MAP_STAGE, s1 - b);
     __syncthreads();
-    if (threadIdx.x < nb) s_rec[threadIdx.x] = recs[list[b + threadIdx.x]];
+    if (threadIdx.x < nb) {
+      MapRec r = recs[list[b + threadIdx.x]];
+      r.layer = map_band_mask(r, tx0, ty0);
+      s_rec[threadIdx.x] = r;
+    }
     __syncthreads();
     for (uint32_t j = wave; j < nb; j += 4u) {
       const MapRec r = s_rec[j];
       const bool xin = x >= r.x0 && x <= r.x1;
       const float dx = px - r.ctx;
-      // bands [k0, k1] of this tile meet the record's rows (the same for every lane)
-      const int k0 = max(r.y0 - ty0, 0) >> 2, k1 = min(r.y1 - ty0, 15) >> 2;
+      const uint32_t mask = r.layer;                     // the same for every lane
 #pragma unroll
       for (int k = 0; k < 4; k++) {
-        if (k < k0 || k > k1) continue;
+        if (!((mask >> k) & 1u)) continue;
         const int y = ty0 + 4 * k + yl;
         if (xin && y >= r.y0 && y <= r.y1) map_texel(r, dx, (float)y + 0.5f, fix, a[k][0], a[k][1], a[k][2]);
       }
@@ -497,75 +539,64 @@ __device__ __forceinline__ void map_tile_sums(const MapRec *__restrict__ recs, c
   __syncthreads();
 }
 
-// Persistent: workgroups take chunks of MAP_CHUNK consecutive entries of the tile-sorted pair list by ticket (a
-// crowded tile is shared by several workgroups, and a workgroup that drew cheap chunks takes more). Inside a chunk,
-// each run of equal tile ids is one tile's records; a tile whose run lies inside one chunk is stored, a tile split
-// across chunks is merged with integer atomic adds (order-independent). acc of the dirty layers is zero on entry;
-// tiles with no records are not touched. ticket is zero on entry. stats[0] = pairs, stats[1] += tiles with records
-// (zero on entry). LDS is under 20 KiB per workgroup, so 8 workgroups (32 waves) fit a CU's 160 KiB.
-extern "C" __global__ void __launch_bounds__(256)
+// One workgroup per chunk of MAP_CHUNK consecutive entries of the tile-sorted pair list (the grid covers the
+// pair capacity; workgroups past the pair count exit at once). An ordinary grid rather than a persistent one, so
+// kernels of another stream get CUs as these workgroups retire. Inside a chunk, each run of equal tile ids is one
+// tile's records; a tile whose run lies inside one chunk is stored, a tile split across chunks (a crowded tile is
+// shared by several workgroups) is merged with integer atomic adds (order-independent). acc of the dirty layers is
+// zero on entry; tiles with no records are not touched. stats[0] = pairs, stats[1] += tiles with records (zero on
+// entry). LDS is under 20 KiB per workgroup, so 8 workgroups (32 waves) fit a CU's 160 KiB.
+extern "C" __global__ void __launch_bounds__(256, 8)
 sl_map_accumulate(const MapRec *__restrict__ recs, const uint32_t *__restrict__ keys,
                   const uint32_t *__restrict__ list, const FrameCounters *counters, uint32_t res, uint32_t tiles_x,
-                  uint32_t tpl, float fix, uint32_t *__restrict__ acc, uint32_t *__restrict__ ticket,
-                  uint32_t *__restrict__ stats) {
+                  uint32_t tpl, float fix, uint32_t *__restrict__ acc, uint32_t *__restrict__ stats) {
   __shared__ MapRec s_rec[MAP_STAGE];
   __shared__ uint32_t s_keys[MAP_CHUNK];
   __shared__ uint32_t s_acc[3 * 256];
-  // the drawn chunk, broadcast through a word of s_rec: every wave reads it before the barrier that precedes the
-  // next record staging
-  uint32_t *s_ticket = &s_rec[0].layer;
   const uint32_t total = __hip_atomic_load(&counters->pairs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  const uint32_t n_chunks = (total + MAP_CHUNK - 1) / MAP_CHUNK;
+  if (blockIdx.x == 0 && threadIdx.x == 0) stats[0] = total;
+  const uint32_t lo = blockIdx.x * MAP_CHUNK;
+  if (lo >= total) return;
+  const uint32_t hi = min(lo + MAP_CHUNK, total), n = hi - lo;
+  const uint32_t prev = lo > 0u ? keys[lo - 1u] : 0xffffffffu, next = hi < total ? keys[hi] : 0xffffffffu;
   for (uint32_t c = 0; c < 3; c++) s_acc[c * 256 + threadIdx.x] = 0u;
+  for (uint32_t j = threadIdx.x; j < n; j += 256u) s_keys[j] = keys[lo + j];
+  __syncthreads();
   uint32_t used = 0;
-  for (;;) {
-    __syncthreads();                                   // every wave is done with s_rec and s_keys
-    if (threadIdx.x == 0) *s_ticket = atomicAdd(ticket, 1u);
-    __syncthreads();
-    const uint32_t c = *s_ticket;
-    if (c >= n_chunks) break;
-    const uint32_t lo = c * MAP_CHUNK, hi = min(lo + MAP_CHUNK, total), n = hi - lo;
-    const uint32_t prev = lo > 0u ? keys[lo - 1u] : 0xffffffffu, next = hi < total ? keys[hi] : 0xffffffffu;
-    for (uint32_t j = threadIdx.x; j < n; j += 256u) s_keys[j] = keys[lo + j];
-    __syncthreads();
-    for (uint32_t s0 = 0; s0 < n;) {
-      const uint32_t tile = s_keys[s0];
-      uint32_t lo2 = s0 + 1u, hi2 = n;                 // the run's end: the first key past `tile`
-      while (lo2 < hi2) {
-        const uint32_t mid = (lo2 + hi2) >> 1;
-        if (s_keys[mid] <= tile) lo2 = mid + 1u; else hi2 = mid;
-      }
-      const uint32_t s1 = lo2;
-      const bool starts = s0 > 0u || prev != tile, ends = s1 < n || next != tile;
-      used += starts;
-      const uint32_t layer = tile / tpl, rem = tile - layer * tpl, ty = rem / tiles_x, tx = rem - ty * tiles_x;
-      map_tile_sums(recs, list, lo + s0, lo + s1, (int)(tx * MT), (int)(ty * MT), fix, s_rec, s_acc);
-      // texel threadIdx.x of the tile: its three sums, then cleared for the next tile (read and cleared by this
-      // thread only; the next adds come after at least one more barrier)
-      const uint32_t x = tx * MT + (threadIdx.x & 15u), y = ty * MT + (threadIdx.x >> 4);
-      const uint32_t a0 = s_acc[threadIdx.x], a1 = s_acc[256 + threadIdx.x], a2 = s_acc[512 + threadIdx.x];
-      s_acc[threadIdx.x] = 0u;
-      s_acc[256 + threadIdx.x] = 0u;
-      s_acc[512 + threadIdx.x] = 0u;
-      if (x < res && y < res) {
-        uint32_t *k = acc + ((size_t)layer * res * res + (size_t)y * res + (size_t)x) * 3;
-        if (starts && ends) {
-          k[0] = a0;
-          k[1] = a1;
-          k[2] = a2;
-        } else {
-          if (a0) atomicAdd(k, a0);
-          if (a1) atomicAdd(k + 1, a1);
-          if (a2) atomicAdd(k + 2, a2);
-        }
-      }
-      s0 = s1;
+  for (uint32_t s0 = 0; s0 < n;) {
+    const uint32_t tile = s_keys[s0];
+    uint32_t lo2 = s0 + 1u, hi2 = n;                   // the run's end: the first key past `tile`
+    while (lo2 < hi2) {
+      const uint32_t mid = (lo2 + hi2) >> 1;
+      if (s_keys[mid] <= tile) lo2 = mid + 1u; else hi2 = mid;
     }
+    const uint32_t s1 = lo2;
+    const bool starts = s0 > 0u || prev != tile, ends = s1 < n || next != tile;
+    used += starts;
+    const uint32_t layer = tile / tpl, rem = tile - layer * tpl, ty = rem / tiles_x, tx = rem - ty * tiles_x;
+    map_tile_sums(recs, list, lo + s0, lo + s1, (int)(tx * MT), (int)(ty * MT), fix, s_rec, s_acc);
+    // texel threadIdx.x of the tile: its three sums, then cleared for the next tile (read and cleared by this
+    // thread only; the next adds come after at least one more barrier)
+    const uint32_t x = tx * MT + (threadIdx.x & 15u), y = ty * MT + (threadIdx.x >> 4);
+    const uint32_t a0 = s_acc[threadIdx.x], a1 = s_acc[256 + threadIdx.x], a2 = s_acc[512 + threadIdx.x];
+    s_acc[threadIdx.x] = 0u;
+    s_acc[256 + threadIdx.x] = 0u;
+    s_acc[512 + threadIdx.x] = 0u;
+    if (x < res && y < res) {
+      uint32_t *k = acc + ((size_t)layer * res * res + (size_t)y * res + (size_t)x) * 3;
+      if (starts && ends) {
+        k[0] = a0;
+        k[1] = a1;
+        k[2] = a2;
+      } else {
+        if (a0) atomicAdd(k, a0);
+        if (a1) atomicAdd(k + 1, a1);
+        if (a2) atomicAdd(k + 2, a2);
+      }
+    }
+    s0 = s1;
   }
-  if (threadIdx.x == 0) {
-    if (used) atomicAdd(stats + 1, used);
-    if (blockIdx.x == 0) stats[0] = total;
-  }
+  if (threadIdx.x == 0 && used) atomicAdd(stats + 1, used);
 }
 
 namespace lt {

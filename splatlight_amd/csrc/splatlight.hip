@@ -1207,8 +1207,8 @@ void run_map_splat(sl_context *ctx, hipStream_t st, uint32_t which, size_t n_rec
     std::swap(vin, vout);
   }
   if (timers) (void)hipEventRecord(ctx->mev[2 * which], st);
-  sl_map_accumulate<<<ctx->cus * 8, 256, 0, st>>>((const MapRec *)m.rec.p, kin, vin, counters, res, tiles_x,
-                                                  tpl, fix, acc, w + ms.tickets + 4, stats);
+  sl_map_accumulate<<<(uint32_t)((cap + MAP_CHUNK - 1) / MAP_CHUNK), 256, 0, st>>>(
+      (const MapRec *)m.rec.p, kin, vin, counters, res, tiles_x, tpl, fix, acc, stats);
   if (timers) (void)hipEventRecord(ctx->mev[2 * which + 1], st);
   ctx->map_timed[which] = timers;
   ctx->map_records[which] = (uint32_t)n_rec;
