@@ -403,6 +403,92 @@ sl_opacity_down(float4 *__restrict__ mip, uint32_t n_layers, uint32_t rk, uint32
                   0.25f * (((a.z + b.z) + c.z) + d.z), 0.25f * (((a.w + b.w) + c.w) + d.w));
 }
 
+// The same resolve and mips, fused: one 256-thread workgroup per 32x32 texel region of a layer (blockIdx.y is the
+// layer; small workgroups, so they find CUs beside another stream's kernels). Each thread resolves a 2x2 quad of
+// level 0 and averages it into level 1; levels 2-5 of the region come from LDS. Same arithmetic and order as
+// sl_opacity_resolve / sl_opacity_down, so the levels are bit-identical. Needs res a multiple of 32; levels past 5
+// are left to sl_opacity_down / sl_opacity_tail.
+struct LevelOffs { uint32_t off[16]; };
+__device__ __forceinline__ float4 resolve_texel(const uint32_t *__restrict__ acc, size_t i, float fix) {
+  const float s0 = (float)acc[i * 3] / fix;
+  float4 o = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (s0 > 1e-5f) {
+    const float a = (float)(-expm1(-(double)s0));
+    o = make_float4(a, a * (float)acc[i * 3 + 1] / (fix * s0), a * (float)acc[i * 3 + 2] / (fix * s0), 0.0f);
+  }
+  return o;
+}
+__device__ __forceinline__ float4 avg4(float4 a, float4 b, float4 c, float4 d) {
+  return make_float4(0.25f * (((a.x + b.x) + c.x) + d.x), 0.25f * (((a.y + b.y) + c.y) + d.y),
+                     0.25f * (((a.z + b.z) + c.z) + d.z), 0.25f * (((a.w + b.w) + c.w) + d.w));
+}
+// levels k0 .. k1 - 1 of a square block held in LDS at `src` (side `side`, the level k0 - 1 values), written to
+// lm at level offsets lo.off with row length res >> k and the block's origin (bx, by) in level-k0-1 texels / 2^..
+__device__ __forceinline__ void lds_levels(float4 *src, float4 *dst, uint32_t side, uint32_t k0, uint32_t k1,
+                                           uint32_t res, uint32_t bx, uint32_t by, const LevelOffs &lo, float4 *lm) {
+  for (uint32_t k = k0; k < k1; k++) {
+    __syncthreads();
+    const uint32_t ns = side >> 1;
+    for (uint32_t j = threadIdx.x; j < ns * ns; j += blockDim.x) {
+      const uint32_t tx = j % ns, ty = j / ns;
+      const float4 *p = src + (2u * ty) * side + 2u * tx;
+      const float4 w = avg4(p[0], p[1], p[side], p[side + 1]);
+      dst[j] = w;
+      lm[lo.off[k] + (size_t)(by * ns + ty) * (res >> k) + (bx * ns + tx)] = w;
+    }
+    float4 *t = src;
+    src = dst;
+    dst = t;
+    side = ns;
+  }
+}
+extern "C" __global__ void __launch_bounds__(256)
+sl_opacity_pyramid(const uint32_t *__restrict__ acc, float4 *__restrict__ mip, uint32_t res, uint32_t levels,
+                   LevelOffs lo, uint32_t layer_texels, float fix, const uint32_t *__restrict__ dirty) {
+  __shared__ float4 s1[16 * 16];
+  __shared__ float4 s2[8 * 8];
+  const uint32_t l = blockIdx.y;
+  if (!dirty[l]) return;
+  const uint32_t regions_x = res / 32u, rx = blockIdx.x % regions_x, ry = blockIdx.x / regions_x;
+  const uint32_t qx = threadIdx.x & 15u, qy = threadIdx.x >> 4;
+  const uint32_t x = rx * 32u + 2u * qx, y = ry * 32u + 2u * qy;
+  const uint32_t *lacc = acc + (size_t)l * res * res * 3;
+  float4 *lm = mip + (size_t)l * layer_texels;
+  const size_t i00 = (size_t)y * res + x;
+  const float4 a = resolve_texel(lacc, i00, fix), b = resolve_texel(lacc, i00 + 1, fix);
+  const float4 c = resolve_texel(lacc, i00 + res, fix), d = resolve_texel(lacc, i00 + res + 1, fix);
+  lm[i00] = a;
+  lm[i00 + 1] = b;
+  lm[i00 + res] = c;
+  lm[i00 + res + 1] = d;
+  if (levels < 2u) return;
+  const float4 v = avg4(a, b, c, d);
+  lm[lo.off[1] + (size_t)(y >> 1) * (res >> 1) + (x >> 1)] = v;
+  s1[threadIdx.x] = v;
+  lds_levels(s1, s2, 16u, 2u, min(levels, 6u), res, rx, ry, lo, lm);
+}
+
+// Levels k_first .. levels - 1 of every dirty layer, one workgroup per layer: level k_first (side at most 32) from
+// level k_first - 1 in mip, the rest through LDS. Same arithmetic and order as sl_opacity_down.
+extern "C" __global__ void __launch_bounds__(256)
+sl_opacity_tail(float4 *__restrict__ mip, uint32_t res, uint32_t k_first, uint32_t levels, LevelOffs lo,
+                uint32_t layer_texels, const uint32_t *__restrict__ dirty) {
+  __shared__ float4 sa[32 * 32];
+  __shared__ float4 sb[16 * 16];
+  const uint32_t l = blockIdx.x;
+  if (!dirty[l]) return;
+  float4 *lm = mip + (size_t)l * layer_texels;
+  const uint32_t side = res >> k_first, pr = side * 2u;
+  for (uint32_t j = threadIdx.x; j < side * side; j += blockDim.x) {
+    const uint32_t tx = j % side, ty = j / side;
+    const float4 *p = lm + lo.off[k_first - 1] + (size_t)(2u * ty) * pr + 2u * tx;
+    const float4 w = avg4(p[0], p[1], p[pr], p[pr + 1]);
+    sa[j] = w;
+    lm[lo.off[k_first] + j] = w;
+  }
+  lds_levels(sa, sb, side, k_first + 1u, levels, res, 0u, 0u, lo, lm);
+}
+
 // ---- opacity-map splats as tiled sums -------------------------------------------------------------------------
 //
 // The reference splats each Gaussian with one thread looping over its texel box and three u32 atomics per texel

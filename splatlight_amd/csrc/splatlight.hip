@@ -1119,9 +1119,9 @@ namespace {
 template <class T>
 void read_struct(const void *src, T *dst) { memcpy(dst, src, sizeof(T)); }
 
-// every mip level of a layer lies inside it
+// every mip level of a layer lies inside it, level 0 first (the resolve writes it at the layer's start)
 bool levels_fit(uint32_t res, uint32_t levels, const uint32_t *level_off, uint32_t layer_texels) {
-  if (res < 1 || res > 16384 || (res & (res - 1)) != 0 || levels < 1 || levels > 16) return false;
+  if (res < 1 || res > 16384 || (res & (res - 1)) != 0 || levels < 1 || levels > 16 || level_off[0] != 0) return false;
   for (uint32_t k = 0; k < levels; k++) {
     const uint32_t rk = res >> k;
     if (rk < 1) return false;
@@ -1223,6 +1223,21 @@ uint32_t map_span(uint32_t max_half, uint32_t res) {
 void run_opacity_maps(hipStream_t st, float4 *mip, uint32_t n_layers, uint32_t res, uint32_t levels,
                       const uint32_t *level_off, uint32_t layer_texels, float fix, const uint32_t *acc,
                       const uint32_t *dirty_dev) {
+  LevelOffs lo = {};
+  for (uint32_t k = 0; k < levels && k < 16u; k++) lo.off[k] = level_off[k];
+  if (res % 32u == 0u) {
+    // levels 0-5 per 32x32 region, then full-size downs while a level is wider than 32, then the rest per layer
+    sl_opacity_pyramid<<<dim3((res / 32u) * (res / 32u), n_layers), 256, 0, st>>>(acc, mip, res, levels, lo,
+                                                                                  layer_texels, fix, dirty_dev);
+    uint32_t k = 6;
+    for (; k < levels && (res >> k) > 32u; k++) {
+      const uint32_t rk = res >> k;
+      sl_opacity_down<<<(uint32_t)(((size_t)rk * rk * n_layers + 255) / 256), 256, 0, st>>>(
+          mip, n_layers, rk, level_off[k - 1], level_off[k], layer_texels, dirty_dev);
+    }
+    if (k < levels) sl_opacity_tail<<<n_layers, 256, 0, st>>>(mip, res, k, levels, lo, layer_texels, dirty_dev);
+    return;
+  }
   const size_t texels = (size_t)res * res;
   sl_opacity_resolve<<<(uint32_t)((texels * n_layers + 255) / 256), 256, 0, st>>>(acc, mip, n_layers, (uint32_t)texels,
                                                                                  layer_texels, fix, dirty_dev);
