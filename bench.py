@@ -6,8 +6,8 @@ sort -> tile ranges -> composite. The scene is resident in HBM before timing sta
 1M Gaussians, 64 emissive Gaussians and a sun with a 2048^2 map, 1920x1080 (16x16 tiles: 8,160). The rate with
 the lighting cached (nothing moved, the reference's still-skip case) is reported beside it.
 
-Prints one JSON line (value = Mpix/s over the timed frames) with the composite kernel's HBM roofline and the
-CPU restatement's rate on a bounded sample of the same workload.
+Prints one JSON line (value = Mpix/s over the timed frames) with the roofline of the kernel launch that takes the
+most time per frame and the CPU restatement's rate on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -41,6 +41,11 @@ def parse():
     ap.add_argument("--pmc-summary", nargs=4, metavar=("FETCH_DIR", "WRITE_DIR", "COPY_FETCH_DIR", "COPY_WRITE_DIR"),
                     help="summarise rocprofv3 --pmc output directories into --traffic and exit")
     ap.add_argument("--pmc-command", default="", help="the profiling command, recorded in the summary")
+    ap.add_argument("--pmc-valu", default="",
+                    help="with --pmc-summary: a rocprofv3 --pmc run of SQ_INSTS_VALU SQ_INSTS_SALU SQ_INSTS_LDS "
+                         "SQ_WAVES GRBM_GUI_ACTIVE, summarised beside the traffic")
+    ap.add_argument("--stats-summary", nargs=2, metavar=("KERNEL_STATS_CSV", "OUT"),
+                    help="write a markdown summary of a rocprofv3 --kernel-trace --stats run and exit")
     ap.add_argument("--batched", type=int, default=1, help="also measure the batched form (64 views, 8 scenes)")
     ap.add_argument("--serving", type=int, default=1, help="also measure the served form (Poisson arrivals)")
     ap.add_argument("--rate", type=float, default=1000.0, help="serving arrival rate, requests per second")
@@ -291,63 +296,89 @@ def storage_leg(N, L, scenes, steps=5):
 
 
 def kernel_bytes(passes, P, T, target_bytes, ls):
-    """Per-launch algorithmic HBM bytes and HIP-event milliseconds of the two heaviest kernels of a relit frame.
+    """Per-launch algorithmic HBM bytes and HIP-event milliseconds of the heaviest kernel launches of a relit frame.
 
     sl_composite (one launch): a 48 B splat fetch and a 4 B pair value per pair, 8 B per tile range, the target.
-    sl_map_accumulate (two launches, emitter map then sun maps): a 64 B record and a 4 B list entry per
-    (record, map tile) pair, 4 B per tile start, 12 B per texel of every map tile that has records.
+    sl_map_accumulate<0u> (emitter map) and <1u> (sun maps), one launch each: a 64 B record and a 4 B list entry
+    per (record, map tile) pair, 4 B per tile key, 12 B per texel of every map tile that has records.
     """
-    comp = {"kernel": "sl_composite", "launches": 1, "ms": float(np.mean(passes["splat"])),
-            "bytes": 48 * P + 4 * P + 8 * T + target_bytes}
-    if "lsm_sum" not in passes and "sun_sum" not in passes:
-        return comp, None
-    b, ms, n = 0, 0.0, 0
-    for m, key in (("lsm", "lsm_sum"), ("sun", "sun_sum")):
+    out = [{"kernel": "sl_composite", "ms": float(np.mean(passes["splat"])),
+            "bytes": 48 * P + 4 * P + 8 * T + target_bytes}]
+    for kind, m, key in ((0, "lsm", "lsm_sum"), (1, "sun", "sun_sum")):
         if key in passes:
-            b += 68 * ls[f"{m}_pairs"] + 4 * (ls[f"{m}_tiles"] + 1) + 256 * 12 * ls[f"{m}_tiles_used"]
-            ms += float(np.mean(passes[key]))
-            n += 1
-    return comp, {"kernel": "sl_map_accumulate", "launches": n, "ms": ms / n, "bytes": b // n}
+            out.append({"kernel": f"sl_map_accumulate<{kind}u>", "ms": float(np.mean(passes[key])),
+                        "bytes": 68 * ls[f"{m}_pairs"] + 4 * (ls[f"{m}_tiles"] + 1)
+                        + 256 * 12 * ls[f"{m}_tiles_used"]})
+    return out
 
 
 def roofline(passes, P, T, target_bytes, ls, traffic_file):
-    """The roofline object of the kernel that takes the most time per frame (HIP events on the library's stream)."""
-    comp, msum = kernel_bytes(passes, P, T, target_bytes, ls)
-    k = comp if msum is None or comp["ms"] * comp["launches"] >= msum["ms"] * msum["launches"] else msum
+    """The roofline object of the kernel launch that takes the most time per frame (HIP events on the stream the
+    kernel is launched on)."""
+    k = max(kernel_bytes(passes, P, T, target_bytes, ls), key=lambda e: e["ms"])
     achieved = k["bytes"] / (k["ms"] * 1e-3) / 1e9
-    return {"bound": "hbm", "kernel": k["kernel"], "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(traffic_file,
-                                                                                                 k["kernel"]),
-            "algorithmic_bytes": int(k["bytes"]), "avg_ms": round(k["ms"], 5), "launches_per_frame": k["launches"]}
+    out = {"bound": "hbm", "kernel": k["kernel"], "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+           "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+           "traffic": summary_entry(traffic_file, "TRAFFIC", k["kernel"], "bytes_per_launch"),
+           "algorithmic_bytes": int(k["bytes"]), "avg_ms": round(k["ms"], 5), "launches_per_frame": 1}
+    valu = summary_entry(traffic_file, "VALU", k["kernel"], None)
+    if valu:
+        out["valu"] = valu
+    return out
 
 
-def measured_traffic(path, kernel):
-    """HBM bytes per launch of `kernel` from a rocprof FETCH_SIZE / WRITE_SIZE summary (see --pmc-summary), or
-    None when no summary for it is at hand."""
+def short_name(kernel):
+    """rocprof's kernel name without 'void ' and the argument list: sl_map_accumulate<1u>."""
+    n = kernel[5:] if kernel.startswith("void ") else kernel
+    return n.split("(")[0]
+
+
+def summary_entry(path, tag, kernel, field):
+    """A per-kernel figure from the rocprof summary written by --pmc-summary (its `tag` lines), or None."""
     if not path or not os.path.exists(path):
         return None
     with open(path) as f:
         for line in f:
-            if line.startswith("TRAFFIC "):
-                d = json.loads(line[len("TRAFFIC "):])
+            if line.startswith(tag + " "):
+                d = json.loads(line[len(tag) + 1:])
                 if d.get("kernel") == kernel:
-                    return d["bytes_per_launch"]
+                    return d if field is None else d[field]
     return None
 
 
-def pmc_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, out, command):
-    """Per-kernel HBM bytes from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs (KiB per dispatch), each scaled by
-    what the counter reads for a plain copy kernel of known size; writes a short text summary."""
+def stats_summary(csv_path, out, command):
+    """Markdown table of a rocprofv3 --kernel-trace --stats kernel_stats.csv, with the command that made it."""
     import csv
+    rows = list(csv.DictReader(open(csv_path)))
+    lines = ["# Kernel time, relit frame: hill 1M Gaussians, 64 emitters + sun 2048^2, 1920x1080 rgba16f", "",
+             "Command (on one MI355X):", "", "    " + command, "",
+             "Per relit frame: one launch each of sl_map_accumulate<0u> (emitter map) and <1u> (sun maps), "
+             "sl_shade_emit, sl_env_direct and the frame path; the cached-lighting frames add frame-path launches "
+             "only. The emitter-map chain and the sun-map chain run on two streams at once, so kernel times include "
+             "sharing the GPU.", "",
+             "| kernel | calls | avg us | min us | max us | share |", "|---|---|---|---|---|---|"]
+    for r in rows:
+        lines.append(f"| {short_name(r['Name'])} | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
+                     f"{float(r['MinNs']) / 1e3:.1f} | {float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.2f}% |")
+    with open(out, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
+def pmc_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, out, command, valu_dir=""):
+    """Per-kernel HBM bytes from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs (KiB per dispatch), each scaled by
+    what the counter reads for a plain copy kernel of known size, and, from a valu_dir run, instruction counts per
+    dispatch; writes a short text summary."""
+    import csv
+    import glob
     from collections import defaultdict
 
     def per_kernel(d, counter):
-        import glob
         vals = defaultdict(list)
         for p in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
             for r in csv.DictReader(open(p)):
                 if r["Counter_Name"] == counter:
-                    vals[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+                    vals[short_name(r["Kernel_Name"])].append(float(r["Counter_Value"]))
         return vals
 
     def copy_scale(d, counter, nbytes):
@@ -355,6 +386,7 @@ def pmc_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, out, comma
         k = max(v, key=lambda name: np.mean(v[name]))
         return nbytes / (np.mean(v[k]) * 1024.0), k, float(np.mean(v[k]))
 
+    keep = ("sl_composite", "sl_map_accumulate<0u>", "sl_map_accumulate<1u>", "sl_shade_emit")
     copy_bytes = 1024 << 20
     fs, fk, fv = copy_scale(copy_fetch_dir, "FETCH_SIZE", copy_bytes)
     ws, wk, wv = copy_scale(copy_write_dir, "WRITE_SIZE", copy_bytes)
@@ -369,14 +401,36 @@ def pmc_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, out, comma
     for name in sorted(set(fetch) | set(write), key=lambda n: -np.mean(fetch.get(n, [0]))):
         f_kib, w_kib = float(np.mean(fetch.get(name, [0]))), float(np.mean(write.get(name, [0])))
         b = f_kib * 1024 * fs + w_kib * 1024 * ws
-        rows.append((name, len(fetch.get(name, [])), f_kib, w_kib, b))
+        rows.append((name, f_kib, w_kib, b))
         lines.append(f"| {name[:48]} | {len(fetch.get(name, []))} | {f_kib:.0f} | {w_kib:.0f} | {b:.4g} |")
     lines.append("")
-    for name, _, f_kib, w_kib, b in rows:
-        if name in ("sl_composite", "sl_map_accumulate"):
+    for name, f_kib, w_kib, b in rows:
+        if name in keep:
             lines.append("TRAFFIC " + json.dumps({"kernel": name, "bytes_per_launch": int(b),
                                                   "fetch_kib": round(f_kib, 1), "write_kib": round(w_kib, 1),
                                                   "fetch_scale": round(fs, 4), "write_scale": round(ws, 4)}))
+    if valu_dir:
+        cnt = {c: per_kernel(valu_dir, c) for c in ("SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_LDS", "SQ_WAVES",
+                                                     "GRBM_GUI_ACTIVE")}
+        lines += ["", "Instructions per dispatch (wave instructions, summed over the chip; rocprofv3 --pmc "
+                  "SQ_INSTS_VALU SQ_INSTS_SALU SQ_INSTS_LDS SQ_WAVES GRBM_GUI_ACTIVE, one run). GRBM_GUI_ACTIVE "
+                  "sums the 8 XCDs' busy cycles, so CU-cycles = 32 x GRBM_GUI_ACTIVE (256 CUs) and VALU issue per "
+                  "CU-cycle = SQ_INSTS_VALU / CU-cycles.", "",
+                  "| kernel | dispatches | VALU | SALU | LDS | waves | GRBM_GUI_ACTIVE | VALU per CU-cycle |",
+                  "|---|---|---|---|---|---|---|---|"]
+        valu_rows = []
+        for name in sorted(cnt["SQ_INSTS_VALU"], key=lambda n: -np.mean(cnt["SQ_INSTS_VALU"][n])):
+            m = {c: float(np.mean(cnt[c].get(name, [0]))) for c in cnt}
+            rate = m["SQ_INSTS_VALU"] / (32.0 * m["GRBM_GUI_ACTIVE"]) if m["GRBM_GUI_ACTIVE"] else 0.0
+            valu_rows.append((name, m, rate))
+            lines.append(f"| {name[:48]} | {len(cnt['SQ_INSTS_VALU'][name])} | {m['SQ_INSTS_VALU']:.4g} | "
+                         f"{m['SQ_INSTS_SALU']:.4g} | {m['SQ_INSTS_LDS']:.4g} | {m['SQ_WAVES']:.0f} | "
+                         f"{m['GRBM_GUI_ACTIVE']:.4g} | {rate:.3f} |")
+        lines.append("")
+        for name, m, rate in valu_rows:
+            if name in keep:
+                lines.append("VALU " + json.dumps({"kernel": name, "insts_per_launch": int(m["SQ_INSTS_VALU"]),
+                                                   "issue_per_cu_cycle": round(rate, 3)}))
     with open(out, "w") as f:
         f.write("\n".join(lines) + "\n")
     print("\n".join(lines))
@@ -385,7 +439,10 @@ def pmc_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, out, comma
 def main():
     a = parse()
     if a.pmc_summary:
-        pmc_summary(*a.pmc_summary, a.traffic, a.pmc_command)
+        pmc_summary(*a.pmc_summary, a.traffic, a.pmc_command, a.pmc_valu)
+        return
+    if a.stats_summary:
+        stats_summary(*a.stats_summary, a.pmc_command)
         return
     if a.copy_mib:
         copy_calibration(a.copy_mib)

@@ -155,8 +155,9 @@ __device__ __forceinline__ float vgrid_cell(const uint8_t *v, uint32_t i, uint32
   return (float)v[i * 16u + j] * (1.0f / 255.0f);
 }
 // bilinear between cell centres, so continuous across cell boundaries: the float asin's last-ulp behaviour moves V
-// by rounding only (the host restatement rounds asin once from double)
-__device__ __forceinline__ float vgrid_lookup(const uint8_t *v, v3 w) {   // common.wgsl:162-177
+// by rounding only (the host restatement rounds asin once from double). cell(i, j): the grid byte scaled by 1/255.
+template <typename Cell>
+__device__ __forceinline__ float vgrid_lookup_f(Cell cell, v3 w) {   // common.wgsl:162-177
   const float el = asinf(clampf_(w.z, -1.0f, 1.0f));
   const float fi = clampf_((0.5f * PI_F - el) * (8.0f / PI_F) - 0.5f, 0.0f, 7.0f);
   const uint32_t i0 = min((uint32_t)fi, 6u);
@@ -167,9 +168,12 @@ __device__ __forceinline__ float vgrid_lookup(const uint8_t *v, v3 w) {   // com
   const float tj = fj - j0f;
   const uint32_t j0 = (uint32_t)((int32_t)j0f + 16) % 16u;
   const uint32_t j1 = (j0 + 1u) % 16u;
-  const float a = mixf(vgrid_cell(v, i0, j0), vgrid_cell(v, i0, j1), tj);
-  const float b = mixf(vgrid_cell(v, i0 + 1u, j0), vgrid_cell(v, i0 + 1u, j1), tj);
+  const float a = mixf(cell(i0, j0), cell(i0, j1), tj);
+  const float b = mixf(cell(i0 + 1u, j0), cell(i0 + 1u, j1), tj);
   return mixf(a, b, ti);
+}
+__device__ __forceinline__ float vgrid_lookup(const uint8_t *v, v3 w) {
+  return vgrid_lookup_f([&](uint32_t i, uint32_t j) { return vgrid_cell(v, i, j); }, w);
 }
 __device__ __forceinline__ v3 basis_k(v3 w) {
   v3 a = V3(0.0f, 0.0f, 1.0f);
@@ -272,7 +276,10 @@ sl_iso_init(uint32_t n, const float4 *__restrict__ g2, uint32_t st, const float4
 
 // ---- shade_emit (runtime/wgsl/emissive.wgsl:123-150, 853-980), M39 matte ----------------------------------------
 
-extern "C" __global__ void __launch_bounds__(64)
+// STACK: the cut walk's stack depth in LDS, 48 as the reference's, or 16 when the host has shown that walking every
+// internal node of this tree never holds more than 16 entries (then no cut can either, and the walk is the same).
+template <uint32_t STACK>
+__global__ void __launch_bounds__(64)
 sl_shade_emit(FrameUniformsDev U, EmitParamsDev EP, const uint32_t *__restrict__ order,
               const float4 *__restrict__ emit_nodes, uint32_t n_nodes,
               const float4 *__restrict__ g0, const float4 *__restrict__ g1, const float4 *__restrict__ g2,
@@ -299,12 +306,31 @@ sl_shade_emit(FrameUniformsDev U, EmitParamsDev EP, const uint32_t *__restrict__
   const float detC = s3 * s3;
   const v3 wo = norm3(sub3(V3(U.eye[0], U.eye[1], U.eye[2]), P));
   const NodeL nd = load_node(nodes, g);
-  const uint8_t *vg = vgrid + 128 * (size_t)g;
+  // the Gaussian's 128 B visibility grid, read once into LDS as 32 words per lane in a [word][lane] layout (lanes
+  // on distinct banks whatever cells they read): the walk looks up 4 cells per used node, and re-reading them from
+  // memory costs a cache line each time once other waves have evicted it
+  __shared__ uint32_t s_vg[32][64];
+  uint32_t *vcol = &s_vg[0][threadIdx.x];
+  {
+    const uint4 *src = (const uint4 *)(vgrid + 128 * (size_t)g);
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const uint4 v = src[q];
+      vcol[64 * (4 * q)] = v.x;
+      vcol[64 * (4 * q + 1)] = v.y;
+      vcol[64 * (4 * q + 2)] = v.z;
+      vcol[64 * (4 * q + 3)] = v.w;
+    }
+  }
+  auto vcell = [&](uint32_t i, uint32_t j) {
+    const uint32_t c = i * 16u + j;
+    return (float)((vcol[64 * (c >> 2)] >> (8u * (c & 3u))) & 0xFFu) * (1.0f / 255.0f);
+  };
   const v3 en = V3(EP.normal[0], EP.normal[1], EP.normal[2]);
   v3 acc = V3(0.0f, 0.0f, 0.0f);
-  // the cut walk's stack (48 entries, as the reference's) in LDS, one column per lane, node indices as u16
-  // (the host caps the tree at 65535 nodes; a child index beyond that becomes 0xFFFF, which the walk skips)
-  __shared__ uint16_t s_stack[48][64];
+  // the cut walk's stack in LDS, one column per lane, node indices as u16 (the host caps the tree at 65535 nodes;
+  // a child index beyond that becomes 0xFFFF, which the walk skips)
+  __shared__ uint16_t s_stack[STACK][64];
   uint16_t *stack = &s_stack[0][threadIdx.x];
   stack[0] = (uint16_t)min(EP.root, 0xFFFFu);
   uint32_t sp = 1;
@@ -323,7 +349,7 @@ sl_shade_emit(FrameUniformsDev U, EmitParamsDev EP, const uint32_t *__restrict__
       use = dot3(d, d) > EP.theta * EP.theta * e0.w * e0.w;
     }
     if (!use) {
-      if (sp + 2u <= 48u) {
+      if (sp + 2u <= STACK) {
         stack[64 * sp] = (uint16_t)min(clo, 0xFFFFu);
         stack[64 * (sp + 1)] = (uint16_t)min(__float_as_uint(e3.w), 0xFFFFu);
         sp += 2;
@@ -336,7 +362,7 @@ sl_shade_emit(FrameUniformsDev U, EmitParamsDev EP, const uint32_t *__restrict__
     const float sd = sqrtf(d2), isd = __builtin_amdgcn_rcpf(sd);
     const v3 ww = V3(d.x * isd, d.y * isd, d.z * isd);
     const v3 wi = xf_dir_inv(xf, ww);
-    const float vi = vgrid_lookup(vg, wi);
+    const float vi = vgrid_lookup_f(vcell, wi);
     if (vi <= 1e-7f) continue;
     const float fp = footprint(nd, detC, wi);
     // intensity(e, -ww) (emissive.wgsl:123-130)
@@ -637,11 +663,14 @@ __device__ __forceinline__ void map_tile_sums(const MapRec *__restrict__ recs, c
 // tile's records; a tile whose run lies inside one chunk is stored, a tile split across chunks (a crowded tile is
 // shared by several workgroups) is merged with integer atomic adds (order-independent). acc of the dirty layers is
 // zero on entry; tiles with no records are not touched. stats[0] = pairs, stats[1] += tiles with records (zero on
-// entry). LDS is under 20 KiB per workgroup, so 8 workgroups (32 waves) fit a CU's 160 KiB.
-extern "C" __global__ void __launch_bounds__(256, 8)
+// entry). LDS is under 20 KiB per workgroup, so 8 workgroups (32 waves) fit a CU's 160 KiB. KIND: 0 the emitter
+// map (fixed point x65536, light_shadow.wgsl), 1 the sun maps (x4096, sun_map.wgsl), one instance each.
+template <uint32_t KIND>
+__global__ void __launch_bounds__(256, 8)
 sl_map_accumulate(const MapRec *__restrict__ recs, const uint32_t *__restrict__ keys,
                   const uint32_t *__restrict__ list, const FrameCounters *counters, uint32_t res, uint32_t tiles_x,
-                  uint32_t tpl, float fix, uint32_t *__restrict__ acc, uint32_t *__restrict__ stats) {
+                  uint32_t tpl, uint32_t *__restrict__ acc, uint32_t *__restrict__ stats) {
+  constexpr float fix = KIND == 0 ? 65536.0f : 4096.0f;
   __shared__ MapRec s_rec[MAP_STAGE];
   __shared__ uint32_t s_keys[MAP_CHUNK];
   __shared__ uint32_t s_acc[3 * 256];

@@ -1130,6 +1130,28 @@ bool levels_fit(uint32_t res, uint32_t levels, const uint32_t *level_off, uint32
   return true;
 }
 
+// The largest stack sl_shade_emit's cut walk can hold for this emitter tree: the walk that expands every internal
+// node (any cut holds a subset of its pending entries when it reaches a node), with the kernel's 48-entry cap.
+// A tree that does not terminate within 4n + 8 pops (a cycle) gets the full 48.
+uint32_t emit_stack_need(const uint8_t *nodes, uint32_t n, uint32_t root) {
+  std::vector<uint32_t> st(1, std::min(root, 0xFFFFu));
+  size_t most = 1, pops = 0;
+  while (!st.empty()) {
+    if (++pops > 4 * (size_t)n + 8) return 48u;
+    const uint32_t ni = st.back();
+    st.pop_back();
+    if (ni >= n) continue;
+    const uint32_t clo = rd_u32(nodes + 96 * (size_t)ni, 11), chi = rd_u32(nodes + 96 * (size_t)ni, 15);
+    if (clo == NO_CHILD) continue;                       // a leaf
+    if (st.size() + 2 <= 48) {
+      st.push_back(std::min(clo, 0xFFFFu));
+      st.push_back(std::min(chi, 0xFFFFu));
+      most = std::max(most, st.size());
+    }
+  }
+  return (uint32_t)most;
+}
+
 sl_status grow_ctx(sl_context *ctx, Grow *gb, size_t bytes) {
   if (gb->p && gb->bytes >= bytes) return SL_OK;
   if (gb->p) {
@@ -1177,9 +1199,9 @@ sl_status ensure_map(sl_context *ctx, uint32_t which, size_t n_rec, uint32_t spa
 
 // The records and tile boxes written by a prep pass -> (tile, record) pairs (sl_pairs_emit over the layers'
 // stacked tile grid) -> sorted by tile (onesweep) -> per-tile sums into acc, whose dirty layers are zero, on stream
-// st. which: 0 emitter map, 1 sun maps (buffers, stats slot and timer events).
+// st. which: 0 emitter map, 1 sun maps (buffers, stats slot, timer events and the sum kernel's fixed point).
 void run_map_splat(sl_context *ctx, hipStream_t st, uint32_t which, size_t n_rec, uint32_t n_layers, uint32_t res,
-                   uint32_t span, float fix, uint32_t *acc, bool timers) {
+                   uint32_t span, uint32_t *acc, bool timers) {
   sl_context::MapBufs &m = ctx->map[which];
   const uint32_t tiles_x = (res + MT - 1) / MT, tpl = tiles_x * tiles_x, n_tiles = n_layers * tpl;
   const uint32_t tile_passes = n_tiles > 65536u ? 3u : 2u;
@@ -1207,8 +1229,13 @@ void run_map_splat(sl_context *ctx, hipStream_t st, uint32_t which, size_t n_rec
     std::swap(vin, vout);
   }
   if (timers) (void)hipEventRecord(ctx->mev[2 * which], st);
-  sl_map_accumulate<<<(uint32_t)((cap + MAP_CHUNK - 1) / MAP_CHUNK), 256, 0, st>>>(
-      (const MapRec *)m.rec.p, kin, vin, counters, res, tiles_x, tpl, fix, acc, stats);
+  const uint32_t chunks = (uint32_t)((cap + MAP_CHUNK - 1) / MAP_CHUNK);
+  if (which == 0)
+    sl_map_accumulate<0><<<chunks, 256, 0, st>>>((const MapRec *)m.rec.p, kin, vin, counters, res, tiles_x, tpl, acc,
+                                                 stats);
+  else
+    sl_map_accumulate<1><<<chunks, 256, 0, st>>>((const MapRec *)m.rec.p, kin, vin, counters, res, tiles_x, tpl, acc,
+                                                 stats);
   if (timers) (void)hipEventRecord(ctx->mev[2 * which + 1], st);
   ctx->map_timed[which] = timers;
   ctx->map_records[which] = (uint32_t)n_rec;
@@ -1390,7 +1417,7 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
                                             (const SunLayer *)sc->sun_layers.p, sc->g[0], sc->g[1], sc->g[2],
                                             sc->storage, sc->nodes, sc->parts, max_layers,
                                             (MapRec *)ctx->map[1].rec.p, (uint2 *)ctx->map[1].tbox.p);
-      run_map_splat(ctx, ss, 1, sun_rec, SM.n_layers, SM.res, sun_span, 4096.0f, (uint32_t *)sc->sun_acc.p, timers);
+      run_map_splat(ctx, ss, 1, sun_rec, SM.n_layers, SM.res, sun_span, (uint32_t *)sc->sun_acc.p, timers);
       run_opacity_maps(ss, (float4 *)sc->sun_mip.p, SM.n_layers, SM.res, SM.levels, SM.level_off, SM.layer_texels,
                        4096.0f, (const uint32_t *)sc->sun_acc.p, (const uint32_t *)sc->sun_dirty.p);
     }
@@ -1417,7 +1444,7 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
       if (sc->count)
         sl_lsm_prep<<<gblocks, 64, 0, st>>>(sc->count, LS, sc->g[0], sc->g[1], sc->g[2], sc->storage, sc->nodes,
                                             sc->parts, (MapRec *)ctx->map[0].rec.p, (uint2 *)ctx->map[0].tbox.p);
-      run_map_splat(ctx, st, 0, sc->count, LS.part_count, LS.res, ls_span, 65536.0f, (uint32_t *)sc->lsm_acc.p, timers);
+      run_map_splat(ctx, st, 0, sc->count, LS.part_count, LS.res, ls_span, (uint32_t *)sc->lsm_acc.p, timers);
       run_opacity_maps(st, (float4 *)sc->lsm_mip.p, LS.part_count, LS.res, LS.levels, LS.level_off, LS.layer_texels,
                        65536.0f, (const uint32_t *)sc->lsm_acc.p, (const uint32_t *)sc->lsm_dirty.p);
     }
@@ -1426,9 +1453,14 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
     lampT = sc->lamp_T;
   }
   if (timers) SL_HIP(ctx, hipEventRecord(ctx->lev[1], st));
-  sl_shade_emit<<<gblocks, 64, 0, st>>>(U, EP, sc->order, (const float4 *)sc->emit_nodes.p, d->emit_node_count, sc->g[0],
-                                        sc->g[1], sc->g[2], sc->g[3], sc->storage, sc->nodes, sc->vgrid, sc->parts,
-                                        lampT, sc->materials, sc->iso_norm, sc->colour);
+  if (emit_stack_need((const uint8_t *)d->emit_nodes, d->emit_node_count, EP.root) <= 16u)
+    sl_shade_emit<16><<<gblocks, 64, 0, st>>>(U, EP, sc->order, (const float4 *)sc->emit_nodes.p, d->emit_node_count,
+                                              sc->g[0], sc->g[1], sc->g[2], sc->g[3], sc->storage, sc->nodes,
+                                              sc->vgrid, sc->parts, lampT, sc->materials, sc->iso_norm, sc->colour);
+  else
+    sl_shade_emit<48><<<gblocks, 64, 0, st>>>(U, EP, sc->order, (const float4 *)sc->emit_nodes.p, d->emit_node_count,
+                                              sc->g[0], sc->g[1], sc->g[2], sc->g[3], sc->storage, sc->nodes,
+                                              sc->vgrid, sc->parts, lampT, sc->materials, sc->iso_norm, sc->colour);
   if (timers) SL_HIP(ctx, hipEventRecord(ctx->lev[2], st));
   // distant lights: env_T (the side stream's, or 1 without sun maps), then env_direct
   if (d->env) {
