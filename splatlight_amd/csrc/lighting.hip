@@ -251,24 +251,35 @@ using namespace lt;
 extern "C" __global__ void __launch_bounds__(64)
 sl_iso_init(uint32_t n, const float4 *__restrict__ g2, uint32_t st, const float4 *__restrict__ nodes,
             const uint8_t *__restrict__ vgrid, float *__restrict__ iso_norm) {
+  // the 128 cells' directions and solid angles, the same for every Gaussian: computed once per workgroup into LDS
+  __shared__ float4 s_cell[128];
+  for (uint32_t c = threadIdx.x; c < 128u; c += 64u) {
+    const uint32_t i = c / 16u, j = c % 16u;
+    const float el = 0.5f * PI_F - ((float)i + 0.5f) * PI_F / 8.0f;
+    const float az = -PI_F + ((float)j + 0.5f) * 2.0f * PI_F / 16.0f;
+    const float top = 0.5f * PI_F - (float)i * PI_F / 8.0f;
+    const float om = (2.0f * PI_F / 16.0f) * (sinf(top) - sinf(top - PI_F / 8.0f));
+    s_cell[c] = make_float4(cosf(el) * cosf(az), cosf(el) * sinf(az), sinf(el), om);
+  }
+  __syncthreads();
   const uint32_t g = blockIdx.x * 64u + threadIdx.x;
   if (g >= n) return;
   const float4 s = sl_scale(g2, st, g);
   const NodeL nd = load_node(nodes, g);
   const float s3 = s.x * s.y * s.z;
-  const uint8_t *vg = vgrid + 128 * (size_t)g;
+  const uint4 *vg = (const uint4 *)(vgrid + 128 * (size_t)g);   // the grid's cache line, read 16 B at a time
   float S = 0.0f, S0 = 0.0f;
-  for (uint32_t c = 0; c < 128u; c++) {
-    const uint32_t i = c / 16u, j = c % 16u;
-    const float el = 0.5f * PI_F - ((float)i + 0.5f) * PI_F / 8.0f;
-    const float az = -PI_F + ((float)j + 0.5f) * 2.0f * PI_F / 16.0f;
-    const v3 w = V3(cosf(el) * cosf(az), cosf(el) * sinf(az), sinf(el));
-    const float top = 0.5f * PI_F - (float)i * PI_F / 8.0f;
-    const float om = (2.0f * PI_F / 16.0f) * (sinf(top) - sinf(top - PI_F / 8.0f));
-    const float v = (float)vg[c] * (1.0f / 255.0f);
-    const float a = footprint(nd, s3 * s3, w) * om;
-    S += a * v;
-    S0 += a;
+  for (uint32_t q = 0; q < 8u; q++) {
+    const uint4 v4 = vg[q];
+    const uint32_t words[4] = {v4.x, v4.y, v4.z, v4.w};
+#pragma unroll
+    for (uint32_t k = 0; k < 16u; k++) {
+      const float4 cw = s_cell[16u * q + k];
+      const float v = (float)((words[k >> 2] >> (8u * (k & 3u))) & 0xFFu) * (1.0f / 255.0f);
+      const float a = footprint(nd, s3 * s3, V3(cw.x, cw.y, cw.z)) * cw.w;
+      S += a * v;
+      S0 += a;
+    }
   }
   const float m = S > 0.02f * S0 ? S : 0.02f * S0;
   iso_norm[g] = 1.0f / (m > 1e-30f ? m : 1e-30f);
