@@ -1197,49 +1197,73 @@ sl_status ensure_map(sl_context *ctx, uint32_t which, size_t n_rec, uint32_t spa
   return grow_ctx(ctx, &m.scratch, 4 * ms.words);
 }
 
-// The records and tile boxes written by a prep pass -> (tile, record) pairs (sl_pairs_emit over the layers'
-// stacked tile grid) -> sorted by tile (onesweep) -> per-tile sums into acc, whose dirty layers are zero, on stream
-// st. which: 0 emitter map, 1 sun maps (buffers, stats slot, timer events and the sum kernel's fixed point).
-void run_map_splat(sl_context *ctx, hipStream_t st, uint32_t which, size_t n_rec, uint32_t n_layers, uint32_t res,
-                   uint32_t span, uint32_t *acc, bool timers) {
+// Map splats in two steps, each on stream st. map_bin: the records and tile boxes written by a prep pass ->
+// (tile, record) pairs (sl_pairs_emit over the layers' stacked tile grid) -> sorted by tile (onesweep). map_sum:
+// per-tile sums into acc, whose dirty layers are zero. which: 0 emitter map, 1 sun maps (buffers, stats slot,
+// timer events and the sum kernel's fixed point).
+struct MapShape {
+  uint32_t tiles_x, tpl, n_tiles, tile_passes;
+  size_t cap;
+  MapShape(size_t n_rec, uint32_t n_layers, uint32_t res, uint32_t span) {
+    tiles_x = (res + MT - 1) / MT;
+    tpl = tiles_x * tiles_x;
+    n_tiles = n_layers * tpl;
+    tile_passes = n_tiles > 65536u ? 3u : 2u;
+    cap = std::max<size_t>(n_rec * span * span, 1);
+  }
+};
+
+void map_bin(sl_context *ctx, hipStream_t st, uint32_t which, size_t n_rec, uint32_t n_layers, uint32_t res,
+             uint32_t span) {
   sl_context::MapBufs &m = ctx->map[which];
-  const uint32_t tiles_x = (res + MT - 1) / MT, tpl = tiles_x * tiles_x, n_tiles = n_layers * tpl;
-  const uint32_t tile_passes = n_tiles > 65536u ? 3u : 2u;
-  const size_t cap = std::max<size_t>(n_rec * span * span, 1);
-  const MapScratch ms(n_rec, cap);
+  const MapShape sh(n_rec, n_layers, res, span);
+  const MapScratch ms(n_rec, sh.cap);
   uint32_t *w = (uint32_t *)m.scratch.p;
   FrameCounters *counters = (FrameCounters *)(w + ms.counters);
   uint32_t *stats = (uint32_t *)ctx->map_stats.p + 2 * which;
   sl_fill_u32<<<grid_for(ms.words, 256, 2048), 256, 0, st>>>(w, 0u, (uint32_t)ms.words);
   sl_fill_u32<<<1, 256, 0, st>>>(&counters->visible, (uint32_t)n_rec, 1u);   // the records are the "ranks"
-  sl_fill_u32<<<1, 256, 0, st>>>(w + ms.tiles_x, tiles_x, 1u);
+  sl_fill_u32<<<1, 256, 0, st>>>(w + ms.tiles_x, sh.tiles_x, 1u);
   sl_fill_u32<<<1, 256, 0, st>>>(stats + 1, 0u, 1u);
   uint32_t *k0 = (uint32_t *)m.k0.p, *v0 = (uint32_t *)m.v0.p;
   uint32_t *k1 = (uint32_t *)m.k1.p, *v1 = (uint32_t *)m.v1.p;
   sl_pairs_emit<<<grid_for(n_rec, PE_CHUNK, 2u * ctx->cus), PE_THREADS, 0, st>>>(
-      nullptr, (const uint2 *)m.tbox.p, w + ms.zeros, w + ms.zeros, w + ms.tiles_x, 1u, tile_passes,
-      (uint32_t)cap, counters, k0, v0, (uint64_t *)(w + ms.chunk), w + ms.tickets, w + ms.hist);
-  sl_radix_bases<<<tile_passes, RS_THREADS, 0, st>>>(w + ms.hist);
-  const uint32_t grid = std::max(std::min<uint32_t>((uint32_t)((cap + 4095) / 4096), 4u * ctx->cus), 1u);
+      nullptr, (const uint2 *)m.tbox.p, w + ms.zeros, w + ms.zeros, w + ms.tiles_x, 1u, sh.tile_passes,
+      (uint32_t)sh.cap, counters, k0, v0, (uint64_t *)(w + ms.chunk), w + ms.tickets, w + ms.hist);
+  sl_radix_bases<<<sh.tile_passes, RS_THREADS, 0, st>>>(w + ms.hist);
+  const uint32_t grid = std::max(std::min<uint32_t>((uint32_t)((sh.cap + 4095) / 4096), 4u * ctx->cus), 1u);
   uint32_t *kin = k0, *vin = v0, *kout = k1, *vout = v1;
-  for (uint32_t p = 0; p < tile_passes; p++) {
+  for (uint32_t p = 0; p < sh.tile_passes; p++) {
     sl_radix_pass<16><<<grid, RS_THREADS, 0, st>>>(kin, vin, kout, vout, &counters->pairs, 0u, 8u * p,
                                                    w + ms.hist + 256 * p, w + ms.tstat[p], w + ms.tickets + 1 + p);
     std::swap(kin, kout);
     std::swap(vin, vout);
   }
+}
+
+void map_sum(sl_context *ctx, hipStream_t st, uint32_t which, size_t n_rec, uint32_t n_layers, uint32_t res,
+             uint32_t span, uint32_t *acc, bool timers) {
+  sl_context::MapBufs &m = ctx->map[which];
+  const MapShape sh(n_rec, n_layers, res, span);
+  const MapScratch ms(n_rec, sh.cap);
+  uint32_t *w = (uint32_t *)m.scratch.p;
+  const FrameCounters *counters = (const FrameCounters *)(w + ms.counters);
+  uint32_t *stats = (uint32_t *)ctx->map_stats.p + 2 * which;
+  // the sorted pairs are in (k0, v0) after an even number of passes, else in (k1, v1)
+  const uint32_t *keys = (const uint32_t *)(sh.tile_passes % 2u == 0u ? m.k0.p : m.k1.p);
+  const uint32_t *vals = (const uint32_t *)(sh.tile_passes % 2u == 0u ? m.v0.p : m.v1.p);
   if (timers) (void)hipEventRecord(ctx->mev[2 * which], st);
-  const uint32_t chunks = (uint32_t)((cap + MAP_CHUNK - 1) / MAP_CHUNK);
+  const uint32_t chunks = (uint32_t)((sh.cap + MAP_CHUNK - 1) / MAP_CHUNK);
   if (which == 0)
-    sl_map_accumulate<0><<<chunks, 256, 0, st>>>((const MapRec *)m.rec.p, kin, vin, counters, res, tiles_x, tpl, acc,
-                                                 stats);
+    sl_map_accumulate<0><<<chunks, 256, 0, st>>>((const MapRec *)m.rec.p, keys, vals, counters, res, sh.tiles_x,
+                                                 sh.tpl, acc, stats);
   else
-    sl_map_accumulate<1><<<chunks, 256, 0, st>>>((const MapRec *)m.rec.p, kin, vin, counters, res, tiles_x, tpl, acc,
-                                                 stats);
+    sl_map_accumulate<1><<<chunks, 256, 0, st>>>((const MapRec *)m.rec.p, keys, vals, counters, res, sh.tiles_x,
+                                                 sh.tpl, acc, stats);
   if (timers) (void)hipEventRecord(ctx->mev[2 * which + 1], st);
   ctx->map_timed[which] = timers;
   ctx->map_records[which] = (uint32_t)n_rec;
-  ctx->map_tiles[which] = n_tiles;
+  ctx->map_tiles[which] = sh.n_tiles;
 }
 
 // tiles a box of at most 2 * max_half + 1 texels can span per axis
@@ -1393,7 +1417,9 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
   SL_HIP(ctx, hipMemcpyAsync(sc->emit_nodes.p, d->emit_nodes, (size_t)d->emit_node_count * 96, hipMemcpyHostToDevice, st));
   if (fresh_stats) sl_fill_u32<<<1, 256, 0, st>>>((uint32_t *)ctx->map_stats.p, 0u, 4u);
   if (timers) SL_HIP(ctx, hipEventRecord(ctx->lev[0], st));
-  // side stream: sun maps -> env_T (after everything already on the context's stream)
+  // side stream: sun maps -> env_T, after everything already on the context's stream. Enqueued before the emitter
+  // map, whose short kernels then fill in around the sun maps' binning and sums (measured faster than ordering the
+  // two chains' sums or the shading against each other with events).
   if (have_sun) {
     hipStream_t ss = ctx->side;
     SL_HIP(ctx, hipEventRecord(ctx->fork, st));
@@ -1417,7 +1443,8 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
                                             (const SunLayer *)sc->sun_layers.p, sc->g[0], sc->g[1], sc->g[2],
                                             sc->storage, sc->nodes, sc->parts, max_layers,
                                             (MapRec *)ctx->map[1].rec.p, (uint2 *)ctx->map[1].tbox.p);
-      run_map_splat(ctx, ss, 1, sun_rec, SM.n_layers, SM.res, sun_span, (uint32_t *)sc->sun_acc.p, timers);
+      map_bin(ctx, ss, 1, sun_rec, SM.n_layers, SM.res, sun_span);
+      map_sum(ctx, ss, 1, sun_rec, SM.n_layers, SM.res, sun_span, (uint32_t *)sc->sun_acc.p, timers);
       run_opacity_maps(ss, (float4 *)sc->sun_mip.p, SM.n_layers, SM.res, SM.levels, SM.level_off, SM.layer_texels,
                        4096.0f, (const uint32_t *)sc->sun_acc.p, (const uint32_t *)sc->sun_dirty.p);
     }
@@ -1444,7 +1471,8 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
       if (sc->count)
         sl_lsm_prep<<<gblocks, 64, 0, st>>>(sc->count, LS, sc->g[0], sc->g[1], sc->g[2], sc->storage, sc->nodes,
                                             sc->parts, (MapRec *)ctx->map[0].rec.p, (uint2 *)ctx->map[0].tbox.p);
-      run_map_splat(ctx, st, 0, sc->count, LS.part_count, LS.res, ls_span, (uint32_t *)sc->lsm_acc.p, timers);
+      map_bin(ctx, st, 0, sc->count, LS.part_count, LS.res, ls_span);
+      map_sum(ctx, st, 0, sc->count, LS.part_count, LS.res, ls_span, (uint32_t *)sc->lsm_acc.p, timers);
       run_opacity_maps(st, (float4 *)sc->lsm_mip.p, LS.part_count, LS.res, LS.levels, LS.level_off, LS.layer_texels,
                        65536.0f, (const uint32_t *)sc->lsm_acc.p, (const uint32_t *)sc->lsm_dirty.p);
     }
