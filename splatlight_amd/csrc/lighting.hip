@@ -849,8 +849,9 @@ sl_lsm_prep(uint32_t n, LSParamsDev LS, const float4 *__restrict__ g0, const flo
   const uint32_t g = blockIdx.x * 64u + threadIdx.x;
   if (g >= n) return;
   const MapRec r = lsm_rec(g, LS, g0, g1, g2, st, nodes, parts);
-  recs[g] = r;
-  tbox[g] = map_tbox(r, (LS.res + MT - 1) / MT);
+  const uint2 b = map_tbox(r, (LS.res + MT - 1) / MT);
+  tbox[g] = b;
+  if (b.x != 0xffffffffu) recs[g] = r;   // an empty record is in no tile's list and never read
 }
 
 extern "C" __global__ void __launch_bounds__(64)
@@ -943,10 +944,7 @@ sl_sun_prep(uint32_t n, SunParamsDev SM, const SunLayerDesc *__restrict__ desc, 
   const float4 A = g0[g], Q = sl_quat(g1, st, g), S4 = sl_scale(g2, st, g);
   const uint32_t p = __float_as_uint(S4.w) & 0xFFFFu;
   const uint32_t first = p < 32u ? SM.part_layers[2 * p] : 0u, cnt = p < 32u ? min(SM.part_layers[2 * p + 1], max_layers) : 0u;
-  for (uint32_t j = cnt; j < max_layers; j++) {
-    out[j] = empty_rec();
-    obox[j] = make_uint2(0xffffffffu, 0u);
-  }
+  for (uint32_t j = cnt; j < max_layers; j++) obox[j] = make_uint2(0xffffffffu, 0u);   // empty: in no tile's list
   if (cnt == 0u) return;
   const Xf xf = load_xf(parts, p);
   const m3 R = part_rot(xf);
@@ -959,7 +957,6 @@ sl_sun_prep(uint32_t n, SunParamsDev SM, const SunLayerDesc *__restrict__ desc, 
     const uint32_t l = first + j;
     const SunLayerDesc d = desc[l];
     if (d.dirty == 0u || SM.lights[d.light][3] < 0.0f) {
-      out[j] = empty_rec();
       obox[j] = make_uint2(0xffffffffu, 0u);
       continue;
     }
@@ -988,7 +985,6 @@ sl_sun_prep(uint32_t n, SunParamsDev SM, const SunLayerDesc *__restrict__ desc, 
     const int hx = min((int)ceilf(3.0f * sqrtf(S.c0.x)), 32), hy = min((int)ceilf(3.0f * sqrtf(S.c1.y)), 32);
     const int cx = (int)floorf(ct.x), cy = (int)floorf(ct.y);
     if (tau0 * 4096.0f < 0.5f) {   // no texel reaches half a fixed-point unit
-      out[j] = empty_rec();
       obox[j] = make_uint2(0xffffffffu, 0u);
       continue;
     }
@@ -999,8 +995,9 @@ sl_sun_prep(uint32_t n, SunParamsDev SM, const SunLayerDesc *__restrict__ desc, 
     o.layer = l;
     o.x0 = max(cx - hx, 0); o.x1 = min(cx + hx, res - 1);
     o.y0 = max(cy - hy, 0); o.y1 = min(cy + hy, res - 1);
-    out[j] = o;
-    obox[j] = map_tbox(o, tiles_y);
+    const uint2 b = map_tbox(o, tiles_y);
+    obox[j] = b;
+    if (b.x != 0xffffffffu) out[j] = o;   // an empty record is in no tile's list and never read
   }
 }
 
