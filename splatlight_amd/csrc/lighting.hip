@@ -48,10 +48,10 @@ __device__ __forceinline__ uint32_t u32_sat(float f) {
 }
 // e^x for the opacity-map splats: Cody-Waite reduction by ln 2 (split in two, fused) and the degree-7 Taylor
 // polynomial on [-ln2/2, ln2/2] in Horner form with explicit fused multiply-adds, so every IEEE operation is
-// the same on any host or device that has fma; within 2 ulp of e^x. x is clamped to [-86, 88] (normal results);
-// the splats never pass NaN (a NaN footprint value is skipped before).
-__device__ __forceinline__ float exp_map(float x) {
-  x = fminf(fmaxf(x, -86.0f), 88.0f);
+// the same on any host or device that has fma; within 2 ulp of e^x. exp_map clamps x to [-86, 88] (normal
+// results); exp_map_in takes x already in that range: the map sums call it with -q/2, where 0 <= q <= 9 up to
+// rounding (a positive definite form after the 0.3 texel^2 regulariser; NaN and q > 9 are skipped before).
+__device__ __forceinline__ float exp_map_in(float x) {
   const float k = rintf(x * 1.44269504f);
   float r = fmaf(k, -0.693145751953125f, x);
   r = fmaf(k, -1.42860677e-6f, r);
@@ -65,6 +65,7 @@ __device__ __forceinline__ float exp_map(float x) {
   p = fmaf(p, r, 1.0f);
   return ldexpf(p, (int)k);
 }
+__device__ __forceinline__ float exp_map(float x) { return exp_map_in(fminf(fmaxf(x, -86.0f), 88.0f)); }
 __device__ __forceinline__ v3 m3v(m3 m, v3 v) { return add3(add3(mul3s(m.c0, v.x), mul3s(m.c1, v.y)), mul3s(m.c2, v.z)); }
 __device__ __forceinline__ m3 m3mul(m3 a, m3 b) { return {m3v(a, b.c0), m3v(a, b.c1), m3v(a, b.c2)}; }
 __device__ __forceinline__ m3 m3t(m3 m) {
@@ -575,7 +576,7 @@ __device__ __forceinline__ void map_texel(const MapRec &r, float dx, float py, f
   const m2 Si = {V2(r.s00, r.s01), V2(r.s10, r.s11)};
   const float q = dot2(dt, m2v(Si, dt));
   if (!(q <= 9.0f)) return;                         // outside 3 sigma, or a NaN footprint: adds nothing
-  float t = r.tau0 * exp_map(-0.5f * q);
+  float t = r.tau0 * exp_map_in(-0.5f * q);
   t = t < 64.0f ? t : 64.0f;
   const float t_fix = t * fix;
   if (t_fix < 0.5f) return;
@@ -586,41 +587,65 @@ __device__ __forceinline__ void map_texel(const MapRec &r, float dx, float py, f
   a2 += u32_sat(t_fix * (z * z + r.sz2) + 0.5f);
 }
 
-// Which of the 4 bands (16x4 texels, rows ty0 + 4k ..) of the 16x16 tile at (tx0, ty0) may hold a texel where
-// record r adds something (q <= 9): bit k set unless the minimum of the record's quadratic form over the band's
-// texel-centre rectangle (clipped to the record's box) exceeds 9 by a margin. The rectangle's corners are the same
-// float differences px - ctx the sums compute, and rounding is monotonic, so every texel's offsets lie inside it;
-// the margin (1e-4 of the largest term magnitude on the rectangle, plus 1e-3) is some 100x the float rounding of
-// both the per-texel q and this minimum. Every band is kept when the form is not positive definite or not finite,
-// so a band with a contributing texel is never dropped.
+// Whether record r may add something (q <= 9) at a texel of the texel-centre rectangle [xa, xb] x [ya, yb]
+// (inside its box): false only when the minimum of its quadratic form over the rectangle exceeds 9 by a margin.
+// The rectangle's corners are the same float differences px - ctx the sums compute, and rounding is monotonic, so
+// every texel's offsets lie inside it; the margin (1e-4 of the largest term magnitude on the rectangle, plus 1e-3)
+// is some 100x the float rounding of both the per-texel q and this minimum. true whenever the form is not positive
+// definite or not finite, so a rectangle with a contributing texel is never ruled out.
+struct MapForm {
+  float a, b, c;
+  bool pd;
+  __device__ __forceinline__ explicit MapForm(const MapRec &r)
+      : a(r.s00), b(0.5f * (r.s01 + r.s10)), c(r.s11) {
+    pd = a > 0.0f && c > 0.0f && a * c - b * b > 0.0f && a < 1e30f && c < 1e30f;
+  }
+};
+__device__ __forceinline__ bool map_rect_may_hit(const MapRec &r, const MapForm &f, int xa, int xb, int ya, int yb) {
+  if (!f.pd) return true;
+  const float a = f.a, b = f.b, c = f.c;
+  const float u0 = ((float)xa + 0.5f) - r.ctx, u1 = ((float)xb + 0.5f) - r.ctx;
+  const float v0 = ((float)ya + 0.5f) - r.cty, v1 = ((float)yb + 0.5f) - r.cty;
+  auto q = [&](float u, float v) { return (a * u * u + 2.0f * b * u * v) + c * v * v; };
+  float m;
+  if (u0 <= 0.0f && u1 >= 0.0f && v0 <= 0.0f && v1 >= 0.0f) {
+    m = 0.0f;                                            // the centre lies inside
+  } else {
+    // the minimum lies on an edge: on u = const the best v is -b u / c, on v = const the best u is -b v / a
+    m = fminf(fminf(q(u0, fminf(fmaxf(-b * u0 / c, v0), v1)), q(u1, fminf(fmaxf(-b * u1 / c, v0), v1))),
+              fminf(q(fminf(fmaxf(-b * v0 / a, u0), u1), v0), q(fminf(fmaxf(-b * v1 / a, u0), u1), v1)));
+  }
+  const float um = fmaxf(fabsf(u0), fabsf(u1)), vm = fmaxf(fabsf(v0), fabsf(v1));
+  const float big = a * um * um + (fabsf(r.s01) + fabsf(r.s10)) * um * vm + c * vm * vm;
+  return m <= 9.0f + 1e-4f * big + 1e-3f;
+}
+
+// Which of the 4 bands (16x4 texels, rows ty0 + 4k ..) of the 16x16 tile at (tx0, ty0) record r may touch: bit k.
 __device__ __forceinline__ uint32_t map_band_mask(const MapRec &r, int tx0, int ty0) {
-  const float a = r.s00, b = 0.5f * (r.s01 + r.s10), c = r.s11;
-  const bool pd = a > 0.0f && c > 0.0f && a * c - b * b > 0.0f && a < 1e30f && c < 1e30f;
+  const MapForm f(r);
   const int xa = max(r.x0, tx0), xb = min(r.x1, tx0 + 15);
-  const float u0 = ((float)xa + 0.5f) - r.ctx, u1 = ((float)xb + 0.5f) - r.ctx, um = fmaxf(fabsf(u0), fabsf(u1));
   uint32_t mask = 0;
 #pragma unroll 1
   for (int k = 0; k < 4; k++) {
     const int ya = max(r.y0, ty0 + 4 * k), yb = min(r.y1, ty0 + 4 * k + 3);
-    if (xa > xb || ya > yb) continue;
-    if (!pd) {
-      mask |= 1u << k;
-      continue;
-    }
-    const float v0 = ((float)ya + 0.5f) - r.cty, v1 = ((float)yb + 0.5f) - r.cty;
-    auto q = [&](float u, float v) { return (a * u * u + 2.0f * b * u * v) + c * v * v; };
-    float m;
-    if (u0 <= 0.0f && u1 >= 0.0f && v0 <= 0.0f && v1 >= 0.0f) {
-      m = 0.0f;                                          // the centre lies inside
-    } else {
-      // the minimum lies on an edge: on u = const the best v is -b u / c, on v = const the best u is -b v / a
-      m = fminf(fminf(q(u0, fminf(fmaxf(-b * u0 / c, v0), v1)), q(u1, fminf(fmaxf(-b * u1 / c, v0), v1))),
-                fminf(q(fminf(fmaxf(-b * v0 / a, u0), u1), v0), q(fminf(fmaxf(-b * v1 / a, u0), u1), v1)));
-    }
-    const float vm = fmaxf(fabsf(v0), fabsf(v1));
-    const float big = a * um * um + (fabsf(r.s01) + fabsf(r.s10)) * um * vm + c * vm * vm;
-    if (m <= 9.0f + 1e-4f * big + 1e-3f) mask |= 1u << k;
+    if (xa <= xb && ya <= yb && map_rect_may_hit(r, f, xa, xb, ya, yb)) mask |= 1u << k;
   }
+  return mask;
+}
+
+// Which tiles of record r's tile box (at most 8x8) it may touch: bit (row * box width + column), the order
+// sl_pairs_emit walks a box in. Empty records: 0.
+__device__ __forceinline__ uint64_t map_tile_mask(const MapRec &r) {
+  if (r.x0 > r.x1 || r.y0 > r.y1) return 0ull;
+  const MapForm f(r);
+  const int tx0 = r.x0 / MT, tx1 = r.x1 / MT, ty0 = r.y0 / MT, ty1 = r.y1 / MT, bw = tx1 - tx0 + 1;
+  uint64_t mask = 0;
+  for (int ty = ty0; ty <= ty1; ty++)
+    for (int tx = tx0; tx <= tx1; tx++) {
+      const int xa = max(r.x0, tx * MT), xb = min(r.x1, tx * MT + MT - 1);
+      const int ya = max(r.y0, ty * MT), yb = min(r.y1, ty * MT + MT - 1);
+      if (map_rect_may_hit(r, f, xa, xb, ya, yb)) mask |= 1ull << ((ty - ty0) * bw + (tx - tx0));
+    }
   return mask;
 }
 
@@ -845,13 +870,17 @@ __device__ MapRec lsm_rec(uint32_t g, const LSParamsDev &LS, const float4 *__res
 extern "C" __global__ void __launch_bounds__(64)
 sl_lsm_prep(uint32_t n, LSParamsDev LS, const float4 *__restrict__ g0, const float4 *__restrict__ g1,
             const float4 *__restrict__ g2, uint32_t st, const float4 *__restrict__ nodes,
-            const float4 *__restrict__ parts, MapRec *__restrict__ recs, uint2 *__restrict__ tbox) {
+            const float4 *__restrict__ parts, MapRec *__restrict__ recs, uint2 *__restrict__ tbox,
+            uint64_t *__restrict__ tmask) {
   const uint32_t g = blockIdx.x * 64u + threadIdx.x;
   if (g >= n) return;
   const MapRec r = lsm_rec(g, LS, g0, g1, g2, st, nodes, parts);
   const uint2 b = map_tbox(r, (LS.res + MT - 1) / MT);
   tbox[g] = b;
-  if (b.x != 0xffffffffu) recs[g] = r;   // an empty record is in no tile's list and never read
+  if (b.x != 0xffffffffu) {   // an empty record is in no tile's list and never read
+    recs[g] = r;
+    tmask[g] = map_tile_mask(r);
+  }
 }
 
 extern "C" __global__ void __launch_bounds__(64)
@@ -936,7 +965,7 @@ extern "C" __global__ void __launch_bounds__(64)
 sl_sun_prep(uint32_t n, SunParamsDev SM, const SunLayerDesc *__restrict__ desc, const SunLayer *__restrict__ layers,
             const float4 *__restrict__ g0, const float4 *__restrict__ g1, const float4 *__restrict__ g2,
             uint32_t st, const float4 *__restrict__ nodes, const float4 *__restrict__ parts, uint32_t max_layers,
-            MapRec *__restrict__ recs, uint2 *__restrict__ tbox) {
+            MapRec *__restrict__ recs, uint2 *__restrict__ tbox, uint64_t *__restrict__ tmask) {
   const uint32_t g = blockIdx.x * 64u + threadIdx.x;
   if (g >= n) return;
   MapRec *out = recs + (size_t)g * max_layers;
@@ -997,7 +1026,10 @@ sl_sun_prep(uint32_t n, SunParamsDev SM, const SunLayerDesc *__restrict__ desc, 
     o.y0 = max(cy - hy, 0); o.y1 = min(cy + hy, res - 1);
     const uint2 b = map_tbox(o, tiles_y);
     obox[j] = b;
-    if (b.x != 0xffffffffu) out[j] = o;   // an empty record is in no tile's list and never read
+    if (b.x != 0xffffffffu) {   // an empty record is in no tile's list and never read
+      out[j] = o;
+      tmask[(size_t)g * max_layers + j] = map_tile_mask(o);
+    }
   }
 }
 

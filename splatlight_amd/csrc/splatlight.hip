@@ -151,7 +151,7 @@ struct sl_context {
   // shared by every scene lit on this context: records, their tile boxes, the (tile, record) pairs and their sorted
   // copies, a cleared scratch block; per-map stats (two words per kind)
   struct MapBufs {
-    Grow rec, tbox, k0, v0, k1, v1, scratch;
+    Grow rec, tbox, tmask, k0, v0, k1, v1, scratch;
   } map[2];
   Grow map_stats;
   uint32_t map_records[2] = {0, 0}, map_tiles[2] = {0, 0};
@@ -399,8 +399,8 @@ sl_status enqueue_batch(sl_context *ctx, const Batch &b, bool timers) {
                d_grid);
   }
   if ((s = mark(3)) != SL_OK) return s;
-  sl_pairs_emit<<<grid_for(n, PE_CHUNK, 2u * ctx->cus), PE_THREADS, 0, st>>>(
-      w.order, w.tbox, w.slot_base, w.tile_base, w.tiles_x, V, b.tile_passes, w.pairs_cap, w.counters, w.pk0, w.pv0,
+  sl_pairs_emit<false><<<grid_for(n, PE_CHUNK, 2u * ctx->cus), PE_THREADS, 0, st>>>(
+      w.order, w.tbox, nullptr, w.slot_base, w.tile_base, w.tiles_x, V, b.tile_passes, w.pairs_cap, w.counters, w.pk0, w.pv0,
       w.chunk_status, w.tickets + 3, w.tile_hist);
   if ((s = mark(4)) != SL_OK) return s;
   const uint32_t p_grid = std::max(std::min<uint32_t>((w.pairs_cap + rs_tile - 1) / rs_tile, tu.grid_mul * ctx->cus), 1u);
@@ -577,7 +577,7 @@ extern "C" void sl_ctx_destroy(sl_context *ctx) {
   if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
   if (ctx->side) (void)hipStreamSynchronize(ctx->side);
   for (sl_context::MapBufs &m : ctx->map)
-    for (Grow *gb : {&m.rec, &m.tbox, &m.k0, &m.v0, &m.k1, &m.v1, &m.scratch})
+    for (Grow *gb : {&m.rec, &m.tbox, &m.tmask, &m.k0, &m.v0, &m.k1, &m.v1, &m.scratch})
       if (gb->p) (void)hipFree(gb->p);
   if (ctx->map_stats.p) (void)hipFree(ctx->map_stats.p);
   for (hipEvent_t e : {ctx->fork, ctx->join, ctx->sev[0], ctx->sev[1]})
@@ -1192,6 +1192,7 @@ sl_status ensure_map(sl_context *ctx, uint32_t which, size_t n_rec, uint32_t spa
   sl_context::MapBufs &m = ctx->map[which];
   if ((s = grow_ctx(ctx, &m.rec, std::max<size_t>(n_rec, 1) * sizeof(MapRec))) != SL_OK) return s;
   if ((s = grow_ctx(ctx, &m.tbox, std::max<size_t>(n_rec, 1) * 8)) != SL_OK) return s;
+  if ((s = grow_ctx(ctx, &m.tmask, std::max<size_t>(n_rec, 1) * 8)) != SL_OK) return s;
   for (Grow *gb : {&m.k0, &m.v0, &m.k1, &m.v1})
     if ((s = grow_ctx(ctx, gb, 4 * cap)) != SL_OK) return s;
   return grow_ctx(ctx, &m.scratch, 4 * ms.words);
@@ -1227,8 +1228,8 @@ void map_bin(sl_context *ctx, hipStream_t st, uint32_t which, size_t n_rec, uint
   sl_fill_u32<<<1, 256, 0, st>>>(stats + 1, 0u, 1u);
   uint32_t *k0 = (uint32_t *)m.k0.p, *v0 = (uint32_t *)m.v0.p;
   uint32_t *k1 = (uint32_t *)m.k1.p, *v1 = (uint32_t *)m.v1.p;
-  sl_pairs_emit<<<grid_for(n_rec, PE_CHUNK, 2u * ctx->cus), PE_THREADS, 0, st>>>(
-      nullptr, (const uint2 *)m.tbox.p, w + ms.zeros, w + ms.zeros, w + ms.tiles_x, 1u, sh.tile_passes,
+  sl_pairs_emit<true><<<grid_for(n_rec, PE_CHUNK, 2u * ctx->cus), PE_THREADS, 0, st>>>(
+      nullptr, (const uint2 *)m.tbox.p, (const uint64_t *)m.tmask.p, w + ms.zeros, w + ms.zeros, w + ms.tiles_x, 1u, sh.tile_passes,
       (uint32_t)sh.cap, counters, k0, v0, (uint64_t *)(w + ms.chunk), w + ms.tickets, w + ms.hist);
   sl_radix_bases<<<sh.tile_passes, RS_THREADS, 0, st>>>(w + ms.hist);
   const uint32_t grid = std::max(std::min<uint32_t>((uint32_t)((sh.cap + 4095) / 4096), 4u * ctx->cus), 1u);
@@ -1442,7 +1443,8 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
         sl_sun_prep<<<gblocks, 64, 0, ss>>>(sc->count, SM, (const SunLayerDesc *)sc->sun_desc.p,
                                             (const SunLayer *)sc->sun_layers.p, sc->g[0], sc->g[1], sc->g[2],
                                             sc->storage, sc->nodes, sc->parts, max_layers,
-                                            (MapRec *)ctx->map[1].rec.p, (uint2 *)ctx->map[1].tbox.p);
+                                            (MapRec *)ctx->map[1].rec.p, (uint2 *)ctx->map[1].tbox.p,
+                                            (uint64_t *)ctx->map[1].tmask.p);
       map_bin(ctx, ss, 1, sun_rec, SM.n_layers, SM.res, sun_span);
       map_sum(ctx, ss, 1, sun_rec, SM.n_layers, SM.res, sun_span, (uint32_t *)sc->sun_acc.p, timers);
       run_opacity_maps(ss, (float4 *)sc->sun_mip.p, SM.n_layers, SM.res, SM.levels, SM.level_off, SM.layer_texels,
@@ -1470,7 +1472,8 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
                                                                        (uint32_t)(texels * 3));
       if (sc->count)
         sl_lsm_prep<<<gblocks, 64, 0, st>>>(sc->count, LS, sc->g[0], sc->g[1], sc->g[2], sc->storage, sc->nodes,
-                                            sc->parts, (MapRec *)ctx->map[0].rec.p, (uint2 *)ctx->map[0].tbox.p);
+                                            sc->parts, (MapRec *)ctx->map[0].rec.p, (uint2 *)ctx->map[0].tbox.p,
+                                            (uint64_t *)ctx->map[0].tmask.p);
       map_bin(ctx, st, 0, sc->count, LS.part_count, LS.res, ls_span);
       map_sum(ctx, st, 0, sc->count, LS.part_count, LS.res, ls_span, (uint32_t *)sc->lsm_acc.p, timers);
       run_opacity_maps(st, (float4 *)sc->lsm_mip.p, LS.part_count, LS.res, LS.levels, LS.level_off, LS.layer_texels,

@@ -19,17 +19,33 @@
 #define PE_FLAG_MASK (3ull << 62)
 #define PE_VALUE_MASK ((1ull << 62) - 1ull)
 
+// The position of the k-th set bit (from 0) of m, which has more than k set bits.
+__device__ __forceinline__ uint32_t nth_set_bit(uint64_t m, uint32_t k) {
+  uint32_t pos = 0, c = (uint32_t)__popcll(m & 0xffffffffull);
+  if (k >= c) { k -= c; m >>= 32; pos = 32; }
+  uint32_t x = (uint32_t)m;
+  c = (uint32_t)__popc(x & 0xffffu); if (k >= c) { k -= c; x >>= 16; pos += 16; }
+  c = (uint32_t)__popc(x & 0xffu); if (k >= c) { k -= c; x >>= 8; pos += 8; }
+  c = (uint32_t)__popc(x & 0xfu); if (k >= c) { k -= c; x >>= 4; pos += 4; }
+  c = (uint32_t)__popc(x & 0x3u); if (k >= c) { k -= c; x >>= 2; pos += 2; }
+  return pos + (k >= (x & 1u) ? 1u : 0u);
+}
+
 // order == nullptr: rank r is slot r. Batches: a rank's slot belongs to the view v with slot_base[v] <= slot < slot_base[v + 1]; its pairs' tile ids
 // are global, tile_base[v] + ty * tiles_x[v] + tx. tile_passes (2 or 3) digit histograms are built.
-extern "C" __global__ void __launch_bounds__(PE_THREADS)
-sl_pairs_emit(const uint32_t *__restrict__ order, const uint2 *__restrict__ tbox, const uint32_t *__restrict__ slot_base,
-              const uint32_t *__restrict__ tile_base, const uint32_t *__restrict__ tiles_x, uint32_t n_views,
-              uint32_t tile_passes, uint32_t capacity, FrameCounters *counters, uint32_t *__restrict__ pair_keys,
-              uint32_t *__restrict__ pair_vals, uint64_t *chunk_status, uint32_t *ticket,
-              uint32_t *__restrict__ tile_hist) {
+// MASKED (the opacity-map splats, boxes of at most 64 tiles): only the box's tiles whose bit (row * box width +
+// column) is set in tmask[slot] get a pair; otherwise every tile of the box does.
+template <bool MASKED>
+__global__ void __launch_bounds__(PE_THREADS)
+sl_pairs_emit(const uint32_t *__restrict__ order, const uint2 *__restrict__ tbox, const uint64_t *__restrict__ tmask,
+              const uint32_t *__restrict__ slot_base, const uint32_t *__restrict__ tile_base,
+              const uint32_t *__restrict__ tiles_x, uint32_t n_views, uint32_t tile_passes, uint32_t capacity,
+              FrameCounters *counters, uint32_t *__restrict__ pair_keys, uint32_t *__restrict__ pair_vals,
+              uint64_t *chunk_status, uint32_t *ticket, uint32_t *__restrict__ tile_hist) {
   __shared__ uint32_t s_start[PE_CHUNK + 1];
   __shared__ uint2 s_box[PE_CHUNK];
   __shared__ uint32_t s_g[PE_CHUNK];
+  __shared__ uint64_t s_mask[MASKED ? PE_CHUNK : 1];
   __shared__ uint32_t s_hist[3 * 256];
   __shared__ uint32_t s_wsum[PE_THREADS / SL_WAVE];
   __shared__ uint32_t s_chunk;
@@ -59,8 +75,15 @@ sl_pairs_emit(const uint32_t *__restrict__ order, const uint2 *__restrict__ tbox
       if (r < V) {
         g = order ? order[r] : r;
         b = tbox[g];
-        if (b.x != 0xffffffffu)
-          n = ((b.y & 0xffffu) - (b.x & 0xffffu) + 1u) * ((b.y >> 16) - (b.x >> 16) + 1u);
+        if (b.x != 0xffffffffu) {
+          if (MASKED) {
+            const uint64_t m = tmask[g];
+            s_mask[lr] = m;
+            n = (uint32_t)__popcll(m);
+          } else {
+            n = ((b.y & 0xffffu) - (b.x & 0xffffu) + 1u) * ((b.y >> 16) - (b.x >> 16) + 1u);
+          }
+        }
       }
       s_box[lr] = b;
       s_g[lr] = g;
@@ -123,7 +146,7 @@ sl_pairs_emit(const uint32_t *__restrict__ order, const uint2 *__restrict__ tbox
         if (s_start[mid] <= j) lo = mid; else hi = mid;
       }
       const uint2 b = s_box[lo];
-      const uint32_t local = j - s_start[lo];
+      const uint32_t local = MASKED ? nth_set_bit(s_mask[lo], j - s_start[lo]) : j - s_start[lo];
       const uint32_t tx0 = b.x & 0xffffu, ty0 = b.x >> 16;
       const uint32_t bw = (b.y & 0xffffu) - tx0 + 1u;
       const uint32_t ty = ty0 + local / bw, tx = tx0 + local % bw;
