@@ -331,7 +331,7 @@ def roofline(passes, P, T, target_bytes, ls, traffic_file):
 def short_name(kernel):
     """rocprof's kernel name without 'void ' and the argument list: sl_map_accumulate<1u>."""
     n = kernel[5:] if kernel.startswith("void ") else kernel
-    return n.split("(")[0]
+    return n.replace("(anonymous namespace)::", "").split("(")[0]
 
 
 def summary_entry(path, tag, kernel, field):
