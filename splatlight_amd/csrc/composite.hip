@@ -32,25 +32,25 @@ struct __align__(16) SplatBounds {   // tight ellipse bounds (NDC), widened by a
   float x0, x1, y0, y1;
 };
 
-struct __align__(16) RawSplat {      // the splat's own words, for the exact re-evaluation
-  float a0x, a0y, a1x, a1y;
-  float det, rr, tau0, pad;
-};
-
-// The exact coverage of one fragment (the CPU restatement's expression order, no contraction, correctly
-// rounded exp): alpha, or -1 when the pixel centre is outside the quad.
-__device__ __noinline__ float exact_alpha(float dx, float dy, const RawSplat rs) {
+// The exact coverage of one fragment of splat s, read again from memory (the case is rare): the CPU
+// restatement's expression order, no contraction, correctly rounded exp; alpha, or -1 when the pixel centre is
+// outside the quad.
+__device__ __noinline__ float exact_alpha(float dx, float dy, const Splat *__restrict__ sp) {
 #pragma clang fp contract(off)
-  const float u = (dx * rs.a1y - rs.a1x * dy) / rs.det;
-  const float v = (rs.a0x * dy - dx * rs.a0y) / rs.det;
+  const Splat s = *sp;
+  const float a0x = s.center_axis0.z, a0y = s.center_axis0.w, a1x = s.axis1_r_tau.x, a1y = s.axis1_r_tau.y;
+  const float rr = s.axis1_r_tau.z, tau0 = s.axis1_r_tau.w;
+  const float det = a0x * a1y - a1x * a0y;
+  const float u = (dx * a1y - a1x * dy) / det;
+  const float v = (a0x * dy - dx * a0y) / det;
   if (fabsf(u) > 1.0f || fabsf(v) > 1.0f) return -1.0f;
-  const float lx = u * rs.rr, ly = v * rs.rr;
+  const float lx = u * rr, ly = v * rr;
   const float e = (float)exp((double)(-0.5f * (lx * lx + ly * ly)));
-  return 1.0f - (float)exp((double)(-rs.tau0 * e));
+  return 1.0f - (float)exp((double)(-tau0 * e));
 }
 
 template <uint32_t CP_BATCH_T>
-__global__ void __launch_bounds__(CP_THREADS)
+__global__ void __launch_bounds__(CP_THREADS, 8)
 sl_composite(const ViewDev *__restrict__ views, const uint32_t *__restrict__ tile_base, uint32_t n_views,
              const ViewCounters *__restrict__ view_counters, const Splat *__restrict__ splats,
              const uint32_t *__restrict__ vals, const uint2 *__restrict__ ranges,
@@ -58,7 +58,7 @@ sl_composite(const ViewDev *__restrict__ views, const uint32_t *__restrict__ til
   __shared__ StagedSplat s_sp[CP_BATCH_T];
   __shared__ float4 s_col[CP_BATCH_T];
   __shared__ SplatBounds s_bd[CP_BATCH_T];
-  __shared__ RawSplat s_raw[CP_BATCH_T];
+  __shared__ uint32_t s_idx[CP_BATCH_T];              // the splat's slot, for the exact re-evaluation
 
   // workgroup b takes the b-th heaviest tile of the batch (sl_tile_order): longest-first over the XCDs
   const uint32_t gtile = tile_order[blockIdx.x];
@@ -126,10 +126,7 @@ sl_composite(const ViewDev *__restrict__ views, const uint32_t *__restrict__ til
       b.x0 = cx - ex; b.x1 = cx + ex; b.y0 = cy - ey; b.y1 = cy + ey;
       if (!ok) { b.x0 = 2.0f; b.x1 = -2.0f; }            // culled by every wave
       s_bd[threadIdx.x] = b;
-      RawSplat rs;
-      rs.a0x = a0x; rs.a0y = a0y; rs.a1x = a1x; rs.a1y = a1y;
-      rs.det = det; rs.rr = r; rs.tau0 = tau0; rs.pad = 0.0f;
-      s_raw[threadIdx.x] = rs;
+      s_idx[threadIdx.x] = g;
     }
     __syncthreads();
     for (uint32_t k0 = 0; k0 < nb; k0 += 64u) {
@@ -155,7 +152,7 @@ sl_composite(const ViewDev *__restrict__ views, const uint32_t *__restrict__ til
         const float4 c = s_col[k];
         float alpha;
         if (s >= sp.lo) {                                 // q in the band around 1: decide exactly
-          alpha = exact_alpha(dx, dy, s_raw[k]);
+          alpha = exact_alpha(dx, dy, splats + s_idx[k]);
           if (alpha < alpha_min) continue;
         } else {
           alpha = 1.0f - __builtin_amdgcn_exp2f(c.w * __builtin_amdgcn_exp2f(-s));
