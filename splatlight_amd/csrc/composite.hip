@@ -142,27 +142,30 @@ sl_composite(const ViewDev *__restrict__ views, const uint32_t *__restrict__ til
       while (mask) {
         const uint32_t k = k0 + (uint32_t)__ffsll((unsigned long long)mask) - 1u;
         mask &= mask - 1ull;
-        if (done) continue;
+        // straight-line for every lane; a lane that has stopped, or whose q is beyond the band (alpha < 1/255),
+        // keeps its state through the selects below
         const StagedSplat sp = s_sp[k];
+        const float4 c = s_col[k];
         const float dx = __fsub_rn(nx, sp.cx), dy = __fsub_rn(ny, sp.cy);
         const float u = dx * sp.ux + dy * sp.uy;
         const float v = dx * sp.vx + dy * sp.vy;
         const float s = u * u + v * v;
-        if (s > sp.hi) continue;                          // q beyond the band: alpha < 1/255
-        const float4 c = s_col[k];
-        float alpha;
-        if (s >= sp.lo) {                                 // q in the band around 1: decide exactly
+        bool use = !done && s <= sp.hi;
+        if (__ballot(use) == 0ull) continue;              // no lane of the wave takes this splat
+        float alpha = 1.0f - __builtin_amdgcn_exp2f(c.w * __builtin_amdgcn_exp2f(-s));
+        if (use && s >= sp.lo) {                          // q in the band around 1: decide exactly (rare)
           alpha = exact_alpha(dx, dy, splats + s_idx[k]);
-          if (alpha < alpha_min) continue;
-        } else {
-          alpha = 1.0f - __builtin_amdgcn_exp2f(c.w * __builtin_amdgcn_exp2f(-s));
+          use = alpha >= alpha_min;
         }
-        if (overdraw) { nfrag += 1.0f; continue; }
+        if (overdraw) {
+          nfrag = use ? nfrag + 1.0f : nfrag;
+          continue;
+        }
         const float wgt = T * alpha;
-        cr += wgt * c.x;
-        cg += wgt * c.y;
-        cb += wgt * c.z;
-        T -= wgt;
+        cr = use ? cr + wgt * c.x : cr;
+        cg = use ? cg + wgt * c.y : cg;
+        cb = use ? cb + wgt * c.z : cb;
+        T = use ? T - wgt : T;
         done = T < t_eps;
       }
     }
