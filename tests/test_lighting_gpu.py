@@ -94,6 +94,23 @@ def test_hill_lighting_with_sun(ctx):
     scene.destroy()
 
 
+def test_map_culling_stress(ctx):
+    """Needle-thin Gaussians at every angle and Gaussians whose 3-sigma box is clamped to the map's maximum half-size:
+    the cases where the map sums' ellipse masks (which tiles a record may touch, which 16x4 bands of a tile) and the
+    box clamp matter most. The maps must still equal the oracle's bit for bit."""
+    log("lighting parity: hill n=24000 with needles and clamped giants, sun map 512^2")
+    sc = scenes.hill(n=24_000, seed=11, width=320, height=180)
+    g = sc.gaussians
+    rng = np.random.default_rng(5)
+    idx = rng.permutation(np.flatnonzero(g["part_surface"] == 0))      # the ground part
+    needles, giants = idx[: len(idx) // 3], idx[len(idx) // 3: 2 * len(idx) // 3]
+    g["scale"][needles] = np.sort(np.column_stack([np.full(len(needles), 0.002), np.full(len(needles), 0.004),
+                                                   rng.uniform(0.2, 0.6, len(needles))]), axis=1)
+    g["scale"][giants] = np.sort(rng.uniform(0.3, 1.5, (len(giants), 3)), axis=1)
+    scene, _, _ = check_lighting(ctx, sc, sun_res=512)
+    scene.destroy()
+
+
 def test_lit_frame_against_oracle(ctx):
     """Relight then draw on the device; the oracle relights and draws on the host."""
     sc = scenes.cornell(n=20_000, seed=5, width=320, height=240)
