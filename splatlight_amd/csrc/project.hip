@@ -37,10 +37,34 @@ __device__ __forceinline__ uint32_t depth_key(float depth, float near_, float lo
 
 }  // namespace
 
+// The splat colour of a visible slot (project.wgsl:317-350): the lit colour plus gain x bounce in the default view,
+// else the albedo or the part palette, scaled.
+__device__ __forceinline__ void splat_colour(const FrameDev &f, uint32_t material, float4 C4,
+                                             const float4 *__restrict__ colour, const float4 *__restrict__ lo_b,
+                                             uint32_t i, float col[3]) {
+  col[0] = C4.x; col[1] = C4.y; col[2] = C4.z;
+  if (material != SL_EMITTER_MATERIAL) {
+    if (f.view_mode == 3u) {
+      const float4 cv = colour ? colour[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+      const float4 lb = lo_b ? lo_b[3 * (size_t)i] : make_float4(0.f, 0.f, 0.f, 0.f);
+      col[0] = cv.x + f.gain * lb.x;
+      col[1] = cv.y + f.gain * lb.y;
+      col[2] = cv.z + f.gain * lb.z;
+      if (material < 8u) { col[0] *= f.palette[material][0]; col[1] *= f.palette[material][1]; col[2] *= f.palette[material][2]; }
+    } else if (material < 8u && f.palette[material][3] > 0.5f) {
+      col[0] = f.palette[material][0]; col[1] = f.palette[material][1]; col[2] = f.palette[material][2];
+    }
+    col[0] *= f.albedo_scale; col[1] *= f.albedo_scale; col[2] *= f.albedo_scale;
+  }
+}
+
 // One lane per global slot of a batch; the 256-slot block belongs to one view. A valid slot's key is
 // (view << 16) | the 16-bit depth key, so one stable sort orders views, then depth, then slot; slots past a view's
-// count (padding to 256) get INVALID.
-extern "C" __global__ void __launch_bounds__(256)
+// count (padding to 256) get INVALID. COLOUR: the splats' colour words and the per-block colour max are written
+// here; without it they are left to sl_splat_colour (so the geometry need not wait for the lighting), the colour
+// words are 0 and the partials' colour max is 0.
+template <bool COLOUR>
+__global__ void __launch_bounds__(256)
 sl_project_kernel(const ViewDev *__restrict__ views, const uint32_t *__restrict__ slot_base, uint32_t n_views,
                   Splat *__restrict__ splats, uint32_t *__restrict__ keys, uint2 *__restrict__ tbox,
                   uint2 *__restrict__ partials) {
@@ -179,21 +203,8 @@ sl_project_kernel(const ViewDev *__restrict__ views, const uint32_t *__restrict_
     const float ex = fabsf(a0x) + fabsf(a1x), ey = fabsf(a0y) + fabsf(a1y);
     ok = ok && !(fabsf(cenx) - ex > 1.0f || fabsf(ceny) - ey > 1.0f);
     if (ok) {
-      // colour (project.wgsl:317-350)
-      float col[3] = {C4.x, C4.y, C4.z};
-      if (material != SL_EMITTER_MATERIAL) {
-        if (f.view_mode == 3u) {
-          const float4 cv = colour ? colour[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-          const float4 lb = lo_b ? lo_b[3 * (size_t)i] : make_float4(0.f, 0.f, 0.f, 0.f);
-          col[0] = cv.x + f.gain * lb.x;
-          col[1] = cv.y + f.gain * lb.y;
-          col[2] = cv.z + f.gain * lb.z;
-          if (material < 8u) { col[0] *= f.palette[material][0]; col[1] *= f.palette[material][1]; col[2] *= f.palette[material][2]; }
-        } else if (material < 8u && f.palette[material][3] > 0.5f) {
-          col[0] = f.palette[material][0]; col[1] = f.palette[material][1]; col[2] = f.palette[material][2];
-        }
-        col[0] *= f.albedo_scale; col[1] *= f.albedo_scale; col[2] *= f.albedo_scale;
-      }
+      float col[3] = {0.0f, 0.0f, 0.0f};
+      if (COLOUR) splat_colour(f, material, C4, colour, lo_b, i, col);
       out.center_axis0 = make_float4(cenx, ceny, a0x, a0y);
       out.axis1_r_tau = make_float4(a1x, a1y, r, tau0);
       // SPLAT_GLASS (shaders/lens.wgsl:21) in the depth word's low bits (project.wgsl:167-168)
@@ -202,7 +213,7 @@ sl_project_kernel(const ViewDev *__restrict__ views, const uint32_t *__restrict_
       out.color_depth = make_uint4(__float_as_uint(col[0]), __float_as_uint(col[1]), __float_as_uint(col[2]),
                                    (__float_as_uint(depth) & ~3u) | glass);
       key = (v << 16) | depth_key(depth, f.near_, f.log_far_near);
-      cmax = fmaxf(fmaxf(fabsf(col[0]), fabsf(col[1])), fabsf(col[2]));
+      if (COLOUR) cmax = fmaxf(fmaxf(fabsf(col[0]), fabsf(col[1])), fabsf(col[2]));
       // tile box of the extent box in pixel centres (the pair definition)
       const float hw = 0.5f * (float)f.width, hh = 0.5f * (float)f.height;
       float fx0 = ceilf((cenx - ex + 1.0f) * hw - 0.5f), fx1 = floorf((cenx + ex + 1.0f) * hw - 0.5f);
@@ -239,11 +250,55 @@ sl_project_kernel(const ViewDev *__restrict__ views, const uint32_t *__restrict_
   }
 }
 
+// The colour words of the visible splats and the per-block colour max, for a batch projected without COLOUR:
+// the same slots, colours and block partials (their .y) the full projection writes. Visible: a valid key.
+extern "C" __global__ void __launch_bounds__(256)
+sl_splat_colour(const ViewDev *__restrict__ views, const uint32_t *__restrict__ slot_base, uint32_t n_views,
+                const uint32_t *__restrict__ keys, Splat *__restrict__ splats, uint2 *__restrict__ partials) {
+  __shared__ uint32_t s_max[256 / SL_WAVE];
+  const uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t v = n_views > 1u ? view_of(slot_base, n_views, blockIdx.x * blockDim.x) : 0u;
+  const ViewDev &vd = views[v];
+  const FrameDev &f = vd.f;
+  const uint32_t i = gi - vd.slot_base;
+  float cmax = 0.0f;
+  if (keys[gi] != SL_INVALID_KEY) {
+    float4 C4;
+    uint32_t material = SL_EMITTER_MATERIAL;
+    if (i > f.count) {
+      C4 = vd.e3[i - f.count - 1u];
+    } else {
+      C4 = sl_albedo(vd.g1, vd.g2, vd.g3, vd.storage, i);
+      uint32_t part = __float_as_uint(sl_scale(vd.g2, vd.storage, i).w) & 0xffffu;
+      part = part < SL_MAX_PARTS - 1u ? part : SL_MAX_PARTS - 1u;
+      material = f.part_material[part];
+    }
+    float col[3];
+    splat_colour(f, material, C4, vd.colour, vd.lo_b, i, col);
+    uint32_t *cw = reinterpret_cast<uint32_t *>(&splats[gi].color_depth);
+    cw[0] = __float_as_uint(col[0]);
+    cw[1] = __float_as_uint(col[1]);
+    cw[2] = __float_as_uint(col[2]);
+    cmax = fmaxf(fmaxf(fabsf(col[0]), fabsf(col[1])), fabsf(col[2]));
+  }
+  float wmax = cmax;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) wmax = fmaxf(wmax, __shfl_xor(wmax, o));
+  if (lane_id() == 0) s_max[threadIdx.x >> 6] = __float_as_uint(wmax);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = s_max[0];
+#pragma unroll
+    for (int k = 1; k < 256 / SL_WAVE; k++) t = max(t, s_max[k]);   // non-negative floats order as their bits
+    partials[blockIdx.x].y = t;
+  }
+}
+
 // Per-view totals from project's per-block partials (one workgroup per view): visible count and colour max into
-// view_counters[v]; the batch's visible total into counters.
+// view_counters[v]; the batch's visible total into counters. what: bit 0 the visible counts, bit 1 the colour max.
 extern "C" __global__ void __launch_bounds__(1024)
 sl_frame_totals(const uint2 *__restrict__ partials, const uint32_t *__restrict__ slot_base,
-                ViewCounters *__restrict__ view_counters, FrameCounters *__restrict__ counters) {
+                ViewCounters *__restrict__ view_counters, FrameCounters *__restrict__ counters, uint32_t what) {
   __shared__ uint2 s_w[1024 / SL_WAVE];
   const uint32_t v = blockIdx.x;
   const uint32_t b0 = slot_base[v] / 256u, b1 = slot_base[v + 1] / 256u;
@@ -266,9 +321,13 @@ sl_frame_totals(const uint2 *__restrict__ partials, const uint32_t *__restrict__
       a.x += s_w[k].x;
       a.y = max(a.y, s_w[k].y);
     }
-    view_counters[v].visible = a.x;
-    view_counters[v].color_max_bits = a.y;
-    if (a.x) atomicAdd(&counters->visible, a.x);
-    atomicMax(&counters->color_max_bits, a.y);
+    if (what & 1u) {
+      view_counters[v].visible = a.x;
+      if (a.x) atomicAdd(&counters->visible, a.x);
+    }
+    if (what & 2u) {
+      view_counters[v].color_max_bits = a.y;
+      atomicMax(&counters->color_max_bits, a.y);
+    }
   }
 }

@@ -132,6 +132,7 @@ struct sl_context {
   Workspace ws;
   FrameCounters *counters_host = nullptr;   // pinned
   hipEvent_t ev[kPasses + 1] = {};
+  hipEvent_t ev_comp = nullptr;   // before the composite when the geometry ran beside the lighting
   hipEvent_t lev[5] = {};         // lighting: shadow, shade, sun, env
   hipEvent_t mev[4] = {};         // the two map-sum launches (emitter map, sun maps): begin, end
   bool map_timed[2] = {false, false};
@@ -160,6 +161,12 @@ struct sl_context {
   hipStream_t side = nullptr;
   hipEvent_t fork = nullptr, join = nullptr, sev[2] = {};
   bool sun_timed = false;
+  // the frame geometry's stream (created with the side stream), the event before the last lighting encode that the
+  // geometry waits for, the event after it that the context's stream waits for; light_pending: a lighting encode
+  // since the last batch, so the next batch's geometry may run beside it
+  hipStream_t geo = nullptr;
+  hipEvent_t pre_light = nullptr, geo_done = nullptr;
+  bool light_pending = false, timed_overlap = false;
 };
 
 namespace {
@@ -236,6 +243,7 @@ sl_status ensure_workspace(sl_context *ctx, uint32_t slots, uint32_t pairs, uint
   void *base = nullptr;
   if (w.base) {
     SL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->geo) SL_HIP(ctx, hipStreamSynchronize(ctx->geo));
     SL_HIP(ctx, hipFree(w.base));
     w.base = nullptr;
   }
@@ -354,24 +362,32 @@ uint32_t grid_for(uint64_t n, uint32_t per_block, uint32_t cap) {
   return (uint32_t)std::min<uint64_t>(g, cap);
 }
 
-// The stage sequence of a batch on the context's stream: one launch per stage over every view.
-sl_status enqueue_batch(sl_context *ctx, const Batch &b, bool timers) {
+// The stage sequence of a batch: one launch per stage over every view. overlap (a batch encoded after a lighting
+// encode): the geometry stages (projection without colour, sorts, pair emission, ranges) run on the geometry
+// stream after the work queued before that lighting, beside it; the context's stream waits for them and for the
+// lighting, then writes the splats' colours and colour max and composites. Otherwise every stage runs on the
+// context's stream.
+sl_status enqueue_batch(sl_context *ctx, const Batch &b, bool timers, bool overlap) {
   Workspace &w = ctx->ws;
-  hipStream_t st = ctx->stream;
+  const hipStream_t ms = ctx->stream, st = overlap ? ctx->geo : ctx->stream;
   const uint32_t V = (uint32_t)b.views.size();
-  auto mark = [&](int k) -> sl_status {
-    if (timers) SL_HIP(ctx, hipEventRecord(ctx->ev[k], st));
+  auto mark = [&](int k, hipStream_t on) -> sl_status {
+    if (timers) SL_HIP(ctx, hipEventRecord(ctx->ev[k], on));
     return SL_OK;
   };
   sl_status s;
-  if ((s = mark(0)) != SL_OK) return s;
+  if ((s = mark(0, st)) != SL_OK) return s;
   sl_fill_u32<<<grid_for(w.scratch_words, 256, 2048), 256, 0, st>>>((uint32_t *)w.scratch, 0u,
                                                                     (uint32_t)w.scratch_words);
-  if ((s = mark(1)) != SL_OK) return s;
-  sl_project_kernel<<<b.total_slots / 256u, 256, 0, st>>>(w.views, w.slot_base, V, w.splats, w.keys, w.tbox,
-                                                          w.partials);
-  sl_frame_totals<<<V, 1024, 0, st>>>(w.partials, w.slot_base, w.view_counters, w.counters);
-  if ((s = mark(2)) != SL_OK) return s;
+  if ((s = mark(1, st)) != SL_OK) return s;
+  if (overlap)
+    sl_project_kernel<false><<<b.total_slots / 256u, 256, 0, st>>>(w.views, w.slot_base, V, w.splats, w.keys, w.tbox,
+                                                                   w.partials);
+  else
+    sl_project_kernel<true><<<b.total_slots / 256u, 256, 0, st>>>(w.views, w.slot_base, V, w.splats, w.keys, w.tbox,
+                                                                  w.partials);
+  sl_frame_totals<<<V, 1024, 0, st>>>(w.partials, w.slot_base, w.view_counters, w.counters, overlap ? 1u : 3u);
+  if ((s = mark(2, st)) != SL_OK) return s;
   // depth order: stable sort of (view, 16-bit depth) keys, values = global slot
   const uint32_t n = b.total_slots;
   const sl_tuning &tu = ctx->tuning;
@@ -398,11 +414,11 @@ sl_status enqueue_batch(sl_context *ctx, const Batch &b, bool timers) {
     radix_pass(w.pk1, w.pv1, nullptr, w.order, nullptr, n, 16u, w.depth_hist + 512, w.dstatus[2], w.tickets + 2,
                d_grid);
   }
-  if ((s = mark(3)) != SL_OK) return s;
+  if ((s = mark(3, st)) != SL_OK) return s;
   sl_pairs_emit<false><<<grid_for(n, PE_CHUNK, 2u * ctx->cus), PE_THREADS, 0, st>>>(
       w.order, w.tbox, nullptr, w.slot_base, w.tile_base, w.tiles_x, V, b.tile_passes, w.pairs_cap, w.counters, w.pk0, w.pv0,
       w.chunk_status, w.tickets + 3, w.tile_hist);
-  if ((s = mark(4)) != SL_OK) return s;
+  if ((s = mark(4, st)) != SL_OK) return s;
   const uint32_t p_grid = std::max(std::min<uint32_t>((w.pairs_cap + rs_tile - 1) / rs_tile, tu.grid_mul * ctx->cus), 1u);
   sl_radix_bases<<<b.tile_passes, RS_THREADS, 0, st>>>(w.tile_hist);
   uint32_t *kin = w.pk0, *vin = w.pv0, *kout = w.pk1, *vout = w.pv1;
@@ -414,41 +430,53 @@ sl_status enqueue_batch(sl_context *ctx, const Batch &b, bool timers) {
   }
   ctx->final_keys = kin;
   ctx->final_vals = vin;
-  if ((s = mark(5)) != SL_OK) return s;
+  if ((s = mark(5, st)) != SL_OK) return s;
   sl_tile_ranges<<<grid_for(w.pairs_cap, 256, 4096), 256, 0, st>>>(kin, w.counters, w.ranges);
   sl_tile_order<<<1, 1024, 0, st>>>(w.ranges, b.total_tiles, w.tile_order);
   if (V > 1) sl_view_pairs<<<V, 256, 0, st>>>(w.ranges, w.tile_base, w.view_counters);   // one view: the total
-  if ((s = mark(6)) != SL_OK) return s;
+  if ((s = mark(6, st)) != SL_OK) return s;
+  if (overlap) {
+    SL_HIP(ctx, hipEventRecord(ctx->geo_done, st));
+    SL_HIP(ctx, hipStreamWaitEvent(ms, ctx->geo_done, 0));
+    sl_splat_colour<<<b.total_slots / 256u, 256, 0, ms>>>(w.views, w.slot_base, V, w.keys, w.splats, w.partials);
+    sl_frame_totals<<<V, 1024, 0, ms>>>(w.partials, w.slot_base, w.view_counters, w.counters, 2u);
+    if (timers) SL_HIP(ctx, hipEventRecord(ctx->ev_comp, ms));
+  }
+  ctx->timed_overlap = overlap;
   switch (tu.composite_batch) {
     case 64u:
-      sl_composite<64><<<b.total_tiles, CP_THREADS, 0, st>>>(w.views, w.tile_base, V, w.view_counters, w.splats, vin,
+      sl_composite<64><<<b.total_tiles, CP_THREADS, 0, ms>>>(w.views, w.tile_base, V, w.view_counters, w.splats, vin,
                                                              w.ranges, w.tile_order);
       break;
     case 128u:
-      sl_composite<128><<<b.total_tiles, CP_THREADS, 0, st>>>(w.views, w.tile_base, V, w.view_counters, w.splats, vin,
+      sl_composite<128><<<b.total_tiles, CP_THREADS, 0, ms>>>(w.views, w.tile_base, V, w.view_counters, w.splats, vin,
                                                               w.ranges, w.tile_order);
       break;
     default:
-      sl_composite<256><<<b.total_tiles, CP_THREADS, 0, st>>>(w.views, w.tile_base, V, w.view_counters, w.splats, vin,
+      sl_composite<256><<<b.total_tiles, CP_THREADS, 0, ms>>>(w.views, w.tile_base, V, w.view_counters, w.splats, vin,
                                                               w.ranges, w.tile_order);
   }
-  if ((s = mark(7)) != SL_OK) return s;
+  if ((s = mark(7, ms)) != SL_OK) return s;
   return check_launch(ctx, "frame kernels");
 }
 
 // Upload the view table, run the batch (replaying its graph when one was captured for the same layout), and
-// queue the counters' read-back.
+// queue the counters' read-back. After a lighting encode (and without a graph) the geometry stages run beside the
+// lighting: the view table then goes up on the geometry stream, after the work queued before that lighting.
 sl_status launch_batch(sl_context *ctx, const Batch &b) {
   Workspace &w = ctx->ws;
   hipStream_t st = ctx->stream;
   const uint32_t V = (uint32_t)b.views.size();
+  const bool overlap = !b.graph && ctx->light_pending;
+  ctx->light_pending = false;
+  if (overlap) SL_HIP(ctx, hipStreamWaitEvent(ctx->geo, ctx->pre_light, 0));
   std::vector<uint8_t> blk(kViewBlock, 0);
   memcpy(blk.data(), b.views.data(), sizeof(ViewDev) * V);
   uint32_t *sb = (uint32_t *)(blk.data() + sizeof(ViewDev) * kMaxViews);
   memcpy(sb, b.slot_base.data(), 4 * (V + 1));
   memcpy(sb + kMaxViews + 1, b.tile_base.data(), 4 * (V + 1));
   memcpy(sb + 2 * (kMaxViews + 1), b.tiles_x.data(), 4 * V);
-  SL_HIP(ctx, hipMemcpyAsync(w.views, blk.data(), blk.size(), hipMemcpyHostToDevice, st));
+  SL_HIP(ctx, hipMemcpyAsync(w.views, blk.data(), blk.size(), hipMemcpyHostToDevice, overlap ? ctx->geo : st));
   const bool timers = (b.flags & SL_FRAME_TIMERS) != 0 && !b.graph;
   sl_status s;
   if (b.graph) {
@@ -463,7 +491,7 @@ sl_status launch_batch(sl_context *ctx, const Batch &b) {
       }
       hipGraph_t g = nullptr;
       SL_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      s = enqueue_batch(ctx, b, false);
+      s = enqueue_batch(ctx, b, false, false);
       hipError_t e = hipStreamEndCapture(st, &g);
       if (s != SL_OK) {
         if (g) (void)hipGraphDestroy(g);
@@ -481,7 +509,7 @@ sl_status launch_batch(sl_context *ctx, const Batch &b) {
       ctx->final_vals = b.tile_passes % 2 ? ww.pv1 : ww.pv0;
     }
     SL_HIP(ctx, hipGraphLaunch(ctx->graph, st));
-  } else if ((s = enqueue_batch(ctx, b, timers)) != SL_OK) {
+  } else if ((s = enqueue_batch(ctx, b, timers, overlap)) != SL_OK) {
     return s;
   }
   SL_HIP(ctx, hipMemcpyAsync(ctx->counters_host, w.counters, sizeof(FrameCounters), hipMemcpyDeviceToHost, st));
@@ -553,6 +581,7 @@ extern "C" sl_status sl_ctx_create(int device_ordinal, sl_context **out) {
     return bail(e);
   for (int k = 0; k <= kPasses; k++)
     if ((e = hipEventCreate(&ctx->ev[k])) != hipSuccess) return bail(e);
+  if ((e = hipEventCreate(&ctx->ev_comp)) != hipSuccess) return bail(e);
   for (int k = 0; k < 5; k++)
     if ((e = hipEventCreate(&ctx->lev[k])) != hipSuccess) return bail(e);
   for (int k = 0; k < 4; k++)
@@ -565,9 +594,12 @@ extern "C" void sl_ctx_destroy(sl_context *ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->side) (void)hipStreamSynchronize(ctx->side);
+  if (ctx->geo) (void)hipStreamSynchronize(ctx->geo);
   if (ctx->ws.base) (void)hipFree(ctx->ws.base);
   for (int k = 0; k <= kPasses; k++)
     if (ctx->ev[k]) (void)hipEventDestroy(ctx->ev[k]);
+  if (ctx->ev_comp) (void)hipEventDestroy(ctx->ev_comp);
   for (int k = 0; k < 5; k++)
     if (ctx->lev[k]) (void)hipEventDestroy(ctx->lev[k]);
   for (int k = 0; k < 4; k++)
@@ -580,9 +612,10 @@ extern "C" void sl_ctx_destroy(sl_context *ctx) {
     for (Grow *gb : {&m.rec, &m.tbox, &m.tmask, &m.k0, &m.v0, &m.k1, &m.v1, &m.scratch})
       if (gb->p) (void)hipFree(gb->p);
   if (ctx->map_stats.p) (void)hipFree(ctx->map_stats.p);
-  for (hipEvent_t e : {ctx->fork, ctx->join, ctx->sev[0], ctx->sev[1]})
+  for (hipEvent_t e : {ctx->fork, ctx->join, ctx->sev[0], ctx->sev[1], ctx->pre_light, ctx->geo_done})
     if (e) (void)hipEventDestroy(e);
   if (ctx->side) (void)hipStreamDestroy(ctx->side);
+  if (ctx->geo) (void)hipStreamDestroy(ctx->geo);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -764,6 +797,8 @@ sl_status grow(sl_context *ctx, sl_scene *sc, Grow *gb, size_t bytes) {
   if (gb->p && gb->bytes >= bytes) return SL_OK;
   if (gb->p) {
     SL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->side) SL_HIP(ctx, hipStreamSynchronize(ctx->side));
+    if (ctx->geo) SL_HIP(ctx, hipStreamSynchronize(ctx->geo));
     SL_HIP(ctx, hipFree(gb->p));
     sc->bytes -= gb->bytes;
     gb->p = nullptr;
@@ -880,6 +915,7 @@ extern "C" void sl_scene_destroy(sl_context *ctx, sl_scene *scene) {
   if (!scene) return;
   if (ctx && ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   if (ctx && ctx->side) (void)hipStreamSynchronize(ctx->side);
+  if (ctx && ctx->geo) (void)hipStreamSynchronize(ctx->geo);
   if (ctx && ctx->has_last && std::find(ctx->last.scenes.begin(), ctx->last.scenes.end(), scene) != ctx->last.scenes.end())
     ctx->has_last = false;
   scene_free(scene);
@@ -1018,7 +1054,8 @@ sl_status finish_batch(sl_context *ctx, sl_frame_stats *stats, uint32_t n_stats)
   if (ctx->timers_valid) {
     for (int k = 0; k < kPasses; k++) {
       float ms = 0.0f;
-      (void)hipEventElapsedTime(&ms, ctx->ev[k], ctx->ev[k + 1]);
+      // overlapped: "splat" is the composite alone (its start waited for the lighting, then the colour pass)
+      (void)hipEventElapsedTime(&ms, k == kPasses - 1 && ctx->timed_overlap ? ctx->ev_comp : ctx->ev[k], ctx->ev[k + 1]);
       ctx->ms[k] = ms;
     }
   }
@@ -1404,6 +1441,11 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
       have_sun = true;
     }
   }
+  if (!ctx->geo) {
+    SL_HIP(ctx, hipStreamCreateWithFlags(&ctx->geo, hipStreamNonBlocking));
+    SL_HIP(ctx, hipEventCreateWithFlags(&ctx->pre_light, hipEventDisableTiming));
+    SL_HIP(ctx, hipEventCreateWithFlags(&ctx->geo_done, hipEventDisableTiming));
+  }
   if (have_sun && !ctx->side) {
     SL_HIP(ctx, hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
     SL_HIP(ctx, hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming));
@@ -1415,6 +1457,9 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
   const bool timers = (d->flags & SL_LIGHT_TIMERS) != 0;
   ctx->map_timed[0] = ctx->map_timed[1] = false;
   ctx->sun_timed = false;
+  // the next batch's geometry may run from here on, beside this lighting
+  SL_HIP(ctx, hipEventRecord(ctx->pre_light, st));
+  ctx->light_pending = true;
   SL_HIP(ctx, hipMemcpyAsync(sc->emit_nodes.p, d->emit_nodes, (size_t)d->emit_node_count * 96, hipMemcpyHostToDevice, st));
   if (fresh_stats) sl_fill_u32<<<1, 256, 0, st>>>((uint32_t *)ctx->map_stats.p, 0u, 4u);
   if (timers) SL_HIP(ctx, hipEventRecord(ctx->lev[0], st));
