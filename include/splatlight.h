@@ -88,7 +88,9 @@ typedef struct sl_frame_opts {
   uint32_t flags;              /* SL_FRAME_* */
 } sl_frame_opts;
 /* Enqueue one frame: project, depth order, pair emission, tile sort, tile ranges, composite. out_device holds
- * width * height pixels of out_format, rows top-down. frame_bytes: the 1200 B RenderFrame. */
+ * width * height pixels of out_format, rows top-down. frame_bytes: the 1200 B RenderFrame. Encoded after
+ * sl_lighting_encode, the frame's geometry stages run beside that lighting (they read nothing it writes) and only
+ * the colours and the composite wait for it; the results are the same. */
 sl_status sl_frame_encode(sl_context *ctx, sl_scene *scene, const void *frame_bytes, size_t frame_len,
                           const sl_frame_opts *opts, void *out_device);
 typedef struct sl_frame_stats {
