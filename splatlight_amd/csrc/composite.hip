@@ -139,32 +139,51 @@ sl_composite(const ViewDev *__restrict__ views, const uint32_t *__restrict__ til
         hit = b.x0 <= wx1 && b.x1 >= wx0 && b.y0 <= wy1 && b.y1 >= wy0;
       }
       uint64_t mask = __ballot(hit);
+      // two splats per step: their coverage does not depend on the transmittance, so both are computed before
+      // they are blended, in order; every lane runs the same straight-line code, and a lane that has stopped, or
+      // whose q is beyond the band (alpha < 1/255), keeps its state through the selects
       while (mask) {
-        const uint32_t k = k0 + (uint32_t)__ffsll((unsigned long long)mask) - 1u;
+        const uint32_t ka = k0 + (uint32_t)__ffsll((unsigned long long)mask) - 1u;
         mask &= mask - 1ull;
-        // straight-line for every lane; a lane that has stopped, or whose q is beyond the band (alpha < 1/255),
-        // keeps its state through the selects below
-        const StagedSplat sp = s_sp[k];
-        const float4 c = s_col[k];
-        const float dx = __fsub_rn(nx, sp.cx), dy = __fsub_rn(ny, sp.cy);
-        const float u = dx * sp.ux + dy * sp.uy;
-        const float v = dx * sp.vx + dy * sp.vy;
-        const float s = u * u + v * v;
-        bool use = !done && s <= sp.hi;
-        if (__ballot(use) == 0ull) continue;              // no lane of the wave takes this splat
-        float alpha = 1.0f - __builtin_amdgcn_exp2f(c.w * __builtin_amdgcn_exp2f(-s));
-        if (use && s >= sp.lo) {                          // q in the band around 1: decide exactly (rare)
-          alpha = exact_alpha(dx, dy, splats + s_idx[k]);
-          use = alpha >= alpha_min;
+        const bool two = mask != 0ull;
+        const uint32_t kb = two ? k0 + (uint32_t)__ffsll((unsigned long long)mask) - 1u : ka;
+        if (two) mask &= mask - 1ull;
+        const StagedSplat pa = s_sp[ka], pb = s_sp[kb];
+        const float4 ca = s_col[ka], cb4 = s_col[kb];
+        const float dxa = __fsub_rn(nx, pa.cx), dya = __fsub_rn(ny, pa.cy);
+        const float dxb = __fsub_rn(nx, pb.cx), dyb = __fsub_rn(ny, pb.cy);
+        const float ua = dxa * pa.ux + dya * pa.uy, va = dxa * pa.vx + dya * pa.vy;
+        const float ub = dxb * pb.ux + dyb * pb.uy, vb = dxb * pb.vx + dyb * pb.vy;
+        const float sa = ua * ua + va * va, sb = ub * ub + vb * vb;
+        bool ina = sa <= pa.hi, inb = two && sb <= pb.hi;   // within the band (or inside)
+        if (__ballot(!done && (ina || inb)) == 0ull) continue;   // no lane of the wave takes either
+        float alpha_a = 1.0f - __builtin_amdgcn_exp2f(ca.w * __builtin_amdgcn_exp2f(-sa));
+        float alpha_b = 1.0f - __builtin_amdgcn_exp2f(cb4.w * __builtin_amdgcn_exp2f(-sb));
+        if (!done && ina && sa >= pa.lo) {                // q in the band around 1: decide exactly (rare)
+          alpha_a = exact_alpha(dxa, dya, splats + s_idx[ka]);
+          ina = alpha_a >= alpha_min;
+        }
+        if (!done && inb && sb >= pb.lo) {
+          alpha_b = exact_alpha(dxb, dyb, splats + s_idx[kb]);
+          inb = alpha_b >= alpha_min;
         }
         if (overdraw) {
-          nfrag = use ? nfrag + 1.0f : nfrag;
+          nfrag = !done && ina ? nfrag + 1.0f : nfrag;
+          nfrag = !done && inb ? nfrag + 1.0f : nfrag;
           continue;
         }
-        const float wgt = T * alpha;
-        cr = use ? cr + wgt * c.x : cr;
-        cg = use ? cg + wgt * c.y : cg;
-        cb = use ? cb + wgt * c.z : cb;
+        bool use = !done && ina;
+        float wgt = T * alpha_a;
+        cr = use ? cr + wgt * ca.x : cr;
+        cg = use ? cg + wgt * ca.y : cg;
+        cb = use ? cb + wgt * ca.z : cb;
+        T = use ? T - wgt : T;
+        done = T < t_eps;
+        use = !done && inb;
+        wgt = T * alpha_b;
+        cr = use ? cr + wgt * cb4.x : cr;
+        cg = use ? cg + wgt * cb4.y : cg;
+        cb = use ? cb + wgt * cb4.z : cb;
         T = use ? T - wgt : T;
         done = T < t_eps;
       }
