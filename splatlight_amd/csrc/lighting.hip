@@ -288,8 +288,8 @@ sl_iso_init(uint32_t n, const float4 *__restrict__ g2, uint32_t st, const float4
 
 // ---- shade_emit (runtime/wgsl/emissive.wgsl:123-150, 853-980), M39 matte ----------------------------------------
 
-// STACK: the cut walk's stack depth in LDS, 48 as the reference's, or 16 when the host has shown that walking every
-// internal node of this tree never holds more than 16 entries (then no cut can either, and the walk is the same).
+// One cut walk per lane. STACK: the walk's stack depth in LDS, 48 as the reference's; used for trees whose full
+// walk can hold more than 16 entries (sl_shade_emit_wave below takes the others).
 template <uint32_t STACK>
 __global__ void __launch_bounds__(64)
 sl_shade_emit(FrameUniformsDev U, EmitParamsDev EP, const uint32_t *__restrict__ order,
@@ -394,6 +394,138 @@ sl_shade_emit(FrameUniformsDev U, EmitParamsDev EP, const uint32_t *__restrict__
     const float k = qdiv(fp, d2);
     acc = add3(acc, V3(e3.x * ik * vi * k, e3.y * ik * vi * k, e3.z * ik * vi * k));
   }
+  const float T = lamp_T ? lamp_T[g] : 1.0f;
+  const float tk = T * iso_norm[g];
+  v3 c = V3(C.x * acc.x * tk, C.y * acc.y * tk, C.z * acc.z * tk);
+  if ((mflags & 1u) != 0u && ((mflags & 4u) == 0u || dot3(xf_dir_inv(xf, wo), Rq.c0) > 0.0f)) {
+    c = add3(c, V3(unpack_half(mat[6] & 0xFFFFu), unpack_half(mat[6] >> 16), unpack_half(mat[7] & 0xFFFFu)));
+  }
+  colour[g] = make_float4(c.x, c.y, c.z, 1.0f);
+}
+
+// The same shading with one cut walk per wave instead of one per lane, for trees whose full walk never holds more
+// than 16 stack entries (then no cut can either: a cut's stack is a subset of the full walk's when it reaches a
+// node). The wave walks the union of its lanes' cuts in the lanes' own
+// depth-first order (children pushed low then high, the high one popped first): a node is expanded when any active
+// lane expands it, and a lane that uses a node sits out that node's subtree (it is active again once the walk pops
+// below the node's stack slot). Each lane so meets its own used nodes in its own order and sums them exactly as
+// sl_shade_emit does. The node index, the node's words and the stack (16 x 16-bit entries in four 64-bit words) are
+// the same for every lane, so they live in scalar registers; only the V-grid uses LDS (8 KiB per wave).
+__global__ void __launch_bounds__(64)
+sl_shade_emit_wave(FrameUniformsDev U, EmitParamsDev EP, const uint32_t *__restrict__ order,
+                   const float4 *__restrict__ emit_nodes, uint32_t n_nodes,
+                   const float4 *__restrict__ g0, const float4 *__restrict__ g1, const float4 *__restrict__ g2,
+                   const float4 *__restrict__ g3, uint32_t st, const float4 *__restrict__ nodes,
+                   const uint8_t *__restrict__ vgrid, const float4 *__restrict__ parts,
+                   const float *__restrict__ lamp_T, const uint32_t *__restrict__ materials,
+                   const float *__restrict__ iso_norm, float4 *__restrict__ colour) {
+  const uint32_t t = blockIdx.x * 64u + threadIdx.x;
+  const bool live = t < U.gaussian_count;
+  const uint32_t g = live ? (order ? order[t] : t) : 0u;   // a lane past the count reads Gaussian 0 and sits out
+  const float4 A = g0[g], Q = sl_quat(g1, st, g), S = sl_scale(g2, st, g), C = sl_albedo(g1, g2, g3, st, g);
+  const uint32_t mi = __float_as_uint(C.w) & 0xFFFFu;
+  const uint32_t mmax = max(U.material_count, 1u) - 1u;
+  const uint32_t *mat = materials + 8 * (size_t)min(mi, mmax);
+  const uint32_t mflags = U.material_count ? mat[5] : 0u;
+  bool lane_on = live;
+  if (live && EP.count == 0u && (mflags & 1u) == 0u) {
+    colour[g] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    lane_on = false;
+  }
+  const Xf xf = load_xf(parts, __float_as_uint(S.w) & 0xFFFFu);
+  const v3 P = xf_point(xf, V3(A.x, A.y, A.z));
+  const m3 Rq = quat_matrix(Q);
+  const float s3 = S.x * S.y * S.z;
+  const float detC = s3 * s3;
+  const v3 wo = norm3(sub3(V3(U.eye[0], U.eye[1], U.eye[2]), P));
+  const NodeL nd = load_node(nodes, g);
+  __shared__ uint32_t s_vg[32][64];                    // as in sl_shade_emit
+  uint32_t *vcol = &s_vg[0][threadIdx.x];
+  {
+    const uint4 *src = (const uint4 *)(vgrid + 128 * (size_t)g);
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const uint4 v = src[q];
+      vcol[64 * (4 * q)] = v.x;
+      vcol[64 * (4 * q + 1)] = v.y;
+      vcol[64 * (4 * q + 2)] = v.z;
+      vcol[64 * (4 * q + 3)] = v.w;
+    }
+  }
+  auto vcell = [&](uint32_t i, uint32_t j) {
+    const uint32_t c = i * 16u + j;
+    return (float)((vcol[64 * (c >> 2)] >> (8u * (c & 3u))) & 0xFFu) * (1.0f / 255.0f);
+  };
+  const v3 en = V3(EP.normal[0], EP.normal[1], EP.normal[2]);
+  v3 acc = V3(0.0f, 0.0f, 0.0f);
+  // the wave's stack: entry 0 (the top) in the low 16 bits of w0
+  uint64_t w0 = min(EP.root, 0xFFFFu), w1 = 0, w2 = 0, w3 = 0;
+  uint32_t sp = 1;
+  uint32_t floor_at = 0xFFFFFFFFu;                     // this lane sits out pops at stack slots >= floor_at
+  while (sp > 0) {
+    const uint32_t ni = (uint32_t)(w0 & 0xFFFFu);
+    w0 = (w0 >> 16) | (w1 << 48);
+    w1 = (w1 >> 16) | (w2 << 48);
+    w2 = (w2 >> 16) | (w3 << 48);
+    w3 >>= 16;
+    sp -= 1;
+    const uint32_t at = sp;                            // the popped node's slot
+    const bool act = lane_on && at < floor_at;
+    if (act) floor_at = 0xFFFFFFFFu;
+    if (__ballot(act) == 0ull) continue;
+    if (ni >= n_nodes) continue;                      // a malformed tree: the node does not exist
+    const float4 *e = emit_nodes + 6 * (size_t)ni;
+    const float4 e0 = e[0], e1 = e[1], e2 = e[2], e3 = e[3];
+    const uint32_t clo = __float_as_uint(e2.w);
+    bool use;
+    if (clo == NO_CHILD || EP.mode == 0u) {
+      use = clo == NO_CHILD;
+    } else {
+      const v3 d = sub3(P, V3(e0.x, e0.y, e0.z));
+      use = dot3(d, d) > EP.theta * EP.theta * e0.w * e0.w;
+    }
+    if (__ballot(act && !use) != 0ull && at + 2u <= 16u) {
+      const uint64_t lo = min(clo, 0xFFFFu), hi = min(__float_as_uint(e3.w), 0xFFFFu);
+      w3 = (w3 << 32) | (w2 >> 32);
+      w2 = (w2 << 32) | (w1 >> 32);
+      w1 = (w1 << 32) | (w0 >> 32);
+      w0 = (w0 << 32) | (lo << 16) | hi;               // hi on top, as the per-lane walk pushes it last
+      sp += 2;
+    }
+    const bool contrib = act && use;
+    if (contrib) floor_at = at;                        // sit out this node's subtree
+    if (__ballot(contrib) == 0ull) continue;
+    if (contrib) {
+      const v3 d = sub3(V3(e0.x, e0.y, e0.z), P);
+      const float d2 = dot3(d, d);
+      if (d2 >= 1e-8f) {
+        const float sd = sqrtf(d2), isd = __builtin_amdgcn_rcpf(sd);
+        const v3 ww = V3(d.x * isd, d.y * isd, d.z * isd);
+        const v3 wi = xf_dir_inv(xf, ww);
+        const float vi = vgrid_lookup_f(vcell, wi);
+        if (vi > 1e-7f) {
+          const float fp = footprint(nd, detC, wi);
+          // intensity(e, -ww) (emissive.wgsl:123-130)
+          const v3 w = V3(-ww.x, -ww.y, -ww.z);
+          float s = 1.0f;
+          if (EP.one_sided != 0u || (__float_as_uint(e1.w) & 2u) != 0u) {
+            const float c = dot3(w, en);
+            if (EP.edge_soft > 0.0f) {
+              const float tt = clampf_((c - -EP.edge_soft) / (EP.edge_soft - -EP.edge_soft), 0.0f, 1.0f);
+              s = tt * tt * (3.0f - 2.0f * tt);
+            } else {
+              s = c > 0.0f ? 1.0f : 0.0f;
+            }
+          }
+          const float q = dot3(w, sym_mul(V3(e1.x, e1.y, e1.z), V3(e2.x, e2.y, e2.z), w));
+          const float ik = s * EP.scale * sqrtf(q > 0.0f ? q : 0.0f);
+          const float k = qdiv(fp, d2);
+          acc = add3(acc, V3(e3.x * ik * vi * k, e3.y * ik * vi * k, e3.z * ik * vi * k));
+        }
+      }
+    }
+  }
+  if (!lane_on) return;
   const float T = lamp_T ? lamp_T[g] : 1.0f;
   const float tk = T * iso_norm[g];
   v3 c = V3(C.x * acc.x * tk, C.y * acc.y * tk, C.z * acc.z * tk);

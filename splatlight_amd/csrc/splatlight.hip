@@ -1530,9 +1530,9 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
   }
   if (timers) SL_HIP(ctx, hipEventRecord(ctx->lev[1], st));
   if (emit_stack_need((const uint8_t *)d->emit_nodes, d->emit_node_count, EP.root) <= 16u)
-    sl_shade_emit<16><<<gblocks, 64, 0, st>>>(U, EP, sc->order, (const float4 *)sc->emit_nodes.p, d->emit_node_count,
-                                              sc->g[0], sc->g[1], sc->g[2], sc->g[3], sc->storage, sc->nodes,
-                                              sc->vgrid, sc->parts, lampT, sc->materials, sc->iso_norm, sc->colour);
+    sl_shade_emit_wave<<<gblocks, 64, 0, st>>>(U, EP, sc->order, (const float4 *)sc->emit_nodes.p, d->emit_node_count,
+                                               sc->g[0], sc->g[1], sc->g[2], sc->g[3], sc->storage, sc->nodes,
+                                               sc->vgrid, sc->parts, lampT, sc->materials, sc->iso_norm, sc->colour);
   else
     sl_shade_emit<48><<<gblocks, 64, 0, st>>>(U, EP, sc->order, (const float4 *)sc->emit_nodes.p, d->emit_node_count,
                                               sc->g[0], sc->g[1], sc->g[2], sc->g[3], sc->storage, sc->nodes,
