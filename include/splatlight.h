@@ -217,11 +217,49 @@ typedef struct sl_lighting_stats {
 } sl_lighting_stats;
 sl_status sl_lighting_stats_read(sl_context *ctx, sl_scene *scene, sl_lighting_stats *out);
 
+/* ---- sky and bounce through probes: Lighting.initBounce / encodeBounce (src/lighting.js:723-840, 1020-1103) ----
+ * The default path of runtime/wgsl/bounce.wgsl: no sender LOD, no near field, M39 matte receivers with SH2 probes
+ * (FrameUniforms.iso bits 0 and 1). The scene's tree (the reference's nodes.bin + kids.bin): 96 B Node records,
+ * the scene's leaves first (nodes [0, gaussian_count)), internal nodes after, children kids[child_lo, child_hi);
+ * each part's tree starts at Part.root (written with the part table). max_depth: the asset's walk.max_depth (node
+ * sums run for depths max_depth - 1 .. 1). The tree's records replace the scene's leaf array. */
+typedef struct sl_bounce_tree {
+  const void *nodes; uint32_t node_count; uint32_t node_stride;   /* 96 B */
+  const uint32_t *kids; uint32_t kid_count;
+  uint32_t max_depth;
+} sl_bounce_tree;
+/* probe_params: the 9632 B ProbeParams initBounce writes (runtime/wgsl/bounce.wgsl:23-56, src/lighting.js:776-797);
+ * its layout fixes the probe count. Allocates the bounce state (zeros) and makes the scene's bounce (lo_b, which
+ * sl_frame_encode adds with the RenderFrame's bounce gain) the device-computed one. */
+sl_status sl_bounce_init(sl_context *ctx, sl_scene *scene, const sl_bounce_tree *tree, const void *probe_params,
+                         size_t probe_params_len);
+#define SL_BOUNCE_TIMERS 1u       /* "bounce" and "probe_gather" in sl_timers_read */
+#define SL_BOUNCE_XFER 2u         /* run iso_xfer (the reference's isoStale: a forced refresh or a part moved) */
+#define SL_BOUNCE_LIGHT_STILL 4u  /* lightStill: only the eye-dependent read (bounce_sh) */
+/* One encodeBounce after the scene's sl_lighting_encode of the same frame (it reads that encode's FrameUniforms,
+ * Env, shadow T and the shading's emitter light): the senders' light toward the probes, the node sums and SH2, the
+ * probe gather (sky at escape), probe validity and SH2, the transfer when asked, and the bounce into lo_b. On the
+ * context's stream, so the next sl_frame_encode's colours see it. probe_params: this frame's ProbeParams (alpha,
+ * force_near, subset, phase as encodeBounce sets them; alpha must lie in (0, 1]). */
+sl_status sl_bounce_encode(sl_context *ctx, sl_scene *scene, const void *probe_params, size_t probe_params_len,
+                           uint32_t flags);
+/* Bounce state, for parity checks (after the context's stream finishes): */
+#define SL_BOUNCE_LO_DIR 1u       /* vec4 per Gaussian: light toward the probes */
+#define SL_BOUNCE_LO_B 2u         /* 3 vec4 per Gaussian: bounce toward the eye, the probes, the lanes */
+#define SL_BOUNCE_NODE_ACC 3u     /* 14 vec4 per node: (sum w L, sum w) */
+#define SL_BOUNCE_NODE_SH 4u      /* 14 u32 per node: SH2 rgb as 27 halves */
+#define SL_BOUNCE_PROBE_DATA 5u   /* 2 vec4 per (probe, cell): (rgb, first-hit part), (m1, m2, valid, near) */
+#define SL_BOUNCE_PROBE_SH 6u     /* 9 vec4 per probe */
+#define SL_BOUNCE_XFER_SH 7u      /* 3 vec4 per Gaussian: the transfer's SH2 */
+#define SL_BOUNCE_C0 8u           /* vec4 per Gaussian: the shading's emitter light before own emission */
+sl_status sl_bounce_read(sl_context *ctx, sl_scene *scene, uint32_t which, void *host, size_t bytes, size_t *written);
+
 /* Per-pass GPU milliseconds of the last frame encoded with SL_FRAME_TIMERS; names as in the reference's PASSES
  * (src/renderer.js:7-8) where one exists, then, after sl_lighting_encode with SL_LIGHT_TIMERS, its passes
  * "shadow", "shade", "sun_wait", "env_direct", "sunmap" (the sun-map chain, which runs on a second stream during
  * shadow and shade; sun_wait is what remains of it after shade), "lighting" (the whole encode) and the map-sum
- * kernels alone, "lsm_sum" and "sun_sum". Fills up to `cap` entries; returns the count in *n. */
+ * kernels alone, "lsm_sum" and "sun_sum"; after sl_bounce_encode with SL_BOUNCE_TIMERS, "bounce" (the whole
+ * encode) and "probe_gather". Fills up to `cap` entries; returns the count in *n. */
 sl_status sl_timers_read(sl_context *ctx, const char **names, float *ms, uint32_t cap, uint32_t *n);
 
 #ifdef __cplusplus

@@ -68,6 +68,21 @@ def lib():
             fn = getattr(L, name)
             fn.restype = None
             fn.argtypes = [vp] * n
+        L.or_f16_bits.restype = u32
+        L.or_f16_bits.argtypes = [f32]
+        for name, n in (("or_bounce_lo_dir", 11), ("or_node_leaf", 9), ("or_node_sh2", 3), ("or_probe_valid", 4),
+                        ("or_probe_sh2", 4), ("or_iso_xfer", 9), ("or_bounce_sh", 10)):
+            fn = getattr(L, name)
+            fn.restype = None
+            fn.argtypes = [vp] * n
+        L.or_node_up.restype = None
+        L.or_node_up.argtypes = [vp, vp, vp, vp, vp, u32, u32, u32, vp]
+        L.or_probe_gather.restype = None
+        L.or_probe_gather.argtypes = [vp] * 10 + [u32, u32]
+        L.or_probe_weights.restype = None
+        L.or_probe_weights.argtypes = [vp] * 6 + [f32, f32, f32, u32, vp, vp]
+        L.or_probe_pos.restype = None
+        L.or_probe_pos.argtypes = [vp, vp, vp, vp, u32, vp]
         _lib = L
     return _lib
 
@@ -247,3 +262,127 @@ def env_direct(U, ENV, gaussians, nodes, vgrid, parts, materials, iso_norm, env_
                         _p(_b(parts)), _p(_b(materials)), _p(np.ascontiguousarray(iso_norm, np.float32)),
                         _p(np.ascontiguousarray(env_T, np.float32)), _p(col))
     return col
+
+
+# ---- sky and bounce through probes ------------------------------------------------------------------------------
+
+def f16_bits(x: float) -> int:
+    return int(lib().or_f16_bits(x))
+
+
+def _f(a):
+    return np.ascontiguousarray(a, np.float32)
+
+
+def cells_table(dirs, omega):
+    """The 128 x 4 cell table (direction, solid angle) the bounce passes take."""
+    t = np.zeros((128, 4), np.float32)
+    t[:, :3] = dirs
+    t[:, 3] = omega
+    return t
+
+
+def bounce_lo_dir(U, ENV, gaussians, nodes, vgrid, parts, materials, iso_norm, env_T, c0, lo_dir):
+    out = _f(lo_dir).copy()
+    lib().or_bounce_lo_dir(_p(_b(U)), None if ENV is None else _p(_b(ENV)), _p(_b(gaussians)), _p(_b(nodes)),
+                           _p(np.ascontiguousarray(vgrid, np.uint8)), _p(_b(parts)), _p(_b(materials)),
+                           _p(_f(iso_norm)), None if env_T is None else _p(_f(env_T)), _p(_f(c0)), _p(out))
+    return out
+
+
+def node_leaf(U, PP, gaussians, nodes, vgrid, parts, lo_dir, lo_b, node_acc):
+    out = _f(node_acc).copy()
+    lib().or_node_leaf(_p(_b(U)), _p(_b(PP)), _p(_b(gaussians)), _p(_b(nodes)), _p(np.ascontiguousarray(vgrid, np.uint8)),
+                       _p(_b(parts)), _p(_f(lo_dir)), _p(_f(lo_b)), _p(out))
+    return out
+
+
+def node_up(U, PP, nodes, kids, up_nodes, levels, max_depth, node_acc):
+    """node_up for depths max_depth - 1 .. 1 (levels: (depth, offset, count) per depth)."""
+    out = _f(node_acc).copy()
+    k = np.ascontiguousarray(kids, np.uint32)
+    un = np.ascontiguousarray(up_nodes, np.uint32)
+    for d in range(max_depth - 1, 0, -1):
+        dep, off, cnt = levels[d]
+        lib().or_node_up(_p(_b(U)), _p(_b(PP)), _p(_b(nodes)), _p(k), _p(un), dep, off, cnt, _p(out))
+    return out
+
+
+def node_sh2(PP, node_acc, node_count):
+    out = np.zeros((node_count, 14), np.uint32)
+    lib().or_node_sh2(_p(_b(PP)), _p(_f(node_acc)), _p(out))
+    return out
+
+
+def probe_gather(U, PP, ENV, nodes, vgrid, parts, kids, node_sh, cells, probe_data, id0=0, id1=None):
+    out = _f(probe_data).copy()
+    n = out.size // 8
+    lib().or_probe_gather(_p(_b(U)), _p(_b(PP)), None if ENV is None else _p(_b(ENV)), _p(_b(nodes)),
+                          _p(np.ascontiguousarray(vgrid, np.uint8)), _p(_b(parts)), _p(np.ascontiguousarray(kids, np.uint32)),
+                          _p(np.ascontiguousarray(node_sh, np.uint32)), _p(_f(cells)), _p(out), id0, n if id1 is None else id1)
+    return out
+
+
+def probe_valid(U, PP, parts, probe_data):
+    out = _f(probe_data).copy()
+    lib().or_probe_valid(_p(_b(U)), _p(_b(PP)), _p(_b(parts)), _p(out))
+    return out
+
+
+def probe_sh2(PP, probe_data, cells, n_probes):
+    out = np.zeros((n_probes, 9, 4), np.float32)
+    lib().or_probe_sh2(_p(_b(PP)), _p(_f(probe_data)), _p(_f(cells)), _p(out))
+    return out
+
+
+def iso_xfer(U, PP, gaussians, nodes, vgrid, parts, iso_norm, cells, xfer):
+    out = _f(xfer).copy()
+    lib().or_iso_xfer(_p(_b(U)), _p(_b(PP)), _p(_b(gaussians)), _p(_b(nodes)), _p(np.ascontiguousarray(vgrid, np.uint8)),
+                      _p(_b(parts)), _p(_f(iso_norm)), _p(_f(cells)), _p(out))
+    return out
+
+
+def probe_weights(U, PP, nodes, parts, cells, probe_data, P, own):
+    pid = np.zeros(8, np.uint32)
+    w = np.zeros(8, np.float32)
+    lib().or_probe_weights(_p(_b(U)), _p(_b(PP)), _p(_b(nodes)), _p(_b(parts)), _p(_f(cells)), _p(_f(probe_data)),
+                           float(P[0]), float(P[1]), float(P[2]), own, _p(pid), _p(w))
+    return pid, w
+
+
+def bounce_sh(U, PP, gaussians, nodes, parts, cells, probe_data, probe_sh, xfer, lo_b):
+    out = _f(lo_b).copy()
+    lib().or_bounce_sh(_p(_b(U)), _p(_b(PP)), _p(_b(gaussians)), _p(_b(nodes)), _p(_b(parts)), _p(_f(cells)),
+                       _p(_f(probe_data)), _p(_f(probe_sh)), _p(_f(xfer)), _p(out))
+    return out
+
+
+def probe_pos(U, PP, nodes, parts, count):
+    out = np.zeros((count, 3), np.float32)
+    lib().or_probe_pos(_p(_b(U)), _p(_b(PP)), _p(_b(nodes)), _p(_b(parts)), count, _p(out))
+    return out
+
+
+def bounce_frame(st: dict, U, PP, ENV, gaussians, nodes, vgrid, parts, materials, iso_norm, env_T, c0, kids,
+                 up_nodes, levels, max_depth, cells, n_probes, run_xfer=True):
+    """One encodeBounce (src/lighting.js:1020-1103, default path) on the CPU. st holds the state carried across
+    frames (lo_dir, lo_b, node_acc, probe_data, xfer; zeros at first) and is updated in place."""
+    G = gaussians.shape[0]
+    N = nodes.shape[0]
+    if run_xfer:
+        st["xfer"] = iso_xfer(U, PP, gaussians, nodes, vgrid, parts, iso_norm, cells, st["xfer"])
+    st["lo_dir"] = bounce_lo_dir(U, ENV, gaussians, nodes, vgrid, parts, materials, iso_norm, env_T, c0, st["lo_dir"])
+    st["node_acc"] = node_leaf(U, PP, gaussians, nodes, vgrid, parts, st["lo_dir"], st["lo_b"], st["node_acc"])
+    st["node_acc"] = node_up(U, PP, nodes, kids, up_nodes, levels, max_depth, st["node_acc"])
+    st["node_sh"] = node_sh2(PP, st["node_acc"], N)
+    st["probe_data"] = probe_gather(U, PP, ENV, nodes, vgrid, parts, kids, st["node_sh"], cells, st["probe_data"])
+    st["probe_data"] = probe_valid(U, PP, parts, st["probe_data"])
+    st["probe_sh"] = probe_sh2(PP, st["probe_data"], cells, n_probes)
+    st["lo_b"] = bounce_sh(U, PP, gaussians, nodes, parts, cells, st["probe_data"], st["probe_sh"], st["xfer"], st["lo_b"])
+    return st
+
+
+def bounce_state(G, N, n_probes):
+    return {"lo_dir": np.zeros((G, 4), np.float32), "lo_b": np.zeros((G, 3, 4), np.float32),
+            "node_acc": np.zeros((N, 14, 4), np.float32), "probe_data": np.zeros((n_probes, 128, 2, 4), np.float32),
+            "xfer": np.zeros((G, 3, 4), np.float32)}

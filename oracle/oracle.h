@@ -87,6 +87,32 @@ void or_env_direct(const void *U, const void *ENV, const void *gaussians, const 
                    const void *parts, const void *materials, const float *iso_norm, const float *env_T,
                    float *colour);
 
+/* ---- sky and bounce through probes (bounce.wgsl, default path; bounce.c) ------------------------------------
+ * cells: 128 x (direction xyz, solid angle) floats; probe_data: 2 vec4 per (probe, cell) ray; node_acc: 14 vec4 per
+ * node; node_sh: 14 u32 (27 halves) per node; probe_sh: 9 vec4 per probe; xfer: 3 vec4 per Gaussian; lo_dir: vec4
+ * per Gaussian; lo_b: 3 vec4 per Gaussian (eye, send, lanes). */
+uint32_t or_f16_bits(float f);
+void or_bounce_lo_dir(const void *U, const void *ENV, const void *gaussians, const void *nodes, const uint8_t *vgrid,
+                      const void *parts, const void *materials, const float *iso_norm, const float *env_T,
+                      const float *c0, float *lo_dir);
+void or_node_leaf(const void *U, const void *PP, const void *gaussians, const void *nodes, const uint8_t *vgrid,
+                  const void *parts, const float *lo_dir, const float *lo_b, float *node_acc);
+void or_node_up(const void *U, const void *PP, const void *nodes, const uint32_t *kids, const uint32_t *up_nodes,
+                uint32_t depth, uint32_t r0, uint32_t count, float *node_acc);
+void or_node_sh2(const void *PP, const float *node_acc, uint32_t *node_sh);
+void or_probe_gather(const void *U, const void *PP, const void *ENV, const void *nodes, const uint8_t *vgrid,
+                     const void *parts, const uint32_t *kids, const uint32_t *node_sh, const float *cells,
+                     float *probe_data, uint32_t id0, uint32_t id1);
+void or_probe_valid(const void *U, const void *PP, const void *parts, float *probe_data);
+void or_probe_sh2(const void *PP, const float *probe_data, const float *cells, float *probe_sh);
+void or_iso_xfer(const void *U, const void *PP, const void *gaussians, const void *nodes, const uint8_t *vgrid,
+                 const void *parts, const float *iso_norm, const float *cells, float *xfer);
+void or_probe_weights(const void *U, const void *PP, const void *nodes, const void *parts, const float *cells,
+                      const float *probe_data, float px, float py, float pz, uint32_t own, uint32_t *pid, float *w);
+void or_bounce_sh(const void *U, const void *PP, const void *gaussians, const void *nodes, const void *parts,
+                  const float *cells, const float *probe_data, const float *probe_sh, const float *xfer, float *lo_b);
+void or_probe_pos(const void *U, const void *PP, const void *nodes, const void *parts, uint32_t count, float *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -298,7 +298,7 @@ sl_shade_emit(FrameUniformsDev U, EmitParamsDev EP, const uint32_t *__restrict__
               const float4 *__restrict__ g3, uint32_t st, const float4 *__restrict__ nodes,
               const uint8_t *__restrict__ vgrid, const float4 *__restrict__ parts, const float *__restrict__ lamp_T,
               const uint32_t *__restrict__ materials, const float *__restrict__ iso_norm,
-              float4 *__restrict__ colour) {
+              float4 *__restrict__ colour, float4 *__restrict__ c0_out) {
   const uint32_t t = blockIdx.x * 64u + threadIdx.x;
   if (t >= U.gaussian_count) return;
   const uint32_t g = order ? order[t] : t;
@@ -309,6 +309,7 @@ sl_shade_emit(FrameUniformsDev U, EmitParamsDev EP, const uint32_t *__restrict__
   const uint32_t mflags = U.material_count ? mat[5] : 0u;
   if (EP.count == 0u && (mflags & 1u) == 0u) {
     colour[g] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    if (c0_out) c0_out[g] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     return;
   }
   const Xf xf = load_xf(parts, __float_as_uint(S.w) & 0xFFFFu);
@@ -397,6 +398,7 @@ sl_shade_emit(FrameUniformsDev U, EmitParamsDev EP, const uint32_t *__restrict__
   const float T = lamp_T ? lamp_T[g] : 1.0f;
   const float tk = T * iso_norm[g];
   v3 c = V3(C.x * acc.x * tk, C.y * acc.y * tk, C.z * acc.z * tk);
+  if (c0_out) c0_out[g] = make_float4(c.x, c.y, c.z, 0.0f);
   if ((mflags & 1u) != 0u && ((mflags & 4u) == 0u || dot3(xf_dir_inv(xf, wo), Rq.c0) > 0.0f)) {
     c = add3(c, V3(unpack_half(mat[6] & 0xFFFFu), unpack_half(mat[6] >> 16), unpack_half(mat[7] & 0xFFFFu)));
   }
@@ -418,7 +420,7 @@ sl_shade_emit_wave(FrameUniformsDev U, EmitParamsDev EP, const uint32_t *__restr
                    const float4 *__restrict__ g3, uint32_t st, const float4 *__restrict__ nodes,
                    const uint8_t *__restrict__ vgrid, const float4 *__restrict__ parts,
                    const float *__restrict__ lamp_T, const uint32_t *__restrict__ materials,
-                   const float *__restrict__ iso_norm, float4 *__restrict__ colour) {
+                   const float *__restrict__ iso_norm, float4 *__restrict__ colour, float4 *__restrict__ c0_out) {
   const uint32_t t = blockIdx.x * 64u + threadIdx.x;
   const bool live = t < U.gaussian_count;
   const uint32_t g = live ? (order ? order[t] : t) : 0u;   // a lane past the count reads Gaussian 0 and sits out
@@ -430,6 +432,7 @@ sl_shade_emit_wave(FrameUniformsDev U, EmitParamsDev EP, const uint32_t *__restr
   bool lane_on = live;
   if (live && EP.count == 0u && (mflags & 1u) == 0u) {
     colour[g] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    if (c0_out) c0_out[g] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     lane_on = false;
   }
   const Xf xf = load_xf(parts, __float_as_uint(S.w) & 0xFFFFu);
@@ -529,6 +532,7 @@ sl_shade_emit_wave(FrameUniformsDev U, EmitParamsDev EP, const uint32_t *__restr
   const float T = lamp_T ? lamp_T[g] : 1.0f;
   const float tk = T * iso_norm[g];
   v3 c = V3(C.x * acc.x * tk, C.y * acc.y * tk, C.z * acc.z * tk);
+  if (c0_out) c0_out[g] = make_float4(c.x, c.y, c.z, 0.0f);
   if ((mflags & 1u) != 0u && ((mflags & 4u) == 0u || dot3(xf_dir_inv(xf, wo), Rq.c0) > 0.0f)) {
     c = add3(c, V3(unpack_half(mat[6] & 0xFFFFu), unpack_half(mat[6] >> 16), unpack_half(mat[7] & 0xFFFFu)));
   }

@@ -24,6 +24,7 @@
 #include "sort.hip"
 #include "tiles.hip"
 #include "lighting.hip"
+#include "bounce.hip"
 #pragma clang fp contract(fast)
 
 namespace {
@@ -96,6 +97,19 @@ struct sl_scene {
 
   uint32_t lsm_parts = 0, lsm_res = 0, lsm_layer_texels = 0;
   uint32_t sun_n = 0, sun_res = 0, sun_layer_texels = 0;
+  // sky and bounce through probes (sl_bounce_init): the whole tree's nodes (leaves first) replace the leaf array
+  // above; the state the bounce passes carry from frame to frame; the last lighting encode's uniforms
+  struct Bounce {
+    bool on = false, lit = false;
+    uint32_t node_count = 0, kid_count = 0, max_depth = 0, n_probes = 0;
+    uint32_t *kids = nullptr, *up_nodes = nullptr, *node_sh = nullptr;
+    std::vector<uint32_t> lv_off, lv_cnt;     // node_up's lists per depth
+    float4 *lo_dir = nullptr, *lo_b = nullptr, *node_acc = nullptr, *probe_data = nullptr, *probe_sh = nullptr;
+    float4 *xfer = nullptr, *c0 = nullptr, *cells = nullptr;
+    void *pp = nullptr;                        // ProbeParams
+    FrameUniformsDev U{};
+    EnvDev ENV{};
+  } bounce;
 };
 
 struct Workspace {
@@ -135,6 +149,8 @@ struct sl_context {
   hipEvent_t ev_comp = nullptr;   // before the composite when the geometry ran beside the lighting
   hipEvent_t lev[5] = {};         // lighting: shadow, shade, sun, env
   hipEvent_t mev[4] = {};         // the two map-sum launches (emitter map, sun maps): begin, end
+  hipEvent_t bev[4] = {};         // bounce: begin, before probe_gather, after it, end
+  bool bounce_timed = false;
   bool map_timed[2] = {false, false};
   bool timers_valid = false;
   bool light_timers_valid = false;
@@ -586,6 +602,8 @@ extern "C" sl_status sl_ctx_create(int device_ordinal, sl_context **out) {
     if ((e = hipEventCreate(&ctx->lev[k])) != hipSuccess) return bail(e);
   for (int k = 0; k < 4; k++)
     if ((e = hipEventCreate(&ctx->mev[k])) != hipSuccess) return bail(e);
+  for (int k = 0; k < 4; k++)
+    if ((e = hipEventCreate(&ctx->bev[k])) != hipSuccess) return bail(e);
   *out = ctx;
   return SL_OK;
 }
@@ -604,6 +622,8 @@ extern "C" void sl_ctx_destroy(sl_context *ctx) {
     if (ctx->lev[k]) (void)hipEventDestroy(ctx->lev[k]);
   for (int k = 0; k < 4; k++)
     if (ctx->mev[k]) (void)hipEventDestroy(ctx->mev[k]);
+  for (int k = 0; k < 4; k++)
+    if (ctx->bev[k]) (void)hipEventDestroy(ctx->bev[k]);
   if (ctx->counters_host) (void)hipHostFree(ctx->counters_host);
   if (ctx->view_counters_host) (void)hipHostFree(ctx->view_counters_host);
   if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
@@ -1529,14 +1549,19 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
     lampT = sc->lamp_T;
   }
   if (timers) SL_HIP(ctx, hipEventRecord(ctx->lev[1], st));
+  // with a bounce set up, the shading also keeps each Gaussian's emitter light before its own emission (the light
+  // direct_dirs_emit sends toward the probes) for the next sl_bounce_encode
+  float4 *c0_out = sc->bounce.on ? sc->bounce.c0 : nullptr;
   if (emit_stack_need((const uint8_t *)d->emit_nodes, d->emit_node_count, EP.root) <= 16u)
     sl_shade_emit_wave<<<gblocks, 64, 0, st>>>(U, EP, sc->order, (const float4 *)sc->emit_nodes.p, d->emit_node_count,
                                                sc->g[0], sc->g[1], sc->g[2], sc->g[3], sc->storage, sc->nodes,
-                                               sc->vgrid, sc->parts, lampT, sc->materials, sc->iso_norm, sc->colour);
+                                               sc->vgrid, sc->parts, lampT, sc->materials, sc->iso_norm, sc->colour,
+                                               c0_out);
   else
     sl_shade_emit<48><<<gblocks, 64, 0, st>>>(U, EP, sc->order, (const float4 *)sc->emit_nodes.p, d->emit_node_count,
                                               sc->g[0], sc->g[1], sc->g[2], sc->g[3], sc->storage, sc->nodes,
-                                              sc->vgrid, sc->parts, lampT, sc->materials, sc->iso_norm, sc->colour);
+                                              sc->vgrid, sc->parts, lampT, sc->materials, sc->iso_norm, sc->colour,
+                                              c0_out);
   if (timers) SL_HIP(ctx, hipEventRecord(ctx->lev[2], st));
   // distant lights: env_T (the side stream's, or 1 without sun maps), then env_direct
   if (d->env) {
@@ -1553,7 +1578,224 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
   }
   if (timers) SL_HIP(ctx, hipEventRecord(ctx->lev[4], st));
   ctx->light_timers_valid = timers;
+  sc->bounce.U = U;
+  sc->bounce.ENV = ENV;
+  sc->bounce.lit = true;
   return check_launch(ctx, "lighting kernels");
+}
+
+// ---- sky and bounce through probes (Lighting.initBounce / encodeBounce, src/lighting.js:723-840, 1020-1103) -----
+
+namespace {
+
+// Checks a ProbeParams block against the scene and tree: every probe index the passes form stays below the probe
+// count (lattice, then the movable parts' 8 each, then the static volumes), the lattice has 2+ probes per axis,
+// the gather blend is an EMA (0 < alpha <= 1) and near-field reflection is off. Returns the probe count or 0.
+uint32_t check_probe_params(sl_context *ctx, const sl_scene *sc, const ProbeParamsDev &PP, uint32_t node_count) {
+  auto bad = [&](const std::string &m) { fail(ctx, SL_ERR_INVALID, "ProbeParams: " + m); return 0u; };
+  if (PP.node_count != node_count) return bad("node_count does not match the tree");
+  if (PP.dims[0] < 2 || PP.dims[1] < 2 || PP.dims[2] < 2 || PP.dims[2] > 16 || (uint64_t)PP.dims[0] * PP.dims[1] > 1024)
+    return bad("lattice dims");
+  if (!(PP.alpha > 0.0f && PP.alpha <= 1.0f)) {
+    fail(ctx, SL_ERR_UNSUPPORTED, "ProbeParams: only the EMA gather blend (0 < alpha <= 1) is on this path");
+    return 0u;
+  }
+  if (PP.refl[0] != 0u) {
+    fail(ctx, SL_ERR_UNSUPPORTED, "ProbeParams: near-field reflection is not on this path");
+    return 0u;
+  }
+  if (PP.subset == 0 || PP.phase >= PP.subset) return bad("subset / phase");
+  const uint64_t lat = (uint64_t)PP.dims[0] * PP.dims[1] * PP.dims[2];
+  if (PP.vol_info[0] > PP.part_probes || PP.vol_info[0] % 8u) return bad("part probes");
+  for (uint32_t q = 0; q < 32; q++) {
+    const uint32_t *v = PP.vols[q];
+    if (v[1] == 0u) continue;
+    if (v[1] < 2 || v[2] < 2 || v[3] < 2 || q >= sc->part_count) return bad("volume dims");
+    if ((uint64_t)v[0] + (uint64_t)v[1] * v[2] * v[3] > PP.part_probes || v[0] < PP.vol_info[0]) return bad("volume range");
+  }
+  const uint64_t np = lat + PP.part_probes;
+  if (np * 128u * 2u > (1ull << 31)) return bad("too many probes");
+  return (uint32_t)np;
+}
+
+}  // namespace
+
+extern "C" sl_status sl_bounce_init(sl_context *ctx, sl_scene *sc, const sl_bounce_tree *t, const void *probe_params,
+                                    size_t probe_params_len) {
+  SL_LIVE(ctx);
+  if (!sc || !t || !t->nodes || !t->kids || !probe_params) return fail(ctx, SL_ERR_INVALID, "null argument");
+  if (!sc->nodes) return fail(ctx, SL_ERR_INVALID, "the bounce needs the scene's relighting records");
+  if (sc->bounce.on) return fail(ctx, SL_ERR_INVALID, "the scene's bounce is already set up");
+  if (t->node_stride != 96u || t->node_count < sc->count || t->node_count >= (1u << 31))
+    return fail(ctx, SL_ERR_LAYOUT, "tree nodes: 96 B records, leaves first, at least one per Gaussian");
+  if (probe_params_len != sizeof(ProbeParamsDev)) return fail(ctx, SL_ERR_LAYOUT, "ProbeParams is 9632 B");
+  const uint32_t N = t->node_count, G = sc->count;
+  const uint8_t *nb = (const uint8_t *)t->nodes;
+  for (uint32_t k = 0; k < t->kid_count; k++)
+    if (t->kids[k] >= N) return fail(ctx, SL_ERR_INVALID, "kids: node index out of range");
+  // node_up's lists: the internal nodes of each depth <= max_depth that belong to a part (src/lighting.js:762-772)
+  std::vector<std::vector<uint32_t>> by(t->max_depth + 1);
+  for (uint32_t i = G; i < N; i++) {
+    const uint32_t clo = rd_u32(nb + 96 * (size_t)i, 19), chi = rd_u32(nb + 96 * (size_t)i, 20);
+    const uint32_t part = rd_u32(nb + 96 * (size_t)i, 15), dep = rd_u32(nb + 96 * (size_t)i, 21);
+    if (clo > chi || chi > t->kid_count) return fail(ctx, SL_ERR_INVALID, "tree node: child range outside kids");
+    if (part != 0xFFFFFFFFu && dep <= t->max_depth) by[dep].push_back(i);
+  }
+  for (uint32_t q = 0; q < sc->part_count; q++) {
+    const uint8_t *pq = sc->parts_host.data() + 128 * (size_t)q;
+    if (rd_u32(pq, 30) > rd_u32(pq, 29) && rd_u32(pq, 28) >= N) return fail(ctx, SL_ERR_INVALID, "part root outside the tree");
+  }
+  ProbeParamsDev PP;
+  read_struct(probe_params, &PP);
+  const uint32_t NP = check_probe_params(ctx, sc, PP, N);
+  if (!NP) return ctx->poisoned != SL_OK ? ctx->poisoned : SL_ERR_INVALID;
+  SL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  sl_scene::Bounce &B = sc->bounce;
+  std::vector<uint32_t> up;
+  for (uint32_t d = 0; d <= t->max_depth; d++) {
+    B.lv_off.push_back((uint32_t)up.size());
+    B.lv_cnt.push_back((uint32_t)by[d].size());
+    up.insert(up.end(), by[d].begin(), by[d].end());
+  }
+  // the 128 cells' directions and solid angles, in double rounded once (bounce.wgsl cell_dir / cell_omega)
+  float cells[128][4];
+  for (int c = 0; c < 128; c++) {
+    const int i = c / 16, j = c % 16;
+    const double PI = 3.14159265358979323846;
+    const double el = 0.5 * PI - (i + 0.5) * PI / 8.0, az = -PI + (j + 0.5) * 2.0 * PI / 16.0;
+    const double top = 0.5 * PI - i * PI / 8.0;
+    cells[c][0] = (float)(cos(el) * cos(az));
+    cells[c][1] = (float)(cos(el) * sin(az));
+    cells[c][2] = (float)sin(el);
+    cells[c][3] = (float)((2.0 * PI / 16.0) * (sin(top) - sin(top - PI / 8.0)));
+  }
+  void *p = nullptr;
+  sl_status s;
+  struct Buf { void **dst; size_t bytes; };
+  void *nodes_new = nullptr;
+  const Buf bufs[] = {{&nodes_new, (size_t)N * 96}, {(void **)&B.kids, (size_t)std::max(t->kid_count, 1u) * 4},
+                      {(void **)&B.up_nodes, std::max<size_t>(up.size(), 1) * 4},
+                      {(void **)&B.node_acc, (size_t)N * 14 * 16}, {(void **)&B.node_sh, (size_t)N * 14 * 4},
+                      {(void **)&B.probe_data, (size_t)NP * 128 * 32}, {(void **)&B.probe_sh, (size_t)NP * 9 * 16},
+                      {(void **)&B.lo_dir, (size_t)G * 16}, {(void **)&B.lo_b, (size_t)G * 48},
+                      {(void **)&B.xfer, (size_t)G * 48}, {(void **)&B.c0, (size_t)G * 16},
+                      {(void **)&B.cells, sizeof(cells)}, {&B.pp, sizeof(ProbeParamsDev)}};
+  for (const Buf &b : bufs) {
+    if ((s = scene_alloc(ctx, sc, b.bytes, &p)) != SL_OK) return s;
+    *b.dst = p;
+    SL_HIP(ctx, hipMemsetAsync(p, 0, b.bytes, ctx->stream));
+  }
+  SL_HIP(ctx, hipMemcpyAsync(nodes_new, t->nodes, (size_t)N * 96, hipMemcpyHostToDevice, ctx->stream));
+  SL_HIP(ctx, hipMemcpyAsync(B.kids, t->kids, (size_t)t->kid_count * 4, hipMemcpyHostToDevice, ctx->stream));
+  if (!up.empty())
+    SL_HIP(ctx, hipMemcpyAsync(B.up_nodes, up.data(), up.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+  SL_HIP(ctx, hipMemcpyAsync(B.cells, cells, sizeof(cells), hipMemcpyHostToDevice, ctx->stream));
+  SL_HIP(ctx, hipMemcpyAsync(B.pp, &PP, sizeof(PP), hipMemcpyHostToDevice, ctx->stream));
+  SL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  // the whole tree replaces the scene's leaf array (its leaves are the same records)
+  auto it = std::find(sc->allocs.begin(), sc->allocs.end(), (void *)sc->nodes);
+  if (it != sc->allocs.end()) {
+    (void)hipFree(*it);
+    sc->allocs.erase(it);
+    sc->bytes -= (size_t)G * 96;
+  }
+  sc->nodes = (float4 *)nodes_new;
+  B.node_count = N;
+  B.kid_count = t->kid_count;
+  B.max_depth = t->max_depth;
+  B.n_probes = NP;
+  B.on = true;
+  B.lit = false;
+  sc->lo_b = B.lo_b;
+  return SL_OK;
+}
+
+extern "C" sl_status sl_bounce_encode(sl_context *ctx, sl_scene *sc, const void *probe_params, size_t len,
+                                      uint32_t flags) {
+  SL_LIVE(ctx);
+  if (!sc || !probe_params) return fail(ctx, SL_ERR_INVALID, "null argument");
+  sl_scene::Bounce &B = sc->bounce;
+  if (!B.on) return fail(ctx, SL_ERR_INVALID, "sl_bounce_init has not run for this scene");
+  if (!B.lit) return fail(ctx, SL_ERR_INVALID, "the bounce reads the scene's last lighting encode: none yet");
+  if (len != sizeof(ProbeParamsDev)) return fail(ctx, SL_ERR_LAYOUT, "ProbeParams is 9632 B");
+  ProbeParamsDev PP;
+  read_struct(probe_params, &PP);
+  const uint32_t NP = check_probe_params(ctx, sc, PP, B.node_count);
+  if (!NP) return ctx->poisoned != SL_OK ? ctx->poisoned : SL_ERR_INVALID;
+  if (NP != B.n_probes) return fail(ctx, SL_ERR_INVALID, "ProbeParams: the probe count changed since sl_bounce_init");
+  if ((B.U.iso & 3u) != 3u) return fail(ctx, SL_ERR_UNSUPPORTED, "the bounce path needs FrameUniforms.iso bits 0 and 1");
+  const bool timers = (flags & SL_BOUNCE_TIMERS) != 0u;
+  hipStream_t st = ctx->stream;
+  const FrameUniformsDev &U = B.U;
+  const ProbeParamsDev *PPd = (const ProbeParamsDev *)B.pp;
+  const uint32_t G = sc->count, gblocks = (G + 63) / 64;
+  if (timers) SL_HIP(ctx, hipEventRecord(ctx->bev[0], st));
+  SL_HIP(ctx, hipMemcpyAsync(B.pp, &PP, sizeof(PP), hipMemcpyHostToDevice, st));
+  if ((flags & SL_BOUNCE_LIGHT_STILL) == 0u) {
+    if (flags & SL_BOUNCE_XFER)
+      sl_iso_xfer<<<gblocks, 64, 0, st>>>(U, PPd, sc->g[0], sc->g[2], sc->storage, sc->nodes, sc->vgrid, sc->parts,
+                                           sc->iso_norm, B.cells, B.xfer);
+    const uint32_t nd = (U.iso & 2u) != 0u ? (G + 3) / 4 : G;
+    sl_bounce_lo_dir<<<(nd + 63) / 64, 64, 0, st>>>(U, B.ENV, sc->g[1], sc->g[2], sc->g[3], sc->storage, sc->nodes,
+                                                    sc->vgrid, sc->parts, sc->materials, sc->iso_norm,
+                                                    sc->env_T, B.c0, B.lo_dir);
+    sl_node_leaf<<<gblocks, 64, 0, st>>>(U, PPd, sc->g[2], sc->storage, sc->nodes, sc->vgrid, sc->parts, B.lo_dir,
+                                         B.lo_b, B.node_acc);
+    for (int d = (int)B.max_depth - 1; d >= 1; d--) {
+      const uint32_t cnt = B.lv_cnt[d];
+      if (cnt)
+        sl_node_up<<<(cnt + 63) / 64, 64, 0, st>>>(G, PPd, sc->nodes, B.kids, B.up_nodes, (uint32_t)d, B.lv_off[d],
+                                                    cnt, B.node_acc);
+    }
+    sl_node_sh2<<<(B.node_count + 63) / 64, 64, 0, st>>>(PPd, B.node_acc, B.node_sh);
+    if (timers) SL_HIP(ctx, hipEventRecord(ctx->bev[1], st));
+    const uint32_t rays = (NP * 128u + PP.subset - 1u) / PP.subset;
+    sl_probe_gather<<<(rays + 63) / 64, 64, 0, st>>>(U, PPd, B.ENV, sc->nodes, sc->vgrid, sc->parts, B.kids, B.node_sh,
+                                                     B.cells, B.probe_data, rays);
+    if (timers) SL_HIP(ctx, hipEventRecord(ctx->bev[2], st));
+    sl_probe_valid<<<(NP + 63) / 64, 64, 0, st>>>(U, PPd, sc->parts, B.probe_data);
+    sl_probe_sh2<<<(NP + 63) / 64, 64, 0, st>>>(PPd, B.probe_data, B.cells, B.probe_sh);
+  } else if (timers) {
+    SL_HIP(ctx, hipEventRecord(ctx->bev[1], st));
+    SL_HIP(ctx, hipEventRecord(ctx->bev[2], st));
+  }
+  const uint32_t ns = PP.sh_full ? G : (G + PP.subset - 1u) / PP.subset;
+  sl_bounce_sh<<<(ns + 63) / 64, 64, 0, st>>>(U, PPd, sc->g[0], sc->g[1], sc->g[2], sc->g[3], sc->storage, sc->nodes,
+                                              sc->parts, B.probe_data, B.probe_sh, B.xfer, B.lo_b);
+  if (timers) SL_HIP(ctx, hipEventRecord(ctx->bev[3], st));
+  ctx->bounce_timed = timers;
+  sc->lo_b = B.lo_b;
+  return check_launch(ctx, "bounce kernels");
+}
+
+extern "C" sl_status sl_bounce_read(sl_context *ctx, sl_scene *sc, uint32_t which, void *host, size_t bytes,
+                                    size_t *written) {
+  SL_LIVE(ctx);
+  if (!sc) return fail(ctx, SL_ERR_INVALID, "null argument");
+  sl_scene::Bounce &B = sc->bounce;
+  if (!B.on) return fail(ctx, SL_ERR_INVALID, "sl_bounce_init has not run for this scene");
+  const void *src = nullptr;
+  size_t n = 0;
+  switch (which) {
+    case SL_BOUNCE_LO_DIR: src = B.lo_dir; n = (size_t)sc->count * 16; break;
+    case SL_BOUNCE_LO_B: src = B.lo_b; n = (size_t)sc->count * 48; break;
+    case SL_BOUNCE_NODE_ACC: src = B.node_acc; n = (size_t)B.node_count * 14 * 16; break;
+    case SL_BOUNCE_NODE_SH: src = B.node_sh; n = (size_t)B.node_count * 14 * 4; break;
+    case SL_BOUNCE_PROBE_DATA: src = B.probe_data; n = (size_t)B.n_probes * 128 * 32; break;
+    case SL_BOUNCE_PROBE_SH: src = B.probe_sh; n = (size_t)B.n_probes * 9 * 16; break;
+    case SL_BOUNCE_XFER_SH: src = B.xfer; n = (size_t)sc->count * 48; break;
+    case SL_BOUNCE_C0: src = B.c0; n = (size_t)sc->count * 16; break;
+    default: return fail(ctx, SL_ERR_INVALID, "unknown bounce buffer");
+  }
+  if (!host) {   // the size alone
+    if (written) *written = n;
+    return SL_OK;
+  }
+  if (bytes < n) return fail(ctx, SL_ERR_INVALID, "host buffer too small");
+  SL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  SL_HIP(ctx, hipMemcpy(host, src, n, hipMemcpyDeviceToHost));
+  if (written) *written = n;
+  return SL_OK;
 }
 
 extern "C" sl_status sl_scene_read(sl_context *ctx, sl_scene *sc, uint32_t which, void *host, size_t bytes,
@@ -1646,6 +1888,18 @@ extern "C" sl_status sl_timers_read(sl_context *ctx, const char **names, float *
         if (ms) ms[k] = v;
         k++;
       }
+  }
+  if (ctx->bounce_timed) {
+    // the last sl_bounce_encode with SL_BOUNCE_TIMERS: the whole encode and its probe gather
+    SL_HIP(ctx, hipEventSynchronize(ctx->bev[3]));
+    const int ev[2][2] = {{0, 3}, {1, 2}};
+    static const char *const bnames[] = {"bounce", "probe_gather"};
+    for (int j = 0; j < 2 && k < cap; j++, k++) {
+      float v = 0.0f;
+      SL_HIP(ctx, hipEventElapsedTime(&v, ctx->bev[ev[j][0]], ctx->bev[ev[j][1]]));
+      if (names) names[k] = bnames[j];
+      if (ms) ms[k] = v;
+    }
   }
   if (n) *n = k;
   return SL_OK;

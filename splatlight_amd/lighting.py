@@ -365,7 +365,7 @@ def lighting_inputs(sc, eye=None, fwd=None, dirty_all=True, sun_res=None, frame=
            "desc": None}
     if sc.sun is not None:
         d = sc.sun["dir"]
-        out["ENV"] = env_bytes([(d, sc.sun["std"], sc.sun["E"], True)])
+        out["ENV"] = env_bytes([(d, sc.sun["std"], sc.sun["E"], True)], getattr(sc, "sky", None))
         SM, desc, nl, res = sun_plan(sc.parts, [(d, sc.sun["std"], True)], eye, fwd,
                                      res=sun_res or sc.sun.get("res", 512))
         if not dirty_all:
@@ -373,6 +373,8 @@ def lighting_inputs(sc, eye=None, fwd=None, dirty_all=True, sun_res=None, frame=
             du[:, 3] = 0
             desc = du.tobytes()
         out["SM"], out["desc"] = SM, desc
+    elif getattr(sc, "sky", None) is not None:
+        out["ENV"] = env_bytes([], sc.sky)          # a sky and no distant lights
     return out
 
 

@@ -32,7 +32,7 @@ EXPORTS = [
     "sl_lighting_iso_init", "sl_lighting_encode", "sl_scene_read", "sl_lighting_stats_read",
     "sl_batch_encode", "sl_batch_finish", "sl_ctx_set_tuning", "sl_ctx_get_tuning",
     "sl_server_create", "sl_server_submit", "sl_server_poll", "sl_server_info", "sl_server_error",
-    "sl_server_destroy",
+    "sl_server_destroy", "sl_bounce_init", "sl_bounce_encode", "sl_bounce_read",
 ]
 SL_BATCH_GRAPH = 2
 SCENE_COLOUR, SCENE_LAMP_T, SCENE_ENV_T, SCENE_ISO_NORM = 1, 2, 3, 4
@@ -91,6 +91,16 @@ class LightingDesc(C.Structure):
                 ("flags", C.c_uint32)]
 
 
+class BounceTree(C.Structure):
+    _fields_ = [("nodes", C.c_void_p), ("node_count", C.c_uint32), ("node_stride", C.c_uint32),
+                ("kids", C.c_void_p), ("kid_count", C.c_uint32), ("max_depth", C.c_uint32)]
+
+
+SL_BOUNCE_TIMERS, SL_BOUNCE_XFER, SL_BOUNCE_LIGHT_STILL = 1, 2, 4
+(SL_BOUNCE_LO_DIR, SL_BOUNCE_LO_B, SL_BOUNCE_NODE_ACC, SL_BOUNCE_NODE_SH, SL_BOUNCE_PROBE_DATA, SL_BOUNCE_PROBE_SH,
+ SL_BOUNCE_XFER_SH, SL_BOUNCE_C0) = range(1, 9)
+
+
 class SplatlightError(RuntimeError):
     def __init__(self, status: int, message: str):
         super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
@@ -147,6 +157,9 @@ def lib() -> C.CDLL:
         "sl_server_destroy": (None, [vp]),
         "sl_ctx_get_tuning": (i32, [vp, C.POINTER(Tuning)]),
         "sl_batch_finish": (i32, [vp, C.POINTER(FrameStats), u32]),
+        "sl_bounce_init": (i32, [vp, vp, C.POINTER(BounceTree), vp, sz]),
+        "sl_bounce_encode": (i32, [vp, vp, vp, sz, u32]),
+        "sl_bounce_read": (i32, [vp, vp, u32, vp, sz, C.POINTER(sz)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -302,6 +315,27 @@ class Scene:
         st = LightingStats()
         self.ctx.check(self.ctx.L.sl_lighting_stats_read(self.ctx.h, self.h, C.byref(st)))
         return {name: int(getattr(st, name)) for name, _ in LightingStats._fields_}
+
+    def bounce_init(self, nodes: np.ndarray, kids: np.ndarray, max_depth: int, probe_params: bytes):
+        """sl_bounce_init: the scene's tree (leaves first) and the probe layout's ProbeParams."""
+        nd = np.ascontiguousarray(nodes)
+        kd = np.ascontiguousarray(kids, np.uint32)
+        t = BounceTree(nd.ctypes.data_as(C.c_void_p), nd.nbytes // 96, 96, kd.ctypes.data_as(C.c_void_p), kd.size,
+                       max_depth)
+        pp = (C.c_uint8 * len(probe_params)).from_buffer_copy(probe_params)
+        self.ctx.check(self.ctx.L.sl_bounce_init(self.ctx.h, self.h, C.byref(t), C.cast(pp, C.c_void_p), len(probe_params)))
+
+    def bounce_encode(self, probe_params: bytes, flags: int = 0):
+        """sl_bounce_encode: one encodeBounce after this frame's sl_lighting_encode."""
+        pp = (C.c_uint8 * len(probe_params)).from_buffer_copy(probe_params)
+        self.ctx.check(self.ctx.L.sl_bounce_encode(self.ctx.h, self.h, C.cast(pp, C.c_void_p), len(probe_params), flags))
+
+    def bounce_read(self, which: int, dtype=np.float32) -> np.ndarray:
+        n = C.c_size_t()
+        self.ctx.check(self.ctx.L.sl_bounce_read(self.ctx.h, self.h, which, None, 0, C.byref(n)))
+        out = np.empty(n.value // np.dtype(dtype).itemsize, dtype)
+        self.ctx.check(self.ctx.L.sl_bounce_read(self.ctx.h, self.h, which, _ptr(out), out.nbytes, C.byref(n)))
+        return out
 
     def read(self, which: int, dtype=np.float32) -> np.ndarray:
         n = C.c_size_t()

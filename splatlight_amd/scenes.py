@@ -49,6 +49,7 @@ class Scene:
     height: int
     far: float
     sun: dict | None = None          # {"dir": unit vector toward the sun, "E": rgb, "std": rad, "res": int}
+    sky: np.ndarray | None = None    # 16 x rgb real SH of the sky radiance (Env.sh), seen through the probes
     one_sided: bool = False          # emitters emit only toward +short axis (EmitParams.one_sided)
 
     @property
@@ -135,7 +136,7 @@ def _parts(g, part_of):
         lo, hi = mu.min(0), mu.max(0)
         c = 0.5 * (lo + hi)
         r = float(np.linalg.norm(mu - c, axis=1).max() + 3 * g["scale"][idx].max())
-        out.append(R.identity_part(int(idx[0]), int(idx[-1]) + 1, bound=(*c, r)))
+        out.append(R.identity_part(int(idx[0]), int(idx[-1]) + 1, bound=(*c, r), flags=1))   # static
     return np.concatenate(out)
 
 
@@ -211,8 +212,12 @@ def hill(n: int = 1_000_000, seed: int = 1, width: int = 1920, height: int = 108
     el, az = math.radians(35.0), math.radians(30.0)
     sun = {"dir": (math.cos(el) * math.cos(az), math.cos(el) * math.sin(az), math.sin(el)),
            "E": (3.0, 2.9, 2.7), "std": math.radians(0.27), "res": shadow_res}
+    # sky: radiance A + B z, brighter toward the zenith, as real SH bands 0-1
+    sky = np.zeros((16, 3))
+    sky[0] = np.array([0.35, 0.45, 0.60]) / 0.282095
+    sky[2] = np.array([0.15, 0.20, 0.30]) / 0.488603
     return Scene("hill", g, parts, _materials(), em, (0.0, -15.0, 4.0), (0.0, 0.0, 1.5), 40.0,
-                 width, height, _far(g), sun)
+                 width, height, _far(g), sun, sky=sky)
 
 
 def leaf_nodes(g: np.ndarray) -> np.ndarray:
