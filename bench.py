@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--serving", type=int, default=1, help="also measure the served form (Poisson arrivals)")
     ap.add_argument("--rate", type=float, default=1000.0, help="serving arrival rate, requests per second")
     ap.add_argument("--storage", type=int, default=1, help="also measure f16 / bf16 storage at 4K over 4M Gaussians")
+    ap.add_argument("--bounce", type=int, default=1, help="also measure the relit frame with sky and bounce through "
+                                                          "probes (lighting, bounce, frame at bounce gain 1)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB, 5 times (HBM counter calibration)")
     return ap.parse_args()
@@ -295,6 +297,61 @@ def storage_leg(N, L, scenes, steps=5):
     return out
 
 
+def bounce_leg(N, L, scenes, sc, steps=8, warmup=2, half=True):
+    """The hill frame with the sky and bounce through probes each frame (Lighting.encodeBounce after the lighting,
+    the frame adding gain x bounce): the bounce's GPU time, its probe gather, and the whole relit frame's rate."""
+    from splatlight_amd import bounce as BN
+    from splatlight_amd import tree
+    t0 = time.perf_counter()
+    tr = tree.scene_tree(sc)
+    build_s = time.perf_counter() - t0
+    W, H = sc.width, sc.height
+    fmt = N.SL_OUT_RGBA16F if half else N.SL_OUT_RGBA32F
+    ctx = N.Context(0)
+    scene = N.Scene(ctx, sc.gaussians, sc.parts, emitters=sc.emitters, materials=sc.materials,
+                    nodes=tr.nodes[:sc.count], vgrid=scenes.vgrid_open(sc.count))
+    b = BN.Bounce(scene, sc, tr)
+    fb = sc.frame_bytes(bounce_gain=1.0)
+    out = ctx.alloc(W * H * (8 if half else 16))
+    inp = L.lighting_inputs(sc, dirty_all=True)
+    bt, gt, ft = [], [], []
+
+    def frame(timed):
+        scene.light(inp["U"], inp["EP"], inp["emit_nodes"], inp["LS"], inp["ENV"], inp["SM"], inp["desc"])
+        b.encode(timers=timed)
+        N.frame_encode(ctx, scene, fb, W, H, out, fmt)
+        N.frame_finish(ctx)
+        if timed:
+            tm = N.timers(ctx)
+            bt.append(tm["bounce"])
+            gt.append(tm["probe_gather"])
+
+    for _ in range(warmup):
+        frame(False)
+    ctx.sync()
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        t = time.perf_counter()
+        frame(False)
+        ft.append((time.perf_counter() - t) * 1e3)
+    ctx.sync()
+    total = time.perf_counter() - t1
+    for _ in range(3):
+        frame(True)
+    rays = b.n_probes * 128
+    res = {"workload": f"hill {sc.count} Gaussians, tree of {tr.count} nodes, {b.n_probes} probes x 128 rays, "
+                       f"sky + sun + 64 emitters, {W}x{H}",
+           "relit_frame_ms": round(total / steps * 1e3, 4), "mpix_s": round(W * H * steps / total / 1e6, 3),
+           "frame_ms_p50": round(float(np.percentile(ft, 50)), 4),
+           "bounce_ms": round(float(np.mean(bt)), 4), "probe_gather_ms": round(float(np.mean(gt)), 4),
+           "probe_rays_per_s": round(rays / (float(np.mean(gt)) * 1e-3), 1), "probes": b.n_probes,
+           "nodes": tr.count, "tree_build_s": round(build_s, 2)}
+    ctx.free(out)
+    scene.destroy()
+    ctx.close()
+    return res
+
+
 def kernel_bytes(passes, P, T, target_bytes, ls):
     """Per-launch algorithmic HBM bytes and HIP-event milliseconds of the heaviest kernel launches of a relit frame.
 
@@ -533,6 +590,8 @@ def main():
     ctx.free(out)
     scene.destroy()
     ctx.close()
+    if a.bounce:
+        res["bounce"] = bounce_leg(N, L, scenes, sc)
     if a.batched:
         res["batched"] = batched_leg(N, L, scenes)
     if a.serving:
