@@ -246,12 +246,14 @@ sl_status sl_bounce_encode(sl_context *ctx, sl_scene *scene, const void *probe_p
 /* Bounce state, for parity checks (after the context's stream finishes): */
 #define SL_BOUNCE_LO_DIR 1u       /* vec4 per Gaussian: light toward the probes */
 #define SL_BOUNCE_LO_B 2u         /* 3 vec4 per Gaussian: bounce toward the eye, the probes, the lanes */
-#define SL_BOUNCE_NODE_ACC 3u     /* 14 vec4 per node: (sum w L, sum w) */
+#define SL_BOUNCE_NODE_ACC 3u     /* 14 vec4 per node: (sum w L, sum w); internal nodes only (leaves' rows zero:
+                                     they are formed from LEAF_W and the light where they are read) */
 #define SL_BOUNCE_NODE_SH 4u      /* 14 u32 per node: SH2 rgb as 27 halves */
 #define SL_BOUNCE_PROBE_DATA 5u   /* 2 vec4 per (probe, cell): (rgb, first-hit part), (m1, m2, valid, near) */
 #define SL_BOUNCE_PROBE_SH 6u     /* 9 vec4 per probe */
 #define SL_BOUNCE_XFER_SH 7u      /* 3 vec4 per Gaussian: the transfer's SH2 */
 #define SL_BOUNCE_C0 8u           /* vec4 per Gaussian: the shading's emitter light before own emission */
+#define SL_BOUNCE_LEAF_W 9u       /* 14 floats per Gaussian: its weights A V toward the 14 directions */
 sl_status sl_bounce_read(sl_context *ctx, sl_scene *scene, uint32_t which, void *host, size_t bytes, size_t *written);
 
 /* Per-pass GPU milliseconds of the last frame encoded with SL_FRAME_TIMERS; names as in the reference's PASSES

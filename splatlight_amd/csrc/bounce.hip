@@ -227,66 +227,15 @@ sl_bounce_lo_dir(FrameUniformsDev U, EnvDev ENV, const float4 *__restrict__ g1, 
 }
 
 // ---- node emitters (bounce.wgsl:1166-1233) ------------------------------------------------------------------------
-// node_leaf: per leaf, the 14 directions' weights A(D_k) V(D_k) and its light toward the probes (this frame's
-// direct light plus last frame's bounce), node_acc[g][k] = (L w, w). node_acc: 14 float4 per node.
-extern "C" __global__ void __launch_bounds__(64)
-sl_node_leaf(FrameUniformsDev U, const ProbeParamsDev *__restrict__ PP, const float4 *__restrict__ g2, uint32_t st,
-             const float4 *__restrict__ nodes, const uint8_t *__restrict__ vgrid, const float4 *__restrict__ parts,
-             const float4 *__restrict__ lo_dir, const float4 *__restrict__ lo_b, float4 *__restrict__ node_acc) {
-  const uint32_t g = blockIdx.x * 64u + threadIdx.x;
-  if (g >= U.gaussian_count) return;
-  const float4 S = sl_scale(g2, st, g);
-  const uint32_t own = __float_as_uint(S.w) & 0xFFFFu;
-  const NodeL nd = load_node(nodes, g);
-  const Xf xf = load_xf(parts, own);
-  const float s3 = s3_of(S);
-  const uint8_t *vg = vgrid + 128 * (size_t)g;
-  const bool dark = PP->refl[3] != 0u && own == PP->refl[3] - 1u;
-  const float4 a = lo_dir[g], b = lo_b[3 * (size_t)g + 1];
-  const v3 L = dark ? V3(0.0f, 0.0f, 0.0f) : V3(a.x + b.x, a.y + b.y, a.z + b.z);
-  float4 *o = node_acc + 14 * (size_t)g;
-  for (int k = 0; k < 14; k++) {
-    const v3 wo = xf_dir_inv(xf, V3(PP->dirs[k][0], PP->dirs[k][1], PP->dirs[k][2]));
-    const float w = footprint(nd, s3 * s3, wo) * vgrid_lookup(vg, wo);
-    o[k] = make_float4(L.x * w, L.y * w, L.z * w, w);
-  }
-}
-
-// node_up for one depth: up_nodes[r0, r0 + count) are that depth's internal nodes; each sums its children's 14
-// entries in child order (src/lighting.js:762-772)
-extern "C" __global__ void __launch_bounds__(64)
-sl_node_up(uint32_t G, const ProbeParamsDev *__restrict__ PP, const float4 *__restrict__ nodes,
-           const uint32_t *__restrict__ kids, const uint32_t *__restrict__ up_nodes, uint32_t depth, uint32_t r0,
-           uint32_t count, float4 *__restrict__ node_acc) {
-  const uint32_t t = blockIdx.x * 64u + threadIdx.x;
-  if (t >= count) return;
-  const uint32_t i = up_nodes[r0 + t];
-  if (i < G || i >= PP->node_count) return;
-  const float4 *n = nodes + 6 * (size_t)i;
-  const uint32_t part = __float_as_uint(n[3].w), clo = __float_as_uint(n[4].w);
-  const float4 n5 = n[5];
-  const uint32_t chi = __float_as_uint(n5.x), dep = __float_as_uint(n5.y);
-  if (dep != depth || part == NONE) return;
-  for (int k = 0; k < 14; k++) {
-    float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    for (uint32_t j = clo; j < chi; j++) {
-      const float4 a = node_acc[14 * (size_t)kids[j] + k];
-      s = make_float4(s.x + a.x, s.y + a.y, s.z + a.z, s.w + a.w);
-    }
-    node_acc[14 * (size_t)i + k] = s;
-  }
-}
-
-// node_sh2: per node, the 14 directions' mean light fitted to SH2 (PP.pinv), 27 halves in 14 words
-extern "C" __global__ void __launch_bounds__(64)
-sl_node_sh2(const ProbeParamsDev *__restrict__ PP, const float4 *__restrict__ node_acc, uint32_t *__restrict__ node_sh) {
-  const uint32_t i = blockIdx.x * 64u + threadIdx.x;
-  if (i >= PP->node_count) return;
+namespace bn {
+// node_sh2_one (bounce.wgsl:1217-1233): a node's 14 sums (sum w L, sum w) -> the mean light per direction -> SH2 by
+// PP.pinv -> 27 halves (round to nearest even) in 14 words
+__device__ __forceinline__ void sh2_pack(const ProbeParamsDev *PP, const float4 (&a)[14], uint32_t *o) {
   v3 L[14];
+#pragma unroll
   for (int k = 0; k < 14; k++) {
-    const float4 a = node_acc[14 * (size_t)i + k];
-    const float d = fmaxf(a.w, 1e-12f);
-    L[k] = V3(a.x / d, a.y / d, a.z / d);
+    const float d = fmaxf(a[k].w, 1e-12f);
+    L[k] = V3(a[k].x / d, a[k].y / d, a[k].z / d);
   }
   float h[28];
   h[27] = 0.0f;
@@ -298,12 +247,106 @@ sl_node_sh2(const ProbeParamsDev *__restrict__ PP, const float4 *__restrict__ no
     for (int k = 0; k < 14; k++) c = add3(c, mul3s(L[k], pinv[m * 14 + k]));
     h[3 * m] = c.x; h[3 * m + 1] = c.y; h[3 * m + 2] = c.z;
   }
-  uint32_t *o = node_sh + 14 * (size_t)i;
 #pragma unroll
   for (int j = 0; j < 14; j++) {
     const uint32_t lo = __half_as_ushort(__float2half_rn(h[2 * j])), hi = __half_as_ushort(__float2half_rn(h[2 * j + 1]));
     o[j] = lo | (hi << 16);
   }
+}
+}  // namespace bn
+
+// The leaves' weights toward the 14 directions, A(D_k) V(D_k) (node_leaf's w): they change only with the part's
+// pose, so they are kept per leaf (14 floats) and rebuilt when the part table changes.
+extern "C" __global__ void __launch_bounds__(64)
+sl_leaf_weights(uint32_t G, const ProbeParamsDev *__restrict__ PP, const float4 *__restrict__ g2, uint32_t st,
+                const float4 *__restrict__ nodes, const uint8_t *__restrict__ vgrid, const float4 *__restrict__ parts,
+                float *__restrict__ wcache) {
+  const uint32_t g = blockIdx.x * 64u + threadIdx.x;
+  if (g >= G) return;
+  const float4 S = sl_scale(g2, st, g);
+  const NodeL nd = load_node(nodes, g);
+  const Xf xf = load_xf(parts, __float_as_uint(S.w) & 0xFFFFu);
+  const float s3 = s3_of(S);
+  const uint8_t *vg = vgrid + 128 * (size_t)g;
+  for (int k = 0; k < 14; k++) {
+    const v3 wo = xf_dir_inv(xf, V3(PP->dirs[k][0], PP->dirs[k][1], PP->dirs[k][2]));
+    wcache[14 * (size_t)g + k] = footprint(nd, s3 * s3, wo) * vgrid_lookup(vg, wo);
+  }
+}
+
+// The node sums without leaf rows: node_acc holds the internal nodes' 14 sums direction-major ([k][node], so a
+// depth's consecutive nodes read their children's rows coalesced); a leaf's entries (L w_k, w_k) are formed where
+// they are read, from its light toward the probes L (lo_dir + last frame's bounce toward the probes; nothing for the
+// PP.refl.w part) and its cached weights. The same float operations in the same order as node_leaf / node_up /
+// node_sh2, so the same bits.
+namespace bn {
+__device__ __forceinline__ v3 leaf_light(const ProbeParamsDev *PP, const float4 *g2, uint32_t st, const float4 *lo_dir,
+                                         const float4 *lo_b, uint32_t c) {
+  const uint32_t dark = PP->refl[3];
+  const uint32_t own = __float_as_uint(sl_scale(g2, st, c).w) & 0xFFFFu;
+  if (dark != 0u && own == dark - 1u) return V3(0.0f, 0.0f, 0.0f);
+  const float4 a = lo_dir[c], b = lo_b[3 * (size_t)c + 1];
+  return V3(a.x + b.x, a.y + b.y, a.z + b.z);
+}
+}  // namespace bn
+
+// the leaves' SH2 (node_sh2 of node_leaf's sums), one lane per leaf
+extern "C" __global__ void __launch_bounds__(64)
+sl_leaf_sh(uint32_t G, const ProbeParamsDev *__restrict__ PP, const float4 *__restrict__ g2, uint32_t st,
+           const float *__restrict__ wcache, const float4 *__restrict__ lo_dir, const float4 *__restrict__ lo_b,
+           uint32_t *__restrict__ node_sh) {
+  const uint32_t g = blockIdx.x * 64u + threadIdx.x;
+  if (g >= G) return;
+  const v3 L = leaf_light(PP, g2, st, lo_dir, lo_b, g);
+  float4 s[14];
+#pragma unroll
+  for (int k = 0; k < 14; k++) {
+    const float w = wcache[14 * (size_t)g + k];
+    s[k] = make_float4(L.x * w, L.y * w, L.z * w, w);
+  }
+  sh2_pack(PP, s, node_sh + 14 * (size_t)g);
+}
+
+// node_up + node_sh2 for one depth (up_nodes[r0, r0 + count)): each node sums its children in child order and
+// stores its sums and SH2
+extern "C" __global__ void __launch_bounds__(64)
+sl_node_up_sh(uint32_t G, uint32_t N, const ProbeParamsDev *__restrict__ PP, const float4 *__restrict__ g2, uint32_t st,
+              const float4 *__restrict__ nodes, const uint32_t *__restrict__ kids, const uint32_t *__restrict__ up_nodes,
+              uint32_t depth, uint32_t r0, uint32_t count, const float *__restrict__ wcache,
+              const float4 *__restrict__ lo_dir, const float4 *__restrict__ lo_b, float4 *__restrict__ node_acc,
+              uint32_t *__restrict__ node_sh) {
+  const uint32_t t = blockIdx.x * 64u + threadIdx.x;
+  if (t >= count) return;
+  const uint32_t i = up_nodes[r0 + t];
+  if (i < G || i >= N) return;
+  const float4 *n = nodes + 6 * (size_t)i;
+  const uint32_t part = __float_as_uint(n[3].w), clo = __float_as_uint(n[4].w);
+  const float4 n5 = n[5];
+  const uint32_t chi = __float_as_uint(n5.x), dep = __float_as_uint(n5.y);
+  if (dep != depth || part == NONE) return;
+  float4 s[14];
+#pragma unroll
+  for (int k = 0; k < 14; k++) s[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  for (uint32_t j = clo; j < chi; j++) {
+    const uint32_t c = kids[j];
+    if (c < G) {
+      const v3 Lc = leaf_light(PP, g2, st, lo_dir, lo_b, c);
+#pragma unroll
+      for (int k = 0; k < 14; k++) {
+        const float w = wcache[14 * (size_t)c + k];
+        s[k] = make_float4(s[k].x + Lc.x * w, s[k].y + Lc.y * w, s[k].z + Lc.z * w, s[k].w + w);
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 14; k++) {
+        const float4 a = node_acc[(size_t)k * N + c];
+        s[k] = make_float4(s[k].x + a.x, s[k].y + a.y, s[k].z + a.z, s[k].w + a.w);
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 14; k++) node_acc[(size_t)k * N + i] = s[k];
+  sh2_pack(PP, s, node_sh + 14 * (size_t)i);
 }
 
 // ---- probe_gather (bounce.wgsl:1255-1415) ---------------------------------------------------------------------------
@@ -314,18 +357,23 @@ sl_node_sh2(const ProbeParamsDev *__restrict__ PP, const float4 *__restrict__ no
 // the farthest hit into its last entry. The list is then composited front to back with each node's SH2 light, the
 // sky added for what escapes to tmax. No lamps end a ray (the scenes carry emissive Gaussians, not rect lamps):
 // t1 = tmax. alpha in (0, 1]: the new gather blended into the old (1: taken as is).
-extern "C" __global__ void __launch_bounds__(64)
+// SL: the LDS stack's entries per lane. The reference's walk keeps 64 (a push past them is dropped); the host takes
+// 32 when the tree cannot fill more than 32 (no push is ever dropped then either), 64 otherwise. A wave takes 64
+// directions of one probe: they walk the same part of the tree, which keeps its nodes in cache (a wave of one
+// direction over 64 probes walked alike but ran 20 % slower).
+template <uint32_t SL>
+__global__ void __launch_bounds__(64)
 sl_probe_gather(FrameUniformsDev U, const ProbeParamsDev *__restrict__ PP, EnvDev ENV, const float4 *__restrict__ nodes,
                 const uint8_t *__restrict__ vgrid, const float4 *__restrict__ parts, const uint32_t *__restrict__ kids,
                 const uint32_t *__restrict__ node_sh, const float4 *__restrict__ cells, float4 *__restrict__ probe_data,
                 uint32_t n_threads) {
-  __shared__ uint32_t s_stack[STACK][64];
+  __shared__ uint32_t s_stack[SL][64];
   const uint32_t lane = threadIdx.x;
-  const uint32_t gid = blockIdx.x * 64u + lane;
-  if (gid >= n_threads) return;
+  const uint32_t t = blockIdx.x * 64u + lane;
+  if (t >= n_threads) return;
   const uint32_t G = U.gaussian_count;
   const uint32_t np = lat_count(PP);
-  const uint32_t id = gid * PP->subset + PP->phase;
+  const uint32_t id = t * PP->subset + PP->phase;
   if (id >= (np + PP->part_probes) * NCELL) return;
   const uint32_t p = id / NCELL, cell = id % NCELL;
   uint32_t skip = NONE;
@@ -406,7 +454,7 @@ sl_probe_gather(FrameUniformsDev U, const ProbeParamsDev *__restrict__ PP, EnvDe
       } else {
         const uint32_t clo = __float_as_uint(n[4].w), chi = __float_as_uint(n[5].x);
         for (uint32_t j = clo; j < chi; j++)
-          if (sp < STACK) { s_stack[sp][lane] = kids[j]; sp += 1u; }
+          if (sp < SL) { s_stack[sp][lane] = kids[j]; sp += 1u; }
       }
     }
   }

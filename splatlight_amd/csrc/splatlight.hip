@@ -75,6 +75,7 @@ struct Grow {
 
 struct sl_scene {
   uint32_t count = 0, part_count = 0, n_emit = 0, material_count = 0, storage = SL_STORAGE_F32;
+  uint32_t parts_version = 0;       // counts sl_scene_write_parts calls (what depends on the poses rebuilds)
   float4 *g[4] = {nullptr, nullptr, nullptr, nullptr};
   float4 *e[4] = {nullptr, nullptr, nullptr, nullptr};
   float4 *parts = nullptr;
@@ -107,6 +108,11 @@ struct sl_scene {
     float4 *lo_dir = nullptr, *lo_b = nullptr, *node_acc = nullptr, *probe_data = nullptr, *probe_sh = nullptr;
     float4 *xfer = nullptr, *c0 = nullptr, *cells = nullptr;
     void *pp = nullptr;                        // ProbeParams
+    // the leaves' weights toward the 14 directions and the part-table version they were built for; node_acc
+    // holds the internal nodes' sums direction-major ([14][node_count])
+    float *wcache = nullptr;
+    uint32_t w_version = 0xFFFFFFFFu;
+    uint32_t walk_stack = 0;   // the most entries the gather's depth-first walk can hold on this tree
     FrameUniformsDev U{};
     EnvDev ENV{};
   } bounce;
@@ -952,6 +958,7 @@ extern "C" sl_status sl_scene_write_parts(sl_context *ctx, sl_scene *sc, const v
   memcpy(sc->parts_host.data(), bytes, n);
   SL_HIP(ctx, hipMemcpyAsync(sc->parts, sc->parts_host.data(), n, hipMemcpyHostToDevice, ctx->stream));
   SL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  sc->parts_version++;
   return SL_OK;
 }
 
@@ -1649,8 +1656,29 @@ extern "C" sl_status sl_bounce_init(sl_context *ctx, sl_scene *sc, const sl_boun
   read_struct(probe_params, &PP);
   const uint32_t NP = check_probe_params(ctx, sc, PP, N);
   if (!NP) return ctx->poisoned != SL_OK ? ctx->poisoned : SL_ERR_INVALID;
+  // the gather's stack bound: a node popped with P entries on the stack (itself included) pushes its k kids,
+  // holding P - 1 + k; its i-th kid (pushed i-th, popped after the later ones) is popped with P + i
+  uint32_t walk_stack = 1;
+  {
+    std::vector<std::pair<uint32_t, uint32_t>> todo;   // (node, entries when popped)
+    std::vector<uint8_t> seen(N, 0);
+    for (uint32_t q = 0; q < sc->part_count; q++) {
+      const uint8_t *pq = sc->parts_host.data() + 128 * (size_t)q;
+      if (rd_u32(pq, 30) > rd_u32(pq, 29)) todo.push_back({rd_u32(pq, 28), 1u});
+    }
+    while (!todo.empty()) {
+      const auto [i, P] = todo.back();
+      todo.pop_back();
+      if (i < G || seen[i]) continue;
+      seen[i] = 1;
+      const uint32_t clo = rd_u32(nb + 96 * (size_t)i, 19), chi = rd_u32(nb + 96 * (size_t)i, 20);
+      walk_stack = std::max(walk_stack, P - 1u + (chi - clo));
+      for (uint32_t j = clo; j < chi; j++) todo.push_back({t->kids[j], P + (j - clo)});
+    }
+  }
   SL_HIP(ctx, hipStreamSynchronize(ctx->stream));
   sl_scene::Bounce &B = sc->bounce;
+  B.walk_stack = walk_stack;
   std::vector<uint32_t> up;
   for (uint32_t d = 0; d <= t->max_depth; d++) {
     B.lv_off.push_back((uint32_t)up.size());
@@ -1679,7 +1707,8 @@ extern "C" sl_status sl_bounce_init(sl_context *ctx, sl_scene *sc, const sl_boun
                       {(void **)&B.probe_data, (size_t)NP * 128 * 32}, {(void **)&B.probe_sh, (size_t)NP * 9 * 16},
                       {(void **)&B.lo_dir, (size_t)G * 16}, {(void **)&B.lo_b, (size_t)G * 48},
                       {(void **)&B.xfer, (size_t)G * 48}, {(void **)&B.c0, (size_t)G * 16},
-                      {(void **)&B.cells, sizeof(cells)}, {&B.pp, sizeof(ProbeParamsDev)}};
+                      {(void **)&B.cells, sizeof(cells)}, {&B.pp, sizeof(ProbeParamsDev)},
+                      {(void **)&B.wcache, (size_t)G * 14 * 4}};
   for (const Buf &b : bufs) {
     if ((s = scene_alloc(ctx, sc, b.bytes, &p)) != SL_OK) return s;
     *b.dst = p;
@@ -1692,6 +1721,7 @@ extern "C" sl_status sl_bounce_init(sl_context *ctx, sl_scene *sc, const sl_boun
   SL_HIP(ctx, hipMemcpyAsync(B.cells, cells, sizeof(cells), hipMemcpyHostToDevice, ctx->stream));
   SL_HIP(ctx, hipMemcpyAsync(B.pp, &PP, sizeof(PP), hipMemcpyHostToDevice, ctx->stream));
   SL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  B.w_version = 0xFFFFFFFFu;
   // the whole tree replaces the scene's leaf array (its leaves are the same records)
   auto it = std::find(sc->allocs.begin(), sc->allocs.end(), (void *)sc->nodes);
   if (it != sc->allocs.end()) {
@@ -1739,19 +1769,28 @@ extern "C" sl_status sl_bounce_encode(sl_context *ctx, sl_scene *sc, const void 
     sl_bounce_lo_dir<<<(nd + 63) / 64, 64, 0, st>>>(U, B.ENV, sc->g[1], sc->g[2], sc->g[3], sc->storage, sc->nodes,
                                                     sc->vgrid, sc->parts, sc->materials, sc->iso_norm,
                                                     sc->env_T, B.c0, B.lo_dir);
-    sl_node_leaf<<<gblocks, 64, 0, st>>>(U, PPd, sc->g[2], sc->storage, sc->nodes, sc->vgrid, sc->parts, B.lo_dir,
-                                         B.lo_b, B.node_acc);
+    // node_leaf, node_up per depth (deepest first) and node_sh2: the leaves' weights follow the part table, the
+    // leaves' sums are formed where they are read
+    if (B.w_version != sc->parts_version) {
+      sl_leaf_weights<<<gblocks, 64, 0, st>>>(G, PPd, sc->g[2], sc->storage, sc->nodes, sc->vgrid, sc->parts, B.wcache);
+      B.w_version = sc->parts_version;
+    }
+    sl_leaf_sh<<<gblocks, 64, 0, st>>>(G, PPd, sc->g[2], sc->storage, B.wcache, B.lo_dir, B.lo_b, B.node_sh);
     for (int d = (int)B.max_depth - 1; d >= 1; d--) {
       const uint32_t cnt = B.lv_cnt[d];
       if (cnt)
-        sl_node_up<<<(cnt + 63) / 64, 64, 0, st>>>(G, PPd, sc->nodes, B.kids, B.up_nodes, (uint32_t)d, B.lv_off[d],
-                                                    cnt, B.node_acc);
+        sl_node_up_sh<<<(cnt + 63) / 64, 64, 0, st>>>(G, B.node_count, PPd, sc->g[2], sc->storage, sc->nodes, B.kids,
+                                                       B.up_nodes, (uint32_t)d, B.lv_off[d], cnt, B.wcache, B.lo_dir,
+                                                       B.lo_b, B.node_acc, B.node_sh);
     }
-    sl_node_sh2<<<(B.node_count + 63) / 64, 64, 0, st>>>(PPd, B.node_acc, B.node_sh);
     if (timers) SL_HIP(ctx, hipEventRecord(ctx->bev[1], st));
     const uint32_t rays = (NP * 128u + PP.subset - 1u) / PP.subset;
-    sl_probe_gather<<<(rays + 63) / 64, 64, 0, st>>>(U, PPd, B.ENV, sc->nodes, sc->vgrid, sc->parts, B.kids, B.node_sh,
-                                                     B.cells, B.probe_data, rays);
+    if (B.walk_stack <= 32u)
+      sl_probe_gather<32><<<(rays + 63) / 64, 64, 0, st>>>(U, PPd, B.ENV, sc->nodes, sc->vgrid, sc->parts, B.kids,
+                                                           B.node_sh, B.cells, B.probe_data, rays);
+    else
+      sl_probe_gather<64><<<(rays + 63) / 64, 64, 0, st>>>(U, PPd, B.ENV, sc->nodes, sc->vgrid, sc->parts, B.kids,
+                                                           B.node_sh, B.cells, B.probe_data, rays);
     if (timers) SL_HIP(ctx, hipEventRecord(ctx->bev[2], st));
     sl_probe_valid<<<(NP + 63) / 64, 64, 0, st>>>(U, PPd, sc->parts, B.probe_data);
     sl_probe_sh2<<<(NP + 63) / 64, 64, 0, st>>>(PPd, B.probe_data, B.cells, B.probe_sh);
@@ -1785,6 +1824,7 @@ extern "C" sl_status sl_bounce_read(sl_context *ctx, sl_scene *sc, uint32_t whic
     case SL_BOUNCE_PROBE_SH: src = B.probe_sh; n = (size_t)B.n_probes * 9 * 16; break;
     case SL_BOUNCE_XFER_SH: src = B.xfer; n = (size_t)sc->count * 48; break;
     case SL_BOUNCE_C0: src = B.c0; n = (size_t)sc->count * 16; break;
+    case SL_BOUNCE_LEAF_W: src = B.wcache; n = (size_t)sc->count * 56; break;
     default: return fail(ctx, SL_ERR_INVALID, "unknown bounce buffer");
   }
   if (!host) {   // the size alone
@@ -1793,7 +1833,15 @@ extern "C" sl_status sl_bounce_read(sl_context *ctx, sl_scene *sc, uint32_t whic
   }
   if (bytes < n) return fail(ctx, SL_ERR_INVALID, "host buffer too small");
   SL_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  SL_HIP(ctx, hipMemcpy(host, src, n, hipMemcpyDeviceToHost));
+  if (which == SL_BOUNCE_NODE_ACC) {   // stored direction-major: handed out node-major
+    std::vector<float4> tmp((size_t)B.node_count * 14);
+    SL_HIP(ctx, hipMemcpy(tmp.data(), src, n, hipMemcpyDeviceToHost));
+    float4 *o = (float4 *)host;
+    for (size_t k = 0; k < 14; k++)
+      for (size_t i = 0; i < B.node_count; i++) o[i * 14 + k] = tmp[k * B.node_count + i];
+  } else {
+    SL_HIP(ctx, hipMemcpy(host, src, n, hipMemcpyDeviceToHost));
+  }
   if (written) *written = n;
   return SL_OK;
 }

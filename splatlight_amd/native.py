@@ -98,7 +98,7 @@ class BounceTree(C.Structure):
 
 SL_BOUNCE_TIMERS, SL_BOUNCE_XFER, SL_BOUNCE_LIGHT_STILL = 1, 2, 4
 (SL_BOUNCE_LO_DIR, SL_BOUNCE_LO_B, SL_BOUNCE_NODE_ACC, SL_BOUNCE_NODE_SH, SL_BOUNCE_PROBE_DATA, SL_BOUNCE_PROBE_SH,
- SL_BOUNCE_XFER_SH, SL_BOUNCE_C0) = range(1, 9)
+ SL_BOUNCE_XFER_SH, SL_BOUNCE_C0, SL_BOUNCE_LEAF_W) = range(1, 10)
 
 
 class SplatlightError(RuntimeError):

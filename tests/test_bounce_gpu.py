@@ -87,12 +87,31 @@ def check_frames(ctx, sc, frames=2):
             # node sums from the device's lo_dir and last frame's lo_b
             o = oracle.node_leaf(U, PP, sc.gaussians, t.nodes, vg, sc.parts, s["lo_dir"], prev["lo_b"],
                                  np.zeros((Nn, 14, 4), np.float32))
-            o = oracle.node_up(U, PP, t.nodes, t.kids, up, levels, t.max_depth, o)
-            for c in range(4):
-                e = rel(s["node_acc"][..., c], o[..., c], 1e-4 * max(float(np.abs(o[..., c]).max()), 1e-12))
-                log(f"    node_acc[{c}] rel {e:.2e}")
+            w = scene.bounce_read(N.SL_BOUNCE_LEAF_W)
+            if w.size:
+                # one refit launch: the leaves' cached weights against node_leaf's, then the internal sums and every
+                # node's SH2 bit for bit from the device's leaf weights and light
+                w = w.reshape(G, 14)
+                e = rel(w, o[:G, :, 3], 1e-4 * max(float(o[:G, :, 3].max()), 1e-12))
+                log(f"    leaf weights rel {e:.2e}")
                 assert e < 1e-4
-            o = oracle.node_sh2(PP, s["node_acc"], Nn)
+                Lg = s["lo_dir"][:, :3] + prev["lo_b"][:, 1, :3]
+                leaf = np.zeros((Nn, 14, 4), np.float32)
+                leaf[:G, :, :3] = Lg[:, None, :] * w[:, :, None]
+                leaf[:G, :, 3] = w
+                o = oracle.node_up(U, PP, t.nodes, t.kids, up, levels, t.max_depth, leaf)
+                mism = int(np.count_nonzero(s["node_acc"][G:] != o[G:]))
+                log(f"    internal node sums: {mism} of {o[G:].size} differ")
+                assert mism == 0
+                acc_for_sh = o
+            else:
+                o = oracle.node_up(U, PP, t.nodes, t.kids, up, levels, t.max_depth, o)
+                for c in range(4):
+                    e = rel(s["node_acc"][..., c], o[..., c], 1e-4 * max(float(np.abs(o[..., c]).max()), 1e-12))
+                    log(f"    node_acc[{c}] rel {e:.2e}")
+                    assert e < 1e-4
+                acc_for_sh = s["node_acc"]
+            o = oracle.node_sh2(PP, acc_for_sh, Nn)
             mism = int(np.count_nonzero(o != s["node_sh"]))
             log(f"    node_sh: {mism} of {o.size} words differ")
             assert mism == 0
