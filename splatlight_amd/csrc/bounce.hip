@@ -360,16 +360,20 @@ sl_node_up_sh(uint32_t G, uint32_t N, const ProbeParamsDev *__restrict__ PP, con
 // SL: the LDS stack's entries per lane. The reference's walk keeps 64 (a push past them is dropped); the host takes
 // 32 when the tree cannot fill more than 32 (no push is ever dropped then either), 64 otherwise. A wave takes 64
 // directions of one probe: they walk the same part of the tree, which keeps its nodes in cache (a wave of one
-// direction over 64 probes walked alike but ran 20 % slower).
+// direction over 64 probes walked alike but ran 20 % slower). Workgroups take the rays from the last probe down:
+// the part and volume probes, which sit inside objects and walk the longest (a thousand nodes a ray), start first.
+// The walk reads a compact copy of each part's tree (built at set-up): nodes renumbered breadth first per part, so a
+// node's children are consecutive and in kids order, 32 B a node: (mean, radius), (smax, first child, child count,
+// node id). A visit is one 32 B read; only a used node reads its full record (by node id).
 template <uint32_t SL>
 __global__ void __launch_bounds__(64)
 sl_probe_gather(FrameUniformsDev U, const ProbeParamsDev *__restrict__ PP, EnvDev ENV, const float4 *__restrict__ nodes,
-                const uint8_t *__restrict__ vgrid, const float4 *__restrict__ parts, const uint32_t *__restrict__ kids,
-                const uint32_t *__restrict__ node_sh, const float4 *__restrict__ cells, float4 *__restrict__ probe_data,
-                uint32_t n_threads) {
+                const uint8_t *__restrict__ vgrid, const float4 *__restrict__ parts, const float4 *__restrict__ walk,
+                const uint32_t *__restrict__ walk_root, const uint32_t *__restrict__ node_sh,
+                const float4 *__restrict__ cells, float4 *__restrict__ probe_data, uint32_t n_threads) {
   __shared__ uint32_t s_stack[SL][64];
   const uint32_t lane = threadIdx.x;
-  const uint32_t t = blockIdx.x * 64u + lane;
+  const uint32_t t = (gridDim.x - 1u - blockIdx.x) * 64u + lane;
   if (t >= n_threads) return;
   const uint32_t G = U.gaussian_count;
   const uint32_t np = lat_count(PP);
@@ -403,23 +407,24 @@ sl_probe_gather(FrameUniformsDev U, const ProbeParamsDev *__restrict__ PP, EnvDe
     if (q == skip) continue;
     const Xf xf = load_xf(parts, q);
     const v3 Po = xf_point_inv(xf, o), wr = xf_dir_inv(xf, w), cr = xf_point_inv(xf, cen);
-    s_stack[0][lane] = part_root(parts, q);
+    s_stack[0][lane] = walk_root[q];
     uint32_t sp = 1;
     while (sp > 0u) {
       sp -= 1u;
-      const uint32_t ni = s_stack[sp][lane];
-      const float4 *n = nodes + 6 * (size_t)ni;
-      const float4 n0 = n[0];
+      const uint32_t wi = s_stack[sp][lane];
+      const float4 n0 = walk[2 * (size_t)wi];
       const v3 mean = xyz(n0);
       const v3 d = sub3(mean, Po);
       const float tt = clampb(dot3(d, wr), 0.0f, t1);
       const v3 e = sub3(d, mul3s(wr, tt));
       if (dot3(e, e) > n0.w * n0.w) continue;
-      const float4 n1 = n[1], n2 = n[2];
+      const float4 wb = walk[2 * (size_t)wi + 1];
+      const uint32_t ni = __float_as_uint(wb.w);
       bool use = ni < G;
-      if (!use) use = len3(sub3(cr, mean)) - brad > theta * n2.w;
+      if (!use) use = len3(sub3(cr, mean)) - brad > theta * wb.x;
       if (use) {
-        const float4 n3 = n[3], n4 = n[4];
+        const float4 *n = nodes + 6 * (size_t)ni;
+        const float4 n1 = n[1], n2 = n[2], n3 = n[3], n4 = n[4];
         const NodeL nd = {xyz(n1), xyz(n2), xyz(n3), xyz(n4), n1.w};
         const float tau = nd.kappa * dirm(nd, wr) * seg_geom(mean, nd, Po, wr, 0.0f, t1);
         if (tau > 1e-6f) {
@@ -452,9 +457,9 @@ sl_probe_gather(FrameUniformsDev U, const ProbeParamsDev *__restrict__ PP, EnvDe
           k = min(k + 1u, HIT_MAX);
         }
       } else {
-        const uint32_t clo = __float_as_uint(n[4].w), chi = __float_as_uint(n[5].x);
-        for (uint32_t j = clo; j < chi; j++)
-          if (sp < SL) { s_stack[sp][lane] = kids[j]; sp += 1u; }
+        const uint32_t fc = __float_as_uint(wb.y), nk = __float_as_uint(wb.z);
+        for (uint32_t j = 0; j < nk; j++)
+          if (sp < SL) { s_stack[sp][lane] = fc + j; sp += 1u; }
       }
     }
   }

@@ -113,6 +113,8 @@ struct sl_scene {
     float *wcache = nullptr;
     uint32_t w_version = 0xFFFFFFFFu;
     uint32_t walk_stack = 0;   // the most entries the gather's depth-first walk can hold on this tree
+    float4 *walk = nullptr;    // the gather's compact tree: 2 float4 per walk node (breadth first per part)
+    uint32_t *walk_root = nullptr;
     FrameUniformsDev U{};
     EnvDev ENV{};
   } bounce;
@@ -1676,6 +1678,51 @@ extern "C" sl_status sl_bounce_init(sl_context *ctx, sl_scene *sc, const sl_boun
       for (uint32_t j = clo; j < chi; j++) todo.push_back({t->kids[j], P + (j - clo)});
     }
   }
+  // the gather's compact tree: each part's subtree breadth first from its root, so a node's children get
+  // consecutive walk indices in kids order; 32 B a node: mean, radius, smax, first child, child count, node id
+  std::vector<float> walk;
+  std::vector<uint32_t> walk_root(SL_MAX_PARTS, 0u);
+  {
+    std::vector<uint32_t> order;   // walk index -> node id
+    for (uint32_t q = 0; q < sc->part_count; q++) {
+      const uint8_t *pq = sc->parts_host.data() + 128 * (size_t)q;
+      if (rd_u32(pq, 30) <= rd_u32(pq, 29)) continue;
+      walk_root[q] = (uint32_t)order.size();
+      size_t head = order.size();
+      order.push_back(rd_u32(pq, 28));
+      while (head < order.size()) {
+        const uint32_t i = order[head++];
+        if (i < G) continue;
+        for (uint32_t j = rd_u32(nb + 96 * (size_t)i, 19); j < rd_u32(nb + 96 * (size_t)i, 20); j++) order.push_back(t->kids[j]);
+        if (order.size() > (size_t)8 * N) return fail(ctx, SL_ERR_INVALID, "tree: the parts' subtrees are not trees");
+      }
+    }
+    walk.resize(order.size() * 8);
+    for (size_t w = 0; w < order.size(); w++) {
+      const uint32_t i = order[w];
+      const uint8_t *r = nb + 96 * (size_t)i;
+      float *o = walk.data() + 8 * w;
+      o[0] = rd_f32(r, 0); o[1] = rd_f32(r, 1); o[2] = rd_f32(r, 2); o[3] = rd_f32(r, 3);
+      o[4] = rd_f32(r, 11);
+      const uint32_t nk = i >= G ? rd_u32(r, 20) - rd_u32(r, 19) : 0u;
+      memcpy(o + 6, &nk, 4);
+      memcpy(o + 7, &i, 4);
+    }
+    // first-child indices: replay the breadth-first assignment per part (a node's children follow every child
+    // block handed out before it)
+    for (uint32_t q = 0; q < sc->part_count; q++) {
+      const uint8_t *pq = sc->parts_host.data() + 128 * (size_t)q;
+      if (rd_u32(pq, 30) <= rd_u32(pq, 29)) continue;
+      size_t end = walk_root[q] + 1;
+      for (size_t w = walk_root[q]; w < end; w++) {
+        uint32_t nk;
+        memcpy(&nk, walk.data() + 8 * w + 6, 4);
+        const uint32_t fc = (uint32_t)end;
+        memcpy(walk.data() + 8 * w + 5, &fc, 4);
+        end += nk;
+      }
+    }
+  }
   SL_HIP(ctx, hipStreamSynchronize(ctx->stream));
   sl_scene::Bounce &B = sc->bounce;
   B.walk_stack = walk_stack;
@@ -1708,7 +1755,9 @@ extern "C" sl_status sl_bounce_init(sl_context *ctx, sl_scene *sc, const sl_boun
                       {(void **)&B.lo_dir, (size_t)G * 16}, {(void **)&B.lo_b, (size_t)G * 48},
                       {(void **)&B.xfer, (size_t)G * 48}, {(void **)&B.c0, (size_t)G * 16},
                       {(void **)&B.cells, sizeof(cells)}, {&B.pp, sizeof(ProbeParamsDev)},
-                      {(void **)&B.wcache, (size_t)G * 14 * 4}};
+                      {(void **)&B.wcache, (size_t)G * 14 * 4},
+                      {(void **)&B.walk, std::max<size_t>(walk.size(), 8) * 4},
+                      {(void **)&B.walk_root, (size_t)SL_MAX_PARTS * 4}};
   for (const Buf &b : bufs) {
     if ((s = scene_alloc(ctx, sc, b.bytes, &p)) != SL_OK) return s;
     *b.dst = p;
@@ -1720,6 +1769,9 @@ extern "C" sl_status sl_bounce_init(sl_context *ctx, sl_scene *sc, const sl_boun
     SL_HIP(ctx, hipMemcpyAsync(B.up_nodes, up.data(), up.size() * 4, hipMemcpyHostToDevice, ctx->stream));
   SL_HIP(ctx, hipMemcpyAsync(B.cells, cells, sizeof(cells), hipMemcpyHostToDevice, ctx->stream));
   SL_HIP(ctx, hipMemcpyAsync(B.pp, &PP, sizeof(PP), hipMemcpyHostToDevice, ctx->stream));
+  if (!walk.empty())
+    SL_HIP(ctx, hipMemcpyAsync(B.walk, walk.data(), walk.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+  SL_HIP(ctx, hipMemcpyAsync(B.walk_root, walk_root.data(), (size_t)SL_MAX_PARTS * 4, hipMemcpyHostToDevice, ctx->stream));
   SL_HIP(ctx, hipStreamSynchronize(ctx->stream));
   B.w_version = 0xFFFFFFFFu;
   // the whole tree replaces the scene's leaf array (its leaves are the same records)
@@ -1786,11 +1838,11 @@ extern "C" sl_status sl_bounce_encode(sl_context *ctx, sl_scene *sc, const void 
     if (timers) SL_HIP(ctx, hipEventRecord(ctx->bev[1], st));
     const uint32_t rays = (NP * 128u + PP.subset - 1u) / PP.subset;
     if (B.walk_stack <= 32u)
-      sl_probe_gather<32><<<(rays + 63) / 64, 64, 0, st>>>(U, PPd, B.ENV, sc->nodes, sc->vgrid, sc->parts, B.kids,
-                                                           B.node_sh, B.cells, B.probe_data, rays);
+      sl_probe_gather<32><<<(rays + 63) / 64, 64, 0, st>>>(U, PPd, B.ENV, sc->nodes, sc->vgrid, sc->parts, B.walk,
+                                                           B.walk_root, B.node_sh, B.cells, B.probe_data, rays);
     else
-      sl_probe_gather<64><<<(rays + 63) / 64, 64, 0, st>>>(U, PPd, B.ENV, sc->nodes, sc->vgrid, sc->parts, B.kids,
-                                                           B.node_sh, B.cells, B.probe_data, rays);
+      sl_probe_gather<64><<<(rays + 63) / 64, 64, 0, st>>>(U, PPd, B.ENV, sc->nodes, sc->vgrid, sc->parts, B.walk,
+                                                           B.walk_root, B.node_sh, B.cells, B.probe_data, rays);
     if (timers) SL_HIP(ctx, hipEventRecord(ctx->bev[2], st));
     sl_probe_valid<<<(NP + 63) / 64, 64, 0, st>>>(U, PPd, sc->parts, B.probe_data);
     sl_probe_sh2<<<(NP + 63) / 64, 64, 0, st>>>(PPd, B.probe_data, B.cells, B.probe_sh);
