@@ -54,6 +54,16 @@ class Lighting {
   }
   // uniforms: {frameUniforms, emitParams, emitNodes, lsParams?, env?, sunParams?, sunLayers?} (bytes)
   encode(encoder, uniforms, timers) { addon.lightingEncode(this.renderer.scene, uniforms, !!timers); }
+  // initBounce (src/lighting.js:723-840): the asset's tree {nodes, kids, maxDepth} and the ProbeParams bytes the
+  // host's layout code writes (probeLayout, D14, pinvD14 unchanged)
+  initBounce(tree, probeParams) {
+    addon.bounceInit(this.renderer.scene, {nodes: tree.nodes, kids: tree.kids, maxDepth: tree.maxDepth, probeParams});
+  }
+  // encodeBounce (src/lighting.js:1020-1103) after this frame's encode: probeParams with this frame's words
+  // (alpha, force_near, subset, phase); xfer: the transfer is stale (forceNear or a part moved)
+  encodeBounce(encoder, timers, lightStill, probeParams, xfer) {
+    addon.bounceEncode(this.renderer.scene, probeParams, {timers: !!timers, lightStill: !!lightStill, xfer: !!xfer});
+  }
 }
 
 module.exports = {acquireGpuSession, Device, SplatRenderer, Lighting, PASSES, OUT_RGBA16F, OUT_RGBA32F};

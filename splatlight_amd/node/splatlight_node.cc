@@ -294,6 +294,60 @@ napi_value LightingIsoInit(napi_env env, napi_callback_info info) {
 }
 
 // lightingEncode(scene, {frameUniforms, emitParams, emitNodes, lsParams?, env?, sunParams?, sunLayers?}, timers)
+// bounceInit(scene, {nodes, kids, maxDepth, probeParams}): Lighting.initBounce (src/lighting.js:723-840) with the
+// asset's tree (nodes.bin: 96 B records, leaves first; kids.bin: u32) and the ProbeParams bytes it writes
+napi_value BounceInit(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value a[2];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, a, nullptr, nullptr));
+  Scene *s = unwrap<Scene>(env, a[0]);
+  if (!s) return nullptr;
+  const void *nodes = nullptr, *kids = nullptr, *pp = nullptr;
+  size_t nn = 0, nk = 0, npp = 0;
+  if (!bytes_of(env, prop(env, a[1], "nodes"), &nodes, &nn) || !bytes_of(env, prop(env, a[1], "kids"), &kids, &nk) ||
+      !bytes_of(env, prop(env, a[1], "probeParams"), &pp, &npp)) {
+    napi_throw_type_error(env, nullptr, "bounceInit: nodes, kids and probeParams must be ArrayBuffers or typed arrays");
+    return nullptr;
+  }
+  uint32_t max_depth = 0;
+  napi_get_value_uint32(env, prop(env, a[1], "maxDepth"), &max_depth);
+  sl_bounce_tree t;
+  t.nodes = nodes;
+  t.node_count = (uint32_t)(nn / 96);
+  t.node_stride = 96;
+  t.kids = (const uint32_t *)kids;
+  t.kid_count = (uint32_t)(nk / 4);
+  t.max_depth = max_depth;
+  if (sl_status rc = sl_bounce_init(s->ctx, s->scene, &t, pp, npp)) return throw_sl(env, s->ctx, rc, "Lighting.initBounce");
+  return nullptr;
+}
+
+// bounceEncode(scene, probeParams, {timers, xfer, lightStill}): Lighting.encodeBounce (src/lighting.js:1020-1103)
+napi_value BounceEncode(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value a[3];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, a, nullptr, nullptr));
+  Scene *s = unwrap<Scene>(env, a[0]);
+  if (!s) return nullptr;
+  const void *pp = nullptr;
+  size_t npp = 0;
+  if (!bytes_of(env, a[1], &pp, &npp)) {
+    napi_throw_type_error(env, nullptr, "bounceEncode: probeParams must be an ArrayBuffer or typed array");
+    return nullptr;
+  }
+  uint32_t flags = 0;
+  if (argc > 2) {
+    bool v = false;
+    if (napi_get_value_bool(env, prop(env, a[2], "timers"), &v) == napi_ok && v) flags |= SL_BOUNCE_TIMERS;
+    v = false;
+    if (napi_get_value_bool(env, prop(env, a[2], "xfer"), &v) == napi_ok && v) flags |= SL_BOUNCE_XFER;
+    v = false;
+    if (napi_get_value_bool(env, prop(env, a[2], "lightStill"), &v) == napi_ok && v) flags |= SL_BOUNCE_LIGHT_STILL;
+  }
+  if (sl_status rc = sl_bounce_encode(s->ctx, s->scene, pp, npp, flags)) return throw_sl(env, s->ctx, rc, "Lighting.encodeBounce");
+  return nullptr;
+}
+
 napi_value LightingEncode(napi_env env, napi_callback_info info) {
   size_t argc = 3;
   napi_value a[3];
@@ -440,6 +494,8 @@ NAPI_MODULE_INIT() {
       {"sceneWriteColour", nullptr, SceneWriteColour, nullptr, nullptr, nullptr, kMethod, nullptr},
       {"lightingIsoInit", nullptr, LightingIsoInit, nullptr, nullptr, nullptr, kMethod, nullptr},
       {"lightingEncode", nullptr, LightingEncode, nullptr, nullptr, nullptr, kMethod, nullptr},
+      {"bounceInit", nullptr, BounceInit, nullptr, nullptr, nullptr, kMethod, nullptr},
+      {"bounceEncode", nullptr, BounceEncode, nullptr, nullptr, nullptr, kMethod, nullptr},
       {"frameEncode", nullptr, FrameEncode, nullptr, nullptr, nullptr, kMethod, nullptr},
       {"frameFinish", nullptr, FrameFinish, nullptr, nullptr, nullptr, kMethod, nullptr},
       {"timersRead", nullptr, TimersRead, nullptr, nullptr, nullptr, kMethod, nullptr},
