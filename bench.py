@@ -413,7 +413,10 @@ def stats_summary(csv_path, out, command):
              "Per relit frame: one launch each of sl_map_accumulate<0u> (emitter map) and <1u> (sun maps), "
              "sl_shade_emit, sl_env_direct and the frame path; the cached-lighting frames add frame-path launches "
              "only. The emitter-map chain and the sun-map chain run on two streams at once, so kernel times include "
-             "sharing the GPU.", "",
+             "sharing the GPU. The bounce leg's frames add the sky and bounce through probes (tree of 2M nodes, "
+             "1,339 probes x 128 rays): sl_bounce_lo_dir, sl_leaf_sh, sl_node_up_sh (one launch per tree depth), "
+             "sl_probe_gather, sl_probe_valid, sl_probe_sh2, sl_bounce_sh each frame, sl_iso_xfer and "
+             "sl_leaf_weights once.", "",
              "| kernel | calls | avg us | min us | max us | share |", "|---|---|---|---|---|---|"]
     for r in rows:
         lines.append(f"| {short_name(r['Name'])} | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
@@ -444,7 +447,7 @@ def pmc_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, out, comma
         k = max(v, key=lambda name: np.mean(v[name]))
         return nbytes / (np.mean(v[k]) * 1024.0), k, float(np.mean(v[k]))
 
-    keep = ("sl_composite<256u>", "sl_map_accumulate<0u>", "sl_map_accumulate<1u>", "sl_shade_emit<16u>",
+    keep = ("sl_composite<256u>", "sl_map_accumulate<0u>", "sl_map_accumulate<1u>", "sl_shade_emit_wave",
             "sl_shade_emit<48u>")
     copy_bytes = 1024 << 20
     fs, fk, fv = copy_scale(copy_fetch_dir, "FETCH_SIZE", copy_bytes)
