@@ -52,7 +52,7 @@ def read_state(scene, n_probes, G, N_nodes):
     }
 
 
-def check_frames(ctx, sc, frames=2):
+def check_frames(ctx, sc, frames=2, moved_mask=0):
     scene, t, vg, b = setup(ctx, sc)
     G, Nn, NP = sc.count, t.count, b.n_probes
     cells = oracle.cells_table(*BN.cell_table())
@@ -64,8 +64,8 @@ def check_frames(ctx, sc, frames=2):
         for f in range(frames):
             inp = L.lighting_inputs(sc, frame=f)
             scene.light(inp["U"], inp["EP"], inp["emit_nodes"], inp["LS"], inp["ENV"], inp["SM"], inp["desc"])
-            run_xfer = b.force_near
-            PP = b.encode()
+            run_xfer = b.force_near or moved_mask != 0
+            PP = b.encode(moved_mask=moved_mask)
             ctx.sync()
             s = read_state(scene, NP, G, Nn)
             U, ENV = inp["U"], inp["ENV"]
@@ -169,6 +169,18 @@ def test_bounce_cornell_room(ctx):
     """The room lattice in the Cornell box: emitter light only (no environment)."""
     log("bounce: cornell 20k")
     scene, _ = check_frames(ctx, scenes.cornell(n=20_000))
+    scene.destroy()
+
+
+def test_bounce_movable_part(ctx):
+    """A movable part that moved this frame (Part.flags bit 0 clear, bit 1 set, its previous pose shifted): its 8
+    part probes gather without it and serve its Gaussians alone, and every frame rebuilds the transfer of the
+    Gaussians on it or within its bound (plus near_r) of either pose."""
+    log("bounce: cornell 20k, the tall box movable and moved")
+    sc = scenes.cornell(n=20_000)
+    sc.parts["flags"][2] = 2
+    sc.parts["xf_prev"][2][0][3] = 0.15
+    scene, b = check_frames(ctx, sc, moved_mask=1 << 2)
     scene.destroy()
 
 
