@@ -119,12 +119,19 @@ __device__ v3 probe_pos(const ProbeParamsDev *PP, uint32_t part_count, const flo
   return V3(PP->lo[0] + PP->step[0] * (float)i, PP->lo[1] + PP->step[1] * (float)j, lat_ground(PP, i, j) + lat_level(PP, k));
 }
 
-// bounce.wgsl:199-204 (asin and atan2 in double, rounded once)
+// bounce.wgsl:199-204, with asin and atan2 defined as double, rounded once. The float functions (a few ulp) move the
+// row and column coordinates by under 1e-5, so where a coordinate lies more than 1e-4 from a cell boundary its
+// floor is the double definition's; the double functions run only near a boundary (or for NaN).
 __device__ __forceinline__ uint32_t cell_of(v3 w) {
-  const float el = asin_cr(clampb(w.z, -1.0f, 1.0f));
-  const float fi = fmaxf((0.5f * PI_F - el) / (PI_F / 8.0f), 0.0f);
+  const float z = clampb(w.z, -1.0f, 1.0f);
+  float fi = fmaxf((0.5f * PI_F - asinf(z)) / (PI_F / 8.0f), 0.0f);
+  const float ri = fi - floorf(fi);
+  if (!(ri > 1e-4f && ri < 1.0f - 1e-4f)) fi = fmaxf((0.5f * PI_F - asin_cr(z)) / (PI_F / 8.0f), 0.0f);
   const uint32_t i = min((uint32_t)fi, 7u);
-  const uint32_t j = ((uint32_t)floorf((atan2_cr(w.y, w.x) + PI_F) / (2.0f * PI_F) * 16.0f)) % 16u;
+  float fj = (atan2f(w.y, w.x) + PI_F) / (2.0f * PI_F) * 16.0f;
+  const float rj = fj - floorf(fj);
+  if (!(rj > 1e-4f && rj < 1.0f - 1e-4f)) fj = (atan2_cr(w.y, w.x) + PI_F) / (2.0f * PI_F) * 16.0f;
+  const uint32_t j = ((uint32_t)floorf(fj)) % 16u;
   return i * 16u + j;
 }
 __device__ __forceinline__ void sh2_basis(v3 d, float b[9]) {   // bounce.wgsl:206-210
