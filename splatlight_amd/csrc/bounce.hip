@@ -314,17 +314,13 @@ sl_leaf_sh(uint32_t G, const ProbeParamsDev *__restrict__ PP, const float4 *__re
   sh2_pack(PP, s, node_sh + 14 * (size_t)g);
 }
 
-// node_up + node_sh2 for one depth (up_nodes[r0, r0 + count)): each node sums its children in child order and
-// stores its sums and SH2
-extern "C" __global__ void __launch_bounds__(64)
-sl_node_up_sh(uint32_t G, uint32_t N, const ProbeParamsDev *__restrict__ PP, const float4 *__restrict__ g2, uint32_t st,
-              const float4 *__restrict__ nodes, const uint32_t *__restrict__ kids, const uint32_t *__restrict__ up_nodes,
-              uint32_t depth, uint32_t r0, uint32_t count, const float *__restrict__ wcache,
-              const float4 *__restrict__ lo_dir, const float4 *__restrict__ lo_b, float4 *__restrict__ node_acc,
-              uint32_t *__restrict__ node_sh) {
-  const uint32_t t = blockIdx.x * 64u + threadIdx.x;
-  if (t >= count) return;
-  const uint32_t i = up_nodes[r0 + t];
+// node_up + node_sh2 of node i when it sits at this depth: sums its children in child order, stores its sums and SH2
+__device__ __forceinline__ void node_up_one(uint32_t G, uint32_t N, const ProbeParamsDev *__restrict__ PP,
+                                            const float4 *__restrict__ g2, uint32_t st, const float4 *__restrict__ nodes,
+                                            const uint32_t *__restrict__ kids, uint32_t i, uint32_t depth,
+                                            const float *__restrict__ wcache, const float4 *__restrict__ lo_dir,
+                                            const float4 *__restrict__ lo_b, float4 *__restrict__ node_acc,
+                                            uint32_t *__restrict__ node_sh) {
   if (i < G || i >= N) return;
   const float4 *n = nodes + 6 * (size_t)i;
   const uint32_t part = __float_as_uint(n[3].w), clo = __float_as_uint(n[4].w);
@@ -354,6 +350,38 @@ sl_node_up_sh(uint32_t G, uint32_t N, const ProbeParamsDev *__restrict__ PP, con
 #pragma unroll
   for (int k = 0; k < 14; k++) node_acc[(size_t)k * N + i] = s[k];
   sh2_pack(PP, s, node_sh + 14 * (size_t)i);
+}
+
+// node_up + node_sh2 for one depth (up_nodes[r0, r0 + count)), one lane per node
+extern "C" __global__ void __launch_bounds__(64)
+sl_node_up_sh(uint32_t G, uint32_t N, const ProbeParamsDev *__restrict__ PP, const float4 *__restrict__ g2, uint32_t st,
+              const float4 *__restrict__ nodes, const uint32_t *__restrict__ kids, const uint32_t *__restrict__ up_nodes,
+              uint32_t depth, uint32_t r0, uint32_t count, const float *__restrict__ wcache,
+              const float4 *__restrict__ lo_dir, const float4 *__restrict__ lo_b, float4 *__restrict__ node_acc,
+              uint32_t *__restrict__ node_sh) {
+  const uint32_t t = blockIdx.x * 64u + threadIdx.x;
+  if (t >= count) return;
+  node_up_one(G, N, PP, g2, st, nodes, kids, up_nodes[r0 + t], depth, wcache, lo_dir, lo_b, node_acc, node_sh);
+}
+
+// The shallow depths (depth d_hi up to 1, each of at most UP_TAIL_MAX nodes) in one workgroup, deepest first, with a
+// workgroup barrier between depths: a depth reads the sums its children's depth stored, and all lanes of a workgroup
+// share one CU's vector cache, so workgroup-scope ordering makes them visible. One launch in place of one per depth.
+constexpr uint32_t UP_TAIL_MAX = 2048u, UP_TAIL_DEPTHS = 32u;
+struct UpTail {
+  uint32_t off[UP_TAIL_DEPTHS + 1], cnt[UP_TAIL_DEPTHS + 1];
+};
+extern "C" __global__ void __launch_bounds__(1024)
+sl_node_up_tail(uint32_t G, uint32_t N, const ProbeParamsDev *__restrict__ PP, const float4 *__restrict__ g2,
+                uint32_t st, const float4 *__restrict__ nodes, const uint32_t *__restrict__ kids,
+                const uint32_t *__restrict__ up_nodes, UpTail T, uint32_t d_hi, const float *__restrict__ wcache,
+                const float4 *__restrict__ lo_dir, const float4 *__restrict__ lo_b, float4 *__restrict__ node_acc,
+                uint32_t *__restrict__ node_sh) {
+  for (uint32_t d = d_hi; d >= 1u; d--) {
+    for (uint32_t t = threadIdx.x; t < T.cnt[d]; t += blockDim.x)
+      node_up_one(G, N, PP, g2, st, nodes, kids, up_nodes[T.off[d] + t], d, wcache, lo_dir, lo_b, node_acc, node_sh);
+    __syncthreads();
+  }
 }
 
 // ---- probe_gather (bounce.wgsl:1255-1415) ---------------------------------------------------------------------------
@@ -515,61 +543,58 @@ sl_probe_gather(FrameUniformsDev U, const ProbeParamsDev *__restrict__ PP, EnvDe
   probe_data[2 * (size_t)id + 1] = make_float4(mixf(old1.x, m1, a), mixf(old1.y, m2, a), z1, near);
 }
 
-// probe_valid (bounce.wgsl:1417-1445): a lattice probe is dropped when most of its rays first hit one movable part,
-// or more than half start near a surface or hit a Gaussian from its buried side; part and volume probes stay valid
+// probe_valid (bounce.wgsl:1417-1445) and probe_sh2 (bounce.wgsl:1028-1041), one wave per probe, two cells a lane.
+// Validity: a lattice probe is dropped when most of its rays first hit one movable part, or more than half start near
+// a surface or hit a Gaussian from its buried side (counts by ballot); part and volume probes stay valid. SH2 of the
+// probe's incoming light over its 128 cells: each cell's weighted light and basis go to LDS, then lane 3m + ch sums
+// coefficient m of channel ch over the cells in cell order, the order of the reference's serial loop.
 extern "C" __global__ void __launch_bounds__(64)
-sl_probe_valid(FrameUniformsDev U, const ProbeParamsDev *__restrict__ PP, const float4 *__restrict__ parts,
-               float4 *__restrict__ probe_data) {
-  const uint32_t p = blockIdx.x * 64u + threadIdx.x;
+sl_probe_finish(FrameUniformsDev U, const ProbeParamsDev *__restrict__ PP, const float4 *__restrict__ parts,
+                const float4 *__restrict__ cells, float4 *__restrict__ probe_data, float *__restrict__ probe_sh) {
+  __shared__ float s_l[NCELL][3];
+  __shared__ float s_y[NCELL][9];
+  const uint32_t p = blockIdx.x, lane = threadIdx.x;
   const uint32_t np = lat_count(PP);
   if (p >= np + PP->part_probes) return;
-  float4 *at = probe_data + 2 * (size_t)p * NCELL + 1;
-  if (p >= np) {
-    at->z = 1.0f;
-    return;
-  }
-  uint32_t cnt[17];
+  const float4 *rec = probe_data + 2 * (size_t)p * NCELL;
+  uint32_t near = 0u, back = 0u, qs[2];
 #pragma unroll
-  for (int i = 0; i < 17; i++) cnt[i] = 0u;
-  uint32_t near = 0, back = 0;
-  for (uint32_t c = 0; c < NCELL; c++) {
-    const float4 r0 = probe_data[2 * ((size_t)p * NCELL + c)], r1 = probe_data[2 * ((size_t)p * NCELL + c) + 1];
-    const uint32_t fl = (uint32_t)(fmaxf(r1.w, 0.0f) + 0.5f);
-    near += fl & 1u;
-    back += (fl >> 1u) & 1u;
-    if (r0.w >= 0.0f) {
-      const uint32_t q = min((uint32_t)r0.w, 16u);
-#pragma unroll
-      for (int i = 0; i < 17; i++) cnt[i] += (uint32_t)i == q ? 1u : 0u;
-    }
-  }
-  bool enclosed = false;
-#pragma unroll
-  for (int q = 0; q < 16; q++)
-    if ((uint32_t)q < U.part_count && (part_flags(parts, q) & 1u) == 0u && (float)cnt[q] > PP->g_valid * (float)NCELL)
-      enclosed = true;
-  at->z = (enclosed || near * 2u > NCELL || back * 2u > NCELL) ? 0.0f : 1.0f;
-}
-
-// probe_sh2 (bounce.wgsl:1028-1041): SH2 of each probe's incoming light over its 128 cells
-extern "C" __global__ void __launch_bounds__(64)
-sl_probe_sh2(const ProbeParamsDev *__restrict__ PP, const float4 *__restrict__ probe_data,
-             const float4 *__restrict__ cells, float4 *__restrict__ probe_sh) {
-  const uint32_t p = blockIdx.x * 64u + threadIdx.x;
-  if (p >= lat_count(PP) + PP->part_probes) return;
-  v3 c9[9];
-#pragma unroll
-  for (int m = 0; m < 9; m++) c9[m] = V3(0.0f, 0.0f, 0.0f);
-  for (uint32_t c = 0; c < NCELL; c++) {
-    const float4 cl = cells[c];
-    const v3 L = mul3s(xyz(probe_data[2 * ((size_t)p * NCELL + c)]), cl.w);
+  for (uint32_t h = 0; h < 2u; h++) {
+    const uint32_t c = lane + 64u * h;
+    const float4 r0 = rec[2 * c], r1 = rec[2 * c + 1], cl = cells[c];
+    const v3 L = mul3s(xyz(r0), cl.w);
+    s_l[c][0] = L.x;
+    s_l[c][1] = L.y;
+    s_l[c][2] = L.z;
     float y[9];
     sh2_basis(xyz(cl), y);
 #pragma unroll
-    for (int m = 0; m < 9; m++) c9[m] = add3(c9[m], mul3s(L, y[m]));
+    for (int m = 0; m < 9; m++) s_y[c][m] = y[m];
+    const uint32_t fl = (uint32_t)(fmaxf(r1.w, 0.0f) + 0.5f);
+    near += (uint32_t)__popcll(__ballot((fl & 1u) != 0u));
+    back += (uint32_t)__popcll(__ballot(((fl >> 1u) & 1u) != 0u));
+    qs[h] = r0.w >= 0.0f ? min((uint32_t)r0.w, 16u) : 17u;
   }
-#pragma unroll
-  for (int m = 0; m < 9; m++) probe_sh[9 * (size_t)p + m] = make_float4(c9[m].x, c9[m].y, c9[m].z, 0.0f);
+  float valid = 1.0f;
+  if (p < np) {
+    bool enclosed = false;
+    for (uint32_t q = 0; q < 16u && q < U.part_count; q++) {
+      if ((part_flags(parts, q) & 1u) != 0u) continue;
+      const uint32_t cnt = (uint32_t)__popcll(__ballot(qs[0] == q)) + (uint32_t)__popcll(__ballot(qs[1] == q));
+      if ((float)cnt > PP->g_valid * (float)NCELL) enclosed = true;
+    }
+    valid = (enclosed || near * 2u > NCELL || back * 2u > NCELL) ? 0.0f : 1.0f;
+  }
+  if (lane == 0u) probe_data[2 * (size_t)p * NCELL + 1].z = valid;
+  __syncthreads();
+  if (lane < 27u) {
+    const uint32_t m = lane / 3u, ch = lane % 3u;
+    float acc = 0.0f;
+    for (uint32_t c = 0; c < NCELL; c++) acc += s_l[c][ch] * s_y[c][m];
+    probe_sh[36 * (size_t)p + 4 * m + ch] = acc;
+  } else if (lane < 36u) {
+    probe_sh[36 * (size_t)p + 4 * (lane - 27u) + 3] = 0.0f;
+  }
 }
 
 // iso_xfer (bounce.wgsl:993-1026): per Gaussian near a part that moved (or on a forced refresh), the SH2 of its

@@ -730,11 +730,13 @@ __device__ __forceinline__ void map_texel(const MapRec &r, float dx, float py, f
 // is some 100x the float rounding of both the per-texel q and this minimum. true whenever the form is not positive
 // definite or not finite, so a rectangle with a contributing texel is never ruled out.
 struct MapForm {
-  float a, b, c;
+  float a, b, c, nba, nbc;   // -b/a, -b/c: where the form's minimum lies along a row (v fixed) or a column (u fixed)
   bool pd;
   __device__ __forceinline__ explicit MapForm(const MapRec &r)
       : a(r.s00), b(0.5f * (r.s01 + r.s10)), c(r.s11) {
     pd = a > 0.0f && c > 0.0f && a * c - b * b > 0.0f && a < 1e30f && c < 1e30f;
+    nba = pd ? -b * __builtin_amdgcn_rcpf(a) : 0.0f;
+    nbc = pd ? -b * __builtin_amdgcn_rcpf(c) : 0.0f;
   }
 };
 __device__ __forceinline__ bool map_rect_may_hit(const MapRec &r, const MapForm &f, int xa, int xb, int ya, int yb) {
@@ -747,26 +749,58 @@ __device__ __forceinline__ bool map_rect_may_hit(const MapRec &r, const MapForm 
   if (u0 <= 0.0f && u1 >= 0.0f && v0 <= 0.0f && v1 >= 0.0f) {
     m = 0.0f;                                            // the centre lies inside
   } else {
-    // the minimum lies on an edge: on u = const the best v is -b u / c, on v = const the best u is -b v / a
-    m = fminf(fminf(q(u0, fminf(fmaxf(-b * u0 / c, v0), v1)), q(u1, fminf(fmaxf(-b * u1 / c, v0), v1))),
-              fminf(q(fminf(fmaxf(-b * v0 / a, u0), u1), v0), q(fminf(fmaxf(-b * v1 / a, u0), u1), v1)));
+    // the minimum lies on an edge: on u = const the best v is -b u / c, on v = const the best u is -b v / a (the
+    // reciprocals' rounding moves a candidate by ulps, which raises q there by far less than the margin below)
+    m = fminf(fminf(q(u0, fminf(fmaxf(f.nbc * u0, v0), v1)), q(u1, fminf(fmaxf(f.nbc * u1, v0), v1))),
+              fminf(q(fminf(fmaxf(f.nba * v0, u0), u1), v0), q(fminf(fmaxf(f.nba * v1, u0), u1), v1)));
   }
   const float um = fmaxf(fabsf(u0), fabsf(u1)), vm = fmaxf(fabsf(v0), fabsf(v1));
   const float big = a * um * um + (fabsf(r.s01) + fabsf(r.s10)) * um * vm + c * vm * vm;
   return m <= 9.0f + 1e-4f * big + 1e-3f;
 }
 
-// Which of the 4 bands (16x4 texels, rows ty0 + 4k ..) of the 16x16 tile at (tx0, ty0) record r may touch: bit k.
-__device__ __forceinline__ uint32_t map_band_mask(const MapRec &r, int tx0, int ty0) {
+// Lane layouts of a 16x16 tile's 256 texels as four units of 64 (one wave each): 0 bands of 16x4 (unit k: rows 4k..4k+3;
+// lane: column lane & 15, row lane >> 4 of the band), 1 the transpose, columns of 4x16, 2 quads of 8x8 (unit k:
+// columns 8 (k & 1).., rows 8 (k >> 1)..; lane: column lane & 7, row lane >> 3). Tile-local texel of a lane in unit k:
+template <int LAY>
+__device__ __forceinline__ void map_unit_texel(uint32_t lane, int k, int &cx, int &cy) {
+  if (LAY == 0) {
+    cx = (int)(lane & 15u);
+    cy = 4 * k + (int)(lane >> 4);
+  } else if (LAY == 1) {
+    cx = 4 * k + (int)(lane >> 4);
+    cy = (int)(lane & 15u);
+  } else {
+    cx = 8 * (k & 1) + (int)(lane & 7u);
+    cy = 8 * (k >> 1) + (int)(lane >> 3);
+  }
+}
+
+// The layout record r is evaluated in over the 16x16 tile at texel (tx0, ty0), and which of its 4 units r may touch:
+// mask bits 0-3, layout in bits 4-5. The layout is the one whose units the record's box (within the tile) overlaps
+// fewest (bands on ties); each overlapped unit is kept unless the rectangle test rules it out.
+__device__ __forceinline__ uint32_t map_unit_mask(const MapRec &r, int tx0, int ty0) {
+  const int xa = max(r.x0 - tx0, 0), xb = min(r.x1 - tx0, 15), ya = max(r.y0 - ty0, 0), yb = min(r.y1 - ty0, 15);
+  if (xa > xb || ya > yb) return 0u;
+  const int nb = (yb >> 2) - (ya >> 2) + 1, nv = (xb >> 2) - (xa >> 2) + 1;
+  const int nq = ((yb >> 3) - (ya >> 3) + 1) * ((xb >> 3) - (xa >> 3) + 1);
+  const int lay = (nv < nb && nv <= nq) ? 1 : (nq < nb && nq < nv) ? 2 : 0;
+  const int nu = lay == 0 ? nb : lay == 1 ? nv : nq;
+  if (nu == 1) {                                     // one unit holds the box: the tile's own test kept it
+    const int k = lay == 0 ? ya >> 2 : lay == 1 ? xa >> 2 : 2 * (ya >> 3) + (xa >> 3);
+    return (1u << k) | ((uint32_t)lay << 4);
+  }
   const MapForm f(r);
-  const int xa = max(r.x0, tx0), xb = min(r.x1, tx0 + 15);
   uint32_t mask = 0;
 #pragma unroll 1
   for (int k = 0; k < 4; k++) {
-    const int ya = max(r.y0, ty0 + 4 * k), yb = min(r.y1, ty0 + 4 * k + 3);
-    if (xa <= xb && ya <= yb && map_rect_may_hit(r, f, xa, xb, ya, yb)) mask |= 1u << k;
+    const int ux0 = lay == 0 ? 0 : lay == 1 ? 4 * k : 8 * (k & 1);
+    const int uy0 = lay == 0 ? 4 * k : lay == 1 ? 0 : 8 * (k >> 1);
+    const int uw = lay == 0 ? 15 : lay == 1 ? 3 : 7, uh = lay == 0 ? 3 : lay == 1 ? 15 : 7;
+    const int rxa = max(xa, ux0), rxb = min(xb, ux0 + uw), rya = max(ya, uy0), ryb = min(yb, uy0 + uh);
+    if (rxa <= rxb && rya <= ryb && map_rect_may_hit(r, f, tx0 + rxa, tx0 + rxb, ty0 + rya, ty0 + ryb)) mask |= 1u << k;
   }
-  return mask;
+  return mask | ((uint32_t)lay << 4);
 }
 
 // Which tiles of record r's tile box (at most 8x8) it may touch: bit (row * box width + column), the order
@@ -785,47 +819,78 @@ __device__ __forceinline__ uint64_t map_tile_mask(const MapRec &r) {
   return mask;
 }
 
-// Per-tile sums of the records list[s0, s1) of the 16x16 tile at texel (tx0, ty0). Records are staged MAP_STAGE
-// at a time in LDS with one coalesced load per lane, each with its band mask for this tile (in place of its layer,
-// which the sums do not read); wave w evaluates staged records w, w + 4, ... over the whole tile, as four bands of
-// 16x4 texels (lane: column lane & 15, row lane >> 4 of the band), skipping the bands the mask rules out. Each
-// record is read back from LDS by one wave only. The four waves' sums meet in s_acc (integer adds,
-// order-independent); on return s_acc[c][t] holds texel t's sum c (t = row * 16 + column) and the caller clears it
-// after reading.
-__device__ __forceinline__ void map_tile_sums(const MapRec *__restrict__ recs, const uint32_t *__restrict__ list,
-                                              uint32_t s0, uint32_t s1, int tx0, int ty0, float fix, MapRec *s_rec,
-                                              uint32_t *s_acc) {
-  const uint32_t lane = threadIdx.x & 63u, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int x = tx0 + (int)(lane & 15u), yl = (int)(lane >> 4);
-  const float px = (float)x + 0.5f;
+// The wave's staged records [g0, g1) of one layout, each over its units of the tile, skipping the units its mask
+// rules out; the wave's sums are then added into s_acc (integer adds, order-independent).
+template <int LAY>
+__device__ __forceinline__ void map_group_sums(const MapRec *w_rec, uint32_t g0, uint32_t g1, uint32_t lane, int tx0,
+                                               int ty0, float fix, uint32_t *s_acc) {
+  if (g0 >= g1) return;                              // the same for every lane
   uint32_t a[4][3] = {};
-  for (uint32_t b = s0; b < s1; b += MAP_STAGE) {
-    const uint32_t nb = min(MAP_STAGE, s1 - b);
-    __syncthreads();
-    if (threadIdx.x < nb) {
-      MapRec r = recs[list[b + threadIdx.x]];
-      r.layer = map_band_mask(r, tx0, ty0);
-      s_rec[threadIdx.x] = r;
-    }
-    __syncthreads();
-    for (uint32_t j = wave; j < nb; j += 4u) {
-      const MapRec r = s_rec[j];
-      const bool xin = x >= r.x0 && x <= r.x1;
-      const float dx = px - r.ctx;
-      const uint32_t mask = r.layer;                     // the same for every lane
+  for (uint32_t j = g0; j < g1; j++) {
+    const MapRec r = w_rec[j];
+    const uint32_t mask = r.layer;                   // the same for every lane
 #pragma unroll
-      for (int k = 0; k < 4; k++) {
-        if (!((mask >> k) & 1u)) continue;
-        const int y = ty0 + 4 * k + yl;
-        if (xin && y >= r.y0 && y <= r.y1) map_texel(r, dx, (float)y + 0.5f, fix, a[k][0], a[k][1], a[k][2]);
-      }
+    for (int k = 0; k < 4; k++) {
+      if (!((mask >> k) & 1u)) continue;
+      int cx, cy;
+      map_unit_texel<LAY>(lane, k, cx, cy);
+      const int x = tx0 + cx, y = ty0 + cy;
+      if (x >= r.x0 && x <= r.x1 && y >= r.y0 && y <= r.y1)
+        map_texel(r, ((float)x + 0.5f) - r.ctx, (float)y + 0.5f, fix, a[k][0], a[k][1], a[k][2]);
     }
   }
 #pragma unroll
-  for (int k = 0; k < 4; k++)
+  for (int k = 0; k < 4; k++) {
+    int cx, cy;
+    map_unit_texel<LAY>(lane, k, cx, cy);
 #pragma unroll
-    for (int c = 0; c < 3; c++)
-      if (a[k][c]) atomicAdd(&s_acc[c * 256 + 64 * k + lane], a[k][c]);
+    for (int c = 0; c < 3; c++) atomicAdd(&s_acc[c * 256 + cy * 16 + cx], a[k][c]);
+  }
+}
+
+// Per-tile sums of the records list[s0, s1) of the 16x16 tile at texel (tx0, ty0). Wave w takes records w, w + 4, ...
+// of the run and stages MAP_WSTAGE of them at a time in its own part of LDS (one load per lane), each with its layout
+// and unit mask for this tile (in place of its layer, which the sums do not read), grouped by layout (slots ranked
+// by ballot); each group is then evaluated in its layout (map_group_sums). A wave reads only what it staged, so the
+// rounds need no workgroup barrier. On return s_acc[c][t] holds texel t's sum c (t = row * 16 + column) and the
+// caller clears it after reading.
+constexpr uint32_t MAP_WSTAGE = MAP_STAGE / 4u;
+__device__ __forceinline__ void map_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+__device__ __forceinline__ void map_tile_sums(const MapRec *__restrict__ recs, const uint32_t *__restrict__ list,
+                                              uint32_t s0, uint32_t s1, int tx0, int ty0, float fix, MapRec *s_rec,
+                                              uint32_t *s_acc) {
+  static_assert(MAP_WSTAGE <= 64u, "one record per lane");
+  const uint32_t lane = threadIdx.x & 63u, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  MapRec *w_rec = s_rec + wave * MAP_WSTAGE;
+  const uint32_t n = s1 - s0, mine = n > wave ? (n - wave + 3u) / 4u : 0u;
+  __syncthreads();                                   // the last tile's sums are read and cleared
+  for (uint32_t b = 0; b < mine; b += MAP_WSTAGE) {
+    const uint32_t nb = min(MAP_WSTAGE, mine - b);
+    uint32_t lay = 3u;
+    MapRec r;
+    if (lane < nb) {
+      r = recs[list[s0 + wave + 4u * (b + lane)]];
+      const uint32_t m = map_unit_mask(r, tx0, ty0);
+      lay = m >> 4;
+      r.layer = m & 15u;
+    }
+    const uint64_t b0 = __ballot(lay == 0u), b1 = __ballot(lay == 1u), b2 = __ballot(lay == 2u);
+    const uint32_t c0 = (uint32_t)__popcll(b0), c1 = (uint32_t)__popcll(b1);
+    if (lane < nb) {
+      const uint64_t bm = lay == 0u ? b0 : lay == 1u ? b1 : b2;
+      const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(bm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bm, 0u));
+      w_rec[(lay >= 1u ? c0 : 0u) + (lay >= 2u ? c1 : 0u) + rank] = r;
+    }
+    map_wave_sync();
+    map_group_sums<0>(w_rec, 0u, c0, lane, tx0, ty0, fix, s_acc);
+    map_group_sums<1>(w_rec, c0, c0 + c1, lane, tx0, ty0, fix, s_acc);
+    map_group_sums<2>(w_rec, c0 + c1, nb, lane, tx0, ty0, fix, s_acc);
+    map_wave_sync();                                 // read before the next round overwrites them
+  }
   __syncthreads();
 }
 
@@ -838,7 +903,7 @@ __device__ __forceinline__ void map_tile_sums(const MapRec *__restrict__ recs, c
 // entry). LDS is under 20 KiB per workgroup, so 8 workgroups (32 waves) fit a CU's 160 KiB. KIND: 0 the emitter
 // map (fixed point x65536, light_shadow.wgsl), 1 the sun maps (x4096, sun_map.wgsl), one instance each.
 template <uint32_t KIND>
-__global__ void __launch_bounds__(256, 8)
+__global__ void __launch_bounds__(256, 6)
 sl_map_accumulate(const MapRec *__restrict__ recs, const uint32_t *__restrict__ keys,
                   const uint32_t *__restrict__ list, const FrameCounters *counters, uint32_t res, uint32_t tiles_x,
                   uint32_t tpl, uint32_t *__restrict__ acc, uint32_t *__restrict__ stats) {

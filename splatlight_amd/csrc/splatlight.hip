@@ -105,6 +105,8 @@ struct sl_scene {
     uint32_t node_count = 0, kid_count = 0, max_depth = 0, n_probes = 0;
     uint32_t *kids = nullptr, *up_nodes = nullptr, *node_sh = nullptr;
     std::vector<uint32_t> lv_off, lv_cnt;     // node_up's lists per depth
+    UpTail tail{};                            // depths 1..tail_hi (0: none), each within UP_TAIL_MAX nodes
+    uint32_t tail_hi = 0;
     float4 *lo_dir = nullptr, *lo_b = nullptr, *node_acc = nullptr, *probe_data = nullptr, *probe_sh = nullptr;
     float4 *xfer = nullptr, *c0 = nullptr, *cells = nullptr;
     void *pp = nullptr;                        // ProbeParams
@@ -1727,10 +1729,18 @@ extern "C" sl_status sl_bounce_init(sl_context *ctx, sl_scene *sc, const sl_boun
   sl_scene::Bounce &B = sc->bounce;
   B.walk_stack = walk_stack;
   std::vector<uint32_t> up;
+  B.lv_off.clear();
+  B.lv_cnt.clear();
   for (uint32_t d = 0; d <= t->max_depth; d++) {
     B.lv_off.push_back((uint32_t)up.size());
     B.lv_cnt.push_back((uint32_t)by[d].size());
     up.insert(up.end(), by[d].begin(), by[d].end());
+  }
+  B.tail_hi = 0;
+  while (B.tail_hi + 1u < t->max_depth && B.tail_hi + 1u <= UP_TAIL_DEPTHS && B.lv_cnt[B.tail_hi + 1u] <= UP_TAIL_MAX) {
+    B.tail_hi++;
+    B.tail.off[B.tail_hi] = B.lv_off[B.tail_hi];
+    B.tail.cnt[B.tail_hi] = B.lv_cnt[B.tail_hi];
   }
   // the 128 cells' directions and solid angles, in double rounded once (bounce.wgsl cell_dir / cell_omega)
   float cells[128][4];
@@ -1828,13 +1838,17 @@ extern "C" sl_status sl_bounce_encode(sl_context *ctx, sl_scene *sc, const void 
       B.w_version = sc->parts_version;
     }
     sl_leaf_sh<<<gblocks, 64, 0, st>>>(G, PPd, sc->g[2], sc->storage, B.wcache, B.lo_dir, B.lo_b, B.node_sh);
-    for (int d = (int)B.max_depth - 1; d >= 1; d--) {
+    // the deep depths one launch each; the shallow ones (depth 1 up to tail_hi) in one workgroup
+    for (int d = (int)B.max_depth - 1; d > (int)B.tail_hi; d--) {
       const uint32_t cnt = B.lv_cnt[d];
       if (cnt)
         sl_node_up_sh<<<(cnt + 63) / 64, 64, 0, st>>>(G, B.node_count, PPd, sc->g[2], sc->storage, sc->nodes, B.kids,
                                                        B.up_nodes, (uint32_t)d, B.lv_off[d], cnt, B.wcache, B.lo_dir,
                                                        B.lo_b, B.node_acc, B.node_sh);
     }
+    if (B.tail_hi >= 1u)
+      sl_node_up_tail<<<1, 1024, 0, st>>>(G, B.node_count, PPd, sc->g[2], sc->storage, sc->nodes, B.kids, B.up_nodes,
+                                          B.tail, B.tail_hi, B.wcache, B.lo_dir, B.lo_b, B.node_acc, B.node_sh);
     if (timers) SL_HIP(ctx, hipEventRecord(ctx->bev[1], st));
     const uint32_t rays = (NP * 128u + PP.subset - 1u) / PP.subset;
     if (B.walk_stack <= 32u)
@@ -1844,8 +1858,7 @@ extern "C" sl_status sl_bounce_encode(sl_context *ctx, sl_scene *sc, const void 
       sl_probe_gather<64><<<(rays + 63) / 64, 64, 0, st>>>(U, PPd, B.ENV, sc->nodes, sc->vgrid, sc->parts, B.walk,
                                                            B.walk_root, B.node_sh, B.cells, B.probe_data, rays);
     if (timers) SL_HIP(ctx, hipEventRecord(ctx->bev[2], st));
-    sl_probe_valid<<<(NP + 63) / 64, 64, 0, st>>>(U, PPd, sc->parts, B.probe_data);
-    sl_probe_sh2<<<(NP + 63) / 64, 64, 0, st>>>(PPd, B.probe_data, B.cells, B.probe_sh);
+    sl_probe_finish<<<NP, 64, 0, st>>>(U, PPd, sc->parts, B.cells, B.probe_data, (float *)B.probe_sh);
   } else if (timers) {
     SL_HIP(ctx, hipEventRecord(ctx->bev[1], st));
     SL_HIP(ctx, hipEventRecord(ctx->bev[2], st));
