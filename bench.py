@@ -414,9 +414,9 @@ def stats_summary(csv_path, out, command):
              "sl_shade_emit, sl_env_direct and the frame path; the cached-lighting frames add frame-path launches "
              "only. The emitter-map chain and the sun-map chain run on two streams at once, so kernel times include "
              "sharing the GPU. The bounce leg's frames add the sky and bounce through probes (tree of 2M nodes, "
-             "1,339 probes x 128 rays): sl_bounce_lo_dir, sl_leaf_sh, sl_node_up_sh (one launch per tree depth), "
-             "sl_probe_gather, sl_probe_valid, sl_probe_sh2, sl_bounce_sh each frame, sl_iso_xfer and "
-             "sl_leaf_weights once.", "",
+             "1,339 probes x 128 rays): sl_bounce_lo_dir, sl_leaf_sh, sl_node_up_sh (one launch per deep tree "
+             "depth), sl_node_up_tail (the shallow depths), sl_probe_gather, sl_probe_finish, sl_bounce_sh each "
+             "frame, sl_iso_xfer and sl_leaf_weights once.", "",
              "| kernel | calls | avg us | min us | max us | share |", "|---|---|---|---|---|---|"]
     for r in rows:
         lines.append(f"| {short_name(r['Name'])} | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "

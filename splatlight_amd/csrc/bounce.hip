@@ -314,37 +314,53 @@ sl_leaf_sh(uint32_t G, const ProbeParamsDev *__restrict__ PP, const float4 *__re
   sh2_pack(PP, s, node_sh + 14 * (size_t)g);
 }
 
-// node_up + node_sh2 of node i when it sits at this depth: sums its children in child order, stores its sums and SH2
-__device__ __forceinline__ void node_up_one(uint32_t G, uint32_t N, const ProbeParamsDev *__restrict__ PP,
-                                            const float4 *__restrict__ g2, uint32_t st, const float4 *__restrict__ nodes,
-                                            const uint32_t *__restrict__ kids, uint32_t i, uint32_t depth,
-                                            const float *__restrict__ wcache, const float4 *__restrict__ lo_dir,
-                                            const float4 *__restrict__ lo_b, float4 *__restrict__ node_acc,
-                                            uint32_t *__restrict__ node_sh) {
-  if (i < G || i >= N) return;
-  const float4 *n = nodes + 6 * (size_t)i;
-  const uint32_t part = __float_as_uint(n[3].w), clo = __float_as_uint(n[4].w);
-  const float4 n5 = n[5];
-  const uint32_t chi = __float_as_uint(n5.x), dep = __float_as_uint(n5.y);
-  if (dep != depth || part == NONE) return;
+// One child's terms toward its parent's 14 sums: a leaf's light times its weights (node_leaf), or an internal node's
+// stored sums
+__device__ __forceinline__ void kid_terms(uint32_t G, uint32_t N, const ProbeParamsDev *__restrict__ PP,
+                                          const float4 *__restrict__ g2, uint32_t st, uint32_t c,
+                                          const float *__restrict__ wcache, const float4 *__restrict__ lo_dir,
+                                          const float4 *__restrict__ lo_b, const float4 *__restrict__ node_acc,
+                                          float4 (&x)[14]) {
+  if (c < G) {
+    const v3 L = leaf_light(PP, g2, st, lo_dir, lo_b, c);
+#pragma unroll
+    for (int k = 0; k < 14; k++) {
+      const float w = wcache[14 * (size_t)c + k];
+      x[k] = make_float4(L.x * w, L.y * w, L.z * w, w);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 14; k++) x[k] = node_acc[(size_t)k * N + c];
+  }
+}
+
+// node_up + node_sh2 of one entry of a depth's list, e = (node, child count, first child, second child) — with more
+// than two children e.z is the node's first index into kids: the node's sums are its children's terms added in child
+// order from zero, then stored with its SH2. Two children (the binary trees' case) load both at once.
+__device__ __forceinline__ void node_up_ent(uint32_t G, uint32_t N, const ProbeParamsDev *__restrict__ PP,
+                                            const float4 *__restrict__ g2, uint32_t st, const uint32_t *__restrict__ kids,
+                                            uint4 e, const float *__restrict__ wcache,
+                                            const float4 *__restrict__ lo_dir, const float4 *__restrict__ lo_b,
+                                            float4 *__restrict__ node_acc, uint32_t *__restrict__ node_sh) {
+  const uint32_t i = e.x, nk = e.y;
   float4 s[14];
+  if (nk == 2u) {
+    float4 x0[14], x1[14];
+    kid_terms(G, N, PP, g2, st, e.z, wcache, lo_dir, lo_b, node_acc, x0);
+    kid_terms(G, N, PP, g2, st, e.w, wcache, lo_dir, lo_b, node_acc, x1);
 #pragma unroll
-  for (int k = 0; k < 14; k++) s[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  for (uint32_t j = clo; j < chi; j++) {
-    const uint32_t c = kids[j];
-    if (c < G) {
-      const v3 Lc = leaf_light(PP, g2, st, lo_dir, lo_b, c);
+    for (int k = 0; k < 14; k++)
+      s[k] = make_float4((0.0f + x0[k].x) + x1[k].x, (0.0f + x0[k].y) + x1[k].y, (0.0f + x0[k].z) + x1[k].z,
+                         (0.0f + x0[k].w) + x1[k].w);
+  } else {
 #pragma unroll
-      for (int k = 0; k < 14; k++) {
-        const float w = wcache[14 * (size_t)c + k];
-        s[k] = make_float4(s[k].x + Lc.x * w, s[k].y + Lc.y * w, s[k].z + Lc.z * w, s[k].w + w);
-      }
-    } else {
+    for (int k = 0; k < 14; k++) s[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (uint32_t j = 0; j < nk; j++) {
+      float4 x[14];
+      kid_terms(G, N, PP, g2, st, nk == 1u ? e.z : kids[e.z + j], wcache, lo_dir, lo_b, node_acc, x);
 #pragma unroll
-      for (int k = 0; k < 14; k++) {
-        const float4 a = node_acc[(size_t)k * N + c];
-        s[k] = make_float4(s[k].x + a.x, s[k].y + a.y, s[k].z + a.z, s[k].w + a.w);
-      }
+      for (int k = 0; k < 14; k++)
+        s[k] = make_float4(s[k].x + x[k].x, s[k].y + x[k].y, s[k].z + x[k].z, s[k].w + x[k].w);
     }
   }
 #pragma unroll
@@ -352,34 +368,33 @@ __device__ __forceinline__ void node_up_one(uint32_t G, uint32_t N, const ProbeP
   sh2_pack(PP, s, node_sh + 14 * (size_t)i);
 }
 
-// node_up + node_sh2 for one depth (up_nodes[r0, r0 + count)), one lane per node
+// node_up + node_sh2 for one depth (entries up_ent[r0, r0 + count)), one lane per node
 extern "C" __global__ void __launch_bounds__(64)
 sl_node_up_sh(uint32_t G, uint32_t N, const ProbeParamsDev *__restrict__ PP, const float4 *__restrict__ g2, uint32_t st,
-              const float4 *__restrict__ nodes, const uint32_t *__restrict__ kids, const uint32_t *__restrict__ up_nodes,
-              uint32_t depth, uint32_t r0, uint32_t count, const float *__restrict__ wcache,
-              const float4 *__restrict__ lo_dir, const float4 *__restrict__ lo_b, float4 *__restrict__ node_acc,
-              uint32_t *__restrict__ node_sh) {
+              const uint32_t *__restrict__ kids, const uint4 *__restrict__ up_ent, uint32_t r0, uint32_t count,
+              const float *__restrict__ wcache, const float4 *__restrict__ lo_dir, const float4 *__restrict__ lo_b,
+              float4 *__restrict__ node_acc, uint32_t *__restrict__ node_sh) {
   const uint32_t t = blockIdx.x * 64u + threadIdx.x;
   if (t >= count) return;
-  node_up_one(G, N, PP, g2, st, nodes, kids, up_nodes[r0 + t], depth, wcache, lo_dir, lo_b, node_acc, node_sh);
+  node_up_ent(G, N, PP, g2, st, kids, up_ent[r0 + t], wcache, lo_dir, lo_b, node_acc, node_sh);
 }
 
 // The shallow depths (depth d_hi up to 1, each of at most UP_TAIL_MAX nodes) in one workgroup, deepest first, with a
 // workgroup barrier between depths: a depth reads the sums its children's depth stored, and all lanes of a workgroup
-// share one CU's vector cache, so workgroup-scope ordering makes them visible. One launch in place of one per depth.
-constexpr uint32_t UP_TAIL_MAX = 2048u, UP_TAIL_DEPTHS = 32u;
+// share one CU's vector cache, so workgroup-scope ordering makes them visible. One launch in place of one per depth;
+// each depth costs about two dependent memory round trips (its entries, then its children's terms).
+constexpr uint32_t UP_TAIL_MAX = 2048u, UP_TAIL_DEPTHS = 32u, UP_TAIL_THREADS = 512u;
 struct UpTail {
   uint32_t off[UP_TAIL_DEPTHS + 1], cnt[UP_TAIL_DEPTHS + 1];
 };
-extern "C" __global__ void __launch_bounds__(1024)
+extern "C" __global__ void __launch_bounds__(UP_TAIL_THREADS)
 sl_node_up_tail(uint32_t G, uint32_t N, const ProbeParamsDev *__restrict__ PP, const float4 *__restrict__ g2,
-                uint32_t st, const float4 *__restrict__ nodes, const uint32_t *__restrict__ kids,
-                const uint32_t *__restrict__ up_nodes, UpTail T, uint32_t d_hi, const float *__restrict__ wcache,
-                const float4 *__restrict__ lo_dir, const float4 *__restrict__ lo_b, float4 *__restrict__ node_acc,
-                uint32_t *__restrict__ node_sh) {
+                uint32_t st, const uint32_t *__restrict__ kids, const uint4 *__restrict__ up_ent, UpTail T,
+                uint32_t d_hi, const float *__restrict__ wcache, const float4 *__restrict__ lo_dir,
+                const float4 *__restrict__ lo_b, float4 *__restrict__ node_acc, uint32_t *__restrict__ node_sh) {
   for (uint32_t d = d_hi; d >= 1u; d--) {
-    for (uint32_t t = threadIdx.x; t < T.cnt[d]; t += blockDim.x)
-      node_up_one(G, N, PP, g2, st, nodes, kids, up_nodes[T.off[d] + t], d, wcache, lo_dir, lo_b, node_acc, node_sh);
+    for (uint32_t t = threadIdx.x; t < T.cnt[d]; t += UP_TAIL_THREADS)
+      node_up_ent(G, N, PP, g2, st, kids, up_ent[T.off[d] + t], wcache, lo_dir, lo_b, node_acc, node_sh);
     __syncthreads();
   }
 }

@@ -103,7 +103,8 @@ struct sl_scene {
   struct Bounce {
     bool on = false, lit = false;
     uint32_t node_count = 0, kid_count = 0, max_depth = 0, n_probes = 0;
-    uint32_t *kids = nullptr, *up_nodes = nullptr, *node_sh = nullptr;
+    uint32_t *kids = nullptr, *node_sh = nullptr;
+    uint4 *up_ent = nullptr;                  // node_up's entries, depth by depth: (node, kid count, kid, kid)
     std::vector<uint32_t> lv_off, lv_cnt;     // node_up's lists per depth
     UpTail tail{};                            // depths 1..tail_hi (0: none), each within UP_TAIL_MAX nodes
     uint32_t tail_hi = 0;
@@ -1728,13 +1729,19 @@ extern "C" sl_status sl_bounce_init(sl_context *ctx, sl_scene *sc, const sl_boun
   SL_HIP(ctx, hipStreamSynchronize(ctx->stream));
   sl_scene::Bounce &B = sc->bounce;
   B.walk_stack = walk_stack;
+  // node_up's entries per depth: (node, kid count, first kid, second kid), or (node, kid count, first index into
+  // kids, 0) for more than two kids
   std::vector<uint32_t> up;
   B.lv_off.clear();
   B.lv_cnt.clear();
   for (uint32_t d = 0; d <= t->max_depth; d++) {
-    B.lv_off.push_back((uint32_t)up.size());
+    B.lv_off.push_back((uint32_t)(up.size() / 4));
     B.lv_cnt.push_back((uint32_t)by[d].size());
-    up.insert(up.end(), by[d].begin(), by[d].end());
+    for (uint32_t i : by[d]) {
+      const uint32_t clo = rd_u32(nb + 96 * (size_t)i, 19), nk = rd_u32(nb + 96 * (size_t)i, 20) - clo;
+      const bool two = nk <= 2u;
+      up.insert(up.end(), {i, nk, two && nk > 0u ? t->kids[clo] : clo, two && nk > 1u ? t->kids[clo + 1] : 0u});
+    }
   }
   B.tail_hi = 0;
   while (B.tail_hi + 1u < t->max_depth && B.tail_hi + 1u <= UP_TAIL_DEPTHS && B.lv_cnt[B.tail_hi + 1u] <= UP_TAIL_MAX) {
@@ -1759,7 +1766,7 @@ extern "C" sl_status sl_bounce_init(sl_context *ctx, sl_scene *sc, const sl_boun
   struct Buf { void **dst; size_t bytes; };
   void *nodes_new = nullptr;
   const Buf bufs[] = {{&nodes_new, (size_t)N * 96}, {(void **)&B.kids, (size_t)std::max(t->kid_count, 1u) * 4},
-                      {(void **)&B.up_nodes, std::max<size_t>(up.size(), 1) * 4},
+                      {(void **)&B.up_ent, std::max<size_t>(up.size(), 4) * 4},
                       {(void **)&B.node_acc, (size_t)N * 14 * 16}, {(void **)&B.node_sh, (size_t)N * 14 * 4},
                       {(void **)&B.probe_data, (size_t)NP * 128 * 32}, {(void **)&B.probe_sh, (size_t)NP * 9 * 16},
                       {(void **)&B.lo_dir, (size_t)G * 16}, {(void **)&B.lo_b, (size_t)G * 48},
@@ -1776,7 +1783,7 @@ extern "C" sl_status sl_bounce_init(sl_context *ctx, sl_scene *sc, const sl_boun
   SL_HIP(ctx, hipMemcpyAsync(nodes_new, t->nodes, (size_t)N * 96, hipMemcpyHostToDevice, ctx->stream));
   SL_HIP(ctx, hipMemcpyAsync(B.kids, t->kids, (size_t)t->kid_count * 4, hipMemcpyHostToDevice, ctx->stream));
   if (!up.empty())
-    SL_HIP(ctx, hipMemcpyAsync(B.up_nodes, up.data(), up.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    SL_HIP(ctx, hipMemcpyAsync(B.up_ent, up.data(), up.size() * 4, hipMemcpyHostToDevice, ctx->stream));
   SL_HIP(ctx, hipMemcpyAsync(B.cells, cells, sizeof(cells), hipMemcpyHostToDevice, ctx->stream));
   SL_HIP(ctx, hipMemcpyAsync(B.pp, &PP, sizeof(PP), hipMemcpyHostToDevice, ctx->stream));
   if (!walk.empty())
@@ -1842,13 +1849,14 @@ extern "C" sl_status sl_bounce_encode(sl_context *ctx, sl_scene *sc, const void 
     for (int d = (int)B.max_depth - 1; d > (int)B.tail_hi; d--) {
       const uint32_t cnt = B.lv_cnt[d];
       if (cnt)
-        sl_node_up_sh<<<(cnt + 63) / 64, 64, 0, st>>>(G, B.node_count, PPd, sc->g[2], sc->storage, sc->nodes, B.kids,
-                                                       B.up_nodes, (uint32_t)d, B.lv_off[d], cnt, B.wcache, B.lo_dir,
-                                                       B.lo_b, B.node_acc, B.node_sh);
+        sl_node_up_sh<<<(cnt + 63) / 64, 64, 0, st>>>(G, B.node_count, PPd, sc->g[2], sc->storage, B.kids, B.up_ent,
+                                                       B.lv_off[d], cnt, B.wcache, B.lo_dir, B.lo_b, B.node_acc,
+                                                       B.node_sh);
     }
     if (B.tail_hi >= 1u)
-      sl_node_up_tail<<<1, 1024, 0, st>>>(G, B.node_count, PPd, sc->g[2], sc->storage, sc->nodes, B.kids, B.up_nodes,
-                                          B.tail, B.tail_hi, B.wcache, B.lo_dir, B.lo_b, B.node_acc, B.node_sh);
+      sl_node_up_tail<<<1, UP_TAIL_THREADS, 0, st>>>(G, B.node_count, PPd, sc->g[2], sc->storage, B.kids, B.up_ent,
+                                                     B.tail, B.tail_hi, B.wcache, B.lo_dir, B.lo_b, B.node_acc,
+                                                     B.node_sh);
     if (timers) SL_HIP(ctx, hipEventRecord(ctx->bev[1], st));
     const uint32_t rays = (NP * 128u + PP.subset - 1u) / PP.subset;
     if (B.walk_stack <= 32u)
