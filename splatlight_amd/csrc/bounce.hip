@@ -415,6 +415,11 @@ sl_node_up_tail(uint32_t G, uint32_t N, const ProbeParamsDev *__restrict__ PP, c
 // The walk reads a compact copy of each part's tree (built at set-up): nodes renumbered breadth first per part, so a
 // node's children are consecutive and in kids order, 32 B a node: (mean, radius), (smax, first child, child count,
 // node id). A visit is one 32 B read; only a used node reads its full record (by node id).
+// A used node costs some six times a visit (its footprint, segment geometry and the list insertion), and the lanes of
+// a wave use and open nodes at different steps, so the walk queues each lane's used nodes (GQ at most, in LDS) and
+// the wave then takes them all, each lane in its own walk order: the hit list sees the same insertions in the same
+// order, while both kinds of step run with most lanes busy.
+constexpr uint32_t GQ = 64;
 template <uint32_t SL>
 __global__ void __launch_bounds__(64)
 sl_probe_gather(FrameUniformsDev U, const ProbeParamsDev *__restrict__ PP, EnvDev ENV, const float4 *__restrict__ nodes,
@@ -422,6 +427,7 @@ sl_probe_gather(FrameUniformsDev U, const ProbeParamsDev *__restrict__ PP, EnvDe
                 const uint32_t *__restrict__ walk_root, const uint32_t *__restrict__ node_sh,
                 const float4 *__restrict__ cells, float4 *__restrict__ probe_data, uint32_t n_threads) {
   __shared__ uint32_t s_stack[SL][64];
+  __shared__ uint32_t s_queue[GQ][64];
   const uint32_t lane = threadIdx.x;
   const uint32_t t = (gridDim.x - 1u - blockIdx.x) * 64u + lane;
   if (t >= n_threads) return;
@@ -459,20 +465,37 @@ sl_probe_gather(FrameUniformsDev U, const ProbeParamsDev *__restrict__ PP, EnvDe
     const v3 Po = xf_point_inv(xf, o), wr = xf_dir_inv(xf, w), cr = xf_point_inv(xf, cen);
     s_stack[0][lane] = walk_root[q];
     uint32_t sp = 1;
-    while (sp > 0u) {
-      sp -= 1u;
-      const uint32_t wi = s_stack[sp][lane];
-      const float4 n0 = walk[2 * (size_t)wi];
-      const v3 mean = xyz(n0);
-      const v3 d = sub3(mean, Po);
-      const float tt = clampb(dot3(d, wr), 0.0f, t1);
-      const v3 e = sub3(d, mul3s(wr, tt));
-      if (dot3(e, e) > n0.w * n0.w) continue;
-      const float4 wb = walk[2 * (size_t)wi + 1];
-      const uint32_t ni = __float_as_uint(wb.w);
-      bool use = ni < G;
-      if (!use) use = len3(sub3(cr, mean)) - brad > theta * wb.x;
-      if (use) {
+    for (;;) {
+      // the walk: until this lane's stack is empty or its queue of used nodes is full
+      uint32_t qn = 0;
+      while (sp > 0u && qn < GQ) {
+        sp -= 1u;
+        const uint32_t wi = s_stack[sp][lane];
+        const float4 n0 = walk[2 * (size_t)wi];
+        const v3 mean = xyz(n0);
+        const v3 d = sub3(mean, Po);
+        const float tt = clampb(dot3(d, wr), 0.0f, t1);
+        const v3 e = sub3(d, mul3s(wr, tt));
+        if (dot3(e, e) > n0.w * n0.w) continue;
+        const float4 wb = walk[2 * (size_t)wi + 1];
+        const uint32_t ni = __float_as_uint(wb.w);
+        bool use = ni < G;
+        if (!use) use = len3(sub3(cr, mean)) - brad > theta * wb.x;
+        if (use) {
+          s_queue[qn][lane] = wi;
+          qn += 1u;
+        } else {
+          const uint32_t fc = __float_as_uint(wb.y), nk = __float_as_uint(wb.z);
+          for (uint32_t j = 0; j < nk; j++)
+            if (sp < SL) { s_stack[sp][lane] = fc + j; sp += 1u; }
+        }
+      }
+      // the used nodes, in walk order
+      for (uint32_t u = 0; u < qn; u++) {
+        const uint32_t wi = s_queue[u][lane];
+        const v3 mean = xyz(walk[2 * (size_t)wi]);
+        const v3 d = sub3(mean, Po);
+        const uint32_t ni = __float_as_uint(walk[2 * (size_t)wi + 1].w);
         const float4 *n = nodes + 6 * (size_t)ni;
         const float4 n1 = n[1], n2 = n[2], n3 = n[3], n4 = n[4];
         const NodeL nd = {xyz(n1), xyz(n2), xyz(n3), xyz(n4), n1.w};
@@ -506,11 +529,8 @@ sl_probe_gather(FrameUniformsDev U, const ProbeParamsDev *__restrict__ PP, EnvDe
           }
           k = min(k + 1u, HIT_MAX);
         }
-      } else {
-        const uint32_t fc = __float_as_uint(wb.y), nk = __float_as_uint(wb.z);
-        for (uint32_t j = 0; j < nk; j++)
-          if (sp < SL) { s_stack[sp][lane] = fc + j; sp += 1u; }
       }
+      if (!__any(sp > 0u)) break;
     }
   }
   float T = 1.0f, m1 = 0.0f, m2 = 0.0f, first_part = -1.0f, near = 0.0f;
