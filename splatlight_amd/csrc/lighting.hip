@@ -870,17 +870,14 @@ __device__ __forceinline__ void map_tile_sums(const MapRec *__restrict__ recs, c
   __syncthreads();                                   // the last tile's sums are read and cleared
   for (uint32_t b = 0; b < mine; b += MAP_WSTAGE) {
     const uint32_t nb = min(MAP_WSTAGE, mine - b);
-    uint32_t lay = 3u;
-    MapRec r;
-    if (lane < nb) {
-      r = recs[list[s0 + wave + 4u * (b + lane)]];
-      const uint32_t m = map_unit_mask(r, tx0, ty0);
-      lay = m >> 4;
-      r.layer = m & 15u;
-    }
+    const bool act = lane < nb;
+    MapRec r = act ? recs[list[s0 + wave + 4u * (b + lane)]] : empty_rec();
+    const uint32_t m = act ? map_unit_mask(r, tx0, ty0) : 0u;
+    const uint32_t lay = act ? m >> 4 : 3u;
+    r.layer = m & 15u;
     const uint64_t b0 = __ballot(lay == 0u), b1 = __ballot(lay == 1u), b2 = __ballot(lay == 2u);
     const uint32_t c0 = (uint32_t)__popcll(b0), c1 = (uint32_t)__popcll(b1);
-    if (lane < nb) {
+    if (act) {
       const uint64_t bm = lay == 0u ? b0 : lay == 1u ? b1 : b2;
       const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(bm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bm, 0u));
       w_rec[(lay >= 1u ? c0 : 0u) + (lay >= 2u ? c1 : 0u) + rank] = r;
