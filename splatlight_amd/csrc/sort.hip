@@ -205,3 +205,13 @@ extern "C" __global__ void __launch_bounds__(256)
 sl_fill_u32(uint32_t *__restrict__ p, uint32_t value, uint32_t n) {
   for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) p[i] = value;
 }
+
+// The same fill with words i0 and i1 of p set to v0 and v1 instead, and the word q (outside p) to qv: one launch for
+// a pass's cleared scratch and its set-up words.
+extern "C" __global__ void __launch_bounds__(256)
+sl_fill_u32_set(uint32_t *__restrict__ p, uint32_t value, uint32_t n, uint32_t i0, uint32_t v0, uint32_t i1,
+                uint32_t v1, uint32_t *__restrict__ q, uint32_t qv) {
+  if (blockIdx.x == 0u && threadIdx.x == 0u) *q = qv;
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u)
+    p[i] = i == i0 ? v0 : (i == i1 ? v1 : value);
+}

@@ -1291,10 +1291,11 @@ void map_bin(sl_context *ctx, hipStream_t st, uint32_t which, size_t n_rec, uint
   uint32_t *w = (uint32_t *)m.scratch.p;
   FrameCounters *counters = (FrameCounters *)(w + ms.counters);
   uint32_t *stats = (uint32_t *)ctx->map_stats.p + 2 * which;
-  sl_fill_u32<<<grid_for(ms.words, 256, 2048), 256, 0, st>>>(w, 0u, (uint32_t)ms.words);
-  sl_fill_u32<<<1, 256, 0, st>>>(&counters->visible, (uint32_t)n_rec, 1u);   // the records are the "ranks"
-  sl_fill_u32<<<1, 256, 0, st>>>(w + ms.tiles_x, sh.tiles_x, 1u);
-  sl_fill_u32<<<1, 256, 0, st>>>(stats + 1, 0u, 1u);
+  // scratch cleared; the records are the "ranks" (counters->visible, word 0), the tile columns, the tile count
+  static_assert(offsetof(FrameCounters, visible) == 0, "visible is the counters' first word");
+  sl_fill_u32_set<<<grid_for(ms.words, 256, 2048), 256, 0, st>>>(w, 0u, (uint32_t)ms.words, (uint32_t)ms.counters,
+                                                                 (uint32_t)n_rec, (uint32_t)ms.tiles_x, sh.tiles_x,
+                                                                 stats + 1, 0u);
   uint32_t *k0 = (uint32_t *)m.k0.p, *v0 = (uint32_t *)m.v0.p;
   uint32_t *k1 = (uint32_t *)m.k1.p, *v1 = (uint32_t *)m.v1.p;
   sl_pairs_emit<true><<<grid_for(n_rec, PE_CHUNK, 2u * ctx->cus), PE_THREADS, 0, st>>>(
@@ -1511,10 +1512,14 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
     SL_HIP(ctx, hipMemcpyAsync(sc->sun_dirty.p, sun_flags.data(), 4 * (size_t)SM.n_layers, hipMemcpyHostToDevice, ss));
     sl_sun_frame<<<(SM.n_layers + 63) / 64, 64, 0, ss>>>(SM, (const SunLayerDesc *)sc->sun_desc.p, sc->parts,
                                                          (SunLayer *)sc->sun_layers.p);
-    for (uint32_t l = 0; l < SM.n_layers; l++)
-      if (sun_flags[l])
-        sl_fill_u32<<<grid_for(texels * 3, 256, 4096), 256, 0, ss>>>((uint32_t *)sc->sun_acc.p + texels * 3 * l, 0u,
-                                                                     (uint32_t)(texels * 3));
+    for (uint32_t l = 0; l < SM.n_layers;) {            // each run of dirty layers cleared by one launch
+      if (!sun_flags[l]) { l++; continue; }
+      uint32_t e = l + 1u;
+      while (e < SM.n_layers && sun_flags[e] && texels * 3 * (e + 1u - l) < (1ull << 31)) e++;
+      sl_fill_u32<<<grid_for(texels * 3 * (e - l), 256, 4096), 256, 0, ss>>>(
+          (uint32_t *)sc->sun_acc.p + texels * 3 * l, 0u, (uint32_t)(texels * 3 * (e - l)));
+      l = e;
+    }
     if (sun_any) {
       if (sun_rec)
         sl_sun_prep<<<gblocks, 64, 0, ss>>>(sc->count, SM, (const SunLayerDesc *)sc->sun_desc.p,
@@ -1543,10 +1548,14 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
     sc->lsm_layer_texels = LS.layer_texels;
     SL_HIP(ctx, hipMemcpyAsync(sc->lsm_dirty.p, ls_flags, sizeof(ls_flags), hipMemcpyHostToDevice, st));
     if (LS.dirty) {
-      for (uint32_t p = 0; p < LS.part_count; p++)
-        if (ls_flags[p])
-          sl_fill_u32<<<grid_for(texels * 3, 256, 4096), 256, 0, st>>>((uint32_t *)sc->lsm_acc.p + texels * 3 * p, 0u,
-                                                                       (uint32_t)(texels * 3));
+      for (uint32_t p = 0; p < LS.part_count;) {        // each run of dirty parts cleared by one launch
+        if (!ls_flags[p]) { p++; continue; }
+        uint32_t e = p + 1u;
+        while (e < LS.part_count && ls_flags[e] && texels * 3 * (e + 1u - p) < (1ull << 31)) e++;
+        sl_fill_u32<<<grid_for(texels * 3 * (e - p), 256, 4096), 256, 0, st>>>(
+            (uint32_t *)sc->lsm_acc.p + texels * 3 * p, 0u, (uint32_t)(texels * 3 * (e - p)));
+        p = e;
+      }
       if (sc->count)
         sl_lsm_prep<<<gblocks, 64, 0, st>>>(sc->count, LS, sc->g[0], sc->g[1], sc->g[2], sc->storage, sc->nodes,
                                             sc->parts, (MapRec *)ctx->map[0].rec.p, (uint2 *)ctx->map[0].tbox.p,
