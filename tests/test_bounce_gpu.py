@@ -30,8 +30,59 @@ def rel(a, b, floor):
     return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), floor))) if a.size else 0.0
 
 
-def setup(ctx, sc):
+def mixed_arity(t, G):
+    """The scene tree with nodes of three or more kids (every 7th internal node takes its first internal child's
+    kids in that child's place; the child is left unreferenced, out of every part) and single-kid nodes (every 5th
+    wraps its first leaf child in a new node holding that leaf's record): the node passes' general path. Depths are
+    recounted from the scene root, which stays the last node."""
+    N, root = t.count, t.scene_root
+    lo, hi = t.nodes["child_lo"], t.nodes["child_hi"]
+    kids = {i: [int(k) for k in t.kids[lo[i]:hi[i]]] for i in range(G, N)}
+    root_kids = kids.pop(root)
+    orphan, wraps = set(), []
+    for n, p in enumerate(i for i in range(G, N) if i != root):
+        if p in orphan:
+            continue
+        ks = kids[p]
+        if n % 7 == 0 and ks and ks[0] >= G and ks[0] not in orphan:
+            orphan.add(ks[0])
+            kids[p] = kids[ks[0]] + ks[1:]
+        elif n % 5 == 0:
+            for j, k in enumerate(ks):
+                if k < G:
+                    y = N - 1 + len(wraps)                # before the scene root, which moves to the end
+                    wraps.append(k)
+                    kids[y] = [k]
+                    ks[j] = y
+                    break
+    W = len(wraps)
+    new_root = N - 1 + W
+    nodes = np.concatenate([t.nodes[:N - 1], t.nodes[np.array(wraps, np.int64)], t.nodes[N - 1:]]) if W else t.nodes.copy()
+    kids[new_root] = root_kids
+    for q in orphan:
+        nodes["part"][q] = tree.NO_PART
+    flat = []
+    for i in range(nodes.shape[0]):
+        ks = kids.get(i, []) if i >= G else []
+        nodes["child_lo"][i], nodes["child_hi"][i] = len(flat), len(flat) + len(ks)
+        flat += ks
+    depth = np.zeros(nodes.shape[0], np.int64)
+    stack = [(new_root, 0)]
+    while stack:
+        i, d = stack.pop()
+        depth[i] = d
+        stack += [(k, d + 1) for k in kids.get(i, []) if i >= G]
+    nodes["depth"] = depth.astype(nodes["depth"].dtype)
+    roots = list(t.roots)                              # part roots keep their ids
+    log(f"    tree reshaped: {len(orphan)} nodes spliced into their parents, {W} single-kid wrappers, "
+        f"kids per node up to {max(len(v) for k, v in kids.items() if k != new_root)}")
+    return tree.Tree(nodes, np.array(flat, np.uint32), roots, new_root, int(depth[:G].max()))
+
+
+def setup(ctx, sc, shape=None):
     t = tree.scene_tree(sc)
+    if shape is not None:
+        t = shape(t, sc.count)
     vg = scenes.vgrid_open(sc.count)
     scene = N.Scene(ctx, sc.gaussians, sc.parts, emitters=sc.emitters, materials=sc.materials,
                     nodes=t.nodes[:sc.count], vgrid=vg)
@@ -52,8 +103,8 @@ def read_state(scene, n_probes, G, N_nodes):
     }
 
 
-def check_frames(ctx, sc, frames=2, moved_mask=0):
-    scene, t, vg, b = setup(ctx, sc)
+def check_frames(ctx, sc, frames=2, moved_mask=0, shape=None):
+    scene, t, vg, b = setup(ctx, sc, shape)
     G, Nn, NP = sc.count, t.count, b.n_probes
     cells = oracle.cells_table(*BN.cell_table())
     up, levels = BN.up_levels(t.nodes, G, t.max_depth)
@@ -169,6 +220,14 @@ def test_bounce_cornell_room(ctx):
     """The room lattice in the Cornell box: emitter light only (no environment)."""
     log("bounce: cornell 20k")
     scene, _ = check_frames(ctx, scenes.cornell(n=20_000))
+    scene.destroy()
+
+
+def test_bounce_tree_of_mixed_arity(ctx):
+    """A tree whose nodes have one, two, three or more kids (the format allows any count): node sums and SH2 bit
+    for bit, and every pass to its bar, as on the binary trees."""
+    log("bounce: hill 12k, tree with single-kid and many-kid nodes")
+    scene, _ = check_frames(ctx, scenes.hill(n=12_000, shadow_res=256), shape=mixed_arity)
     scene.destroy()
 
 
