@@ -442,7 +442,7 @@ sl_status enqueue_batch(sl_context *ctx, const Batch &b, bool timers, bool overl
                d_grid);
   }
   if ((s = mark(3, st)) != SL_OK) return s;
-  sl_pairs_emit<false><<<grid_for(n, PE_CHUNK, 2u * ctx->cus), PE_THREADS, 0, st>>>(
+  sl_pairs_emit<false><<<grid_for(n, PE_CHUNK, 4u * ctx->cus), PE_THREADS, 0, st>>>(
       w.order, w.tbox, nullptr, w.slot_base, w.tile_base, w.tiles_x, V, b.tile_passes, w.pairs_cap, w.counters, w.pk0, w.pv0,
       w.chunk_status, w.tickets + 3, w.tile_hist);
   if ((s = mark(4, st)) != SL_OK) return s;
@@ -1298,7 +1298,7 @@ void map_bin(sl_context *ctx, hipStream_t st, uint32_t which, size_t n_rec, uint
                                                                  stats + 1, 0u);
   uint32_t *k0 = (uint32_t *)m.k0.p, *v0 = (uint32_t *)m.v0.p;
   uint32_t *k1 = (uint32_t *)m.k1.p, *v1 = (uint32_t *)m.v1.p;
-  sl_pairs_emit<true><<<grid_for(n_rec, PE_CHUNK, 2u * ctx->cus), PE_THREADS, 0, st>>>(
+  sl_pairs_emit<true><<<grid_for(n_rec, PE_CHUNK, 4u * ctx->cus), PE_THREADS, 0, st>>>(
       nullptr, (const uint2 *)m.tbox.p, (const uint64_t *)m.tmask.p, w + ms.zeros, w + ms.zeros, w + ms.tiles_x, 1u, sh.tile_passes,
       (uint32_t)sh.cap, counters, k0, v0, (uint64_t *)(w + ms.chunk), w + ms.tickets, w + ms.hist);
   sl_radix_bases<<<sh.tile_passes, RS_THREADS, 0, st>>>(w + ms.hist);
