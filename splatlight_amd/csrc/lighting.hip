@@ -616,9 +616,12 @@ __device__ __forceinline__ void lds_levels(float4 *src, float4 *dst, uint32_t si
     side = ns;
   }
 }
+// touched (when given): one bit per 16x16 tile of the layers, set by the sums for every tile they gave records; a
+// clear bit's tile holds the zero sums its layer was cleared to, whose resolve is zero, so its sums are not read.
 extern "C" __global__ void __launch_bounds__(256)
 sl_opacity_pyramid(const uint32_t *__restrict__ acc, float4 *__restrict__ mip, uint32_t res, uint32_t levels,
-                   LevelOffs lo, uint32_t layer_texels, float fix, const uint32_t *__restrict__ dirty) {
+                   LevelOffs lo, uint32_t layer_texels, float fix, const uint32_t *__restrict__ dirty,
+                   const uint32_t *__restrict__ touched) {
   __shared__ float4 s1[16 * 16];
   __shared__ float4 s2[8 * 8];
   const uint32_t l = blockIdx.y;
@@ -629,8 +632,15 @@ sl_opacity_pyramid(const uint32_t *__restrict__ acc, float4 *__restrict__ mip, u
   const uint32_t *lacc = acc + (size_t)l * res * res * 3;
   float4 *lm = mip + (size_t)l * layer_texels;
   const size_t i00 = (size_t)y * res + x;
-  const float4 a = resolve_texel(lacc, i00, fix), b = resolve_texel(lacc, i00 + 1, fix);
-  const float4 c = resolve_texel(lacc, i00 + res, fix), d = resolve_texel(lacc, i00 + res + 1, fix);
+  bool live = true;                                  // this thread's 2x2 texels lie in one 16x16 tile
+  if (touched != nullptr) {
+    const uint32_t tx = res / 16u, tile = (l * tx + (y >> 4)) * tx + (x >> 4);
+    live = ((touched[tile >> 5] >> (tile & 31u)) & 1u) != 0u;
+  }
+  const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const float4 a = live ? resolve_texel(lacc, i00, fix) : z, b = live ? resolve_texel(lacc, i00 + 1, fix) : z;
+  const float4 c = live ? resolve_texel(lacc, i00 + res, fix) : z;
+  const float4 d = live ? resolve_texel(lacc, i00 + res + 1, fix) : z;
   lm[i00] = a;
   lm[i00 + 1] = b;
   lm[i00 + res] = c;
@@ -903,7 +913,7 @@ template <uint32_t KIND>
 __global__ void __launch_bounds__(256, 6)
 sl_map_accumulate(const MapRec *__restrict__ recs, const uint32_t *__restrict__ keys,
                   const uint32_t *__restrict__ list, const FrameCounters *counters, uint32_t res, uint32_t tiles_x,
-                  uint32_t tpl, uint32_t *__restrict__ acc, uint32_t *__restrict__ stats) {
+                  uint32_t tpl, uint32_t *__restrict__ acc, uint32_t *__restrict__ stats, uint32_t *touched) {
   constexpr float fix = KIND == 0 ? 65536.0f : 4096.0f;
   __shared__ MapRec s_rec[MAP_STAGE];
   __shared__ uint32_t s_keys[MAP_CHUNK];
@@ -928,6 +938,7 @@ sl_map_accumulate(const MapRec *__restrict__ recs, const uint32_t *__restrict__ 
     const uint32_t s1 = lo2;
     const bool starts = s0 > 0u || prev != tile, ends = s1 < n || next != tile;
     used += starts;
+    if (threadIdx.x == 0u) atomicOr(touched + (tile >> 5), 1u << (tile & 31u));
     const uint32_t layer = tile / tpl, rem = tile - layer * tpl, ty = rem / tiles_x, tx = rem - ty * tiles_x;
     map_tile_sums(recs, list, lo + s0, lo + s1, (int)(tx * MT), (int)(ty * MT), fix, s_rec, s_acc);
     // texel threadIdx.x of the tile: its three sums, then cleared for the next tile (read and cleared by this

@@ -183,6 +183,7 @@ struct sl_context {
   } map[2];
   Grow map_stats;
   uint32_t map_records[2] = {0, 0}, map_tiles[2] = {0, 0};
+  const uint32_t *map_touched[2] = {nullptr, nullptr};   // the last sums' tile bits (map scratch)
   // the sun-map chain's stream (created on first use, default priority), the fork/join events that order it with
   // the main stream, and its begin/end timer events
   hipStream_t side = nullptr;
@@ -1240,24 +1241,26 @@ sl_status grow_ctx(sl_context *ctx, Grow *gb, size_t bytes) {
 // words, and the onesweep look-back words of up to 3 passes.
 struct MapScratch {
   size_t counters = 0, tickets = 64, hist = 80, tiles_x = 848, zeros = 849, chunk = 856, tstat[3] = {0, 0, 0};
+  size_t touched = 0;   // one bit per map tile: the sums gave it records (cleared with the rest each binning)
   size_t words = 0, pass_words = 0;
-  MapScratch(size_t n_rec, size_t cap) {
+  MapScratch(size_t n_rec, size_t cap, size_t n_tiles) {
     const size_t chunks = (n_rec + PE_CHUNK - 1) / PE_CHUNK;
     pass_words = (cap + 4095) / 4096 * 256;
     tstat[0] = align_up(chunk + 2 * std::max<size_t>(chunks, 1), 64);
     tstat[1] = tstat[0] + pass_words;
     tstat[2] = tstat[1] + pass_words;
-    words = tstat[2] + pass_words;
+    touched = tstat[2] + pass_words;
+    words = touched + (n_tiles + 31) / 32;
   }
 };
 
 // Buffers of map kind `which` for n_rec records, each touching at most span x span tiles. Called before any launch
 // of the frame's lighting: growing synchronises both streams.
-sl_status ensure_map(sl_context *ctx, uint32_t which, size_t n_rec, uint32_t span) {
+sl_status ensure_map(sl_context *ctx, uint32_t which, size_t n_rec, uint32_t span, size_t n_tiles) {
   sl_status s;
   const size_t cap = std::max<size_t>(n_rec * span * span, 1);
   if (cap >= (1ull << 32)) return fail(ctx, SL_ERR_CAPACITY, "opacity-map splat needs more than 2^32 pairs");
-  const MapScratch ms(n_rec, cap);
+  const MapScratch ms(n_rec, cap, n_tiles);
   sl_context::MapBufs &m = ctx->map[which];
   if ((s = grow_ctx(ctx, &m.rec, std::max<size_t>(n_rec, 1) * sizeof(MapRec))) != SL_OK) return s;
   if ((s = grow_ctx(ctx, &m.tbox, std::max<size_t>(n_rec, 1) * 8)) != SL_OK) return s;
@@ -1287,7 +1290,7 @@ void map_bin(sl_context *ctx, hipStream_t st, uint32_t which, size_t n_rec, uint
              uint32_t span) {
   sl_context::MapBufs &m = ctx->map[which];
   const MapShape sh(n_rec, n_layers, res, span);
-  const MapScratch ms(n_rec, sh.cap);
+  const MapScratch ms(n_rec, sh.cap, sh.n_tiles);
   uint32_t *w = (uint32_t *)m.scratch.p;
   FrameCounters *counters = (FrameCounters *)(w + ms.counters);
   uint32_t *stats = (uint32_t *)ctx->map_stats.p + 2 * which;
@@ -1316,7 +1319,7 @@ void map_sum(sl_context *ctx, hipStream_t st, uint32_t which, size_t n_rec, uint
              uint32_t span, uint32_t *acc, bool timers) {
   sl_context::MapBufs &m = ctx->map[which];
   const MapShape sh(n_rec, n_layers, res, span);
-  const MapScratch ms(n_rec, sh.cap);
+  const MapScratch ms(n_rec, sh.cap, sh.n_tiles);
   uint32_t *w = (uint32_t *)m.scratch.p;
   const FrameCounters *counters = (const FrameCounters *)(w + ms.counters);
   uint32_t *stats = (uint32_t *)ctx->map_stats.p + 2 * which;
@@ -1325,12 +1328,14 @@ void map_sum(sl_context *ctx, hipStream_t st, uint32_t which, size_t n_rec, uint
   const uint32_t *vals = (const uint32_t *)(sh.tile_passes % 2u == 0u ? m.v0.p : m.v1.p);
   if (timers) (void)hipEventRecord(ctx->mev[2 * which], st);
   const uint32_t chunks = (uint32_t)((sh.cap + MAP_CHUNK - 1) / MAP_CHUNK);
+  uint32_t *touched = w + ms.touched;
   if (which == 0)
     sl_map_accumulate<0><<<chunks, 256, 0, st>>>((const MapRec *)m.rec.p, keys, vals, counters, res, sh.tiles_x,
-                                                 sh.tpl, acc, stats);
+                                                 sh.tpl, acc, stats, touched);
   else
     sl_map_accumulate<1><<<chunks, 256, 0, st>>>((const MapRec *)m.rec.p, keys, vals, counters, res, sh.tiles_x,
-                                                 sh.tpl, acc, stats);
+                                                 sh.tpl, acc, stats, touched);
+  ctx->map_touched[which] = touched;
   if (timers) (void)hipEventRecord(ctx->mev[2 * which + 1], st);
   ctx->map_timed[which] = timers;
   ctx->map_records[which] = (uint32_t)n_rec;
@@ -1344,13 +1349,14 @@ uint32_t map_span(uint32_t max_half, uint32_t res) {
 
 void run_opacity_maps(hipStream_t st, float4 *mip, uint32_t n_layers, uint32_t res, uint32_t levels,
                       const uint32_t *level_off, uint32_t layer_texels, float fix, const uint32_t *acc,
-                      const uint32_t *dirty_dev) {
+                      const uint32_t *dirty_dev, const uint32_t *touched) {
   LevelOffs lo = {};
   for (uint32_t k = 0; k < levels && k < 16u; k++) lo.off[k] = level_off[k];
   if (res % 32u == 0u) {
     // levels 0-5 per 32x32 region, then full-size downs while a level is wider than 32, then the rest per layer
     sl_opacity_pyramid<<<dim3((res / 32u) * (res / 32u), n_layers), 256, 0, st>>>(acc, mip, res, levels, lo,
-                                                                                  layer_texels, fix, dirty_dev);
+                                                                                  layer_texels, fix, dirty_dev,
+                                                                                  touched);
     uint32_t k = 6;
     for (; k < levels && (res >> k) > 32u; k++) {
       const uint32_t rk = res >> k;
@@ -1424,7 +1430,7 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
       ls_span = map_span(48, LS.res);
       if ((uint64_t)LS.part_count * tiles_x > 65535u)
         return fail(ctx, SL_ERR_CAPACITY, "emitter map: parts x tile rows over 65535");
-      if ((s = ensure_map(ctx, 0, sc->count, ls_span)) != SL_OK) return s;
+      if ((s = ensure_map(ctx, 0, sc->count, ls_span, (size_t)LS.part_count * tiles_x * tiles_x)) != SL_OK) return s;
     }
   }
   // sun map plan
@@ -1469,7 +1475,7 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
         sun_rec = (size_t)sc->count * max_layers;
         if ((uint64_t)SM.n_layers * tiles_x > 65535u)
           return fail(ctx, SL_ERR_CAPACITY, "sun maps: layers x tile rows over 65535");
-        if ((s = ensure_map(ctx, 1, sun_rec, sun_span)) != SL_OK) return s;
+        if ((s = ensure_map(ctx, 1, sun_rec, sun_span, (size_t)SM.n_layers * tiles_x * tiles_x)) != SL_OK) return s;
       }
       have_sun = true;
     }
@@ -1530,7 +1536,8 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
       map_bin(ctx, ss, 1, sun_rec, SM.n_layers, SM.res, sun_span);
       map_sum(ctx, ss, 1, sun_rec, SM.n_layers, SM.res, sun_span, (uint32_t *)sc->sun_acc.p, timers);
       run_opacity_maps(ss, (float4 *)sc->sun_mip.p, SM.n_layers, SM.res, SM.levels, SM.level_off, SM.layer_texels,
-                       4096.0f, (const uint32_t *)sc->sun_acc.p, (const uint32_t *)sc->sun_dirty.p);
+                       4096.0f, (const uint32_t *)sc->sun_acc.p, (const uint32_t *)sc->sun_dirty.p,
+                       ctx->map_touched[1]);
     }
     sl_sun_lookup<<<gblocks, 64, 0, ss>>>(sc->count, SM, (const SunLayerDesc *)sc->sun_desc.p,
                                           (const SunLayer *)sc->sun_layers.p, sc->g[0], sc->g[2], sc->storage,
@@ -1563,7 +1570,8 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
       map_bin(ctx, st, 0, sc->count, LS.part_count, LS.res, ls_span);
       map_sum(ctx, st, 0, sc->count, LS.part_count, LS.res, ls_span, (uint32_t *)sc->lsm_acc.p, timers);
       run_opacity_maps(st, (float4 *)sc->lsm_mip.p, LS.part_count, LS.res, LS.levels, LS.level_off, LS.layer_texels,
-                       65536.0f, (const uint32_t *)sc->lsm_acc.p, (const uint32_t *)sc->lsm_dirty.p);
+                       65536.0f, (const uint32_t *)sc->lsm_acc.p, (const uint32_t *)sc->lsm_dirty.p,
+                       ctx->map_touched[0]);
     }
     sl_lsm_lookup<<<gblocks, 64, 0, st>>>(sc->count, LS, sc->g[0], sc->g[2], sc->storage, sc->parts,
                                           (const float4 *)sc->lsm_mip.p, sc->lamp_T);
