@@ -2,7 +2,11 @@
  *
  * A plain-C restatement of the reference's WGSL/JS for the path, used by tests/, the smoke check and the
  * CPU-baseline leg of bench.py as the checker. The shipped library never links, loads or calls it.
- * Each function cites the reference file:line it follows; DESIGN.md lists how each one is pinned.
+ * Each function cites the reference file:line it follows. Each is pinned by the CPU tests in tests/test_oracle.py,
+ * tests/test_lighting_oracle.py and tests/test_bounce_oracle.py: the reference's own fixtures where it ships them
+ * (the hill_lite tree), closed forms (a lone splat's coverage, a sky seen through nothing), invariants (sums of
+ * leaves up a tree, order independence of integer sums) and brute force on small inputs, chosen so that a dropped
+ * term or a wrong index fails one of them.
  */
 #ifndef SL_ORACLE_H
 #define SL_ORACLE_H

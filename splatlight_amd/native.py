@@ -1,7 +1,7 @@
 """ctypes binding of libsplatlight_hip.so (include/splatlight.h).
 
-This is the binding a Python host uses; INTEGRATION.md shows the N-API one. The library is the only compute
-path: if it is missing or fails to load this module raises, there is no fallback.
+This is the binding a Python host uses; the JavaScript host's is the N-API addon in splatlight_amd/node/. The
+library is the only compute path: if it is missing or fails to load this module raises, there is no fallback.
 """
 from __future__ import annotations
 
