@@ -187,7 +187,7 @@ struct sl_context {
   // the sun-map chain's stream (created on first use, default priority), the fork/join events that order it with
   // the main stream, and its begin/end timer events
   hipStream_t side = nullptr;
-  hipEvent_t fork = nullptr, join = nullptr, sev[2] = {};
+  hipEvent_t fork = nullptr, join = nullptr, cleared = nullptr, sev[2] = {};
   bool sun_timed = false;
   // the frame geometry's stream (created with the side stream), the event before the last lighting encode that the
   // geometry waits for, the event after it that the context's stream waits for; light_pending: a lighting encode
@@ -644,7 +644,7 @@ extern "C" void sl_ctx_destroy(sl_context *ctx) {
     for (Grow *gb : {&m.rec, &m.tbox, &m.tmask, &m.k0, &m.v0, &m.k1, &m.v1, &m.scratch})
       if (gb->p) (void)hipFree(gb->p);
   if (ctx->map_stats.p) (void)hipFree(ctx->map_stats.p);
-  for (hipEvent_t e : {ctx->fork, ctx->join, ctx->sev[0], ctx->sev[1], ctx->pre_light, ctx->geo_done})
+  for (hipEvent_t e : {ctx->fork, ctx->join, ctx->cleared, ctx->sev[0], ctx->sev[1], ctx->pre_light, ctx->geo_done})
     if (e) (void)hipEventDestroy(e);
   if (ctx->side) (void)hipStreamDestroy(ctx->side);
   if (ctx->geo) (void)hipStreamDestroy(ctx->geo);
@@ -1286,19 +1286,29 @@ struct MapShape {
   }
 };
 
+// The binning scratch of map kind `which` cleared, with the records as the "ranks" (counters->visible, word 0),
+// the tile columns and the tile count set, on stream st.
+void map_clear(sl_context *ctx, hipStream_t st, uint32_t which, size_t n_rec, uint32_t n_layers, uint32_t res,
+               uint32_t span) {
+  const MapShape sh(n_rec, n_layers, res, span);
+  const MapScratch ms(n_rec, sh.cap, sh.n_tiles);
+  uint32_t *w = (uint32_t *)ctx->map[which].scratch.p;
+  uint32_t *stats = (uint32_t *)ctx->map_stats.p + 2 * which;
+  static_assert(offsetof(FrameCounters, visible) == 0, "visible is the counters' first word");
+  sl_fill_u32_set<<<grid_for(ms.words, 256, 2048), 256, 0, st>>>(w, 0u, (uint32_t)ms.words, (uint32_t)ms.counters,
+                                                                 (uint32_t)n_rec, (uint32_t)ms.tiles_x, sh.tiles_x,
+                                                                 stats + 1, 0u);
+}
+
+// Binning of map kind `which` (its scratch cleared here unless `cleared`: map_clear already queued before it).
 void map_bin(sl_context *ctx, hipStream_t st, uint32_t which, size_t n_rec, uint32_t n_layers, uint32_t res,
-             uint32_t span) {
+             uint32_t span, bool cleared = false) {
   sl_context::MapBufs &m = ctx->map[which];
   const MapShape sh(n_rec, n_layers, res, span);
   const MapScratch ms(n_rec, sh.cap, sh.n_tiles);
   uint32_t *w = (uint32_t *)m.scratch.p;
   FrameCounters *counters = (FrameCounters *)(w + ms.counters);
-  uint32_t *stats = (uint32_t *)ctx->map_stats.p + 2 * which;
-  // scratch cleared; the records are the "ranks" (counters->visible, word 0), the tile columns, the tile count
-  static_assert(offsetof(FrameCounters, visible) == 0, "visible is the counters' first word");
-  sl_fill_u32_set<<<grid_for(ms.words, 256, 2048), 256, 0, st>>>(w, 0u, (uint32_t)ms.words, (uint32_t)ms.counters,
-                                                                 (uint32_t)n_rec, (uint32_t)ms.tiles_x, sh.tiles_x,
-                                                                 stats + 1, 0u);
+  if (!cleared) map_clear(ctx, st, which, n_rec, n_layers, res, span);
   uint32_t *k0 = (uint32_t *)m.k0.p, *v0 = (uint32_t *)m.v0.p;
   uint32_t *k1 = (uint32_t *)m.k1.p, *v1 = (uint32_t *)m.v1.p;
   sl_pairs_emit<true><<<grid_for(n_rec, PE_CHUNK, 4u * ctx->cus), PE_THREADS, 0, st>>>(
@@ -1489,6 +1499,7 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
     SL_HIP(ctx, hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
     SL_HIP(ctx, hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming));
     SL_HIP(ctx, hipEventCreateWithFlags(&ctx->join, hipEventDisableTiming));
+    SL_HIP(ctx, hipEventCreateWithFlags(&ctx->cleared, hipEventDisableTiming));
     for (hipEvent_t &e : ctx->sev) SL_HIP(ctx, hipEventCreate(&e));
   }
 
@@ -1516,16 +1527,23 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
     sc->sun_layer_texels = SM.layer_texels;
     SL_HIP(ctx, hipMemcpyAsync(sc->sun_desc.p, desc.data(), sizeof(SunLayerDesc) * SM.n_layers, hipMemcpyHostToDevice, ss));
     SL_HIP(ctx, hipMemcpyAsync(sc->sun_dirty.p, sun_flags.data(), 4 * (size_t)SM.n_layers, hipMemcpyHostToDevice, ss));
-    sl_sun_frame<<<(SM.n_layers + 63) / 64, 64, 0, ss>>>(SM, (const SunLayerDesc *)sc->sun_desc.p, sc->parts,
-                                                         (SunLayer *)sc->sun_layers.p);
+    // the dirty layers' sums and the binning scratch are cleared on the geometry stream (idle until the frame's
+    // encode), beside the layers' set-up and records: after the fork, so after every reader of the last frame's
+    // maps; the binning waits for them
+    hipStream_t gs = ctx->geo;
+    SL_HIP(ctx, hipStreamWaitEvent(gs, ctx->fork, 0));
     for (uint32_t l = 0; l < SM.n_layers;) {            // each run of dirty layers cleared by one launch
       if (!sun_flags[l]) { l++; continue; }
       uint32_t e = l + 1u;
       while (e < SM.n_layers && sun_flags[e] && texels * 3 * (e + 1u - l) < (1ull << 31)) e++;
-      sl_fill_u32<<<grid_for(texels * 3 * (e - l), 256, 4096), 256, 0, ss>>>(
+      sl_fill_u32<<<grid_for(texels * 3 * (e - l), 256, 4096), 256, 0, gs>>>(
           (uint32_t *)sc->sun_acc.p + texels * 3 * l, 0u, (uint32_t)(texels * 3 * (e - l)));
       l = e;
     }
+    if (sun_any) map_clear(ctx, gs, 1, sun_rec, SM.n_layers, SM.res, sun_span);
+    SL_HIP(ctx, hipEventRecord(ctx->cleared, gs));
+    sl_sun_frame<<<(SM.n_layers + 63) / 64, 64, 0, ss>>>(SM, (const SunLayerDesc *)sc->sun_desc.p, sc->parts,
+                                                         (SunLayer *)sc->sun_layers.p);
     if (sun_any) {
       if (sun_rec)
         sl_sun_prep<<<gblocks, 64, 0, ss>>>(sc->count, SM, (const SunLayerDesc *)sc->sun_desc.p,
@@ -1533,7 +1551,10 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
                                             sc->storage, sc->nodes, sc->parts, max_layers,
                                             (MapRec *)ctx->map[1].rec.p, (uint2 *)ctx->map[1].tbox.p,
                                             (uint64_t *)ctx->map[1].tmask.p);
-      map_bin(ctx, ss, 1, sun_rec, SM.n_layers, SM.res, sun_span);
+    }
+    SL_HIP(ctx, hipStreamWaitEvent(ss, ctx->cleared, 0));
+    if (sun_any) {
+      map_bin(ctx, ss, 1, sun_rec, SM.n_layers, SM.res, sun_span, true);
       map_sum(ctx, ss, 1, sun_rec, SM.n_layers, SM.res, sun_span, (uint32_t *)sc->sun_acc.p, timers);
       run_opacity_maps(ss, (float4 *)sc->sun_mip.p, SM.n_layers, SM.res, SM.levels, SM.level_off, SM.layer_texels,
                        4096.0f, (const uint32_t *)sc->sun_acc.p, (const uint32_t *)sc->sun_dirty.p,
