@@ -96,8 +96,9 @@ def test_hill_lighting_with_sun(ctx):
 
 def test_map_culling_stress(ctx):
     """Needle-thin Gaussians at every angle and Gaussians whose 3-sigma box is clamped to the map's maximum half-size:
-    the cases where the map sums' ellipse masks (which tiles a record may touch, which 16x4 bands of a tile) and the
-    box clamp matter most. The maps must still equal the oracle's bit for bit."""
+    the cases where the map sums' ellipse masks (which tiles a record may touch, which of a tile's 64-texel units),
+    their per-record lane layout (tall boxes take 4x16 columns, wide ones 16x4 bands, squarish ones 8x8 quads) and
+    the box clamp matter most. The maps must still equal the oracle's bit for bit."""
     log("lighting parity: hill n=24000 with needles and clamped giants, sun map 512^2")
     sc = scenes.hill(n=24_000, seed=11, width=320, height=180)
     g = sc.gaussians
