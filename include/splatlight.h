@@ -105,6 +105,20 @@ typedef struct sl_frame_stats {
  * pair count exceeded them. */
 sl_status sl_frame_finish(sl_context *ctx, sl_frame_stats *stats);
 
+/* sl_frame_encode for callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). No counterpart in the reference. It runs the same kernels with the same
+ * results, but its workspace (splats, keys, order, tile boxes, pair lists, counters) lies in a fixed device array of
+ * SL_NOALLOC_SCRATCH_BYTES per device, held from library load to process end, instead of the context's growable
+ * workspace: the call allocates and frees nothing. The frame's slots (count + 1 + emitters) and tiles must fit
+ * beside at least one pair per slot (about 80 B per slot: 200k Gaussians at 640x480 fit), or it returns
+ * SL_ERR_CAPACITY at once; the rest of the array holds the pair lists, and sl_frame_finish returns SL_ERR_CAPACITY
+ * (stats still filled) instead of re-running when the pairs overflow it. Every such call on one device shares that
+ * device's array, which keeps the last call's contents: calls on one device that overlap in time (from different
+ * contexts or threads, or before the previous frame's sl_frame_finish / sl_frame_read) race. */
+#define SL_NOALLOC_SCRATCH_BYTES (64u << 20)
+sl_status sl_frame_encode_noalloc(sl_context *ctx, sl_scene *scene, const void *frame_bytes, size_t frame_len,
+                                  const sl_frame_opts *opts, void *out_device);
+
 /* ---- batches: many views over resident scenes in one launch per stage -------------------------------------
  * The views (each: a scene, its 1200 B RenderFrame, its own target size and format) are laid out back to back in
  * one slot space and one tile space; project, the depth sort (keyed by view, then depth), pair emission, the tile

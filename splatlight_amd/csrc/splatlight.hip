@@ -58,6 +58,7 @@ struct Batch {
   uint32_t depth_passes = 2, tile_passes = 2;
   uint32_t flags = 0;
   bool graph = false;
+  bool fixed = false;   // laid over the device's no-alloc scratch instead of the context's workspace
 };
 
 uint32_t rd_u32(const void *p, int word) { uint32_t v; memcpy(&v, (const char *)p + 4 * word, 4); return v; }
@@ -175,6 +176,11 @@ struct sl_context {
   hipGraphExec_t graph = nullptr;
   std::vector<uint64_t> graph_key;
   sl_tuning tuning = {256u, 16u, 4u, 0u};
+  // the no-alloc frame (sl_frame_encode_noalloc): the device's scratch array's address, looked up once per device
+  // at context creation, and the workspace laid over it for the last such frame
+  uint8_t *fixed_base = nullptr;
+  std::string fixed_err;
+  Workspace fws;
   // opacity-map splat binning, one set per map kind (0 emitter map, 1 sun maps) so the two chains can run at once,
   // shared by every scene lit on this context: records, their tile boxes, the (tile, record) pairs and their sorted
   // copies, a cleared scratch block; per-map stats (two words per kind)
@@ -235,13 +241,38 @@ sl_status dev_alloc(sl_context *ctx, size_t bytes, void **p) {
   return SL_OK;
 }
 
-// Lay out the frame workspace for `slots` splat slots, `pairs` pairs and `tiles` tiles.
-sl_status ensure_workspace(sl_context *ctx, uint32_t slots, uint32_t pairs, uint32_t tiles) {
-  Workspace &w = ctx->ws;
-  if (w.base && slots <= w.slots_cap && pairs <= w.pairs_cap && tiles <= w.tiles_cap) return SL_OK;
-  const uint32_t s_cap = std::max(slots, w.slots_cap);
-  const uint32_t p_cap = std::max(pairs, w.pairs_cap);
-  const uint32_t t_cap = std::max(tiles, w.tiles_cap);
+// The no-alloc frame's workspace (sl_frame_encode_noalloc): one array per device, held until the process ends. In
+// host code `scratch` names a host-side stand-in, never the device memory: its device address comes from
+// hipGetSymbolAddress, per device (each device holds its own copy at its own address).
+static __device__ __align__(256) uint8_t scratch[SL_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_dev[kMaxDevices] = {};
+
+// The device address of `scratch` on `device`, which must be current; looked up once per device.
+uint8_t *scratch_address(int device, std::string *err) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (device < 0 || device >= kMaxDevices) {
+    *err = "device ordinal beyond the no-alloc scratch table";
+    return nullptr;
+  }
+  if (!g_scratch_dev[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();   // leave no sticky error for the context's later launch checks
+      *err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return nullptr;
+    }
+    g_scratch_dev[device] = p;
+  }
+  return (uint8_t *)g_scratch_dev[device];
+}
+
+// Lay the frame workspace for `s_cap` splat slots, `p_cap` pairs and `t_cap` tiles over `base` (nullptr: only
+// measure); returns the bytes it spans.
+size_t layout_workspace(uint8_t *b, uint32_t s_cap, uint32_t p_cap, uint32_t t_cap, Workspace &w) {
   const uint32_t d_tiles = (s_cap + RS_TILE_MIN - 1) / RS_TILE_MIN;
   const uint32_t p_tiles = (p_cap + RS_TILE_MIN - 1) / RS_TILE_MIN;
   const uint32_t chunks = (s_cap + PE_CHUNK - 1) / PE_CHUNK;
@@ -268,22 +299,8 @@ sl_status ensure_workspace(sl_context *ctx, uint32_t slots, uint32_t pairs, uint
   const size_t o_ranges = take((size_t)t_cap * 8);
   const size_t o_torder = take((size_t)t_cap * 4);
   const size_t total = off;
-  void *base = nullptr;
-  if (w.base) {
-    SL_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    if (ctx->geo) SL_HIP(ctx, hipStreamSynchronize(ctx->geo));
-    SL_HIP(ctx, hipFree(w.base));
-    w.base = nullptr;
-  }
-  if (ctx->graph) {   // captured with the old buffers
-    (void)hipGraphExecDestroy(ctx->graph);
-    ctx->graph = nullptr;
-    ctx->graph_key.clear();
-  }
-  sl_status s = dev_alloc(ctx, total, &base);
-  if (s != SL_OK) return s;
-  uint8_t *b = (uint8_t *)base;
-  w.base = base;
+  if (!b) return total;
+  w.base = b;
   w.bytes = total;
   w.slots_cap = s_cap;
   w.pairs_cap = p_cap;
@@ -319,6 +336,36 @@ sl_status ensure_workspace(sl_context *ctx, uint32_t slots, uint32_t pairs, uint
   w.chunk_status = (uint64_t *)(b + o_chunk);
   w.ranges = (uint2 *)(b + o_ranges);
   w.tile_order = (uint32_t *)(b + o_torder);
+  return total;
+}
+
+void drop_graph(sl_context *ctx) {
+  if (ctx->graph) {   // captured with the old buffers
+    (void)hipGraphExecDestroy(ctx->graph);
+    ctx->graph = nullptr;
+    ctx->graph_key.clear();
+  }
+}
+
+// Grow the context's own frame workspace to hold `slots` splat slots, `pairs` pairs and `tiles` tiles.
+sl_status ensure_workspace(sl_context *ctx, uint32_t slots, uint32_t pairs, uint32_t tiles) {
+  Workspace &w = ctx->ws;
+  if (w.base && slots <= w.slots_cap && pairs <= w.pairs_cap && tiles <= w.tiles_cap) return SL_OK;
+  const uint32_t s_cap = std::max(slots, w.slots_cap);
+  const uint32_t p_cap = std::max(pairs, w.pairs_cap);
+  const uint32_t t_cap = std::max(tiles, w.tiles_cap);
+  const size_t total = layout_workspace(nullptr, s_cap, p_cap, t_cap, w);
+  if (w.base) {
+    SL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->geo) SL_HIP(ctx, hipStreamSynchronize(ctx->geo));
+    SL_HIP(ctx, hipFree(w.base));
+    w.base = nullptr;
+  }
+  drop_graph(ctx);
+  void *base = nullptr;
+  sl_status s = dev_alloc(ctx, total, &base);
+  if (s != SL_OK) return s;
+  layout_workspace((uint8_t *)base, s_cap, p_cap, t_cap, w);
   return SL_OK;
 }
 
@@ -396,7 +443,7 @@ uint32_t grid_for(uint64_t n, uint32_t per_block, uint32_t cap) {
 // lighting, then writes the splats' colours and colour max and composites. Otherwise every stage runs on the
 // context's stream.
 sl_status enqueue_batch(sl_context *ctx, const Batch &b, bool timers, bool overlap) {
-  Workspace &w = ctx->ws;
+  Workspace &w = b.fixed ? ctx->fws : ctx->ws;
   const hipStream_t ms = ctx->stream, st = overlap ? ctx->geo : ctx->stream;
   const uint32_t V = (uint32_t)b.views.size();
   auto mark = [&](int k, hipStream_t on) -> sl_status {
@@ -492,7 +539,7 @@ sl_status enqueue_batch(sl_context *ctx, const Batch &b, bool timers, bool overl
 // queue the counters' read-back. After a lighting encode (and without a graph) the geometry stages run beside the
 // lighting: the view table then goes up on the geometry stream, after the work queued before that lighting.
 sl_status launch_batch(sl_context *ctx, const Batch &b) {
-  Workspace &w = ctx->ws;
+  Workspace &w = b.fixed ? ctx->fws : ctx->ws;
   hipStream_t st = ctx->stream;
   const uint32_t V = (uint32_t)b.views.size();
   const bool overlap = !b.graph && ctx->light_pending;
@@ -532,7 +579,7 @@ sl_status launch_batch(sl_context *ctx, const Batch &b) {
       ctx->graph_key = key;
     } else {
       // the same launches: the pair lists end in the buffers the capture left them in
-      Workspace &ww = ctx->ws;
+      Workspace &ww = b.fixed ? ctx->fws : ctx->ws;
       ctx->final_keys = b.tile_passes % 2 ? ww.pk1 : ww.pk0;
       ctx->final_vals = b.tile_passes % 2 ? ww.pv1 : ww.pv0;
     }
@@ -601,6 +648,7 @@ extern "C" sl_status sl_ctx_create(int device_ordinal, sl_context **out) {
   if ((e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking)) != hipSuccess) return bail(e);
   (void)hipDeviceGetAttribute(&ctx->cus, hipDeviceAttributeMultiprocessorCount, device_ordinal);
   if (ctx->cus <= 0) ctx->cus = 256;
+  ctx->fixed_base = scratch_address(device_ordinal, &ctx->fixed_err);   // the device is current
   if ((e = hipHostMalloc((void **)&ctx->counters_host, sizeof(FrameCounters), hipHostMallocDefault)) != hipSuccess)
     return bail(e);
   memset(ctx->counters_host, 0, sizeof(FrameCounters));
@@ -997,13 +1045,16 @@ extern "C" sl_status sl_scene_write_bounce(sl_context *ctx, sl_scene *sc, const 
 
 namespace {
 
-// Decode, validate and lay out a batch; grow the workspace; record it as the context's last batch and launch it.
-sl_status encode_batch(sl_context *ctx, const sl_view *views, uint32_t n_views, uint32_t flags) {
+// Decode, validate and lay out a batch; grow the workspace (fixed: lay it over the device's no-alloc scratch
+// instead, allocating nothing); record it as the context's last batch and launch it.
+sl_status encode_batch(sl_context *ctx, const sl_view *views, uint32_t n_views, uint32_t flags, bool fixed = false) {
   if (!views || n_views == 0) return fail(ctx, SL_ERR_INVALID, "no views");
   if (n_views > kMaxViews) return fail(ctx, SL_ERR_CAPACITY, "more than 256 views in one batch");
+  if (fixed && !ctx->fixed_base) return fail(ctx, SL_ERR_DEVICE, "no-alloc scratch unavailable: " + ctx->fixed_err);
   Batch b;
   b.flags = flags;
-  b.graph = (flags & SL_BATCH_GRAPH) != 0;
+  b.graph = !fixed && (flags & SL_BATCH_GRAPH) != 0;
+  b.fixed = fixed;
   b.views.resize(n_views);
   b.scenes.resize(n_views);
   b.slot_base.assign(n_views + 1, 0);
@@ -1048,6 +1099,27 @@ sl_status encode_batch(sl_context *ctx, const sl_view *views, uint32_t n_views, 
   b.depth_passes = n_views > 1 ? 3u : 2u;
   b.tile_passes = tiles > 65536u ? 3u : 2u;
   SL_HIP(ctx, hipSetDevice(ctx->device));
+  if (fixed) {
+    // this frame's slots and tiles, then as many pairs as the rest of the scratch holds (at least one per slot:
+    // a batch's three-pass depth sort borrows the second pair buffers)
+    const size_t cap = SL_NOALLOC_SCRATCH_BYTES;
+    const size_t fixed_bytes = layout_workspace(nullptr, b.total_slots, b.total_slots, b.total_tiles, ctx->fws);
+    if (fixed_bytes > cap)
+      return fail(ctx, SL_ERR_CAPACITY, "no-alloc frame: " + std::to_string(b.total_slots) + " slots and " +
+                                            std::to_string(b.total_tiles) + " tiles need " +
+                                            std::to_string(fixed_bytes) + " B, more than the scratch's " +
+                                            std::to_string(cap) + " B");
+    uint32_t lo = b.total_slots, hi = (uint32_t)std::min<size_t>(cap / 16, 1u << 30);
+    while (lo < hi) {
+      const uint32_t mid = lo + (hi - lo + 1) / 2;
+      if (layout_workspace(nullptr, b.total_slots, mid, b.total_tiles, ctx->fws) <= cap) lo = mid;
+      else hi = mid - 1;
+    }
+    layout_workspace(ctx->fixed_base, b.total_slots, lo, b.total_tiles, ctx->fws);
+    ctx->has_last = true;
+    ctx->last = std::move(b);
+    return launch_batch(ctx, ctx->last);
+  }
   const uint32_t pairs = std::max<uint32_t>(ctx->ws.pairs_cap, (uint32_t)std::min<uint64_t>(want_pairs + 4096, 1u << 29));
   sl_status s;
   if ((s = ensure_workspace(ctx, b.total_slots, std::max(pairs, b.total_slots), b.total_tiles)) != SL_OK) return s;
@@ -1062,6 +1134,18 @@ sl_status finish_batch(sl_context *ctx, sl_frame_stats *stats, uint32_t n_stats)
   SL_HIP(ctx, hipStreamSynchronize(ctx->stream));
   FrameCounters c = *ctx->counters_host;
   const Batch &b = ctx->last;
+  if (c.overflow > 0 && b.fixed) {
+    if (stats) {
+      memset(stats, 0, sizeof(*stats));
+      stats->slots = b.views[0].f.nslots;
+      stats->pairs = c.pairs;
+      stats->overflow = c.overflow;
+      stats->tiles = b.total_tiles;
+    }
+    return fail(ctx, SL_ERR_CAPACITY, "no-alloc frame: " + std::to_string((uint64_t)c.pairs + c.overflow) +
+                                          " pairs do not fit the scratch's " + std::to_string(ctx->fws.pairs_cap) +
+                                          " at this frame's slots and tiles (the image is incomplete)");
+  }
   if (c.overflow > 0) {
     const uint64_t need = (uint64_t)c.pairs + c.overflow;
     const uint64_t grown = std::min<uint64_t>((need * 5) / 4 + 4096, 1u << 30);
@@ -1105,6 +1189,14 @@ extern "C" sl_status sl_frame_encode(sl_context *ctx, sl_scene *sc, const void *
   return encode_batch(ctx, &v, 1, opts->flags & SL_FRAME_TIMERS);
 }
 
+extern "C" sl_status sl_frame_encode_noalloc(sl_context *ctx, sl_scene *sc, const void *frame_bytes,
+                                             size_t frame_len, const sl_frame_opts *opts, void *out_device) {
+  SL_LIVE(ctx);
+  if (!sc || !frame_bytes || !opts || !out_device) return fail(ctx, SL_ERR_INVALID, "null argument");
+  const sl_view v = {sc, frame_bytes, frame_len, opts->width, opts->height, opts->out_format, out_device};
+  return encode_batch(ctx, &v, 1, opts->flags & SL_FRAME_TIMERS, true);
+}
+
 extern "C" sl_status sl_frame_finish(sl_context *ctx, sl_frame_stats *stats) {
   SL_LIVE(ctx);
   return finish_batch(ctx, stats, 1);
@@ -1143,11 +1235,11 @@ extern "C" sl_status sl_frame_read(sl_context *ctx, uint32_t which, void *host, 
   if (!ctx->has_last) return fail(ctx, SL_ERR_INVALID, "no frame encoded");
   SL_HIP(ctx, hipStreamSynchronize(ctx->stream));
   const FrameCounters &c = *ctx->counters_host;
-  const Workspace &w = ctx->ws;
+  const Batch &b = ctx->last;
+  const Workspace &w = b.fixed ? ctx->fws : ctx->ws;
   const void *src = nullptr;
   size_t n = 0;
   // a single frame: its own slots; a batch: the global slot space (views at their slot_base)
-  const Batch &b = ctx->last;
   const size_t slots = b.views.size() == 1 ? b.views[0].f.nslots : b.total_slots;
   switch (which) {
     case SL_ART_KEYS: src = w.keys; n = slots * 4; break;

@@ -14,6 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsplatlight_hip.so")
 
 SL_OK = 0
+SL_ERR_CAPACITY = 7
 STATUS_NAMES = {0: "SL_OK", 1: "SL_ERR_INVALID", 2: "SL_ERR_LAYOUT", 3: "SL_ERR_DEVICE", 4: "SL_ERR_POISONED",
                 5: "SL_ERR_UNSUPPORTED", 6: "SL_ERR_BUSY", 7: "SL_ERR_CAPACITY"}
 SL_STORAGE_F32, SL_STORAGE_F16, SL_STORAGE_BF16 = 0, 1, 2
@@ -28,7 +29,7 @@ EXPORTS = [
     "sl_buf_alloc", "sl_buf_free", "sl_buf_write", "sl_buf_read",
     "sl_scene_create", "sl_scene_destroy", "sl_scene_write_parts", "sl_scene_write_colour",
     "sl_scene_write_bounce", "sl_scene_device_bytes",
-    "sl_frame_encode", "sl_frame_finish", "sl_frame_read", "sl_timers_read",
+    "sl_frame_encode", "sl_frame_finish", "sl_frame_read", "sl_timers_read", "sl_frame_encode_noalloc",
     "sl_lighting_iso_init", "sl_lighting_encode", "sl_scene_read", "sl_lighting_stats_read",
     "sl_batch_encode", "sl_batch_finish", "sl_ctx_set_tuning", "sl_ctx_get_tuning",
     "sl_server_create", "sl_server_submit", "sl_server_poll", "sl_server_info", "sl_server_error",
@@ -140,6 +141,7 @@ def lib() -> C.CDLL:
         "sl_scene_write_bounce": (i32, [vp, vp, vp, sz]),
         "sl_scene_device_bytes": (sz, [vp]),
         "sl_frame_encode": (i32, [vp, vp, vp, sz, C.POINTER(FrameOpts), vp]),
+        "sl_frame_encode_noalloc": (i32, [vp, vp, vp, sz, C.POINTER(FrameOpts), vp]),
         "sl_frame_finish": (i32, [vp, C.POINTER(FrameStats)]),
         "sl_frame_read": (i32, [vp, u32, vp, sz, C.POINTER(sz)]),
         "sl_timers_read": (i32, [vp, C.POINTER(C.c_char_p), C.POINTER(C.c_float), u32, C.POINTER(u32)]),
@@ -350,6 +352,18 @@ def frame_encode(ctx: Context, scene: Scene, frame: bytes, width: int, height: i
     fb = (C.c_uint8 * len(frame)).from_buffer_copy(frame)
     o = FrameOpts(width, height, out_format, SL_FRAME_TIMERS if timers else 0)
     ctx.check(ctx.L.sl_frame_encode(ctx.h, scene.h, fb, len(frame), C.byref(o), C.c_void_p(out_dev)))
+
+
+# Size of the device array the no-alloc frame lays its workspace over (SL_NOALLOC_SCRATCH_BYTES).
+NOALLOC_SCRATCH_BYTES = 64 << 20
+
+
+def frame_encode_noalloc(ctx: Context, scene: Scene, frame: bytes, width: int, height: int, out_dev: int,
+                         out_format: int = SL_OUT_RGBA32F, timers: bool = False):
+    """sl_frame_encode_noalloc: the same frame with its workspace in the device's fixed scratch array."""
+    fb = (C.c_uint8 * len(frame)).from_buffer_copy(frame)
+    o = FrameOpts(width, height, out_format, SL_FRAME_TIMERS if timers else 0)
+    ctx.check(ctx.L.sl_frame_encode_noalloc(ctx.h, scene.h, fb, len(frame), C.byref(o), C.c_void_p(out_dev)))
 
 
 def batch_encode(ctx: Context, views, graph: bool = False, timers: bool = False):
