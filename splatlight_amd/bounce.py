@@ -84,7 +84,9 @@ def scene_header(sc) -> dict:
               "gaussians": [int(sc.parts["g_lo"][p]), int(sc.parts["g_hi"][p])]} for p in range(sc.parts.shape[0])]
     h = {"parts": parts, "environment": sc.sun is not None or getattr(sc, "sky", None) is not None,
          "probes": {}}
-    if sc.name == "cornell":
+    if getattr(sc, "room", None) is not None:
+        h["room"] = dict(sc.room)
+    elif sc.name == "cornell":
         h["room"] = {"x": [-1.0, 1.0], "y": [-1.0, 1.0], "z": [0.0, 2.0]}
     else:
         mu = sc.gaussians["mu"].astype(np.float64)

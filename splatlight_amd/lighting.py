@@ -84,9 +84,12 @@ def _lum(c):
     return 0.2126 * c[0] + 0.7152 * c[1] + 0.0722 * c[2]
 
 
-def build_emitter_tree(rec: np.ndarray):
+def build_emitter_tree(rec: np.ndarray, split_at=None):
     """buildEmitterTree (src/lighting.js:99-169) over emitter records (GAUSSIAN_DTYPE or float32 [n, 16]).
 
+    split_at: None, the reference's median split (len >> 1); else a function of a node's leaf count giving how many
+    of its leaves, in order along the longest axis, go to its first child (1 gives a chain as deep as the count: the
+    trees a host that adds emitters one at a time would build, which need the shading's deep-walk stack).
     Returns (nodes float32 [2n-1, 16] with u32 words at 11 and 15, root index, cov float32 [2n-1, 8]).
     """
     r = np.ascontiguousarray(rec).view(np.float32).reshape(-1, 16).astype(np.float64)
@@ -142,7 +145,7 @@ def build_emitter_tree(rec: np.ndarray):
             if hi[k] - lo[k] > hi[ax] - lo[ax]:
                 ax = k
         srt = sorted(idx, key=lambda i: r[i][ax])
-        half = len(srt) >> 1
+        half = len(srt) >> 1 if split_at is None else min(max(int(split_at(len(srt))), 1), len(srt) - 1)
         ia = build(srt[:half])
         ib = build(srt[half:])
         kids = [info[ia], info[ib]]
@@ -184,6 +187,39 @@ def one_sided_normal(rec0) -> list:
     """The first emitter's short axis R x (src/lighting.js:703-705)."""
     x, y, z, w = [float(v) for v in rec0["quat"]]
     return [1 - 2 * (y * y + z * z), 2 * (x * y + w * z), 2 * (x * z - w * y)]
+
+
+def asset_emitters(gaussians: np.ndarray, g0: int, g1: int, emission, center=None) -> dict:
+    """setAssetEmitters (src/lighting.js:690-721): an asset's emissive part [g0, g1) as the light set. Its records
+    are moved so their rest centroid sits at `center` (None: where it is) and carry Le = `emission` in the albedo
+    words; the emitting normal is the first record's short axis; EP.scale is the overlap correction, the area of the
+    grid of emitter centres (extent along the first record's y and z axes, each stretched by cols / (cols - 1) over
+    its distinct columns at 1/1000 of the span) over the emitters' summed footprint 2 pi s_y s_z Ein(3). Returns
+    {"records", "normal", "scale"}; the part itself is EP.skip_part."""
+    src = gaussians[g0:g1]
+    n = int(src.shape[0])
+    rec = src.copy()
+    mu = src["mu"].astype(np.float64)
+    c0 = mu.sum(0) / n
+    if center is not None:
+        rec["mu"] = (mu + (np.asarray(center, np.float64) - c0)).astype(np.float32)
+    rec["albedo"] = np.asarray(emission, np.float32)
+    x, y, z, w = [float(v) for v in rec["quat"][0]]
+    normal = [1 - 2 * (y * y + z * z), 2 * (x * y + w * z), 2 * (x * z - w * y)]
+    axes = ([2 * (x * y - w * z), 1 - 2 * (x * x + z * z), 2 * (y * z + w * x)],
+            [2 * (x * z + w * y), 2 * (y * z - w * x), 1 - 2 * (x * x + y * y)])
+    rm = rec["mu"].astype(np.float64)
+    extent = []
+    for a in axes:
+        vals = a[0] * rm[:, 0] + a[1] * rm[:, 1] + a[2] * rm[:, 2]
+        lo, hi = float(vals.min()), float(vals.max())
+        cols = len({int(math.floor((v - lo) / (hi - lo + 1e-9) * 1000 + 0.5)) for v in vals.tolist()})   # Math.round
+        extent.append((hi - lo) * cols / max(cols - 1, 1))
+    sc = rec["scale"].astype(np.float64)
+    foot = 0.0
+    for i in range(n):
+        foot += 2 * math.pi * sc[i, 1] * sc[i, 2] * EIN3
+    return {"records": rec, "normal": normal, "scale": extent[0] * extent[1] / foot}
 
 
 def eig3(m):
@@ -343,9 +379,10 @@ def sun_plan(parts: np.ndarray, lights, eye, fwd, res=512, cap_bytes=None, stati
 
 # ---- the per-scene lighting setup the host hands to sl_lighting_encode ----------------------------------------
 
-def lighting_inputs(sc, eye=None, fwd=None, dirty_all=True, sun_res=None, frame=0):
+def lighting_inputs(sc, eye=None, fwd=None, dirty_all=True, sun_res=None, frame=0, emit_split=None):
     """All uniform bytes for one lit frame of a synthetic scene (scenes.Scene): FrameUniforms, EmitParams, the
-    emitter tree, LSParams, Env, SunParams and the sun layer plan. Maps are marked dirty when dirty_all."""
+    emitter tree (emit_split: build_emitter_tree's split_at), LSParams, Env, SunParams and the sun layer plan. Maps
+    are marked dirty when dirty_all."""
     eye = tuple(eye or sc.eye)
     if fwd is None:
         d = np.asarray(sc.target, np.float64) - np.asarray(eye, np.float64)
@@ -353,14 +390,20 @@ def lighting_inputs(sc, eye=None, fwd=None, dirty_all=True, sun_res=None, frame=
     U = R.frame_uniforms(frame=frame, gaussian_count=sc.count, part_count=sc.parts.shape[0],
                          theta=8.0, alpha_min=1 / 16, alpha_part=0.5, eye=eye,
                          material_count=sc.materials.shape[0], iso=1 | 2)
-    nodes, root, cov = build_emitter_tree(sc.emitters)
-    n = sc.emitters.shape[0]
-    normal = one_sided_normal(sc.emitters[0]) if sc.one_sided else [0.0, 0.0, 1.0]
-    EP = R.emit_params(count=n, root=root, scale=1.0, theta=EMIT_THETA, mode=1, node_count=nodes.shape[0],
-                       skip_part=0xFFFFFFFF, one_sided=1 if sc.one_sided else 0, normal=normal, n5=1 << 3,
+    es = getattr(sc, "emit_set", None)
+    if es is None:                  # the scene's own emitter records, drawn after its Gaussians, EP.scale 1
+        recs, scale, skip = sc.emitters, 1.0, 0xFFFFFFFF
+        normal = one_sided_normal(sc.emitters[0]) if sc.one_sided else [0.0, 0.0, 1.0]
+    else:                           # an asset's emissive part (asset_emitters): drawn with the scene, skipped
+        recs, scale, skip = es["records"], es["scale"], es["part"]
+        normal = es["normal"]
+    nodes, root, cov = build_emitter_tree(recs, split_at=emit_split)
+    n = recs.shape[0]
+    EP = R.emit_params(count=n, root=root, scale=scale, theta=EMIT_THETA, mode=1, node_count=nodes.shape[0],
+                       skip_part=skip, one_sided=1 if sc.one_sided else 0, normal=normal, n5=1 << 3,
                        edge_soft=EMIT_EDGE_SOFT)
     P = sc.parts.shape[0]
-    LS = ls_params(nodes, root, cov, normal, sc.one_sided, P, ((1 << P) - 1) if dirty_all else 0)
+    LS = ls_params(nodes, root, cov, normal, sc.one_sided, P, ((1 << P) - 1) if dirty_all else 0, skip_part=skip)
     out = {"U": U, "EP": EP, "emit_nodes": pack_emit_nodes(nodes, cov), "LS": LS, "ENV": None, "SM": None,
            "desc": None}
     if sc.sun is not None:

@@ -51,6 +51,8 @@ class Scene:
     sun: dict | None = None          # {"dir": unit vector toward the sun, "E": rgb, "std": rad, "res": int}
     sky: np.ndarray | None = None    # 16 x rgb real SH of the sky radiance (Env.sh), seen through the probes
     one_sided: bool = False          # emitters emit only toward +short axis (EmitParams.one_sided)
+    emit_set: dict | None = None     # an asset's emissive part as the light set (lighting.asset_emitters + "part")
+    room: dict | None = None         # an asset's room box {"x", "y", "z": [lo, hi]} (the probe lattice's)
 
     @property
     def count(self) -> int:

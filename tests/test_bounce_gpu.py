@@ -79,13 +79,20 @@ def mixed_arity(t, G):
     return tree.Tree(nodes, np.array(flat, np.uint32), roots, new_root, int(depth[:G].max()))
 
 
-def setup(ctx, sc, shape=None):
-    t = tree.scene_tree(sc)
+def setup(ctx, sc, shape=None, leaves=None, vg=None):
+    """The scene, its tree (over the given leaf Nodes, else the derived ones) and V grids (given, else V = 1)."""
+    if leaves is None:
+        t = tree.scene_tree(sc)
+    else:
+        t = tree.build_tree(sc.gaussians, sc.parts, leaves)
+        for p, r in enumerate(t.roots):
+            if r >= 0:
+                sc.parts["root"][p] = r
     if shape is not None:
         t = shape(t, sc.count)
-    vg = scenes.vgrid_open(sc.count)
-    scene = N.Scene(ctx, sc.gaussians, sc.parts, emitters=sc.emitters, materials=sc.materials,
-                    nodes=t.nodes[:sc.count], vgrid=vg)
+    vg = scenes.vgrid_open(sc.count) if vg is None else vg
+    scene = N.Scene(ctx, sc.gaussians, sc.parts, emitters=sc.emitters if sc.emitters.shape[0] else None,
+                    materials=sc.materials, nodes=t.nodes[:sc.count], vgrid=vg)
     b = BN.Bounce(scene, sc, t)
     return scene, t, vg, b
 
@@ -103,8 +110,8 @@ def read_state(scene, n_probes, G, N_nodes):
     }
 
 
-def check_frames(ctx, sc, frames=2, moved_mask=0, shape=None):
-    scene, t, vg, b = setup(ctx, sc, shape)
+def check_frames(ctx, sc, frames=2, moved_mask=0, shape=None, leaves=None, vg=None):
+    scene, t, vg, b = setup(ctx, sc, shape, leaves, vg)
     G, Nn, NP = sc.count, t.count, b.n_probes
     cells = oracle.cells_table(*BN.cell_table())
     up, levels = BN.up_levels(t.nodes, G, t.max_depth)
@@ -123,9 +130,12 @@ def check_frames(ctx, sc, frames=2, moved_mask=0, shape=None):
             iso = scene.read(N.SCENE_ISO_NORM)
             env_T = scene.read(N.SCENE_ENV_T).reshape(G, 4) if ENV is not None else None
             log(f"  frame {f}: G {G}, nodes {Nn}, probes {NP}")
-            # c0: the shading's emitter light (non-emissive materials: the lit colour before env_direct)
+            # c0: the shading's emitter light before a Gaussian's own emission (the lit colour before env_direct
+            # with the materials' emissive bit cleared)
+            no_emit = sc.materials.copy()
+            no_emit["flags"] &= ~np.uint32(1)
             o_col = oracle.shade_emit(U, inp["EP"], inp["emit_nodes"], sc.gaussians, t.nodes[:G], vg, sc.parts,
-                                      scene.read(N.SCENE_LAMP_T), sc.materials, iso)
+                                      scene.read(N.SCENE_LAMP_T), no_emit, iso)
             e = rel(s["c0"][:, :3], o_col[:, :3], 1e-3)
             log(f"    c0 rel {e:.2e}")
             assert e < 1e-3

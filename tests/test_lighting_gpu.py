@@ -22,10 +22,12 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def make(ctx, sc):
-    nd = scenes.leaf_nodes(sc.gaussians)
-    vg = scenes.vgrid_open(sc.count)
-    scene = N.Scene(ctx, sc.gaussians, sc.parts, emitters=sc.emitters, materials=sc.materials, nodes=nd, vgrid=vg)
+def make(ctx, sc, nd=None, vg=None):
+    """The scene on the device with leaf Nodes and V grids: given (baked ones), else derived (S_hat = I, V = 1)."""
+    nd = scenes.leaf_nodes(sc.gaussians) if nd is None else nd
+    vg = scenes.vgrid_open(sc.count) if vg is None else vg
+    em = sc.emitters if sc.emitters.shape[0] else None
+    scene = N.Scene(ctx, sc.gaussians, sc.parts, emitters=em, materials=sc.materials, nodes=nd, vgrid=vg)
     return scene, nd, vg
 
 
@@ -33,9 +35,9 @@ def rel_err(a, b, floor):
     return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), floor)))
 
 
-def check_lighting(ctx, sc, sun_res=None):
-    scene, nd, vg = make(ctx, sc)
-    inp = L.lighting_inputs(sc, sun_res=sun_res)
+def check_lighting(ctx, sc, sun_res=None, nd=None, vg=None, emit_split=None):
+    scene, nd, vg = make(ctx, sc, nd, vg)
+    inp = L.lighting_inputs(sc, sun_res=sun_res, emit_split=emit_split)
     try:
         scene.light(inp["U"], inp["EP"], inp["emit_nodes"], inp["LS"], inp["ENV"], inp["SM"], inp["desc"])
         iso = scene.read(N.SCENE_ISO_NORM)
@@ -134,6 +136,41 @@ def test_lit_frame_against_oracle(ctx):
     psnr = math.inf if mse == 0 else 10 * math.log10(1.0 / mse)
     log(f"  lit frame: max|err| {err:.3g} psnr {psnr:.1f} dB, mean rgb {ref[..., :3].mean():.3f}")
     assert err <= 1 / 1024 and psnr >= 55
+
+
+def test_relit_frame_repeats_bit_exact(ctx):
+    """Two fully relit frames of one scene in one process: the sun-map chain on the context's second stream, the
+    frame's geometry on the third beside the lighting. Maps, transmittances, colours and the image are the same bit
+    for bit both times (integer map sums are order-independent; nothing depends on which stream finishes first)."""
+    log("relit frame twice: hill n=120000, sun map 1024^2, 640x360 rgba16f")
+    sc = scenes.hill(n=120_000, seed=4, width=640, height=360)
+    scene, nd, vg = make(ctx, sc)
+    W, H = sc.width, sc.height
+    out = ctx.alloc(W * H * 8)
+    runs = []
+    try:
+        fb = sc.frame_bytes()
+        for _ in range(2):
+            inp = L.lighting_inputs(sc, sun_res=1024)
+            scene.light(inp["U"], inp["EP"], inp["emit_nodes"], inp["LS"], inp["ENV"], inp["SM"], inp["desc"])
+            N.frame_encode(ctx, scene, fb, W, H, out, N.SL_OUT_RGBA16F)
+            st = N.frame_finish(ctx)
+            runs.append({
+                "lsm": scene.read(N.SCENE_LSM_ACC, np.uint32), "sun": scene.read(N.SCENE_SUN_ACC, np.uint32),
+                "lamp_T": scene.read(N.SCENE_LAMP_T), "env_T": scene.read(N.SCENE_ENV_T),
+                "colour": scene.read(N.SCENE_COLOUR), "image": ctx.read(out, W * H * 8, np.uint16),
+                "pairs": st.pairs, "visible": st.visible})
+    finally:
+        ctx.free(out)
+        scene.destroy()
+    a, b = runs
+    log(f"  visible {a['visible']}, pairs {a['pairs']}, sun texels hit {int(np.count_nonzero(a['sun']))}")
+    for k in a:
+        if isinstance(a[k], np.ndarray):
+            assert np.array_equal(a[k], b[k]), k
+        else:
+            assert a[k] == b[k], k
+    assert np.count_nonzero(a["sun"]) > 0 and np.count_nonzero(a["image"]) > 0
 
 
 def test_maps_kept_when_nothing_moved(ctx):
