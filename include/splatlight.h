@@ -170,12 +170,18 @@ typedef struct sl_request_result {
   double submit_s, start_s, done_s;   /* steady clock, seconds since sl_server_create */
 } sl_request_result;
 sl_status sl_server_create(sl_context *ctx, const sl_server_opts *opts, sl_server **out);
-/* copies the frame bytes; the view's target receives the image */
+/* copies the frame bytes; the view's target receives the image. The request's lane waits for the work queued on
+ * the server's context when it is submitted (a lighting encode of the scene there needs no sl_sync first). */
 sl_status sl_server_submit(sl_server *server, const sl_view *view, uint64_t *request_id);
 /* completed requests (up to cap), waiting up to wait_us for the first when none is ready */
 sl_status sl_server_poll(sl_server *server, sl_request_result *out, uint32_t cap, uint32_t *n, uint32_t wait_us);
 sl_status sl_server_info(sl_server *server, uint32_t *pending, uint32_t *running, uint64_t *in_flight_bytes);
-const char *sl_server_error(sl_server *server);
+const char *sl_server_error(sl_server *server);   /* the last lane error (a copy, valid until this thread calls again) */
+/* Failure-handling check from the host: the worker's next completion check of a busy lane reports a device error
+ * instead of querying the lane's event, as a faulted batch would; its requests complete with SL_ERR_DEVICE, the lane
+ * is poisoned (later batches on it complete with SL_ERR_POISONED) and sl_server_error names it. The device is not
+ * touched. No counterpart in the reference. */
+sl_status sl_server_fail_next(sl_server *server);
 /* finishes every submitted request, then stops the worker */
 void sl_server_destroy(sl_server *server);
 

@@ -19,6 +19,7 @@ struct ServerRequest {
   std::vector<uint8_t> frame;
   uint64_t bytes = 0;            // HBM estimate while in flight
   double t_submit = 0.0;
+  hipEvent_t ready = nullptr;    // the owner context's queued work at submit (a lighting of the scene, uploads)
 };
 
 struct ServerLane {
@@ -41,6 +42,7 @@ struct sl_server {
   uint64_t next_id = 1;
   uint64_t in_flight_bytes = 0;
   bool stop = false;
+  std::atomic<bool> fail_next{false};         // sl_server_fail_next: the next completion check reports an error
   std::string err;
   std::thread worker;
   std::chrono::steady_clock::time_point t0;
@@ -58,11 +60,20 @@ uint64_t request_bytes(const sl_view &v) {
   return px + slots * (48 + 4 * 4 + 8) + 8 * slots * 16 + tiles * 12;
 }
 
-void finish_lane(sl_server *s, ServerLane &ln) {
+// A lane whose completion check returned q (hipSuccess, or the error that ended it): its requests' results, with
+// the batch's status (on an error, every request carries it and the server's error names it).
+void finish_lane(sl_server *s, ServerLane &ln, hipError_t q) {
   std::vector<sl_frame_stats> st(ln.reqs.size());
-  const sl_status rc = finish_batch(ln.ctx, st.data(), (uint32_t)st.size());
+  sl_status rc;
+  if (q == hipSuccess) {
+    rc = finish_batch(ln.ctx, st.data(), (uint32_t)st.size());
+  } else {
+    (void)hipStreamSynchronize(ln.ctx->stream);   // nothing of the batch still runs on the lane
+    rc = hip_fail(ln.ctx, q, "server lane: batch completion");
+  }
   const double t = s->now();
   std::lock_guard<std::mutex> g(s->mu);
+  if (rc != SL_OK) s->err = ln.ctx->err;
   for (size_t k = 0; k < ln.reqs.size(); k++) {
     sl_request_result r{};
     r.id = ln.reqs[k].id;
@@ -107,7 +118,13 @@ bool launch_lane(sl_server *s, ServerLane &ln) {
   ln.reqs = std::move(batch);
   ln.bytes = bytes;
   ln.busy = true;
-  sl_status rc = encode_batch(ln.ctx, views.data(), (uint32_t)views.size(), 0u);
+  sl_status rc = ln.ctx->poisoned != SL_OK ? SL_ERR_POISONED : SL_OK;
+  for (ServerRequest &r : ln.reqs) {             // ordered after what the owner had queued when it was submitted
+    if (rc == SL_OK && hipStreamWaitEvent(ln.ctx->stream, r.ready, 0) != hipSuccess) rc = SL_ERR_DEVICE;
+    (void)hipEventDestroy(r.ready);
+    r.ready = nullptr;
+  }
+  if (rc == SL_OK) rc = encode_batch(ln.ctx, views.data(), (uint32_t)views.size(), 0u);
   if (rc == SL_OK && hipEventRecord(ln.done, ln.ctx->stream) != hipSuccess) rc = SL_ERR_DEVICE;
   if (rc != SL_OK) {
     const double t = s->now();
@@ -122,7 +139,7 @@ bool launch_lane(sl_server *s, ServerLane &ln) {
       res.done_s = t;
       s->results.push_back(res);
     }
-    s->err = ln.ctx->err;
+    s->err = ln.ctx->err.empty() ? std::string("server lane: device error") : ln.ctx->err;
     s->in_flight_bytes -= ln.bytes;
     ln.reqs.clear();
     ln.bytes = 0;
@@ -136,13 +153,20 @@ void server_loop(sl_server *s) {
   (void)hipSetDevice(s->owner->device);
   for (;;) {
     bool progress = false;
+    for (ServerLane &ln : s->lanes) {
+      if (!ln.busy) continue;
+      // any status but not-ready ends the batch: a device error must not leave the lane busy forever
+      const hipError_t q = s->fail_next.exchange(false) ? hipErrorLaunchFailure : hipEventQuery(ln.done);
+      if (q == hipErrorNotReady) continue;
+      finish_lane(s, ln, q);
+      progress = true;
+    }
+    // a lane poisoned by a device error takes no more work while a healthy lane remains (with none left, requests
+    // still run through it and complete with SL_ERR_POISONED rather than wait forever)
+    bool any_live = false;
+    for (const ServerLane &ln : s->lanes) any_live = any_live || ln.ctx->poisoned == SL_OK;
     for (ServerLane &ln : s->lanes)
-      if (ln.busy && hipEventQuery(ln.done) == hipSuccess) {
-        finish_lane(s, ln);
-        progress = true;
-      }
-    for (ServerLane &ln : s->lanes)
-      if (!ln.busy && launch_lane(s, ln)) progress = true;
+      if (!ln.busy && (ln.ctx->poisoned == SL_OK || !any_live) && launch_lane(s, ln)) progress = true;
     std::unique_lock<std::mutex> g(s->mu);
     bool busy = false;
     for (const ServerLane &ln : s->lanes) busy = busy || ln.busy;
@@ -208,9 +232,19 @@ extern "C" sl_status sl_server_submit(sl_server *s, const sl_view *v, uint64_t *
   r.frame.assign((const uint8_t *)v->frame_bytes, (const uint8_t *)v->frame_bytes + kFrameBytes);
   r.bytes = request_bytes(*v);
   r.t_submit = s->now();
+  // the lane that renders it waits for the work the owner context had queued (e.g. sl_lighting_encode of the scene)
+  SL_HIP(ctx, hipEventCreateWithFlags(&r.ready, hipEventDisableTiming));
+  const hipError_t er = hipEventRecord(r.ready, ctx->stream);
+  if (er != hipSuccess) {
+    (void)hipEventDestroy(r.ready);
+    return hip_fail(ctx, er, "hipEventRecord");
+  }
   {
     std::lock_guard<std::mutex> g(s->mu);
-    if (s->stop) return SL_ERR_INVALID;
+    if (s->stop) {
+      (void)hipEventDestroy(r.ready);
+      return SL_ERR_INVALID;
+    }
     r.id = s->next_id++;
     if (id) *id = r.id;
     s->pending.push_back(std::move(r));
@@ -245,7 +279,20 @@ extern "C" sl_status sl_server_info(sl_server *s, uint32_t *pending, uint32_t *r
   return SL_OK;
 }
 
-extern "C" const char *sl_server_error(sl_server *s) { return s ? s->err.c_str() : ""; }
+// A copy taken under the lock, per calling thread (the worker may write the error meanwhile).
+extern "C" const char *sl_server_error(sl_server *s) {
+  thread_local std::string copy;
+  if (!s) return "";
+  std::lock_guard<std::mutex> g(s->mu);
+  copy = s->err;
+  return copy.c_str();
+}
+
+extern "C" sl_status sl_server_fail_next(sl_server *s) {
+  if (!s) return SL_ERR_INVALID;
+  s->fail_next.store(true);
+  return SL_OK;
+}
 
 extern "C" void sl_server_destroy(sl_server *s) {
   if (!s) return;

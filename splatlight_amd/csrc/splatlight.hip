@@ -1319,6 +1319,7 @@ sl_status grow_ctx(sl_context *ctx, Grow *gb, size_t bytes) {
   if (gb->p) {
     SL_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (ctx->side) SL_HIP(ctx, hipStreamSynchronize(ctx->side));
+    if (ctx->geo) SL_HIP(ctx, hipStreamSynchronize(ctx->geo));   // the sun maps' clears run there
     SL_HIP(ctx, hipFree(gb->p));
     gb->p = nullptr;
     gb->bytes = 0;

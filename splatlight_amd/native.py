@@ -33,7 +33,7 @@ EXPORTS = [
     "sl_lighting_iso_init", "sl_lighting_encode", "sl_scene_read", "sl_lighting_stats_read",
     "sl_batch_encode", "sl_batch_finish", "sl_ctx_set_tuning", "sl_ctx_get_tuning",
     "sl_server_create", "sl_server_submit", "sl_server_poll", "sl_server_info", "sl_server_error",
-    "sl_server_destroy", "sl_bounce_init", "sl_bounce_encode", "sl_bounce_read",
+    "sl_server_destroy", "sl_server_fail_next", "sl_bounce_init", "sl_bounce_encode", "sl_bounce_read",
 ]
 SL_BATCH_GRAPH = 2
 SCENE_COLOUR, SCENE_LAMP_T, SCENE_ENV_T, SCENE_ISO_NORM = 1, 2, 3, 4
@@ -157,6 +157,7 @@ def lib() -> C.CDLL:
         "sl_server_info": (i32, [vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(C.c_uint64)]),
         "sl_server_error": (C.c_char_p, [vp]),
         "sl_server_destroy": (None, [vp]),
+        "sl_server_fail_next": (i32, [vp]),
         "sl_ctx_get_tuning": (i32, [vp, C.POINTER(Tuning)]),
         "sl_batch_finish": (i32, [vp, C.POINTER(FrameStats), u32]),
         "sl_bounce_init": (i32, [vp, vp, C.POINTER(BounceTree), vp, sz]),
@@ -413,6 +414,13 @@ class Server:
         p, r, b = C.c_uint32(), C.c_uint32(), C.c_uint64()
         self.ctx.L.sl_server_info(self.h, C.byref(p), C.byref(r), C.byref(b))
         return int(p.value), int(r.value), int(b.value)
+
+    def error(self) -> str:
+        return (self.ctx.L.sl_server_error(self.h) or b"").decode()
+
+    def fail_next(self):
+        """sl_server_fail_next: the next completion check of a busy lane reports a device error (host side)."""
+        self.ctx.check(self.ctx.L.sl_server_fail_next(self.h))
 
     def close(self):
         if self.h:

@@ -2,12 +2,17 @@
 // its WebGPU calls. Handles are napi_external values with finalizers; byte arguments are ArrayBuffers or typed
 // arrays passed straight through (the C ABI copies them); errors throw with sl_last_error's message. Frame
 // completion (sl_frame_finish) runs on the libuv pool through napi_async_work and resolves a Promise on the JS
-// thread; nothing is called from HIP threads.
+// thread; nothing is called from HIP threads. A context is not thread-safe, so every call into one first waits
+// until no frameFinish of it is queued or running on the pool and then holds the context's lock: a host that encodes
+// the next frame before awaiting the last (src/main.js:886-906) gets each frame's own counts, and the pool never
+// touches a context while the JS thread does.
 #include <node_api.h>
 
 #include <stdint.h>
 #include <string.h>
 
+#include <condition_variable>
+#include <mutex>
 #include <string>
 
 #include "splatlight.h"
@@ -26,8 +31,17 @@ namespace {
 // any order when the environment shuts down, so the count is kept here rather than by JS references).
 struct Ctx {
   sl_context *ctx = nullptr;
-  int refs = 1;
+  int refs = 1;                 // touched on the JS thread only
+  std::mutex mu;
+  std::condition_variable idle;
+  int finishing = 0;            // frameFinish calls queued or running on the pool
 };
+// The context's lock, taken once no frameFinish of it is pending (see the file comment).
+std::unique_lock<std::mutex> quiet(Ctx *c) {
+  std::unique_lock<std::mutex> lk(c->mu);
+  c->idle.wait(lk, [c] { return c->finishing == 0; });
+  return lk;
+}
 void ctx_release(Ctx *c) {
   if (--c->refs == 0) {
     sl_ctx_destroy(c->ctx);
@@ -177,6 +191,7 @@ napi_value BufAlloc(napi_env env, napi_callback_info info) {
   Buf *b = new Buf();
   b->ctx = c->ctx;
   b->bytes = (size_t)bytes;
+  auto lk = quiet(c);
   if (sl_status s = sl_buf_alloc(c->ctx, b->bytes, &b->dev)) {
     delete b;
     return throw_sl(env, c->ctx, s, "bufAlloc");
@@ -188,7 +203,10 @@ napi_value BufAlloc(napi_env env, napi_callback_info info) {
       env, b,
       [](napi_env, void *p, void *) {
         Buf *b = static_cast<Buf *>(p);
-        sl_buf_free(b->ctx, b->dev);
+        {
+          auto lk = quiet(b->owner);
+          sl_buf_free(b->ctx, b->dev);
+        }
         ctx_release(b->owner);
         delete b;
       },
@@ -205,6 +223,7 @@ napi_value BufRead(napi_env env, napi_callback_info info) {
   void *data = nullptr;
   napi_value ab;
   NAPI_OK(napi_create_arraybuffer(env, b->bytes, &data, &ab));
+  auto lk = quiet(b->owner);
   if (sl_status s = sl_buf_read(b->ctx, b->dev, 0, data, b->bytes)) return throw_sl(env, b->ctx, s, "bufRead");
   return ab;
 }
@@ -236,6 +255,7 @@ napi_value SceneCreate(napi_env env, napi_callback_info info) {
   d.storage = u32_arg(env, prop(env, a[1], "storage"));
   Scene *s = new Scene();
   s->ctx = c->ctx;
+  auto lk = quiet(c);
   if (sl_status rc = sl_scene_create(c->ctx, &d, &s->scene)) {
     delete s;
     return throw_sl(env, c->ctx, rc, "sceneCreate");
@@ -247,7 +267,10 @@ napi_value SceneCreate(napi_env env, napi_callback_info info) {
       env, s,
       [](napi_env, void *p, void *) {
         Scene *s = static_cast<Scene *>(p);
-        sl_scene_destroy(s->ctx, s->scene);
+        {
+          auto lk = quiet(s->owner);
+          sl_scene_destroy(s->ctx, s->scene);
+        }
         ctx_release(s->owner);
         delete s;
       },
@@ -265,6 +288,7 @@ napi_value SceneWriteParts(napi_env env, napi_callback_info info) {
   const void *p;
   size_t n;
   if (!bytes_of(env, a[1], &p, &n)) return nullptr;
+  auto lk = quiet(s->owner);
   if (sl_status rc = sl_scene_write_parts(s->ctx, s->scene, p, n)) return throw_sl(env, s->ctx, rc, "writeParts");
   return nullptr;
 }
@@ -278,6 +302,7 @@ napi_value SceneWriteColour(napi_env env, napi_callback_info info) {
   const void *p;
   size_t n;
   if (!bytes_of(env, a[1], &p, &n)) return nullptr;
+  auto lk = quiet(s->owner);
   if (sl_status rc = sl_scene_write_colour(s->ctx, s->scene, (const float *)p, n / 16))
     return throw_sl(env, s->ctx, rc, "bindColour");
   return nullptr;
@@ -289,6 +314,7 @@ napi_value LightingIsoInit(napi_env env, napi_callback_info info) {
   NAPI_OK(napi_get_cb_info(env, info, &argc, a, nullptr, nullptr));
   Scene *s = unwrap<Scene>(env, a[0]);
   if (!s) return nullptr;
+  auto lk = quiet(s->owner);
   if (sl_status rc = sl_lighting_iso_init(s->ctx, s->scene)) return throw_sl(env, s->ctx, rc, "runIsoInit");
   return nullptr;
 }
@@ -318,6 +344,7 @@ napi_value BounceInit(napi_env env, napi_callback_info info) {
   t.kids = (const uint32_t *)kids;
   t.kid_count = (uint32_t)(nk / 4);
   t.max_depth = max_depth;
+  auto lk = quiet(s->owner);
   if (sl_status rc = sl_bounce_init(s->ctx, s->scene, &t, pp, npp)) return throw_sl(env, s->ctx, rc, "Lighting.initBounce");
   return nullptr;
 }
@@ -344,6 +371,7 @@ napi_value BounceEncode(napi_env env, napi_callback_info info) {
     v = false;
     if (napi_get_value_bool(env, prop(env, a[2], "lightStill"), &v) == napi_ok && v) flags |= SL_BOUNCE_LIGHT_STILL;
   }
+  auto lk = quiet(s->owner);
   if (sl_status rc = sl_bounce_encode(s->ctx, s->scene, pp, npp, flags)) return throw_sl(env, s->ctx, rc, "Lighting.encodeBounce");
   return nullptr;
 }
@@ -374,6 +402,7 @@ napi_value LightingEncode(napi_env env, napi_callback_info info) {
   bool timers = false;
   if (argc > 2) napi_get_value_bool(env, a[2], &timers);
   d.flags = timers ? SL_LIGHT_TIMERS : 0u;
+  auto lk = quiet(s->owner);
   if (sl_status rc = sl_lighting_encode(s->ctx, s->scene, &d)) return throw_sl(env, s->ctx, rc, "Lighting.encode");
   return nullptr;
 }
@@ -396,11 +425,13 @@ napi_value FrameEncode(napi_env env, napi_callback_info info) {
   bool timers = false;
   if (argc > 6) napi_get_value_bool(env, a[6], &timers);
   o.flags = timers ? SL_FRAME_TIMERS : 0u;
+  auto lk = quiet(s->owner);
   if (sl_status rc = sl_frame_encode(s->ctx, s->scene, fb, flen, &o, t->dev)) return throw_sl(env, s->ctx, rc, "encode");
   return nullptr;
 }
 
-// frameFinish(ctx) -> Promise<{slots, visible, pairs, overflow, tiles}>, waited for on the libuv pool
+// frameFinish(ctx) -> Promise<{slots, visible, pairs, overflow, tiles}> of the frame encoded last, waited for on the
+// libuv pool (which holds the context's lock meanwhile; the next call into the context waits for it)
 struct FinishWork {
   napi_async_work work = nullptr;
   napi_deferred deferred = nullptr;
@@ -417,6 +448,10 @@ napi_value FrameFinish(napi_env env, napi_callback_info info) {
   NAPI_OK(napi_get_cb_info(env, info, &argc, a, nullptr, nullptr));
   Ctx *c = unwrap<Ctx>(env, a[0]);
   if (!c) return nullptr;
+  {
+    auto lk = quiet(c);    // one finish at a time: each reports the frame encoded before it
+    c->finishing++;
+  }
   FinishWork *w = new FinishWork();
   w->ctx = c->ctx;
   w->owner = c;
@@ -428,8 +463,13 @@ napi_value FrameFinish(napi_env env, napi_callback_info info) {
       env, nullptr, name,
       [](napi_env, void *p) {
         FinishWork *w = static_cast<FinishWork *>(p);
-        w->status = sl_frame_finish(w->ctx, &w->stats);
-        if (w->status) w->err = sl_last_error(w->ctx);
+        {
+          std::lock_guard<std::mutex> lk(w->owner->mu);
+          w->status = sl_frame_finish(w->ctx, &w->stats);
+          if (w->status) w->err = sl_last_error(w->ctx);
+          w->owner->finishing--;
+        }
+        w->owner->idle.notify_all();
       },
       [](napi_env env, napi_status, void *p) {
         FinishWork *w = static_cast<FinishWork *>(p);
@@ -467,6 +507,7 @@ napi_value TimersRead(napi_env env, napi_callback_info info) {
   const char *names[32];
   float ms[32];
   uint32_t n = 0;
+  auto lk = quiet(c);
   if (sl_status rc = sl_timers_read(c->ctx, names, ms, 32, &n)) return throw_sl(env, c->ctx, rc, "timers");
   napi_value o;
   NAPI_OK(napi_create_object(env, &o));

@@ -98,3 +98,31 @@ def test_js_host_bounce_frame_equals_python_frame(ctx, tmp_path):
         scene.destroy()
     assert js_img == py_img
     print("  JS host with bounce: target bytes identical to the ctypes path", file=sys.stderr, flush=True)
+
+
+def test_js_host_encodes_next_frame_before_awaiting(ctx, tmp_path):
+    """Lighting and frame encoded for four cameras back to back before any frame's counts are awaited (as the
+    reference's capture loop does): each frame's promise resolves to that frame's own counts, the same as when every
+    frame is awaited before the next, and the last target's bytes are the same."""
+    sc = scenes.cornell(n=6_000, seed=22, width=160, height=120)
+    nd = scenes.leaf_nodes(sc.gaussians)
+    inp = L.lighting_inputs(sc)
+    files = {"gaussians.bin": sc.gaussians, "parts.bin": sc.parts, "materials.bin": sc.materials, "nodes.bin": nd,
+             "emitters.bin": sc.emitters, "U.bin": inp["U"], "EP.bin": inp["EP"],
+             "emit_nodes.bin": np.ascontiguousarray(inp["emit_nodes"], np.float32), "LS.bin": inp["LS"]}
+    for k, r in enumerate((2.5, 3.5, 5.0, 3.0)):
+        files[f"frame{k}.bin"] = sc.frame_bytes(eye=scenes.orbit_eye(sc, r, -1.2 + 0.4 * k, 0.3))
+    for name, v in files.items():
+        (tmp_path / name).write_bytes(v if isinstance(v, (bytes, bytearray)) else np.ascontiguousarray(v).tobytes())
+    (tmp_path / "meta.json").write_text(json.dumps({"width": sc.width, "height": sc.height, "frames": 4}))
+    print("node pipelined frames: 4 cameras", file=sys.stderr, flush=True)
+    r = subprocess.run([NODE, os.path.join(ROOT, "tests", "node_pipeline.js"), str(tmp_path)], cwd=ROOT,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = json.loads((tmp_path / "piped.json").read_text())
+    piped = [(f["visible"], f["pairs"]) for f in out["piped"]]
+    serial = [(f["visible"], f["pairs"]) for f in out["serial"]]
+    print(f"  pipelined {piped}, awaited {serial}", file=sys.stderr, flush=True)
+    assert piped == serial
+    assert len(set(serial)) > 1                 # the cameras differ, so a stale count would show
+    assert out["hdrEqual"]

@@ -98,3 +98,81 @@ def test_server_rejects_bad_requests(ctx):
         srv.close()
         ctx.free(out)
         h.destroy()
+
+
+def test_server_contains_a_failed_batch(ctx):
+    """A lane whose batch ends in an error (forced from the host: the worker's next completion check reports a
+    device error instead of querying the lane's event; the device itself is untouched) completes every request of
+    that batch with the error, names it in sl_server_error, takes its lane out of further work and still shuts down;
+    the other lane serves every other request, and a new server on the same context is unaffected."""
+    sc = scenes.cornell(n=3000, seed=2)
+    h = N.Scene(ctx, sc.gaussians, sc.parts, emitters=sc.emitters)
+    h.write_colour(np.concatenate([sc.gaussians["albedo"], np.ones((sc.count, 1))], 1).astype(np.float32))
+    W, H = 160, 120
+    fb = sc.frame_bytes(width=W, height=H)
+    outs = [ctx.alloc(W * H * 16) for _ in range(12)]
+    try:
+        srv = N.Server(ctx, hbm_budget=1 << 30, lanes=2, max_batch_views=2)
+        try:
+            srv.fail_next()
+            ids = [srv.submit(h, fb, W, H, o) for o in outs]
+            got = {}
+            t0 = time.perf_counter()
+            while len(got) < len(ids) and time.perf_counter() - t0 < 30:
+                for r in srv.poll(wait_us=20_000):
+                    got[r.id] = r.status
+            assert len(got) == len(ids)
+            st = [got[i] for i in ids]
+            log(f"  statuses {st}; server error: {srv.error()!r}")
+            assert 1 <= st.count(3) <= 2                              # SL_ERR_DEVICE for the failed batch's views
+            assert st.count(0) == len(st) - st.count(3)               # the healthy lane served every other request
+            assert "server lane: batch completion" in srv.error()
+        finally:
+            srv.close()                                               # returns: no lane is left busy
+        srv2 = N.Server(ctx, hbm_budget=1 << 30, lanes=1, max_batch_views=4)
+        try:
+            rid = srv2.submit(h, fb, W, H, outs[0])
+            res = []
+            t0 = time.perf_counter()
+            while not res and time.perf_counter() - t0 < 30:
+                res = srv2.poll(wait_us=20_000)
+            assert res and res[0].id == rid and res[0].status == 0
+        finally:
+            srv2.close()
+        assert ctx.L.sl_ctx_status(ctx.h) == 0
+    finally:
+        for o in outs:
+            ctx.free(o)
+        h.destroy()
+
+
+def test_served_request_waits_for_the_owner_lighting(ctx):
+    """A scene lit asynchronously on the server's context (sl_lighting_encode, no sl_sync) and then submitted: the
+    lane waits for that lighting, so the served image equals the one drawn on the context after the lighting."""
+    from splatlight_amd import lighting as L
+    sc = scenes.hill(n=40_000, seed=21, width=320, height=180)
+    h = N.Scene(ctx, sc.gaussians, sc.parts, emitters=sc.emitters, materials=sc.materials,
+                nodes=scenes.leaf_nodes(sc.gaussians), vgrid=scenes.vgrid_open(sc.count))
+    W, H = sc.width, sc.height
+    fb = sc.frame_bytes()
+    o1, o2 = ctx.alloc(W * H * 16), ctx.alloc(W * H * 16)
+    srv = N.Server(ctx, hbm_budget=1 << 30, lanes=1, max_batch_views=4)
+    try:
+        inp = L.lighting_inputs(sc, sun_res=1024)
+        h.light(inp["U"], inp["EP"], inp["emit_nodes"], inp["LS"], inp["ENV"], inp["SM"], inp["desc"])
+        rid = srv.submit(h, fb, W, H, o1)                       # no sync between the lighting and the submit
+        res = []
+        t0 = time.perf_counter()
+        while not res and time.perf_counter() - t0 < 30:
+            res = srv.poll(wait_us=20_000)
+        assert res and res[0].id == rid and res[0].status == 0
+        N.frame_encode(ctx, h, fb, W, H, o2)
+        N.frame_finish(ctx)
+        a = ctx.read(o1, W * H * 16, np.uint8)
+        b = ctx.read(o2, W * H * 16, np.uint8)
+        assert np.array_equal(a, b)
+    finally:
+        srv.close()
+        ctx.free(o1)
+        ctx.free(o2)
+        h.destroy()
