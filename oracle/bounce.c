@@ -236,6 +236,7 @@ void or_bounce_lo_dir(const void *U, const void *ENV, const void *gaussians, con
   const void *parts = parts_;
   uint32_t n = rdu(U, 1), mcount = rdu(U, 13), iso = rdu(U, 15), frame = rdu(U, 0);
   int env_on = ENV && rdu(ENV, 96) > 0u && (iso & 1u) != 0u;
+#pragma omp parallel for schedule(dynamic, 512)
   for (uint32_t g = 0; g < n; g++) {
     if ((iso & 2u) != 0u && g % 4u != frame % 4u) continue;
     const char *G = GAUSS(g);
@@ -280,6 +281,7 @@ void or_node_leaf(const void *U, const void *PP, const void *gaussians, const vo
                   const void *parts_, const float *lo_dir, const float *lo_b, float *node_acc) {
   const void *parts = parts_;
   uint32_t n = rdu(U, 1), dark_part = rdu(PP, 207);
+#pragma omp parallel for schedule(dynamic, 512)
   for (uint32_t g = 0; g < n; g++) {
     const char *G = GAUSS(g);
     const char *nd = NODE(g);
@@ -318,6 +320,7 @@ void or_node_up(const void *U, const void *PP, const void *nodes, const uint32_t
 
 void or_node_sh2(const void *PP, const float *node_acc, uint32_t *node_sh) {
   uint32_t N = rdu(PP, 7);
+#pragma omp parallel for schedule(dynamic, 512)
   for (uint32_t i = 0; i < N; i++) {
     v3 L[14];
     for (uint32_t k = 0; k < 14u; k++) {
@@ -361,6 +364,7 @@ void or_probe_gather(const void *U, const void *PP, const void *ENV, const void 
   uint32_t G = rdu(U, 1);
   uint32_t np = lat_count(PP), part_probes = rdu(PP, 18);
   float theta = rdf(PP, 9), tmax = rdf(PP, 10), alpha = rdf(PP, 12);
+#pragma omp parallel for schedule(dynamic, 64)
   for (uint32_t gid = id0; gid < id1; gid++) {
     uint32_t id = gid * rdu(PP, 16) + rdu(PP, 17);
     if (id >= (np + part_probes) * NCELL) continue;
@@ -516,6 +520,7 @@ void or_iso_xfer(const void *U, const void *PP, const void *gaussians, const voi
   const void *parts = parts_;
   uint32_t n = rdu(U, 1), pc = rdu(U, 2);
   if ((rdu(U, 15) & 3u) != 3u) return;
+#pragma omp parallel for schedule(dynamic, 256)
   for (uint32_t g = 0; g < n; g++) {
     const char *G = GAUSS(g);
     uint32_t own = rdu(G, 11) & 0xFFFFu;
@@ -676,6 +681,7 @@ void or_bounce_sh(const void *U, const void *PP, const void *gaussians, const vo
   const void *parts = parts_;
   uint32_t n = rdu(U, 1);
   if ((rdu(U, 15) & 3u) != 3u) return;
+#pragma omp parallel for schedule(dynamic, 512)
   for (uint32_t gid = 0; gid < n; gid++) {
     uint32_t g = rdu(PP, 19) != 0u ? gid : gid * rdu(PP, 16) + rdu(PP, 17);
     if (g >= n) continue;

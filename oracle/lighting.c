@@ -171,6 +171,7 @@ static v3 basis_k(v3 w) {   /* runtime/wgsl/common.wgsl:181-185 */
 /* ---- iso_init (runtime/wgsl/iso_init.wgsl:29-56) ------------------------------------------------------------- */
 
 void or_iso_init(const void *gaussians, const void *nodes, const uint8_t *vgrid, uint32_t n, float *iso_norm) {
+#pragma omp parallel for schedule(dynamic, 256)
   for (uint32_t g = 0; g < n; g++) {
     const char *G = (const char *)gaussians + 64 * (size_t)g;
     const char *nd = (const char *)nodes + 96 * (size_t)g;
@@ -228,6 +229,7 @@ void or_shade_emit(const void *U, const void *EP, const void *emit_nodes, const 
   v3 eye = rd3(U, 8);
   float theta = rdf(EP, 3);
   uint32_t mode = rdu(EP, 4);
+#pragma omp parallel for schedule(dynamic, 256)
   for (uint32_t g = 0; g < n; g++) {
     const char *G = (const char *)gaussians + 64 * (size_t)g;
     const char *nd = (const char *)nodes + 96 * (size_t)g;
@@ -405,6 +407,7 @@ void or_lsm_splat(const void *U, const void *LS, const void *gaussians, const vo
   const int MAX_HALF = 48;
   uint32_t n = rdu(U, 1), res = rdu(LS, 3), pc = rdu(LS, 19), skip = rdu(LS, 17), dirty = rdu(LS, 15);
   v3 center = rd3(LS, 0), nrm = rd3(LS, 4);
+#pragma omp parallel for schedule(dynamic, 1024)
   for (uint32_t g = 0; g < n; g++) {
     const char *G = (const char *)gaussians + 64 * (size_t)g;
     uint32_t p = rdu(G, 11) & 0xFFFFu;
@@ -461,9 +464,14 @@ void or_lsm_splat(const void *U, const void *LS, const void *gaussians, const vo
         float z = r + dot2(gz, dt);
         z = z > 0.0f ? z : 0.0f;
         size_t k = (base + (size_t)y * res + (size_t)x) * 3;
-        acc[k] += u32_sat(t_fix + 0.5f);
-        acc[k + 1] += u32_sat(t_fix * z + 0.5f);
-        acc[k + 2] += u32_sat(t_fix * (z * z + sz2) + 0.5f);
+        const uint32_t a0 = u32_sat(t_fix + 0.5f), a1 = u32_sat(t_fix * z + 0.5f);
+        const uint32_t a2 = u32_sat(t_fix * (z * z + sz2) + 0.5f);
+#pragma omp atomic
+        acc[k] += a0;
+#pragma omp atomic
+        acc[k + 1] += a1;
+#pragma omp atomic
+        acc[k + 2] += a2;
       }
   }
 }
@@ -487,6 +495,7 @@ void or_lsm_lookup(const void *U, const void *LS, const void *gaussians, const v
   v3 center = rd3(LS, 0);
   uint32_t level_off[16];
   for (int k = 0; k < 16; k++) level_off[k] = rdu(LS, 36 + k);
+#pragma omp parallel for schedule(dynamic, 1024)
   for (uint32_t g = 0; g < n; g++) {
     const char *G = (const char *)gaussians + 64 * (size_t)g;
     uint32_t own = rdu(G, 11) & 0xFFFFu;
@@ -567,6 +576,7 @@ void or_sun_splat(const void *U, const void *SM, const void *desc, const float *
   const float FIX = 4096.0f, LOWPASS = 0.3f;
   const int MAX_HALF = 32;
   uint32_t n = rdu(U, 1), res = rdu(SM, 1);
+#pragma omp parallel for schedule(dynamic, 1024)
   for (uint32_t g = 0; g < n; g++) {
     const char *G = (const char *)gaussians + 64 * (size_t)g;
     uint32_t p = rdu(G, 11) & 0xFFFFu;
@@ -621,9 +631,14 @@ void or_sun_splat(const void *U, const void *SM, const void *desc, const float *
           float z = z0 - dot2(gz, dt);
           z = z > 0.0f ? z : 0.0f;
           size_t i = (base + (size_t)y * res + (size_t)xx) * 3;
-          acc[i] += u32_sat(t_fix + 0.5f);
-          acc[i + 1] += u32_sat(t_fix * z + 0.5f);
-          acc[i + 2] += u32_sat(t_fix * (z * z + sz2) + 0.5f);
+          const uint32_t a0 = u32_sat(t_fix + 0.5f), a1 = u32_sat(t_fix * z + 0.5f);
+          const uint32_t a2 = u32_sat(t_fix * (z * z + sz2) + 0.5f);
+#pragma omp atomic
+          acc[i] += a0;
+#pragma omp atomic
+          acc[i + 1] += a1;
+#pragma omp atomic
+          acc[i + 2] += a2;
         }
     }
   }
@@ -645,6 +660,7 @@ void or_sun_lookup(const void *U, const void *SM, const void *desc, const float 
   uint32_t n = rdu(U, 1), nl = rdu(SM, 0), res = rdu(SM, 1), levels = rdu(SM, 2), lt = rdu(SM, 3);
   uint32_t level_off[16];
   for (int k = 0; k < 16; k++) level_off[k] = rdu(SM, 92 + k);
+#pragma omp parallel for schedule(dynamic, 1024)
   for (uint32_t g = 0; g < n; g++) {
     const char *G = (const char *)gaussians + 64 * (size_t)g;
     uint32_t own = rdu(G, 11) & 0xFFFFu;
@@ -696,6 +712,7 @@ void or_env_direct(const void *U, const void *ENV, const void *gaussians, const 
                    float *colour) {
   uint32_t n = rdu(U, 1), mcount = rdu(U, 13), nlights = rdu(ENV, 96);
   if (nlights == 0u || (rdu(U, 15) & 1u) == 0u) return;
+#pragma omp parallel for schedule(dynamic, 1024)
   for (uint32_t g = 0; g < n; g++) {
     const char *G = (const char *)gaussians + 64 * (size_t)g;
     uint32_t mi = rdu(G, 15) & 0xFFFFu;

@@ -85,6 +85,7 @@ uint32_t or_project_f32(const void *frame, const void *gaussians, const void *pa
   uint32_t count = uw(frame, FR_COUNT);
   uint32_t nslots = count + 1u + n_emit;
   uint32_t visible = 0;
+#pragma omp parallel for schedule(dynamic, 4096) reduction(+ : visible)
   for (uint32_t i = 0; i < nslots; i++) {
     float s[12];
     float depth;
@@ -103,6 +104,7 @@ void or_project_f64(const void *frame, const void *gaussians, const void *parts,
                     const float *lo_b, const void *emitters, uint32_t n_emit, double *out, uint8_t *valid) {
   uint32_t count = uw(frame, FR_COUNT);
   uint32_t nslots = count + 1u + n_emit;
+#pragma omp parallel for schedule(dynamic, 4096)
   for (uint32_t i = 0; i < nslots; i++) {
     double s[12];
     double depth = 0.0;
@@ -229,59 +231,76 @@ void or_composite(const void *frame, const uint32_t *splats, const uint32_t *ord
   or_composite_rows(frame, splats, order, visible, W, H, mode, 1u, out);
 }
 
-/* The same blend on the rows py % row_step == 0 only (a bounded sample for timing); other rows stay 0. */
+/* The same blend on the rows py % row_step == 0 only (a bounded sample for timing); other rows stay 0. Threads take
+ * bands of 16 rows; within a band every splat is visited in the back-to-front order, so each pixel's sequence of
+ * blends (and its float result) is the serial one. */
 void or_composite_rows(const void *frame, const uint32_t *splats, const uint32_t *order, uint32_t visible,
                        uint32_t W, uint32_t H, int mode, uint32_t row_step, float *out) {
   const float ALPHA_MIN = 1.0f / 255.0f;
+  const uint32_t BAND = 16u;
   uint32_t view_mode = uw(frame, FR_VIEW_MODE);
   float near_ = fw(frame, FR_NEAR), far_ = fw(frame, FR_FAR);
   memset(out, 0, sizeof(float) * 4 * (size_t)W * H);
-  for (uint32_t r = 0; r < visible; r++) {          /* back to front: ascending key (shaders/splat.wgsl:52-55) */
+  /* pixel range of each quad's box, in pixels (not tiles); rows past H when it draws nothing */
+  uint32_t *boxes = (uint32_t *)malloc(sizeof(uint32_t) * 4 * (size_t)(visible ? visible : 1));
+#pragma omp parallel for schedule(dynamic, 4096)
+  for (uint32_t r = 0; r < visible; r++) {
+    uint32_t *b = boxes + 4 * (size_t)r;
     const uint32_t *s = splats + 12 * (size_t)order[r];
-    float cx = bits_f(s[0]), cy = bits_f(s[1]);
-    float a0x = bits_f(s[2]), a0y = bits_f(s[3]), a1x = bits_f(s[4]), a1y = bits_f(s[5]);
-    float rr = bits_f(s[6]), tau0 = bits_f(s[7]);
-    float rgb[3] = {bits_f(s[8]), bits_f(s[9]), bits_f(s[10])};
-    float depth = bits_f(s[11]);
-    if (view_mode == 1u) {
-      /* depth view: rgb = 1 - clamp(t) (shaders/splat.wgsl:28,41,76) */
-      float t = logf(depth / near_) / logf(far_ / near_);
-      t = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
-      rgb[0] = rgb[1] = rgb[2] = 1.0f - t;
-    }
-    uint32_t box[4];
-    /* pixel range of the quad's box, in pixels (not tiles) */
-    if (!or_tile_box(s, W, H, 1, box)) continue;
-    float det = a0x * a1y - a1x * a0y;
-    if (det == 0.0f) continue;
-    for (uint32_t py = (box[1] + row_step - 1u) / row_step * row_step; py <= box[3]; py += row_step) {
-      float ny = 1.0f - ((float)py + 0.5f) * 2.0f / (float)H;
-      for (uint32_t px = box[0]; px <= box[2]; px++) {
-        float nx = ((float)px + 0.5f) * 2.0f / (float)W - 1.0f;
-        float dx = nx - cx, dy = ny - cy;
-        /* the pixel's corner coordinates on the quad c + u a0 + v a1 (the interpolated `corner`) */
-        float u = (dx * a1y - a1x * dy) / det;
-        float v = (a0x * dy - dx * a0y) / det;
-        if (fabsf(u) > 1.0f || fabsf(v) > 1.0f) continue;
-        float lx = u * rr, ly = v * rr;
-        float e = expf_cr(-0.5f * (lx * lx + ly * ly));
-        float alpha = 1.0f - expf_cr(-tau0 * e);
-        if (alpha < ALPHA_MIN) continue;            /* shaders/splat.wgsl:73 (tau_c = 0: cov = 0) */
-        float *d = out + 4 * ((size_t)py * W + px);
-        if (view_mode == 2u) {                      /* overdraw: additive (1, 0, 0, 0) (src/renderer.js:141-143) */
-          d[0] = d[0] + 1.0f;
-          if (mode == 2) d[0] = to_half_rtne(d[0]);
-          continue;
-        }
-        float src[4] = {rgb[0] * alpha, rgb[1] * alpha, rgb[2] * alpha, alpha};
-        float keep = 1.0f - alpha;
-        for (int k = 0; k < 4; k++) {
-          d[k] = src[k] + d[k] * keep;               /* over: src * 1 + dst * (1 - src.a) */
-          if (mode == 2) d[k] = to_half_rtne(d[k]);
+    if (!or_tile_box(s, W, H, 1, b)) b[1] = b[3] = 0xFFFFFFFFu;
+  }
+  const uint32_t bands = (H + BAND - 1u) / BAND;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (uint32_t band = 0; band < bands; band++) {
+    const uint32_t y0 = band * BAND, y1 = (y0 + BAND < H ? y0 + BAND : H) - 1u;
+    for (uint32_t r = 0; r < visible; r++) {        /* back to front: ascending key (shaders/splat.wgsl:52-55) */
+      const uint32_t *box = boxes + 4 * (size_t)r;
+      if (box[1] == 0xFFFFFFFFu || box[1] > y1 || box[3] < y0) continue;
+      const uint32_t *s = splats + 12 * (size_t)order[r];
+      float cx = bits_f(s[0]), cy = bits_f(s[1]);
+      float a0x = bits_f(s[2]), a0y = bits_f(s[3]), a1x = bits_f(s[4]), a1y = bits_f(s[5]);
+      float rr = bits_f(s[6]), tau0 = bits_f(s[7]);
+      float rgb[3] = {bits_f(s[8]), bits_f(s[9]), bits_f(s[10])};
+      float depth = bits_f(s[11]);
+      if (view_mode == 1u) {
+        /* depth view: rgb = 1 - clamp(t) (shaders/splat.wgsl:28,41,76) */
+        float t = logf(depth / near_) / logf(far_ / near_);
+        t = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
+        rgb[0] = rgb[1] = rgb[2] = 1.0f - t;
+      }
+      float det = a0x * a1y - a1x * a0y;
+      if (det == 0.0f) continue;
+      const uint32_t lo = box[1] > y0 ? box[1] : y0, hi = box[3] < y1 ? box[3] : y1;
+      for (uint32_t py = (lo + row_step - 1u) / row_step * row_step; py <= hi; py += row_step) {
+        float ny = 1.0f - ((float)py + 0.5f) * 2.0f / (float)H;
+        for (uint32_t px = box[0]; px <= box[2]; px++) {
+          float nx = ((float)px + 0.5f) * 2.0f / (float)W - 1.0f;
+          float dx = nx - cx, dy = ny - cy;
+          /* the pixel's corner coordinates on the quad c + u a0 + v a1 (the interpolated `corner`) */
+          float u = (dx * a1y - a1x * dy) / det;
+          float v = (a0x * dy - dx * a0y) / det;
+          if (fabsf(u) > 1.0f || fabsf(v) > 1.0f) continue;
+          float lx = u * rr, ly = v * rr;
+          float e = expf_cr(-0.5f * (lx * lx + ly * ly));
+          float alpha = 1.0f - expf_cr(-tau0 * e);
+          if (alpha < ALPHA_MIN) continue;            /* shaders/splat.wgsl:73 (tau_c = 0: cov = 0) */
+          float *d = out + 4 * ((size_t)py * W + px);
+          if (view_mode == 2u) {                      /* overdraw: additive (1, 0, 0, 0) (src/renderer.js:141-143) */
+            d[0] = d[0] + 1.0f;
+            if (mode == 2) d[0] = to_half_rtne(d[0]);
+            continue;
+          }
+          float src[4] = {rgb[0] * alpha, rgb[1] * alpha, rgb[2] * alpha, alpha};
+          float keep = 1.0f - alpha;
+          for (int k = 0; k < 4; k++) {
+            d[k] = src[k] + d[k] * keep;               /* over: src * 1 + dst * (1 - src.a) */
+            if (mode == 2) d[k] = to_half_rtne(d[k]);
+          }
         }
       }
     }
   }
+  free(boxes);
 }
 
 void or_composite_f64(const void *frame, const double *sp, const uint32_t *order, uint32_t visible,
