@@ -54,6 +54,7 @@ def lib():
         L.or_erfc_pos.restype = f32
         L.or_erfc_pos.argtypes = [f32]
         L.or_ein.restype = f32
+        L.or_threads.restype = C.c_uint32
         L.or_ein.argtypes = [f32]
         L.or_footprint.restype = f32
         L.or_footprint.argtypes = [vp, f32, f32, f32, f32]
@@ -178,6 +179,11 @@ def exp_map(x):
 
 def erfc_pos(z):
     return float(lib().or_erfc_pos(z))
+
+
+def threads() -> int:
+    """Threads the oracle's parallel loops run on."""
+    return int(lib().or_threads())
 
 
 def ein(x):

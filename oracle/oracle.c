@@ -9,6 +9,9 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define INVALID_KEY 0xffffffffu
 #define NUM_BUCKETS 65536u
@@ -347,4 +350,13 @@ void or_composite_f64(const void *frame, const double *sp, const uint32_t *order
       }
     }
   }
+}
+
+/* The threads the oracle's parallel loops use (OMP_NUM_THREADS, else the host's cores). */
+uint32_t or_threads(void) {
+#ifdef _OPENMP
+  return (uint32_t)omp_get_max_threads();
+#else
+  return 1u;
+#endif
 }

@@ -17,6 +17,9 @@
 extern "C" {
 #endif
 
+/* threads of the parallel loops (OMP_NUM_THREADS, else the host cores) */
+uint32_t or_threads(void);
+
 /* 16-bit log-depth key (shaders/project.wgsl:366-367) with log fixed as the correctly rounded float log. */
 uint32_t or_depth_key(float depth, float near_, float far_);
 
