@@ -55,6 +55,8 @@ def lib():
         L.or_erfc_pos.argtypes = [f32]
         L.or_ein.restype = f32
         L.or_threads.restype = C.c_uint32
+        L.or_phase_specular.restype = f32
+        L.or_asg_phase.restype = f32
         L.or_ein.argtypes = [f32]
         L.or_footprint.restype = f32
         L.or_footprint.argtypes = [vp, f32, f32, f32, f32]
@@ -179,6 +181,33 @@ def exp_map(x):
 
 def erfc_pos(z):
     return float(lib().or_erfc_pos(z))
+
+
+def _f3(v):
+    return np.ascontiguousarray(v, np.float32).ravel()
+
+
+def phase_specular(Sd, So, wi, wo):
+    """The specular flake phase (material_ext.wgsl:27-37); wo one direction or [n, 3]."""
+    a = [_f3(Sd), _f3(So), _f3(wi)]
+    w = _f3(wo)
+    if w.size == 3:
+        return float(lib().or_phase_specular(*[_p(x) for x in a], _p(w)))
+    out = np.zeros(w.size // 3, np.float32)
+    lib().or_phase_specular_n(*[_p(x) for x in a], _p(w), out.size, _p(out))
+    return out
+
+
+def onb(n):
+    out = np.zeros(6, np.float32)
+    lib().or_onb(_p(_f3(n)), _p(out))
+    return out.reshape(2, 3)
+
+
+def asg_phase(wi, lb, sl, n0, t1, t2, flake, Sd, So, wo):
+    """The specular phase over a light ASG (material_ext.wgsl:128-181); lb [2, 3], sl (xx, xy, yy)."""
+    args = [_f3(x) for x in (wi, lb, sl, n0, t1, t2, flake, Sd, So, wo)]
+    return float(lib().or_asg_phase(*[_p(x) for x in args]))
 
 
 def threads() -> int:

@@ -194,7 +194,7 @@ void or_iso_init(const void *gaussians, const void *nodes, const uint8_t *vgrid,
   }
 }
 
-/* ---- shade_emit (runtime/wgsl/emissive.wgsl:123-150, 853-980), M39 matte ------------------------------------- */
+/* ---- shade_emit (runtime/wgsl/emissive.wgsl:123-150, 853-980), M39 transfer, matte and shiny ----------------- */
 
 /* EmitParams words: count 0, root 1, scale 2, theta 3, mode 4, node_count 5, skip 6, one_sided 7, normal 8,
  * n5 11, edge_soft 12. EmitNode words (24): mean 0, radius 3, m_diag 4, tag 7, m_off 8, child_lo 11, Le 12,
@@ -220,6 +220,187 @@ static float unpack_half(uint32_t h) {
   uint32_t s = (h >> 15) & 1u, e = (h >> 10) & 31u, m = h & 1023u;
   float v = e == 0 ? ldexpf((float)m, -24) : (e == 31 ? INFINITY : ldexpf((float)(m | 1024u), (int)e - 25));
   return s ? -v : v;
+}
+
+/* ---- shiny receivers (runtime/wgsl/material_ext.wgsl:10-181) ------------------------------------------------ */
+
+/* material_ext.wgsl:12-14 */
+static float sym_det(v3 d, v3 o) {
+  return d.x * (d.y * d.z - o.z * o.z) - o.x * (o.x * d.z - o.z * o.y) + o.y * (o.x * o.z - d.y * o.y);
+}
+/* w^T S^-1 w det S via the adjugate (material_ext.wgsl:17-25) */
+static float sym_adj_quad(v3 d, v3 o, v3 w) {
+  float a00 = d.y * d.z - o.z * o.z, a11 = d.x * d.z - o.y * o.y, a22 = d.x * d.y - o.x * o.x;
+  float a01 = o.y * o.z - o.x * d.z, a02 = o.x * o.z - o.y * d.y, a12 = o.x * o.y - d.x * o.z;
+  return a00 * w.x * w.x + a11 * w.y * w.y + a22 * w.z * w.z +
+         2.0f * (a01 * w.x * w.y + a02 * w.x * w.z + a12 * w.y * w.z);
+}
+/* specular flake phase, Heitz et al. 2015 (material_ext.wgsl:27-37) */
+static float phase_specular(v3 Sd, v3 So, v3 wi, v3 wo) {
+  v3 h = add3(wi, wo);
+  float hl = len3(h);
+  if (hl < 1e-6f) return 0.0f;
+  v3 wh = V3(h.x / hl, h.y / hl, h.z / hl);
+  float det = sym_det(Sd, So);
+  det = det > 1e-20f ? det : 1e-20f;
+  float q = sym_adj_quad(Sd, So, wh) / det;
+  float D = 1.0f / (PI_F * sqrtf(det) * q * q);
+  float sq = dot3(wi, sym_mul(Sd, So, wi));
+  float sigma = sqrtf(sq > 1e-12f ? sq : 1e-12f);
+  return D / (4.0f * sigma);
+}
+/* the phase toward n directions wo[3n] (for quadrature) */
+void or_phase_specular_n(const float *Sd, const float *So, const float *wi, const float *wo, uint32_t n, float *out) {
+  v3 d = V3(Sd[0], Sd[1], Sd[2]), o = V3(So[0], So[1], So[2]), i = V3(wi[0], wi[1], wi[2]);
+#pragma omp parallel for schedule(static)
+  for (uint32_t k = 0; k < n; k++) out[k] = phase_specular(d, o, i, V3(wo[3 * k], wo[3 * k + 1], wo[3 * k + 2]));
+}
+float or_phase_specular(const float *Sd, const float *So, const float *wi, const float *wo) {
+  return phase_specular(V3(Sd[0], Sd[1], Sd[2]), V3(So[0], So[1], So[2]), V3(wi[0], wi[1], wi[2]),
+                        V3(wo[0], wo[1], wo[2]));
+}
+/* spec_weight (material_ext.wgsl:39-42): the mirror-flake mix k of a shiny, non-emissive material */
+static float spec_weight(const char *mat, uint32_t mflags) {
+  if ((mflags & 8u) == 0u || (mflags & 1u) != 0u) return 0.0f;
+  return clampf_(unpack_half(rdu(mat, 6) & 0xFFFFu), 0.0f, 1.0f);
+}
+/* iso_lobe_from, iso_damp, mirror (material_ext.wgsl:56-67) */
+static float iso_lobe_from(float k, float p_spec, float damp) { return (1.0f - k) + k * 4.0f * PI_F * p_spec * damp; }
+static float iso_damp(float inorm, float area_vis_r) {
+  float m = 4.0f * PI_F * inorm * area_vis_r;
+  float r = 1.0f / (m > 1e-30f ? m : 1e-30f);
+  return r < 1.0f ? r : 1.0f;
+}
+static v3 mirror(v3 wi, v3 n) { return sub3(mul3s(n, 2.0f * dot3(wi, n)), wi); }
+
+/* ASG lobe fit (material_ext.wgsl:79-80) */
+static const float ASG_W[3] = {0.61163f, 0.35083f, 0.0267f};
+static const float ASG_A[3] = {2.54696f, 0.6893f, 0.10727f};
+/* orthonormal tangent pair, Duff et al. 2017 (material_ext.wgsl:83-88) */
+static void onb(v3 n, v3 out[2]) {
+  float s = n.z >= 0.0f ? 1.0f : -1.0f;
+  float a = -1.0f / (s + n.z);
+  float b = n.x * n.y * a;
+  out[0] = V3(1.0f + s * n.x * n.x * a, s * b, -s * n.x);
+  out[1] = V3(b, s + n.y * n.y * a, -n.y);
+}
+void or_onb(const float *n, float *out) {
+  v3 b[2];
+  onb(V3(n[0], n[1], n[2]), b);
+  out[0] = b[0].x; out[1] = b[0].y; out[2] = b[0].z; out[3] = b[1].x; out[4] = b[1].y; out[5] = b[1].z;
+}
+/* symmetric 2x2 as (xx, xy, yy) (material_ext.wgsl:91-115) */
+static float s2_det(v3 s) { return s.x * s.z - s.y * s.y; }
+static float s2_quad(v3 s, v2 v) { return s.x * v.x * v.x + 2.0f * s.y * v.x * v.y + s.z * v.y * v.y; }
+static float s2_quad_inv(v3 s, v2 v) {
+  float d = s2_det(s);
+  return (s.z * v.x * v.x - 2.0f * s.y * v.x * v.y + s.x * v.y * v.y) / (d > 1e-30f ? d : 1e-30f);
+}
+static v3 s2_xf(v3 s, v2 t0, v2 t1) {
+  v2 a = V2(s.x * t0.x + s.y * t0.y, s.y * t0.x + s.z * t0.y);
+  v2 b = V2(s.x * t1.x + s.y * t1.y, s.y * t1.x + s.z * t1.y);
+  return V3(dot2(t0, a), dot2(t0, b), dot2(t1, b));
+}
+static v3 s2_major(v3 s) {
+  float h = 0.5f * (s.x - s.z);
+  float lam = 0.5f * (s.x + s.z) + sqrtf(h * h + s.y * s.y);
+  v2 v = s.x >= s.z ? V2(lam - s.z, s.y) : V2(s.y, lam - s.x);
+  float l = sqrtf(dot2(v, v));
+  v2 d = l > 1e-20f ? V2(v.x / l, v.y / l) : V2(1.0f, 0.0f);
+  return V3(d.x, d.y, lam);
+}
+static float s2_minor(v3 s) {
+  float h = 0.5f * (s.x - s.z);
+  return 0.5f * (s.x + s.z) - sqrtf(h * h + s.y * s.y);
+}
+/* unit a in the tangent plane at c: geodesic angle times direction (material_ext.wgsl:118-124) */
+static v2 log_map(v3 c, const v3 cb[2], v3 a) {
+  float ca = clampf_(dot3(c, a), -1.0f, 1.0f);
+  v3 v = sub3(a, mul3s(c, ca));
+  float nv = len3(v);
+  if (nv < 1e-9f) return V2(0.0f, 0.0f);
+  float k = acosf(ca) / nv;
+  return V2(k * dot3(cb[0], v), k * dot3(cb[1], v));
+}
+static v3 norm3s(v3 a) { float l = len3(a); return V3(a.x / l, a.y / l, a.z / l); }
+/* the specular phase integrated over a light ASG (material_ext.wgsl:128-181) */
+static float asg_phase(v3 wi, const v3 lb[2], v3 sl, v3 n0, v3 t1, v3 t2, v3 flake, v3 Sd, v3 So, v3 wo) {
+  float pc = phase_specular(Sd, So, wi, wo);
+  float fm = flake.y > flake.z ? flake.y : flake.z;
+  if (flake.x <= fm * 1.0001f) return pc;
+  float co = dot3(n0, wo);
+  v3 n = co >= 0.0f ? n0 : V3(-n0.x, -n0.y, -n0.z);
+  float cos_o = fabsf(co);
+  v3 wr = sub3(mul3s(n, 2.0f * cos_o), wo);
+  v3 rb[2];
+  onb(wr, rb);
+  v3 dw1 = add3(mul3s(n, 2.0f * dot3(wo, t1)), mul3s(t1, 2.0f * cos_o));
+  v3 dw2 = add3(mul3s(n, 2.0f * dot3(wo, t2)), mul3s(t2, 2.0f * cos_o));
+  v2 j0 = V2(dot3(rb[0], dw1), dot3(rb[0], dw2));
+  v2 j1 = V2(dot3(rb[1], dw1), dot3(rb[1], dw2));
+  v2 ia = V2(flake.y / (flake.x - flake.y), flake.z / (flake.x - flake.z));
+  v3 s0 = V3(dot2(V2(j0.x * j0.x, j0.y * j0.y), ia), dot2(V2(j0.x * j1.x, j0.y * j1.y), ia),
+             dot2(V2(j1.x * j1.x, j1.y * j1.y), ia));
+  float qr = dot3(wr, sym_mul(Sd, So, wr));
+  float pk = flake.x * flake.x / (PI_F * sqrtf(flake.x * flake.y * flake.z)) /
+             (4.0f * sqrtf(qr > 1e-12f ? qr : 1e-12f));
+  v3 cs = add3(wi, wr);
+  float cl = len3(cs);
+  if (cl < 1e-4f) return pc;
+  v3 c = V3(cs.x / cl, cs.y / cl, cs.z / cl);
+  v3 cb[2];
+  onb(c, cb);
+  v2 lr = log_map(c, cb, wr), li = log_map(c, cb, wi);
+  v2 dm = V2(lr.x - li.x, lr.y - li.y);
+  v3 sL = s2_xf(sl, V2(dot3(cb[0], lb[0]), dot3(cb[0], lb[1])), V2(dot3(cb[1], lb[0]), dot3(cb[1], lb[1])));
+  v3 sR = s2_xf(s0, V2(dot3(cb[0], rb[0]), dot3(cb[0], rb[1])), V2(dot3(cb[1], rb[0]), dot3(cb[1], rb[1])));
+  float ldm = sqrtf(dot2(dm, dm));
+  float ndm = ldm > 1e-9f ? ldm : 1e-9f;
+  float a2 = 2.0f * ASG_A[2];
+  float d = sqrtf(s2_quad_inv(V3(sL.x + sR.x / a2, sL.y + sR.y / a2, sL.z + sR.z / a2), dm));
+  float qa = s2_quad(sL, dm);
+  float along = sqrtf(qa > 0.0f ? qa : 0.0f) / ndm;
+  float far_ = smoothstep_(1.0f, 2.5f, d) * (1.0f - smoothstep_(0.15f, 0.3f, along / ndm));
+  v3 maj = s2_major(sl);
+  float mn = s2_minor(s0) / (2.0f * ASG_A[0]);
+  float ratio = sqrtf(maj.z / (mn > 1e-30f ? mn : 1e-30f));
+  float t = 1.0f - smoothstep_(0.12f, 0.3f, ratio);
+  t = far_ > t ? far_ : t;
+  float p_asg = 0.0f;
+  if (t < 1.0f) {
+    for (int k = 0; k < 3; k++) {
+      float dk = 2.0f * ASG_A[k];
+      v3 sk = V3(sR.x / dk, sR.y / dk, sR.z / dk);
+      v3 st = add3(sL, sk);
+      float dsk = s2_det(sk), dst = s2_det(st);
+      p_asg += ASG_W[k] * sqrtf((dsk > 0.0f ? dsk : 0.0f) / (dst > 1e-30f ? dst : 1e-30f)) *
+               expf(-0.5f * s2_quad_inv(st, dm));
+    }
+    p_asg *= pk;
+  }
+  float p_pt = pc;
+  if (t > 0.0f) {
+    float sq = sqrtf(3.0f * maj.z);
+    v3 ax = mul3s(add3(mul3s(lb[0], maj.x), mul3s(lb[1], maj.y)), sq);
+    p_pt = (2.0f / 3.0f) * pc + (1.0f / 6.0f) * (phase_specular(Sd, So, norm3s(add3(wi, ax)), wo) +
+                                                 phase_specular(Sd, So, norm3s(sub3(wi, ax)), wo));
+  }
+  return mixf(p_asg, p_pt, t);
+}
+/* asg_phase with its arguments as flat arrays: lb 6 floats, sl 3, n0/t1/t2/flake/Sd/So/wi/wo 3 each */
+float or_asg_phase(const float *wi, const float *lb, const float *sl, const float *n0, const float *t1,
+                   const float *t2, const float *flake, const float *Sd, const float *So, const float *wo) {
+  v3 b[2] = {V3(lb[0], lb[1], lb[2]), V3(lb[3], lb[4], lb[5])};
+  return asg_phase(V3(wi[0], wi[1], wi[2]), b, V3(sl[0], sl[1], sl[2]), V3(n0[0], n0[1], n0[2]),
+                   V3(t1[0], t1[1], t1[2]), V3(t2[0], t2[1], t2[2]), V3(flake[0], flake[1], flake[2]),
+                   V3(Sd[0], Sd[1], Sd[2]), V3(So[0], So[1], So[2]), V3(wo[0], wo[1], wo[2]));
+}
+/* S = R diag(flake) R^T in the rest frame as (diag, off = xy, xz, yz) (emissive.wgsl:141-150) */
+static void flake_matrix(m3 Rq, v3 flake, v3 *Sd, v3 *So) {
+  m3 D = {V3(flake.x, 0.0f, 0.0f), V3(0.0f, flake.y, 0.0f), V3(0.0f, 0.0f, flake.z)};
+  m3 Sl = m3mul(m3mul(Rq, D), m3t(Rq));
+  *Sd = V3(Sl.c0.x, Sl.c1.y, Sl.c2.z);
+  *So = V3(Sl.c1.x, Sl.c2.x, Sl.c2.y);
 }
 
 void or_shade_emit(const void *U, const void *EP, const void *emit_nodes, const void *gaussians, const void *nodes,
@@ -250,6 +431,13 @@ void or_shade_emit(const void *U, const void *EP, const void *emit_nodes, const 
     v3 wo = norm3(sub3(eye, P));
     const uint8_t *vg = vgrid + 128 * (size_t)g;
     v3 acc = V3(0.0f, 0.0f, 0.0f);
+    /* a shiny receiver (emissive.wgsl:868-876, 897-930): its flake matrix, the eye in its rest frame, and whether
+     * each cut node is taken as a shape (EP.n5 bit 3) */
+    float ks = spec_weight(mat, mflags);
+    int shape = ks > 0.0f && (rdu(EP, 11) & 8u) != 0u;
+    v3 flake = rd3(mat, 0), Sd = V3(0.0f, 0.0f, 0.0f), So = V3(0.0f, 0.0f, 0.0f);
+    v3 wo_r = xf_dir_inv(xf, wo);
+    if (ks > 0.0f) flake_matrix(Rq, flake, &Sd, &So);
     uint32_t stack[48];
     stack[0] = rdu(EP, 1);
     uint32_t sp = 1;
@@ -280,7 +468,27 @@ void or_shade_emit(const void *U, const void *EP, const void *emit_nodes, const 
       v3 I = intensity(EP, e, V3(-ww.x, -ww.y, -ww.z));
       float k = fp / d2 * 1.0f;
       v3 light = V3(I.x * vi * k, I.y * vi * k, I.z * vi * k);
-      acc = add3(acc, light);
+      if (ks <= 0.0f) {
+        acc = add3(acc, light);
+        continue;
+      }
+      v3 wr = mirror(wi, Rq.c0);
+      float damp = iso_damp(iso_norm[g], or_footprint(nd, detC, wr.x, wr.y, wr.z) * or_vgrid_lookup(vg, wr.x, wr.y, wr.z));
+      float lobe;
+      if (shape) {
+        /* the node's footprint seen from the receiver, B C_s B^T / d^2, in a tangent basis at ww */
+        v3 bw[2];
+        onb(ww, bw);
+        v3 cd = rd3(e, 16), cof = rd3(e, 20);
+        v3 c0 = sym_mul(cd, cof, bw[0]), c1 = sym_mul(cd, cof, bw[1]);
+        v3 sl = V3(dot3(bw[0], c0) / d2, dot3(bw[0], c1) / d2, dot3(bw[1], c1) / d2);
+        v3 lb[2] = {xf_dir_inv(xf, bw[0]), xf_dir_inv(xf, bw[1])};
+        float p = asg_phase(wi, lb, sl, Rq.c0, Rq.c1, Rq.c2, flake, Sd, So, wo_r);
+        lobe = iso_lobe_from(ks, p, damp);
+      } else {
+        lobe = iso_lobe_from(ks, phase_specular(Sd, So, wi, wo_r), damp);
+      }
+      acc = add3(acc, mul3s(light, lobe));
     }
     float T = lamp_T ? lamp_T[g] : 1.0f;
     float inorm = iso_norm[g];
@@ -704,7 +912,7 @@ void or_sun_lookup(const void *U, const void *SM, const void *desc, const float 
   }
 }
 
-/* ---- env_direct (environment.wgsl:207-229, 266-277), matte ------------------------------------------------------- */
+/* ---- env_direct (environment.wgsl:207-229, 266-307), matte and shiny -------------------------------------------- */
 
 /* Env words: sh 0-63, lights 64 + 8k (dir_std, irr), info 96. */
 void or_env_direct(const void *U, const void *ENV, const void *gaussians, const void *nodes, const uint8_t *vgrid,
@@ -721,6 +929,41 @@ void or_env_direct(const void *U, const void *ENV, const void *gaussians, const 
     const char *nd = (const char *)nodes + 96 * (size_t)g;
     const char *xf = (const char *)parts + 128 * (size_t)(rdu(G, 11) & 0xFFFFu);
     float s3 = rdf(G, 8) * rdf(G, 9) * rdf(G, 10);
+    const char *mat = (const char *)materials + 32 * (size_t)(mi < mmax ? mi : mmax);
+    float ks = mcount ? spec_weight(mat, rdu(mat, 5)) : 0.0f;
+    if (ks > 0.0f) {
+      /* shiny (environment.wgsl:278-307): (1 - k) of the matte light plus each light's highlight, its shape the
+       * light's ASG of angular std dir_std.w */
+      v3 P = xf_point(xf, rd3(G, 0));
+      v3 wo = xf_dir_inv(xf, norm3(sub3(rd3(U, 8), P)));
+      m3 Rq = quat_matrix(rdf(G, 4), rdf(G, 5), rdf(G, 6), rdf(G, 7));
+      v3 flake = rd3(mat, 0), Sd, So;
+      flake_matrix(Rq, flake, &Sd, &So);
+      float inorm = iso_norm[g];
+      v3 diff = V3(0.0f, 0.0f, 0.0f), spec = V3(0.0f, 0.0f, 0.0f);
+      for (uint32_t k = 0; k < (nlights < 4u ? nlights : 4u); k++) {
+        v3 wi = xf_dir_inv(xf, rd3(ENV, 64 + 8 * k));
+        float v = or_vgrid_lookup(vgrid + 128 * (size_t)g, wi.x, wi.y, wi.z);
+        if (v <= 1e-7f) continue;
+        v3 light = mul3s(rd3(ENV, 68 + 8 * k), env_T[4 * (size_t)g + k] * v * or_footprint(nd, s3 * s3, wi.x, wi.y, wi.z));
+        v3 wr = mirror(wi, Rq.c0);
+        float damp = iso_damp(inorm, or_footprint(nd, s3 * s3, wr.x, wr.y, wr.z) *
+                                         or_vgrid_lookup(vgrid + 128 * (size_t)g, wr.x, wr.y, wr.z));
+        float sd = rdf(ENV, 67 + 8 * k);
+        float s2 = sd * sd;
+        v3 lb[2];
+        onb(wi, lb);
+        float p = asg_phase(wi, lb, V3(s2, 0.0f, s2), Rq.c0, Rq.c1, Rq.c2, flake, Sd, So, wo);
+        diff = add3(diff, mul3s(light, 1.0f - ks));
+        spec = add3(spec, mul3s(light, iso_lobe_from(ks, p, damp) - (1.0f - ks)));
+      }
+      v3 alb = rd3(G, 12);
+      v3 a = V3(alb.x * inorm, alb.y * inorm, alb.z * inorm);
+      colour[4 * (size_t)g + 0] = colour[4 * (size_t)g + 0] + a.x * (diff.x + spec.x);
+      colour[4 * (size_t)g + 1] = colour[4 * (size_t)g + 1] + a.y * (diff.y + spec.y);
+      colour[4 * (size_t)g + 2] = colour[4 * (size_t)g + 2] + a.z * (diff.z + spec.z);
+      continue;
+    }
     v3 acc = V3(0.0f, 0.0f, 0.0f);
     for (uint32_t k = 0; k < (nlights < 4u ? nlights : 4u); k++) {
       v3 wi = xf_dir_inv(xf, rd3(ENV, 64 + 8 * k));

@@ -120,6 +120,14 @@ void or_bounce_sh(const void *U, const void *PP, const void *gaussians, const vo
                   const float *cells, const float *probe_data, const float *probe_sh, const float *xfer, float *lo_b);
 void or_probe_pos(const void *U, const void *PP, const void *nodes, const void *parts, uint32_t count, float *out);
 
+/* shiny receivers (runtime/wgsl/material_ext.wgsl): the specular flake phase, the Duff et al. tangent pair, the
+ * specular phase over a light ASG (lb: 2 x 3 floats, sl: (xx, xy, yy)) */
+float or_phase_specular(const float *Sd, const float *So, const float *wi, const float *wo);
+void or_phase_specular_n(const float *Sd, const float *So, const float *wi, const float *wo, uint32_t n, float *out);
+void or_onb(const float *n, float *out6);
+float or_asg_phase(const float *wi, const float *lb, const float *sl, const float *n0, const float *t1,
+                   const float *t2, const float *flake, const float *Sd, const float *So, const float *wo);
+
 #ifdef __cplusplus
 }
 #endif

@@ -189,6 +189,148 @@ __device__ __forceinline__ m3 leaf_cov(float4 q, float4 s, m3 R) {
 }
 __device__ __forceinline__ float unpack_half(uint32_t h) { return __half2float(__ushort_as_half((unsigned short)h)); }
 
+// ---- shiny receivers (runtime/wgsl/material_ext.wgsl:10-181) -----------------------------------------------------
+// A material with flags bit 3 mixes mirror flakes (weight k, pack2x16float(k, 0) in pad0) into the matte transfer:
+// each light's term takes iso_lobe = (1 - k) + k 4 pi p damp, p the specular flake phase (Heitz et al. 2015) for a
+// point light or its integral over the light's angular ASG when lights are shapes (EmitParams.n5 bit 3).
+__device__ __forceinline__ float sym_det(v3 d, v3 o) {
+  return d.x * (d.y * d.z - o.z * o.z) - o.x * (o.x * d.z - o.z * o.y) + o.y * (o.x * o.z - d.y * o.y);
+}
+__device__ __forceinline__ float sym_adj_quad(v3 d, v3 o, v3 w) {   // w^T S^-1 w det S via the adjugate
+  const float a00 = d.y * d.z - o.z * o.z, a11 = d.x * d.z - o.y * o.y, a22 = d.x * d.y - o.x * o.x;
+  const float a01 = o.y * o.z - o.x * d.z, a02 = o.x * o.z - o.y * d.y, a12 = o.x * o.y - d.x * o.z;
+  return a00 * w.x * w.x + a11 * w.y * w.y + a22 * w.z * w.z +
+         2.0f * (a01 * w.x * w.y + a02 * w.x * w.z + a12 * w.y * w.z);
+}
+__device__ __forceinline__ float phase_specular(v3 Sd, v3 So, v3 wi, v3 wo) {   // material_ext.wgsl:27-37
+  const v3 h = add3(wi, wo);
+  const float hl = len3(h);
+  if (hl < 1e-6f) return 0.0f;
+  const v3 wh = V3(h.x / hl, h.y / hl, h.z / hl);
+  const float det = fmaxf(sym_det(Sd, So), 1e-20f);
+  const float q = sym_adj_quad(Sd, So, wh) / det;
+  const float D = 1.0f / (PI_F * sqrtf(det) * q * q);
+  const float sigma = sqrtf(fmaxf(dot3(wi, sym_mul(Sd, So, wi)), 1e-12f));
+  return D / (4.0f * sigma);
+}
+__device__ __forceinline__ float spec_weight(const uint32_t *mat, uint32_t mflags) {   // material_ext.wgsl:39-42
+  if ((mflags & 8u) == 0u || (mflags & 1u) != 0u) return 0.0f;
+  return clampf_(unpack_half(mat[6] & 0xFFFFu), 0.0f, 1.0f);
+}
+__device__ __forceinline__ float iso_lobe_from(float k, float p_spec, float damp) {
+  return (1.0f - k) + k * 4.0f * PI_F * p_spec * damp;
+}
+__device__ __forceinline__ float iso_damp(float inorm, float area_vis_r) {
+  return fminf(1.0f, 1.0f / fmaxf(4.0f * PI_F * inorm * area_vis_r, 1e-30f));
+}
+__device__ __forceinline__ v3 mirror(v3 wi, v3 n) { return sub3(mul3s(n, 2.0f * dot3(wi, n)), wi); }
+__device__ __forceinline__ void onb(v3 n, v3 &b0, v3 &b1) {   // Duff et al. 2017 (material_ext.wgsl:83-88)
+  const float s = n.z >= 0.0f ? 1.0f : -1.0f;
+  const float a = -1.0f / (s + n.z);
+  const float b = n.x * n.y * a;
+  b0 = V3(1.0f + s * n.x * n.x * a, s * b, -s * n.x);
+  b1 = V3(b, s + n.y * n.y * a, -n.y);
+}
+__device__ __forceinline__ float s2_det(v3 s) { return s.x * s.z - s.y * s.y; }
+__device__ __forceinline__ float s2_quad(v3 s, v2 v) { return s.x * v.x * v.x + 2.0f * s.y * v.x * v.y + s.z * v.y * v.y; }
+__device__ __forceinline__ float s2_quad_inv(v3 s, v2 v) {
+  return (s.z * v.x * v.x - 2.0f * s.y * v.x * v.y + s.x * v.y * v.y) / fmaxf(s2_det(s), 1e-30f);
+}
+__device__ __forceinline__ v3 s2_xf(v3 s, v2 t0, v2 t1) {
+  const v2 a = V2(s.x * t0.x + s.y * t0.y, s.y * t0.x + s.z * t0.y);
+  const v2 b = V2(s.x * t1.x + s.y * t1.y, s.y * t1.x + s.z * t1.y);
+  return V3(dot2(t0, a), dot2(t0, b), dot2(t1, b));
+}
+__device__ __forceinline__ v3 s2_major(v3 s) {
+  const float h = 0.5f * (s.x - s.z);
+  const float lam = 0.5f * (s.x + s.z) + sqrtf(h * h + s.y * s.y);
+  const v2 v = s.x >= s.z ? V2(lam - s.z, s.y) : V2(s.y, lam - s.x);
+  const float l = sqrtf(dot2(v, v));
+  return l > 1e-20f ? V3(v.x / l, v.y / l, lam) : V3(1.0f, 0.0f, lam);
+}
+__device__ __forceinline__ float s2_minor(v3 s) {
+  const float h = 0.5f * (s.x - s.z);
+  return 0.5f * (s.x + s.z) - sqrtf(h * h + s.y * s.y);
+}
+__device__ __forceinline__ v2 log_map(v3 c, v3 cb0, v3 cb1, v3 a) {   // material_ext.wgsl:118-124
+  const float ca = clampf_(dot3(c, a), -1.0f, 1.0f);
+  const v3 v = sub3(a, mul3s(c, ca));
+  const float nv = len3(v);
+  if (nv < 1e-9f) return V2(0.0f, 0.0f);
+  const float k = acosf(ca) / nv;
+  return V2(k * dot3(cb0, v), k * dot3(cb1, v));
+}
+__device__ __forceinline__ float smooth_(float e0, float e1, float x) {
+  const float t = clampf_((x - e0) / (e1 - e0), 0.0f, 1.0f);
+  return t * t * (3.0f - 2.0f * t);
+}
+// the specular phase integrated over a light ASG centred on wi, covariance sl (xx, xy, yy) in basis lb0, lb1
+// (material_ext.wgsl:128-181); n0, t1, t2 the receiver's axes, flake their eigenvalues
+__device__ __noinline__ float asg_phase(v3 wi, v3 lb0, v3 lb1, v3 sl, v3 n0, v3 t1, v3 t2, v3 flake, v3 Sd, v3 So,
+                                        v3 wo) {
+  const float pc = phase_specular(Sd, So, wi, wo);
+  if (flake.x <= fmaxf(flake.y, flake.z) * 1.0001f) return pc;
+  const float co = dot3(n0, wo);
+  const v3 n = co >= 0.0f ? n0 : V3(-n0.x, -n0.y, -n0.z);
+  const float cos_o = fabsf(co);
+  const v3 wr = sub3(mul3s(n, 2.0f * cos_o), wo);
+  v3 rb0, rb1;
+  onb(wr, rb0, rb1);
+  const v3 dw1 = add3(mul3s(n, 2.0f * dot3(wo, t1)), mul3s(t1, 2.0f * cos_o));
+  const v3 dw2 = add3(mul3s(n, 2.0f * dot3(wo, t2)), mul3s(t2, 2.0f * cos_o));
+  const v2 j0 = V2(dot3(rb0, dw1), dot3(rb0, dw2));
+  const v2 j1 = V2(dot3(rb1, dw1), dot3(rb1, dw2));
+  const v2 ia = V2(flake.y / (flake.x - flake.y), flake.z / (flake.x - flake.z));
+  const v3 s0 = V3(dot2(V2(j0.x * j0.x, j0.y * j0.y), ia), dot2(V2(j0.x * j1.x, j0.y * j1.y), ia),
+                   dot2(V2(j1.x * j1.x, j1.y * j1.y), ia));
+  const float pk = flake.x * flake.x / (PI_F * sqrtf(flake.x * flake.y * flake.z)) /
+                   (4.0f * sqrtf(fmaxf(dot3(wr, sym_mul(Sd, So, wr)), 1e-12f)));
+  const v3 cs = add3(wi, wr);
+  const float cl = len3(cs);
+  if (cl < 1e-4f) return pc;
+  const v3 c = V3(cs.x / cl, cs.y / cl, cs.z / cl);
+  v3 cb0, cb1;
+  onb(c, cb0, cb1);
+  const v2 lr = log_map(c, cb0, cb1, wr), li = log_map(c, cb0, cb1, wi);
+  const v2 dm = V2(lr.x - li.x, lr.y - li.y);
+  const v3 sL = s2_xf(sl, V2(dot3(cb0, lb0), dot3(cb0, lb1)), V2(dot3(cb1, lb0), dot3(cb1, lb1)));
+  const v3 sR = s2_xf(s0, V2(dot3(cb0, rb0), dot3(cb0, rb1)), V2(dot3(cb1, rb0), dot3(cb1, rb1)));
+  const float ndm = fmaxf(sqrtf(dot2(dm, dm)), 1e-9f);
+  const float a2 = 2.0f * 0.10727f;
+  const float d = sqrtf(s2_quad_inv(V3(sL.x + sR.x / a2, sL.y + sR.y / a2, sL.z + sR.z / a2), dm));
+  const float along = sqrtf(fmaxf(s2_quad(sL, dm), 0.0f)) / ndm;
+  const float far_ = smooth_(1.0f, 2.5f, d) * (1.0f - smooth_(0.15f, 0.3f, along / ndm));
+  const v3 maj = s2_major(sl);
+  const float ratio = sqrtf(maj.z / fmaxf(s2_minor(s0) / (2.0f * 2.54696f), 1e-30f));
+  const float t = fmaxf(far_, 1.0f - smooth_(0.12f, 0.3f, ratio));
+  float p_asg = 0.0f;
+  if (t < 1.0f) {
+    const float aw[3] = {0.61163f, 0.35083f, 0.0267f}, aa[3] = {2.54696f, 0.6893f, 0.10727f};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const float dk = 2.0f * aa[k];
+      const v3 sk = V3(sR.x / dk, sR.y / dk, sR.z / dk);
+      const v3 st = add3(sL, sk);
+      p_asg += aw[k] * sqrtf(fmaxf(s2_det(sk), 0.0f) / fmaxf(s2_det(st), 1e-30f)) * expf(-0.5f * s2_quad_inv(st, dm));
+    }
+    p_asg *= pk;
+  }
+  float p_pt = pc;
+  if (t > 0.0f) {
+    const v3 ax = mul3s(add3(mul3s(lb0, maj.x), mul3s(lb1, maj.y)), sqrtf(3.0f * maj.z));
+    p_pt = (2.0f / 3.0f) * pc + (1.0f / 6.0f) * (phase_specular(Sd, So, norm3(add3(wi, ax)), wo) +
+                                                 phase_specular(Sd, So, norm3(sub3(wi, ax)), wo));
+  }
+  return mixf(p_asg, p_pt, t);
+}
+// S = R diag(flake) R^T in the rest frame as (diag, off = xy, xz, yz) (emissive.wgsl:141-150)
+__device__ __forceinline__ void flake_matrix(const m3 &Rq, v3 flake, v3 &Sd, v3 &So) {
+  const m3 D = {V3(flake.x, 0.0f, 0.0f), V3(0.0f, flake.y, 0.0f), V3(0.0f, 0.0f, flake.z)};
+  const m3 Sl = m3mul(m3mul(Rq, D), m3t(Rq));
+  Sd = V3(Sl.c0.x, Sl.c1.y, Sl.c2.z);
+  So = V3(Sl.c1.x, Sl.c2.x, Sl.c2.y);
+}
+
 }  // namespace lt
 
 // Uniform layouts (the bytes the host writes)
@@ -286,11 +428,54 @@ sl_iso_init(uint32_t n, const float4 *__restrict__ g2, uint32_t st, const float4
   iso_norm[g] = 1.0f / (m > 1e-30f ? m : 1e-30f);
 }
 
-// ---- shade_emit (runtime/wgsl/emissive.wgsl:123-150, 853-980), M39 matte ----------------------------------------
+// ---- shade_emit (runtime/wgsl/emissive.wgsl:123-150, 853-980), M39 transfer, matte and shiny -----------------------
+
+// A shiny receiver's set-up (emissive.wgsl:141-150, 868-876): its mirror-flake weight, whether the cut's nodes are
+// taken as shapes (EmitParams.n5 bit 3), its axes, flake matrix and the eye in its rest frame, and its iso_norm.
+struct Shiny {
+  float ks, inorm;
+  bool shape;
+  v3 n, t1, t2, flake, Sd, So, wo;
+};
+__device__ __forceinline__ Shiny shiny_receiver(const uint32_t *mat, uint32_t mflags, uint32_t n5, const m3 &Rq,
+                                                v3 wo_rest, float inorm) {
+  Shiny r;
+  r.ks = spec_weight(mat, mflags);
+  r.shape = r.ks > 0.0f && (n5 & 8u) != 0u;
+  r.n = Rq.c0;
+  r.t1 = Rq.c1;
+  r.t2 = Rq.c2;
+  r.flake = V3(__uint_as_float(mat[0]), __uint_as_float(mat[1]), __uint_as_float(mat[2]));
+  r.Sd = r.So = V3(0.0f, 0.0f, 0.0f);
+  if (r.ks > 0.0f) flake_matrix(Rq, r.flake, r.Sd, r.So);
+  r.wo = wo_rest;
+  r.inorm = inorm;
+  return r;
+}
+// iso_lobe of one cut node's light on a shiny receiver (emissive.wgsl:897-930): damped toward the mirror direction,
+// the node a shape (its covariance words 16-22 seen from the receiver, B C_s B^T / d^2) or a point
+template <typename Cell>
+__device__ __forceinline__ float shiny_lobe(const Shiny &r, const NodeL &nd, float detC, Cell vcell, const Xf &xf,
+                                            v3 wi, v3 ww, float d2, const float4 *e) {
+  const v3 wr = mirror(wi, r.n);
+  const float damp = iso_damp(r.inorm, footprint(nd, detC, wr) * vgrid_lookup_f(vcell, wr));
+  if (r.shape) {
+    v3 b0, b1;
+    onb(ww, b0, b1);
+    const float4 e4 = e[4], e5 = e[5];
+    const v3 cd = V3(e4.x, e4.y, e4.z), co = V3(e5.x, e5.y, e5.z);
+    const v3 c0 = sym_mul(cd, co, b0), c1 = sym_mul(cd, co, b1);
+    const v3 sl = V3(dot3(b0, c0) / d2, dot3(b0, c1) / d2, dot3(b1, c1) / d2);
+    const float p = asg_phase(wi, xf_dir_inv(xf, b0), xf_dir_inv(xf, b1), sl, r.n, r.t1, r.t2, r.flake, r.Sd, r.So,
+                              r.wo);
+    return iso_lobe_from(r.ks, p, damp);
+  }
+  return iso_lobe_from(r.ks, phase_specular(r.Sd, r.So, wi, r.wo), damp);
+}
 
 // One cut walk per lane. STACK: the walk's stack depth in LDS, 48 as the reference's; used for trees whose full
 // walk can hold more than 16 entries (sl_shade_emit_wave below takes the others).
-template <uint32_t STACK>
+template <uint32_t STACK, bool SHINY>
 __global__ void __launch_bounds__(64)
 sl_shade_emit(FrameUniformsDev U, EmitParamsDev EP, const uint32_t *__restrict__ order,
               const float4 *__restrict__ emit_nodes, uint32_t n_nodes,
@@ -319,6 +504,8 @@ sl_shade_emit(FrameUniformsDev U, EmitParamsDev EP, const uint32_t *__restrict__
   const float detC = s3 * s3;
   const v3 wo = norm3(sub3(V3(U.eye[0], U.eye[1], U.eye[2]), P));
   const NodeL nd = load_node(nodes, g);
+  Shiny sh{};
+  if constexpr (SHINY) sh = shiny_receiver(mat, mflags, EP.n5, Rq, xf_dir_inv(xf, wo), iso_norm[g]);
   // the Gaussian's 128 B visibility grid, read once into LDS as 32 words per lane in a [word][lane] layout (lanes
   // on distinct banks whatever cells they read): the walk looks up 4 cells per used node, and re-reading them from
   // memory costs a cache line each time once other waves have evicted it
@@ -373,7 +560,8 @@ sl_shade_emit(FrameUniformsDev U, EmitParamsDev EP, const uint32_t *__restrict__
     const float d2 = dot3(d, d);
     if (d2 < 1e-8f) continue;
     const float sd = sqrtf(d2), isd = __builtin_amdgcn_rcpf(sd);
-    const v3 ww = V3(d.x * isd, d.y * isd, d.z * isd);
+    // shiny receivers divide exactly: the ASG phase differences two nearly equal directions near the mirror one
+    const v3 ww = SHINY ? V3(d.x / sd, d.y / sd, d.z / sd) : V3(d.x * isd, d.y * isd, d.z * isd);
     const v3 wi = xf_dir_inv(xf, ww);
     const float vi = vgrid_lookup_f(vcell, wi);
     if (vi <= 1e-7f) continue;
@@ -393,7 +581,11 @@ sl_shade_emit(FrameUniformsDev U, EmitParamsDev EP, const uint32_t *__restrict__
     const float q = dot3(w, sym_mul(V3(e1.x, e1.y, e1.z), V3(e2.x, e2.y, e2.z), w));
     const float ik = s * EP.scale * sqrtf(q > 0.0f ? q : 0.0f);
     const float k = qdiv(fp, d2);
-    acc = add3(acc, V3(e3.x * ik * vi * k, e3.y * ik * vi * k, e3.z * ik * vi * k));
+    v3 light = V3(e3.x * ik * vi * k, e3.y * ik * vi * k, e3.z * ik * vi * k);
+    if constexpr (SHINY) {
+      if (sh.ks > 0.0f) light = mul3s(light, shiny_lobe(sh, nd, detC, vcell, xf, wi, ww, d2, e));
+    }
+    acc = add3(acc, light);
   }
   const float T = lamp_T ? lamp_T[g] : 1.0f;
   const float tk = T * iso_norm[g];
@@ -413,6 +605,7 @@ sl_shade_emit(FrameUniformsDev U, EmitParamsDev EP, const uint32_t *__restrict__
 // below the node's stack slot). Each lane so meets its own used nodes in its own order and sums them exactly as
 // sl_shade_emit does. The node index, the node's words and the stack (16 x 16-bit entries in four 64-bit words) are
 // the same for every lane, so they live in scalar registers; only the V-grid uses LDS (8 KiB per wave).
+template <bool SHINY>
 __global__ void __launch_bounds__(64)
 sl_shade_emit_wave(FrameUniformsDev U, EmitParamsDev EP, const uint32_t *__restrict__ order,
                    const float4 *__restrict__ emit_nodes, uint32_t n_nodes,
@@ -442,6 +635,8 @@ sl_shade_emit_wave(FrameUniformsDev U, EmitParamsDev EP, const uint32_t *__restr
   const float detC = s3 * s3;
   const v3 wo = norm3(sub3(V3(U.eye[0], U.eye[1], U.eye[2]), P));
   const NodeL nd = load_node(nodes, g);
+  Shiny sh{};
+  if constexpr (SHINY) sh = shiny_receiver(mat, mflags, EP.n5, Rq, xf_dir_inv(xf, wo), iso_norm[g]);
   __shared__ uint32_t s_vg[32][64];                    // as in sl_shade_emit
   uint32_t *vcol = &s_vg[0][threadIdx.x];
   {
@@ -503,7 +698,7 @@ sl_shade_emit_wave(FrameUniformsDev U, EmitParamsDev EP, const uint32_t *__restr
       const float d2 = dot3(d, d);
       if (d2 >= 1e-8f) {
         const float sd = sqrtf(d2), isd = __builtin_amdgcn_rcpf(sd);
-        const v3 ww = V3(d.x * isd, d.y * isd, d.z * isd);
+        const v3 ww = SHINY ? V3(d.x / sd, d.y / sd, d.z / sd) : V3(d.x * isd, d.y * isd, d.z * isd);   // as above
         const v3 wi = xf_dir_inv(xf, ww);
         const float vi = vgrid_lookup_f(vcell, wi);
         if (vi > 1e-7f) {
@@ -523,7 +718,11 @@ sl_shade_emit_wave(FrameUniformsDev U, EmitParamsDev EP, const uint32_t *__restr
           const float q = dot3(w, sym_mul(V3(e1.x, e1.y, e1.z), V3(e2.x, e2.y, e2.z), w));
           const float ik = s * EP.scale * sqrtf(q > 0.0f ? q : 0.0f);
           const float k = qdiv(fp, d2);
-          acc = add3(acc, V3(e3.x * ik * vi * k, e3.y * ik * vi * k, e3.z * ik * vi * k));
+          v3 light = V3(e3.x * ik * vi * k, e3.y * ik * vi * k, e3.z * ik * vi * k);
+          if constexpr (SHINY) {
+            if (sh.ks > 0.0f) light = mul3s(light, shiny_lobe(sh, nd, detC, vcell, xf, wi, ww, d2, e));
+          }
+          acc = add3(acc, light);
         }
       }
     }
@@ -1288,8 +1487,10 @@ sl_sun_lookup(uint32_t n, SunParamsDev SM, const SunLayerDesc *__restrict__ desc
 
 // ---- env_direct (environment.wgsl:207-229, 266-277), matte -------------------------------------------------------
 
-extern "C" __global__ void __launch_bounds__(64)
-sl_env_direct(FrameUniformsDev U, EnvDev ENV, const float4 *__restrict__ g1, const float4 *__restrict__ g2,
+template <bool SHINY>
+__global__ void __launch_bounds__(64)
+sl_env_direct(FrameUniformsDev U, EnvDev ENV, const float4 *__restrict__ g0, const float4 *__restrict__ g1,
+              const float4 *__restrict__ g2,
               const float4 *__restrict__ g3, uint32_t st, const float4 *__restrict__ nodes, const uint8_t *__restrict__ vgrid, const float4 *__restrict__ parts,
               const uint32_t *__restrict__ materials, const float *__restrict__ iso_norm,
               const float4 *__restrict__ env_T, float4 *__restrict__ colour) {
@@ -1298,12 +1499,52 @@ sl_env_direct(FrameUniformsDev U, EnvDev ENV, const float4 *__restrict__ g1, con
   const float4 S4 = sl_scale(g2, st, g), C = sl_albedo(g1, g2, g3, st, g);
   const uint32_t mi = __float_as_uint(C.w) & 0xFFFFu;
   const uint32_t mmax = max(U.material_count, 1u) - 1u;
-  if (U.material_count && (materials[8 * (size_t)min(mi, mmax) + 5] & 1u)) return;
+  const uint32_t *mat = materials + 8 * (size_t)min(mi, mmax);
+  if (U.material_count && (mat[5] & 1u)) return;
   const NodeL nd = load_node(nodes, g);
   const Xf xf = load_xf(parts, __float_as_uint(S4.w) & 0xFFFFu);
   const float s3 = S4.x * S4.y * S4.z;
   const float4 Tg = env_T ? env_T[g] : make_float4(1.0f, 1.0f, 1.0f, 1.0f);
   const float Tk[4] = {Tg.x, Tg.y, Tg.z, Tg.w};
+  if constexpr (SHINY) {
+    const float ks = U.material_count ? spec_weight(mat, mat[5]) : 0.0f;
+    if (ks > 0.0f) {
+      // shiny (environment.wgsl:278-307): (1 - k) of the matte light plus each light's highlight, its shape the
+      // light's ASG of angular std dir_std.w
+      const float4 A = g0[g], Q = sl_quat(g1, st, g);
+      const v3 P = xf_point(xf, V3(A.x, A.y, A.z));
+      const v3 wo = xf_dir_inv(xf, norm3(sub3(V3(U.eye[0], U.eye[1], U.eye[2]), P)));
+      const m3 Rq = quat_matrix(Q);
+      const v3 flake = V3(__uint_as_float(mat[0]), __uint_as_float(mat[1]), __uint_as_float(mat[2]));
+      v3 Sd, So;
+      flake_matrix(Rq, flake, Sd, So);
+      const float inorm = iso_norm[g];
+      const uint8_t *vg = vgrid + 128 * (size_t)g;
+      v3 diff = V3(0.0f, 0.0f, 0.0f), spec = V3(0.0f, 0.0f, 0.0f);
+      const uint32_t nl = min(ENV.info[0], 4u);
+      for (uint32_t k = 0; k < nl; k++) {
+        const v3 wi = xf_dir_inv(xf, V3(ENV.lights[k][0], ENV.lights[k][1], ENV.lights[k][2]));
+        const float v = vgrid_lookup(vg, wi);
+        if (v <= 1e-7f) continue;
+        const v3 light = mul3s(V3(ENV.lights[k][4], ENV.lights[k][5], ENV.lights[k][6]),
+                               Tk[k] * v * footprint(nd, s3 * s3, wi));
+        const v3 wr = mirror(wi, Rq.c0);
+        const float damp = iso_damp(inorm, footprint(nd, s3 * s3, wr) * vgrid_lookup(vg, wr));
+        const float sd = ENV.lights[k][3];
+        const float s2 = sd * sd;
+        v3 lb0, lb1;
+        onb(wi, lb0, lb1);
+        const float p = asg_phase(wi, lb0, lb1, V3(s2, 0.0f, s2), Rq.c0, Rq.c1, Rq.c2, flake, Sd, So, wo);
+        diff = add3(diff, mul3s(light, 1.0f - ks));
+        spec = add3(spec, mul3s(light, iso_lobe_from(ks, p, damp) - (1.0f - ks)));
+      }
+      const v3 a = V3(C.x * inorm, C.y * inorm, C.z * inorm);
+      const float4 c = colour[g];
+      colour[g] = make_float4(c.x + a.x * (diff.x + spec.x), c.y + a.y * (diff.y + spec.y),
+                              c.z + a.z * (diff.z + spec.z), c.w);
+      return;
+    }
+  }
   v3 acc = V3(0.0f, 0.0f, 0.0f);
   const uint32_t nl = min(ENV.info[0], 4u);
   for (uint32_t k = 0; k < nl; k++) {

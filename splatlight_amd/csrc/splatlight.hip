@@ -1495,11 +1495,14 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
     return fail(ctx, SL_ERR_INVALID, "FrameUniforms.material_count does not match the scene's materials");
   if ((U.iso & 1u) == 0u || (U.iso & (8u | 16u)) != 0u)
     return fail(ctx, SL_ERR_UNSUPPORTED, "only the M39 transfer without transmission or reflection split is on this path");
+  // shiny receivers (material flags bit 3, not emissive): the shading and env_direct take their specular variants
+  bool shiny = false;
   for (uint32_t m = 0; m < U.material_count; m++) {
     const uint32_t fl = rd_u32(sc->materials_host.data() + 32 * (size_t)m, 5);
-    if ((fl & 1u) == 0u && (fl & 8u) != 0u)
-      return fail(ctx, SL_ERR_UNSUPPORTED, "shiny materials (flags bit 3) are not on this path");
+    shiny = shiny || ((fl & 1u) == 0u && (fl & 8u) != 0u);
   }
+  if (shiny && (EP.n5 & 16u) != 0u)
+    return fail(ctx, SL_ERR_UNSUPPORTED, "the spec cut (EmitParams.n5 bit 4) is not on this path");
   if (EP.lights != 0u) return fail(ctx, SL_ERR_UNSUPPORTED, "unified lights are not on this path");
   if (d->emit_node_count == 0 || EP.root >= d->emit_node_count || (EP.count && d->emit_node_count < 2 * EP.count - 1))
     return fail(ctx, SL_ERR_INVALID, "emitter tree does not match EmitParams");
@@ -1695,16 +1698,16 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
   // with a bounce set up, the shading also keeps each Gaussian's emitter light before its own emission (the light
   // direct_dirs_emit sends toward the probes) for the next sl_bounce_encode
   float4 *c0_out = sc->bounce.on ? sc->bounce.c0 : nullptr;
-  if (emit_stack_need((const uint8_t *)d->emit_nodes, d->emit_node_count, EP.root) <= 16u)
-    sl_shade_emit_wave<<<gblocks, 64, 0, st>>>(U, EP, sc->order, (const float4 *)sc->emit_nodes.p, d->emit_node_count,
-                                               sc->g[0], sc->g[1], sc->g[2], sc->g[3], sc->storage, sc->nodes,
-                                               sc->vgrid, sc->parts, lampT, sc->materials, sc->iso_norm, sc->colour,
-                                               c0_out);
+  const bool wave_walk = emit_stack_need((const uint8_t *)d->emit_nodes, d->emit_node_count, EP.root) <= 16u;
+  auto shade = [&](auto kernel) {
+    kernel<<<gblocks, 64, 0, st>>>(U, EP, sc->order, (const float4 *)sc->emit_nodes.p, d->emit_node_count, sc->g[0],
+                                   sc->g[1], sc->g[2], sc->g[3], sc->storage, sc->nodes, sc->vgrid, sc->parts, lampT,
+                                   sc->materials, sc->iso_norm, sc->colour, c0_out);
+  };
+  if (wave_walk)
+    shiny ? shade(sl_shade_emit_wave<true>) : shade(sl_shade_emit_wave<false>);
   else
-    sl_shade_emit<48><<<gblocks, 64, 0, st>>>(U, EP, sc->order, (const float4 *)sc->emit_nodes.p, d->emit_node_count,
-                                              sc->g[0], sc->g[1], sc->g[2], sc->g[3], sc->storage, sc->nodes,
-                                              sc->vgrid, sc->parts, lampT, sc->materials, sc->iso_norm, sc->colour,
-                                              c0_out);
+    shiny ? shade(sl_shade_emit<48, true>) : shade(sl_shade_emit<48, false>);
   if (timers) SL_HIP(ctx, hipEventRecord(ctx->lev[2], st));
   // distant lights: env_T (the side stream's, or 1 without sun maps), then env_direct
   if (d->env) {
@@ -1714,8 +1717,9 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
       sl_fill_u32<<<grid_for((size_t)sc->count * 4, 256, 4096), 256, 0, st>>>((uint32_t *)sc->env_T, 0x3f800000u,
                                                                                sc->count * 4);
     if (timers) SL_HIP(ctx, hipEventRecord(ctx->lev[3], st));
-    sl_env_direct<<<gblocks, 64, 0, st>>>(U, ENV, sc->g[1], sc->g[2], sc->g[3], sc->storage, sc->nodes, sc->vgrid,
-                                          sc->parts, sc->materials, sc->iso_norm, sc->env_T, sc->colour);
+    auto env = shiny ? sl_env_direct<true> : sl_env_direct<false>;
+    env<<<gblocks, 64, 0, st>>>(U, ENV, sc->g[0], sc->g[1], sc->g[2], sc->g[3], sc->storage, sc->nodes, sc->vgrid,
+                                sc->parts, sc->materials, sc->iso_norm, sc->env_T, sc->colour);
   } else if (timers) {
     SL_HIP(ctx, hipEventRecord(ctx->lev[3], st));
   }
@@ -1769,6 +1773,12 @@ extern "C" sl_status sl_bounce_init(sl_context *ctx, sl_scene *sc, const sl_boun
   if (!sc || !t || !t->nodes || !t->kids || !probe_params) return fail(ctx, SL_ERR_INVALID, "null argument");
   if (!sc->nodes) return fail(ctx, SL_ERR_INVALID, "the bounce needs the scene's relighting records");
   if (sc->bounce.on) return fail(ctx, SL_ERR_INVALID, "the scene's bounce is already set up");
+  for (size_t m = 0; m + 32 <= sc->materials_host.size(); m += 32) {
+    const uint32_t fl = rd_u32(sc->materials_host.data() + m, 5);
+    if ((fl & 1u) == 0u && (fl & 8u) != 0u)
+      return fail(ctx, SL_ERR_UNSUPPORTED, "the bounce toward shiny receivers (per-cell gloss, bounce.wgsl) is not "
+                                           "on this path");
+  }
   if (t->node_stride != 96u || t->node_count < sc->count || t->node_count >= (1u << 31))
     return fail(ctx, SL_ERR_LAYOUT, "tree nodes: 96 B records, leaves first, at least one per Gaussian");
   if (probe_params_len != sizeof(ProbeParamsDev)) return fail(ctx, SL_ERR_LAYOUT, "ProbeParams is 9632 B");
