@@ -82,6 +82,10 @@ size_t sl_scene_device_bytes(const sl_scene *scene);
 #define SL_OUT_RGBA32F 0u
 #define SL_OUT_RGBA16F 1u      /* the reference's rgba16float target (src/renderer.js:13) */
 #define SL_FRAME_TIMERS 1u     /* record hipEvent pairs per pass (sl_timers_read) */
+/* The depth order by the reference's counting scatter (shaders/scatter.wgsl:9-18, ?sort=counting, src/main.js:204)
+ * instead of the stable radix sort: keys ascending, equal keys in the order their atomics land (not deterministic,
+ * as in the reference); one view only (sl_batch_encode refuses it for several). */
+#define SL_FRAME_COUNTING_SORT 4u
 typedef struct sl_frame_opts {
   uint32_t width, height;      /* must equal the RenderFrame viewport */
   uint32_t out_format;         /* SL_OUT_* */

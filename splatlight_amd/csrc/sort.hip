@@ -215,3 +215,54 @@ sl_fill_u32_set(uint32_t *__restrict__ p, uint32_t value, uint32_t n, uint32_t i
   for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u)
     p[i] = i == i0 ? v0 : (i == i1 ? v1 : value);
 }
+
+// ---- counting scatter (shaders/scatter.wgsl:9-18, the reference's ?sort=counting) ----------------------------------
+// The depth order by a 65,536-bucket count of the 16-bit keys (project.wgsl:369), an exclusive scan of the counts
+// (scan.wgsl:16-43) and one atomic placement per slot. Unlike the radix sort, equal keys land in whatever order their
+// atomics arrive. Bucket 65,536 takes the invalid slots, so they still fill the order's tail after the visible ones.
+#define CS_BUCKETS 65537u
+
+extern "C" __global__ void __launch_bounds__(256)
+sl_count_hist(const uint32_t *__restrict__ keys, uint32_t n, uint32_t *__restrict__ hist) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t k = keys[i];
+  atomicAdd(&hist[k == 0xFFFFFFFFu ? 65536u : (k & 0xFFFFu)], 1u);
+}
+
+// one workgroup: each thread takes 65 consecutive buckets (1024 x 65 >= 65,537), in place
+extern "C" __global__ void __launch_bounds__(1024)
+sl_count_scan(uint32_t *__restrict__ hist) {
+  __shared__ uint32_t s_w[1024 / SL_WAVE];
+  constexpr uint32_t PER = 65u;
+  const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+  const uint32_t base = t * PER;
+  uint32_t sum = 0;
+  for (uint32_t k = 0; k < PER; k++)
+    if (base + k < CS_BUCKETS) sum += hist[base + k];
+  uint32_t x = sum;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o);
+    if (lane >= (uint32_t)o) x += y;
+  }
+  if (lane == 63u) s_w[w] = x;
+  __syncthreads();
+  uint32_t run = x - sum;
+  for (uint32_t k = 0; k < w; k++) run += s_w[k];
+  for (uint32_t k = 0; k < PER; k++) {
+    if (base + k >= CS_BUCKETS) break;
+    const uint32_t h = hist[base + k];
+    hist[base + k] = run;
+    run += h;
+  }
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+sl_count_scatter(const uint32_t *__restrict__ keys, uint32_t n, uint32_t *__restrict__ offsets,
+                 uint32_t *__restrict__ order) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t k = keys[i];
+  order[atomicAdd(&offsets[k == 0xFFFFFFFFu ? 65536u : (k & 0xFFFFu)], 1u)] = i;
+}

@@ -59,6 +59,7 @@ struct Batch {
   uint32_t flags = 0;
   bool graph = false;
   bool fixed = false;   // laid over the device's no-alloc scratch instead of the context's workspace
+  bool counting = false;   // depth order by the counting scatter (SL_FRAME_COUNTING_SORT) instead of the radix sort
 };
 
 uint32_t rd_u32(const void *p, int word) { uint32_t v; memcpy(&v, (const char *)p + 4 * word, 4); return v; }
@@ -143,6 +144,7 @@ struct Workspace {
   size_t scratch_words = 0;
   FrameCounters *counters = nullptr;
   uint32_t *tickets = nullptr, *depth_hist = nullptr, *tile_hist = nullptr;
+  uint32_t *count_hist = nullptr;  // the counting scatter's 65,537 buckets
   uint32_t *dstatus[3] = {nullptr, nullptr, nullptr}, *tstatus[3] = {nullptr, nullptr, nullptr};
   uint64_t *chunk_status = nullptr;
   uint2 *ranges = nullptr;
@@ -291,6 +293,7 @@ size_t layout_workspace(uint8_t *b, uint32_t s_cap, uint32_t p_cap, uint32_t t_c
   const size_t o_counters = take(sizeof(FrameCounters));
   const size_t o_tickets = take(16 * 4);
   const size_t o_dhist = take(3 * 256 * 4), o_thist = take(3 * 256 * 4);
+  const size_t o_chist = take(65537 * 4);
   const size_t o_ds0 = take((size_t)d_tiles * 256 * 4), o_ds1 = take((size_t)d_tiles * 256 * 4);
   const size_t o_ds2 = take((size_t)d_tiles * 256 * 4);
   const size_t o_ts0 = take((size_t)p_tiles * 256 * 4), o_ts1 = take((size_t)p_tiles * 256 * 4);
@@ -327,6 +330,7 @@ size_t layout_workspace(uint8_t *b, uint32_t s_cap, uint32_t p_cap, uint32_t t_c
   w.tickets = (uint32_t *)(b + o_tickets);
   w.depth_hist = (uint32_t *)(b + o_dhist);
   w.tile_hist = (uint32_t *)(b + o_thist);
+  w.count_hist = (uint32_t *)(b + o_chist);
   w.dstatus[0] = (uint32_t *)(b + o_ds0);
   w.dstatus[1] = (uint32_t *)(b + o_ds1);
   w.dstatus[2] = (uint32_t *)(b + o_ds2);
@@ -476,6 +480,11 @@ sl_status enqueue_batch(sl_context *ctx, const Batch &b, bool timers, bool overl
     else
       sl_radix_pass<16><<<grid, RS_THREADS, 0, st>>>(ki, vi, ko, vo, n_ptr, n_static, shift, bases, status, ticket);
   };
+  if (b.counting) {     // one view: keys are 16-bit depth keys or INVALID
+    sl_count_hist<<<(n + 255u) / 256u, 256, 0, st>>>(w.keys, n, w.count_hist);
+    sl_count_scan<<<1, 1024, 0, st>>>(w.count_hist);
+    sl_count_scatter<<<(n + 255u) / 256u, 256, 0, st>>>(w.keys, n, w.count_hist, w.order);
+  } else {
   sl_radix_hist<<<grid_for(n, 256 * 16, 1024), RS_THREADS, 0, st>>>(w.keys, nullptr, n, 0u, b.depth_passes,
                                                                    w.depth_hist);
   sl_radix_bases<<<b.depth_passes, RS_THREADS, 0, st>>>(w.depth_hist);
@@ -488,6 +497,7 @@ sl_status enqueue_batch(sl_context *ctx, const Batch &b, bool timers, bool overl
                d_grid);
     radix_pass(w.pk1, w.pv1, nullptr, w.order, nullptr, n, 16u, w.depth_hist + 512, w.dstatus[2], w.tickets + 2,
                d_grid);
+  }
   }
   if ((s = mark(3, st)) != SL_OK) return s;
   sl_pairs_emit<false><<<grid_for(n, PE_CHUNK, 4u * ctx->cus), PE_THREADS, 0, st>>>(
@@ -556,7 +566,7 @@ sl_status launch_batch(sl_context *ctx, const Batch &b) {
   sl_status s;
   if (b.graph) {
     // what the captured launches depend on: the layout and the workspace (the view table is read at run time)
-    std::vector<uint64_t> key = {(uint64_t)(uintptr_t)w.base, V, b.total_slots, b.total_tiles, w.pairs_cap,
+    std::vector<uint64_t> key = {(uint64_t)(uintptr_t)w.base, V, b.counting, b.total_slots, b.total_tiles, w.pairs_cap,
                                  b.depth_passes, b.tile_passes, ctx->tuning.composite_batch,
                                  ctx->tuning.radix_items, ctx->tuning.grid_mul};
     if (!ctx->graph || key != ctx->graph_key) {
@@ -1055,6 +1065,9 @@ sl_status encode_batch(sl_context *ctx, const sl_view *views, uint32_t n_views, 
   b.flags = flags;
   b.graph = !fixed && (flags & SL_BATCH_GRAPH) != 0;
   b.fixed = fixed;
+  b.counting = (flags & SL_FRAME_COUNTING_SORT) != 0;
+  if (b.counting && n_views > 1)
+    return fail(ctx, SL_ERR_UNSUPPORTED, "the counting scatter orders one view's 16-bit depth keys, not a batch");
   b.views.resize(n_views);
   b.scenes.resize(n_views);
   b.slot_base.assign(n_views + 1, 0);
@@ -1186,7 +1199,7 @@ extern "C" sl_status sl_frame_encode(sl_context *ctx, sl_scene *sc, const void *
   SL_LIVE(ctx);
   if (!sc || !frame_bytes || !opts || !out_device) return fail(ctx, SL_ERR_INVALID, "null argument");
   const sl_view v = {sc, frame_bytes, frame_len, opts->width, opts->height, opts->out_format, out_device};
-  return encode_batch(ctx, &v, 1, opts->flags & SL_FRAME_TIMERS);
+  return encode_batch(ctx, &v, 1, opts->flags & (SL_FRAME_TIMERS | SL_FRAME_COUNTING_SORT));
 }
 
 extern "C" sl_status sl_frame_encode_noalloc(sl_context *ctx, sl_scene *sc, const void *frame_bytes,
@@ -1194,7 +1207,7 @@ extern "C" sl_status sl_frame_encode_noalloc(sl_context *ctx, sl_scene *sc, cons
   SL_LIVE(ctx);
   if (!sc || !frame_bytes || !opts || !out_device) return fail(ctx, SL_ERR_INVALID, "null argument");
   const sl_view v = {sc, frame_bytes, frame_len, opts->width, opts->height, opts->out_format, out_device};
-  return encode_batch(ctx, &v, 1, opts->flags & SL_FRAME_TIMERS, true);
+  return encode_batch(ctx, &v, 1, opts->flags & (SL_FRAME_TIMERS | SL_FRAME_COUNTING_SORT), true);
 }
 
 extern "C" sl_status sl_frame_finish(sl_context *ctx, sl_frame_stats *stats) {

@@ -20,6 +20,7 @@ STATUS_NAMES = {0: "SL_OK", 1: "SL_ERR_INVALID", 2: "SL_ERR_LAYOUT", 3: "SL_ERR_
 SL_STORAGE_F32, SL_STORAGE_F16, SL_STORAGE_BF16 = 0, 1, 2
 SL_OUT_RGBA32F, SL_OUT_RGBA16F = 0, 1
 SL_FRAME_TIMERS = 1
+SL_FRAME_COUNTING_SORT = 4
 ART_KEYS, ART_ORDER, ART_SPLATS, ART_PAIR_TILES, ART_PAIR_SLOTS, ART_RANGES = 1, 2, 3, 4, 5, 6
 
 # Every symbol include/splatlight.h declares (checked by the CPU tests).
@@ -349,9 +350,10 @@ class Scene:
 
 
 def frame_encode(ctx: Context, scene: Scene, frame: bytes, width: int, height: int, out_dev: int,
-                 out_format: int = SL_OUT_RGBA32F, timers: bool = False):
+                 out_format: int = SL_OUT_RGBA32F, timers: bool = False, counting: bool = False):
+    """sl_frame_encode; counting: the depth order by the counting scatter instead of the stable radix sort."""
     fb = (C.c_uint8 * len(frame)).from_buffer_copy(frame)
-    o = FrameOpts(width, height, out_format, SL_FRAME_TIMERS if timers else 0)
+    o = FrameOpts(width, height, out_format, (SL_FRAME_TIMERS if timers else 0) | (SL_FRAME_COUNTING_SORT if counting else 0))
     ctx.check(ctx.L.sl_frame_encode(ctx.h, scene.h, fb, len(frame), C.byref(o), C.c_void_p(out_dev)))
 
 
