@@ -61,22 +61,27 @@ uint32_t or_depth_key(float depth, float near_, float far_) {
 
 /* float log as the correctly rounded value (double log rounded once), the definition the HIP kernels share */
 static float logf_cr(float x) { return (float)log((double)x); }
+static float expf_cr_(float x) { return (float)exp((double)x); }
 
 #define REAL float
 #define SUFFIX f32
 #define SQRTR sqrtf
 #define RLOG logf_cr
+#define REXP expf_cr_
 #include "project_body.inc"
 #undef REAL
 #undef SUFFIX
 #undef SQRTR
 #undef RLOG
+#undef REXP
 
 #define REAL double
 #define SUFFIX f64
 #define SQRTR sqrt
 #define RLOG log
+#define REXP exp
 #include "project_body.inc"
+#undef REXP
 #undef REAL
 #undef SUFFIX
 #undef SQRTR

@@ -63,7 +63,7 @@ __device__ __forceinline__ void splat_colour(const FrameDev &f, uint32_t materia
 // count (padding to 256) get INVALID. COLOUR: the splats' colour words and the per-block colour max are written
 // here; without it they are left to sl_splat_colour (so the geometry need not wait for the lighting), the colour
 // words are 0 and the partials' colour max is 0.
-template <bool COLOUR>
+template <bool COLOUR, bool BLUR>
 __global__ void __launch_bounds__(256)
 sl_project_kernel(const ViewDev *__restrict__ views, const uint32_t *__restrict__ slot_base, uint32_t n_views,
                   Splat *__restrict__ splats, uint32_t *__restrict__ keys, uint2 *__restrict__ tbox,
@@ -82,6 +82,7 @@ sl_project_kernel(const ViewDev *__restrict__ views, const uint32_t *__restrict_
   float mu[3], rot[3][3];
   uint32_t material = SL_EMITTER_MATERIAL;
   float keep = 1.0f;
+  float vel[3] = {0.0f, 0.0f, 0.0f};   // BLUR: world displacement over the exposure (project.wgsl:138-149)
   if (ok) {
     if (i > f.count) {                                                // emitters (project.wgsl:123-127)
       const uint32_t k = i - f.count - 1u;
@@ -107,6 +108,17 @@ sl_project_kernel(const ViewDev *__restrict__ views, const uint32_t *__restrict_
       for (int j = 0; j < 3; j++)
 #pragma unroll
         for (int r = 0; r < 3; r++) rot[j][r] = (xr[r][0] * q[j][0] + xr[r][1] * q[j][1]) + xr[r][2] * q[j][2];
+      if (BLUR && f.shutter > 0.0f && (pflags & 2u) != 0u) {
+        // the chord from the previous pose (xf_prev, words 12-23) over the shutter, centred mid-exposure
+        const float4 p0 = parts[part * 8 + 3], p1 = parts[part * 8 + 4], p2 = parts[part * 8 + 5];
+        const float prev[3] = {((p0.x * A.x + p0.y * A.y) + p0.z * A.z) + p0.w * 1.0f,
+                               ((p1.x * A.x + p1.y * A.y) + p1.z * A.z) + p1.w * 1.0f,
+                               ((p2.x * A.x + p2.y * A.y) + p2.z * A.z) + p2.w * 1.0f};
+#pragma unroll
+        for (int r = 0; r < 3; r++) vel[r] = f.shutter * (mu[r] - prev[r]);
+#pragma unroll
+        for (int r = 0; r < 3; r++) mu[r] -= 0.5f * vel[r];
+      }
       material = f.part_material[part];
     }
   }
@@ -140,6 +152,23 @@ sl_project_kernel(const ViewDev *__restrict__ views, const uint32_t *__restrict_
     for (int j = 0; j < 3; j++)
 #pragma unroll
       for (int r = 0; r < 3; r++) cov[j][r] = (m[0][r] * m[0][j] + m[1][r] * m[1][j]) + m[2][r] * m[2][j];
+    const bool blurred = BLUR && (vel[0] != 0.0f || vel[1] != 0.0f || vel[2] != 0.0f);
+    float cov_s[3][3];                                               // the sharp footprint (project.wgsl:208-210)
+    if (BLUR) {
+#pragma unroll
+      for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int r = 0; r < 3; r++) cov_s[j][r] = cov[j][r];
+    }
+    if (blurred) {
+      float wv[3];
+#pragma unroll
+      for (int r = 0; r < 3; r++) wv[r] = (f.view[r] * vel[0] + f.view[4 + r] * vel[1]) + f.view[8 + r] * vel[2];
+#pragma unroll
+      for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int r = 0; r < 3; r++) cov[j][r] = cov[j][r] + (wv[r] * wv[j]) * (1.0f / 12.0f);
+    }
 
     // EWA Jacobian with the 1.3x tangent clamp (project.wgsl:212-231)
     const float iz = 1.0f / depth;
@@ -170,9 +199,45 @@ sl_project_kernel(const ViewDev *__restrict__ views, const uint32_t *__restrict_
     float dl[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) dl[k] = (wr[k][0] * d[0] + wr[k][1] * d[1]) + wr[k][2] * d[2];
-    const float q = ((dl[0] * dl[0]) * (1.0f / s2[0]) + (dl[1] * dl[1]) * (1.0f / s2[1])) +
-                    (dl[2] * dl[2]) * (1.0f / s2[2]);
-    float tau0 = A.w * sqrtf(6.283185307f / q) * sqrtf(fmaxf(det0, 0.0f) / det);
+    float q = ((dl[0] * dl[0]) * (1.0f / s2[0]) + (dl[1] * dl[1]) * (1.0f / s2[1])) +
+              (dl[2] * dl[2]) * (1.0f / s2[2]);
+    float kappa = A.w;
+    float tau0;
+    if (blurred) {
+      // Sherman-Morrison for d^T (S + v v^T / 12)^-1 d and the mass-keeping density (project.wgsl:242-251)
+      const float q_s = q;
+      float vl[3];
+#pragma unroll
+      for (int k = 0; k < 3; k++) vl[k] = (rot[k][0] * vel[0] + rot[k][1] * vel[1]) + rot[k][2] * vel[2];
+      const float mv2 = ((vl[0] * vl[0]) * (1.0f / s2[0]) + (vl[1] * vl[1]) * (1.0f / s2[1])) +
+                        (vl[2] * vl[2]) * (1.0f / s2[2]);
+      const float dv = ((dl[0] * vl[0]) * (1.0f / s2[0]) + (dl[1] * vl[1]) * (1.0f / s2[1])) +
+                       (dl[2] * vl[2]) * (1.0f / s2[2]);
+      q -= dv * dv / (12.0f + mv2);
+      kappa /= sqrtf(1.0f + mv2 / 12.0f);
+      tau0 = kappa * sqrtf(6.283185307f / q) * sqrtf(fmaxf(det0, 0.0f) / det);
+      if (f.blur_cover) {
+        // coverage cap: the peak alpha at most the sharp splat's x sqrt(det_s / det) (project.wgsl:253-268); exp
+        // and log correctly rounded, as the host restatement defines them
+        float sj0[3], sj1[3];
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+          sj0[r] = (cov_s[0][r] * j0[0] + cov_s[1][r] * j0[1]) + cov_s[2][r] * j0[2];
+          sj1[r] = (cov_s[0][r] * j1[0] + cov_s[1][r] * j1[1]) + cov_s[2][r] * j1[2];
+        }
+        const float a_s = ((j0[0] * sj0[0] + j0[1] * sj0[1]) + j0[2] * sj0[2]) + f.lowpass;
+        const float b_s = (j0[0] * sj1[0] + j0[1] * sj1[1]) + j0[2] * sj1[2];
+        const float c_s = ((j1[0] * sj1[0] + j1[1] * sj1[1]) + j1[2] * sj1[2]) + f.lowpass;
+        const float det0_s = (a_s - f.lowpass) * (c_s - f.lowpass) - b_s * b_s;
+        const float det_s = a_s * c_s - b_s * b_s;
+        const float tau_s = A.w * sqrtf(6.283185307f / q_s) * sqrtf(fmaxf(det0_s, 0.0f) / det_s);
+        const float ap = (1.0f - (float)exp((double)-tau_s)) * sqrtf(det_s / det);
+        const float cap = -(float)log((double)fmaxf(1.0f - ap, 1e-6f));
+        tau0 = fminf(tau0, cap);
+      }
+    } else {
+      tau0 = kappa * sqrtf(6.283185307f / q) * sqrtf(fmaxf(det0, 0.0f) / det);
+    }
     tau0 *= keep;
     // cut radius r: alpha >= 1/255 (project.wgsl:290-295); tau_min = -log(1 - 1/255) correctly rounded
     const float tau_min = 0x1.018258p-8f;

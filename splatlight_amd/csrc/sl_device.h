@@ -27,6 +27,8 @@ struct FrameDev {
   uint32_t view_mode;
   uint32_t glass_mask;       // bit m: material m's mat_view flags bit 0 (coverage set: SPLAT_GLASS)
   uint32_t width, height, tiles_x, tiles_y;
+  float shutter;             // motion blur: shutter fraction (palette[3].z); moving parts blur when > 0
+  uint32_t blur_cover;       // the blurred splats' coverage cap (palette[3].w > 0.5)
 };
 
 // Gaussian attribute storage (SL_STORAGE_*). F32: four float4 streams g0 (mu, kappa), g1 quat, g2 (scale,

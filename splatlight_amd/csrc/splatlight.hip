@@ -60,6 +60,7 @@ struct Batch {
   bool graph = false;
   bool fixed = false;   // laid over the device's no-alloc scratch instead of the context's workspace
   bool counting = false;   // depth order by the counting scatter (SL_FRAME_COUNTING_SORT) instead of the radix sort
+  bool blur = false;       // some view blurs its moving parts (shutter > 0): the projection's motion-blur variant
 };
 
 uint32_t rd_u32(const void *p, int word) { uint32_t v; memcpy(&v, (const char *)p + 4 * word, 4); return v; }
@@ -418,11 +419,8 @@ sl_status decode_frame(sl_context *ctx, const sl_scene *sc, const void *fb, cons
   if (rd_u32(fb, 180) != 0u) return fail(ctx, SL_ERR_UNSUPPORTED, "screen-space LOD (frame.lod) is not on this path");
   if (rd_f32(fb, 194) > 0.5f) return fail(ctx, SL_ERR_UNSUPPORTED, "explode is not on this path");
   if (rd_u32(fb, 296) != 0u) return fail(ctx, SL_ERR_UNSUPPORTED, "particles are not on this path");
-  if (shutter > 0.0f) {
-    for (uint32_t p = 0; p < sc->part_count; p++)
-      if (rd_u32(sc->parts_host.data() + 128 * p, 31) & 2u)
-        return fail(ctx, SL_ERR_UNSUPPORTED, "motion blur of a moving part is not on this path");
-  }
+  f->shutter = shutter > 0.0f ? shutter : 0.0f;   // moving parts (Part.flags bit 1) blur over it (project.wgsl:138)
+  f->blur_cover = rd_f32(fb, 47) > 0.5f ? 1u : 0u;
   if (!(f->near_ > 0.0f) || !(f->far_ > f->near_)) return fail(ctx, SL_ERR_INVALID, "bad near / far");
   if ((uint32_t)f->vw != o->width || (uint32_t)f->vh != o->height || o->width == 0 || o->height == 0)
     return fail(ctx, SL_ERR_INVALID, "RenderFrame viewport does not match the output size");
@@ -459,12 +457,13 @@ sl_status enqueue_batch(sl_context *ctx, const Batch &b, bool timers, bool overl
   sl_fill_u32<<<grid_for(w.scratch_words, 256, 2048), 256, 0, st>>>((uint32_t *)w.scratch, 0u,
                                                                     (uint32_t)w.scratch_words);
   if ((s = mark(1, st)) != SL_OK) return s;
-  if (overlap)
-    sl_project_kernel<false><<<b.total_slots / 256u, 256, 0, st>>>(w.views, w.slot_base, V, w.splats, w.keys, w.tbox,
-                                                                   w.partials);
+  auto project = [&](auto kernel) {
+    kernel<<<b.total_slots / 256u, 256, 0, st>>>(w.views, w.slot_base, V, w.splats, w.keys, w.tbox, w.partials);
+  };
+  if (b.blur)
+    overlap ? project(sl_project_kernel<false, true>) : project(sl_project_kernel<true, true>);
   else
-    sl_project_kernel<true><<<b.total_slots / 256u, 256, 0, st>>>(w.views, w.slot_base, V, w.splats, w.keys, w.tbox,
-                                                                  w.partials);
+    overlap ? project(sl_project_kernel<false, false>) : project(sl_project_kernel<true, false>);
   sl_frame_totals<<<V, 1024, 0, st>>>(w.partials, w.slot_base, w.view_counters, w.counters, overlap ? 1u : 3u);
   if ((s = mark(2, st)) != SL_OK) return s;
   // depth order: stable sort of (view, 16-bit depth) keys, values = global slot
@@ -566,7 +565,7 @@ sl_status launch_batch(sl_context *ctx, const Batch &b) {
   sl_status s;
   if (b.graph) {
     // what the captured launches depend on: the layout and the workspace (the view table is read at run time)
-    std::vector<uint64_t> key = {(uint64_t)(uintptr_t)w.base, V, b.counting, b.total_slots, b.total_tiles, w.pairs_cap,
+    std::vector<uint64_t> key = {(uint64_t)(uintptr_t)w.base, V, b.counting, b.blur, b.total_slots, b.total_tiles, w.pairs_cap,
                                  b.depth_passes, b.tile_passes, ctx->tuning.composite_batch,
                                  ctx->tuning.radix_items, ctx->tuning.grid_mul};
     if (!ctx->graph || key != ctx->graph_key) {
@@ -1099,6 +1098,7 @@ sl_status encode_batch(sl_context *ctx, const sl_view *views, uint32_t n_views, 
     b.slot_base[v] = (uint32_t)slots;
     b.tile_base[v] = (uint32_t)tiles;
     b.tiles_x[v] = vd.f.tiles_x;
+    b.blur = b.blur || vd.f.shutter > 0.0f;
     slots += ((uint64_t)vd.f.nslots + 255u) / 256u * 256u;
     tiles += nt;
     want_pairs += 8ull * vd.f.nslots;

@@ -124,11 +124,12 @@ def perspective(fovy, aspect, near, far) -> np.ndarray:
 
 def render_frame(*, view, width, height, fovy, near=0.02, far=50.0, count, view_mode=VIEW_LIT,
                  part_material=None, palette=None, bounce_gain=1.0, emitter_count=0, albedo_scale=1.0,
-                 lowpass=0.3, exposure=1.0) -> bytes:
+                 lowpass=0.3, exposure=1.0, shutter=0.0, blur_coverage=True) -> bytes:
     """Pack a RenderFrame (1200 B) the way writeFrame does (src/main.js:526-556, 470-475).
 
-    Motion blur, explode, LOD, refraction, particles and the reflection split are left off (zero), which is
-    the static-scene default the splat frame path renders.
+    shutter: the motion blur's shutter fraction (palette[3].z; moving parts blur over it, 0 = off); blur_coverage:
+    the blurred splats' coverage cap (palette[3].w, ?blurcov). Explode, LOD, refraction, particles and the
+    reflection split are left off (zero).
     """
     buf = np.zeros(FRAME_BYTES // 4, np.uint32)
     f32 = buf.view(np.float32)
@@ -137,7 +138,7 @@ def render_frame(*, view, width, height, fovy, near=0.02, far=50.0, count, view_
     pal = palette if palette is not None else [(1.0, 1.0, 1.0)] * 3
     for i in range(3):
         f32[32 + 4 * i: 36 + 4 * i] = [*pal[i], 1.0]
-    f32[44:48] = [bounce_gain, float(emitter_count), 0.0, 1.0]     # palette[3]: gain, emitters, shutter 0, blurcov
+    f32[44:48] = [bounce_gain, float(emitter_count), shutter, 1.0 if blur_coverage else 0.0]   # palette[3]
     pm = part_material if part_material is not None else [0] * MAX_PARTS
     for i, m in enumerate(pm):
         buf[(48 if i < 16 else 164 - 16) + i] = m
