@@ -123,6 +123,31 @@ sl_status sl_frame_finish(sl_context *ctx, sl_frame_stats *stats);
 sl_status sl_frame_encode_noalloc(sl_context *ctx, sl_scene *scene, const void *frame_bytes, size_t frame_len,
                                   const sl_frame_opts *opts, void *out_device);
 
+/* ---- display: the pass after the frame (shaders/display.wgsl:22-189, src/renderer.js:475-485) ----------------
+ * The linear HDR target (what sl_frame_encode wrote, rows top-down) resolved to 8-bit display colour: in the lit
+ * view (RenderFrame.view_mode 3) the environment's background sky (SkyBg) and its lights' discs (Env) fill what the
+ * splats leave uncovered (1 - alpha > 1/1024), then RenderFrame.exposure, the tone curve RenderFrame.tone.x (0 clamp,
+ * 1 AgX, 2 ACES fit, 3 Reinhard on luminance, 4 Khronos PBR Neutral, 5 Hable filmic) and the sRGB encode; view_mode 2
+ * maps the overdraw count to the heat ramp. Each channel is the byte floor(255 v + 0.5) of the shader's expression
+ * with exp, pow, log2, cos and sin correctly rounded (the precision WGSL leaves open, fixed). Enqueued on the
+ * context's stream after whatever wrote hdr_device; sl_sync waits for it. */
+#define SL_DISPLAY_RGBA8 0u    /* rgba8unorm bytes r, g, b, 255 */
+#define SL_DISPLAY_BGRA8 1u    /* bgra8unorm (navigator.gpu.getPreferredCanvasFormat() on most systems) */
+#define SL_DISPLAY_EXACT 1u    /* flag: evaluate every pixel with the correctly rounded functions (same bytes) */
+typedef struct sl_display_opts {
+  uint32_t width, height;      /* must equal the RenderFrame viewport */
+  uint32_t hdr_format;         /* SL_OUT_RGBA32F or SL_OUT_RGBA16F: the target's format */
+  uint32_t out_format;         /* SL_DISPLAY_* */
+  uint32_t flags;              /* SL_DISPLAY_EXACT */
+} sl_display_opts;
+/* env: the 416 B Env uniform (src/environment.js:111-134; NULL: no environment), sky_bg: the 1520 B SkyBg uniform
+ * (src/environment.js:135-150; NULL: none, the lighting sky's 16 coefficients are drawn instead);
+ * SplatRenderer.bindEnv (src/renderer.js:400-404) and the display pass of SplatRenderer.encode. out_device holds
+ * width * height * 4 bytes. */
+sl_status sl_display_encode(sl_context *ctx, const void *frame_bytes, size_t frame_len, const void *env,
+                            size_t env_len, const void *sky_bg, size_t sky_bg_len, const sl_display_opts *opts,
+                            const void *hdr_device, void *out_device);
+
 /* ---- batches: many views over resident scenes in one launch per stage -------------------------------------
  * The views (each: a scene, its 1200 B RenderFrame, its own target size and format) are laid out back to back in
  * one slot space and one tile space; project, the depth sort (keyed by view, then depth), pair emission, the tile

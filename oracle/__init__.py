@@ -421,3 +421,59 @@ def bounce_state(G, N, n_probes):
     return {"lo_dir": np.zeros((G, 4), np.float32), "lo_b": np.zeros((G, 3, 4), np.float32),
             "node_acc": np.zeros((N, 14, 4), np.float32), "probe_data": np.zeros((n_probes, 128, 2, 4), np.float32),
             "xfer": np.zeros((G, 3, 4), np.float32)}
+
+
+# ---- display (display.c) ------------------------------------------------------------------------------------------
+
+def display(frame: bytes, hdr, W, H, env=None, bg=None):
+    """The display pass over an HDR target [H, W, 4]: (encoded colour [H, W, 4] float32, RGBA8 [H, W, 4] uint8)."""
+    L = lib()
+    fb = np.frombuffer(frame, np.uint8).copy()
+    h = np.ascontiguousarray(hdr, np.float32).reshape(H, W, 4)
+    out = np.zeros((H, W, 4), np.float32)
+    by = np.zeros((H, W, 4), np.uint8)
+    e = None if env is None else _b(env)
+    g = None if bg is None else _b(bg)
+    L.or_display.restype = None
+    L.or_display.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
+                             C.c_void_p]
+    L.or_display(_p(fb), _p(h), W, H, _p(e), _p(g), _p(out), _p(by))
+    return out, by
+
+
+def tone_map(x, mode: int):
+    xs = np.ascontiguousarray(x, np.float32).reshape(-1, 3)
+    y = np.zeros_like(xs)
+    lib().or_tone_map_n(_p(xs), C.c_uint32(xs.shape[0]), C.c_uint32(mode), _p(y))
+    return y
+
+
+def srgb_encode(x):
+    xs = np.ascontiguousarray(x, np.float32).ravel()
+    y = np.zeros_like(xs)
+    lib().or_srgb_encode_n(_p(xs), C.c_uint32(xs.size), _p(y))
+    return y
+
+
+def heat(t):
+    ts = np.ascontiguousarray(t, np.float32).ravel()
+    y = np.zeros((ts.size, 3), np.float32)
+    lib().or_heat_n(_p(ts), C.c_uint32(ts.size), _p(y))
+    return y
+
+
+def sky(env, bg, w, which: int):
+    """which 0: the lighting sky (Env SH), 1: the background sky (SkyBg), 2: the lights' discs; w [n, 3]."""
+    ws = np.ascontiguousarray(w, np.float32).reshape(-1, 3)
+    y = np.zeros_like(ws)
+    e = None if env is None else _b(env)
+    g = None if bg is None else _b(bg)
+    lib().or_sky_n(_p(e), _p(g), _p(ws), C.c_uint32(ws.shape[0]), C.c_uint32(which), _p(y))
+    return y
+
+
+def view_dir(frame: bytes, pxy):
+    p = np.ascontiguousarray(pxy, np.float32).reshape(-1, 2)
+    w = np.zeros((p.shape[0], 3), np.float32)
+    lib().or_view_dir_n(_p(np.frombuffer(frame, np.uint8).copy()), _p(p), C.c_uint32(p.shape[0]), _p(w))
+    return w

@@ -128,6 +128,17 @@ void or_onb(const float *n, float *out6);
 float or_asg_phase(const float *wi, const float *lb, const float *sl, const float *n0, const float *t1,
                    const float *t2, const float *flake, const float *Sd, const float *So, const float *wo);
 
+/* display pass (display.c; shaders/display.wgsl:22-189): hdr W*H*4 floats rows top-down; env: Env (416 B) or
+ * NULL; bg: SkyBg (1520 B) or NULL; out: W*H*4 floats (the encoded colour before the 8-bit store, alpha 1) or
+ * NULL; bytes: W*H*4 RGBA8 (round half up of 255 v) or NULL */
+void or_display(const void *frame, const float *hdr, uint32_t W, uint32_t H, const void *env, const void *bg,
+                float *out, uint8_t *bytes);
+void or_tone_map_n(const float *x, uint32_t n, uint32_t mode, float *y);
+void or_srgb_encode_n(const float *x, uint32_t n, float *y);
+void or_heat_n(const float *t, uint32_t n, float *y);
+void or_sky_n(const void *env, const void *bg, const float *w, uint32_t n, uint32_t which, float *y);
+void or_view_dir_n(const void *frame, const float *pxy, uint32_t n, float *w);
+
 #ifdef __cplusplus
 }
 #endif

@@ -25,6 +25,7 @@
 #include "tiles.hip"
 #include "lighting.hip"
 #include "bounce.hip"
+#include "display.hip"
 #pragma clang fp contract(fast)
 
 namespace {
@@ -191,6 +192,7 @@ struct sl_context {
     Grow rec, tbox, tmask, k0, v0, k1, v1, scratch;
   } map[2];
   Grow map_stats;
+  Grow disp_q;        // the display pass's list of pixels to settle exactly: a count word, then one word per pixel
   uint32_t map_records[2] = {0, 0}, map_tiles[2] = {0, 0};
   const uint32_t *map_touched[2] = {nullptr, nullptr};   // the last sums' tile bits (map scratch)
   // the sun-map chain's stream (created on first use, default priority), the fork/join events that order it with
@@ -701,6 +703,7 @@ extern "C" void sl_ctx_destroy(sl_context *ctx) {
     for (Grow *gb : {&m.rec, &m.tbox, &m.tmask, &m.k0, &m.v0, &m.k1, &m.v1, &m.scratch})
       if (gb->p) (void)hipFree(gb->p);
   if (ctx->map_stats.p) (void)hipFree(ctx->map_stats.p);
+  if (ctx->disp_q.p) (void)hipFree(ctx->disp_q.p);
   for (hipEvent_t e : {ctx->fork, ctx->join, ctx->cleared, ctx->sev[0], ctx->sev[1], ctx->pre_light, ctx->geo_done})
     if (e) (void)hipEventDestroy(e);
   if (ctx->side) (void)hipStreamDestroy(ctx->side);
@@ -1213,6 +1216,118 @@ extern "C" sl_status sl_frame_encode_noalloc(sl_context *ctx, sl_scene *sc, cons
 extern "C" sl_status sl_frame_finish(sl_context *ctx, sl_frame_stats *stats) {
   SL_LIVE(ctx);
   return finish_batch(ctx, stats, 1);
+}
+
+// ---- display: the HDR target to 8-bit display colour (shaders/display.wgsl) ------------------------------------
+
+namespace {
+sl_status grow_ctx(sl_context *ctx, Grow *gb, size_t bytes);
+
+// The background sky's coefficient table (DP_SH_ENTRIES float4, the layout sl_display reads): per band l and
+// order m >= 0 at e = l (l + 1) / 2 + m, {sh[l*l + l + m].rgb, f} and at 45 + e {sh[l*l + l - m].rgb, 0}, where f
+// is the recurrence's factor (display.wgsl:49-59): kf = (l - m)! / (l + m)! / (4 pi) built up in the shader's float
+// steps and order, k = sqrt((2l + 1) kf), times 1.41421356 for m > 0. bg: the SkyBg words. Returns lmax.
+uint32_t display_sh_table(uint32_t n, const float *bg, float4 *tab) {
+#pragma clang fp contract(off)
+  const uint32_t lmax = (uint32_t)roundf(sqrtf((float)n)) - 1u;
+  for (uint32_t m = 0; m <= lmax; m++) {
+    float kf = (float)(1.0 / (4.0 * 3.14159265));
+    for (uint32_t j = 1; j <= 2u * m; j++) kf /= (float)j;
+    for (uint32_t l = m; l <= lmax; l++) {
+      if (l > m + 1u) kf *= (float)(l - m) / (float)(l + m);
+      if (l == m + 1u) kf *= 1.0f / (float)(2u * m + 1u);
+      const float k = sqrtf((float)(2u * l + 1u) * kf);
+      const uint32_t e = l * (l + 1u) / 2u + m, ip = l * l + l + m, im = l * l + l - m;
+      tab[e] = make_float4(bg[4 * ip], bg[4 * ip + 1], bg[4 * ip + 2], m == 0u ? k : 1.41421356f * k);
+      tab[45u + e] = make_float4(bg[4 * im], bg[4 * im + 1], bg[4 * im + 2], 0.0f);
+    }
+  }
+  return lmax;
+}
+}  // namespace
+
+extern "C" sl_status sl_display_encode(sl_context *ctx, const void *frame_bytes, size_t frame_len, const void *env,
+                                       size_t env_len, const void *sky_bg, size_t sky_bg_len,
+                                       const sl_display_opts *opts, const void *hdr_device, void *out_device) {
+  SL_LIVE(ctx);
+  if (!frame_bytes || !opts || !hdr_device || !out_device) return fail(ctx, SL_ERR_INVALID, "null argument");
+  if (frame_len != 1200u) return fail(ctx, SL_ERR_LAYOUT, "RenderFrame must be 1200 bytes");
+  if (env && env_len != 416u) return fail(ctx, SL_ERR_LAYOUT, "Env must be 416 bytes");
+  if (sky_bg && sky_bg_len != 1520u) return fail(ctx, SL_ERR_LAYOUT, "SkyBg must be 1520 bytes");
+  if (opts->hdr_format != SL_OUT_RGBA32F && opts->hdr_format != SL_OUT_RGBA16F)
+    return fail(ctx, SL_ERR_INVALID, "hdr_format: SL_OUT_RGBA32F or SL_OUT_RGBA16F");
+  if (opts->out_format != SL_DISPLAY_RGBA8 && opts->out_format != SL_DISPLAY_BGRA8)
+    return fail(ctx, SL_ERR_INVALID, "out_format: SL_DISPLAY_RGBA8 or SL_DISPLAY_BGRA8");
+  if (opts->flags & ~SL_DISPLAY_EXACT) return fail(ctx, SL_ERR_INVALID, "unknown display flags");
+  const float *fr = (const float *)frame_bytes;
+  const uint32_t *fu = (const uint32_t *)frame_bytes;
+  const uint32_t W = opts->width, H = opts->height;
+  if (W == 0u || H == 0u || W > 16384u || H > 16384u)
+    return fail(ctx, SL_ERR_INVALID, "display size must be 1..16384 on each side");
+  if ((float)W != fr[72] || (float)H != fr[73])
+    return fail(ctx, SL_ERR_INVALID, "display size differs from the RenderFrame viewport");
+  DisplayArgs a;
+  memset(&a, 0, sizeof(a));
+  for (int c = 0; c < 3; c++)
+    for (int r = 0; r < 3; r++) a.rot[3 * c + r] = fr[4 * c + r];
+  a.vw = fr[72];
+  a.vh = fr[73];
+  a.fx = fr[74];
+  a.fy = fr[75];
+  a.view_mode = fu[79];
+  a.exposure = fr[82];
+  a.tone = fu[260];
+  a.width = W;
+  a.height = H;
+  float bgw[380] = {};
+  if (env) memcpy(a.env, env, 416);
+  if (sky_bg) memcpy(bgw, sky_bg, 1520);
+  memcpy(a.bg_tail, bgw + 324, sizeof(a.bg_tail));
+  float4 tab[DP_SH_ENTRIES] = {};
+  const uint32_t nsh = std::min(((const uint32_t *)bgw)[4 * 81], 81u);
+  if (nsh > 0u) a.lmax = display_sh_table(nsh, bgw, tab);
+  const uint32_t nr = std::min(((const uint32_t *)bgw)[4 * 81 + 1], 8u);
+  for (uint32_t k = 0; k < nr; k++) {   // the lobe directions in float as the shader forms them, rounded once
+    const float ang = 6.28318531f * (float)k / (float)nr;
+    a.ring_cs[2 * k] = (float)cos((double)ang);
+    a.ring_cs[2 * k + 1] = (float)sin((double)ang);
+  }
+  const uint64_t npx = (uint64_t)W * H;
+  const uint32_t blocks = (uint32_t)((npx + DP_PX - 1) / DP_PX);
+  const bool half_in = opts->hdr_format == SL_OUT_RGBA16F, bgra = opts->out_format == SL_DISPLAY_BGRA8;
+  const bool exact = (opts->flags & SL_DISPLAY_EXACT) != 0u;
+  // the context's display block: the regions' count words, the coefficient table at 256 B, the lists at 2 KiB
+  const uint32_t q_cap = DP_PX * ((blocks + DP_REGIONS - 1u) / DP_REGIONS);
+  sl_status s = grow_ctx(ctx, &ctx->disp_q, 2048 + 4 * (size_t)q_cap * DP_REGIONS);
+  if (s != SL_OK) return s;
+  uint32_t *q_count = (uint32_t *)ctx->disp_q.p, *q_list = q_count + 512;
+  float4 *sh_table = (float4 *)(q_count + 64);
+  uchar4 *out = (uchar4 *)out_device;
+  hipStream_t st = ctx->stream;
+  SL_HIP(ctx, hipMemcpyAsync(sh_table, tab, sizeof(tab), hipMemcpyHostToDevice, st));
+  if (!exact) sl_fill_u32<<<1, 256, 0, st>>>(q_count, 0u, DP_REGIONS);
+  const dim3 settle(std::max(1u, std::min(q_cap / DP_THREADS, 8u)), DP_REGIONS);
+#define SL_DP_LAUNCH(H_, B_)                                                                                  \
+  do {                                                                                                       \
+    if (exact) {                                                                                             \
+      sl_display<H_, B_, true><<<blocks, DP_THREADS, 0, st>>>(hdr_device, out, sh_table, q_count, q_list, q_cap, \
+                                                              a);                                           \
+    } else {                                                                                                 \
+      sl_display<H_, B_, false><<<blocks, DP_THREADS, 0, st>>>(hdr_device, out, sh_table, q_count, q_list,   \
+                                                               q_cap, a);                                    \
+      sl_display_settle<H_, B_><<<settle, DP_THREADS, 0, st>>>(hdr_device, out, sh_table, q_count, q_list,   \
+                                                               q_cap, a);                                    \
+    }                                                                                                        \
+  } while (0)
+  if (half_in) {
+    if (bgra) SL_DP_LAUNCH(true, true);
+    else SL_DP_LAUNCH(true, false);
+  } else {
+    if (bgra) SL_DP_LAUNCH(false, true);
+    else SL_DP_LAUNCH(false, false);
+  }
+#undef SL_DP_LAUNCH
+  return check_launch(ctx, "sl_display");
 }
 
 extern "C" sl_status sl_ctx_set_tuning(sl_context *ctx, const sl_tuning *t) {

@@ -21,6 +21,8 @@ SL_STORAGE_F32, SL_STORAGE_F16, SL_STORAGE_BF16 = 0, 1, 2
 SL_OUT_RGBA32F, SL_OUT_RGBA16F = 0, 1
 SL_FRAME_TIMERS = 1
 SL_FRAME_COUNTING_SORT = 4
+SL_DISPLAY_RGBA8, SL_DISPLAY_BGRA8 = 0, 1
+SL_DISPLAY_EXACT = 1
 ART_KEYS, ART_ORDER, ART_SPLATS, ART_PAIR_TILES, ART_PAIR_SLOTS, ART_RANGES = 1, 2, 3, 4, 5, 6
 
 # Every symbol include/splatlight.h declares (checked by the CPU tests).
@@ -35,6 +37,7 @@ EXPORTS = [
     "sl_batch_encode", "sl_batch_finish", "sl_ctx_set_tuning", "sl_ctx_get_tuning",
     "sl_server_create", "sl_server_submit", "sl_server_poll", "sl_server_info", "sl_server_error",
     "sl_server_destroy", "sl_server_fail_next", "sl_bounce_init", "sl_bounce_encode", "sl_bounce_read",
+    "sl_display_encode",
 ]
 SL_BATCH_GRAPH = 2
 SCENE_COLOUR, SCENE_LAMP_T, SCENE_ENV_T, SCENE_ISO_NORM = 1, 2, 3, 4
@@ -60,6 +63,11 @@ class SceneDesc(C.Structure):
 
 class FrameOpts(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("out_format", C.c_uint32), ("flags", C.c_uint32)]
+
+
+class DisplayOpts(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("hdr_format", C.c_uint32), ("out_format", C.c_uint32),
+                ("flags", C.c_uint32)]
 
 
 class FrameStats(C.Structure):
@@ -144,6 +152,7 @@ def lib() -> C.CDLL:
         "sl_frame_encode": (i32, [vp, vp, vp, sz, C.POINTER(FrameOpts), vp]),
         "sl_frame_encode_noalloc": (i32, [vp, vp, vp, sz, C.POINTER(FrameOpts), vp]),
         "sl_frame_finish": (i32, [vp, C.POINTER(FrameStats)]),
+        "sl_display_encode": (i32, [vp, vp, sz, vp, sz, vp, sz, C.POINTER(DisplayOpts), vp, vp]),
         "sl_frame_read": (i32, [vp, u32, vp, sz, C.POINTER(sz)]),
         "sl_timers_read": (i32, [vp, C.POINTER(C.c_char_p), C.POINTER(C.c_float), u32, C.POINTER(u32)]),
         "sl_lighting_iso_init": (i32, [vp, vp]),
@@ -355,6 +364,20 @@ def frame_encode(ctx: Context, scene: Scene, frame: bytes, width: int, height: i
     fb = (C.c_uint8 * len(frame)).from_buffer_copy(frame)
     o = FrameOpts(width, height, out_format, (SL_FRAME_TIMERS if timers else 0) | (SL_FRAME_COUNTING_SORT if counting else 0))
     ctx.check(ctx.L.sl_frame_encode(ctx.h, scene.h, fb, len(frame), C.byref(o), C.c_void_p(out_dev)))
+
+
+def display_encode(ctx: Context, frame: bytes, width: int, height: int, hdr_dev: int, out_dev: int,
+                   hdr_format: int = SL_OUT_RGBA32F, out_format: int = SL_DISPLAY_RGBA8, env: bytes | None = None,
+                   sky_bg: bytes | None = None, exact: bool = False):
+    """sl_display_encode: the HDR target (hdr_format) to width * height * 4 display bytes, enqueued on the context's
+    stream; env / sky_bg: the Env (416 B) and SkyBg (1520 B) uniforms or None."""
+    fb = (C.c_uint8 * len(frame)).from_buffer_copy(frame)
+    eb = None if env is None else (C.c_uint8 * len(env)).from_buffer_copy(env)
+    bb = None if sky_bg is None else (C.c_uint8 * len(sky_bg)).from_buffer_copy(sky_bg)
+    o = DisplayOpts(width, height, hdr_format, out_format, SL_DISPLAY_EXACT if exact else 0)
+    ctx.check(ctx.L.sl_display_encode(ctx.h, fb, len(frame), eb, 0 if env is None else len(env), bb,
+                                      0 if sky_bg is None else len(sky_bg), C.byref(o), C.c_void_p(hdr_dev),
+                                      C.c_void_p(out_dev)))
 
 
 # Size of the device array the no-alloc frame lays its workspace over (SL_NOALLOC_SCRATCH_BYTES).

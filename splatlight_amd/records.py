@@ -67,6 +67,8 @@ MAX_PARTS = 32              # shaders/common.wgsl:39
 EMITTER_MATERIAL = 254      # shaders/common.wgsl:41
 MAX_EMITTERS = 16384        # src/renderer.js:17
 VIEW_LIT = 3
+# RenderFrame.tone.x: the display's tone curve (shaders/display.wgsl:140-152; AgX by default, src/main.js:198)
+TONE_CLAMP, TONE_AGX, TONE_ACES, TONE_REINHARD, TONE_NEUTRAL, TONE_FILMIC = range(6)
 
 # Material.flags bits (runtime/wgsl/common.wgsl:63)
 MAT_EMISSIVE = 1
@@ -124,11 +126,12 @@ def perspective(fovy, aspect, near, far) -> np.ndarray:
 
 def render_frame(*, view, width, height, fovy, near=0.02, far=50.0, count, view_mode=VIEW_LIT,
                  part_material=None, palette=None, bounce_gain=1.0, emitter_count=0, albedo_scale=1.0,
-                 lowpass=0.3, exposure=1.0, shutter=0.0, blur_coverage=True) -> bytes:
+                 lowpass=0.3, exposure=1.0, shutter=0.0, blur_coverage=True, tone=TONE_AGX) -> bytes:
     """Pack a RenderFrame (1200 B) the way writeFrame does (src/main.js:526-556, 470-475).
 
     shutter: the motion blur's shutter fraction (palette[3].z; moving parts blur over it, 0 = off); blur_coverage:
-    the blurred splats' coverage cap (palette[3].w, ?blurcov). Explode, LOD, refraction, particles and the
+    the blurred splats' coverage cap (palette[3].w, ?blurcov); tone: the display pass's tone curve (tone.x, word
+    260, src/main.js:553). Explode, LOD, refraction, particles and the
     reflection split are left off (zero).
     """
     buf = np.zeros(FRAME_BYTES // 4, np.uint32)
@@ -151,6 +154,7 @@ def render_frame(*, view, width, height, fovy, near=0.02, far=50.0, count, view_
     buf[78] = count
     buf[79] = view_mode
     f32[80:84] = [albedo_scale, 0.01, exposure, lowpass]
+    buf[260] = tone
     return buf.tobytes()
 
 

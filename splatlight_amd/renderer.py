@@ -40,6 +40,9 @@ class SplatRenderer:
         self.hdr = None
         self.width = self.height = 0
         self.hdrFormat = N.SL_OUT_RGBA16F if canvasFormat in (None, "rgba16float") else N.SL_OUT_RGBA32F
+        self.displayFormat = N.SL_DISPLAY_BGRA8 if canvasFormat == "bgra8unorm" else N.SL_DISPLAY_RGBA8
+        self.envBuf = None
+        self.skyBgBuf = None
         self.stats = None
 
     def bindColour(self, colour, bounce=None, emitters=None):
@@ -47,6 +50,13 @@ class SplatRenderer:
         if colour is not None:
             self.scene.write_colour(np.asarray(colour, np.float32))
         self.scene.write_bounce(None if bounce is None else np.asarray(bounce, np.float32))
+
+    def bindEnv(self, envBuf, skyBgBuf=None):
+        """src/renderer.js:400-404: the environment uniform (Env, 416 B) and the background sky (SkyBg, 1520 B) the
+        display pass draws behind the splats; skyBgBuf None keeps the one bound before."""
+        self.envBuf = None if envBuf is None else bytes(envBuf)
+        if skyBgBuf is not None:
+            self.skyBgBuf = bytes(skyBgBuf)
 
     def resize(self, width: int, height: int):
         """src/renderer.js:281: (re)create the linear HDR target."""
@@ -60,8 +70,10 @@ class SplatRenderer:
         """src/renderer.js:407: the per-frame part table."""
         self.scene.write_parts(np.frombuffer(bytes(partBytes), R.PART_DTYPE))
 
-    def encode(self, frameBytes: bytes, viewMode: int | None = None, timers: bool = False):
-        """src/renderer.js:411: one frame into the HDR target. viewMode, when given, must match the frame's."""
+    def encode(self, frameBytes: bytes, viewMode: int | None = None, timers: bool = False, target: int | None = None):
+        """src/renderer.js:411: one frame into the HDR target, then, with a target (a device buffer of width *
+        height * 4 bytes), the display pass into it (exposure, tone curve, sRGB; the bound environment's sky
+        behind the splats). viewMode, when given, must match the frame's."""
         if self.hdr is None:
             raise N.SplatlightError(1, "resize() before encode()")
         if viewMode is not None and int(np.frombuffer(frameBytes, np.uint32)[79]) != viewMode:
@@ -69,7 +81,22 @@ class SplatRenderer:
         N.frame_encode(self.device, self.scene, frameBytes, self.width, self.height, self.hdr, self.hdrFormat,
                        timers)
         self.stats = N.frame_finish(self.device)
+        if target is not None:
+            N.display_encode(self.device, frameBytes, self.width, self.height, self.hdr, target, self.hdrFormat,
+                             self.displayFormat, self.envBuf, self.skyBgBuf)
+            self.device.sync()
         return self.stats
+
+    def captureLinear(self) -> bytes:
+        """The HDR target as captureLinear posts it (src/main.js:909-919): width * height rgba float16, rows
+        top-down, 8 bytes per pixel, no header (the .f16 capture file). An rgba32float target is rounded to
+        float16 (nearest even) first."""
+        if self.hdr is None:
+            raise N.SplatlightError(1, "resize() before captureLinear()")
+        if self.hdrFormat == N.SL_OUT_RGBA16F:
+            return self.device.read(self.hdr, self.width * self.height * 8, np.uint8).tobytes()
+        a = self.device.read(self.hdr, self.width * self.height * 16, np.float32)
+        return a.astype("<f2").tobytes()
 
     def readHdr(self) -> np.ndarray:
         """The HDR target as float32 [H, W, 4], rows top-down (captureLinear's layout, src/main.js:871-920)."""
@@ -79,6 +106,10 @@ class SplatRenderer:
             a = self.device.read(self.hdr, self.width * self.height * 8, np.float16).astype(np.float32)
         return a.reshape(self.height, self.width, 4)
 
+    def readDisplay(self, target: int) -> np.ndarray:
+        """A display target as uint8 [H, W, 4] in its byte order (rgba8unorm or bgra8unorm)."""
+        return self.device.read(target, self.width * self.height * 4, np.uint8).reshape(self.height, self.width, 4)
+
     def timers(self) -> dict:
         return N.timers(self.device)
 
@@ -87,3 +118,13 @@ class SplatRenderer:
             self.device.free(self.hdr)
             self.hdr = None
         self.scene.destroy()
+
+
+def read_f16(data, width: int, height: int) -> np.ndarray:
+    """A .f16 capture (captureLinear's bytes or a file path) as float32 [height, width, 4], rows top-down."""
+    if isinstance(data, str):
+        with open(data, "rb") as f:
+            data = f.read()
+    if len(data) != width * height * 8:
+        raise ValueError(f".f16 capture of {len(data)} bytes is not {width}x{height} rgba float16")
+    return np.frombuffer(data, "<f2").astype(np.float32).reshape(height, width, 4)

@@ -62,14 +62,14 @@ class Scene:
         return R.look_at(self.eye, self.target, (0.0, 0.0, 1.0))
 
     def frame_bytes(self, width=None, height=None, eye=None, target=None, view_mode=R.VIEW_LIT,
-                    bounce_gain=0.0, shutter=0.0, blur_coverage=True) -> bytes:
+                    bounce_gain=0.0, shutter=0.0, blur_coverage=True, exposure=1.0, tone=R.TONE_AGX) -> bytes:
         w = width or self.width
         h = height or self.height
         v = R.look_at(eye or self.eye, target or self.target, (0.0, 0.0, 1.0))
         return R.render_frame(view=v, width=w, height=h, fovy=math.radians(self.fov_deg), near=0.02,
                               far=self.far, count=self.count, view_mode=view_mode,
                               emitter_count=self.emitters.shape[0], bounce_gain=bounce_gain, shutter=shutter,
-                              blur_coverage=blur_coverage)
+                              blur_coverage=blur_coverage, exposure=exposure, tone=tone)
 
 
 def _quats(rng, n):
