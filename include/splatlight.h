@@ -78,6 +78,36 @@ sl_status sl_scene_write_colour(sl_context *ctx, sl_scene *scene, const float *r
 sl_status sl_scene_write_bounce(sl_context *ctx, sl_scene *scene, const float *lo_b, size_t count);
 size_t sl_scene_device_bytes(const sl_scene *scene);
 
+/* ---- the real-asset path: screen-space LOD and display fill ------------------------------------------------
+ * The asset's tree as the LOD (runtime/wgsl/lod.wgsl, src/lod.js; on by default at 2 px, src/main.js:113): every
+ * internal node is drawn as its moment-matched Gaussian in place of its subtree when it is small on screen. Once the
+ * scene holds its tree, a frame whose RenderFrame.lod (words 180-182: threshold px as f32 bits, first node slot,
+ * internal node count; src/main.js:548) is non-zero selects the cut (lod_select: part roots are candidates, a
+ * candidate opens when its on-screen size, by the projected-area rule, exceeds the threshold, and the emitters' part
+ * always opens), colours the drawn nodes from their leaves' lit colour plus bounce (lod_colour) and draws the cut
+ * in place of the leaves. The internal nodes take the splat slots after the emitters, in node order (the reference
+ * puts them after its 16,384 emitter slots: the same relative order, so the same depth order); any first-node-slot
+ * word past this scene's emitters is accepted. A LOD frame renders alone (sl_frame_encode, not a batch, the
+ * server or the no-alloc entry point); its cut lives in the context. */
+typedef struct sl_lod_tree {
+  const void *nodes; uint32_t node_count; uint32_t node_stride;   /* all N nodes, 96 B (nodes.bin): leaves [0, G) */
+  const uint32_t *kids; uint32_t kid_count;                        /* kids.bin */
+  const void *node_gaussians; uint32_t node_gaussian_count;        /* the N - G internal nodes as 64 B Gaussian
+                                                                      records (lod.js nodeGaussians) */
+  uint32_t emit_part;                                              /* the emitters' part, or 0xFFFFFFFF */
+  uint32_t size_rule;                                              /* 1 projected area (default), 0 radius */
+} sl_lod_tree;
+/* Lod's constructor (src/lod.js:114-140) with nodeGaussians' records: validates the tree (leaves first, every
+ * internal node's children one depth below it, each node at most one parent, depth <= 64) and keeps it on the
+ * device. Once per scene. */
+sl_status sl_scene_set_lod(sl_context *ctx, sl_scene *scene, const sl_lod_tree *tree);
+/* Lighting.loadFill (src/lighting.js:961-985): the asset's display fill table (onepath.bin `fill`: u32 offsets
+ * [count + 1], then (source index, f32 weight bits) pairs; targets and sources disjoint). */
+sl_status sl_scene_set_fill(sl_context *ctx, sl_scene *scene, const uint32_t *fill, size_t words);
+/* Lighting.encodeFill (runtime/wgsl/fill.wgsl:15-33), after the lighting and the bounce: each fill target's colour
+ * and bounce toward the eye become the weighted sum of its sources'. Enqueued on the context's stream. */
+sl_status sl_fill_encode(sl_context *ctx, sl_scene *scene);
+
 /* ---- frame: SplatRenderer.encode (src/renderer.js:411-524) up to the linear HDR target ---------------------- */
 #define SL_OUT_RGBA32F 0u
 #define SL_OUT_RGBA16F 1u      /* the reference's rgba16float target (src/renderer.js:13) */
@@ -223,6 +253,9 @@ void sl_server_destroy(sl_server *server);
 #define SL_ART_PAIR_TILES 4u   /* u32 per pair: tile id, ascending */
 #define SL_ART_PAIR_SLOTS 5u   /* u32 per pair: splat slot, each tile's list in back-to-front order */
 #define SL_ART_RANGES 6u       /* 2 u32 per tile: [start, end) into the pair lists */
+#define SL_ART_LOD_FLAGS 7u    /* a LOD frame: u32 per tree node, lod_select's flag (0 none, 1 open, 2 draw, 4 under) */
+#define SL_ART_LOD_COLOUR 8u   /* a lit LOD frame: vec4 per internal node, lod_colour's colour (drawn nodes and the
+                                  nodes under them; others undefined) */
 sl_status sl_frame_read(sl_context *ctx, uint32_t which, void *host, size_t bytes, size_t *written);
 
 /* ---- relighting: Lighting.encode (src/lighting.js:1258-1274) for the default static M39 matte path ----------------

@@ -477,3 +477,102 @@ def view_dir(frame: bytes, pxy):
     w = np.zeros((p.shape[0], 3), np.float32)
     lib().or_view_dir_n(_p(np.frombuffer(frame, np.uint8).copy()), _p(p), C.c_uint32(p.shape[0]), _p(w))
     return w
+
+
+# ---- screen-space LOD and display fill (lod.c) ---------------------------------------------------------------------
+
+class LodParams(C.Structure):
+    _fields_ = [("view", C.c_float * 16), ("threshold", C.c_float), ("focal", C.c_float), ("near_", C.c_float),
+                ("emit_part", C.c_uint32), ("open_parts", C.c_uint32), ("size_rule", C.c_uint32)]
+
+
+class LodView(C.Structure):
+    _fields_ = [("flag", C.c_void_p), ("records", C.c_void_p), ("colour", C.c_void_p), ("n_nodes", C.c_uint32)]
+
+
+def lod_params(view, threshold, focal, near=0.02, emit_part=0xFFFFFFFF, open_parts=0, size_rule=1):
+    p = LodParams()
+    p.view[:] = [float(x) for x in np.asarray(view, np.float32).ravel()]
+    p.threshold, p.focal, p.near_ = threshold, focal, near
+    p.emit_part, p.open_parts, p.size_rule = emit_part, open_parts, size_rule
+    return p
+
+
+def lod_select(params: LodParams, nodes, kids, parts, G: int):
+    """lod_select's flags per node (0 none, 1 open, 2 draw, 4 under)."""
+    nd = _b(nodes)
+    N = nd.nbytes // 96
+    flag = np.zeros(N, np.uint32)
+    L = lib()
+    L.or_lod_select.restype = None
+    L.or_lod_select.argtypes = [C.c_void_p] * 4 + [C.c_uint32, C.c_uint32, C.c_void_p]
+    L.or_lod_select(C.byref(params), _p(nd), _p(np.ascontiguousarray(kids, np.uint32)), _p(_b(parts)), N, G,
+                    _p(flag))
+    return flag
+
+
+def node_mass(nodes, n: int) -> float:
+    L = lib()
+    L.or_node_mass.restype = C.c_float
+    L.or_node_mass.argtypes = [C.c_void_p, C.c_uint32]
+    return float(L.or_node_mass(_p(_b(nodes)), n))
+
+
+def lod_colour(nodes, kids, G: int, flag, colour, lo_b, gain: float):
+    """Node colours (vec4 per internal node) of the drawn nodes and their subtrees; zeros elsewhere."""
+    nd = _b(nodes)
+    N = nd.nbytes // 96
+    out = np.zeros((max(N - G, 1), 4), np.float32)
+    col = np.ascontiguousarray(colour, np.float32).reshape(-1, 4)
+    lb = np.zeros((G, 12), np.float32) if lo_b is None else np.ascontiguousarray(lo_b, np.float32).reshape(-1, 12)
+    L = lib()
+    L.or_lod_colour.restype = None
+    L.or_lod_colour.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                C.c_float, C.c_void_p]
+    L.or_lod_colour(_p(nd), _p(np.ascontiguousarray(kids, np.uint32)), N, G,
+                    _p(np.ascontiguousarray(flag, np.uint32)), _p(col), _p(lb), gain, _p(out))
+    return out[:N - G]
+
+
+def display_fill(fill, G: int, colour, lo_b):
+    """display_fill on copies of colour (vec4 per Gaussian) and lo_b (3 vec4 per Gaussian)."""
+    col = np.array(colour, np.float32).reshape(-1, 4)
+    lb = np.array(lo_b, np.float32).reshape(-1, 12)
+    L = lib()
+    L.or_display_fill.restype = None
+    L.or_display_fill.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]
+    L.or_display_fill(_p(np.ascontiguousarray(fill, np.uint32)), G, _p(col), _p(lb))
+    return col, lb
+
+
+def project_lod(frame: bytes, scene, flag, records, node_colour, colour=None, lo_b=None):
+    """or_project_f32 with the LOD's cut: (splats u32 [slots, 12], keys u32 [slots], visible)."""
+    fb = np.frombuffer(frame, np.uint8).copy()
+    g = np.ascontiguousarray(scene.gaussians)
+    p = np.ascontiguousarray(scene.parts)
+    e = np.ascontiguousarray(scene.emitters) if scene.emitters is not None else np.zeros(0, g.dtype)
+    n_emit = e.shape[0]
+    rec = np.ascontiguousarray(records)
+    nn = rec.nbytes // 64
+    ncol = np.ascontiguousarray(node_colour, np.float32).reshape(-1, 4)
+    fl = np.ascontiguousarray(flag, np.uint32)
+    lv = LodView(_p(fl), _p(rec), _p(ncol), nn)
+    nslots = g.shape[0] + 1 + n_emit + nn
+    col = None if colour is None else np.ascontiguousarray(colour, np.float32).reshape(-1, 4)
+    lb = None if lo_b is None else np.ascontiguousarray(lo_b, np.float32).reshape(-1, 12)
+    splats = np.zeros((nslots, 12), np.uint32)
+    keys = np.zeros(nslots, np.uint32)
+    L = lib()
+    L.or_project_lod_f32.restype = C.c_uint32
+    L.or_project_lod_f32.argtypes = [C.c_void_p] * 6 + [C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]
+    vis = L.or_project_lod_f32(_p(fb), _p(g), _p(p), _p(col), _p(lb), _p(e), n_emit, C.byref(lv), _p(splats),
+                               _p(keys))
+    return splats, keys, int(vis)
+
+
+def lod_node_size(params: LodParams, nodes, parts, n: int) -> float:
+    """lod_select's on-screen size (px) of internal node n (before the threshold)."""
+    L = lib()
+    L.or_lod_node_size.restype = C.c_float
+    L.or_lod_node_size.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32]
+    return float(L.or_lod_node_size(C.byref(params), _p(_b(nodes)), _p(_b(parts)), n))

@@ -90,8 +90,14 @@ static float expf_cr_(float x) { return (float)exp((double)x); }
 uint32_t or_project_f32(const void *frame, const void *gaussians, const void *parts, const float *colour,
                         const float *lo_b, const void *emitters, uint32_t n_emit, uint32_t *splats,
                         uint32_t *keys) {
+  return or_project_lod_f32(frame, gaussians, parts, colour, lo_b, emitters, n_emit, NULL, splats, keys);
+}
+
+uint32_t or_project_lod_f32(const void *frame, const void *gaussians, const void *parts, const float *colour,
+                            const float *lo_b, const void *emitters, uint32_t n_emit, const or_lod_view *lod,
+                            uint32_t *splats, uint32_t *keys) {
   uint32_t count = uw(frame, FR_COUNT);
-  uint32_t nslots = count + 1u + n_emit;
+  uint32_t nslots = count + 1u + n_emit + (lod ? lod->n_nodes : 0u);
   uint32_t visible = 0;
 #pragma omp parallel for schedule(dynamic, 4096) reduction(+ : visible)
   for (uint32_t i = 0; i < nslots; i++) {
@@ -99,7 +105,8 @@ uint32_t or_project_f32(const void *frame, const void *gaussians, const void *pa
     float depth;
     uint32_t sflags = 0;
     keys[i] = INVALID_KEY;
-    if (!project_one_f32(frame, gaussians, parts, colour, lo_b, emitters, n_emit, i, s, &depth, &sflags)) continue;
+    if (!project_one_f32(frame, gaussians, parts, colour, lo_b, emitters, n_emit, lod, i, s, &depth, &sflags))
+      continue;
     for (int k = 0; k < 11; k++) splats[12 * i + k] = f_bits(s[k]);
     splats[12 * i + 11] = (f_bits(depth) & ~3u) | sflags;
     keys[i] = or_depth_key(depth, fw(frame, FR_NEAR), fw(frame, FR_FAR));
@@ -117,7 +124,8 @@ void or_project_f64(const void *frame, const void *gaussians, const void *parts,
     double s[12];
     double depth = 0.0;
     uint32_t sflags = 0;
-    valid[i] = (uint8_t)project_one_f64(frame, gaussians, parts, colour, lo_b, emitters, n_emit, i, s, &depth, &sflags);
+    valid[i] = (uint8_t)project_one_f64(frame, gaussians, parts, colour, lo_b, emitters, n_emit, NULL, i, s, &depth,
+                                        &sflags);
     for (int k = 0; k < 11; k++) out[12 * i + k] = valid[i] ? s[k] : 0.0;
     out[12 * i + 11] = valid[i] ? depth : 0.0;
   }

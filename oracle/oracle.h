@@ -20,6 +20,25 @@ extern "C" {
 /* threads of the parallel loops (OMP_NUM_THREADS, else the host cores) */
 uint32_t or_threads(void);
 
+/* The screen-space LOD's cut as project draws it: flag[N] (lod_select's flags, LOD_DRAW = 2), the N - G internal
+ * nodes' 64 B Gaussian records (lod.js nodeGaussians) and their lit colours (vec4), in splat slots count + 1 +
+ * n_emit + k. */
+typedef struct or_lod_view {
+  const uint32_t *flag;
+  const void *records;
+  const float *colour;
+  uint32_t n_nodes;
+} or_lod_view;
+
+/* lod_select's uniforms (lod.wgsl LodU): the view matrix (column-major), threshold px, focal px, near plane, the
+ * emitters' part (always opened; 0xFFFFFFFF none), parts always opened (bit per part), size_rule bit 0 the
+ * projected-area measure */
+typedef struct or_lod_params {
+  float view[16];
+  float threshold, focal, near_;
+  uint32_t emit_part, open_parts, size_rule;
+} or_lod_params;
+
 /* 16-bit log-depth key (shaders/project.wgsl:366-367) with log fixed as the correctly rounded float log. */
 uint32_t or_depth_key(float depth, float near_, float far_);
 
@@ -30,6 +49,11 @@ uint32_t or_depth_key(float depth, float near_, float far_);
 uint32_t or_project_f32(const void *frame, const void *gaussians, const void *parts, const float *colour,
                         const float *lo_b, const void *emitters, uint32_t n_emit, uint32_t *splats,
                         uint32_t *keys);
+
+/* or_project_f32 with the LOD's cut (lod may be NULL); nslots = count + 1 + n_emit + lod->n_nodes. */
+uint32_t or_project_lod_f32(const void *frame, const void *gaussians, const void *parts, const float *colour,
+                            const float *lo_b, const void *emitters, uint32_t n_emit, const or_lod_view *lod,
+                            uint32_t *splats, uint32_t *keys);
 
 /* The same in float64: out[12*slot] = centre.xy, axis0.xy, axis1.xy, r, tau0, rgb, depth; valid[slot]. */
 void or_project_f64(const void *frame, const void *gaussians, const void *parts, const float *colour,
@@ -127,6 +151,18 @@ void or_phase_specular_n(const float *Sd, const float *So, const float *wi, cons
 void or_onb(const float *n, float *out6);
 float or_asg_phase(const float *wi, const float *lb, const float *sl, const float *n0, const float *t1,
                    const float *t2, const float *flake, const float *Sd, const float *So, const float *wo);
+
+/* screen-space LOD and display fill (lod.c): lod_select's flags per node (N, leaves [0, G) first; 0 none, 1 open,
+ * 2 draw, 4 under), lod_colour's node colours (vec4 per internal node) from the leaves' colour (vec4) and bounce (3
+ * vec4 per leaf, lane 0 toward the eye), node_mass, and display_fill in place on colour and lo_b's lane 0
+ * (fill: offsets [G + 1], then (source index, f32 weight bits) pairs) */
+void or_lod_select(const or_lod_params *L, const void *nodes, const uint32_t *kids, const void *parts, uint32_t N,
+                   uint32_t G, uint32_t *flag);
+float or_node_mass(const void *nodes, uint32_t n);
+float or_lod_node_size(const or_lod_params *L, const void *nodes, const void *parts, uint32_t n);
+void or_lod_colour(const void *nodes, const uint32_t *kids, uint32_t N, uint32_t G, const uint32_t *flag,
+                   const float *colour, const float *lo_b, float gain, float *node_colour);
+void or_display_fill(const uint32_t *fill, uint32_t G, float *colour, float *lo_b);
 
 /* display pass (display.c; shaders/display.wgsl:22-189): hdr W*H*4 floats rows top-down; env: Env (416 B) or
  * NULL; bg: SkyBg (1520 B) or NULL; out: W*H*4 floats (the encoded colour before the 8-bit store, alpha 1) or

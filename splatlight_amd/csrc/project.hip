@@ -39,17 +39,25 @@ __device__ __forceinline__ uint32_t depth_key(float depth, float near_, float lo
 
 // The splat colour of a visible slot (project.wgsl:317-350): the lit colour plus gain x bounce in the default view,
 // else the albedo or the part palette, scaled.
+// node: an LOD node's lit colour (its bounce already included, project.wgsl:325), else null.
 __device__ __forceinline__ void splat_colour(const FrameDev &f, uint32_t material, float4 C4,
                                              const float4 *__restrict__ colour, const float4 *__restrict__ lo_b,
-                                             uint32_t i, float col[3]) {
+                                             uint32_t i, float col[3], const float4 *__restrict__ node) {
   col[0] = C4.x; col[1] = C4.y; col[2] = C4.z;
   if (material != SL_EMITTER_MATERIAL) {
     if (f.view_mode == 3u) {
-      const float4 cv = colour ? colour[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-      const float4 lb = lo_b ? lo_b[3 * (size_t)i] : make_float4(0.f, 0.f, 0.f, 0.f);
-      col[0] = cv.x + f.gain * lb.x;
-      col[1] = cv.y + f.gain * lb.y;
-      col[2] = cv.z + f.gain * lb.z;
+      if (node) {
+        const float4 nc = *node;
+        col[0] = nc.x;
+        col[1] = nc.y;
+        col[2] = nc.z;
+      } else {
+        const float4 cv = colour ? colour[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const float4 lb = lo_b ? lo_b[3 * (size_t)i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        col[0] = cv.x + f.gain * lb.x;
+        col[1] = cv.y + f.gain * lb.y;
+        col[2] = cv.z + f.gain * lb.z;
+      }
       if (material < 8u) { col[0] *= f.palette[material][0]; col[1] *= f.palette[material][1]; col[2] *= f.palette[material][2]; }
     } else if (material < 8u && f.palette[material][3] > 0.5f) {
       col[0] = f.palette[material][0]; col[1] = f.palette[material][1]; col[2] = f.palette[material][2];
@@ -78,20 +86,34 @@ sl_project_kernel(const ViewDev *__restrict__ views, const uint32_t *__restrict_
   const float4 *__restrict__ parts = vd.parts, *__restrict__ colour = vd.colour, *__restrict__ lo_b = vd.lo_b;
   const uint32_t i = gi - vd.slot_base;
   bool ok = i < f.nslots && i != f.count;
+  // the LOD's cut (project.wgsl:90-109): node slots read the node streams, only flagged slots are drawn
+  bool is_node = false;
+  uint32_t src = i;
+  if (ok && vd.lod_flag) {
+    if (i >= vd.node_slot0) {
+      src = i - vd.node_slot0;
+      is_node = true;
+      ok = vd.lod_flag[f.count + src] == SL_LOD_DRAW;
+    } else if (i < f.count) {
+      ok = vd.lod_flag[i] == SL_LOD_DRAW;
+    }
+  }
   float4 A, Q4, S4, C4;
   float mu[3], rot[3][3];
   uint32_t material = SL_EMITTER_MATERIAL;
   float keep = 1.0f;
   float vel[3] = {0.0f, 0.0f, 0.0f};   // BLUR: world displacement over the exposure (project.wgsl:138-149)
   if (ok) {
-    if (i > f.count) {                                                // emitters (project.wgsl:123-127)
+    if (i > f.count && !is_node) {                                    // emitters (project.wgsl:123-127)
       const uint32_t k = i - f.count - 1u;
       A = e0[k]; Q4 = e1[k]; S4 = e2[k]; C4 = e3[k];
       mu[0] = A.x; mu[1] = A.y; mu[2] = A.z;
       quat_to_mat(Q4, rot);
     } else {                                                          // pose (project.wgsl:128-148)
-      A = g0[i]; Q4 = sl_quat(g1, vd.storage, i); S4 = sl_scale(g2, vd.storage, i);
-      C4 = sl_albedo(g1, g2, g3, vd.storage, i);
+      const float4 *__restrict__ s0 = is_node ? vd.n0 : g0, *__restrict__ s1 = is_node ? vd.n1 : g1;
+      const float4 *__restrict__ s2 = is_node ? vd.n2 : g2, *__restrict__ s3 = is_node ? vd.n3 : g3;
+      A = s0[src]; Q4 = sl_quat(s1, vd.storage, src); S4 = sl_scale(s2, vd.storage, src);
+      C4 = sl_albedo(s1, s2, s3, vd.storage, src);
       uint32_t part = __float_as_uint(S4.w) & 0xffffu;
       part = part < SL_MAX_PARTS - 1u ? part : SL_MAX_PARTS - 1u;
       const float4 x0 = parts[part * 8 + 0], x1 = parts[part * 8 + 1], x2 = parts[part * 8 + 2];
@@ -269,12 +291,12 @@ sl_project_kernel(const ViewDev *__restrict__ views, const uint32_t *__restrict_
     ok = ok && !(fabsf(cenx) - ex > 1.0f || fabsf(ceny) - ey > 1.0f);
     if (ok) {
       float col[3] = {0.0f, 0.0f, 0.0f};
-      if (COLOUR) splat_colour(f, material, C4, colour, lo_b, i, col);
+      if (COLOUR) splat_colour(f, material, C4, colour, lo_b, i, col, is_node ? vd.node_colour + src : nullptr);
       out.center_axis0 = make_float4(cenx, ceny, a0x, a0y);
       out.axis1_r_tau = make_float4(a1x, a1y, r, tau0);
       // SPLAT_GLASS (shaders/lens.wgsl:21) in the depth word's low bits (project.wgsl:167-168)
       const uint32_t mat_idx = min(__float_as_uint(C4.w) & 0xffffu, 31u);
-      const uint32_t glass = material != SL_EMITTER_MATERIAL ? (f.glass_mask >> mat_idx) & 1u : 0u;
+      const uint32_t glass = material != SL_EMITTER_MATERIAL && !is_node ? (f.glass_mask >> mat_idx) & 1u : 0u;
       out.color_depth = make_uint4(__float_as_uint(col[0]), __float_as_uint(col[1]), __float_as_uint(col[2]),
                                    (__float_as_uint(depth) & ~3u) | glass);
       key = (v << 16) | depth_key(depth, f.near_, f.log_far_near);
@@ -330,16 +352,20 @@ sl_splat_colour(const ViewDev *__restrict__ views, const uint32_t *__restrict__ 
   if (keys[gi] != SL_INVALID_KEY) {
     float4 C4;
     uint32_t material = SL_EMITTER_MATERIAL;
-    if (i > f.count) {
+    const bool is_node = vd.lod_flag && i >= vd.node_slot0;
+    const uint32_t src = is_node ? i - vd.node_slot0 : i;
+    if (i > f.count && !is_node) {
       C4 = vd.e3[i - f.count - 1u];
     } else {
-      C4 = sl_albedo(vd.g1, vd.g2, vd.g3, vd.storage, i);
-      uint32_t part = __float_as_uint(sl_scale(vd.g2, vd.storage, i).w) & 0xffffu;
+      const float4 *__restrict__ s1 = is_node ? vd.n1 : vd.g1, *__restrict__ s2 = is_node ? vd.n2 : vd.g2;
+      const float4 *__restrict__ s3 = is_node ? vd.n3 : vd.g3;
+      C4 = sl_albedo(s1, s2, s3, vd.storage, src);
+      uint32_t part = __float_as_uint(sl_scale(s2, vd.storage, src).w) & 0xffffu;
       part = part < SL_MAX_PARTS - 1u ? part : SL_MAX_PARTS - 1u;
       material = f.part_material[part];
     }
     float col[3];
-    splat_colour(f, material, C4, vd.colour, vd.lo_b, i, col);
+    splat_colour(f, material, C4, vd.colour, vd.lo_b, i, col, is_node ? vd.node_colour + src : nullptr);
     uint32_t *cw = reinterpret_cast<uint32_t *>(&splats[gi].color_depth);
     cw[0] = __float_as_uint(col[0]);
     cw[1] = __float_as_uint(col[1]);

@@ -29,6 +29,8 @@ struct FrameDev {
   uint32_t width, height, tiles_x, tiles_y;
   float shutter;             // motion blur: shutter fraction (palette[3].z); moving parts blur when > 0
   uint32_t blur_cover;       // the blurred splats' coverage cap (palette[3].w > 0.5)
+  float lod_threshold;       // screen-space LOD: the node footprint (px) to open at (frame.lod.x); 0 = off
+  uint32_t n_nodes;          // its internal nodes drawn after the emitters (frame.lod.z), 0 when off
 };
 
 // Gaussian attribute storage (SL_STORAGE_*). F32: four float4 streams g0 (mu, kappa), g1 quat, g2 (scale,
@@ -72,7 +74,19 @@ struct ViewDev {
   uint32_t out_format;
   uint32_t slot_base, tile_base;
   uint32_t storage;                         // SL_STORAGE_* of the scene's attribute streams
+  // screen-space LOD (lod_flag non-null): slots from node_slot0 on are the tree's internal nodes k < n_nodes, their
+  // records in the n streams (the scene's storage), their lit colour node_colour[k]; only nodes and leaves whose
+  // lod_flag is SL_LOD_DRAW are drawn (node count + k is node k's flag)
+  const uint32_t *lod_flag;
+  const float4 *n0, *n1, *n2, *n3;
+  const float4 *node_colour;
+  uint32_t node_slot0, n_nodes;
 };
+
+#define SL_LOD_NONE 0u
+#define SL_LOD_OPEN 1u
+#define SL_LOD_DRAW 2u
+#define SL_LOD_UNDER 4u
 
 // Per-view results of a batch.
 struct ViewCounters {

@@ -26,6 +26,7 @@
 #include "lighting.hip"
 #include "bounce.hip"
 #include "display.hip"
+#include "lod.hip"
 #pragma clang fp contract(fast)
 
 namespace {
@@ -62,6 +63,8 @@ struct Batch {
   bool fixed = false;   // laid over the device's no-alloc scratch instead of the context's workspace
   bool counting = false;   // depth order by the counting scatter (SL_FRAME_COUNTING_SORT) instead of the radix sort
   bool blur = false;       // some view blurs its moving parts (shutter > 0): the projection's motion-blur variant
+  bool lod = false;        // the (single) view draws a screen-space LOD cut: lod_select / lod_colour first
+  LodDev lod_dev{};
 };
 
 uint32_t rd_u32(const void *p, int word) { uint32_t v; memcpy(&v, (const char *)p + 4 * word, 4); return v; }
@@ -102,6 +105,18 @@ struct sl_scene {
 
   uint32_t lsm_parts = 0, lsm_res = 0, lsm_layer_texels = 0;
   uint32_t sun_n = 0, sun_res = 0, sun_layer_texels = 0;
+  // screen-space LOD (sl_scene_set_lod): the tree as the LOD kernels read it and its internal nodes as Gaussians
+  struct Lod {
+    bool on = false;
+    uint32_t N = 0, emit_part = 0xFFFFFFFFu, size_rule = 1u;
+    float4 *na = nullptr, *nb = nullptr, *nc = nullptr;
+    uint32_t *up = nullptr, *part = nullptr, *kids = nullptr;
+    uint2 *child = nullptr;
+    float *mass = nullptr;
+    float4 *ng[4] = {nullptr, nullptr, nullptr, nullptr};
+  } lod;
+  // display fill (sl_scene_set_fill): offsets [count + 1], then (source, weight bits) pairs
+  uint32_t *fill = nullptr;
   // sky and bounce through probes (sl_bounce_init): the whole tree's nodes (leaves first) replace the leaf array
   // above; the state the bounce passes carry from frame to frame; the last lighting encode's uniforms
   struct Bounce {
@@ -193,6 +208,7 @@ struct sl_context {
   } map[2];
   Grow map_stats;
   Grow disp_q;        // the display pass's list of pixels to settle exactly: a count word, then one word per pixel
+  Grow lod_flag, lod_open, lod_cnt, lod_col;   // the LOD cut of the last frame: flags, open bits, counts, colours
   uint32_t map_records[2] = {0, 0}, map_tiles[2] = {0, 0};
   const uint32_t *map_touched[2] = {nullptr, nullptr};   // the last sums' tile bits (map scratch)
   // the sun-map chain's stream (created on first use, default priority), the fork/join events that order it with
@@ -238,6 +254,8 @@ sl_status check_launch(sl_context *ctx, const char *what) {
   if (e != hipSuccess) return hip_fail(ctx, e, what);
   return SL_OK;
 }
+
+sl_status grow_ctx(sl_context *ctx, Grow *gb, size_t bytes);
 
 sl_status dev_alloc(sl_context *ctx, size_t bytes, void **p) {
   *p = nullptr;
@@ -418,7 +436,18 @@ sl_status decode_frame(sl_context *ctx, const sl_scene *sc, const void *fb, cons
   if (f->view_mode > 3u) return fail(ctx, SL_ERR_INVALID, "view_mode must be 0-3");
   if (refl != 0u && f->view_mode == 3u && rd_f32(fb, 67) < 0.5f)
     return fail(ctx, SL_ERR_UNSUPPORTED, "reflection coverage / Fresnel split (lamp_color.w) is not on this path");
-  if (rd_u32(fb, 180) != 0u) return fail(ctx, SL_ERR_UNSUPPORTED, "screen-space LOD (frame.lod) is not on this path");
+  if (rd_u32(fb, 180) != 0u) {   // screen-space LOD (project.wgsl:90-109): frame.lod = (threshold px, base, nodes)
+    if (!sc->lod.on)
+      return fail(ctx, SL_ERR_UNSUPPORTED, "frame.lod draws the tree's cut: give the scene its tree (sl_scene_set_lod)");
+    const uint32_t n_nodes = rd_u32(fb, 182);
+    if (n_nodes != sc->lod.N - sc->count)
+      return fail(ctx, SL_ERR_INVALID, "frame.lod.z " + std::to_string(n_nodes) + " != the tree's internal nodes " +
+                                           std::to_string(sc->lod.N - sc->count));
+    if (rd_u32(fb, 181) < sc->count + 1u + sc->n_emit)
+      return fail(ctx, SL_ERR_INVALID, "frame.lod.y (the first node slot) lies among the Gaussian or emitter slots");
+    f->lod_threshold = rd_f32(fb, 180);
+    f->n_nodes = n_nodes;
+  }
   if (rd_f32(fb, 194) > 0.5f) return fail(ctx, SL_ERR_UNSUPPORTED, "explode is not on this path");
   if (rd_u32(fb, 296) != 0u) return fail(ctx, SL_ERR_UNSUPPORTED, "particles are not on this path");
   f->shutter = shutter > 0.0f ? shutter : 0.0f;   // moving parts (Part.flags bit 1) blur over it (project.wgsl:138)
@@ -427,7 +456,7 @@ sl_status decode_frame(sl_context *ctx, const sl_scene *sc, const void *fb, cons
   if ((uint32_t)f->vw != o->width || (uint32_t)f->vh != o->height || o->width == 0 || o->height == 0)
     return fail(ctx, SL_ERR_INVALID, "RenderFrame viewport does not match the output size");
   f->n_emit = sc->n_emit;
-  f->nslots = sc->count + 1u + sc->n_emit;
+  f->nslots = sc->count + 1u + sc->n_emit + f->n_nodes;
   f->width = o->width;
   f->height = o->height;
   f->tiles_x = (o->width + 15u) / 16u;
@@ -458,6 +487,23 @@ sl_status enqueue_batch(sl_context *ctx, const Batch &b, bool timers, bool overl
   if ((s = mark(0, st)) != SL_OK) return s;
   sl_fill_u32<<<grid_for(w.scratch_words, 256, 2048), 256, 0, st>>>((uint32_t *)w.scratch, 0u,
                                                                     (uint32_t)w.scratch_words);
+  // the LOD cut (lod_select) before the projection draws it; the node colours (lod_colour) once the leaves' lit
+  // colours are final: here without overlap, after the lighting's wait with it
+  const LodDev &L = b.lod_dev;
+  uint32_t *lod_flag = (uint32_t *)ctx->lod_flag.p;
+  auto lod_colour = [&](hipStream_t on) {
+    const uint32_t ni = L.N - L.G;
+    sl_fill_u32<<<grid_for(ni, 256, 2048), 256, 0, on>>>((uint32_t *)ctx->lod_cnt.p, 0u, ni);
+    sl_lod_colour<<<(L.G + 255u) / 256u, 256, 0, on>>>(L, lod_flag, b.scenes[0]->colour, b.scenes[0]->lo_b,
+                                                        b.views[0].f.gain, (uint32_t *)ctx->lod_cnt.p,
+                                                        (float4 *)ctx->lod_col.p);
+  };
+  const bool lod_lit = b.lod && b.views[0].f.view_mode == 3u;
+  if (b.lod) {
+    sl_lod_open<<<(L.N - L.G + 255u) / 256u, 256, 0, st>>>(L, (uint8_t *)ctx->lod_open.p);
+    sl_lod_flags<<<(L.N + 255u) / 256u, 256, 0, st>>>(L, (const uint8_t *)ctx->lod_open.p, lod_flag);
+    if (lod_lit && !overlap) lod_colour(st);
+  }
   if ((s = mark(1, st)) != SL_OK) return s;
   auto project = [&](auto kernel) {
     kernel<<<b.total_slots / 256u, 256, 0, st>>>(w.views, w.slot_base, V, w.splats, w.keys, w.tbox, w.partials);
@@ -524,6 +570,7 @@ sl_status enqueue_batch(sl_context *ctx, const Batch &b, bool timers, bool overl
   if (overlap) {
     SL_HIP(ctx, hipEventRecord(ctx->geo_done, st));
     SL_HIP(ctx, hipStreamWaitEvent(ms, ctx->geo_done, 0));
+    if (lod_lit) lod_colour(ms);
     sl_splat_colour<<<b.total_slots / 256u, 256, 0, ms>>>(w.views, w.slot_base, V, w.keys, w.splats, w.partials);
     sl_frame_totals<<<V, 1024, 0, ms>>>(w.partials, w.slot_base, w.view_counters, w.counters, 2u);
     if (timers) SL_HIP(ctx, hipEventRecord(ctx->ev_comp, ms));
@@ -703,7 +750,8 @@ extern "C" void sl_ctx_destroy(sl_context *ctx) {
     for (Grow *gb : {&m.rec, &m.tbox, &m.tmask, &m.k0, &m.v0, &m.k1, &m.v1, &m.scratch})
       if (gb->p) (void)hipFree(gb->p);
   if (ctx->map_stats.p) (void)hipFree(ctx->map_stats.p);
-  if (ctx->disp_q.p) (void)hipFree(ctx->disp_q.p);
+  for (Grow *gb : {&ctx->disp_q, &ctx->lod_flag, &ctx->lod_open, &ctx->lod_cnt, &ctx->lod_col})
+    if (gb->p) (void)hipFree(gb->p);
   for (hipEvent_t e : {ctx->fork, ctx->join, ctx->cleared, ctx->sev[0], ctx->sev[1], ctx->pre_light, ctx->geo_done})
     if (e) (void)hipEventDestroy(e);
   if (ctx->side) (void)hipStreamDestroy(ctx->side);
@@ -1093,6 +1141,35 @@ sl_status encode_batch(sl_context *ctx, const sl_view *views, uint32_t n_views, 
     vd.e0 = sc->e[0]; vd.e1 = sc->e[1]; vd.e2 = sc->e[2]; vd.e3 = sc->e[3];
     vd.parts = sc->parts; vd.colour = sc->colour; vd.lo_b = sc->lo_b;
     vd.storage = sc->storage;
+    if (vd.f.n_nodes) {   // the cut lives in the context's LOD buffers: one view at a time, with its own workspace
+      if (n_views > 1 || fixed || (flags & SL_BATCH_GRAPH))
+        return fail(ctx, SL_ERR_UNSUPPORTED, "a screen-space LOD frame renders alone (sl_frame_encode)");
+      const uint32_t N = sc->lod.N, ni = N - sc->count;
+      if ((s = grow_ctx(ctx, &ctx->lod_flag, 4 * (size_t)N)) != SL_OK) return s;
+      if ((s = grow_ctx(ctx, &ctx->lod_open, ni)) != SL_OK) return s;
+      if ((s = grow_ctx(ctx, &ctx->lod_cnt, 4 * (size_t)ni)) != SL_OK) return s;
+      if ((s = grow_ctx(ctx, &ctx->lod_col, 16 * (size_t)ni)) != SL_OK) return s;
+      vd.lod_flag = (const uint32_t *)ctx->lod_flag.p;
+      vd.n0 = sc->lod.ng[0]; vd.n1 = sc->lod.ng[1]; vd.n2 = sc->lod.ng[2]; vd.n3 = sc->lod.ng[3];
+      vd.node_colour = (const float4 *)ctx->lod_col.p;
+      vd.node_slot0 = sc->count + 1u + sc->n_emit;
+      vd.n_nodes = ni;
+      b.lod = true;
+      LodDev &L = b.lod_dev;
+      L.na = sc->lod.na; L.nb = sc->lod.nb; L.nc = sc->lod.nc;
+      L.up = sc->lod.up; L.part = sc->lod.part; L.child = sc->lod.child; L.kids = sc->lod.kids;
+      L.mass = sc->lod.mass;
+      L.parts = sc->parts;
+      L.N = N;
+      L.G = sc->count;
+      memcpy(L.view, vd.f.view, sizeof(L.view));
+      L.threshold = vd.f.lod_threshold;
+      L.focal = vd.f.fx;
+      L.near_ = vd.f.near_;
+      L.emit_part = sc->lod.emit_part;
+      L.open_parts = 0u;
+      L.size_rule = sc->lod.size_rule;
+    }
     vd.out = in.out_device;
     vd.out_format = in.out_format;
     vd.slot_base = (uint32_t)slots;
@@ -1218,11 +1295,125 @@ extern "C" sl_status sl_frame_finish(sl_context *ctx, sl_frame_stats *stats) {
   return finish_batch(ctx, stats, 1);
 }
 
+// ---- the real-asset path: screen-space LOD tree and display fill ----------------------------------------------
+
+extern "C" sl_status sl_scene_set_lod(sl_context *ctx, sl_scene *sc, const sl_lod_tree *t) {
+#pragma clang fp contract(off)
+  SL_LIVE(ctx);
+  if (!sc || !t || !t->nodes || !t->kids || !t->node_gaussians) return fail(ctx, SL_ERR_INVALID, "null argument");
+  if (sc->lod.on) return fail(ctx, SL_ERR_INVALID, "the scene already holds its LOD tree");
+  const uint32_t G = sc->count, N = t->node_count;
+  if (t->node_stride != 96u) return fail(ctx, SL_ERR_LAYOUT, "nodes stride " + std::to_string(t->node_stride) +
+                                                                 ", renderer expects 96");
+  if (N <= G || N > (1u << 25)) return fail(ctx, SL_ERR_LAYOUT, "Bad nodes count " + std::to_string(N));
+  if (t->node_gaussian_count != N - G)
+    return fail(ctx, SL_ERR_LAYOUT, "node Gaussian records " + std::to_string(t->node_gaussian_count) +
+                                        " != internal nodes " + std::to_string(N - G));
+  if (t->size_rule > 1u) return fail(ctx, SL_ERR_INVALID, "size_rule is 0 (radius) or 1 (projected area)");
+  const uint8_t *nd = (const uint8_t *)t->nodes;
+  auto nw = [&](uint32_t n, int k) { return rd_u32(nd + 96 * (size_t)n, k); };
+  auto nfl = [&](uint32_t n, int k) { return rd_f32(nd + 96 * (size_t)n, k); };
+  const uint32_t ni = N - G, K = t->kid_count;
+  std::vector<uint32_t> up(N, 0xFFFFFFFFu), part(N);
+  std::vector<uint2> child(ni);
+  std::vector<float4> na(ni), nb(ni), nc(ni);
+  std::vector<float> mass(N);
+  for (uint32_t n = 0; n < N; n++) {
+    const uint32_t lo = nw(n, 19), hi = nw(n, 20), d = nw(n, 21);
+    part[n] = nw(n, 15);
+    if (d > 64u) return fail(ctx, SL_ERR_LAYOUT, "node " + std::to_string(n) + " deeper than 64");
+    if (n < G) {
+      if (hi != lo) return fail(ctx, SL_ERR_LAYOUT, "node " + std::to_string(n) + " < G has children: leaves first");
+    } else {
+      if (lo >= hi || hi > K) return fail(ctx, SL_ERR_LAYOUT, "node " + std::to_string(n) + " has bad child range");
+      child[n - G] = make_uint2(lo, hi);
+      for (uint32_t k = lo; k < hi; k++) {
+        const uint32_t c = t->kids[k];
+        if (c >= N || up[c] != 0xFFFFFFFFu || nw(c, 21) != d + 1u)
+          return fail(ctx, SL_ERR_LAYOUT, "kids[" + std::to_string(k) + "] is not a child one depth below node " +
+                                              std::to_string(n) + " with a single parent");
+        up[c] = n;
+      }
+      na[n - G] = make_float4(nfl(n, 0), nfl(n, 1), nfl(n, 2), nfl(n, 3));
+      nb[n - G] = make_float4(nfl(n, 4), nfl(n, 5), nfl(n, 6), nfl(n, 7));
+      nc[n - G] = make_float4(nfl(n, 8), nfl(n, 9), nfl(n, 10), rd_f32(nd + 96 * (size_t)n, 15));   // part bits
+    }
+    // node_mass (lod.wgsl:61-65)
+    const float ax = nfl(n, 4), ay = nfl(n, 5), az = nfl(n, 6), ox = nfl(n, 8), oy = nfl(n, 9), oz = nfl(n, 10);
+    const float det_inv = ax * (ay * az - oz * oz) - ox * (ox * az - oz * oy) + oy * (ox * oz - ay * oy);
+    mass[n] = nfl(n, 7) / sqrtf(std::max(det_inv, 1e-30f));
+  }
+  sl_status s;
+  void *p = nullptr;
+  auto put = [&](const void *src, size_t bytes, void **dst) -> sl_status {
+    sl_status r = scene_alloc(ctx, sc, std::max<size_t>(bytes, 16), dst);
+    if (r != SL_OK) return r;
+    SL_HIP(ctx, hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+    return SL_OK;
+  };
+  if ((s = put(na.data(), 16 * (size_t)ni, &p)) != SL_OK) return s;
+  sc->lod.na = (float4 *)p;
+  if ((s = put(nb.data(), 16 * (size_t)ni, &p)) != SL_OK) return s;
+  sc->lod.nb = (float4 *)p;
+  if ((s = put(nc.data(), 16 * (size_t)ni, &p)) != SL_OK) return s;
+  sc->lod.nc = (float4 *)p;
+  if ((s = put(child.data(), 8 * (size_t)ni, &p)) != SL_OK) return s;
+  sc->lod.child = (uint2 *)p;
+  if ((s = put(up.data(), 4 * (size_t)N, &p)) != SL_OK) return s;
+  sc->lod.up = (uint32_t *)p;
+  if ((s = put(part.data(), 4 * (size_t)N, &p)) != SL_OK) return s;
+  sc->lod.part = (uint32_t *)p;
+  if ((s = put(mass.data(), 4 * (size_t)N, &p)) != SL_OK) return s;
+  sc->lod.mass = (float *)p;
+  if ((s = put(t->kids, 4 * (size_t)K, &p)) != SL_OK) return s;
+  sc->lod.kids = (uint32_t *)p;
+  s = sc->storage == SL_STORAGE_F32 ? upload_soa(ctx, sc, t->node_gaussians, ni, sc->lod.ng)
+                                    : upload_packed(ctx, sc, t->node_gaussians, ni, sc->storage, sc->lod.ng);
+  if (s != SL_OK) return s;
+  sc->lod.N = N;
+  sc->lod.emit_part = t->emit_part;
+  sc->lod.size_rule = t->size_rule;
+  sc->lod.on = true;
+  return SL_OK;
+}
+
+extern "C" sl_status sl_scene_set_fill(sl_context *ctx, sl_scene *sc, const uint32_t *fill, size_t words) {
+  SL_LIVE(ctx);
+  if (!sc || !fill) return fail(ctx, SL_ERR_INVALID, "null argument");
+  const uint32_t G = sc->count;
+  if (words < (size_t)G + 1 || fill[0] != 0u) return fail(ctx, SL_ERR_LAYOUT, "fill offsets must start at 0");
+  for (uint32_t g = 0; g < G; g++)
+    if (fill[g + 1] < fill[g]) return fail(ctx, SL_ERR_LAYOUT, "fill offsets are not ascending");
+  const size_t pairs = fill[G];
+  if (words != (size_t)G + 1 + 2 * pairs)
+    return fail(ctx, SL_ERR_LAYOUT, "fill table has " + std::to_string(words) + " words, its offsets span " +
+                                        std::to_string(pairs) + " pairs");
+  std::vector<uint8_t> target(G, 0);
+  for (uint32_t g = 0; g < G; g++) target[g] = fill[g + 1] > fill[g];
+  for (size_t k = 0; k < pairs; k++) {
+    const uint32_t j = fill[G + 1 + 2 * k];
+    if (j >= G || target[j]) return fail(ctx, SL_ERR_LAYOUT, "fill source " + std::to_string(j) +
+                                                                 " is out of range or itself a target");
+  }
+  void *p = nullptr;
+  sl_status s = scene_alloc(ctx, sc, 4 * words, &p);
+  if (s != SL_OK) return s;
+  SL_HIP(ctx, hipMemcpy(p, fill, 4 * words, hipMemcpyHostToDevice));
+  sc->fill = (uint32_t *)p;
+  return SL_OK;
+}
+
+extern "C" sl_status sl_fill_encode(sl_context *ctx, sl_scene *sc) {
+  SL_LIVE(ctx);
+  if (!sc) return fail(ctx, SL_ERR_INVALID, "null argument");
+  if (!sc->fill) return SL_OK;   // assets without a table skip it (src/lighting.js:994-995)
+  sl_display_fill<<<(sc->count + 255u) / 256u, 256, 0, ctx->stream>>>(sc->fill, sc->count, sc->colour, sc->lo_b);
+  return check_launch(ctx, "sl_display_fill");
+}
+
 // ---- display: the HDR target to 8-bit display colour (shaders/display.wgsl) ------------------------------------
 
 namespace {
-sl_status grow_ctx(sl_context *ctx, Grow *gb, size_t bytes);
-
 // The background sky's coefficient table (DP_SH_ENTRIES float4, the layout sl_display reads): per band l and
 // order m >= 0 at e = l (l + 1) / 2 + m, {sh[l*l + l + m].rgb, f} and at 45 + e {sh[l*l + l - m].rgb, 0}, where f
 // is the recurrence's factor (display.wgsl:49-59): kf = (l - m)! / (l + m)! / (4 pi) built up in the shader's float
@@ -1376,6 +1567,16 @@ extern "C" sl_status sl_frame_read(sl_context *ctx, uint32_t which, void *host, 
     case SL_ART_PAIR_TILES: src = ctx->final_keys; n = (size_t)c.pairs * 4; break;
     case SL_ART_PAIR_SLOTS: src = ctx->final_vals; n = (size_t)c.pairs * 4; break;
     case SL_ART_RANGES: src = w.ranges; n = (size_t)b.total_tiles * 8; break;
+    case SL_ART_LOD_FLAGS:
+      if (!b.lod) return fail(ctx, SL_ERR_INVALID, "the last frame drew no LOD cut");
+      src = ctx->lod_flag.p;
+      n = (size_t)b.lod_dev.N * 4;
+      break;
+    case SL_ART_LOD_COLOUR:
+      if (!b.lod) return fail(ctx, SL_ERR_INVALID, "the last frame drew no LOD cut");
+      src = ctx->lod_col.p;
+      n = (size_t)(b.lod_dev.N - b.lod_dev.G) * 16;
+      break;
     default: return fail(ctx, SL_ERR_INVALID, "unknown artefact");
   }
   if (written) *written = n;

@@ -24,6 +24,7 @@ SL_FRAME_COUNTING_SORT = 4
 SL_DISPLAY_RGBA8, SL_DISPLAY_BGRA8 = 0, 1
 SL_DISPLAY_EXACT = 1
 ART_KEYS, ART_ORDER, ART_SPLATS, ART_PAIR_TILES, ART_PAIR_SLOTS, ART_RANGES = 1, 2, 3, 4, 5, 6
+ART_LOD_FLAGS, ART_LOD_COLOUR = 7, 8
 
 # Every symbol include/splatlight.h declares (checked by the CPU tests).
 EXPORTS = [
@@ -37,7 +38,7 @@ EXPORTS = [
     "sl_batch_encode", "sl_batch_finish", "sl_ctx_set_tuning", "sl_ctx_get_tuning",
     "sl_server_create", "sl_server_submit", "sl_server_poll", "sl_server_info", "sl_server_error",
     "sl_server_destroy", "sl_server_fail_next", "sl_bounce_init", "sl_bounce_encode", "sl_bounce_read",
-    "sl_display_encode",
+    "sl_display_encode", "sl_scene_set_lod", "sl_scene_set_fill", "sl_fill_encode",
 ]
 SL_BATCH_GRAPH = 2
 SCENE_COLOUR, SCENE_LAMP_T, SCENE_ENV_T, SCENE_ISO_NORM = 1, 2, 3, 4
@@ -68,6 +69,13 @@ class FrameOpts(C.Structure):
 class DisplayOpts(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("hdr_format", C.c_uint32), ("out_format", C.c_uint32),
                 ("flags", C.c_uint32)]
+
+
+class LodTree(C.Structure):
+    _fields_ = [("nodes", C.c_void_p), ("node_count", C.c_uint32), ("node_stride", C.c_uint32),
+                ("kids", C.c_void_p), ("kid_count", C.c_uint32),
+                ("node_gaussians", C.c_void_p), ("node_gaussian_count", C.c_uint32),
+                ("emit_part", C.c_uint32), ("size_rule", C.c_uint32)]
 
 
 class FrameStats(C.Structure):
@@ -153,6 +161,9 @@ def lib() -> C.CDLL:
         "sl_frame_encode_noalloc": (i32, [vp, vp, vp, sz, C.POINTER(FrameOpts), vp]),
         "sl_frame_finish": (i32, [vp, C.POINTER(FrameStats)]),
         "sl_display_encode": (i32, [vp, vp, sz, vp, sz, vp, sz, C.POINTER(DisplayOpts), vp, vp]),
+        "sl_scene_set_lod": (i32, [vp, vp, C.POINTER(LodTree)]),
+        "sl_scene_set_fill": (i32, [vp, vp, vp, sz]),
+        "sl_fill_encode": (i32, [vp, vp]),
         "sl_frame_read": (i32, [vp, u32, vp, sz, C.POINTER(sz)]),
         "sl_timers_read": (i32, [vp, C.POINTER(C.c_char_p), C.POINTER(C.c_float), u32, C.POINTER(u32)]),
         "sl_lighting_iso_init": (i32, [vp, vp]),
@@ -289,6 +300,26 @@ class Scene:
             return
         b = np.ascontiguousarray(lo_b, dtype=np.float32).reshape(-1, 12)
         self.ctx.check(self.ctx.L.sl_scene_write_bounce(self.ctx.h, self.h, _ptr(b), b.shape[0]))
+
+    def set_lod(self, nodes: np.ndarray, kids: np.ndarray, node_gaussians: np.ndarray,
+                emit_part: int = 0xFFFFFFFF, size_rule: int = 1):
+        """sl_scene_set_lod: the asset's tree (all nodes, leaves first; kids) and its internal nodes as Gaussian
+        records (lod.node_gaussians)."""
+        nd = np.ascontiguousarray(nodes)
+        kd = np.ascontiguousarray(kids, np.uint32)
+        ng = np.ascontiguousarray(node_gaussians)
+        t = LodTree(_ptr(nd), nd.shape[0], nd.nbytes // max(nd.shape[0], 1), _ptr(kd), kd.size, _ptr(ng),
+                    ng.nbytes // 64, emit_part, size_rule)
+        self.ctx.check(self.ctx.L.sl_scene_set_lod(self.ctx.h, self.h, C.byref(t)))
+
+    def set_fill(self, fill: np.ndarray):
+        """sl_scene_set_fill: the display fill table (offsets [count + 1], then (source, weight bits) pairs)."""
+        f = np.ascontiguousarray(fill, np.uint32)
+        self.ctx.check(self.ctx.L.sl_scene_set_fill(self.ctx.h, self.h, _ptr(f), f.size))
+
+    def fill(self):
+        """sl_fill_encode: the display fill on the lit colour and the bounce toward the eye."""
+        self.ctx.check(self.ctx.L.sl_fill_encode(self.ctx.h, self.h))
 
     @property
     def device_bytes(self) -> int:

@@ -126,12 +126,13 @@ def perspective(fovy, aspect, near, far) -> np.ndarray:
 
 def render_frame(*, view, width, height, fovy, near=0.02, far=50.0, count, view_mode=VIEW_LIT,
                  part_material=None, palette=None, bounce_gain=1.0, emitter_count=0, albedo_scale=1.0,
-                 lowpass=0.3, exposure=1.0, shutter=0.0, blur_coverage=True, tone=TONE_AGX) -> bytes:
+                 lowpass=0.3, exposure=1.0, shutter=0.0, blur_coverage=True, tone=TONE_AGX, lod=None) -> bytes:
     """Pack a RenderFrame (1200 B) the way writeFrame does (src/main.js:526-556, 470-475).
 
     shutter: the motion blur's shutter fraction (palette[3].z; moving parts blur over it, 0 = off); blur_coverage:
     the blurred splats' coverage cap (palette[3].w, ?blurcov); tone: the display pass's tone curve (tone.x, word
-    260, src/main.js:553). Explode, LOD, refraction, particles and the
+    260, src/main.js:553); lod: (threshold px, first node slot, internal nodes) for the screen-space LOD (words
+    180-182, src/main.js:548; None or threshold 0 = off). Explode, refraction, particles and the
     reflection split are left off (zero).
     """
     buf = np.zeros(FRAME_BYTES // 4, np.uint32)
@@ -155,6 +156,9 @@ def render_frame(*, view, width, height, fovy, near=0.02, far=50.0, count, view_
     buf[79] = view_mode
     f32[80:84] = [albedo_scale, 0.01, exposure, lowpass]
     buf[260] = tone
+    if lod is not None and lod[0] > 0:
+        f32[180] = lod[0]
+        buf[181], buf[182] = int(lod[1]), int(lod[2])
     return buf.tobytes()
 
 

@@ -15,16 +15,18 @@ class SplatRenderer:
     """src/renderer.js:34 SplatRenderer(device, canvasFormat, shaders, asset, lodNodes, particleCount).
 
     `device` is a native.Context; `asset` a dict with "gaussians" (64 B records), "parts" (128 B records) and
-    optionally "emitters", "materials", "nodes", "vgrid". LOD nodes and particles are not on this path, so a
-    non-empty lodNodes or particleCount raises.
+    optionally "emitters", "materials", "nodes", "vgrid", "kids" and "emitPart". lodNodes: lod.node_gaussians()'s
+    result; with it the scene holds the tree (asset "nodes", all of them, and "kids") and frames whose
+    RenderFrame.lod is set draw its cut, the internal nodes in the slots from nodeBase on (src/renderer.js:42-46).
+    Particles are not on this path, so a non-zero particleCount raises.
     """
 
     PASSES = ["clear", "project", "radix", "pairs", "tilesort", "ranges", "splat"]
 
     def __init__(self, device: N.Context, canvasFormat=None, shaders=None, asset=None, lodNodes=None,
                  particleCount: int = 0, storage: int = N.SL_STORAGE_F32):
-        if lodNodes is not None or particleCount:
-            raise N.SplatlightError(5, "LOD nodes and particles are not on the splat frame path")
+        if particleCount:
+            raise N.SplatlightError(5, "particles are not on the splat frame path")
         self.device = device
         g = asset["gaussians"]
         if g.dtype != R.GAUSSIAN_DTYPE:
@@ -34,9 +36,16 @@ class SplatRenderer:
             p = np.frombuffer(np.ascontiguousarray(p).tobytes(), R.PART_DTYPE)
         self.count = g.shape[0]
         self.partCount = p.shape[0]
+        nodes = asset.get("nodes")
+        leaf_nodes = None if nodes is None else np.asarray(nodes).view(R.NODE_DTYPE).reshape(-1)[:self.count]
         self.scene = N.Scene(device, g, p, emitters=asset.get("emitters"), materials=asset.get("materials"),
-                             nodes=asset.get("nodes"), vgrid=asset.get("vgrid"), storage=storage)
-        self.splatCount = self.count + 1 + self.scene.n_emit
+                             nodes=leaf_nodes, vgrid=asset.get("vgrid"), storage=storage)
+        self.nodeBase = self.count + 1 + self.scene.n_emit
+        self.lodNodeCount = 0
+        if lodNodes is not None:
+            self.scene.set_lod(nodes, asset["kids"], lodNodes["records"], emit_part=asset.get("emitPart", 0xFFFFFFFF))
+            self.lodNodeCount = lodNodes["N"] - lodNodes["G"]
+        self.splatCount = self.nodeBase + self.lodNodeCount
         self.hdr = None
         self.width = self.height = 0
         self.hdrFormat = N.SL_OUT_RGBA16F if canvasFormat in (None, "rgba16float") else N.SL_OUT_RGBA32F
