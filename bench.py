@@ -596,10 +596,18 @@ def main():
     ctx.close()
     if a.bounce:
         res["bounce"] = bounce_leg(N, L, scenes, sc)
+    # batched and served throughput against the plain per-frame path at this workload (one 1080p view over 1M
+    # Gaussians, view only: the batches and requests are unlit), beside the same-views ratios each leg measures
+    plain_1080p = W * H / view_only / 1e6
     if a.batched:
         res["batched"] = batched_leg(N, L, scenes)
+        res["batched"]["plain_1080p_1m_mpix_s"] = round(plain_1080p, 1)
+        res["batched"]["speedup_vs_plain_1080p_1m"] = round(res["batched"]["batched_mpix_s"] / plain_1080p, 3)
     if a.serving:
         res["serving"] = serving_leg(N, scenes, rate=a.rate)
+        res["serving"]["plain_1080p_1m_mpix_s"] = round(plain_1080p, 1)
+        res["serving"]["saturated_speedup_vs_plain_1080p_1m"] = round(
+            res["serving"]["saturated_served_mpix_s"] / plain_1080p, 3)
     if a.storage:
         res["storage_4k"] = storage_leg(N, L, scenes)
     if a.cpu_baseline:

@@ -127,7 +127,8 @@ def test_rgba16f_target_against_fp16_emulation(ctx):
     _, _, o_order, _ = oracle.sort_keys(o_keys)
     ref16 = oracle.composite(fb, o_splats, o_order, o_vis, sc.width, sc.height, mode=2)
     log(f"  rgba16f vs reference fp16-after-every-blend target: max|err| = {np.abs(img16 - ref16).max():.3g}")
-    assert np.abs(img16 - ref16).max() < 0.05 * max(1.0, float(np.abs(ref16).max()))
+    # the reference rounds to half after every blend, this build once at the end: within 1/128 of the peak
+    assert np.abs(img16 - ref16).max() <= max(1.0, float(np.abs(ref16).max())) / 128.0
 
 
 @pytest.mark.parametrize("view_mode", [0, 1, 2])
