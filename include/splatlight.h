@@ -76,6 +76,14 @@ sl_status sl_scene_write_parts(sl_context *ctx, sl_scene *scene, const void *par
 sl_status sl_scene_write_colour(sl_context *ctx, sl_scene *scene, const float *rgba, size_t count);
 /* the bounce per Gaussian (3 vec4 each, lo_b); NULL clears it (bounce gain then adds nothing) */
 sl_status sl_scene_write_bounce(sl_context *ctx, sl_scene *scene, const float *lo_b, size_t count);
+/* The Gaussian records rewritten in place, count * 64 B (the Lighting constructor's re-upload after pointing the box
+ * parts' Gaussians at their own materials, src/lighting.js:467). Waits for the context's streams first; iso_norm and
+ * the bounce's leaf weights are formed again at the next encode that reads them. */
+sl_status sl_scene_write_gaussians(sl_context *ctx, sl_scene *scene, const void *gaussian_bytes, size_t bytes);
+/* The material table rewritten, 1 to SL_MAX_MATERIALS records of 32 B (Lighting's writes of its materials buffer:
+ * setBoxMaterial, setMaterialShine, setRoughness, src/lighting.js:1152-1227). Waits for the context's streams. */
+#define SL_MAX_MATERIALS 256u
+sl_status sl_scene_write_materials(sl_context *ctx, sl_scene *scene, const void *material_bytes, size_t bytes);
 size_t sl_scene_device_bytes(const sl_scene *scene);
 
 /* ---- the real-asset path: screen-space LOD and display fill ------------------------------------------------

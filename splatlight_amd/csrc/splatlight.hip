@@ -90,6 +90,7 @@ struct sl_scene {
   float4 *lo_b = nullptr;
   std::vector<uint8_t> parts_host;
   std::vector<uint8_t> materials_host;
+  size_t materials_cap = 0;         // bytes of the device materials buffer
   std::vector<void *> allocs;
   size_t bytes = 0;
   // relighting state
@@ -865,43 +866,36 @@ uint16_t to_bf16_bits(float f) {
 // F16 / BF16 scenes: g0 = (mu, kappa) f32; g1 = {quat halves, albedo halves}; g2 = {scale halves, part_surface,
 // flags}; packed on the host from the 64 B records (words: mu 0-2, kappa 3, quat 4-7, scale 8-10, part_surface 11,
 // albedo 12-14, flags 15).
-sl_status upload_packed(sl_context *ctx, sl_scene *sc, const void *aos, uint32_t n, uint32_t storage, float4 *dst[4]) {
+// The packed streams' host words of n records (the layout above).
+void pack_streams(const void *aos, uint32_t n, uint32_t storage, std::vector<float4> &g0, std::vector<uint4> &h1,
+                  std::vector<uint4> &h2) {
   auto cv = [&](const uint8_t *r, int word) -> uint32_t {
     const float f = rd_f32(r, word);
     return storage == SL_STORAGE_BF16 ? to_bf16_bits(f) : to_half_bits(f);
   };
-  std::vector<float4> g0(n);
-  std::vector<uint4> h1(n), h2(n);
+  g0.resize(n);
+  h1.resize(n);
+  h2.resize(n);
   for (uint32_t i = 0; i < n; i++) {
     const uint8_t *r = (const uint8_t *)aos + 64 * (size_t)i;
     g0[i] = make_float4(rd_f32(r, 0), rd_f32(r, 1), rd_f32(r, 2), rd_f32(r, 3));
     h1[i] = make_uint4(cv(r, 4) | cv(r, 5) << 16, cv(r, 6) | cv(r, 7) << 16, cv(r, 12) | cv(r, 13) << 16, cv(r, 14));
     h2[i] = make_uint4(cv(r, 8) | cv(r, 9) << 16, cv(r, 10), rd_u32(r, 11), rd_u32(r, 15));
   }
-  const void *src[3] = {g0.data(), h1.data(), h2.data()};
-  for (int k = 0; k < 3; k++) {
-    void *p = nullptr;
-    sl_status s = dev_alloc(ctx, (size_t)std::max(n, 1u) * 16, &p);
-    if (s != SL_OK) return s;
-    sc->allocs.push_back(p);
-    sc->bytes += (size_t)std::max(n, 1u) * 16;
-    dst[k] = (float4 *)p;
-    if (n) SL_HIP(ctx, hipMemcpy(p, src[k], (size_t)n * 16, hipMemcpyHostToDevice));
-  }
-  dst[3] = nullptr;
-  return SL_OK;
 }
 
-sl_status upload_soa(sl_context *ctx, sl_scene *sc, const void *aos, uint32_t n, float4 *dst[4]) {
-  for (int k = 0; k < 4; k++) {
-    void *p = nullptr;
-    sl_status s = dev_alloc(ctx, (size_t)std::max(n, 1u) * 16, &p);
-    if (s != SL_OK) return s;
-    sc->allocs.push_back(p);
-    sc->bytes += (size_t)std::max(n, 1u) * 16;
-    dst[k] = (float4 *)p;
-  }
+// n 64 B records into existing streams: the three packed ones, or the four float4 ones through a staging copy and
+// sl_aos_to_soa. Synchronous.
+sl_status fill_streams(sl_context *ctx, const void *aos, uint32_t n, uint32_t storage, float4 *const dst[4]) {
   if (n == 0) return SL_OK;
+  if (storage != SL_STORAGE_F32) {
+    std::vector<float4> g0;
+    std::vector<uint4> h1, h2;
+    pack_streams(aos, n, storage, g0, h1, h2);
+    const void *src[3] = {g0.data(), h1.data(), h2.data()};
+    for (int k = 0; k < 3; k++) SL_HIP(ctx, hipMemcpy(dst[k], src[k], (size_t)n * 16, hipMemcpyHostToDevice));
+    return SL_OK;
+  }
   void *staging = nullptr;
   sl_status s = dev_alloc(ctx, (size_t)n * 64, &staging);
   if (s != SL_OK) return s;
@@ -915,6 +909,22 @@ sl_status upload_soa(sl_context *ctx, sl_scene *sc, const void *aos, uint32_t n,
   (void)hipFree(staging);
   if (e != hipSuccess) return hip_fail(ctx, e, "scene upload");
   return SL_OK;
+}
+
+// Allocates the record streams of n records (three for F16 / BF16 storage, four for F32) and fills them.
+sl_status upload_streams(sl_context *ctx, sl_scene *sc, const void *aos, uint32_t n, uint32_t storage, float4 *dst[4]) {
+  const int k_n = storage == SL_STORAGE_F32 ? 4 : 3;
+  for (int k = 0; k < 4; k++) {
+    dst[k] = nullptr;
+    if (k >= k_n) continue;
+    void *p = nullptr;
+    sl_status s = dev_alloc(ctx, (size_t)std::max(n, 1u) * 16, &p);
+    if (s != SL_OK) return s;
+    sc->allocs.push_back(p);
+    sc->bytes += (size_t)std::max(n, 1u) * 16;
+    dst[k] = (float4 *)p;
+  }
+  return fill_streams(ctx, aos, n, storage, dst);
 }
 
 sl_status scene_alloc(sl_context *ctx, sl_scene *sc, size_t bytes, void **p) {
@@ -990,9 +1000,8 @@ extern "C" sl_status sl_scene_create(sl_context *ctx, const sl_scene_desc *d, sl
   sc->n_emit = d->emitter_count;
   sc->material_count = d->materials ? d->material_count : 0;
   sc->storage = d->storage;
-  sl_status s = sc->storage == SL_STORAGE_F32 ? upload_soa(ctx, sc, d->gaussians, sc->count, sc->g)
-                                              : upload_packed(ctx, sc, d->gaussians, sc->count, sc->storage, sc->g);
-  if (s == SL_OK) s = upload_soa(ctx, sc, d->emitters, sc->n_emit, sc->e);
+  sl_status s = upload_streams(ctx, sc, d->gaussians, sc->count, sc->storage, sc->g);
+  if (s == SL_OK) s = upload_streams(ctx, sc, d->emitters, sc->n_emit, SL_STORAGE_F32, sc->e);
   void *p = nullptr;
   if (s == SL_OK && (s = scene_alloc(ctx, sc, SL_MAX_PARTS * 128, &p)) == SL_OK) sc->parts = (float4 *)p;
   if (s == SL_OK && (s = scene_alloc(ctx, sc, (size_t)sc->count * 16, &p)) == SL_OK) sc->colour = (float4 *)p;
@@ -1033,6 +1042,7 @@ extern "C" sl_status sl_scene_create(sl_context *ctx, const sl_scene_desc *d, sl
     sc->materials_host.assign((const uint8_t *)d->materials, (const uint8_t *)d->materials + 32 * (size_t)d->material_count);
     if ((s = scene_alloc(ctx, sc, 32 * (size_t)d->material_count, &p)) == SL_OK) {
       sc->materials = (uint32_t *)p;
+      sc->materials_cap = 32 * (size_t)d->material_count;
       hipError_t e = hipMemcpyAsync(sc->materials, sc->materials_host.data(), sc->materials_host.size(),
                                     hipMemcpyHostToDevice, ctx->stream);
       if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -1098,6 +1108,64 @@ extern "C" sl_status sl_scene_write_bounce(sl_context *ctx, sl_scene *sc, const 
   SL_HIP(ctx, hipMemcpyAsync(p, lo_b, count * 48, hipMemcpyHostToDevice, ctx->stream));
   SL_HIP(ctx, hipStreamSynchronize(ctx->stream));
   sc->lo_b = (float4 *)p;
+  return SL_OK;
+}
+
+namespace {
+// Waits for every stream of the context, so a rewrite of a scene buffer never races a kernel still reading it.
+sl_status quiesce(sl_context *ctx) {
+  SL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (ctx->side) SL_HIP(ctx, hipStreamSynchronize(ctx->side));
+  if (ctx->geo) SL_HIP(ctx, hipStreamSynchronize(ctx->geo));
+  return SL_OK;
+}
+}  // namespace
+
+extern "C" sl_status sl_scene_write_gaussians(sl_context *ctx, sl_scene *sc, const void *bytes, size_t n) {
+  SL_LIVE(ctx);
+  if (!sc || !bytes) return SL_ERR_INVALID;
+  if (n != (size_t)sc->count * 64)
+    return fail(ctx, SL_ERR_LAYOUT, "gaussians are " + std::to_string(n) + " B, expected " +
+                                        std::to_string(sc->count) + " x 64 B");
+  for (uint32_t i = 0; i < sc->count; i++) {
+    const uint32_t part = rd_u32((const uint8_t *)bytes + 64 * (size_t)i, 11) & 0xffffu;
+    if (part >= sc->part_count)
+      return fail(ctx, SL_ERR_LAYOUT, "Gaussian " + std::to_string(i) + " names part " + std::to_string(part) +
+                                          " of " + std::to_string(sc->part_count));
+  }
+  SL_HIP(ctx, hipSetDevice(ctx->device));
+  sl_status s = quiesce(ctx);
+  if (s == SL_OK) s = fill_streams(ctx, bytes, sc->count, sc->storage, sc->g);
+  if (s != SL_OK) return s;
+  if (sc->order) {
+    const std::vector<uint32_t> ord = spatial_order(bytes, sc->count);
+    SL_HIP(ctx, hipMemcpy(sc->order, ord.data(), (size_t)sc->count * 4, hipMemcpyHostToDevice));
+  }
+  // what was formed from the records is formed again: iso_norm at the next lighting encode, the bounce's leaf
+  // weights at the next bounce encode (as after a part-table write)
+  sc->iso_done = false;
+  sc->parts_version++;
+  return SL_OK;
+}
+
+extern "C" sl_status sl_scene_write_materials(sl_context *ctx, sl_scene *sc, const void *bytes, size_t n) {
+  SL_LIVE(ctx);
+  if (!sc || !bytes) return SL_ERR_INVALID;
+  if (n == 0 || n % 32 != 0 || n / 32 > SL_MAX_MATERIALS)
+    return fail(ctx, SL_ERR_LAYOUT, "materials are " + std::to_string(n) + " B, expected 1 to " +
+                                        std::to_string(SL_MAX_MATERIALS) + " x 32 B");
+  SL_HIP(ctx, hipSetDevice(ctx->device));
+  sl_status s = quiesce(ctx);
+  if (s != SL_OK) return s;
+  if (!sc->materials || n > sc->materials_cap) {
+    void *p = nullptr;
+    if ((s = scene_alloc(ctx, sc, n, &p)) != SL_OK) return s;
+    sc->materials = (uint32_t *)p;
+    sc->materials_cap = n;
+  }
+  sc->materials_host.assign((const uint8_t *)bytes, (const uint8_t *)bytes + n);
+  sc->material_count = (uint32_t)(n / 32);
+  SL_HIP(ctx, hipMemcpy(sc->materials, sc->materials_host.data(), n, hipMemcpyHostToDevice));
   return SL_OK;
 }
 
@@ -1367,8 +1435,7 @@ extern "C" sl_status sl_scene_set_lod(sl_context *ctx, sl_scene *sc, const sl_lo
   sc->lod.mass = (float *)p;
   if ((s = put(t->kids, 4 * (size_t)K, &p)) != SL_OK) return s;
   sc->lod.kids = (uint32_t *)p;
-  s = sc->storage == SL_STORAGE_F32 ? upload_soa(ctx, sc, t->node_gaussians, ni, sc->lod.ng)
-                                    : upload_packed(ctx, sc, t->node_gaussians, ni, sc->storage, sc->lod.ng);
+  s = upload_streams(ctx, sc, t->node_gaussians, ni, sc->storage, sc->lod.ng);
   if (s != SL_OK) return s;
   sc->lod.N = N;
   sc->lod.emit_part = t->emit_part;

@@ -32,7 +32,7 @@ EXPORTS = [
     "sl_ctx_create", "sl_ctx_destroy", "sl_last_error", "sl_ctx_status", "sl_sync",
     "sl_buf_alloc", "sl_buf_free", "sl_buf_write", "sl_buf_read",
     "sl_scene_create", "sl_scene_destroy", "sl_scene_write_parts", "sl_scene_write_colour",
-    "sl_scene_write_bounce", "sl_scene_device_bytes",
+    "sl_scene_write_bounce", "sl_scene_device_bytes", "sl_scene_write_gaussians", "sl_scene_write_materials",
     "sl_frame_encode", "sl_frame_finish", "sl_frame_read", "sl_timers_read", "sl_frame_encode_noalloc",
     "sl_lighting_iso_init", "sl_lighting_encode", "sl_scene_read", "sl_lighting_stats_read",
     "sl_batch_encode", "sl_batch_finish", "sl_ctx_set_tuning", "sl_ctx_get_tuning",
@@ -156,6 +156,8 @@ def lib() -> C.CDLL:
         "sl_scene_write_parts": (i32, [vp, vp, vp, sz]),
         "sl_scene_write_colour": (i32, [vp, vp, vp, sz]),
         "sl_scene_write_bounce": (i32, [vp, vp, vp, sz]),
+        "sl_scene_write_gaussians": (i32, [vp, vp, vp, sz]),
+        "sl_scene_write_materials": (i32, [vp, vp, vp, sz]),
         "sl_scene_device_bytes": (sz, [vp]),
         "sl_frame_encode": (i32, [vp, vp, vp, sz, C.POINTER(FrameOpts), vp]),
         "sl_frame_encode_noalloc": (i32, [vp, vp, vp, sz, C.POINTER(FrameOpts), vp]),
@@ -293,6 +295,16 @@ class Scene:
     def write_colour(self, rgba: np.ndarray):
         c = np.ascontiguousarray(rgba, dtype=np.float32).reshape(-1, 4)
         self.ctx.check(self.ctx.L.sl_scene_write_colour(self.ctx.h, self.h, _ptr(c), c.shape[0]))
+
+    def write_gaussians(self, gaussians: np.ndarray):
+        """sl_scene_write_gaussians: the records rewritten in place (same count)."""
+        g = np.ascontiguousarray(gaussians)
+        self.ctx.check(self.ctx.L.sl_scene_write_gaussians(self.ctx.h, self.h, _ptr(g), g.nbytes))
+
+    def write_materials(self, materials: np.ndarray):
+        """sl_scene_write_materials: the material table rewritten (1 to 256 records of 32 B)."""
+        m = np.ascontiguousarray(materials)
+        self.ctx.check(self.ctx.L.sl_scene_write_materials(self.ctx.h, self.h, _ptr(m), m.nbytes))
 
     def write_bounce(self, lo_b: np.ndarray | None):
         if lo_b is None:

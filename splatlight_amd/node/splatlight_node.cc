@@ -308,6 +308,144 @@ napi_value SceneWriteColour(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
+// sceneWriteGaussians(scene, bytes) / sceneWriteMaterials(scene, bytes): the Lighting constructor's re-upload of the
+// records and its material table (src/lighting.js:467, 1152-1227)
+napi_value SceneWriteRecords(napi_env env, napi_callback_info info, bool materials) {
+  size_t argc = 2;
+  napi_value a[2];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, a, nullptr, nullptr));
+  Scene *s = unwrap<Scene>(env, a[0]);
+  if (!s) return nullptr;
+  const void *p;
+  size_t n;
+  if (!bytes_of(env, a[1], &p, &n) || !p) {
+    napi_throw_type_error(env, nullptr, "expected an ArrayBuffer or typed array of records");
+    return nullptr;
+  }
+  auto lk = quiet(s->owner);
+  sl_status rc = materials ? sl_scene_write_materials(s->ctx, s->scene, p, n) : sl_scene_write_gaussians(s->ctx, s->scene, p, n);
+  if (rc) return throw_sl(env, s->ctx, rc, materials ? "writeMaterials" : "writeGaussians");
+  return nullptr;
+}
+napi_value SceneWriteGaussians(napi_env env, napi_callback_info info) { return SceneWriteRecords(env, info, false); }
+napi_value SceneWriteMaterials(napi_env env, napi_callback_info info) { return SceneWriteRecords(env, info, true); }
+
+// sceneWriteBounce(scene, Float32Array of 3 vec4 per Gaussian, or null to clear)
+napi_value SceneWriteBounce(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value a[2];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, a, nullptr, nullptr));
+  Scene *s = unwrap<Scene>(env, a[0]);
+  if (!s) return nullptr;
+  const void *p = nullptr;
+  size_t n = 0;
+  if (argc > 1 && !bytes_of(env, a[1], &p, &n)) return nullptr;
+  auto lk = quiet(s->owner);
+  if (sl_status rc = sl_scene_write_bounce(s->ctx, s->scene, (const float *)p, n / 48))
+    return throw_sl(env, s->ctx, rc, "bindColour");
+  return nullptr;
+}
+
+// sceneSetLod(scene, {nodes, kids, nodeGaussians, emitPart, sizeRule}): Lod's constructor (src/lod.js:114-140)
+napi_value SceneSetLod(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value a[2];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, a, nullptr, nullptr));
+  Scene *s = unwrap<Scene>(env, a[0]);
+  if (!s) return nullptr;
+  const void *nodes = nullptr, *kids = nullptr, *ng = nullptr;
+  size_t nn = 0, nk = 0, nng = 0;
+  if (!bytes_of(env, prop(env, a[1], "nodes"), &nodes, &nn) || !bytes_of(env, prop(env, a[1], "kids"), &kids, &nk) ||
+      !bytes_of(env, prop(env, a[1], "nodeGaussians"), &ng, &nng)) {
+    napi_throw_type_error(env, nullptr, "sceneSetLod: nodes, kids and nodeGaussians must be ArrayBuffers or typed arrays");
+    return nullptr;
+  }
+  sl_lod_tree t;
+  t.nodes = nodes;
+  t.node_count = (uint32_t)(nn / 96);
+  t.node_stride = 96;
+  t.kids = (const uint32_t *)kids;
+  t.kid_count = (uint32_t)(nk / 4);
+  t.node_gaussians = ng;
+  t.node_gaussian_count = (uint32_t)(nng / 64);
+  t.emit_part = u32_arg(env, prop(env, a[1], "emitPart"), 0xFFFFFFFFu);
+  t.size_rule = u32_arg(env, prop(env, a[1], "sizeRule"), 1u);
+  auto lk = quiet(s->owner);
+  if (sl_status rc = sl_scene_set_lod(s->ctx, s->scene, &t)) return throw_sl(env, s->ctx, rc, "Lod");
+  return nullptr;
+}
+
+// sceneSetFill(scene, Uint32Array table) / fillEncode(scene): Lighting.loadFill / encodeFill
+// (src/lighting.js:961-1000)
+napi_value SceneSetFill(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value a[2];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, a, nullptr, nullptr));
+  Scene *s = unwrap<Scene>(env, a[0]);
+  if (!s) return nullptr;
+  const void *p = nullptr;
+  size_t n = 0;
+  if (!bytes_of(env, a[1], &p, &n) || !p) {
+    napi_throw_type_error(env, nullptr, "sceneSetFill: the table must be an ArrayBuffer or typed array");
+    return nullptr;
+  }
+  auto lk = quiet(s->owner);
+  if (sl_status rc = sl_scene_set_fill(s->ctx, s->scene, (const uint32_t *)p, n / 4))
+    return throw_sl(env, s->ctx, rc, "loadFill");
+  return nullptr;
+}
+
+napi_value FillEncode(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value a[1];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, a, nullptr, nullptr));
+  Scene *s = unwrap<Scene>(env, a[0]);
+  if (!s) return nullptr;
+  auto lk = quiet(s->owner);
+  if (sl_status rc = sl_fill_encode(s->ctx, s->scene)) return throw_sl(env, s->ctx, rc, "encodeFill");
+  return nullptr;
+}
+
+// displayEncode(ctx, frameBytes, env, skyBg, {width, height, hdrFormat, outFormat, flags}, hdrBuf, outBuf): the display
+// pass of SplatRenderer.encode (shaders/display.wgsl, src/renderer.js:475-485)
+napi_value DisplayEncode(napi_env env, napi_callback_info info) {
+  size_t argc = 7;
+  napi_value a[7];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, a, nullptr, nullptr));
+  Ctx *c = unwrap<Ctx>(env, a[0]);
+  Buf *hdr = c ? unwrap<Buf>(env, a[5]) : nullptr;
+  Buf *out = hdr ? unwrap<Buf>(env, a[6]) : nullptr;
+  if (!c || !hdr || !out) return nullptr;
+  const void *fb = nullptr, *ev = nullptr, *bg = nullptr;
+  size_t nfb = 0, nev = 0, nbg = 0;
+  if (!bytes_of(env, a[1], &fb, &nfb) || !bytes_of(env, a[2], &ev, &nev) || !bytes_of(env, a[3], &bg, &nbg)) {
+    napi_throw_type_error(env, nullptr, "displayEncode: byte arguments must be ArrayBuffers or typed arrays");
+    return nullptr;
+  }
+  sl_display_opts o;
+  o.width = u32_arg(env, prop(env, a[4], "width"));
+  o.height = u32_arg(env, prop(env, a[4], "height"));
+  o.hdr_format = u32_arg(env, prop(env, a[4], "hdrFormat"), SL_OUT_RGBA16F);
+  o.out_format = u32_arg(env, prop(env, a[4], "outFormat"), SL_DISPLAY_RGBA8);
+  o.flags = u32_arg(env, prop(env, a[4], "flags"));
+  auto lk = quiet(c);
+  if (sl_status rc = sl_display_encode(c->ctx, fb, nfb, ev, nev, bg, nbg, &o, hdr->dev, out->dev))
+    return throw_sl(env, c->ctx, rc, "display");
+  return nullptr;
+}
+
+// sync(ctx): onSubmittedWorkDone, synchronously
+napi_value Sync(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value a[1];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, a, nullptr, nullptr));
+  Ctx *c = unwrap<Ctx>(env, a[0]);
+  if (!c) return nullptr;
+  auto lk = quiet(c);
+  if (sl_status rc = sl_sync(c->ctx)) return throw_sl(env, c->ctx, rc, "sync");
+  return nullptr;
+}
+
 napi_value LightingIsoInit(napi_env env, napi_callback_info info) {
   size_t argc = 1;
   napi_value a[1];
@@ -430,8 +568,11 @@ napi_value FrameEncode(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
-// frameFinish(ctx) -> Promise<{slots, visible, pairs, overflow, tiles}> of the frame encoded last, waited for on the
-// libuv pool (which holds the context's lock meanwhile; the next call into the context waits for it)
+// frameFinish(ctx, display?) -> Promise<{slots, visible, pairs, overflow, tiles}> of the frame encoded last, waited for
+// on the libuv pool (which holds the context's lock meanwhile; the next call into the context waits for it). display:
+// {frameBytes, env, skyBg, width, height, hdrFormat, outFormat, flags, hdr, out} -- the display pass into out, enqueued
+// on the pool right after the finish (so after an overflow re-run, and before any later frame's encode can rewrite
+// the HDR target); its inputs are copied at the call.
 struct FinishWork {
   napi_async_work work = nullptr;
   napi_deferred deferred = nullptr;
@@ -440,19 +581,60 @@ struct FinishWork {
   sl_status status = SL_OK;
   sl_frame_stats stats{};
   std::string err;
+  bool display = false;
+  std::string frame, env, sky_bg;        // copies of the display's byte inputs
+  bool has_env = false, has_bg = false;
+  sl_display_opts opts{};
+  const void *hdr = nullptr;
+  void *out = nullptr;
+  napi_ref hdr_ref = nullptr, out_ref = nullptr;   // keep both buffers alive until the work completes
 };
 
+bool copy_bytes(napi_env env, napi_value v, std::string *dst, bool *present) {
+  const void *p = nullptr;
+  size_t n = 0;
+  if (!bytes_of(env, v, &p, &n)) return false;
+  if (present) *present = p != nullptr;
+  if (p) dst->assign((const char *)p, n);
+  return true;
+}
+
 napi_value FrameFinish(napi_env env, napi_callback_info info) {
-  size_t argc = 1;
-  napi_value a[1];
+  size_t argc = 2;
+  napi_value a[2];
   NAPI_OK(napi_get_cb_info(env, info, &argc, a, nullptr, nullptr));
   Ctx *c = unwrap<Ctx>(env, a[0]);
   if (!c) return nullptr;
+  FinishWork *w = new FinishWork();
+  napi_valuetype t = napi_undefined;
+  if (argc > 1) napi_typeof(env, a[1], &t);
+  if (t == napi_object) {
+    napi_value hv = prop(env, a[1], "hdr"), ov = prop(env, a[1], "out");
+    Buf *hdr = unwrap<Buf>(env, hv);
+    Buf *out = hdr ? unwrap<Buf>(env, ov) : nullptr;
+    bool ok = out && copy_bytes(env, prop(env, a[1], "frameBytes"), &w->frame, nullptr) &&
+              copy_bytes(env, prop(env, a[1], "env"), &w->env, &w->has_env) &&
+              copy_bytes(env, prop(env, a[1], "skyBg"), &w->sky_bg, &w->has_bg);
+    if (!ok) {
+      delete w;
+      if (hdr && out) napi_throw_type_error(env, nullptr, "frameFinish: display byte arguments must be ArrayBuffers or typed arrays");
+      return nullptr;
+    }
+    w->display = true;
+    w->opts.width = u32_arg(env, prop(env, a[1], "width"));
+    w->opts.height = u32_arg(env, prop(env, a[1], "height"));
+    w->opts.hdr_format = u32_arg(env, prop(env, a[1], "hdrFormat"), SL_OUT_RGBA16F);
+    w->opts.out_format = u32_arg(env, prop(env, a[1], "outFormat"), SL_DISPLAY_RGBA8);
+    w->opts.flags = u32_arg(env, prop(env, a[1], "flags"));
+    w->hdr = hdr->dev;
+    w->out = out->dev;
+    NAPI_OK(napi_create_reference(env, hv, 1, &w->hdr_ref));
+    NAPI_OK(napi_create_reference(env, ov, 1, &w->out_ref));
+  }
   {
     auto lk = quiet(c);    // one finish at a time: each reports the frame encoded before it
     c->finishing++;
   }
-  FinishWork *w = new FinishWork();
   w->ctx = c->ctx;
   w->owner = c;
   c->refs++;
@@ -466,6 +648,10 @@ napi_value FrameFinish(napi_env env, napi_callback_info info) {
         {
           std::lock_guard<std::mutex> lk(w->owner->mu);
           w->status = sl_frame_finish(w->ctx, &w->stats);
+          if (w->status == SL_OK && w->display)
+            w->status = sl_display_encode(w->ctx, w->frame.data(), w->frame.size(), w->has_env ? w->env.data() : nullptr,
+                                          w->env.size(), w->has_bg ? w->sky_bg.data() : nullptr, w->sky_bg.size(),
+                                          &w->opts, w->hdr, w->out);
           if (w->status) w->err = sl_last_error(w->ctx);
           w->owner->finishing--;
         }
@@ -488,6 +674,8 @@ napi_value FrameFinish(napi_env env, napi_callback_info info) {
           napi_create_error(env, nullptr, msg, &e);
           napi_reject_deferred(env, w->deferred, e);
         }
+        if (w->hdr_ref) napi_delete_reference(env, w->hdr_ref);
+        if (w->out_ref) napi_delete_reference(env, w->out_ref);
         ctx_release(w->owner);
         napi_delete_async_work(env, w->work);
         delete w;
@@ -533,6 +721,14 @@ NAPI_MODULE_INIT() {
       {"sceneCreate", nullptr, SceneCreate, nullptr, nullptr, nullptr, kMethod, nullptr},
       {"sceneWriteParts", nullptr, SceneWriteParts, nullptr, nullptr, nullptr, kMethod, nullptr},
       {"sceneWriteColour", nullptr, SceneWriteColour, nullptr, nullptr, nullptr, kMethod, nullptr},
+      {"sceneWriteBounce", nullptr, SceneWriteBounce, nullptr, nullptr, nullptr, kMethod, nullptr},
+      {"sceneWriteGaussians", nullptr, SceneWriteGaussians, nullptr, nullptr, nullptr, kMethod, nullptr},
+      {"sceneWriteMaterials", nullptr, SceneWriteMaterials, nullptr, nullptr, nullptr, kMethod, nullptr},
+      {"sceneSetLod", nullptr, SceneSetLod, nullptr, nullptr, nullptr, kMethod, nullptr},
+      {"sceneSetFill", nullptr, SceneSetFill, nullptr, nullptr, nullptr, kMethod, nullptr},
+      {"fillEncode", nullptr, FillEncode, nullptr, nullptr, nullptr, kMethod, nullptr},
+      {"displayEncode", nullptr, DisplayEncode, nullptr, nullptr, nullptr, kMethod, nullptr},
+      {"sync", nullptr, Sync, nullptr, nullptr, nullptr, kMethod, nullptr},
       {"lightingIsoInit", nullptr, LightingIsoInit, nullptr, nullptr, nullptr, kMethod, nullptr},
       {"lightingEncode", nullptr, LightingEncode, nullptr, nullptr, nullptr, kMethod, nullptr},
       {"bounceInit", nullptr, BounceInit, nullptr, nullptr, nullptr, kMethod, nullptr},

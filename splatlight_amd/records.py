@@ -163,8 +163,10 @@ def render_frame(*, view, width, height, fovy, near=0.02, far=50.0, count, view_
 
 
 def frame_uniforms(*, frame=0, gaussian_count, part_count, lamp_count=0, theta=0.0, alpha_min=1.0 / 255.0,
-                   alpha_part=0.0, eye, material_count, iso=1 | 2) -> bytes:
-    """FrameUniforms (64 B) as writeUniform packs it (src/lighting.js:1229-1238)."""
+                   alpha_part=0.0, eye, material_count, iso=1 | 2, stratum_step=0, light_samples_axis=0) -> bytes:
+    """FrameUniforms (64 B) as writeUniform packs it (src/lighting.js:1229-1238). stratum_step and
+    light_samples_axis (the reference writes 7 and the asset's light_samples_axis) are read by no kernel on this
+    path."""
     buf = np.zeros(16, np.uint32)
     f = buf.view(np.float32)
     buf[0] = frame
@@ -176,8 +178,8 @@ def frame_uniforms(*, frame=0, gaussian_count, part_count, lamp_count=0, theta=0
     f[6] = alpha_part
     buf[7] = 1
     f[8:11] = eye
-    buf[11] = 0
-    buf[12] = 0
+    buf[11] = stratum_step
+    buf[12] = light_samples_axis
     buf[13] = material_count
     buf[14] = 0
     buf[15] = iso

@@ -23,7 +23,8 @@ def test_addon_exports_the_binding():
     names = set(r.stdout.strip().split(","))
     assert {"sessionAcquire", "sessionRelease", "ctxCreate", "bufAlloc", "bufRead", "sceneCreate", "sceneWriteParts",
             "sceneWriteColour", "lightingIsoInit", "lightingEncode", "frameEncode", "frameFinish",
-            "timersRead", "bounceInit", "bounceEncode"} <= names
+            "timersRead", "bounceInit", "bounceEncode", "sceneWriteBounce", "sceneWriteGaussians",
+            "sceneWriteMaterials", "sceneSetLod", "sceneSetFill", "fillEncode", "displayEncode", "sync"} <= names
 
 
 def test_shim_session_lock_matches_the_reference():
