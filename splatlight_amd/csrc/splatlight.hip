@@ -980,6 +980,9 @@ extern "C" sl_status sl_scene_create(sl_context *ctx, const sl_scene_desc *d, sl
     return fail(ctx, SL_ERR_LAYOUT, std::to_string(d->part_count) + " parts; the renderer handles 32");
   if (d->materials && d->material_stride != kMaterialStride)
     return fail(ctx, SL_ERR_LAYOUT, "materials stride " + std::to_string(d->material_stride) + ", renderer expects 32");
+  if (d->materials && d->material_count > SL_MAX_MATERIALS)
+    return fail(ctx, SL_ERR_CAPACITY, std::to_string(d->material_count) + " materials; the renderer handles " +
+                                          std::to_string(SL_MAX_MATERIALS));
   if (d->nodes && (d->node_stride != kNodeStride || d->node_count < d->gaussian_count))
     return fail(ctx, SL_ERR_LAYOUT, "nodes must be 96 B records, one leaf per Gaussian first");
   if (d->vgrid && (d->vgrid_stride != kVgridStride || d->vgrid_count < d->gaussian_count))
