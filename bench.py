@@ -172,8 +172,8 @@ def batched_leg(N, L, scenes, rounds=5):
             "batch_ms_p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
             "batch_ms_p99": round(float(np.percentile(lat, 99)) * 1e3, 3),
             "plain_ms_per_batch": round(t_plain * 1e3, 3), "tuning": tuning,
-            "tuning_ms": {f"{c['composite_batch']}/{c['radix_items']}/{c['grid_mul']}": round(ms, 3)
-                          for c, ms in table}}
+            "tuning_ms": {f"{c['tile_px']}px/{c['radix_bits']}b/{c['composite_batch']}/{c['radix_items']}/"
+                          f"{c['grid_mul']}": round(ms, 3) for c, ms in table}}
 
 
 def serving_leg(N, scenes, rate=1000.0, seconds=2.0, budget=4 << 30, lanes=2, max_batch=32):

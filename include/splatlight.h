@@ -207,7 +207,12 @@ typedef struct sl_view {
  * per lane of an onesweep tile (8: 2048-key tiles, 16: 4096); grid_mul: persistent radix workgroups per CU (1-16).
  * Defaults 256, 16, 4. No counterpart in the reference. */
 typedef struct sl_tuning {
-  uint32_t composite_batch, radix_items, grid_mul, pad;
+  uint32_t composite_batch;    /* splats staged in LDS per composite round: 64, 128 or 256 */
+  uint32_t radix_items;        /* keys per lane of an onesweep tile: 8 or 16 */
+  uint32_t grid_mul;           /* persistent sort workgroups per CU: 1-16 */
+  uint32_t radix_bits;         /* the tile sort's digit width: 8 or 11 (0: 8) */
+  uint32_t tile_px;            /* the composite tile: 16 or 32 pixels square (0: 16) */
+  uint32_t pad;
 } sl_tuning;
 sl_status sl_ctx_set_tuning(sl_context *ctx, const sl_tuning *tuning);
 sl_status sl_ctx_get_tuning(sl_context *ctx, sl_tuning *tuning);

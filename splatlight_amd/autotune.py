@@ -1,5 +1,6 @@
-"""Autotuner for the frame kernels' launch shape (sl_tuning): LDS batch depth of the composite, keys per lane of
-the onesweep radix passes, persistent radix workgroups per CU.
+"""Autotuner for the frame kernels (sl_tuning): the composite's tile size (16 or 32 px) and LDS batch depth, the tile
+sort's radix digit width (8 or 11 bits: 11-bit digits sort a 1080p frame's 32 px tile ids in one pass), keys per lane
+of the onesweep radix passes and persistent radix workgroups per CU.
 
 Every candidate gives bit-identical results, so the tuner only times: it replays the caller's own batch as a
 hipGraph under each candidate and keeps the fastest. Choices are cached per device and workload signature in a JSON
@@ -16,8 +17,8 @@ import numpy as np
 
 from . import native as N
 
-CANDIDATES = [{"composite_batch": b, "radix_items": i, "grid_mul": g}
-              for b in (64, 128, 256) for i in (8, 16) for g in (2, 4, 8)]
+CANDIDATES = [{"composite_batch": b, "radix_items": i, "grid_mul": g, "radix_bits": rb, "tile_px": tp}
+              for tp in (16, 32) for rb in (8, 11) for b in (64, 128, 256) for i in (8, 16) for g in (2, 4, 8)]
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "splatlight_amd", "tuning.json")
 
 

@@ -1,8 +1,8 @@
-// Front-to-back compositing of 16x16 tiles (the blend of shaders/splat.wgsl:22-78 with the "over" state of
-// src/renderer.js:140-142, evaluated per tile instead of by the rasteriser).
+// Front-to-back compositing of 16x16 or 32x32 tiles (the blend of shaders/splat.wgsl:22-78 with the "over" state
+// of src/renderer.js:140-142, evaluated per tile instead of by the rasteriser).
 //
-// One 256-lane workgroup per tile, launched heaviest tile first; each wave owns an 8x8 pixel block of it, one lane
-// per pixel. The tile's pair range is walked from its end, which is the reference's back-to-front sequence for
+// One workgroup per tile (256 lanes for 16x16, 1,024 for 32x32: a batch staged once serves 4x the pixels),
+// launched heaviest tile first; each wave owns an 8x8 pixel block of it, one lane per pixel. The tile's pair range is walked from its end, which is the reference's back-to-front sequence for
 // every pixel reversed, in batches of CP_BATCH_T splats (64, 128 or 256, tuned): each lane stages one splat into
 // LDS already reduced to what a pixel needs, then each wave culls the batch against its 8x8 block (64 splats per ballot, tight ellipse bounds) and sweeps only the
 // survivors, in order. A pixel stops once its transmittance is below t_eps = 2^-12 / max(1, max|rgb|) (the
@@ -20,7 +20,6 @@
 #include "sl_device.h"
 #include <hip/hip_fp16.h>
 
-#define CP_THREADS 256
 #define LOG2E 1.4426950408889634f
 
 struct __align__(16) StagedSplat {
@@ -49,8 +48,8 @@ __device__ __noinline__ float exact_alpha(float dx, float dy, const Splat *__res
   return 1.0f - (float)exp((double)(-tau0 * e));
 }
 
-template <uint32_t CP_BATCH_T>
-__global__ void __launch_bounds__(CP_THREADS, 8)
+template <uint32_t CP_BATCH_T, uint32_t TSH>
+__global__ void __launch_bounds__((1u << TSH) * (1u << TSH), 8)
 sl_composite(const ViewDev *__restrict__ views, const uint32_t *__restrict__ tile_base, uint32_t n_views,
              const ViewCounters *__restrict__ view_counters, const Splat *__restrict__ splats,
              const uint32_t *__restrict__ vals, const uint2 *__restrict__ ranges,
@@ -68,8 +67,9 @@ sl_composite(const ViewDev *__restrict__ views, const uint32_t *__restrict__ til
   const uint32_t out_format = views[v].out_format;
   const uint32_t tile = gtile - tile_base[v];
   const uint32_t tyi = tile / f.tiles_x, txi = tile - tyi * f.tiles_x;
+  constexpr uint32_t TS = 1u << TSH, WROW = TS / 8u;    // tile side; 8x8 wave blocks per tile row
   const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
-  const uint32_t bx = txi * 16u + (w & 1u) * 8u, by = tyi * 16u + (w >> 1) * 8u;   // this wave's 8x8 block
+  const uint32_t bx = txi * TS + (w % WROW) * 8u, by = tyi * TS + (w / WROW) * 8u;   // this wave's 8x8 block
   const uint32_t px = bx + (lane & 7u), py = by + (lane >> 3);
   const bool inside = px < f.width && py < f.height;
   const float fw = (float)f.width, fh = (float)f.height;

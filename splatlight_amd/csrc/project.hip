@@ -2,7 +2,7 @@
 //
 // One lane per slot. Gaussian attributes are SoA float4 streams (mu|kappa, quat, scale|part_surface,
 // albedo|flags), so each of the four loads of a wavefront reads one contiguous 1 KiB run. Outputs per slot: the
-// 48 B Splat in the reference's layout, the 16-bit depth key (or INVALID), and the splat's inclusive 16x16 tile
+// 48 B Splat in the reference's layout, the 16-bit depth key (or INVALID), and the splat's inclusive tile (16 or 32 px)
 // box packed as (tx0 | ty0 << 16, tx1 | ty1 << 16) for the pair emission.
 //
 // Contraction is off in this file: every float expression rounds as written, in the WGSL evaluation order
@@ -310,8 +310,9 @@ sl_project_kernel(const ViewDev *__restrict__ views, const uint32_t *__restrict_
       fx1 = fx1 > (float)(f.width - 1) ? (float)(f.width - 1) : fx1;
       fy1 = fy1 > (float)(f.height - 1) ? (float)(f.height - 1) : fy1;
       if (fx0 <= fx1 && fy0 <= fy1) {
-        const uint32_t tx0 = (uint32_t)fx0 >> 4, ty0 = (uint32_t)fy0 >> 4;
-        const uint32_t tx1 = (uint32_t)fx1 >> 4, ty1 = (uint32_t)fy1 >> 4;
+        const uint32_t sh = f.tile_shift;
+        const uint32_t tx0 = (uint32_t)fx0 >> sh, ty0 = (uint32_t)fy0 >> sh;
+        const uint32_t tx1 = (uint32_t)fx1 >> sh, ty1 = (uint32_t)fy1 >> sh;
         box = make_uint2(tx0 | (ty0 << 16), tx1 | (ty1 << 16));
       }
     }

@@ -223,7 +223,7 @@ extern "C" sl_status sl_server_submit(sl_server *s, const sl_view *v, uint64_t *
   {
     FrameDev f;
     const sl_frame_opts o = {v->width, v->height, v->out_format, 0u};
-    const sl_status rc = decode_frame(ctx, v->scene, v->frame_bytes, &o, &f);
+    const sl_status rc = decode_frame(ctx, v->scene, v->frame_bytes, &o, &f, 4u);   // lanes composite 16x16 tiles
     if (rc != SL_OK) return rc;
     if (f.tiles_x * f.tiles_y > 65536u) return fail(ctx, SL_ERR_CAPACITY, "more than 65536 16x16 tiles in a view");
     if (f.n_nodes) return fail(ctx, SL_ERR_UNSUPPORTED, "a screen-space LOD frame renders alone (sl_frame_encode)");

@@ -31,6 +31,7 @@ struct FrameDev {
   uint32_t blur_cover;       // the blurred splats' coverage cap (palette[3].w > 0.5)
   float lod_threshold;       // screen-space LOD: the node footprint (px) to open at (frame.lod.x); 0 = off
   uint32_t n_nodes;          // its internal nodes drawn after the emitters (frame.lod.z), 0 when off
+  uint32_t tile_shift;       // the composite tile's side is 1 << tile_shift pixels (4: 16, 5: 32)
 };
 
 // Gaussian attribute storage (SL_STORAGE_*). F32: four float4 streams g0 (mu, kappa), g1 quat, g2 (scale,

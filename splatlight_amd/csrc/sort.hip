@@ -4,13 +4,15 @@
 // the reference's stable order, shaders/radix.wgsl:1-139) and the tile order (tile ids of the (splat, tile)
 // pairs emitted in depth order; a stable sort keeps every tile's list in depth order).
 //
-// One read of the keys builds every pass's 256-digit histogram (sl_radix_hist + sl_radix_bases). Each pass is
-// then one kernel: persistent workgroups take 2048- or 4096-key tiles in ticket order; inside a tile each wave ranks
-// its 1024 elements by 8 ballots per item (peers with the same digit) and a per-wave LDS digit counter, so
-// the rank follows element order; the tile's digit counts are published with a decoupled look-back
+// One read of the keys builds every pass's digit histogram (sl_radix_hist + sl_radix_bases). Each pass is then one
+// kernel: persistent workgroups take 2048- or 4096-key tiles in ticket order; inside a tile each wave ranks its
+// elements by one ballot per digit bit (peers with the same digit) and a per-wave LDS digit counter, so the rank
+// follows element order; the tile's digit counts are published with a decoupled look-back
 // (aggregate / inclusive flags packed with the count in one word) and the tile is reordered by digit in LDS
 // so the scatter writes runs of consecutive addresses.
 #include "sl_device.h"
+
+#include <type_traits>
 
 #define RS_THREADS 256
 #define RS_WAVES (RS_THREADS / SL_WAVE)
@@ -51,61 +53,93 @@ __device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t *s_t
 
 }  // namespace
 
-// Digit histograms of every pass in one read: hist[p * 256 + digit], digits at shift0 + 8 p.
-extern "C" __global__ void __launch_bounds__(RS_THREADS)
+// Digit geometry of a BITS-bit pass: DIG digits; each of the 256 threads owns PER consecutive digits in the per-digit
+// steps (BITS <= 8: one digit, threads past DIG idle).
+template <uint32_t BITS>
+struct RsDigits {
+  static constexpr uint32_t DIG = 1u << BITS;
+  static constexpr uint32_t PER = DIG > RS_THREADS ? DIG / RS_THREADS : 1u;
+  static_assert(BITS >= 4u && BITS <= 11u, "4- to 11-bit digits");
+};
+
+// Digit histograms of every pass (at most 3) in one read: hist[p * DIG + digit], digits at shift0 + BITS p.
+template <uint32_t BITS>
+__global__ void __launch_bounds__(RS_THREADS)
 sl_radix_hist(const uint32_t *__restrict__ keys, const uint32_t *n_ptr, uint32_t n_static, uint32_t shift0,
               uint32_t npasses, uint32_t *__restrict__ hist) {
-  __shared__ uint32_t s_hist[4 * 256];
-  for (uint32_t k = threadIdx.x; k < 4u * 256u; k += RS_THREADS) s_hist[k] = 0u;
+  constexpr uint32_t DIG = RsDigits<BITS>::DIG;
+  __shared__ uint32_t s_hist[3 * DIG];
+  for (uint32_t k = threadIdx.x; k < 3u * DIG; k += RS_THREADS) s_hist[k] = 0u;
   __syncthreads();
   const uint32_t n = load_n(n_ptr, n_static);
   for (uint32_t i = blockIdx.x * RS_THREADS + threadIdx.x; i < n; i += gridDim.x * RS_THREADS) {
     const uint32_t key = keys[i];
-    for (uint32_t p = 0; p < npasses; p++) atomicAdd(&s_hist[p * 256u + ((key >> (shift0 + 8u * p)) & 255u)], 1u);
+    for (uint32_t p = 0; p < npasses; p++) atomicAdd(&s_hist[p * DIG + ((key >> (shift0 + BITS * p)) & (DIG - 1u))], 1u);
   }
   __syncthreads();
-  for (uint32_t k = threadIdx.x; k < npasses * 256u; k += RS_THREADS)
+  for (uint32_t k = threadIdx.x; k < npasses * DIG; k += RS_THREADS)
     if (s_hist[k]) atomicAdd(&hist[k], s_hist[k]);
 }
 
 // Exclusive digit offsets per pass, in place: one workgroup per pass.
-extern "C" __global__ void __launch_bounds__(RS_THREADS) sl_radix_bases(uint32_t *__restrict__ hist) {
+template <uint32_t BITS>
+__global__ void __launch_bounds__(RS_THREADS) sl_radix_bases(uint32_t *__restrict__ hist) {
+  constexpr uint32_t DIG = RsDigits<BITS>::DIG, PER = RsDigits<BITS>::PER;
   __shared__ uint32_t s_tmp[RS_WAVES];
-  uint32_t *h = hist + blockIdx.x * 256u;
-  const uint32_t v = h[threadIdx.x];
+  uint32_t *h = hist + blockIdx.x * DIG;
+  uint32_t v[PER], sum = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < PER; j++) {
+    const uint32_t d = threadIdx.x * PER + j;
+    v[j] = d < DIG ? h[d] : 0u;
+    sum += v[j];
+  }
   uint32_t total;
-  const uint32_t ex = block_excl_scan256(v, s_tmp, &total);
-  h[threadIdx.x] = ex;
+  uint32_t ex = block_excl_scan256(sum, s_tmp, &total);
+#pragma unroll
+  for (uint32_t j = 0; j < PER; j++) {
+    const uint32_t d = threadIdx.x * PER + j;
+    if (d < DIG) h[d] = ex;
+    ex += v[j];
+  }
 }
 
-// One stable pass on digit (key >> shift) & 255. vals_in == nullptr: values are the element index.
+// One stable pass on digit (key >> shift) & (DIG - 1). vals_in == nullptr: values are the element index.
 // keys_out == nullptr: keys are not written (the last pass when only the order is wanted).
-// status: >= ceil(n / tile) * 256 words, zero on entry; ticket: one word, zero on entry.
-// RS_ITEMS keys per lane (8 or 16, tuned): a tile of 256 * RS_ITEMS keys.
-template <uint32_t RS_ITEMS>
+// status: >= ceil(n / tile) * DIG words, zero on entry; ticket: one word, zero on entry.
+// RS_ITEMS keys per lane (8 or 16, tuned): a tile of 256 * RS_ITEMS keys. BITS: the digit width (8, or 11 so that a
+// frame's tile ids sort in one pass); the wave digit counters are 16-bit past 256 digits (a wave ranks at most 1,024
+// keys), which keeps the 11-bit pass at 64 KiB of LDS.
+template <uint32_t RS_ITEMS, uint32_t BITS>
 __global__ void __launch_bounds__(RS_THREADS)
 sl_radix_pass(const uint32_t *__restrict__ keys_in, const uint32_t *__restrict__ vals_in,
               uint32_t *__restrict__ keys_out, uint32_t *__restrict__ vals_out, const uint32_t *n_ptr,
               uint32_t n_static, uint32_t shift, const uint32_t *__restrict__ bases, uint32_t *status,
               uint32_t *ticket) {
   constexpr uint32_t RS_TILE = RS_THREADS * RS_ITEMS, RS_WAVE_SPAN = SL_WAVE * RS_ITEMS;
+  constexpr uint32_t DIG = RsDigits<BITS>::DIG, PER = RsDigits<BITS>::PER;
+  using WCount = typename std::conditional<(DIG > 256u), uint16_t, uint32_t>::type;
   __shared__ uint32_t s_keys[RS_TILE];
   __shared__ uint32_t s_vals[RS_TILE];
-  __shared__ uint32_t s_whist[RS_WAVES * 256];
-  __shared__ uint32_t s_dexcl[256];
-  __shared__ uint32_t s_gbase[256];
+  __shared__ WCount s_whist[RS_WAVES * DIG];
+  __shared__ uint32_t s_dexcl[DIG];
+  __shared__ uint32_t s_gbase[DIG];
   __shared__ uint32_t s_tmp[RS_WAVES];
   __shared__ uint32_t s_tile;
 
   const uint32_t n = load_n(n_ptr, n_static);
   const uint32_t num_tiles = (n + RS_TILE - 1u) / RS_TILE;
   const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
-  const uint32_t digit_base = bases[tid];
+  uint32_t digit_base[PER];
+#pragma unroll
+  for (uint32_t j = 0; j < PER; j++) {
+    const uint32_t d = tid * PER + j;
+    digit_base[j] = d < DIG ? bases[d] : 0u;
+  }
 
   for (;;) {
     if (tid == 0) s_tile = atomicAdd(ticket, 1u);
-#pragma unroll
-    for (int k = 0; k < RS_WAVES; k++) s_whist[k * 256 + tid] = 0u;
+    for (uint32_t k = tid; k < RS_WAVES * DIG; k += RS_THREADS) s_whist[k] = 0u;
     __syncthreads();
     const uint32_t t = s_tile;
     if (t >= num_tiles) break;
@@ -121,16 +155,16 @@ sl_radix_pass(const uint32_t *__restrict__ keys_in, const uint32_t *__restrict__
       key[k] = v ? keys_in[idx] : 0u;
       val[k] = v ? (vals_in ? vals_in[idx] : idx) : 0u;
     }
-    // rank within the wave, element order (k, lane)
-    uint32_t *whist = s_whist + w * 256u;
+    // rank within the wave, element order (k, lane): peers with the same digit by one ballot per digit bit
+    WCount *whist = s_whist + w * DIG;
 #pragma unroll
     for (int k = 0; k < RS_ITEMS; k++) {
       const uint32_t idx = wbase + (uint32_t)k * SL_WAVE + lane;
       const bool v = idx < n;
-      const uint32_t d = (key[k] >> shift) & 255u;
+      const uint32_t d = (key[k] >> shift) & (DIG - 1u);
       uint64_t peers = __ballot(v);
 #pragma unroll
-      for (int b = 0; b < 8; b++) {
+      for (uint32_t b = 0; b < BITS; b++) {
         const bool bit = (d >> b) & 1u;
         const uint64_t bal = __ballot(bit);
         peers &= bit ? bal : ~bal;
@@ -140,49 +174,72 @@ sl_radix_pass(const uint32_t *__restrict__ keys_in, const uint32_t *__restrict__
         const uint32_t below = (uint32_t)__popcll(peers & lanemask_lt());
         const uint32_t old = whist[d];
         r = old + below;
-        if (below == 0u) whist[d] = old + (uint32_t)__popcll(peers);
+        if (below == 0u) whist[d] = (WCount)(old + (uint32_t)__popcll(peers));
       }
       rank[k] = r;
     }
     __syncthreads();
-    // digit tid: per-wave exclusive offsets and the tile's count
-    uint32_t cnt = 0;
+    // each owned digit: per-wave exclusive offsets and the tile's count, then its decoupled look-back
+    uint32_t cnt[PER], excl[PER], sum = 0;
 #pragma unroll
-    for (int k = 0; k < RS_WAVES; k++) {
-      const uint32_t c = s_whist[k * 256 + tid];
-      s_whist[k * 256 + tid] = cnt;
-      cnt += c;
-    }
-    // decoupled look-back for digit tid
-    uint32_t *st = status + (size_t)t * 256u + tid;
-    uint32_t excl = 0;
-    if (t == 0) {
-      __hip_atomic_store(st, RS_FLAG_INC | cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      __hip_atomic_store(st, RS_FLAG_AGG | cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      int32_t j = (int32_t)t - 1;
-      for (;;) {
-        const uint32_t s = __hip_atomic_load(status + (size_t)j * 256u + tid, __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT);
-        const uint32_t fl = s & RS_FLAG_MASK;
-        if (fl == 0u) { __builtin_amdgcn_s_sleep(1); continue; }
-        excl += s & RS_VALUE_MASK;
-        if (fl == RS_FLAG_INC) break;
-        --j;
+    for (uint32_t j = 0; j < PER; j++) {
+      const uint32_t d = tid * PER + j;
+      uint32_t c = 0;
+      if (d < DIG) {
+#pragma unroll
+        for (int k = 0; k < RS_WAVES; k++) {
+          const uint32_t x = s_whist[k * DIG + d];
+          s_whist[k * DIG + d] = (WCount)c;
+          c += x;
+        }
       }
-      __hip_atomic_store(st, RS_FLAG_INC | (excl + cnt), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      cnt[j] = c;
+      sum += c;
     }
-    s_gbase[tid] = digit_base + excl;
+#pragma unroll
+    for (uint32_t j = 0; j < PER; j++) {
+      const uint32_t d = tid * PER + j;
+      excl[j] = 0;
+      if (d >= DIG) continue;
+      uint32_t *st = status + (size_t)t * DIG + d;
+      if (t == 0) {
+        __hip_atomic_store(st, RS_FLAG_INC | cnt[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        __hip_atomic_store(st, RS_FLAG_AGG | cnt[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int32_t jj = (int32_t)t - 1;
+        uint32_t e = 0;
+        for (;;) {
+          const uint32_t s = __hip_atomic_load(status + (size_t)jj * DIG + d, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+          const uint32_t fl = s & RS_FLAG_MASK;
+          if (fl == 0u) { __builtin_amdgcn_s_sleep(1); continue; }
+          e += s & RS_VALUE_MASK;
+          if (fl == RS_FLAG_INC) break;
+          --jj;
+        }
+        __hip_atomic_store(st, RS_FLAG_INC | (e + cnt[j]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        excl[j] = e;
+      }
+    }
     uint32_t total;
-    s_dexcl[tid] = block_excl_scan256(cnt, s_tmp, &total);
+    uint32_t run = block_excl_scan256(sum, s_tmp, &total);
+#pragma unroll
+    for (uint32_t j = 0; j < PER; j++) {
+      const uint32_t d = tid * PER + j;
+      if (d < DIG) {
+        s_gbase[d] = digit_base[j] + excl[j];
+        s_dexcl[d] = run;
+      }
+      run += cnt[j];
+    }
     __syncthreads();
     // reorder the tile by digit in LDS
 #pragma unroll
     for (int k = 0; k < RS_ITEMS; k++) {
       const uint32_t idx = wbase + (uint32_t)k * SL_WAVE + lane;
       if (idx < n) {
-        const uint32_t d = (key[k] >> shift) & 255u;
-        const uint32_t pos = s_dexcl[d] + s_whist[w * 256u + d] + rank[k];
+        const uint32_t d = (key[k] >> shift) & (DIG - 1u);
+        const uint32_t pos = s_dexcl[d] + (uint32_t)s_whist[w * DIG + d] + rank[k];
         s_keys[pos] = key[k];
         s_vals[pos] = val[k];
       }
@@ -190,13 +247,24 @@ sl_radix_pass(const uint32_t *__restrict__ keys_in, const uint32_t *__restrict__
     __syncthreads();
     for (uint32_t j = tid; j < tile_n; j += RS_THREADS) {
       const uint32_t kk = s_keys[j];
-      const uint32_t d = (kk >> shift) & 255u;
+      const uint32_t d = (kk >> shift) & (DIG - 1u);
       const uint32_t dst = s_gbase[d] + (j - s_dexcl[d]);
       if (keys_out) keys_out[dst] = kk;
       vals_out[dst] = s_vals[j];
     }
     __syncthreads();
   }
+}
+
+// Zero the look-back words a tile pass over the device's pair count uses: ceil(pairs / tile_keys) onesweep tiles of
+// `digits` words each, at most cap words (a kernel reading the count, so the clear covers what the frame fills).
+extern "C" __global__ void __launch_bounds__(256)
+sl_fill_pairs_status(uint32_t *__restrict__ p, const uint32_t *pairs, uint32_t tile_keys, uint32_t digits,
+                     uint32_t cap) {
+  const uint32_t n = __hip_atomic_load(pairs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const uint64_t words = (uint64_t)((n + tile_keys - 1u) / tile_keys) * digits;
+  const uint32_t m = (uint32_t)(words < cap ? words : cap);
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < m; i += gridDim.x * 256u) p[i] = 0u;
 }
 
 // Fill n words with a value (clears look-back state and counters; a kernel rather than a memset so the frame

@@ -58,6 +58,8 @@ struct Batch {
   std::vector<uint32_t> slot_base, tile_base, tiles_x;   // V + 1, V + 1, V
   uint32_t total_slots = 0, total_tiles = 0;
   uint32_t depth_passes = 2, tile_passes = 2;
+  uint32_t tile_bits = 8;  // the tile sort's digit width (8 or 11)
+  uint32_t tile_shift = 4; // log2 of the composite tile's side (4: 16 px, 5: 32 px)
   uint32_t flags = 0;
   bool graph = false;
   bool fixed = false;   // laid over the device's no-alloc scratch instead of the context's workspace
@@ -145,6 +147,7 @@ struct sl_scene {
 
 struct Workspace {
   uint32_t slots_cap = 0, pairs_cap = 0, tiles_cap = 0;
+  uint32_t tile_bits = 8;         // the tile sort's digit width its look-back state is laid out for
   void *base = nullptr;           // one allocation for everything below
   size_t bytes = 0;
   Splat *splats = nullptr;
@@ -167,6 +170,9 @@ struct Workspace {
   uint64_t *chunk_status = nullptr;
   uint2 *ranges = nullptr;
   uint32_t *tile_order = nullptr;  // composite launch order (written every frame)
+  // 11-bit tile digits: the tile sort's two look-back arrays (2,048 words per onesweep tile) lie past the zeroed
+  // block, each cleared just before its pass over the tiles the pair count fills
+  size_t wide_words = 0;           // words of one such array
 };
 
 struct sl_context {
@@ -195,7 +201,7 @@ struct sl_context {
   // the last captured batch graph and what it was captured for
   hipGraphExec_t graph = nullptr;
   std::vector<uint64_t> graph_key;
-  sl_tuning tuning = {256u, 16u, 4u, 0u};
+  sl_tuning tuning = {256u, 16u, 4u, 8u, 16u, 0u};
   // the no-alloc frame (sl_frame_encode_noalloc): the device's scratch array's address, looked up once per device
   // at context creation, and the workspace laid over it for the last such frame
   uint8_t *fixed_base = nullptr;
@@ -294,12 +300,13 @@ uint8_t *scratch_address(int device, std::string *err) {
   return (uint8_t *)g_scratch_dev[device];
 }
 
-// Lay the frame workspace for `s_cap` splat slots, `p_cap` pairs and `t_cap` tiles over `base` (nullptr: only
-// measure); returns the bytes it spans.
-size_t layout_workspace(uint8_t *b, uint32_t s_cap, uint32_t p_cap, uint32_t t_cap, Workspace &w) {
+// Lay the frame workspace for `s_cap` splat slots, `p_cap` pairs and `t_cap` tiles, its tile sort in `tile_bits`-bit
+// digits, over `base` (nullptr: only measure); returns the bytes it spans.
+size_t layout_workspace(uint8_t *b, uint32_t s_cap, uint32_t p_cap, uint32_t t_cap, uint32_t tile_bits, Workspace &w) {
   const uint32_t d_tiles = (s_cap + RS_TILE_MIN - 1) / RS_TILE_MIN;
   const uint32_t p_tiles = (p_cap + RS_TILE_MIN - 1) / RS_TILE_MIN;
   const uint32_t chunks = (s_cap + PE_CHUNK - 1) / PE_CHUNK;
+  const bool wide = tile_bits > 8u;
   size_t off = 0;
   auto take = [&](size_t bytes) { const size_t o = off; off = align_up(off + bytes, 256); return o; };
   const size_t o_splats = take((size_t)s_cap * sizeof(Splat));
@@ -311,15 +318,18 @@ size_t layout_workspace(uint8_t *b, uint32_t s_cap, uint32_t p_cap, uint32_t t_c
   const size_t o_pk1 = take((size_t)p_cap * 4), o_pv1 = take((size_t)p_cap * 4);
   const size_t o_views = take(kViewBlock);
   const size_t o_vcount = take(sizeof(ViewCounters) * kMaxViews);
+  const size_t wide_words = (size_t)p_tiles * 2048;
+  const size_t o_wide0 = wide ? take(wide_words * 4) : 0, o_wide1 = wide ? take(wide_words * 4) : 0;
   const size_t o_scratch = off;
   const size_t o_counters = take(sizeof(FrameCounters));
   const size_t o_tickets = take(16 * 4);
-  const size_t o_dhist = take(3 * 256 * 4), o_thist = take(3 * 256 * 4);
+  const size_t o_dhist = take(3 * 256 * 4), o_thist = take(3 * 2048 * 4);
   const size_t o_chist = take(65537 * 4);
   const size_t o_ds0 = take((size_t)d_tiles * 256 * 4), o_ds1 = take((size_t)d_tiles * 256 * 4);
   const size_t o_ds2 = take((size_t)d_tiles * 256 * 4);
-  const size_t o_ts0 = take((size_t)p_tiles * 256 * 4), o_ts1 = take((size_t)p_tiles * 256 * 4);
-  const size_t o_ts2 = take((size_t)p_tiles * 256 * 4);
+  const size_t t_words = wide ? 0 : (size_t)p_tiles * 256;
+  const size_t o_ts0 = take(t_words * 4), o_ts1 = take(t_words * 4);
+  const size_t o_ts2 = take(t_words * 4);
   const size_t o_chunk = take((size_t)chunks * 8);
   const size_t o_ranges = take((size_t)t_cap * 8);
   const size_t o_torder = take((size_t)t_cap * 4);
@@ -330,6 +340,8 @@ size_t layout_workspace(uint8_t *b, uint32_t s_cap, uint32_t p_cap, uint32_t t_c
   w.slots_cap = s_cap;
   w.pairs_cap = p_cap;
   w.tiles_cap = t_cap;
+  w.tile_bits = tile_bits;
+  w.wide_words = wide ? wide_words : 0;
   w.splats = (Splat *)(b + o_splats);
   w.keys = (uint32_t *)(b + o_keys);
   w.keys_alt = (uint32_t *)(b + o_keys_alt);
@@ -356,9 +368,9 @@ size_t layout_workspace(uint8_t *b, uint32_t s_cap, uint32_t p_cap, uint32_t t_c
   w.dstatus[0] = (uint32_t *)(b + o_ds0);
   w.dstatus[1] = (uint32_t *)(b + o_ds1);
   w.dstatus[2] = (uint32_t *)(b + o_ds2);
-  w.tstatus[0] = (uint32_t *)(b + o_ts0);
-  w.tstatus[1] = (uint32_t *)(b + o_ts1);
-  w.tstatus[2] = (uint32_t *)(b + o_ts2);
+  w.tstatus[0] = (uint32_t *)(b + (wide ? o_wide0 : o_ts0));
+  w.tstatus[1] = (uint32_t *)(b + (wide ? o_wide1 : o_ts1));
+  w.tstatus[2] = wide ? nullptr : (uint32_t *)(b + o_ts2);
   w.chunk_status = (uint64_t *)(b + o_chunk);
   w.ranges = (uint2 *)(b + o_ranges);
   w.tile_order = (uint32_t *)(b + o_torder);
@@ -374,13 +386,14 @@ void drop_graph(sl_context *ctx) {
 }
 
 // Grow the context's own frame workspace to hold `slots` splat slots, `pairs` pairs and `tiles` tiles.
-sl_status ensure_workspace(sl_context *ctx, uint32_t slots, uint32_t pairs, uint32_t tiles) {
+sl_status ensure_workspace(sl_context *ctx, uint32_t slots, uint32_t pairs, uint32_t tiles, uint32_t tile_bits) {
   Workspace &w = ctx->ws;
-  if (w.base && slots <= w.slots_cap && pairs <= w.pairs_cap && tiles <= w.tiles_cap) return SL_OK;
+  if (w.base && slots <= w.slots_cap && pairs <= w.pairs_cap && tiles <= w.tiles_cap && tile_bits == w.tile_bits)
+    return SL_OK;
   const uint32_t s_cap = std::max(slots, w.slots_cap);
   const uint32_t p_cap = std::max(pairs, w.pairs_cap);
   const uint32_t t_cap = std::max(tiles, w.tiles_cap);
-  const size_t total = layout_workspace(nullptr, s_cap, p_cap, t_cap, w);
+  const size_t total = layout_workspace(nullptr, s_cap, p_cap, t_cap, tile_bits, w);
   if (w.base) {
     SL_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (ctx->geo) SL_HIP(ctx, hipStreamSynchronize(ctx->geo));
@@ -391,12 +404,13 @@ sl_status ensure_workspace(sl_context *ctx, uint32_t slots, uint32_t pairs, uint
   void *base = nullptr;
   sl_status s = dev_alloc(ctx, total, &base);
   if (s != SL_OK) return s;
-  layout_workspace((uint8_t *)base, s_cap, p_cap, t_cap, w);
+  layout_workspace((uint8_t *)base, s_cap, p_cap, t_cap, tile_bits, w);
   return SL_OK;
 }
 
 // RenderFrame (shaders/common.wgsl:44-77) -> FrameDev, refusing frames that turn on features outside this path.
-sl_status decode_frame(sl_context *ctx, const sl_scene *sc, const void *fb, const sl_frame_opts *o, FrameDev *f) {
+sl_status decode_frame(sl_context *ctx, const sl_scene *sc, const void *fb, const sl_frame_opts *o, FrameDev *f,
+                       uint32_t tile_shift) {
   memset(f, 0, sizeof(*f));
   for (int k = 0; k < 16; k++) f->view[k] = rd_f32(fb, k);
   for (int m = 0; m < 3; m++)
@@ -460,8 +474,9 @@ sl_status decode_frame(sl_context *ctx, const sl_scene *sc, const void *fb, cons
   f->nslots = sc->count + 1u + sc->n_emit + f->n_nodes;
   f->width = o->width;
   f->height = o->height;
-  f->tiles_x = (o->width + 15u) / 16u;
-  f->tiles_y = (o->height + 15u) / 16u;
+  f->tile_shift = tile_shift;
+  f->tiles_x = (o->width + (1u << tile_shift) - 1u) >> tile_shift;
+  f->tiles_y = (o->height + (1u << tile_shift) - 1u) >> tile_shift;
   return SL_OK;
 }
 
@@ -520,44 +535,60 @@ sl_status enqueue_batch(sl_context *ctx, const Batch &b, bool timers, bool overl
   const sl_tuning &tu = ctx->tuning;
   const uint32_t rs_tile = RS_THREADS * tu.radix_items;
   const uint32_t d_grid = std::min<uint32_t>((n + rs_tile - 1) / rs_tile, tu.grid_mul * ctx->cus);
+  // one onesweep pass in `bits`-bit digits (8: the depth sort and 8-bit tile sorts; 11: the tile sort when tuned so)
   auto radix_pass = [&](const uint32_t *ki, const uint32_t *vi, uint32_t *ko, uint32_t *vo, const uint32_t *n_ptr,
                         uint32_t n_static, uint32_t shift, const uint32_t *bases, uint32_t *status, uint32_t *ticket,
-                        uint32_t grid) {
-    if (tu.radix_items == 8u)
-      sl_radix_pass<8><<<grid, RS_THREADS, 0, st>>>(ki, vi, ko, vo, n_ptr, n_static, shift, bases, status, ticket);
-    else
-      sl_radix_pass<16><<<grid, RS_THREADS, 0, st>>>(ki, vi, ko, vo, n_ptr, n_static, shift, bases, status, ticket);
+                        uint32_t grid, uint32_t bits) {
+    if (bits == 11u) {
+      if (tu.radix_items == 8u)
+        sl_radix_pass<8, 11><<<grid, RS_THREADS, 0, st>>>(ki, vi, ko, vo, n_ptr, n_static, shift, bases, status, ticket);
+      else
+        sl_radix_pass<16, 11><<<grid, RS_THREADS, 0, st>>>(ki, vi, ko, vo, n_ptr, n_static, shift, bases, status, ticket);
+    } else if (tu.radix_items == 8u) {
+      sl_radix_pass<8, 8><<<grid, RS_THREADS, 0, st>>>(ki, vi, ko, vo, n_ptr, n_static, shift, bases, status, ticket);
+    } else {
+      sl_radix_pass<16, 8><<<grid, RS_THREADS, 0, st>>>(ki, vi, ko, vo, n_ptr, n_static, shift, bases, status, ticket);
+    }
   };
   if (b.counting) {     // one view: keys are 16-bit depth keys or INVALID
     sl_count_hist<<<(n + 255u) / 256u, 256, 0, st>>>(w.keys, n, w.count_hist);
     sl_count_scan<<<1, 1024, 0, st>>>(w.count_hist);
     sl_count_scatter<<<(n + 255u) / 256u, 256, 0, st>>>(w.keys, n, w.count_hist, w.order);
   } else {
-  sl_radix_hist<<<grid_for(n, 256 * 16, 1024), RS_THREADS, 0, st>>>(w.keys, nullptr, n, 0u, b.depth_passes,
-                                                                   w.depth_hist);
-  sl_radix_bases<<<b.depth_passes, RS_THREADS, 0, st>>>(w.depth_hist);
-  radix_pass(w.keys, nullptr, w.keys_alt, w.vals_alt, nullptr, n, 0u, w.depth_hist, w.dstatus[0], w.tickets + 0, d_grid);
+  sl_radix_hist<8><<<grid_for(n, 256 * 16, 1024), RS_THREADS, 0, st>>>(w.keys, nullptr, n, 0u, b.depth_passes,
+                                                                      w.depth_hist);
+  sl_radix_bases<8><<<b.depth_passes, RS_THREADS, 0, st>>>(w.depth_hist);
+  radix_pass(w.keys, nullptr, w.keys_alt, w.vals_alt, nullptr, n, 0u, w.depth_hist, w.dstatus[0], w.tickets + 0, d_grid,
+             8u);
   if (b.depth_passes == 2) {
     radix_pass(w.keys_alt, w.vals_alt, nullptr, w.order, nullptr, n, 8u, w.depth_hist + 256, w.dstatus[1],
-               w.tickets + 1, d_grid);
+               w.tickets + 1, d_grid, 8u);
   } else {
     radix_pass(w.keys_alt, w.vals_alt, w.pk1, w.pv1, nullptr, n, 8u, w.depth_hist + 256, w.dstatus[1], w.tickets + 1,
-               d_grid);
+               d_grid, 8u);
     radix_pass(w.pk1, w.pv1, nullptr, w.order, nullptr, n, 16u, w.depth_hist + 512, w.dstatus[2], w.tickets + 2,
-               d_grid);
+               d_grid, 8u);
   }
   }
   if ((s = mark(3, st)) != SL_OK) return s;
-  sl_pairs_emit<false><<<grid_for(n, PE_CHUNK, 4u * ctx->cus), PE_THREADS, 0, st>>>(
-      w.order, w.tbox, nullptr, w.slot_base, w.tile_base, w.tiles_x, V, b.tile_passes, w.pairs_cap, w.counters, w.pk0, w.pv0,
-      w.chunk_status, w.tickets + 3, w.tile_hist);
+  const uint32_t tb = b.tile_bits, tdig = 1u << tb;
+  auto pairs_emit = [&](auto kernel) {
+    kernel<<<grid_for(n, PE_CHUNK, 4u * ctx->cus), PE_THREADS, 0, st>>>(
+        w.order, w.tbox, nullptr, w.slot_base, w.tile_base, w.tiles_x, V, b.tile_passes, w.pairs_cap, w.counters, w.pk0,
+        w.pv0, w.chunk_status, w.tickets + 3, w.tile_hist);
+  };
+  if (tb == 11u) pairs_emit(sl_pairs_emit<false, 11>); else pairs_emit(sl_pairs_emit<false, 8>);
   if ((s = mark(4, st)) != SL_OK) return s;
   const uint32_t p_grid = std::max(std::min<uint32_t>((w.pairs_cap + rs_tile - 1) / rs_tile, tu.grid_mul * ctx->cus), 1u);
-  sl_radix_bases<<<b.tile_passes, RS_THREADS, 0, st>>>(w.tile_hist);
+  if (tb == 11u) sl_radix_bases<11><<<b.tile_passes, RS_THREADS, 0, st>>>(w.tile_hist);
+  else sl_radix_bases<8><<<b.tile_passes, RS_THREADS, 0, st>>>(w.tile_hist);
   uint32_t *kin = w.pk0, *vin = w.pv0, *kout = w.pk1, *vout = w.pv1;
   for (uint32_t pass = 0; pass < b.tile_passes; pass++) {
-    radix_pass(kin, vin, kout, vout, &w.counters->pairs, 0u, 8u * pass, w.tile_hist + 256 * pass, w.tstatus[pass],
-               w.tickets + 4 + pass, p_grid);
+    if (tb == 11u)   // the wide look-back array: only the onesweep tiles the pair count fills, zeroed here
+      sl_fill_pairs_status<<<grid_for(w.wide_words, 256, 2048), 256, 0, st>>>(w.tstatus[pass], &w.counters->pairs,
+                                                                              rs_tile, tdig, (uint32_t)w.wide_words);
+    radix_pass(kin, vin, kout, vout, &w.counters->pairs, 0u, tb * pass, w.tile_hist + tdig * pass, w.tstatus[pass],
+               w.tickets + 4 + pass, p_grid, tb);
     std::swap(kin, kout);
     std::swap(vin, vout);
   }
@@ -577,18 +608,20 @@ sl_status enqueue_batch(sl_context *ctx, const Batch &b, bool timers, bool overl
     if (timers) SL_HIP(ctx, hipEventRecord(ctx->ev_comp, ms));
   }
   ctx->timed_overlap = overlap;
+  auto composite = [&](auto kernel, uint32_t threads) {
+    kernel<<<b.total_tiles, threads, 0, ms>>>(w.views, w.tile_base, V, w.view_counters, w.splats, vin, w.ranges,
+                                              w.tile_order);
+  };
+  const bool big = b.tile_shift == 5u;
   switch (tu.composite_batch) {
     case 64u:
-      sl_composite<64><<<b.total_tiles, CP_THREADS, 0, ms>>>(w.views, w.tile_base, V, w.view_counters, w.splats, vin,
-                                                             w.ranges, w.tile_order);
+      big ? composite(sl_composite<64, 5>, 1024u) : composite(sl_composite<64, 4>, 256u);
       break;
     case 128u:
-      sl_composite<128><<<b.total_tiles, CP_THREADS, 0, ms>>>(w.views, w.tile_base, V, w.view_counters, w.splats, vin,
-                                                              w.ranges, w.tile_order);
+      big ? composite(sl_composite<128, 5>, 1024u) : composite(sl_composite<128, 4>, 256u);
       break;
     default:
-      sl_composite<256><<<b.total_tiles, CP_THREADS, 0, ms>>>(w.views, w.tile_base, V, w.view_counters, w.splats, vin,
-                                                              w.ranges, w.tile_order);
+      big ? composite(sl_composite<256, 5>, 1024u) : composite(sl_composite<256, 4>, 256u);
   }
   if ((s = mark(7, ms)) != SL_OK) return s;
   return check_launch(ctx, "frame kernels");
@@ -616,8 +649,8 @@ sl_status launch_batch(sl_context *ctx, const Batch &b) {
   if (b.graph) {
     // what the captured launches depend on: the layout and the workspace (the view table is read at run time)
     std::vector<uint64_t> key = {(uint64_t)(uintptr_t)w.base, V, b.counting, b.blur, b.total_slots, b.total_tiles, w.pairs_cap,
-                                 b.depth_passes, b.tile_passes, ctx->tuning.composite_batch,
-                                 ctx->tuning.radix_items, ctx->tuning.grid_mul};
+                                 b.depth_passes, b.tile_passes, b.tile_bits, b.tile_shift,
+                                 ctx->tuning.composite_batch, ctx->tuning.radix_items, ctx->tuning.grid_mul};
     if (!ctx->graph || key != ctx->graph_key) {
       if (ctx->graph) {
         (void)hipGraphExecDestroy(ctx->graph);
@@ -1187,6 +1220,10 @@ sl_status encode_batch(sl_context *ctx, const sl_view *views, uint32_t n_views, 
   b.graph = !fixed && (flags & SL_BATCH_GRAPH) != 0;
   b.fixed = fixed;
   b.counting = (flags & SL_FRAME_COUNTING_SORT) != 0;
+  // the tuned tile and tile-sort digit width (the no-alloc frame keeps 8-bit digits: its scratch holds no wide
+  // look-back arrays)
+  b.tile_shift = ctx->tuning.tile_px == 32u ? 5u : 4u;
+  b.tile_bits = fixed ? 8u : ctx->tuning.radix_bits;
   if (b.counting && n_views > 1)
     return fail(ctx, SL_ERR_UNSUPPORTED, "the counting scatter orders one view's 16-bit depth keys, not a batch");
   b.views.resize(n_views);
@@ -1204,10 +1241,10 @@ sl_status encode_batch(sl_context *ctx, const sl_view *views, uint32_t n_views, 
     const sl_frame_opts o = {in.width, in.height, in.out_format, 0u};
     ViewDev &vd = b.views[v];
     memset(&vd, 0, sizeof(vd));
-    sl_status s = decode_frame(ctx, sc, in.frame_bytes, &o, &vd.f);
+    sl_status s = decode_frame(ctx, sc, in.frame_bytes, &o, &vd.f, b.tile_shift);
     if (s != SL_OK) return s;
     const uint32_t nt = vd.f.tiles_x * vd.f.tiles_y;
-    if (nt > 65536u) return fail(ctx, SL_ERR_CAPACITY, "more than 65536 16x16 tiles in a view");
+    if (nt > 65536u) return fail(ctx, SL_ERR_CAPACITY, "more than 65536 tiles in a view");
     vd.g0 = sc->g[0]; vd.g1 = sc->g[1]; vd.g2 = sc->g[2]; vd.g3 = sc->g[3];
     vd.e0 = sc->e[0]; vd.e1 = sc->e[1]; vd.e2 = sc->e[2]; vd.e3 = sc->e[3];
     vd.parts = sc->parts; vd.colour = sc->colour; vd.lo_b = sc->lo_b;
@@ -1261,13 +1298,15 @@ sl_status encode_batch(sl_context *ctx, const sl_view *views, uint32_t n_views, 
   b.total_slots = (uint32_t)slots;
   b.total_tiles = (uint32_t)tiles;
   b.depth_passes = n_views > 1 ? 3u : 2u;
-  b.tile_passes = tiles > 65536u ? 3u : 2u;
+  uint32_t tile_id_bits = 1;            // bits of the largest global tile id
+  while (tile_id_bits < 24u && (tiles - 1u) >> tile_id_bits) tile_id_bits++;
+  b.tile_passes = (tile_id_bits + b.tile_bits - 1u) / b.tile_bits;
   SL_HIP(ctx, hipSetDevice(ctx->device));
   if (fixed) {
     // this frame's slots and tiles, then as many pairs as the rest of the scratch holds (at least one per slot:
     // a batch's three-pass depth sort borrows the second pair buffers)
     const size_t cap = SL_NOALLOC_SCRATCH_BYTES;
-    const size_t fixed_bytes = layout_workspace(nullptr, b.total_slots, b.total_slots, b.total_tiles, ctx->fws);
+    const size_t fixed_bytes = layout_workspace(nullptr, b.total_slots, b.total_slots, b.total_tiles, 8u, ctx->fws);
     if (fixed_bytes > cap)
       return fail(ctx, SL_ERR_CAPACITY, "no-alloc frame: " + std::to_string(b.total_slots) + " slots and " +
                                             std::to_string(b.total_tiles) + " tiles need " +
@@ -1276,17 +1315,18 @@ sl_status encode_batch(sl_context *ctx, const sl_view *views, uint32_t n_views, 
     uint32_t lo = b.total_slots, hi = (uint32_t)std::min<size_t>(cap / 16, 1u << 30);
     while (lo < hi) {
       const uint32_t mid = lo + (hi - lo + 1) / 2;
-      if (layout_workspace(nullptr, b.total_slots, mid, b.total_tiles, ctx->fws) <= cap) lo = mid;
+      if (layout_workspace(nullptr, b.total_slots, mid, b.total_tiles, 8u, ctx->fws) <= cap) lo = mid;
       else hi = mid - 1;
     }
-    layout_workspace(ctx->fixed_base, b.total_slots, lo, b.total_tiles, ctx->fws);
+    layout_workspace(ctx->fixed_base, b.total_slots, lo, b.total_tiles, 8u, ctx->fws);
     ctx->has_last = true;
     ctx->last = std::move(b);
     return launch_batch(ctx, ctx->last);
   }
   const uint32_t pairs = std::max<uint32_t>(ctx->ws.pairs_cap, (uint32_t)std::min<uint64_t>(want_pairs + 4096, 1u << 29));
   sl_status s;
-  if ((s = ensure_workspace(ctx, b.total_slots, std::max(pairs, b.total_slots), b.total_tiles)) != SL_OK) return s;
+  if ((s = ensure_workspace(ctx, b.total_slots, std::max(pairs, b.total_slots), b.total_tiles, b.tile_bits)) != SL_OK)
+    return s;
   ctx->has_last = true;
   ctx->last = std::move(b);
   return launch_batch(ctx, ctx->last);
@@ -1314,7 +1354,7 @@ sl_status finish_batch(sl_context *ctx, sl_frame_stats *stats, uint32_t n_stats)
     const uint64_t need = (uint64_t)c.pairs + c.overflow;
     const uint64_t grown = std::min<uint64_t>((need * 5) / 4 + 4096, 1u << 30);
     if (need > grown) return fail(ctx, SL_ERR_CAPACITY, "batch needs more than 2^30 pairs");
-    sl_status s = ensure_workspace(ctx, b.total_slots, (uint32_t)grown, b.total_tiles);
+    sl_status s = ensure_workspace(ctx, b.total_slots, (uint32_t)grown, b.total_tiles, b.tile_bits);
     if (s != SL_OK) return s;
     if ((s = launch_batch(ctx, b)) != SL_OK) return s;
     SL_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -1594,11 +1634,16 @@ extern "C" sl_status sl_display_encode(sl_context *ctx, const void *frame_bytes,
 extern "C" sl_status sl_ctx_set_tuning(sl_context *ctx, const sl_tuning *t) {
   SL_LIVE(ctx);
   if (!t) return fail(ctx, SL_ERR_INVALID, "null argument");
-  if ((t->composite_batch != 64u && t->composite_batch != 128u && t->composite_batch != 256u) ||
-      (t->radix_items != 8u && t->radix_items != 16u) || t->grid_mul < 1u || t->grid_mul > 16u)
-    return fail(ctx, SL_ERR_INVALID, "tuning: composite_batch 64/128/256, radix_items 8/16, grid_mul 1-16");
-  ctx->tuning = *t;
-  ctx->tuning.pad = 0u;
+  sl_tuning u = *t;
+  u.radix_bits = u.radix_bits ? u.radix_bits : 8u;
+  u.tile_px = u.tile_px ? u.tile_px : 16u;
+  u.pad = 0u;
+  if ((u.composite_batch != 64u && u.composite_batch != 128u && u.composite_batch != 256u) ||
+      (u.radix_items != 8u && u.radix_items != 16u) || u.grid_mul < 1u || u.grid_mul > 16u ||
+      (u.radix_bits != 8u && u.radix_bits != 11u) || (u.tile_px != 16u && u.tile_px != 32u))
+    return fail(ctx, SL_ERR_INVALID, "tuning: composite_batch 64/128/256, radix_items 8/16, grid_mul 1-16, "
+                                     "radix_bits 8/11, tile_px 16/32");
+  ctx->tuning = u;   // a workspace laid out for other digits is laid out again at the next encode
   return SL_OK;
 }
 
@@ -1803,14 +1848,14 @@ void map_bin(sl_context *ctx, hipStream_t st, uint32_t which, size_t n_rec, uint
   if (!cleared) map_clear(ctx, st, which, n_rec, n_layers, res, span);
   uint32_t *k0 = (uint32_t *)m.k0.p, *v0 = (uint32_t *)m.v0.p;
   uint32_t *k1 = (uint32_t *)m.k1.p, *v1 = (uint32_t *)m.v1.p;
-  sl_pairs_emit<true><<<grid_for(n_rec, PE_CHUNK, 4u * ctx->cus), PE_THREADS, 0, st>>>(
+  sl_pairs_emit<true, 8><<<grid_for(n_rec, PE_CHUNK, 4u * ctx->cus), PE_THREADS, 0, st>>>(
       nullptr, (const uint2 *)m.tbox.p, (const uint64_t *)m.tmask.p, w + ms.zeros, w + ms.zeros, w + ms.tiles_x, 1u, sh.tile_passes,
       (uint32_t)sh.cap, counters, k0, v0, (uint64_t *)(w + ms.chunk), w + ms.tickets, w + ms.hist);
-  sl_radix_bases<<<sh.tile_passes, RS_THREADS, 0, st>>>(w + ms.hist);
+  sl_radix_bases<8><<<sh.tile_passes, RS_THREADS, 0, st>>>(w + ms.hist);
   const uint32_t grid = std::max(std::min<uint32_t>((uint32_t)((sh.cap + 4095) / 4096), 4u * ctx->cus), 1u);
   uint32_t *kin = k0, *vin = v0, *kout = k1, *vout = v1;
   for (uint32_t p = 0; p < sh.tile_passes; p++) {
-    sl_radix_pass<16><<<grid, RS_THREADS, 0, st>>>(kin, vin, kout, vout, &counters->pairs, 0u, 8u * p,
+    sl_radix_pass<16, 8><<<grid, RS_THREADS, 0, st>>>(kin, vin, kout, vout, &counters->pairs, 0u, 8u * p,
                                                    w + ms.hist + 256 * p, w + ms.tstat[p], w + ms.tickets + 1 + p);
     std::swap(kin, kout);
     std::swap(vin, vout);

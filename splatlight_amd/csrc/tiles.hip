@@ -32,10 +32,11 @@ __device__ __forceinline__ uint32_t nth_set_bit(uint64_t m, uint32_t k) {
 }
 
 // order == nullptr: rank r is slot r. Batches: a rank's slot belongs to the view v with slot_base[v] <= slot < slot_base[v + 1]; its pairs' tile ids
-// are global, tile_base[v] + ty * tiles_x[v] + tx. tile_passes (2 or 3) digit histograms are built.
+// are global, tile_base[v] + ty * tiles_x[v] + tx. tile_passes (1 to 3) digit histograms of BITS-bit digits (the tile
+// sort's digit width) are built.
 // MASKED (the opacity-map splats, boxes of at most 64 tiles): only the box's tiles whose bit (row * box width +
 // column) is set in tmask[slot] get a pair; otherwise every tile of the box does.
-template <bool MASKED>
+template <bool MASKED, uint32_t BITS>
 __global__ void __launch_bounds__(PE_THREADS)
 sl_pairs_emit(const uint32_t *__restrict__ order, const uint2 *__restrict__ tbox, const uint64_t *__restrict__ tmask,
               const uint32_t *__restrict__ slot_base, const uint32_t *__restrict__ tile_base,
@@ -46,7 +47,8 @@ sl_pairs_emit(const uint32_t *__restrict__ order, const uint2 *__restrict__ tbox
   __shared__ uint2 s_box[PE_CHUNK];
   __shared__ uint32_t s_g[PE_CHUNK];
   __shared__ uint64_t s_mask[MASKED ? PE_CHUNK : 1];
-  __shared__ uint32_t s_hist[3 * 256];
+  constexpr uint32_t DIG = 1u << BITS;
+  __shared__ uint32_t s_hist[3 * DIG];
   __shared__ uint32_t s_wsum[PE_THREADS / SL_WAVE];
   __shared__ uint32_t s_chunk;
   __shared__ uint64_t s_excl;
@@ -55,7 +57,7 @@ sl_pairs_emit(const uint32_t *__restrict__ order, const uint2 *__restrict__ tbox
   const uint32_t tb0 = tile_base[0], tx0v = tiles_x[0];   // the only view of a single frame
   const uint32_t V = __hip_atomic_load(&counters->visible, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   const uint32_t num_chunks = (V + PE_CHUNK - 1u) / PE_CHUNK;
-  for (uint32_t k = tid; k < 768u; k += PE_THREADS) s_hist[k] = 0u;
+  for (uint32_t k = tid; k < 3u * DIG; k += PE_THREADS) s_hist[k] = 0u;
 
   for (;;) {
     if (tid == 0) s_chunk = atomicAdd(ticket, 1u);
@@ -108,39 +110,61 @@ sl_pairs_emit(const uint32_t *__restrict__ order, const uint2 *__restrict__ tbox
     uint32_t run = before + x - sum;
 #pragma unroll
     for (int k = 0; k < PE_RPT; k++) { s_start[tid * PE_RPT + k] = run; run += cnt[k]; }
-    if (tid == 0) {
-      s_start[PE_CHUNK] = total;
-      // decoupled look-back over chunks
+    if (w == 0) {
+      // decoupled look-back over chunks, 64 predecessors per round: lane l reads chunk c - 1 - l - 64 k; the nearest
+      // inclusive prefix ends the walk, and every chunk between must have published its aggregate
       uint64_t *st = chunk_status + c;
       uint64_t excl = 0;
       if (c == 0) {
-        __hip_atomic_store(st, PE_FLAG_INC | (uint64_t)total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (lane == 0) __hip_atomic_store(st, PE_FLAG_INC | (uint64_t)total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       } else {
-        __hip_atomic_store(st, PE_FLAG_AGG | (uint64_t)total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        int64_t j = (int64_t)c - 1;
+        if (lane == 0) __hip_atomic_store(st, PE_FLAG_AGG | (uint64_t)total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int64_t top = (int64_t)c - 1;
         for (;;) {
-          const uint64_t s = __hip_atomic_load(chunk_status + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          const uint64_t fl = s & PE_FLAG_MASK;
-          if (fl == 0ull) { __builtin_amdgcn_s_sleep(1); continue; }
-          excl += s & PE_VALUE_MASK;
-          if (fl == PE_FLAG_INC) break;
-          --j;
+          const int64_t j = top - (int64_t)lane;
+          const uint64_t sv = j >= 0 ? __hip_atomic_load(chunk_status + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                     : PE_FLAG_INC;   // before chunk 0: an inclusive zero
+          const uint64_t fl = sv & PE_FLAG_MASK;
+          const uint64_t inc = __ballot(fl == PE_FLAG_INC), none = __ballot(fl == 0ull);
+          const uint32_t first = inc ? (uint32_t)__builtin_ctzll(inc) : 64u;   // the nearest inclusive prefix
+          const uint64_t before = first >= 64u ? ~0ull : ((2ull << first) - 1ull);
+          if (none & before) {                       // a chunk in between has not published yet
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+          }
+          uint64_t v = lane <= first ? (sv & PE_VALUE_MASK) : 0ull;
+#pragma unroll
+          for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+          excl += v;
+          if (first < 64u) break;
+          top -= 64;
         }
-        __hip_atomic_store(st, PE_FLAG_INC | (excl + total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (lane == 0) __hip_atomic_store(st, PE_FLAG_INC | (excl + total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
-      s_excl = excl;
-      if (c == num_chunks - 1u) {
-        const uint64_t all = excl + total;
-        const uint32_t kept = all > capacity ? capacity : (uint32_t)all;
-        __hip_atomic_store(&counters->pairs, kept, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&counters->overflow, (uint32_t)(all - kept), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (lane == 0) {
+        s_start[PE_CHUNK] = total;
+        s_excl = excl;
+        if (c == num_chunks - 1u) {
+          const uint64_t all = excl + total;
+          const uint32_t kept = all > capacity ? capacity : (uint32_t)all;
+          __hip_atomic_store(&counters->pairs, kept, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_store(&counters->overflow, (uint32_t)(all - kept), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
       }
     }
     __syncthreads();
     const uint64_t base = s_excl;
     // cooperative emission: lane j writes pair j of the chunk
+    uint32_t lo = 0;                                    // the last rank with start <= j, carried: j only grows
     for (uint32_t j = tid; j < total; j += PE_THREADS) {
-      uint32_t lo = 0, hi = PE_CHUNK;                   // last rank with start <= j
+      // gallop up from the last owner, then halve (s_start[PE_CHUNK] = total > j ends both): a rank's pairs are
+      // consecutive, so a lane's owner moves on by a few ranks where pairs are dense (the heavy chunks)
+      uint32_t step = 1u, hi = lo + 1u;
+      while (s_start[hi] <= j) {
+        lo = hi;
+        step <<= 1;
+        hi = min(lo + step, (uint32_t)PE_CHUNK);
+      }
       while (hi - lo > 1u) {
         const uint32_t mid = (lo + hi) >> 1;
         if (s_start[mid] <= j) lo = mid; else hi = mid;
@@ -149,7 +173,11 @@ sl_pairs_emit(const uint32_t *__restrict__ order, const uint2 *__restrict__ tbox
       const uint32_t local = MASKED ? nth_set_bit(s_mask[lo], j - s_start[lo]) : j - s_start[lo];
       const uint32_t tx0 = b.x & 0xffffu, ty0 = b.x >> 16;
       const uint32_t bw = (b.y & 0xffffu) - tx0 + 1u;
-      const uint32_t ty = ty0 + local / bw, tx = tx0 + local % bw;
+      // local / bw and local % bw through the float reciprocal (local < 2^24: the quotient is off by at most one)
+      uint32_t qd = (uint32_t)((float)local * __builtin_amdgcn_rcpf((float)bw));
+      int32_t rm = (int32_t)local - (int32_t)(qd * bw);
+      if (rm < 0) { qd--; rm += (int32_t)bw; } else if (rm >= (int32_t)bw) { qd++; rm -= (int32_t)bw; }
+      const uint32_t ty = ty0 + qd, tx = tx0 + (uint32_t)rm;
       uint32_t tile;
       if (n_views > 1u) {
         const uint32_t vw = view_of(slot_base, n_views, s_g[lo]);
@@ -163,26 +191,27 @@ sl_pairs_emit(const uint32_t *__restrict__ order, const uint2 *__restrict__ tbox
         pair_keys[dst] = tile;
         pair_vals[dst] = s_g[lo];
       }
-      // tile-id digit counts for the tile sort, one LDS add per distinct digit of the wave (neighbouring pairs
-      // share digits, so per-lane atomics would serialise on one address)
+      // tile-id digit counts for the tile sort: a digit the wave's kept pairs all share (the high digits, nearly
+      // always: a wave's pairs lie in a few neighbouring tile rows) is one LDS add of the count; otherwise each lane
+      // adds its own (low digits of neighbouring pairs differ, so the adds rarely meet on one address)
+      const uint64_t kept = __ballot(keep);
+      const bool leader = keep && (kept & lanemask_lt()) == 0ull;
 #pragma unroll
       for (uint32_t pass = 0; pass < 3u; pass++) {
         if (pass >= tile_passes) break;
-        const uint32_t d = (tile >> (8 * pass)) & 255u;
-        uint64_t peers = __ballot(keep);
-#pragma unroll
-        for (int b = 0; b < 8; b++) {
-          const bool bit = (d >> b) & 1u;
-          const uint64_t bal = __ballot(bit);
-          peers &= bit ? bal : ~bal;
+        const uint32_t d = (tile >> (BITS * pass)) & (DIG - 1u);
+        const uint32_t d_lead = __builtin_amdgcn_readlane(d, kept ? (uint32_t)__builtin_ctzll(kept) : 0u);
+        if (__ballot(keep && d != d_lead) == 0ull) {
+          if (leader) atomicAdd(&s_hist[pass * DIG + d_lead], (uint32_t)__popcll(kept));
+        } else if (keep) {
+          atomicAdd(&s_hist[pass * DIG + d], 1u);
         }
-        if (keep && (peers & lanemask_lt()) == 0ull) atomicAdd(&s_hist[pass * 256 + d], (uint32_t)__popcll(peers));
       }
     }
     __syncthreads();
   }
   __syncthreads();
-  for (uint32_t k = tid; k < 256u * tile_passes; k += PE_THREADS)
+  for (uint32_t k = tid; k < DIG * tile_passes; k += PE_THREADS)
     if (s_hist[k]) atomicAdd(&tile_hist[k], s_hist[k]);
 }
 

@@ -90,7 +90,7 @@ class View(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("composite_batch", C.c_uint32), ("radix_items", C.c_uint32), ("grid_mul", C.c_uint32),
-                ("pad", C.c_uint32)]
+                ("radix_bits", C.c_uint32), ("tile_px", C.c_uint32), ("pad", C.c_uint32)]
 
 
 class ServerOpts(C.Structure):
@@ -496,14 +496,19 @@ class Server:
             self.h = C.c_void_p()
 
 
-def set_tuning(ctx: Context, composite_batch: int, radix_items: int, grid_mul: int):
-    ctx.check(ctx.L.sl_ctx_set_tuning(ctx.h, C.byref(Tuning(composite_batch, radix_items, grid_mul, 0))))
+def set_tuning(ctx: Context, composite_batch: int, radix_items: int, grid_mul: int, radix_bits: int = 8,
+               tile_px: int = 16):
+    """sl_ctx_set_tuning: composite LDS batch depth, onesweep keys per lane, persistent sort workgroups per CU, the
+    tile sort's digit width (8 or 11 bits) and the composite tile (16 or 32 px)."""
+    ctx.check(ctx.L.sl_ctx_set_tuning(ctx.h, C.byref(Tuning(composite_batch, radix_items, grid_mul, radix_bits,
+                                                             tile_px, 0))))
 
 
 def get_tuning(ctx: Context) -> dict:
     t = Tuning()
     ctx.check(ctx.L.sl_ctx_get_tuning(ctx.h, C.byref(t)))
-    return {"composite_batch": t.composite_batch, "radix_items": t.radix_items, "grid_mul": t.grid_mul}
+    return {"composite_batch": t.composite_batch, "radix_items": t.radix_items, "grid_mul": t.grid_mul,
+            "radix_bits": t.radix_bits, "tile_px": t.tile_px}
 
 
 def frame_finish(ctx: Context) -> FrameStats:

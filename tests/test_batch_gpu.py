@@ -151,15 +151,21 @@ def test_every_tuning_gives_the_same_results(ctx, world):
     scs, handles = world
     frames = frames_for(scs, VIEWS, 0.2)
     ref_imgs, ref_stats, ref_arts = run_batch(ctx, handles, VIEWS, frames, graph=False)
+    refs = {16: (ref_stats, ref_arts)}
     try:
+        N.set_tuning(ctx, 256, 16, 4, tile_px=32)        # the 32 px tiling's own pair lists (eager, 8-bit digits)
+        _, st32, art32 = run_batch(ctx, handles, VIEWS, frames, graph=False)
+        refs[32] = (st32, art32)
         for cand in autotune.CANDIDATES:
             N.set_tuning(ctx, **cand)
             imgs, stats, arts = run_batch(ctx, handles, VIEWS, frames, graph=True)
+            rs, ra = refs[cand["tile_px"]]
             assert all(np.array_equal(a, b) for a, b in zip(imgs, ref_imgs)), cand
-            assert [s.pairs for s in stats] == [s.pairs for s in ref_stats], cand
-            for k in ref_arts:
-                assert np.array_equal(arts[k], ref_arts[k]), (cand, k)
-        log(f"  {len(autotune.CANDIDATES)} tunings: identical images and artefacts")
+            assert [s.pairs for s in stats] == [s.pairs for s in rs], cand
+            for k in ra:
+                assert np.array_equal(arts[k], ra[k]), (cand, k)
+        log(f"  {len(autotune.CANDIDATES)} tunings: images identical across tile sizes and digit widths, "
+            f"artefacts identical per tile size")
     finally:
         N.set_tuning(ctx, 256, 16, 4)
 

@@ -68,14 +68,14 @@ def psnr(a, b, peak=1.0):
     return math.inf if mse == 0 else 10 * math.log10(peak * peak / mse)
 
 
-def check_integer_artefacts(sc, st, art, o_keys, o_order, o_vis, o_splats):
+def check_integer_artefacts(sc, st, art, o_keys, o_order, o_vis, o_splats, tile_px=16):
     W, H = sc.width, sc.height
     assert st.visible == o_vis
     np.testing.assert_array_equal(art["keys"], o_keys)
     np.testing.assert_array_equal(art["order"], o_order)
     valid = o_keys != R.INVALID_KEY
     np.testing.assert_array_equal(art["splats"][valid], o_splats[valid])
-    start, vals = oracle.tile_lists(o_splats, o_order, o_vis, W, H)
+    start, vals = oracle.tile_lists(o_splats, o_order, o_vis, W, H, tile_px)
     assert st.pairs == vals.shape[0]
     np.testing.assert_array_equal(art["slots"], vals)
     tiles = np.repeat(np.arange(start.shape[0] - 1), np.diff(start).astype(np.int64))
@@ -113,6 +113,36 @@ def test_frame_matches_oracle(ctx, n, W, H, seed):
         f"fraction over 1/1024={frac:.2e}; reference fp32 blend vs float64: max|err|={np.abs(ref - ref64).max():.3g}")
     assert psnr(img, ref64) >= MIN_PSNR
     assert frac <= 1e-3
+
+
+@pytest.mark.parametrize("tile_px,bits", [(32, 8), (16, 11), (32, 11)])
+@pytest.mark.parametrize("n,W,H,seed", [(2000, 160, 120, 3), (200_000, 640, 480, 0), (1_000_000, 1920, 1080, 4)])
+def test_tile_size_and_digit_width(ctx, n, W, H, seed, tile_px, bits):
+    """The autotuner's tile size (16 or 32 px) and tile-sort digit width (8 or 11 bits): keys, order and splat words
+    as at the default, every tile's pair list bit-exact against the oracle's lists for that tile size, and the image
+    bit-identical to the default's (a pixel meets the same splats in the same order whatever the tiling)."""
+    log(f"tuning parity: n={n} {W}x{H} tile {tile_px} px, {bits}-bit tile digits")
+    sc = scenes.tiny(n=n, seed=seed, width=W, height=H) if n < 10000 else (
+        scenes.cornell(n=n, seed=seed, width=W, height=H) if n < 500_000 else
+        scenes.hill(n=n, seed=seed, width=W, height=H))
+    col = lit_colour(sc, seed)
+    fb = sc.frame_bytes()
+    scene = N.Scene(ctx, sc.gaussians, sc.parts, emitters=sc.emitters)
+    try:
+        scene.write_colour(col)
+        st0, img0, art0 = render(ctx, sc, col, scene=scene)
+        N.set_tuning(ctx, 256, 16, 4, radix_bits=bits, tile_px=tile_px)
+        try:
+            st, img, art = render(ctx, sc, col, scene=scene)
+        finally:
+            N.set_tuning(ctx, 256, 16, 4)
+    finally:
+        scene.destroy()
+    o_splats, o_keys, o_vis = oracle.project(fb, sc, colour=col)
+    _, _, o_order, _ = oracle.sort_keys(o_keys)
+    check_integer_artefacts(sc, st, art, o_keys, o_order, o_vis, o_splats, tile_px)
+    assert np.array_equal(img, img0)
+    log(f"  pairs {st.pairs} (16 px: {st0.pairs}), tiles {art['ranges'].shape[0]}; lists bit-exact, image identical")
 
 
 def test_rgba16f_target_against_fp16_emulation(ctx):
