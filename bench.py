@@ -176,7 +176,7 @@ def batched_leg(N, L, scenes, rounds=5):
                           f"{c['grid_mul']}": round(ms, 3) for c, ms in table}}
 
 
-def serving_leg(N, scenes, rate=1000.0, seconds=2.0, budget=4 << 30, lanes=2, max_batch=32):
+def serving_leg(N, scenes, rate=1000.0, seconds=2.0, budget=16 << 30, lanes=3, max_batch=32):
     """The served form: Poisson arrivals at `rate` over 32 resident scenes (100k-3M Gaussians, seeds 10-41, colour
     precomputed), 4 sizes from 256^2 to 1920x1080, random orbit poses; admission by a fixed HBM budget, continuous
     batching over `lanes` lanes. Against the plain per-frame path fed the same arrivals one frame at a time. Also

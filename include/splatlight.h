@@ -24,7 +24,7 @@ typedef int sl_status;
 #define SL_ERR_LAYOUT 2        /* record stride / offset / count / byte-size mismatch (src/asset.js:18-38) */
 #define SL_ERR_DEVICE 3        /* a HIP call failed; the context is now poisoned */
 #define SL_ERR_POISONED 4      /* an earlier device error poisoned this context */
-#define SL_ERR_UNSUPPORTED 5   /* a frame feature outside this path (motion blur, LOD, refraction, ...) */
+#define SL_ERR_UNSUPPORTED 5   /* a feature outside this path (explode, particles, unified lights, ...) */
 #define SL_ERR_BUSY 6          /* the GPU session is held elsewhere */
 #define SL_ERR_CAPACITY 7      /* a budget or size limit would be exceeded */
 
@@ -224,13 +224,18 @@ sl_status sl_batch_finish(sl_context *ctx, sl_frame_stats *stats, uint32_t n_sta
 /* ---- serving: continuous batching of render requests over resident scenes --------------------------------
  * A worker thread owns `lanes` execution lanes (each its own stream and workspace on the context's device). Whenever
  * a lane is free it admits waiting requests, in arrival order, into one batch while the batch has at most
- * max_batch_views views and the HBM estimate of all in-flight work stays within hbm_budget, and launches it; up to
- * `lanes` batches run at once, so requests arriving while others render are admitted as lanes and memory free up.
+ * max_batch_views views and the workspace the lane needs for it fits beside the other lanes' workspaces within
+ * hbm_budget, and launches it; up to `lanes` batches run at once, so requests arriving while others render are
+ * admitted as lanes and memory free up. The budget bounds the device memory the lanes allocate (their workspaces:
+ * splat slots, sort buffers, pair lists, tile ranges; the targets are the caller's): a lane grows to at most what
+ * the others leave (a pair-list overflow that would pass it fails its requests with SL_ERR_CAPACITY), idle lanes
+ * give their workspaces back when a batch cannot fit otherwise, and a request whose workspace alone exceeds the
+ * budget is refused by sl_server_submit with SL_ERR_CAPACITY.
  * Each result is the view rendered alone (same kernels, per-view state). The scenes and targets must stay alive
  * until their results are polled. No counterpart in the reference (one frame per requestAnimationFrame). */
 typedef struct sl_server sl_server;
 typedef struct sl_server_opts {
-  uint64_t hbm_budget;         /* bytes of in-flight work (targets, slots, pair lists) */
+  uint64_t hbm_budget;         /* bytes the lanes' workspaces may hold together */
   uint32_t lanes;              /* 1-8 concurrent batches */
   uint32_t max_batch_views;    /* 1-256 */
 } sl_server_opts;
@@ -247,6 +252,7 @@ sl_status sl_server_create(sl_context *ctx, const sl_server_opts *opts, sl_serve
 sl_status sl_server_submit(sl_server *server, const sl_view *view, uint64_t *request_id);
 /* completed requests (up to cap), waiting up to wait_us for the first when none is ready */
 sl_status sl_server_poll(sl_server *server, sl_request_result *out, uint32_t cap, uint32_t *n, uint32_t wait_us);
+/* in_flight_bytes: the lanes' workspaces (held, or reserved for their running batches) */
 sl_status sl_server_info(sl_server *server, uint32_t *pending, uint32_t *running, uint64_t *in_flight_bytes);
 const char *sl_server_error(sl_server *server);   /* the last lane error (a copy, valid until this thread calls again) */
 /* Failure-handling check from the host: the worker's next completion check of a busy lane reports a device error

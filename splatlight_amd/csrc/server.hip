@@ -3,8 +3,11 @@
 //
 // A worker thread owns `lanes` execution lanes, each a context of its own (stream, workspace, counters). Requests
 // wait in arrival order; whenever a lane is free, the worker admits requests from the head of the queue into one
-// batch while the batch stays within max_batch_views and the HBM estimate of everything in flight stays within
-// the budget, and launches it on that lane (sl_batch_encode's path). A lane's completion is polled with an event,
+// batch while the batch stays within max_batch_views and the workspace the lane needs for it (its slots, tiles and
+// pair lists, laid out exactly as the frame path lays them) fits beside the other lanes' workspaces within the HBM
+// budget, and launches it on that lane (sl_batch_encode's path). The budget bounds what the lanes allocate: a lane's
+// growth (its first batches, a pair-list overflow re-run) is capped at what the others leave, idle lanes give their
+// workspaces back when a batch cannot fit otherwise, and a request that could not fit alone is refused at submit. A lane's completion is polled with an event,
 // so requests that arrive while other batches run are admitted as soon as a lane and memory free up, and up to
 // `lanes` batches overlap on the device. Results carry submit / start / done times (seconds on a steady clock).
 #include <atomic>
@@ -17,7 +20,7 @@ struct ServerRequest {
   uint64_t id = 0;
   sl_view view{};
   std::vector<uint8_t> frame;
-  uint64_t bytes = 0;            // HBM estimate while in flight
+  uint32_t slots = 0, tiles = 0;  // its splat slots (rounded to the 256 the batch layout gives a view) and tiles
   double t_submit = 0.0;
   hipEvent_t ready = nullptr;    // the owner context's queued work at submit (a lighting of the scene, uploads)
 };
@@ -27,7 +30,7 @@ struct ServerLane {
   hipEvent_t done = nullptr;
   bool busy = false;
   std::vector<ServerRequest> reqs;
-  uint64_t bytes = 0;
+  uint64_t reserved = 0;          // workspace bytes the lane holds or may grow to for its running batch
   double t_start = 0.0;
 };
 
@@ -51,13 +54,15 @@ struct sl_server {
 
 namespace {
 
-// HBM a request holds while in flight: its target, its splat slots (splat, keys, order, sort temporaries, tile
-// box) and its pair lists at the workspace's initial 8 pairs per slot (key and value, two copies).
-uint64_t request_bytes(const sl_view &v) {
-  const uint64_t slots = (uint64_t)v.scene->count + 1 + v.scene->n_emit;
-  const uint64_t tiles = (uint64_t)((v.width + 15) / 16) * ((v.height + 15) / 16);
-  const uint64_t px = (uint64_t)v.width * v.height * (v.out_format == SL_OUT_RGBA16F ? 8 : 16);
-  return px + slots * (48 + 4 * 4 + 8) + 8 * slots * 16 + tiles * 12;
+// The workspace bytes a lane whose workspace already holds `w` needs for a batch of `slots` slots and `tiles` tiles
+// (the pair lists at 8 per slot + 4096, as the batch path sizes them; never below what the workspace has).
+uint64_t lane_need(const Workspace &w, uint32_t tile_bits, uint64_t slots, uint64_t tiles, uint64_t nslots) {
+  const uint64_t pairs = std::min<uint64_t>(8ull * nslots + 4096, 1u << 29);
+  Workspace tmp;
+  const uint32_t s_cap = (uint32_t)std::max<uint64_t>(slots, w.slots_cap);
+  const uint32_t p_cap = (uint32_t)std::max<uint64_t>(std::max<uint64_t>(pairs, slots), w.pairs_cap);
+  const uint32_t t_cap = (uint32_t)std::max<uint64_t>(tiles, w.tiles_cap);
+  return layout_workspace(nullptr, s_cap, p_cap, t_cap, tile_bits, tmp);
 }
 
 // A lane whose completion check returned q (hipSuccess, or the error that ended it): its requests' results, with
@@ -66,7 +71,12 @@ void finish_lane(sl_server *s, ServerLane &ln, hipError_t q) {
   std::vector<sl_frame_stats> st(ln.reqs.size());
   sl_status rc;
   if (q == hipSuccess) {
+    // a pair-list overflow re-runs the batch with grown lists: at most what the other lanes leave of the budget
+    uint64_t others = 0;
+    for (const ServerLane &l2 : s->lanes) others += &l2 == &ln ? 0 : l2.reserved;
+    ln.ctx->ws_limit = s->opts.hbm_budget > others ? s->opts.hbm_budget - others : 1;
     rc = finish_batch(ln.ctx, st.data(), (uint32_t)st.size());
+    ln.ctx->ws_limit = 0;
   } else {
     (void)hipStreamSynchronize(ln.ctx->stream);   // nothing of the batch still runs on the lane
     rc = hip_fail(ln.ctx, q, "server lane: batch completion");
@@ -85,29 +95,60 @@ void finish_lane(sl_server *s, ServerLane &ln, hipError_t q) {
     r.done_s = t;
     s->results.push_back(r);
   }
-  s->in_flight_bytes -= ln.bytes;
   ln.reqs.clear();
-  ln.bytes = 0;
+  ln.reserved = ln.ctx->ws.base ? ln.ctx->ws.bytes : 0;   // what it holds now
+  s->in_flight_bytes = 0;
+  for (const ServerLane &l2 : s->lanes) s->in_flight_bytes += l2.reserved;
   ln.busy = false;
   s->cv.notify_all();
 }
 
-// Admit from the head of the queue into one batch for a free lane; false when nothing was admitted.
+// An idle lane's workspace given back (it is laid out again by its next batch).
+void release_workspace(ServerLane &ln) {
+  Workspace &w = ln.ctx->ws;
+  if (!w.base) return;
+  (void)hipStreamSynchronize(ln.ctx->stream);
+  (void)hipFree(w.base);
+  w = Workspace();
+  drop_graph(ln.ctx);
+  ln.reserved = 0;
+}
+
+// Admit from the head of the queue into one batch for a free lane; false when nothing was admitted (or, when the
+// head request cannot fit beside the other lanes, an idle lane's workspace was given back: true, try again).
 bool launch_lane(sl_server *s, ServerLane &ln) {
   std::vector<ServerRequest> batch;
-  uint64_t bytes = 0;
+  uint64_t need = 0;
   {
     std::lock_guard<std::mutex> g(s->mu);
+    uint64_t others = 0;
+    for (const ServerLane &l2 : s->lanes) others += &l2 == &ln ? 0 : l2.reserved;
+    const uint32_t bits = ln.ctx->tuning.radix_bits;
+    uint64_t slots = 0, tiles = 0, nslots = 0;
     while (!s->pending.empty() && batch.size() < s->opts.max_batch_views) {
       const ServerRequest &r = s->pending.front();
-      if (s->in_flight_bytes + bytes + r.bytes > s->opts.hbm_budget && !(batch.empty() && s->in_flight_bytes == 0))
-        break;                                   // wait for memory (an oversized request runs alone)
-      bytes += r.bytes;
+      const uint64_t n = lane_need(ln.ctx->ws, bits, slots + r.slots, tiles + r.tiles, nslots + r.slots);
+      if (others + n > s->opts.hbm_budget) break;   // wait for memory
+      need = n;
+      slots += r.slots;
+      tiles += r.tiles;
+      nslots += r.slots;
       batch.push_back(std::move(s->pending.front()));
       s->pending.pop_front();
     }
-    if (batch.empty()) return false;
-    s->in_flight_bytes += bytes;
+    if (batch.empty()) {
+      if (s->pending.empty()) return false;
+      // the head does not fit beside the others: give back the largest idle workspace, if any holds memory
+      ServerLane *idle = nullptr;
+      for (ServerLane &l2 : s->lanes)
+        if (&l2 != &ln && !l2.busy && l2.reserved && (!idle || l2.reserved > idle->reserved)) idle = &l2;
+      if (!idle) return false;                      // wait for a busy lane to finish
+      release_workspace(*idle);
+      return true;
+    }
+    ln.reserved = need;
+    s->in_flight_bytes = 0;
+    for (const ServerLane &l2 : s->lanes) s->in_flight_bytes += l2.reserved;
   }
   std::vector<sl_view> views(batch.size());
   for (size_t k = 0; k < batch.size(); k++) {
@@ -116,7 +157,6 @@ bool launch_lane(sl_server *s, ServerLane &ln) {
   }
   ln.t_start = s->now();
   ln.reqs = std::move(batch);
-  ln.bytes = bytes;
   ln.busy = true;
   sl_status rc = ln.ctx->poisoned != SL_OK ? SL_ERR_POISONED : SL_OK;
   for (ServerRequest &r : ln.reqs) {             // ordered after what the owner had queued when it was submitted
@@ -124,7 +164,11 @@ bool launch_lane(sl_server *s, ServerLane &ln) {
     (void)hipEventDestroy(r.ready);
     r.ready = nullptr;
   }
-  if (rc == SL_OK) rc = encode_batch(ln.ctx, views.data(), (uint32_t)views.size(), 0u);
+  if (rc == SL_OK) {
+    ln.ctx->ws_limit = ln.reserved;                  // the workspace grows to at most what was reserved
+    rc = encode_batch(ln.ctx, views.data(), (uint32_t)views.size(), 0u);
+    ln.ctx->ws_limit = 0;
+  }
   if (rc == SL_OK && hipEventRecord(ln.done, ln.ctx->stream) != hipSuccess) rc = SL_ERR_DEVICE;
   if (rc != SL_OK) {
     const double t = s->now();
@@ -140,9 +184,10 @@ bool launch_lane(sl_server *s, ServerLane &ln) {
       s->results.push_back(res);
     }
     s->err = ln.ctx->err.empty() ? std::string("server lane: device error") : ln.ctx->err;
-    s->in_flight_bytes -= ln.bytes;
     ln.reqs.clear();
-    ln.bytes = 0;
+    ln.reserved = ln.ctx->ws.base ? ln.ctx->ws.bytes : 0;
+    s->in_flight_bytes = 0;
+    for (const ServerLane &l2 : s->lanes) s->in_flight_bytes += l2.reserved;
     ln.busy = false;
     s->cv.notify_all();
   }
@@ -220,18 +265,28 @@ extern "C" sl_status sl_server_submit(sl_server *s, const sl_view *v, uint64_t *
   if (!v || !v->scene || !v->frame_bytes || !v->out_device) return fail(ctx, SL_ERR_INVALID, "null argument");
   if (v->frame_len != kFrameBytes) return fail(ctx, SL_ERR_INVALID, "RenderFrame must be 1200 B");
   if (v->out_format > SL_OUT_RGBA16F) return fail(ctx, SL_ERR_INVALID, "unknown output format");
+  uint32_t r_slots = 0, r_tiles = 0;
   {
     FrameDev f;
     const sl_frame_opts o = {v->width, v->height, v->out_format, 0u};
-    const sl_status rc = decode_frame(ctx, v->scene, v->frame_bytes, &o, &f, 4u);   // lanes composite 16x16 tiles
+    const uint32_t shift = s->lanes[0].ctx->tuning.tile_px == 32u ? 5u : 4u;   // the lanes' tiling
+    const sl_status rc = decode_frame(ctx, v->scene, v->frame_bytes, &o, &f, shift);
     if (rc != SL_OK) return rc;
-    if (f.tiles_x * f.tiles_y > 65536u) return fail(ctx, SL_ERR_CAPACITY, "more than 65536 16x16 tiles in a view");
+    if (f.tiles_x * f.tiles_y > 65536u) return fail(ctx, SL_ERR_CAPACITY, "more than 65536 tiles in a view");
     if (f.n_nodes) return fail(ctx, SL_ERR_UNSUPPORTED, "a screen-space LOD frame renders alone (sl_frame_encode)");
+    r_slots = (f.nslots + 255u) / 256u * 256u;
+    r_tiles = f.tiles_x * f.tiles_y;
   }
+  const uint64_t alone = lane_need(Workspace(), s->lanes[0].ctx->tuning.radix_bits, r_slots, r_tiles, r_slots);
+  if (alone > s->opts.hbm_budget)
+    return fail(ctx, SL_ERR_CAPACITY, "server: the request alone needs a " + std::to_string(alone) +
+                                          " B workspace, more than the HBM budget of " +
+                                          std::to_string(s->opts.hbm_budget) + " B");
   ServerRequest r;
   r.view = *v;
   r.frame.assign((const uint8_t *)v->frame_bytes, (const uint8_t *)v->frame_bytes + kFrameBytes);
-  r.bytes = request_bytes(*v);
+  r.slots = r_slots;
+  r.tiles = r_tiles;
   r.t_submit = s->now();
   // the lane that renders it waits for the work the owner context had queued (e.g. sl_lighting_encode of the scene)
   SL_HIP(ctx, hipEventCreateWithFlags(&r.ready, hipEventDisableTiming));

@@ -202,6 +202,7 @@ struct sl_context {
   hipGraphExec_t graph = nullptr;
   std::vector<uint64_t> graph_key;
   sl_tuning tuning = {256u, 16u, 4u, 8u, 16u, 0u};
+  size_t ws_limit = 0;   // the workspace may grow to at most this many bytes (0: no limit; a server lane's budget)
   // the no-alloc frame (sl_frame_encode_noalloc): the device's scratch array's address, looked up once per device
   // at context creation, and the workspace laid over it for the last such frame
   uint8_t *fixed_base = nullptr;
@@ -394,6 +395,9 @@ sl_status ensure_workspace(sl_context *ctx, uint32_t slots, uint32_t pairs, uint
   const uint32_t p_cap = std::max(pairs, w.pairs_cap);
   const uint32_t t_cap = std::max(tiles, w.tiles_cap);
   const size_t total = layout_workspace(nullptr, s_cap, p_cap, t_cap, tile_bits, w);
+  if (ctx->ws_limit && total > ctx->ws_limit)
+    return fail(ctx, SL_ERR_CAPACITY, "workspace of " + std::to_string(total) + " B over the " +
+                                          std::to_string(ctx->ws_limit) + " B the server's HBM budget leaves this lane");
   if (w.base) {
     SL_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (ctx->geo) SL_HIP(ctx, hipStreamSynchronize(ctx->geo));

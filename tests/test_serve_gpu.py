@@ -2,7 +2,7 @@
 
 Bar: every request's image is bit-identical to the same view rendered alone; requests that arrive while others
 render join later batches (more than one batch, batches of more than one view, and lanes overlapping in time);
-the HBM budget bounds what is in flight.
+the HBM budget bounds the lanes' workspaces (sampled while serving) and a request that cannot fit alone is refused.
 """
 import math
 import sys
@@ -176,3 +176,52 @@ def test_served_request_waits_for_the_owner_lighting(ctx):
         ctx.free(o1)
         ctx.free(o2)
         h.destroy()
+
+
+def test_server_budget_bounds_lane_workspaces(ctx):
+    """With a budget that holds a few views' workspaces, the lanes' reserved and held workspaces never exceed it
+    (sampled while 32 requests render), lanes give idle workspaces back so every request still runs, results equal
+    the views alone, and a request whose workspace alone exceeds the budget is refused at submit."""
+    sc = scenes.hill(n=40_000, seed=14)
+    big = scenes.cornell(n=400_000, seed=15)
+    h = N.Scene(ctx, sc.gaussians, sc.parts, emitters=sc.emitters)
+    hb = N.Scene(ctx, big.gaussians, big.parts, emitters=big.emitters)
+    h.write_colour(np.concatenate([sc.gaussians["albedo"], np.ones((sc.count, 1))], 1).astype(np.float32))
+    budget = 40 << 20
+    rng = np.random.default_rng(6)
+    reqs = []
+    for k in range(32):
+        w, hh = [(128, 96), (320, 180), (640, 360)][k % 3]
+        eye = scenes.orbit_eye(sc, rng.uniform(2.0, 5.0), rng.uniform(0, 2 * math.pi), rng.uniform(0.2, 0.6))
+        reqs.append((w, hh, sc.frame_bytes(width=w, height=hh, eye=eye)))
+    outs = [ctx.alloc(w * hh * 16) for w, hh, _ in reqs]
+    srv = N.Server(ctx, hbm_budget=budget, lanes=3, max_batch_views=8)
+    try:
+        with pytest.raises(N.SplatlightError) as e:
+            srv.submit(hb, big.frame_bytes(width=1920, height=1080), 1920, 1080, outs[0])
+        assert e.value.status == 7
+        ids = {srv.submit(h, fb, w, hh, outs[k]): k for k, (w, hh, fb) in enumerate(reqs)}
+        done, peak = [], 0
+        t0 = time.perf_counter()
+        while len(done) < len(reqs) and time.perf_counter() - t0 < 60:
+            peak = max(peak, srv.info()[2])
+            done += srv.poll(wait_us=200)
+        assert len(done) == len(reqs) and all(r.status == 0 for r in done)
+        log(f"  budget {budget >> 20} MiB: peak lane workspaces {peak / 2**20:.1f} MiB, "
+            f"{len({(r.start_s, r.done_s) for r in done})} batches")
+        assert 0 < peak <= budget
+        imgs = [ctx.read(o, w * hh * 16, np.uint8) for o, (w, hh, _) in zip(outs, reqs)]
+    finally:
+        srv.close()
+    try:
+        for r in done:
+            k = ids[r.id]
+            w, hh, fb = reqs[k]
+            N.frame_encode(ctx, h, fb, w, hh, outs[k])
+            N.frame_finish(ctx)
+            assert np.array_equal(ctx.read(outs[k], w * hh * 16, np.uint8), imgs[k]), f"request {k}"
+    finally:
+        for o in outs:
+            ctx.free(o)
+        h.destroy()
+        hb.destroy()
