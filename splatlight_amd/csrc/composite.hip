@@ -48,6 +48,75 @@ __device__ __noinline__ float exact_alpha(float dx, float dy, const Splat *__res
   return 1.0f - (float)exp((double)(-tau0 * e));
 }
 
+// One wave's sweep over a staged batch of nb splats (front to back): cull 64 at a time against the wave's block,
+// then two survivors per step. A pixel is live while !(T < t_eps) (pixels outside the view start at T = 0, so they
+// never are); a step no live lane takes is skipped, and the exact re-evaluation near the coverage edge is one
+// wave-uniform branch for both splats. OVERDRAW counts the fragments instead of blending them.
+template <bool OVERDRAW>
+__device__ __forceinline__ void sweep_batch(uint32_t nb, uint32_t lane, const StagedSplat *s_sp, const float4 *s_col,
+                                            const SplatBounds *s_bd, const uint32_t *s_idx,
+                                            const Splat *__restrict__ splats, float nx, float ny, float wx0,
+                                            float wx1, float wy0, float wy1, float t_eps, float &T, float &cr,
+                                            float &cg, float &cb, float &nfrag) {
+  const float alpha_min = 1.0f / 255.0f;
+  for (uint32_t k0 = 0; k0 < nb; k0 += 64u) {
+    if (__ballot(!(T < t_eps)) == 0ull) return;       // every pixel of this wave has stopped
+    const uint32_t kk = k0 + lane;
+    bool hit = false;
+    if (kk < nb) {
+      const SplatBounds b = s_bd[kk];
+      hit = b.x0 <= wx1 && b.x1 >= wx0 && b.y0 <= wy1 && b.y1 >= wy0;
+    }
+    uint64_t mask = __ballot(hit);
+    while (mask) {
+      const uint32_t ka = k0 + (uint32_t)__ffsll((unsigned long long)mask) - 1u;
+      mask &= mask - 1ull;
+      const bool two = mask != 0ull;
+      const uint32_t kb = two ? k0 + (uint32_t)__ffsll((unsigned long long)mask) - 1u : ka;
+      mask = two ? mask & (mask - 1ull) : mask;
+      const StagedSplat pa = s_sp[ka], pb = s_sp[kb];
+      const float4 ca = s_col[ka], cb4 = s_col[kb];
+      const float dxa = __fsub_rn(nx, pa.cx), dya = __fsub_rn(ny, pa.cy);
+      const float dxb = __fsub_rn(nx, pb.cx), dyb = __fsub_rn(ny, pb.cy);
+      const float ua = dxa * pa.ux + dya * pa.uy, va = dxa * pa.vx + dya * pa.vy;
+      const float ub = dxb * pb.ux + dyb * pb.uy, vb = dxb * pb.vx + dyb * pb.vy;
+      const float sa = ua * ua + va * va, sb = ub * ub + vb * vb;
+      bool ina = sa <= pa.hi, inb = two && sb <= pb.hi;  // within the band (or inside)
+      const bool live = !(T < t_eps);
+      float alpha_a = 1.0f - __builtin_amdgcn_exp2f(ca.w * __builtin_amdgcn_exp2f(-sa));
+      float alpha_b = 1.0f - __builtin_amdgcn_exp2f(cb4.w * __builtin_amdgcn_exp2f(-sb));
+      const bool xa = live && ina && sa >= pa.lo, xb = live && inb && sb >= pb.lo;
+      if (__ballot(xa || xb) != 0ull) {                  // q in the band around 1: decide exactly (rare)
+        if (xa) {
+          alpha_a = exact_alpha(dxa, dya, splats + s_idx[ka]);
+          ina = alpha_a >= alpha_min;
+        }
+        if (xb) {
+          alpha_b = exact_alpha(dxb, dyb, splats + s_idx[kb]);
+          inb = alpha_b >= alpha_min;
+        }
+      }
+      if (OVERDRAW) {
+        nfrag = live && ina ? nfrag + 1.0f : nfrag;
+        nfrag = live && inb ? nfrag + 1.0f : nfrag;
+        continue;
+      }
+      bool use = live && ina;
+      float wgt = T * alpha_a;
+      cr = use ? cr + wgt * ca.x : cr;
+      cg = use ? cg + wgt * ca.y : cg;
+      cb = use ? cb + wgt * ca.z : cb;
+      T = use ? T - wgt : T;
+      use = !(T < t_eps) && inb;
+      wgt = T * alpha_b;
+      cr = use ? cr + wgt * cb4.x : cr;
+      cg = use ? cg + wgt * cb4.y : cg;
+      cb = use ? cb + wgt * cb4.z : cb;
+      T = use ? T - wgt : T;
+    }
+  }
+}
+
 template <uint32_t CP_BATCH_T, uint32_t TSH>
 __global__ void __launch_bounds__((1u << TSH) * (1u << TSH), 8)
 sl_composite(const ViewDev *__restrict__ views, const uint32_t *__restrict__ tile_base, uint32_t n_views,
@@ -78,15 +147,13 @@ sl_composite(const ViewDev *__restrict__ views, const uint32_t *__restrict__ til
   // the wave's block in NDC (pixel centres)
   const float wx0 = ((float)bx + 0.5f) * 2.0f / fw - 1.0f, wx1 = ((float)bx + 7.5f) * 2.0f / fw - 1.0f;
   const float wy1 = 1.0f - ((float)by + 0.5f) * 2.0f / fh, wy0 = 1.0f - ((float)by + 7.5f) * 2.0f / fh;
-  const float alpha_min = 1.0f / 255.0f;
   const float cmax = __uint_as_float(view_counters[v].color_max_bits);
   const float t_eps = 0x1p-12f / fmaxf(1.0f, cmax);
   const bool overdraw = f.view_mode == 2u;
   const bool depth_view = f.view_mode == 1u;
 
   const uint2 rg = ranges[gtile];
-  float T = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f, nfrag = 0.0f;
-  bool done = !inside;
+  float T = inside ? 1.0f : 0.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f, nfrag = 0.0f;   // outside: never live
   for (uint32_t end = rg.y; end > rg.x; end -= min(end - rg.x, CP_BATCH_T)) {
     const uint32_t nb = min(end - rg.x, CP_BATCH_T);
     if (threadIdx.x < nb) {
@@ -129,66 +196,13 @@ sl_composite(const ViewDev *__restrict__ views, const uint32_t *__restrict__ til
       s_idx[threadIdx.x] = g;
     }
     __syncthreads();
-    for (uint32_t k0 = 0; k0 < nb; k0 += 64u) {
-      if (__ballot(!done) == 0ull) break;               // wave vote: every pixel of this wave has stopped
-      // cull 64 splats at once against this wave's block
-      const uint32_t kk = k0 + lane;
-      bool hit = false;
-      if (kk < nb) {
-        const SplatBounds b = s_bd[kk];
-        hit = b.x0 <= wx1 && b.x1 >= wx0 && b.y0 <= wy1 && b.y1 >= wy0;
-      }
-      uint64_t mask = __ballot(hit);
-      // two splats per step: their coverage does not depend on the transmittance, so both are computed before
-      // they are blended, in order; every lane runs the same straight-line code, and a lane that has stopped, or
-      // whose q is beyond the band (alpha < 1/255), keeps its state through the selects
-      while (mask) {
-        const uint32_t ka = k0 + (uint32_t)__ffsll((unsigned long long)mask) - 1u;
-        mask &= mask - 1ull;
-        const bool two = mask != 0ull;
-        const uint32_t kb = two ? k0 + (uint32_t)__ffsll((unsigned long long)mask) - 1u : ka;
-        if (two) mask &= mask - 1ull;
-        const StagedSplat pa = s_sp[ka], pb = s_sp[kb];
-        const float4 ca = s_col[ka], cb4 = s_col[kb];
-        const float dxa = __fsub_rn(nx, pa.cx), dya = __fsub_rn(ny, pa.cy);
-        const float dxb = __fsub_rn(nx, pb.cx), dyb = __fsub_rn(ny, pb.cy);
-        const float ua = dxa * pa.ux + dya * pa.uy, va = dxa * pa.vx + dya * pa.vy;
-        const float ub = dxb * pb.ux + dyb * pb.uy, vb = dxb * pb.vx + dyb * pb.vy;
-        const float sa = ua * ua + va * va, sb = ub * ub + vb * vb;
-        bool ina = sa <= pa.hi, inb = two && sb <= pb.hi;   // within the band (or inside)
-        if (__ballot(!done && (ina || inb)) == 0ull) continue;   // no lane of the wave takes either
-        float alpha_a = 1.0f - __builtin_amdgcn_exp2f(ca.w * __builtin_amdgcn_exp2f(-sa));
-        float alpha_b = 1.0f - __builtin_amdgcn_exp2f(cb4.w * __builtin_amdgcn_exp2f(-sb));
-        if (!done && ina && sa >= pa.lo) {                // q in the band around 1: decide exactly (rare)
-          alpha_a = exact_alpha(dxa, dya, splats + s_idx[ka]);
-          ina = alpha_a >= alpha_min;
-        }
-        if (!done && inb && sb >= pb.lo) {
-          alpha_b = exact_alpha(dxb, dyb, splats + s_idx[kb]);
-          inb = alpha_b >= alpha_min;
-        }
-        if (overdraw) {
-          nfrag = !done && ina ? nfrag + 1.0f : nfrag;
-          nfrag = !done && inb ? nfrag + 1.0f : nfrag;
-          continue;
-        }
-        bool use = !done && ina;
-        float wgt = T * alpha_a;
-        cr = use ? cr + wgt * ca.x : cr;
-        cg = use ? cg + wgt * ca.y : cg;
-        cb = use ? cb + wgt * ca.z : cb;
-        T = use ? T - wgt : T;
-        done = T < t_eps;
-        use = !done && inb;
-        wgt = T * alpha_b;
-        cr = use ? cr + wgt * cb4.x : cr;
-        cg = use ? cg + wgt * cb4.y : cg;
-        cb = use ? cb + wgt * cb4.z : cb;
-        T = use ? T - wgt : T;
-        done = T < t_eps;
-      }
-    }
-    if (!__syncthreads_or(!done)) break;
+    if (overdraw)
+      sweep_batch<true>(nb, lane, s_sp, s_col, s_bd, s_idx, splats, nx, ny, wx0, wx1, wy0, wy1, t_eps, T, cr, cg, cb,
+                        nfrag);
+    else
+      sweep_batch<false>(nb, lane, s_sp, s_col, s_bd, s_idx, splats, nx, ny, wx0, wx1, wy0, wy1, t_eps, T, cr, cg, cb,
+                         nfrag);
+    if (!__syncthreads_or(!(T < t_eps))) break;
   }
   if (!inside) return;
   const float4 px4 = overdraw ? make_float4(nfrag, 0.0f, 0.0f, 0.0f) : make_float4(cr, cg, cb, 1.0f - T);
