@@ -473,21 +473,39 @@ __device__ __forceinline__ float shiny_lobe(const Shiny &r, const NodeL &nd, flo
   return iso_lobe_from(r.ks, phase_specular(r.Sd, r.So, wi, r.wo), damp);
 }
 
+// Rows of a per-Gaussian array copied into a visiting order: row t of dst = row order[t] of src, rows of `words`
+// 16 B words; one lane per word, so a wave writes contiguous memory.
+extern "C" __global__ void __launch_bounds__(256)
+sl_gather_rows(uint32_t n, const uint32_t *__restrict__ order, const uint4 *__restrict__ src,
+               uint4 *__restrict__ dst, uint32_t words) {
+  const size_t i = (size_t)blockIdx.x * 256u + threadIdx.x;
+  if (i >= (size_t)n * words) return;
+  const uint32_t t = (uint32_t)(i / words), k = (uint32_t)(i - (size_t)t * words);
+  dst[i] = src[(size_t)order[t] * words + k];
+}
+extern "C" __global__ void __launch_bounds__(256)
+sl_gather_f32(uint32_t n, const uint32_t *__restrict__ order, const float *__restrict__ src, float *__restrict__ dst) {
+  const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+  if (t < n) dst[t] = src[order[t]];
+}
+
 // One cut walk per lane. STACK: the walk's stack depth in LDS, 48 as the reference's; used for trees whose full
-// walk can hold more than 16 entries (sl_shade_emit_wave below takes the others).
+// walk can hold more than 16 entries (sl_shade_emit_wave below takes the others). Lane t shades Gaussian order[t]
+// (spatially coherent waves); its inputs r0..r3, nodes, vgrid and iso_norm are copies in that visiting order (row t),
+// so a wave reads them contiguously; lamp_T and the outputs are indexed by Gaussian.
 template <uint32_t STACK, bool SHINY>
 __global__ void __launch_bounds__(64)
 sl_shade_emit(FrameUniformsDev U, EmitParamsDev EP, const uint32_t *__restrict__ order,
               const float4 *__restrict__ emit_nodes, uint32_t n_nodes,
-              const float4 *__restrict__ g0, const float4 *__restrict__ g1, const float4 *__restrict__ g2,
-              const float4 *__restrict__ g3, uint32_t st, const float4 *__restrict__ nodes,
+              const float4 *__restrict__ r0, const float4 *__restrict__ r1, const float4 *__restrict__ r2,
+              const float4 *__restrict__ r3, uint32_t st, const float4 *__restrict__ nodes,
               const uint8_t *__restrict__ vgrid, const float4 *__restrict__ parts, const float *__restrict__ lamp_T,
               const uint32_t *__restrict__ materials, const float *__restrict__ iso_norm,
               float4 *__restrict__ colour, float4 *__restrict__ c0_out) {
   const uint32_t t = blockIdx.x * 64u + threadIdx.x;
   if (t >= U.gaussian_count) return;
-  const uint32_t g = order ? order[t] : t;
-  const float4 A = g0[g], Q = sl_quat(g1, st, g), S = sl_scale(g2, st, g), C = sl_albedo(g1, g2, g3, st, g);
+  const uint32_t g = order[t];
+  const float4 A = r0[t], Q = sl_quat(r1, st, t), S = sl_scale(r2, st, t), C = sl_albedo(r1, r2, r3, st, t);
   const uint32_t mi = __float_as_uint(C.w) & 0xFFFFu;
   const uint32_t mmax = max(U.material_count, 1u) - 1u;
   const uint32_t *mat = materials + 8 * (size_t)min(mi, mmax);
@@ -503,16 +521,16 @@ sl_shade_emit(FrameUniformsDev U, EmitParamsDev EP, const uint32_t *__restrict__
   const float s3 = S.x * S.y * S.z;
   const float detC = s3 * s3;
   const v3 wo = norm3(sub3(V3(U.eye[0], U.eye[1], U.eye[2]), P));
-  const NodeL nd = load_node(nodes, g);
+  const NodeL nd = load_node(nodes, t);
   Shiny sh{};
-  if constexpr (SHINY) sh = shiny_receiver(mat, mflags, EP.n5, Rq, xf_dir_inv(xf, wo), iso_norm[g]);
+  if constexpr (SHINY) sh = shiny_receiver(mat, mflags, EP.n5, Rq, xf_dir_inv(xf, wo), iso_norm[t]);
   // the Gaussian's 128 B visibility grid, read once into LDS as 32 words per lane in a [word][lane] layout (lanes
   // on distinct banks whatever cells they read): the walk looks up 4 cells per used node, and re-reading them from
   // memory costs a cache line each time once other waves have evicted it
   __shared__ uint32_t s_vg[32][64];
   uint32_t *vcol = &s_vg[0][threadIdx.x];
   {
-    const uint4 *src = (const uint4 *)(vgrid + 128 * (size_t)g);
+    const uint4 *src = (const uint4 *)(vgrid + 128 * (size_t)t);
 #pragma unroll
     for (int q = 0; q < 8; q++) {
       const uint4 v = src[q];
@@ -588,7 +606,7 @@ sl_shade_emit(FrameUniformsDev U, EmitParamsDev EP, const uint32_t *__restrict__
     acc = add3(acc, light);
   }
   const float T = lamp_T ? lamp_T[g] : 1.0f;
-  const float tk = T * iso_norm[g];
+  const float tk = T * iso_norm[t];
   v3 c = V3(C.x * acc.x * tk, C.y * acc.y * tk, C.z * acc.z * tk);
   if (c0_out) c0_out[g] = make_float4(c.x, c.y, c.z, 0.0f);
   if ((mflags & 1u) != 0u && ((mflags & 4u) == 0u || dot3(xf_dir_inv(xf, wo), Rq.c0) > 0.0f)) {
@@ -604,20 +622,21 @@ sl_shade_emit(FrameUniformsDev U, EmitParamsDev EP, const uint32_t *__restrict__
 // lane expands it, and a lane that uses a node sits out that node's subtree (it is active again once the walk pops
 // below the node's stack slot). Each lane so meets its own used nodes in its own order and sums them exactly as
 // sl_shade_emit does. The node index, the node's words and the stack (16 x 16-bit entries in four 64-bit words) are
-// the same for every lane, so they live in scalar registers; only the V-grid uses LDS (8 KiB per wave).
+// the same for every lane, so they live in scalar registers; only the V-grid uses LDS (8 KiB per wave). Inputs in
+// visiting order as for sl_shade_emit.
 template <bool SHINY>
 __global__ void __launch_bounds__(64)
 sl_shade_emit_wave(FrameUniformsDev U, EmitParamsDev EP, const uint32_t *__restrict__ order,
                    const float4 *__restrict__ emit_nodes, uint32_t n_nodes,
-                   const float4 *__restrict__ g0, const float4 *__restrict__ g1, const float4 *__restrict__ g2,
-                   const float4 *__restrict__ g3, uint32_t st, const float4 *__restrict__ nodes,
+                   const float4 *__restrict__ r0, const float4 *__restrict__ r1, const float4 *__restrict__ r2,
+                   const float4 *__restrict__ r3, uint32_t st, const float4 *__restrict__ nodes,
                    const uint8_t *__restrict__ vgrid, const float4 *__restrict__ parts,
                    const float *__restrict__ lamp_T, const uint32_t *__restrict__ materials,
                    const float *__restrict__ iso_norm, float4 *__restrict__ colour, float4 *__restrict__ c0_out) {
   const uint32_t t = blockIdx.x * 64u + threadIdx.x;
   const bool live = t < U.gaussian_count;
-  const uint32_t g = live ? (order ? order[t] : t) : 0u;   // a lane past the count reads Gaussian 0 and sits out
-  const float4 A = g0[g], Q = sl_quat(g1, st, g), S = sl_scale(g2, st, g), C = sl_albedo(g1, g2, g3, st, g);
+  const uint32_t ti = live ? t : 0u, g = order[ti];   // a lane past the count reads row 0 and sits out
+  const float4 A = r0[ti], Q = sl_quat(r1, st, ti), S = sl_scale(r2, st, ti), C = sl_albedo(r1, r2, r3, st, ti);
   const uint32_t mi = __float_as_uint(C.w) & 0xFFFFu;
   const uint32_t mmax = max(U.material_count, 1u) - 1u;
   const uint32_t *mat = materials + 8 * (size_t)min(mi, mmax);
@@ -634,13 +653,13 @@ sl_shade_emit_wave(FrameUniformsDev U, EmitParamsDev EP, const uint32_t *__restr
   const float s3 = S.x * S.y * S.z;
   const float detC = s3 * s3;
   const v3 wo = norm3(sub3(V3(U.eye[0], U.eye[1], U.eye[2]), P));
-  const NodeL nd = load_node(nodes, g);
+  const NodeL nd = load_node(nodes, ti);
   Shiny sh{};
-  if constexpr (SHINY) sh = shiny_receiver(mat, mflags, EP.n5, Rq, xf_dir_inv(xf, wo), iso_norm[g]);
+  if constexpr (SHINY) sh = shiny_receiver(mat, mflags, EP.n5, Rq, xf_dir_inv(xf, wo), iso_norm[ti]);
   __shared__ uint32_t s_vg[32][64];                    // as in sl_shade_emit
   uint32_t *vcol = &s_vg[0][threadIdx.x];
   {
-    const uint4 *src = (const uint4 *)(vgrid + 128 * (size_t)g);
+    const uint4 *src = (const uint4 *)(vgrid + 128 * (size_t)ti);
 #pragma unroll
     for (int q = 0; q < 8; q++) {
       const uint4 v = src[q];
@@ -729,7 +748,7 @@ sl_shade_emit_wave(FrameUniformsDev U, EmitParamsDev EP, const uint32_t *__restr
   }
   if (!lane_on) return;
   const float T = lamp_T ? lamp_T[g] : 1.0f;
-  const float tk = T * iso_norm[g];
+  const float tk = T * iso_norm[ti];
   v3 c = V3(C.x * acc.x * tk, C.y * acc.y * tk, C.z * acc.z * tk);
   if (c0_out) c0_out[g] = make_float4(c.x, c.y, c.z, 0.0f);
   if ((mflags & 1u) != 0u && ((mflags & 4u) == 0u || dot3(xf_dir_inv(xf, wo), Rq.c0) > 0.0f)) {
@@ -1220,7 +1239,7 @@ __device__ __forceinline__ m2 oct_jac(const LSParamsDev &LS, v3 w, v3 b0, v3 b1)
 
 // ---- A: emitter light opacity map (light_shadow.wgsl:191-286, 347-392) -------------------------------------------
 
-// the emitter map record of Gaussian g (its own part's layer); z = r + gz . dt
+// the emitter map record of the Gaussian in row g of the given streams and nodes (its own part's layer); z = r + gz . dt
 __device__ MapRec lsm_rec(uint32_t g, const LSParamsDev &LS, const float4 *__restrict__ g0,
                           const float4 *__restrict__ g1, const float4 *__restrict__ g2, uint32_t st,
                           const float4 *__restrict__ nodes, const float4 *__restrict__ parts) {
@@ -1275,19 +1294,21 @@ __device__ MapRec lsm_rec(uint32_t g, const LSParamsDev &LS, const float4 *__res
   return o;
 }
 
+// Record slot t from row t of the streams and leaf nodes in the shading's visiting order (spatially coherent), so a
+// map tile's records, listed in slot order, lie close together in memory; the sums do not depend on slot order.
 extern "C" __global__ void __launch_bounds__(64)
-sl_lsm_prep(uint32_t n, LSParamsDev LS, const float4 *__restrict__ g0, const float4 *__restrict__ g1,
-            const float4 *__restrict__ g2, uint32_t st, const float4 *__restrict__ nodes,
+sl_lsm_prep(uint32_t n, LSParamsDev LS, const float4 *__restrict__ s0, const float4 *__restrict__ s1,
+            const float4 *__restrict__ s2, uint32_t st, const float4 *__restrict__ nodes,
             const float4 *__restrict__ parts, MapRec *__restrict__ recs, uint2 *__restrict__ tbox,
             uint64_t *__restrict__ tmask) {
-  const uint32_t g = blockIdx.x * 64u + threadIdx.x;
-  if (g >= n) return;
-  const MapRec r = lsm_rec(g, LS, g0, g1, g2, st, nodes, parts);
+  const uint32_t t = blockIdx.x * 64u + threadIdx.x;
+  if (t >= n) return;
+  const MapRec r = lsm_rec(t, LS, s0, s1, s2, st, nodes, parts);
   const uint2 b = map_tbox(r, (LS.res + MT - 1) / MT);
-  tbox[g] = b;
+  tbox[t] = b;
   if (b.x != 0xffffffffu) {   // an empty record is in no tile's list and never read
-    recs[g] = r;
-    tmask[g] = map_tile_mask(r);
+    recs[t] = r;
+    tmask[t] = map_tile_mask(r);
   }
 }
 
@@ -1366,26 +1387,29 @@ sl_sun_frame(SunParamsDev SM, const SunLayerDesc *__restrict__ desc, const float
   layers[l] = o;
 }
 
-// Gaussian g's records on the layers of its part: slot g * max_layers + j for the j-th layer, empty when the layer
-// is clean, its light is off or nothing reaches half a fixed-point unit; z = z0 - gz . dt, stored as z0 + (-gz) . dt
-// (the same float result: negation is exact).
+// The records of row t (the streams and leaf nodes in the shading's visiting order) on the layers of its part: slot
+// j n + t for the j-th layer, empty when the layer is clean, its light is off or nothing reaches half a fixed-point
+// unit; z = z0 - gz . dt, stored as z0 + (-gz) . dt (the same float result: negation is exact).
 extern "C" __global__ void __launch_bounds__(64)
 sl_sun_prep(uint32_t n, SunParamsDev SM, const SunLayerDesc *__restrict__ desc, const SunLayer *__restrict__ layers,
-            const float4 *__restrict__ g0, const float4 *__restrict__ g1, const float4 *__restrict__ g2,
+            const float4 *__restrict__ s0, const float4 *__restrict__ s1, const float4 *__restrict__ s2,
             uint32_t st, const float4 *__restrict__ nodes, const float4 *__restrict__ parts, uint32_t max_layers,
             MapRec *__restrict__ recs, uint2 *__restrict__ tbox, uint64_t *__restrict__ tmask) {
-  const uint32_t g = blockIdx.x * 64u + threadIdx.x;
-  if (g >= n) return;
-  MapRec *out = recs + (size_t)g * max_layers;
-  uint2 *obox = tbox + (size_t)g * max_layers;
-  const float4 A = g0[g], Q = sl_quat(g1, st, g), S4 = sl_scale(g2, st, g);
+  const uint32_t t = blockIdx.x * 64u + threadIdx.x;
+  if (t >= n) return;
+  // slot of the row's j-th layer: j n + t (layer-major, rows in visiting order: a map tile's records of one layer
+  // lie close together in memory)
+  MapRec *out = recs + t;
+  uint2 *obox = tbox + t;
+  uint64_t *omask = tmask + t;
+  const float4 A = s0[t], Q = sl_quat(s1, st, t), S4 = sl_scale(s2, st, t);
   const uint32_t p = __float_as_uint(S4.w) & 0xFFFFu;
   const uint32_t first = p < 32u ? SM.part_layers[2 * p] : 0u, cnt = p < 32u ? min(SM.part_layers[2 * p + 1], max_layers) : 0u;
-  for (uint32_t j = cnt; j < max_layers; j++) obox[j] = make_uint2(0xffffffffu, 0u);   // empty: in no tile's list
+  for (uint32_t j = cnt; j < max_layers; j++) obox[(size_t)j * n] = make_uint2(0xffffffffu, 0u);   // in no list
   if (cnt == 0u) return;
   const Xf xf = load_xf(parts, p);
   const m3 R = part_rot(xf);
-  const NodeL nd = load_node(nodes, g);
+  const NodeL nd = load_node(nodes, t);
   const v3 x = xf_point(xf, V3(A.x, A.y, A.z));
   const m3 C = leaf_cov(Q, S4, R);
   const int res = (int)SM.res;
@@ -1394,7 +1418,7 @@ sl_sun_prep(uint32_t n, SunParamsDev SM, const SunLayerDesc *__restrict__ desc, 
     const uint32_t l = first + j;
     const SunLayerDesc d = desc[l];
     if (d.dirty == 0u || SM.lights[d.light][3] < 0.0f) {
-      obox[j] = make_uint2(0xffffffffu, 0u);
+      obox[(size_t)j * n] = make_uint2(0xffffffffu, 0u);
       continue;
     }
     const SunLayer ly = layers[l];
@@ -1422,7 +1446,7 @@ sl_sun_prep(uint32_t n, SunParamsDev SM, const SunLayerDesc *__restrict__ desc, 
     const int hx = min((int)ceilf(3.0f * sqrtf(S.c0.x)), 32), hy = min((int)ceilf(3.0f * sqrtf(S.c1.y)), 32);
     const int cx = (int)floorf(ct.x), cy = (int)floorf(ct.y);
     if (tau0 * 4096.0f < 0.5f) {   // no texel reaches half a fixed-point unit
-      obox[j] = make_uint2(0xffffffffu, 0u);
+      obox[(size_t)j * n] = make_uint2(0xffffffffu, 0u);
       continue;
     }
     MapRec o;
@@ -1433,10 +1457,10 @@ sl_sun_prep(uint32_t n, SunParamsDev SM, const SunLayerDesc *__restrict__ desc, 
     o.x0 = max(cx - hx, 0); o.x1 = min(cx + hx, res - 1);
     o.y0 = max(cy - hy, 0); o.y1 = min(cy + hy, res - 1);
     const uint2 b = map_tbox(o, tiles_y);
-    obox[j] = b;
+    obox[(size_t)j * n] = b;
     if (b.x != 0xffffffffu) {   // an empty record is in no tile's list and never read
-      out[j] = o;
-      tmask[(size_t)g * max_layers + j] = map_tile_mask(o);
+      out[(size_t)j * n] = o;
+      omask[(size_t)j * n] = map_tile_mask(o);
     }
   }
 }

@@ -102,6 +102,16 @@ struct sl_scene {
   float *iso_norm = nullptr;
   bool iso_done = false;
   uint32_t *order = nullptr;      // per-Gaussian kernels visit the Gaussians in this order (spatially coherent waves)
+  // the shading's and map records' per-Gaussian inputs copied into that order (row t = Gaussian order[t]): the
+  // record streams as stored, the leaf nodes, the V-grids and iso_norm; formed at the lighting encode after any of
+  // them changed (valid false)
+  struct Visit {
+    float4 *g[4] = {nullptr, nullptr, nullptr, nullptr};
+    float4 *nodes = nullptr;
+    uint8_t *vgrid = nullptr;
+    float *iso = nullptr;
+    bool valid = false;
+  } visit;
   float *lamp_T = nullptr;
   float4 *env_T = nullptr;
   Grow emit_nodes, lsm_acc, lsm_mip, lsm_dirty, sun_acc, sun_mip, sun_desc, sun_layers, sun_dirty;
@@ -1184,6 +1194,7 @@ extern "C" sl_status sl_scene_write_gaussians(sl_context *ctx, sl_scene *sc, con
   // what was formed from the records is formed again: iso_norm at the next lighting encode, the bounce's leaf
   // weights at the next bounce encode (as after a part-table write)
   sc->iso_done = false;
+  sc->visit.valid = false;
   sc->parts_version++;
   return SL_OK;
 }
@@ -1721,6 +1732,7 @@ extern "C" sl_status sl_lighting_iso_init(sl_context *ctx, sl_scene *sc) {
   if (s != SL_OK) return s;
   SL_HIP(ctx, hipStreamSynchronize(ctx->stream));
   sc->iso_done = true;
+  sc->visit.valid = false;
   return SL_OK;
 }
 
@@ -1929,6 +1941,34 @@ void run_opacity_maps(hipStream_t st, float4 *mip, uint32_t n_layers, uint32_t r
 
 }  // namespace
 
+namespace {
+// The visiting-order copies (sl_scene::Visit), allocated on first use and formed on the context's stream.
+sl_status visit_build(sl_context *ctx, sl_scene *sc) {
+  sl_scene::Visit &v = sc->visit;
+  const uint32_t n = sc->count;
+  void *p = nullptr;
+  sl_status s = SL_OK;
+  for (int k = 0; k < 4 && s == SL_OK; k++)
+    if (sc->g[k] && !v.g[k] && (s = scene_alloc(ctx, sc, (size_t)n * 16, &p)) == SL_OK) v.g[k] = (float4 *)p;
+  if (s == SL_OK && !v.nodes && (s = scene_alloc(ctx, sc, (size_t)n * 96, &p)) == SL_OK) v.nodes = (float4 *)p;
+  if (s == SL_OK && !v.vgrid && (s = scene_alloc(ctx, sc, (size_t)n * 128, &p)) == SL_OK) v.vgrid = (uint8_t *)p;
+  if (s == SL_OK && !v.iso && (s = scene_alloc(ctx, sc, (size_t)n * 4, &p)) == SL_OK) v.iso = (float *)p;
+  if (s != SL_OK) return s;
+  auto rows = [&](const void *src, void *dst, uint32_t words) {
+    sl_gather_rows<<<grid_for((size_t)n * words, 256, 1u << 30), 256, 0, ctx->stream>>>(
+        n, sc->order, (const uint4 *)src, (uint4 *)dst, words);
+  };
+  for (int k = 0; k < 4; k++)
+    if (sc->g[k]) rows(sc->g[k], v.g[k], 1);
+  rows(sc->nodes, v.nodes, 6);
+  rows(sc->vgrid, v.vgrid, 8);
+  sl_gather_f32<<<(n + 255) / 256, 256, 0, ctx->stream>>>(n, sc->order, sc->iso_norm, v.iso);
+  if ((s = check_launch(ctx, "visiting-order copies")) != SL_OK) return s;
+  v.valid = true;
+  return SL_OK;
+}
+}  // namespace
+
 extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_lighting_desc *d) {
   SL_LIVE(ctx);
   if (!sc || !d || !d->frame_uniforms || !d->emit_params || !d->emit_nodes)
@@ -1959,6 +1999,8 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
   hipStream_t st = ctx->stream;
   sl_status s;
   if (!sc->iso_done && (s = sl_lighting_iso_init(ctx, sc)) != SL_OK) return s;
+  if (!sc->visit.valid && (s = visit_build(ctx, sc)) != SL_OK) return s;
+  const sl_scene::Visit &vis = sc->visit;
   // Validate and size both shadow chains before the first launch (a grow synchronises and frees). The chains then
   // run at once: emitter map -> lamp_T -> shade on the context's stream, sun maps -> env_T on the side stream; they
   // meet before env_direct, the first kernel that reads both.
@@ -2091,8 +2133,8 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
     if (sun_any) {
       if (sun_rec)
         sl_sun_prep<<<gblocks, 64, 0, ss>>>(sc->count, SM, (const SunLayerDesc *)sc->sun_desc.p,
-                                            (const SunLayer *)sc->sun_layers.p, sc->g[0], sc->g[1], sc->g[2],
-                                            sc->storage, sc->nodes, sc->parts, max_layers,
+                                            (const SunLayer *)sc->sun_layers.p, vis.g[0], vis.g[1], vis.g[2],
+                                            sc->storage, vis.nodes, sc->parts, max_layers,
                                             (MapRec *)ctx->map[1].rec.p, (uint2 *)ctx->map[1].tbox.p,
                                             (uint64_t *)ctx->map[1].tmask.p);
     }
@@ -2129,7 +2171,7 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
         p = e;
       }
       if (sc->count)
-        sl_lsm_prep<<<gblocks, 64, 0, st>>>(sc->count, LS, sc->g[0], sc->g[1], sc->g[2], sc->storage, sc->nodes,
+        sl_lsm_prep<<<gblocks, 64, 0, st>>>(sc->count, LS, vis.g[0], vis.g[1], vis.g[2], sc->storage, vis.nodes,
                                             sc->parts, (MapRec *)ctx->map[0].rec.p, (uint2 *)ctx->map[0].tbox.p,
                                             (uint64_t *)ctx->map[0].tmask.p);
       map_bin(ctx, st, 0, sc->count, LS.part_count, LS.res, ls_span);
@@ -2148,9 +2190,9 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
   float4 *c0_out = sc->bounce.on ? sc->bounce.c0 : nullptr;
   const bool wave_walk = emit_stack_need((const uint8_t *)d->emit_nodes, d->emit_node_count, EP.root) <= 16u;
   auto shade = [&](auto kernel) {
-    kernel<<<gblocks, 64, 0, st>>>(U, EP, sc->order, (const float4 *)sc->emit_nodes.p, d->emit_node_count, sc->g[0],
-                                   sc->g[1], sc->g[2], sc->g[3], sc->storage, sc->nodes, sc->vgrid, sc->parts, lampT,
-                                   sc->materials, sc->iso_norm, sc->colour, c0_out);
+    kernel<<<gblocks, 64, 0, st>>>(U, EP, sc->order, (const float4 *)sc->emit_nodes.p, d->emit_node_count, vis.g[0],
+                                   vis.g[1], vis.g[2], vis.g[3], sc->storage, vis.nodes, vis.vgrid, sc->parts, lampT,
+                                   sc->materials, vis.iso, sc->colour, c0_out);
   };
   if (wave_walk)
     shiny ? shade(sl_shade_emit_wave<true>) : shade(sl_shade_emit_wave<false>);
@@ -2388,6 +2430,7 @@ extern "C" sl_status sl_bounce_init(sl_context *ctx, sl_scene *sc, const sl_boun
     sc->bytes -= (size_t)G * 96;
   }
   sc->nodes = (float4 *)nodes_new;
+  sc->visit.valid = false;
   B.node_count = N;
   B.kid_count = t->kid_count;
   B.max_depth = t->max_depth;
