@@ -31,14 +31,18 @@ struct __align__(16) SplatBounds {   // tight ellipse bounds (NDC), widened by a
   float x0, x1, y0, y1;
 };
 
-// The exact coverage of one fragment of splat s, read again from memory (the case is rare): the CPU
-// restatement's expression order, no contraction, correctly rounded exp; alpha, or -1 when the pixel centre is
-// outside the quad.
-__device__ __noinline__ float exact_alpha(float dx, float dy, const Splat *__restrict__ sp) {
+struct __align__(8) SplatExact {      // the splat's own axis, radius and density words, for the exact path
+  float a0x, a0y, a1x, a1y, rr, tau0;
+};
+
+// The exact coverage of one fragment of a splat from its own words (staged in LDS beside the fast form; the case
+// is rare): the CPU restatement's expression order, no contraction, correctly rounded exp; alpha, or -1 when the
+// pixel centre is outside the quad.
+__device__ __noinline__ float exact_alpha(float dx, float dy, const SplatExact *sp) {
 #pragma clang fp contract(off)
-  const Splat s = *sp;
-  const float a0x = s.center_axis0.z, a0y = s.center_axis0.w, a1x = s.axis1_r_tau.x, a1y = s.axis1_r_tau.y;
-  const float rr = s.axis1_r_tau.z, tau0 = s.axis1_r_tau.w;
+  const SplatExact s = *sp;
+  const float a0x = s.a0x, a0y = s.a0y, a1x = s.a1x, a1y = s.a1y;
+  const float rr = s.rr, tau0 = s.tau0;
   const float det = a0x * a1y - a1x * a0y;
   const float u = (dx * a1y - a1x * dy) / det;
   const float v = (a0x * dy - dx * a0y) / det;
@@ -54,8 +58,7 @@ __device__ __noinline__ float exact_alpha(float dx, float dy, const Splat *__res
 // wave-uniform branch for both splats. OVERDRAW counts the fragments instead of blending them.
 template <bool OVERDRAW>
 __device__ __forceinline__ void sweep_batch(uint32_t nb, uint32_t lane, const StagedSplat *s_sp, const float4 *s_col,
-                                            const SplatBounds *s_bd, const uint32_t *s_idx,
-                                            const Splat *__restrict__ splats, float nx, float ny, float wx0,
+                                            const SplatBounds *s_bd, const SplatExact *s_ex, float nx, float ny, float wx0,
                                             float wx1, float wy0, float wy1, float t_eps, float &T, float &cr,
                                             float &cg, float &cb, float &nfrag) {
   const float alpha_min = 1.0f / 255.0f;
@@ -88,11 +91,11 @@ __device__ __forceinline__ void sweep_batch(uint32_t nb, uint32_t lane, const St
       const bool xa = live && ina && sa >= pa.lo, xb = live && inb && sb >= pb.lo;
       if (__ballot(xa || xb) != 0ull) {                  // q in the band around 1: decide exactly (rare)
         if (xa) {
-          alpha_a = exact_alpha(dxa, dya, splats + s_idx[ka]);
+          alpha_a = exact_alpha(dxa, dya, s_ex + ka);
           ina = alpha_a >= alpha_min;
         }
         if (xb) {
-          alpha_b = exact_alpha(dxb, dyb, splats + s_idx[kb]);
+          alpha_b = exact_alpha(dxb, dyb, s_ex + kb);
           inb = alpha_b >= alpha_min;
         }
       }
@@ -126,7 +129,7 @@ sl_composite(const ViewDev *__restrict__ views, const uint32_t *__restrict__ til
   __shared__ StagedSplat s_sp[CP_BATCH_T];
   __shared__ float4 s_col[CP_BATCH_T];
   __shared__ SplatBounds s_bd[CP_BATCH_T];
-  __shared__ uint32_t s_idx[CP_BATCH_T];              // the splat's slot, for the exact re-evaluation
+  __shared__ SplatExact s_ex[CP_BATCH_T];             // the splat's own words, for the exact re-evaluation
 
   // workgroup b takes the b-th heaviest tile of the batch (sl_tile_order): longest-first over the XCDs
   const uint32_t gtile = tile_order[blockIdx.x];
@@ -193,14 +196,14 @@ sl_composite(const ViewDev *__restrict__ views, const uint32_t *__restrict__ til
       b.x0 = cx - ex; b.x1 = cx + ex; b.y0 = cy - ey; b.y1 = cy + ey;
       if (!ok) { b.x0 = 2.0f; b.x1 = -2.0f; }            // culled by every wave
       s_bd[threadIdx.x] = b;
-      s_idx[threadIdx.x] = g;
+      s_ex[threadIdx.x] = SplatExact{a0x, a0y, a1x, a1y, r, tau0};
     }
     __syncthreads();
     if (overdraw)
-      sweep_batch<true>(nb, lane, s_sp, s_col, s_bd, s_idx, splats, nx, ny, wx0, wx1, wy0, wy1, t_eps, T, cr, cg, cb,
+      sweep_batch<true>(nb, lane, s_sp, s_col, s_bd, s_ex, nx, ny, wx0, wx1, wy0, wy1, t_eps, T, cr, cg, cb,
                         nfrag);
     else
-      sweep_batch<false>(nb, lane, s_sp, s_col, s_bd, s_idx, splats, nx, ny, wx0, wx1, wy0, wy1, t_eps, T, cr, cg, cb,
+      sweep_batch<false>(nb, lane, s_sp, s_col, s_bd, s_ex, nx, ny, wx0, wx1, wy0, wy1, t_eps, T, cr, cg, cb,
                          nfrag);
     if (!__syncthreads_or(!(T < t_eps))) break;
   }
