@@ -341,7 +341,7 @@ size_t layout_workspace(uint8_t *b, uint32_t s_cap, uint32_t p_cap, uint32_t t_c
   const size_t t_words = wide ? 0 : (size_t)p_tiles * 256;
   const size_t o_ts0 = take(t_words * 4), o_ts1 = take(t_words * 4);
   const size_t o_ts2 = take(t_words * 4);
-  const size_t o_chunk = take((size_t)chunks * 8);
+  const size_t o_chunk = take(((size_t)chunks + 1) * 8);   // the pair emission's chunk first pairs and total
   const size_t o_ranges = take((size_t)t_cap * 8);
   const size_t o_torder = take((size_t)t_cap * 4);
   const size_t total = off;
@@ -586,10 +586,14 @@ sl_status enqueue_batch(sl_context *ctx, const Batch &b, bool timers, bool overl
   }
   if ((s = mark(3, st)) != SL_OK) return s;
   const uint32_t tb = b.tile_bits, tdig = 1u << tb;
+  // pair counts per 1,024-rank chunk, their scan (the chunks' first pairs and the pair total), the pairs
+  const uint32_t pe_chunks = std::max((n + PE_CHUNK - 1) / PE_CHUNK, 1u);
   auto pairs_emit = [&](auto kernel) {
-    kernel<<<grid_for(n, PE_CHUNK, 4u * ctx->cus), PE_THREADS, 0, st>>>(
-        w.order, w.tbox, nullptr, w.slot_base, w.tile_base, w.tiles_x, V, b.tile_passes, w.pairs_cap, w.counters, w.pk0,
-        w.pv0, w.chunk_status, w.tickets + 3, w.tile_hist);
+    sl_pairs_count<false><<<pe_chunks, PE_THREADS, 0, st>>>(w.order, w.tbox, nullptr, w.counters, w.chunk_status);
+    sl_pairs_scan<<<1, 1024, 0, st>>>(w.counters, w.chunk_status, w.pairs_cap);
+    kernel<<<pe_chunks, PE_THREADS, 0, st>>>(w.order, w.tbox, nullptr, w.slot_base, w.tile_base, w.tiles_x, V,
+                                             b.tile_passes, w.pairs_cap, w.counters, w.pk0, w.pv0, w.chunk_status,
+                                             w.tile_hist);
   };
   if (tb == 11u) pairs_emit(sl_pairs_emit<false, 11>); else pairs_emit(sl_pairs_emit<false, 8>);
   if ((s = mark(4, st)) != SL_OK) return s;
@@ -1799,7 +1803,7 @@ struct MapScratch {
   MapScratch(size_t n_rec, size_t cap, size_t n_tiles) {
     const size_t chunks = (n_rec + PE_CHUNK - 1) / PE_CHUNK;
     pass_words = (cap + 4095) / 4096 * 256;
-    tstat[0] = align_up(chunk + 2 * std::max<size_t>(chunks, 1), 64);
+    tstat[0] = align_up(chunk + 2 * (std::max<size_t>(chunks, 1) + 1), 64);
     tstat[1] = tstat[0] + pass_words;
     tstat[2] = tstat[1] + pass_words;
     touched = tstat[2] + pass_words;
@@ -1864,9 +1868,14 @@ void map_bin(sl_context *ctx, hipStream_t st, uint32_t which, size_t n_rec, uint
   if (!cleared) map_clear(ctx, st, which, n_rec, n_layers, res, span);
   uint32_t *k0 = (uint32_t *)m.k0.p, *v0 = (uint32_t *)m.v0.p;
   uint32_t *k1 = (uint32_t *)m.k1.p, *v1 = (uint32_t *)m.v1.p;
-  sl_pairs_emit<true, 8><<<grid_for(n_rec, PE_CHUNK, 4u * ctx->cus), PE_THREADS, 0, st>>>(
+  const uint32_t chunks = (uint32_t)std::max<size_t>((n_rec + PE_CHUNK - 1) / PE_CHUNK, 1);
+  uint64_t *chunk_pairs = (uint64_t *)(w + ms.chunk);
+  sl_pairs_count<true><<<chunks, PE_THREADS, 0, st>>>(nullptr, (const uint2 *)m.tbox.p, (const uint64_t *)m.tmask.p,
+                                                      counters, chunk_pairs);
+  sl_pairs_scan<<<1, 1024, 0, st>>>(counters, chunk_pairs, (uint32_t)sh.cap);
+  sl_pairs_emit<true, 8><<<chunks, PE_THREADS, 0, st>>>(
       nullptr, (const uint2 *)m.tbox.p, (const uint64_t *)m.tmask.p, w + ms.zeros, w + ms.zeros, w + ms.tiles_x, 1u, sh.tile_passes,
-      (uint32_t)sh.cap, counters, k0, v0, (uint64_t *)(w + ms.chunk), w + ms.tickets, w + ms.hist);
+      (uint32_t)sh.cap, counters, k0, v0, chunk_pairs, w + ms.hist);
   sl_radix_bases<8><<<sh.tile_passes, RS_THREADS, 0, st>>>(w + ms.hist);
   const uint32_t grid = std::max(std::min<uint32_t>((uint32_t)((sh.cap + 4095) / 4096), 4u * ctx->cus), 1u);
   uint32_t *kin = k0, *vin = v0, *kout = k1, *vout = v1;

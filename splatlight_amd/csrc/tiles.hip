@@ -4,20 +4,19 @@
 // (shaders/project.wgsl:314). Pairs are emitted in the depth order produced by the depth sort, so after a
 // stable sort by tile id every tile's list is that tile's slice of the reference's global back-to-front order.
 //
-// sl_pairs_emit is one pass over the visible ranks: persistent workgroups take 1024-rank chunks in ticket
-// order, scan the chunk's pair counts, get the chunk's global offset by a decoupled look-back (64-bit status
-// words), then emit the chunk's pairs cooperatively: lane j of the workgroup writes pair j, finding its rank by a
-// binary search over the chunk's offsets in LDS, so the key and value stores are contiguous. The tile-id digit
-// histograms for the tile sort are accumulated on the way.
+// Three launches over 1024-rank chunks, none of which waits on another workgroup: sl_pairs_count writes each chunk's
+// pair count, sl_pairs_scan (one workgroup) turns the counts into the chunks' first pair indices and the pair total,
+// sl_pairs_emit scans its chunk's counts again and emits the chunk's pairs cooperatively in windows of PE_WIN: every
+// rank whose first pair falls in the window marks that position with its rank, an inclusive max-scan over the
+// window gives every position its owner, and lane j writes pair j, so the key and value stores are contiguous. (A
+// single pass with chained chunk offsets spent a third of its time handing out tickets and waiting on its
+// predecessors.) The tile-id digit histograms for the tile sort are accumulated on the way.
 #include "sl_device.h"
 
-#define PE_THREADS 256
-#define PE_RPT 4
+#define PE_THREADS 512
+#define PE_RPT 2
 #define PE_CHUNK (PE_THREADS * PE_RPT)
-#define PE_FLAG_AGG (1ull << 62)
-#define PE_FLAG_INC (2ull << 62)
-#define PE_FLAG_MASK (3ull << 62)
-#define PE_VALUE_MASK ((1ull << 62) - 1ull)
+#define PE_WIN 2048   // pairs per emission window
 
 // The position of the k-th set bit (from 0) of m, which has more than k set bits.
 __device__ __forceinline__ uint32_t nth_set_bit(uint64_t m, uint32_t k) {
@@ -31,9 +30,113 @@ __device__ __forceinline__ uint32_t nth_set_bit(uint64_t m, uint32_t k) {
   return pos + (k >= (x & 1u) ? 1u : 0u);
 }
 
-// order == nullptr: rank r is slot r. Batches: a rank's slot belongs to the view v with slot_base[v] <= slot < slot_base[v + 1]; its pairs' tile ids
-// are global, tile_base[v] + ty * tiles_x[v] + tx. tile_passes (1 to 3) digit histograms of BITS-bit digits (the tile
-// sort's digit width) are built.
+// The pair count of rank r (< ranks): the tiles of its slot's box, or (MASKED) the box's tiles set in its mask.
+// order == nullptr: rank r is slot r.
+template <bool MASKED>
+__device__ __forceinline__ uint32_t pair_count(const uint32_t *__restrict__ order, const uint2 *__restrict__ tbox,
+                                               const uint64_t *__restrict__ tmask, uint32_t r, uint32_t ranks,
+                                               uint32_t &g, uint2 &b, uint64_t &m) {
+  g = 0;
+  b = make_uint2(0xffffffffu, 0u);
+  m = 0;
+  if (r >= ranks) return 0u;
+  g = order ? order[r] : r;
+  b = tbox[g];
+  if (b.x == 0xffffffffu) return 0u;
+  if (MASKED) {
+    m = tmask[g];
+    return (uint32_t)__popcll(m);
+  }
+  return ((b.y & 0xffffu) - (b.x & 0xffffu) + 1u) * ((b.y >> 16) - (b.x >> 16) + 1u);
+}
+
+// Exclusive scan of one value per thread over the workgroup (s_wsum: one word per wave); *total the sum.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *s_wsum, uint32_t *total) {
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o);
+    if (lane >= (uint32_t)o) x += y;
+  }
+  if (lane == 63u) s_wsum[w] = x;
+  __syncthreads();
+  uint32_t before = 0, t = 0;
+#pragma unroll
+  for (int k = 0; k < PE_THREADS / SL_WAVE; k++) {
+    before += (uint32_t)k < w ? s_wsum[k] : 0u;
+    t += s_wsum[k];
+  }
+  *total = t;
+  return before + x - v;
+}
+
+// Chunk c's pair count into chunk_pairs[c] (the visible ranks are counters->visible; chunks past them exit).
+template <bool MASKED>
+__global__ void __launch_bounds__(PE_THREADS)
+sl_pairs_count(const uint32_t *__restrict__ order, const uint2 *__restrict__ tbox, const uint64_t *__restrict__ tmask,
+               const FrameCounters *counters, uint64_t *__restrict__ chunk_pairs) {
+  __shared__ uint32_t s_wsum[PE_THREADS / SL_WAVE];
+  const uint32_t V = __hip_atomic_load(&counters->visible, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const uint32_t r0 = blockIdx.x * PE_CHUNK;
+  if (r0 >= V) return;
+  uint32_t sum = 0;
+#pragma unroll
+  for (int k = 0; k < PE_RPT; k++) {
+    uint32_t g;
+    uint2 b;
+    uint64_t m;
+    sum += pair_count<MASKED>(order, tbox, tmask, r0 + threadIdx.x * PE_RPT + k, V, g, b, m);
+  }
+  uint32_t total;
+  (void)block_excl_scan(sum, s_wsum, &total);
+  if (threadIdx.x == 0) chunk_pairs[blockIdx.x] = total;
+}
+
+// One workgroup: the chunks' pair counts (chunk_pairs, in place) become their first pair indices, and
+// chunk_pairs[chunks] the pair total; the total capped at the capacity and what overflowed go to the counters.
+extern "C" __global__ void __launch_bounds__(1024)
+sl_pairs_scan(FrameCounters *counters, uint64_t *__restrict__ chunk_pairs, uint32_t capacity) {
+  __shared__ uint64_t s_w[1024 / SL_WAVE];
+  __shared__ uint64_t s_carry;
+  const uint32_t V = __hip_atomic_load(&counters->visible, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const uint32_t chunks = (V + PE_CHUNK - 1u) / PE_CHUNK, lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  if (threadIdx.x == 0) s_carry = 0;
+  __syncthreads();
+  for (uint32_t c0 = 0; c0 < chunks; c0 += 1024u) {
+    const uint32_t c = c0 + threadIdx.x;
+    const uint64_t v = c < chunks ? chunk_pairs[c] : 0ull;
+    uint64_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint64_t y = __shfl_up(x, o);
+      if (lane >= (uint32_t)o) x += y;
+    }
+    if (lane == 63u) s_w[w] = x;
+    __syncthreads();
+    uint64_t before = s_carry, tot = 0;
+#pragma unroll
+    for (int k = 0; k < 1024 / SL_WAVE; k++) {
+      before += (uint32_t)k < w ? s_w[k] : 0ull;
+      tot += s_w[k];
+    }
+    if (c < chunks) chunk_pairs[c] = before + x - v;
+    __syncthreads();                                   // s_w and s_carry read before they change
+    if (threadIdx.x == 0) s_carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const uint64_t all = s_carry;
+    chunk_pairs[chunks] = all;
+    const uint32_t kept = all > capacity ? capacity : (uint32_t)all;
+    __hip_atomic_store(&counters->pairs, kept, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&counters->overflow, (uint32_t)(all - kept), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// Chunk c's pairs from its first pair index chunk_first[c]. Batches: a rank's slot belongs to the view v with
+// slot_base[v] <= slot < slot_base[v + 1]; its pairs' tile ids are global, tile_base[v] + ty * tiles_x[v] + tx.
+// tile_passes (1 to 3) digit histograms of BITS-bit digits (the tile sort's digit width) are built.
 // MASKED (the opacity-map splats, boxes of at most 64 tiles): only the box's tiles whose bit (row * box width +
 // column) is set in tmask[slot] get a pair; otherwise every tile of the box does.
 template <bool MASKED, uint32_t BITS>
@@ -41,134 +144,83 @@ __global__ void __launch_bounds__(PE_THREADS)
 sl_pairs_emit(const uint32_t *__restrict__ order, const uint2 *__restrict__ tbox, const uint64_t *__restrict__ tmask,
               const uint32_t *__restrict__ slot_base, const uint32_t *__restrict__ tile_base,
               const uint32_t *__restrict__ tiles_x, uint32_t n_views, uint32_t tile_passes, uint32_t capacity,
-              FrameCounters *counters, uint32_t *__restrict__ pair_keys, uint32_t *__restrict__ pair_vals,
-              uint64_t *chunk_status, uint32_t *ticket, uint32_t *__restrict__ tile_hist) {
+              const FrameCounters *counters, uint32_t *__restrict__ pair_keys, uint32_t *__restrict__ pair_vals,
+              const uint64_t *__restrict__ chunk_first, uint32_t *__restrict__ tile_hist) {
   __shared__ uint32_t s_start[PE_CHUNK + 1];
+  __shared__ uint16_t s_own[PE_WIN];                 // a window's owners: local rank + 1
+  __shared__ uint32_t s_wmax[PE_THREADS / SL_WAVE];
   __shared__ uint2 s_box[PE_CHUNK];
   __shared__ uint32_t s_g[PE_CHUNK];
   __shared__ uint64_t s_mask[MASKED ? PE_CHUNK : 1];
   constexpr uint32_t DIG = 1u << BITS;
   __shared__ uint32_t s_hist[3 * DIG];
   __shared__ uint32_t s_wsum[PE_THREADS / SL_WAVE];
-  __shared__ uint32_t s_chunk;
-  __shared__ uint64_t s_excl;
 
   const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
-  const uint32_t tb0 = tile_base[0], tx0v = tiles_x[0];   // the only view of a single frame
   const uint32_t V = __hip_atomic_load(&counters->visible, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  const uint32_t num_chunks = (V + PE_CHUNK - 1u) / PE_CHUNK;
+  const uint32_t c = blockIdx.x, r0 = c * PE_CHUNK;
+  if (r0 >= V) return;
+  const uint32_t tb0 = tile_base[0], tx0v = tiles_x[0];   // the only view of a single frame
   for (uint32_t k = tid; k < 3u * DIG; k += PE_THREADS) s_hist[k] = 0u;
-
-  for (;;) {
-    if (tid == 0) s_chunk = atomicAdd(ticket, 1u);
-    __syncthreads();
-    const uint32_t c = s_chunk;
-    if (c >= num_chunks) break;
-    const uint32_t r0 = c * PE_CHUNK;
-    // per-thread counts of PE_RPT consecutive ranks
-    uint32_t cnt[PE_RPT], sum = 0;
+  uint32_t cnt[PE_RPT], first[PE_RPT], sum = 0;
 #pragma unroll
-    for (int k = 0; k < PE_RPT; k++) {
-      const uint32_t lr = tid * PE_RPT + k;
-      const uint32_t r = r0 + lr;
-      uint32_t n = 0;
-      uint2 b = make_uint2(0xffffffffu, 0u);
-      uint32_t g = 0;
-      if (r < V) {
-        g = order ? order[r] : r;
-        b = tbox[g];
-        if (b.x != 0xffffffffu) {
-          if (MASKED) {
-            const uint64_t m = tmask[g];
-            s_mask[lr] = m;
-            n = (uint32_t)__popcll(m);
-          } else {
-            n = ((b.y & 0xffffu) - (b.x & 0xffffu) + 1u) * ((b.y >> 16) - (b.x >> 16) + 1u);
-          }
-        }
-      }
-      s_box[lr] = b;
-      s_g[lr] = g;
-      cnt[k] = n;
-      sum += n;
+  for (int k = 0; k < PE_RPT; k++) {
+    const uint32_t lr = tid * PE_RPT + k;
+    uint32_t g;
+    uint2 b;
+    uint64_t m;
+    cnt[k] = pair_count<MASKED>(order, tbox, tmask, r0 + lr, V, g, b, m);
+    if (MASKED) s_mask[lr] = m;
+    s_box[lr] = b;
+    s_g[lr] = g;
+    sum += cnt[k];
+  }
+  uint32_t total;
+  uint32_t run = block_excl_scan(sum, s_wsum, &total);
+#pragma unroll
+  for (int k = 0; k < PE_RPT; k++) { first[k] = run; s_start[tid * PE_RPT + k] = run; run += cnt[k]; }
+  if (tid == 0) s_start[PE_CHUNK] = total;
+  const uint64_t base = chunk_first[c];
+  uint32_t carry = 0;                                   // the owner (local rank + 1) of the pair before the window
+  for (uint32_t w0 = 0; w0 < total; w0 += PE_WIN) {
+    for (uint32_t p = tid; p < PE_WIN; p += PE_THREADS) s_own[p] = 0;
+    __syncthreads();                                    // (also: the chunk's ranks are in LDS)
+#pragma unroll
+    for (int k = 0; k < PE_RPT; k++)                   // a rank marks the position of its first pair
+      if (cnt[k] != 0u && first[k] >= w0 && first[k] - w0 < PE_WIN)
+        s_own[first[k] - w0] = (uint16_t)(tid * PE_RPT + k + 1u);
+    __syncthreads();
+    // inclusive max-scan of the marks (ranks ascend with their first pair, so the running maximum is the owner):
+    // each thread scans PER consecutive entries, then the threads' maxima are scanned across the workgroup
+    constexpr uint32_t PER = PE_WIN / PE_THREADS;
+    uint32_t own[PER], mx = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < PER; i++) {
+      mx = max(mx, (uint32_t)s_own[PER * tid + i]);
+      own[i] = mx;
     }
-    // block exclusive scan of the per-thread sums
-    uint32_t x = sum;
+    uint32_t xm = mx;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
-      const uint32_t y = __shfl_up(x, o);
-      if (lane >= (uint32_t)o) x += y;
+      const uint32_t y = __shfl_up(xm, o);
+      if (lane >= (uint32_t)o) xm = max(xm, y);
     }
-    if (lane == 63u) s_wsum[w] = x;
+    if (lane == 63u) s_wmax[w] = xm;
+    const uint32_t prev_lane = __shfl_up(xm, 1);
     __syncthreads();
-    uint32_t before = 0, total = 0;
+    uint32_t pre = max(carry, lane > 0u ? prev_lane : 0u);
 #pragma unroll
-    for (int k = 0; k < PE_THREADS / SL_WAVE; k++) {
-      before += (uint32_t)k < w ? s_wsum[k] : 0u;
-      total += s_wsum[k];
-    }
-    uint32_t run = before + x - sum;
+    for (int k = 0; k < PE_THREADS / SL_WAVE; k++) pre = (uint32_t)k < w ? max(pre, s_wmax[k]) : pre;
 #pragma unroll
-    for (int k = 0; k < PE_RPT; k++) { s_start[tid * PE_RPT + k] = run; run += cnt[k]; }
-    if (w == 0) {
-      // decoupled look-back over chunks, 64 predecessors per round: lane l reads chunk c - 1 - l - 64 k; the nearest
-      // inclusive prefix ends the walk, and every chunk between must have published its aggregate
-      uint64_t *st = chunk_status + c;
-      uint64_t excl = 0;
-      if (c == 0) {
-        if (lane == 0) __hip_atomic_store(st, PE_FLAG_INC | (uint64_t)total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      } else {
-        if (lane == 0) __hip_atomic_store(st, PE_FLAG_AGG | (uint64_t)total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        int64_t top = (int64_t)c - 1;
-        for (;;) {
-          const int64_t j = top - (int64_t)lane;
-          const uint64_t sv = j >= 0 ? __hip_atomic_load(chunk_status + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                                     : PE_FLAG_INC;   // before chunk 0: an inclusive zero
-          const uint64_t fl = sv & PE_FLAG_MASK;
-          const uint64_t inc = __ballot(fl == PE_FLAG_INC), none = __ballot(fl == 0ull);
-          const uint32_t first = inc ? (uint32_t)__builtin_ctzll(inc) : 64u;   // the nearest inclusive prefix
-          const uint64_t before = first >= 64u ? ~0ull : ((2ull << first) - 1ull);
-          if (none & before) {                       // a chunk in between has not published yet
-            __builtin_amdgcn_s_sleep(1);
-            continue;
-          }
-          uint64_t v = lane <= first ? (sv & PE_VALUE_MASK) : 0ull;
-#pragma unroll
-          for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-          excl += v;
-          if (first < 64u) break;
-          top -= 64;
-        }
-        if (lane == 0) __hip_atomic_store(st, PE_FLAG_INC | (excl + total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
-      if (lane == 0) {
-        s_start[PE_CHUNK] = total;
-        s_excl = excl;
-        if (c == num_chunks - 1u) {
-          const uint64_t all = excl + total;
-          const uint32_t kept = all > capacity ? capacity : (uint32_t)all;
-          __hip_atomic_store(&counters->pairs, kept, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          __hip_atomic_store(&counters->overflow, (uint32_t)(all - kept), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-      }
-    }
+    for (uint32_t i = 0; i < PER; i++) s_own[PER * tid + i] = (uint16_t)max(pre, own[i]);
     __syncthreads();
-    const uint64_t base = s_excl;
-    // cooperative emission: lane j writes pair j of the chunk
-    uint32_t lo = 0;                                    // the last rank with start <= j, carried: j only grows
-    for (uint32_t j = tid; j < total; j += PE_THREADS) {
-      // gallop up from the last owner, then halve (s_start[PE_CHUNK] = total > j ends both): a rank's pairs are
-      // consecutive, so a lane's owner moves on by a few ranks where pairs are dense (the heavy chunks)
-      uint32_t step = 1u, hi = lo + 1u;
-      while (s_start[hi] <= j) {
-        lo = hi;
-        step <<= 1;
-        hi = min(lo + step, (uint32_t)PE_CHUNK);
-      }
-      while (hi - lo > 1u) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (s_start[mid] <= j) lo = mid; else hi = mid;
-      }
+    carry = s_own[PE_WIN - 1];
+    const uint32_t w1 = min(total, w0 + PE_WIN);
+#pragma unroll
+    for (uint32_t i = 0; i < PER; i++) {
+      const uint32_t j = w0 + tid + PE_THREADS * i;
+      if (j >= w1) break;
+      const uint32_t lo = (uint32_t)s_own[j - w0] - 1u;
       const uint2 b = s_box[lo];
       const uint32_t local = MASKED ? nth_set_bit(s_mask[lo], j - s_start[lo]) : j - s_start[lo];
       const uint32_t tx0 = b.x & 0xffffu, ty0 = b.x >> 16;
@@ -208,7 +260,7 @@ sl_pairs_emit(const uint32_t *__restrict__ order, const uint2 *__restrict__ tbox
         }
       }
     }
-    __syncthreads();
+    __syncthreads();                                    // the window's owners are read before the next clear
   }
   __syncthreads();
   for (uint32_t k = tid; k < DIG * tile_passes; k += PE_THREADS)
