@@ -541,19 +541,42 @@ def main():
         N.frame_encode(ctx, scene, fb, W, H, out, fmt, timers=frame_timers)
         return N.frame_finish(ctx)
 
-    for _ in range(a.warmup):
-        st = lit_frame(False)
+    def light():
+        scene.light(inp["U"], inp["EP"], inp["emit_nodes"], inp["LS"], inp["ENV"], inp["SM"], inp["desc"],
+                    timers=True)
+
+    def frames(n, record=None):
+        # n relit frames as a render loop issues them: frame i+1's lighting is encoded before frame i is finished,
+        # so it runs beside frame i's composite (the composite reads nothing the lighting writes)
+        if n <= 0:
+            return None
+        light()
+        N.frame_encode(ctx, scene, fb, W, H, out, fmt)
+        st, t = None, time.perf_counter()
+        for i in range(n):
+            if i + 1 < n:
+                light()
+            st = N.frame_finish(ctx)
+            if record is not None:
+                record(time.perf_counter() - t)
+                t = time.perf_counter()
+            if i + 1 < n:
+                N.frame_encode(ctx, scene, fb, W, H, out, fmt)
+        return st
+
+    frames(a.warmup)
     ctx.sync()
-    # timed: the lighting's pass timers (the map sums' HIP events for the roofline) are recorded live; the frame
-    # path's pass timers come from a few untimed frames after, so the timed frames carry only the events they use
+    # timed: the lighting's pass timers (the map sums' HIP events for the roofline) are recorded live in every
+    # frame and read after each frame's finish (the last completed lighting's); the frame path's pass timers come
+    # from a few untimed frames after, so the timed frames carry only the events they use
     frame_ms, passes = [], {}
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        t = time.perf_counter()
-        st = lit_frame(False)
-        frame_ms.append((time.perf_counter() - t) * 1e3)
+
+    def record(dt):
+        frame_ms.append(dt * 1e3)
         for k, v in N.timers(ctx).items():
             passes.setdefault(k, []).append(v)
+    t0 = time.perf_counter()
+    st = frames(a.steps, record)
     ctx.sync()
     total = time.perf_counter() - t0
     frame_passes = {}

@@ -195,13 +195,17 @@ struct sl_context {
   FrameCounters *counters_host = nullptr;   // pinned
   hipEvent_t ev[kPasses + 1] = {};
   hipEvent_t ev_comp = nullptr;   // before the composite when the geometry ran beside the lighting
-  hipEvent_t lev[5] = {};         // lighting: shadow, shade, sun, env
-  hipEvent_t mev[4] = {};         // the two map-sum launches (emitter map, sun maps): begin, end
+  // a lighting encode's timer events, two sets used in turn so that the set of the last completed encode can be
+  // read while the next one runs: lev the context stream's (shadow, shade, sun wait, env), mev the two map-sum
+  // launches (emitter map, sun maps: begin, end), sev the side stream's begin and end
+  struct LightTimers {
+    hipEvent_t lev[5] = {}, mev[4] = {}, sev[2] = {};
+    bool valid = false, map_timed[2] = {false, false}, sun_timed = false;
+  } lt[2];
+  uint32_t lt_cur = 0;            // the set of the last lighting encode
   hipEvent_t bev[4] = {};         // bounce: begin, before probe_gather, after it, end
   bool bounce_timed = false;
-  bool map_timed[2] = {false, false};
   bool timers_valid = false;
-  bool light_timers_valid = false;
   float ms[kPasses] = {};
   // the last batch (a frame is a batch of one), for sl_*_finish (re-run on overflow) and sl_frame_read
   bool has_last = false;
@@ -232,14 +236,22 @@ struct sl_context {
   // the sun-map chain's stream (created on first use, default priority), the fork/join events that order it with
   // the main stream, and its begin/end timer events
   hipStream_t side = nullptr;
-  hipEvent_t fork = nullptr, join = nullptr, cleared = nullptr, sev[2] = {};
-  bool sun_timed = false;
+  hipEvent_t fork = nullptr, join = nullptr, cleared = nullptr;
   // the frame geometry's stream (created with the side stream), the event before the last lighting encode that the
   // geometry waits for, the event after it that the context's stream waits for; light_pending: a lighting encode
   // since the last batch, so the next batch's geometry may run beside it
   hipStream_t geo = nullptr;
   hipEvent_t pre_light = nullptr, geo_done = nullptr;
   bool light_pending = false, timed_overlap = false;
+  // the composite of a frame encoded after a lighting runs on its own stream once the splats' colours are written
+  // (colours_done), so the next lighting encode, which writes nothing the composite reads, runs beside it;
+  // comp_done follows the composite and the counters' read-back. comp_pending: the context's and the geometry's
+  // streams have not waited for comp_done yet (every entry point but the lighting encode makes them wait first).
+  // light_seq counts lighting encodes; last_light_seq is the count when the last batch was encoded.
+  hipStream_t comp = nullptr;
+  hipEvent_t colours_done = nullptr, comp_done = nullptr;
+  bool comp_pending = false, comp_last = false;
+  uint64_t light_seq = 0, last_light_seq = 0;
 };
 
 namespace {
@@ -261,10 +273,25 @@ sl_status hip_fail(sl_context *ctx, hipError_t e, const char *what) {
     if (e_ != hipSuccess) return hip_fail((ctx), e_, #call); \
   } while (0)
 
-#define SL_LIVE(ctx)                                                                              \
+// The context's and the geometry's streams wait for the last composite that ran on its own stream.
+void join_composite(sl_context *ctx) {
+  if (!ctx->comp_pending) return;
+  (void)hipStreamWaitEvent(ctx->stream, ctx->comp_done, 0);
+  if (ctx->geo) (void)hipStreamWaitEvent(ctx->geo, ctx->comp_done, 0);
+  ctx->comp_pending = false;
+}
+
+// Every entry point: a live context, then the last composite joined (SL_LIVE_BESIDE for the lighting encode only,
+// which may run beside that composite).
+#define SL_LIVE_BESIDE(ctx)                                                                       \
   do {                                                                                            \
     if (!(ctx)) return SL_ERR_INVALID;                                                            \
     if ((ctx)->poisoned != SL_OK) return SL_ERR_POISONED;                                         \
+  } while (0)
+#define SL_LIVE(ctx)                                                                              \
+  do {                                                                                            \
+    SL_LIVE_BESIDE(ctx);                                                                          \
+    join_composite(ctx);                                                                          \
   } while (0)
 
 sl_status check_launch(sl_context *ctx, const char *what) {
@@ -411,6 +438,7 @@ sl_status ensure_workspace(sl_context *ctx, uint32_t slots, uint32_t pairs, uint
   if (w.base) {
     SL_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (ctx->geo) SL_HIP(ctx, hipStreamSynchronize(ctx->geo));
+    if (ctx->comp) SL_HIP(ctx, hipStreamSynchronize(ctx->comp));
     SL_HIP(ctx, hipFree(w.base));
     w.base = nullptr;
   }
@@ -617,6 +645,9 @@ sl_status enqueue_batch(sl_context *ctx, const Batch &b, bool timers, bool overl
   sl_tile_order<<<1, 1024, 0, st>>>(w.ranges, b.total_tiles, w.tile_order);
   if (V > 1) sl_view_pairs<<<V, 256, 0, st>>>(w.ranges, w.tile_base, w.view_counters);   // one view: the total
   if ((s = mark(6, st)) != SL_OK) return s;
+  // overlapped: the composite on its own stream once the colours are in the splats (the context's stream is then
+  // free for the next lighting encode, which writes nothing the composite reads)
+  hipStream_t cs = ms;
   if (overlap) {
     SL_HIP(ctx, hipEventRecord(ctx->geo_done, st));
     SL_HIP(ctx, hipStreamWaitEvent(ms, ctx->geo_done, 0));
@@ -624,10 +655,13 @@ sl_status enqueue_batch(sl_context *ctx, const Batch &b, bool timers, bool overl
     sl_splat_colour<<<b.total_slots / 256u, 256, 0, ms>>>(w.views, w.slot_base, V, w.keys, w.splats, w.partials);
     sl_frame_totals<<<V, 1024, 0, ms>>>(w.partials, w.slot_base, w.view_counters, w.counters, 2u);
     if (timers) SL_HIP(ctx, hipEventRecord(ctx->ev_comp, ms));
+    cs = ctx->comp;
+    SL_HIP(ctx, hipEventRecord(ctx->colours_done, ms));
+    SL_HIP(ctx, hipStreamWaitEvent(cs, ctx->colours_done, 0));
   }
   ctx->timed_overlap = overlap;
   auto composite = [&](auto kernel, uint32_t threads) {
-    kernel<<<b.total_tiles, threads, 0, ms>>>(w.views, w.tile_base, V, w.view_counters, w.splats, vin, w.ranges,
+    kernel<<<b.total_tiles, threads, 0, cs>>>(w.views, w.tile_base, V, w.view_counters, w.splats, vin, w.ranges,
                                               w.tile_order);
   };
   const bool big = b.tile_shift == 5u;
@@ -641,7 +675,7 @@ sl_status enqueue_batch(sl_context *ctx, const Batch &b, bool timers, bool overl
     default:
       big ? composite(sl_composite<256, 5>, 1024u) : composite(sl_composite<256, 4>, 256u);
   }
-  if ((s = mark(7, ms)) != SL_OK) return s;
+  if ((s = mark(7, cs)) != SL_OK) return s;
   return check_launch(ctx, "frame kernels");
 }
 
@@ -697,9 +731,16 @@ sl_status launch_batch(sl_context *ctx, const Batch &b) {
   } else if ((s = enqueue_batch(ctx, b, timers, overlap)) != SL_OK) {
     return s;
   }
-  SL_HIP(ctx, hipMemcpyAsync(ctx->counters_host, w.counters, sizeof(FrameCounters), hipMemcpyDeviceToHost, st));
+  const hipStream_t fin = overlap ? ctx->comp : st;   // the stream the batch ends on
+  SL_HIP(ctx, hipMemcpyAsync(ctx->counters_host, w.counters, sizeof(FrameCounters), hipMemcpyDeviceToHost, fin));
   SL_HIP(ctx, hipMemcpyAsync(ctx->view_counters_host, w.view_counters, sizeof(ViewCounters) * V,
-                             hipMemcpyDeviceToHost, st));
+                             hipMemcpyDeviceToHost, fin));
+  if (overlap) {
+    SL_HIP(ctx, hipEventRecord(ctx->comp_done, fin));
+    ctx->comp_pending = true;
+  }
+  ctx->comp_last = overlap;
+  ctx->last_light_seq = ctx->light_seq;
   ctx->timers_valid = timers;
   return SL_OK;
 }
@@ -768,10 +809,14 @@ extern "C" sl_status sl_ctx_create(int device_ordinal, sl_context **out) {
   for (int k = 0; k <= kPasses; k++)
     if ((e = hipEventCreate(&ctx->ev[k])) != hipSuccess) return bail(e);
   if ((e = hipEventCreate(&ctx->ev_comp)) != hipSuccess) return bail(e);
-  for (int k = 0; k < 5; k++)
-    if ((e = hipEventCreate(&ctx->lev[k])) != hipSuccess) return bail(e);
-  for (int k = 0; k < 4; k++)
-    if ((e = hipEventCreate(&ctx->mev[k])) != hipSuccess) return bail(e);
+  for (sl_context::LightTimers &t : ctx->lt) {
+    for (hipEvent_t &x : t.lev)
+      if ((e = hipEventCreate(&x)) != hipSuccess) return bail(e);
+    for (hipEvent_t &x : t.mev)
+      if ((e = hipEventCreate(&x)) != hipSuccess) return bail(e);
+    for (hipEvent_t &x : t.sev)
+      if ((e = hipEventCreate(&x)) != hipSuccess) return bail(e);
+  }
   for (int k = 0; k < 4; k++)
     if ((e = hipEventCreate(&ctx->bev[k])) != hipSuccess) return bail(e);
   *out = ctx;
@@ -784,14 +829,15 @@ extern "C" void sl_ctx_destroy(sl_context *ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   if (ctx->side) (void)hipStreamSynchronize(ctx->side);
   if (ctx->geo) (void)hipStreamSynchronize(ctx->geo);
+  if (ctx->comp) (void)hipStreamSynchronize(ctx->comp);
   if (ctx->ws.base) (void)hipFree(ctx->ws.base);
   for (int k = 0; k <= kPasses; k++)
     if (ctx->ev[k]) (void)hipEventDestroy(ctx->ev[k]);
   if (ctx->ev_comp) (void)hipEventDestroy(ctx->ev_comp);
-  for (int k = 0; k < 5; k++)
-    if (ctx->lev[k]) (void)hipEventDestroy(ctx->lev[k]);
-  for (int k = 0; k < 4; k++)
-    if (ctx->mev[k]) (void)hipEventDestroy(ctx->mev[k]);
+  for (sl_context::LightTimers &t : ctx->lt)
+    for (hipEvent_t x : {t.lev[0], t.lev[1], t.lev[2], t.lev[3], t.lev[4], t.mev[0], t.mev[1], t.mev[2], t.mev[3],
+                         t.sev[0], t.sev[1]})
+      if (x) (void)hipEventDestroy(x);
   for (int k = 0; k < 4; k++)
     if (ctx->bev[k]) (void)hipEventDestroy(ctx->bev[k]);
   if (ctx->counters_host) (void)hipHostFree(ctx->counters_host);
@@ -804,10 +850,12 @@ extern "C" void sl_ctx_destroy(sl_context *ctx) {
   if (ctx->map_stats.p) (void)hipFree(ctx->map_stats.p);
   for (Grow *gb : {&ctx->disp_q, &ctx->lod_flag, &ctx->lod_open, &ctx->lod_cnt, &ctx->lod_col})
     if (gb->p) (void)hipFree(gb->p);
-  for (hipEvent_t e : {ctx->fork, ctx->join, ctx->cleared, ctx->sev[0], ctx->sev[1], ctx->pre_light, ctx->geo_done})
+  for (hipEvent_t e : {ctx->fork, ctx->join, ctx->cleared, ctx->pre_light, ctx->geo_done, ctx->colours_done,
+                       ctx->comp_done})
     if (e) (void)hipEventDestroy(e);
   if (ctx->side) (void)hipStreamDestroy(ctx->side);
   if (ctx->geo) (void)hipStreamDestroy(ctx->geo);
+  if (ctx->comp) (void)hipStreamDestroy(ctx->comp);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -1000,6 +1048,7 @@ sl_status grow(sl_context *ctx, sl_scene *sc, Grow *gb, size_t bytes) {
     SL_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (ctx->side) SL_HIP(ctx, hipStreamSynchronize(ctx->side));
     if (ctx->geo) SL_HIP(ctx, hipStreamSynchronize(ctx->geo));
+    if (ctx->comp) SL_HIP(ctx, hipStreamSynchronize(ctx->comp));
     SL_HIP(ctx, hipFree(gb->p));
     sc->bytes -= gb->bytes;
     gb->p = nullptr;
@@ -1120,6 +1169,7 @@ extern "C" void sl_scene_destroy(sl_context *ctx, sl_scene *scene) {
   if (ctx && ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   if (ctx && ctx->side) (void)hipStreamSynchronize(ctx->side);
   if (ctx && ctx->geo) (void)hipStreamSynchronize(ctx->geo);
+  if (ctx && ctx->comp) (void)hipStreamSynchronize(ctx->comp);
   if (ctx && ctx->has_last && std::find(ctx->last.scenes.begin(), ctx->last.scenes.end(), scene) != ctx->last.scenes.end())
     ctx->has_last = false;
   scene_free(scene);
@@ -1171,6 +1221,7 @@ sl_status quiesce(sl_context *ctx) {
   SL_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (ctx->side) SL_HIP(ctx, hipStreamSynchronize(ctx->side));
   if (ctx->geo) SL_HIP(ctx, hipStreamSynchronize(ctx->geo));
+  if (ctx->comp) SL_HIP(ctx, hipStreamSynchronize(ctx->comp));
   return SL_OK;
 }
 }  // namespace
@@ -1354,7 +1405,9 @@ sl_status encode_batch(sl_context *ctx, const sl_view *views, uint32_t n_views, 
 // Wait for the last batch; on pair overflow grow the pair buffers and run it again; per-view stats.
 sl_status finish_batch(sl_context *ctx, sl_frame_stats *stats, uint32_t n_stats) {
   if (!ctx->has_last) return fail(ctx, SL_ERR_INVALID, "no frame encoded");
-  SL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  // the batch ends on the composite's stream when it ran there: a lighting encoded since may still run on the
+  // context's stream, and is not waited for
+  SL_HIP(ctx, hipStreamSynchronize(ctx->comp_last ? ctx->comp : ctx->stream));
   FrameCounters c = *ctx->counters_host;
   const Batch &b = ctx->last;
   if (c.overflow > 0 && b.fixed) {
@@ -1373,8 +1426,23 @@ sl_status finish_batch(sl_context *ctx, sl_frame_stats *stats, uint32_t n_stats)
     const uint64_t need = (uint64_t)c.pairs + c.overflow;
     const uint64_t grown = std::min<uint64_t>((need * 5) / 4 + 4096, 1u << 30);
     if (need > grown) return fail(ctx, SL_ERR_CAPACITY, "batch needs more than 2^30 pairs");
+    join_composite(ctx);
     sl_status s = ensure_workspace(ctx, b.total_slots, (uint32_t)grown, b.total_tiles, b.tile_bits);
     if (s != SL_OK) return s;
+    if (ctx->light_seq != ctx->last_light_seq) {
+      // a lighting encoded after this batch has rewritten the colours the batch was drawn with: the buffers are
+      // grown for the next batch, but this one is not drawn again
+      if (stats) {
+        memset(stats, 0, sizeof(*stats));
+        stats->slots = b.views[0].f.nslots;
+        stats->pairs = c.pairs;
+        stats->overflow = c.overflow;
+        stats->tiles = b.total_tiles;
+      }
+      return fail(ctx, SL_ERR_CAPACITY, std::to_string(need) + " pairs overflowed the pair buffers and a lighting "
+                                        "was encoded after the frame, so it is not re-run (the image is incomplete); "
+                                        "the buffers are grown for the next frame");
+    }
     if ((s = launch_batch(ctx, b)) != SL_OK) return s;
     SL_HIP(ctx, hipStreamSynchronize(ctx->stream));
     c = *ctx->counters_host;
@@ -1784,6 +1852,7 @@ sl_status grow_ctx(sl_context *ctx, Grow *gb, size_t bytes) {
     SL_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (ctx->side) SL_HIP(ctx, hipStreamSynchronize(ctx->side));
     if (ctx->geo) SL_HIP(ctx, hipStreamSynchronize(ctx->geo));   // the sun maps' clears run there
+    if (ctx->comp) SL_HIP(ctx, hipStreamSynchronize(ctx->comp));
     SL_HIP(ctx, hipFree(gb->p));
     gb->p = nullptr;
     gb->bytes = 0;
@@ -1898,7 +1967,7 @@ void map_sum(sl_context *ctx, hipStream_t st, uint32_t which, size_t n_rec, uint
   // the sorted pairs are in (k0, v0) after an even number of passes, else in (k1, v1)
   const uint32_t *keys = (const uint32_t *)(sh.tile_passes % 2u == 0u ? m.k0.p : m.k1.p);
   const uint32_t *vals = (const uint32_t *)(sh.tile_passes % 2u == 0u ? m.v0.p : m.v1.p);
-  if (timers) (void)hipEventRecord(ctx->mev[2 * which], st);
+  if (timers) (void)hipEventRecord(ctx->lt[ctx->lt_cur].mev[2 * which], st);
   const uint32_t chunks = (uint32_t)((sh.cap + MAP_CHUNK - 1) / MAP_CHUNK);
   uint32_t *touched = w + ms.touched;
   if (which == 0)
@@ -1908,8 +1977,8 @@ void map_sum(sl_context *ctx, hipStream_t st, uint32_t which, size_t n_rec, uint
     sl_map_accumulate<1><<<chunks, 256, 0, st>>>((const MapRec *)m.rec.p, keys, vals, counters, res, sh.tiles_x,
                                                  sh.tpl, acc, stats, touched);
   ctx->map_touched[which] = touched;
-  if (timers) (void)hipEventRecord(ctx->mev[2 * which + 1], st);
-  ctx->map_timed[which] = timers;
+  if (timers) (void)hipEventRecord(ctx->lt[ctx->lt_cur].mev[2 * which + 1], st);
+  ctx->lt[ctx->lt_cur].map_timed[which] = timers;
   ctx->map_records[which] = (uint32_t)n_rec;
   ctx->map_tiles[which] = sh.n_tiles;
 }
@@ -1979,7 +2048,7 @@ sl_status visit_build(sl_context *ctx, sl_scene *sc) {
 }  // namespace
 
 extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_lighting_desc *d) {
-  SL_LIVE(ctx);
+  SL_LIVE_BESIDE(ctx);
   if (!sc || !d || !d->frame_uniforms || !d->emit_params || !d->emit_nodes)
     return fail(ctx, SL_ERR_INVALID, "null argument");
   if (!sc->nodes || !sc->materials) return fail(ctx, SL_ERR_INVALID, "relighting needs the scene's nodes and materials");
@@ -2089,25 +2158,30 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
     SL_HIP(ctx, hipStreamCreateWithFlags(&ctx->geo, hipStreamNonBlocking));
     SL_HIP(ctx, hipEventCreateWithFlags(&ctx->pre_light, hipEventDisableTiming));
     SL_HIP(ctx, hipEventCreateWithFlags(&ctx->geo_done, hipEventDisableTiming));
+    SL_HIP(ctx, hipStreamCreateWithFlags(&ctx->comp, hipStreamNonBlocking));
+    SL_HIP(ctx, hipEventCreateWithFlags(&ctx->colours_done, hipEventDisableTiming));
+    SL_HIP(ctx, hipEventCreateWithFlags(&ctx->comp_done, hipEventDisableTiming));
   }
   if (have_sun && !ctx->side) {
     SL_HIP(ctx, hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
     SL_HIP(ctx, hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming));
     SL_HIP(ctx, hipEventCreateWithFlags(&ctx->join, hipEventDisableTiming));
     SL_HIP(ctx, hipEventCreateWithFlags(&ctx->cleared, hipEventDisableTiming));
-    for (hipEvent_t &e : ctx->sev) SL_HIP(ctx, hipEventCreate(&e));
   }
 
   const uint32_t gblocks = (sc->count + 63) / 64;
   const bool timers = (d->flags & SL_LIGHT_TIMERS) != 0;
-  ctx->map_timed[0] = ctx->map_timed[1] = false;
-  ctx->sun_timed = false;
+  ctx->lt_cur ^= 1u;                                    // this encode's timer set (the other may still be read)
+  sl_context::LightTimers &LT = ctx->lt[ctx->lt_cur];
+  LT.map_timed[0] = LT.map_timed[1] = false;
+  LT.sun_timed = false;
   // the next batch's geometry may run from here on, beside this lighting
   SL_HIP(ctx, hipEventRecord(ctx->pre_light, st));
   ctx->light_pending = true;
+  ctx->light_seq++;
   SL_HIP(ctx, hipMemcpyAsync(sc->emit_nodes.p, d->emit_nodes, (size_t)d->emit_node_count * 96, hipMemcpyHostToDevice, st));
   if (fresh_stats) sl_fill_u32<<<1, 256, 0, st>>>((uint32_t *)ctx->map_stats.p, 0u, 4u);
-  if (timers) SL_HIP(ctx, hipEventRecord(ctx->lev[0], st));
+  if (timers) SL_HIP(ctx, hipEventRecord(LT.lev[0], st));
   // side stream: sun maps -> env_T, after everything already on the context's stream. Enqueued before the emitter
   // map, whose short kernels then fill in around the sun maps' binning and sums (measured faster than ordering the
   // two chains' sums or the shading against each other with events).
@@ -2115,7 +2189,7 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
     hipStream_t ss = ctx->side;
     SL_HIP(ctx, hipEventRecord(ctx->fork, st));
     SL_HIP(ctx, hipStreamWaitEvent(ss, ctx->fork, 0));
-    if (timers) SL_HIP(ctx, hipEventRecord(ctx->sev[0], ss));
+    if (timers) SL_HIP(ctx, hipEventRecord(LT.sev[0], ss));
     const size_t texels = (size_t)SM.res * SM.res;
     sc->sun_res = SM.res;
     sc->sun_n = SM.n_layers;
@@ -2158,9 +2232,9 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
     sl_sun_lookup<<<gblocks, 64, 0, ss>>>(sc->count, SM, (const SunLayerDesc *)sc->sun_desc.p,
                                           (const SunLayer *)sc->sun_layers.p, sc->g[0], sc->g[2], sc->storage,
                                           sc->parts, (const float4 *)sc->sun_mip.p, sc->env_T);
-    if (timers) SL_HIP(ctx, hipEventRecord(ctx->sev[1], ss));
+    if (timers) SL_HIP(ctx, hipEventRecord(LT.sev[1], ss));
     SL_HIP(ctx, hipEventRecord(ctx->join, ss));
-    ctx->sun_timed = timers;
+    LT.sun_timed = timers;
   }
   // context's stream: the emitter light's opacity map and its lookup -> lamp_T, then the emitter shading
   const float *lampT = nullptr;
@@ -2193,7 +2267,7 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
                                           (const float4 *)sc->lsm_mip.p, sc->lamp_T);
     lampT = sc->lamp_T;
   }
-  if (timers) SL_HIP(ctx, hipEventRecord(ctx->lev[1], st));
+  if (timers) SL_HIP(ctx, hipEventRecord(LT.lev[1], st));
   // with a bounce set up, the shading also keeps each Gaussian's emitter light before its own emission (the light
   // direct_dirs_emit sends toward the probes) for the next sl_bounce_encode
   float4 *c0_out = sc->bounce.on ? sc->bounce.c0 : nullptr;
@@ -2207,7 +2281,7 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
     shiny ? shade(sl_shade_emit_wave<true>) : shade(sl_shade_emit_wave<false>);
   else
     shiny ? shade(sl_shade_emit<48, true>) : shade(sl_shade_emit<48, false>);
-  if (timers) SL_HIP(ctx, hipEventRecord(ctx->lev[2], st));
+  if (timers) SL_HIP(ctx, hipEventRecord(LT.lev[2], st));
   // distant lights: env_T (the side stream's, or 1 without sun maps), then env_direct
   if (d->env) {
     if (have_sun)
@@ -2215,15 +2289,15 @@ extern "C" sl_status sl_lighting_encode(sl_context *ctx, sl_scene *sc, const sl_
     else
       sl_fill_u32<<<grid_for((size_t)sc->count * 4, 256, 4096), 256, 0, st>>>((uint32_t *)sc->env_T, 0x3f800000u,
                                                                                sc->count * 4);
-    if (timers) SL_HIP(ctx, hipEventRecord(ctx->lev[3], st));
+    if (timers) SL_HIP(ctx, hipEventRecord(LT.lev[3], st));
     auto env = shiny ? sl_env_direct<true> : sl_env_direct<false>;
     env<<<gblocks, 64, 0, st>>>(U, ENV, sc->g[0], sc->g[1], sc->g[2], sc->g[3], sc->storage, sc->nodes, sc->vgrid,
                                 sc->parts, sc->materials, sc->iso_norm, sc->env_T, sc->colour);
   } else if (timers) {
-    SL_HIP(ctx, hipEventRecord(ctx->lev[3], st));
+    SL_HIP(ctx, hipEventRecord(LT.lev[3], st));
   }
-  if (timers) SL_HIP(ctx, hipEventRecord(ctx->lev[4], st));
-  ctx->light_timers_valid = timers;
+  if (timers) SL_HIP(ctx, hipEventRecord(LT.lev[4], st));
+  LT.valid = timers;
   sc->bounce.U = U;
   sc->bounce.ENV = ENV;
   sc->bounce.lit = true;
@@ -2614,28 +2688,36 @@ extern "C" sl_status sl_timers_read(sl_context *ctx, const char **names, float *
       if (names) names[k] = kPassNames[k];
       if (ms) ms[k] = ctx->ms[k];
     }
-  if (ctx->light_timers_valid) {
-    // the lighting passes of the last sl_lighting_encode with SL_LIGHT_TIMERS (PASSES names: shadow, shade).
+  // the last sl_lighting_encode's set, or the one before it while the last still runs (a frame's lighting is
+  // complete once its sl_frame_finish returned, while the next frame's lighting may already run)
+  uint32_t use = ctx->lt_cur;
+  if (ctx->lt[use].valid && ctx->lt[use ^ 1u].valid && hipEventQuery(ctx->lt[use].lev[4]) == hipErrorNotReady) {
+    (void)hipGetLastError();                           // not ready is no error: keep it from the next launch check
+    use ^= 1u;
+  }
+  const sl_context::LightTimers &LT = ctx->lt[use];
+  if (LT.valid) {
+    // the lighting passes of that sl_lighting_encode with SL_LIGHT_TIMERS (PASSES names: shadow, shade).
     // "sunmap" runs on the side stream alongside shadow and shade; "sun_wait" is the context stream's wait for it
     // before env_direct, and "lighting" the whole encode.
     static const char *const lnames[] = {"shadow", "shade", "sun_wait", "env_direct"};
-    SL_HIP(ctx, hipEventSynchronize(ctx->lev[4]));
+    SL_HIP(ctx, hipEventSynchronize(LT.lev[4]));
     for (int j = 0; j < 4 && k < cap; j++, k++) {
       float v = 0.0f;
-      SL_HIP(ctx, hipEventElapsedTime(&v, ctx->lev[j], ctx->lev[j + 1]));
+      SL_HIP(ctx, hipEventElapsedTime(&v, LT.lev[j], LT.lev[j + 1]));
       if (names) names[k] = lnames[j];
       if (ms) ms[k] = v;
     }
-    if (ctx->sun_timed && k < cap) {
+    if (LT.sun_timed && k < cap) {
       float v = 0.0f;
-      SL_HIP(ctx, hipEventElapsedTime(&v, ctx->sev[0], ctx->sev[1]));
+      SL_HIP(ctx, hipEventElapsedTime(&v, LT.sev[0], LT.sev[1]));
       if (names) names[k] = "sunmap";
       if (ms) ms[k] = v;
       k++;
     }
     if (k < cap) {
       float v = 0.0f;
-      SL_HIP(ctx, hipEventElapsedTime(&v, ctx->lev[0], ctx->lev[4]));
+      SL_HIP(ctx, hipEventElapsedTime(&v, LT.lev[0], LT.lev[4]));
       if (names) names[k] = "lighting";
       if (ms) ms[k] = v;
       k++;
@@ -2643,9 +2725,9 @@ extern "C" sl_status sl_timers_read(sl_context *ctx, const char **names, float *
     // the map-sum kernel alone (sl_map_accumulate), inside "shadow" and "sunmap"
     static const char *const mnames[] = {"lsm_sum", "sun_sum"};
     for (int j = 0; j < 2 && k < cap; j++)
-      if (ctx->map_timed[j]) {
+      if (LT.map_timed[j]) {
         float v = 0.0f;
-        SL_HIP(ctx, hipEventElapsedTime(&v, ctx->mev[2 * j], ctx->mev[2 * j + 1]));
+        SL_HIP(ctx, hipEventElapsedTime(&v, LT.mev[2 * j], LT.mev[2 * j + 1]));
         if (names) names[k] = mnames[j];
         if (ms) ms[k] = v;
         k++;
