@@ -132,7 +132,8 @@ typedef struct sl_frame_opts {
 /* Enqueue one frame: project, depth order, pair emission, tile sort, tile ranges, composite. out_device holds
  * width * height pixels of out_format, rows top-down. frame_bytes: the 1200 B RenderFrame. Encoded after
  * sl_lighting_encode, the frame's geometry stages run beside that lighting (they read nothing it writes) and only
- * the colours and the composite wait for it; the results are the same. */
+ * the colours and the composite wait for it, the composite on a stream of its own, so that the next lighting
+ * encode may run beside it; the results are the same. */
 sl_status sl_frame_encode(sl_context *ctx, sl_scene *scene, const void *frame_bytes, size_t frame_len,
                           const sl_frame_opts *opts, void *out_device);
 typedef struct sl_frame_stats {
@@ -144,7 +145,8 @@ typedef struct sl_frame_stats {
   uint32_t pad[3];
 } sl_frame_stats;
 /* Wait for the last frame and report its counts. Re-runs the frame once with grown pair buffers when the
- * pair count exceeded them. */
+ * pair count exceeded them; when a lighting was encoded after the frame (its colours are then gone) it grows the
+ * buffers for the next frame and returns SL_ERR_CAPACITY instead (the image is incomplete, stats filled). */
 sl_status sl_frame_finish(sl_context *ctx, sl_frame_stats *stats);
 
 /* sl_frame_encode for callers that cannot allocate device memory (code under HIP graph capture, tight serving
@@ -296,7 +298,9 @@ typedef struct sl_lighting_desc {
 /* iso_norm of every leaf from its V grid (runIsoInit, src/lighting.js:546-555); once per asset */
 sl_status sl_lighting_iso_init(sl_context *ctx, sl_scene *scene);
 /* the emitter and sun opacity maps of dirty parts / layers (LSParams.dirty, LayerDesc.dirty), their lookups,
- * shade_emit and env_direct, leaving the lit colour the next sl_frame_encode draws */
+ * shade_emit and env_direct, leaving the lit colour the next sl_frame_encode draws. It may be encoded before the
+ * last frame's sl_frame_finish (as a render loop encodes the next frame): it then runs beside that frame's
+ * composite, which reads nothing it writes. */
 sl_status sl_lighting_encode(sl_context *ctx, sl_scene *scene, const sl_lighting_desc *desc);
 
 /* Scene-resident lighting state, for parity checks: */
@@ -361,7 +365,8 @@ sl_status sl_bounce_read(sl_context *ctx, sl_scene *scene, uint32_t which, void 
  * (src/renderer.js:7-8) where one exists, then, after sl_lighting_encode with SL_LIGHT_TIMERS, its passes
  * "shadow", "shade", "sun_wait", "env_direct", "sunmap" (the sun-map chain, which runs on a second stream during
  * shadow and shade; sun_wait is what remains of it after shade), "lighting" (the whole encode) and the map-sum
- * kernels alone, "lsm_sum" and "sun_sum"; after sl_bounce_encode with SL_BOUNCE_TIMERS, "bounce" (the whole
+ * kernels alone, "lsm_sum" and "sun_sum" (of the last such encode, or of the one before it while the last still
+ * runs); after sl_bounce_encode with SL_BOUNCE_TIMERS, "bounce" (the whole
  * encode) and "probe_gather". Fills up to `cap` entries; returns the count in *n. */
 sl_status sl_timers_read(sl_context *ctx, const char **names, float *ms, uint32_t cap, uint32_t *n);
 
