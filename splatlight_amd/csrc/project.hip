@@ -386,16 +386,17 @@ sl_splat_colour(const ViewDev *__restrict__ views, const uint32_t *__restrict__ 
   }
 }
 
-// Per-view totals from project's per-block partials (one workgroup per view): visible count and colour max into
+// Per-view totals from project's per-block partials (one 256-lane workgroup per view: it starts on a CU with four free wave slots, which the map sums
+// running beside it leave sooner than sixteen): visible count and colour max into
 // view_counters[v]; the batch's visible total into counters. what: bit 0 the visible counts, bit 1 the colour max.
-extern "C" __global__ void __launch_bounds__(1024)
+extern "C" __global__ void __launch_bounds__(256)
 sl_frame_totals(const uint2 *__restrict__ partials, const uint32_t *__restrict__ slot_base,
                 ViewCounters *__restrict__ view_counters, FrameCounters *__restrict__ counters, uint32_t what) {
-  __shared__ uint2 s_w[1024 / SL_WAVE];
+  __shared__ uint2 s_w[256 / SL_WAVE];
   const uint32_t v = blockIdx.x;
   const uint32_t b0 = slot_base[v] / 256u, b1 = slot_base[v + 1] / 256u;
   uint2 t = make_uint2(0u, 0u);
-  for (uint32_t i = b0 + threadIdx.x; i < b1; i += 1024u) {
+  for (uint32_t i = b0 + threadIdx.x; i < b1; i += 256u) {
     const uint2 p = partials[i];
     t.x += p.x;
     t.y = max(t.y, p.y);
@@ -409,7 +410,7 @@ sl_frame_totals(const uint2 *__restrict__ partials, const uint32_t *__restrict__
   __syncthreads();
   if (threadIdx.x == 0) {
     uint2 a = s_w[0];
-    for (int k = 1; k < 1024 / SL_WAVE; k++) {
+    for (int k = 1; k < 256 / SL_WAVE; k++) {
       a.x += s_w[k].x;
       a.y = max(a.y, s_w[k].y);
     }

@@ -570,7 +570,7 @@ sl_status enqueue_batch(sl_context *ctx, const Batch &b, bool timers, bool overl
     overlap ? project(sl_project_kernel<false, true>) : project(sl_project_kernel<true, true>);
   else
     overlap ? project(sl_project_kernel<false, false>) : project(sl_project_kernel<true, false>);
-  sl_frame_totals<<<V, 1024, 0, st>>>(w.partials, w.slot_base, w.view_counters, w.counters, overlap ? 1u : 3u);
+  sl_frame_totals<<<V, 256, 0, st>>>(w.partials, w.slot_base, w.view_counters, w.counters, overlap ? 1u : 3u);
   if ((s = mark(2, st)) != SL_OK) return s;
   // depth order: stable sort of (view, 16-bit depth) keys, values = global slot
   const uint32_t n = b.total_slots;
@@ -618,7 +618,7 @@ sl_status enqueue_batch(sl_context *ctx, const Batch &b, bool timers, bool overl
   const uint32_t pe_chunks = std::max((n + PE_CHUNK - 1) / PE_CHUNK, 1u);
   auto pairs_emit = [&](auto kernel) {
     sl_pairs_count<false><<<pe_chunks, PE_THREADS, 0, st>>>(w.order, w.tbox, nullptr, w.counters, w.chunk_status);
-    sl_pairs_scan<<<1, 1024, 0, st>>>(w.counters, w.chunk_status, w.pairs_cap);
+    sl_pairs_scan<<<1, 256, 0, st>>>(w.counters, w.chunk_status, w.pairs_cap);
     kernel<<<pe_chunks, PE_THREADS, 0, st>>>(w.order, w.tbox, nullptr, w.slot_base, w.tile_base, w.tiles_x, V,
                                              b.tile_passes, w.pairs_cap, w.counters, w.pk0, w.pv0, w.chunk_status,
                                              w.tile_hist);
@@ -642,7 +642,7 @@ sl_status enqueue_batch(sl_context *ctx, const Batch &b, bool timers, bool overl
   ctx->final_vals = vin;
   if ((s = mark(5, st)) != SL_OK) return s;
   sl_tile_ranges<<<grid_for(w.pairs_cap, 256, 4096), 256, 0, st>>>(kin, w.counters, w.ranges);
-  sl_tile_order<<<1, 1024, 0, st>>>(w.ranges, b.total_tiles, w.tile_order);
+  sl_tile_order<<<1, 256, 0, st>>>(w.ranges, b.total_tiles, w.tile_order);
   if (V > 1) sl_view_pairs<<<V, 256, 0, st>>>(w.ranges, w.tile_base, w.view_counters);   // one view: the total
   if ((s = mark(6, st)) != SL_OK) return s;
   // overlapped: the composite on its own stream once the colours are in the splats (the context's stream is then
@@ -653,7 +653,7 @@ sl_status enqueue_batch(sl_context *ctx, const Batch &b, bool timers, bool overl
     SL_HIP(ctx, hipStreamWaitEvent(ms, ctx->geo_done, 0));
     if (lod_lit) lod_colour(ms);
     sl_splat_colour<<<b.total_slots / 256u, 256, 0, ms>>>(w.views, w.slot_base, V, w.keys, w.splats, w.partials);
-    sl_frame_totals<<<V, 1024, 0, ms>>>(w.partials, w.slot_base, w.view_counters, w.counters, 2u);
+    sl_frame_totals<<<V, 256, 0, ms>>>(w.partials, w.slot_base, w.view_counters, w.counters, 2u);
     if (timers) SL_HIP(ctx, hipEventRecord(ctx->ev_comp, ms));
     cs = ctx->comp;
     SL_HIP(ctx, hipEventRecord(ctx->colours_done, ms));
@@ -1941,7 +1941,7 @@ void map_bin(sl_context *ctx, hipStream_t st, uint32_t which, size_t n_rec, uint
   uint64_t *chunk_pairs = (uint64_t *)(w + ms.chunk);
   sl_pairs_count<true><<<chunks, PE_THREADS, 0, st>>>(nullptr, (const uint2 *)m.tbox.p, (const uint64_t *)m.tmask.p,
                                                       counters, chunk_pairs);
-  sl_pairs_scan<<<1, 1024, 0, st>>>(counters, chunk_pairs, (uint32_t)sh.cap);
+  sl_pairs_scan<<<1, 256, 0, st>>>(counters, chunk_pairs, (uint32_t)sh.cap);
   sl_pairs_emit<true, 8><<<chunks, PE_THREADS, 0, st>>>(
       nullptr, (const uint2 *)m.tbox.p, (const uint64_t *)m.tmask.p, w + ms.zeros, w + ms.zeros, w + ms.tiles_x, 1u, sh.tile_passes,
       (uint32_t)sh.cap, counters, k0, v0, chunk_pairs, w + ms.hist);

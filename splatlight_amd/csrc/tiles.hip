@@ -93,17 +93,17 @@ sl_pairs_count(const uint32_t *__restrict__ order, const uint2 *__restrict__ tbo
   if (threadIdx.x == 0) chunk_pairs[blockIdx.x] = total;
 }
 
-// One workgroup: the chunks' pair counts (chunk_pairs, in place) become their first pair indices, and
+// One 256-lane workgroup (small enough to start beside long-running kernels): the chunks' pair counts (chunk_pairs, in place) become their first pair indices, and
 // chunk_pairs[chunks] the pair total; the total capped at the capacity and what overflowed go to the counters.
-extern "C" __global__ void __launch_bounds__(1024)
+extern "C" __global__ void __launch_bounds__(256)
 sl_pairs_scan(FrameCounters *counters, uint64_t *__restrict__ chunk_pairs, uint32_t capacity) {
-  __shared__ uint64_t s_w[1024 / SL_WAVE];
+  __shared__ uint64_t s_w[256 / SL_WAVE];
   __shared__ uint64_t s_carry;
   const uint32_t V = __hip_atomic_load(&counters->visible, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   const uint32_t chunks = (V + PE_CHUNK - 1u) / PE_CHUNK, lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
   if (threadIdx.x == 0) s_carry = 0;
   __syncthreads();
-  for (uint32_t c0 = 0; c0 < chunks; c0 += 1024u) {
+  for (uint32_t c0 = 0; c0 < chunks; c0 += 256u) {
     const uint32_t c = c0 + threadIdx.x;
     const uint64_t v = c < chunks ? chunk_pairs[c] : 0ull;
     uint64_t x = v;
@@ -116,7 +116,7 @@ sl_pairs_scan(FrameCounters *counters, uint64_t *__restrict__ chunk_pairs, uint3
     __syncthreads();
     uint64_t before = s_carry, tot = 0;
 #pragma unroll
-    for (int k = 0; k < 1024 / SL_WAVE; k++) {
+    for (int k = 0; k < 256 / SL_WAVE; k++) {
       before += (uint32_t)k < w ? s_w[k] : 0ull;
       tot += s_w[k];
     }
@@ -298,7 +298,7 @@ sl_tile_ranges(const uint32_t *__restrict__ keys, const FrameCounters *counters,
 // light ones fill in behind them (greedy longest-first), and consecutive workgroups go to different XCDs, so
 // each XCD gets an even share of every cost class. Order within a bucket is arbitrary: a tile's pixels do not
 // depend on when it runs.
-extern "C" __global__ void __launch_bounds__(1024)
+extern "C" __global__ void __launch_bounds__(256)
 sl_tile_order(const uint2 *__restrict__ ranges, uint32_t ntiles, uint32_t *__restrict__ order) {
   __shared__ uint32_t s_cnt[64];
   __shared__ uint32_t s_next[64];
@@ -310,7 +310,7 @@ sl_tile_order(const uint2 *__restrict__ ranges, uint32_t ntiles, uint32_t *__res
     const uint32_t half = msb > 0u ? (c >> (msb - 1u)) & 1u : 0u;
     return 62u - min(61u, 2u * msb + half);         // heavier -> smaller bucket index
   };
-  for (uint32_t t = threadIdx.x; t < ntiles; t += 1024u) {
+  for (uint32_t t = threadIdx.x; t < ntiles; t += 256u) {
     const uint2 r = ranges[t];
     atomicAdd(&s_cnt[bucket(r.y - r.x)], 1u);
   }
@@ -320,7 +320,7 @@ sl_tile_order(const uint2 *__restrict__ ranges, uint32_t ntiles, uint32_t *__res
     for (int b = 0; b < 64; b++) { s_next[b] = run; run += s_cnt[b]; }
   }
   __syncthreads();
-  for (uint32_t t = threadIdx.x; t < ntiles; t += 1024u) {
+  for (uint32_t t = threadIdx.x; t < ntiles; t += 256u) {
     const uint2 r = ranges[t];
     order[atomicAdd(&s_next[bucket(r.y - r.x)], 1u)] = t;
   }
