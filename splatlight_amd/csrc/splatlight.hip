@@ -273,12 +273,19 @@ sl_status hip_fail(sl_context *ctx, hipError_t e, const char *what) {
     if (e_ != hipSuccess) return hip_fail((ctx), e_, #call); \
   } while (0)
 
-// The context's and the geometry's streams wait for the last composite that ran on its own stream.
-void join_composite(sl_context *ctx) {
-  if (!ctx->comp_pending) return;
-  (void)hipStreamWaitEvent(ctx->stream, ctx->comp_done, 0);
-  if (ctx->geo) (void)hipStreamWaitEvent(ctx->geo, ctx->comp_done, 0);
+// The context's and the geometry's streams wait for the last composite that ran on its own stream (a failed wait
+// poisons the context: later work would not be ordered after that composite).
+sl_status join_composite(sl_context *ctx) {
+  if (!ctx->comp_pending) return SL_OK;
+  hipError_t e = hipStreamWaitEvent(ctx->stream, ctx->comp_done, 0);
+  if (e == hipSuccess && ctx->geo) e = hipStreamWaitEvent(ctx->geo, ctx->comp_done, 0);
+  if (e != hipSuccess) {
+    ctx->poisoned = SL_ERR_DEVICE;
+    ctx->err = std::string("joining the composite's stream: ") + hipGetErrorString(e);
+    return SL_ERR_DEVICE;
+  }
   ctx->comp_pending = false;
+  return SL_OK;
 }
 
 // Every entry point: a live context, then the last composite joined (SL_LIVE_BESIDE for the lighting encode only,
@@ -291,7 +298,7 @@ void join_composite(sl_context *ctx) {
 #define SL_LIVE(ctx)                                                                              \
   do {                                                                                            \
     SL_LIVE_BESIDE(ctx);                                                                          \
-    join_composite(ctx);                                                                          \
+    if (join_composite(ctx) != SL_OK) return SL_ERR_DEVICE;                                       \
   } while (0)
 
 sl_status check_launch(sl_context *ctx, const char *what) {
@@ -1426,8 +1433,9 @@ sl_status finish_batch(sl_context *ctx, sl_frame_stats *stats, uint32_t n_stats)
     const uint64_t need = (uint64_t)c.pairs + c.overflow;
     const uint64_t grown = std::min<uint64_t>((need * 5) / 4 + 4096, 1u << 30);
     if (need > grown) return fail(ctx, SL_ERR_CAPACITY, "batch needs more than 2^30 pairs");
-    join_composite(ctx);
-    sl_status s = ensure_workspace(ctx, b.total_slots, (uint32_t)grown, b.total_tiles, b.tile_bits);
+    sl_status s = join_composite(ctx);
+    if (s != SL_OK) return s;
+    s = ensure_workspace(ctx, b.total_slots, (uint32_t)grown, b.total_tiles, b.tile_bits);
     if (s != SL_OK) return s;
     if (ctx->light_seq != ctx->last_light_seq) {
       // a lighting encoded after this batch has rewritten the colours the batch was drawn with: the buffers are
