@@ -115,6 +115,22 @@ def test_frame_matches_oracle(ctx, n, W, H, seed):
     assert frac <= 1e-3
 
 
+def test_pair_emission_many_windows(ctx):
+    # 2,000 large splats: about 650 pairs per visible rank, so each 1,024-rank chunk emits some 320 windows of 2,048
+    # pairs (owners carried across windows) and the first run overflows the pair buffers (grown, re-run)
+    sc = scenes.cornell(n=2_000, seed=5, width=640, height=480, scale_mu=-1.0)
+    col = lit_colour(sc, 5)
+    fb = sc.frame_bytes()
+    st, img, art = render(ctx, sc, col)
+    o_splats, o_keys, o_vis = oracle.project(fb, sc, colour=col)
+    _, _, o_order, _ = oracle.sort_keys(o_keys)
+    check_integer_artefacts(sc, st, art, o_keys, o_order, o_vis, o_splats)
+    log(f"  visible={st.visible} pairs={st.pairs} ({st.pairs / max(st.visible, 1):.0f} a rank)")
+    assert st.pairs > 300 * st.visible
+    ref = oracle.composite(fb, o_splats, o_order, o_vis, sc.width, sc.height, mode=0)
+    assert np.abs(img - ref).max() <= TOL_ABS and psnr(img, ref) >= MIN_PSNR
+
+
 @pytest.mark.parametrize("tile_px,bits", [(32, 8), (16, 11), (32, 11)])
 @pytest.mark.parametrize("n,W,H,seed", [(2000, 160, 120, 3), (200_000, 640, 480, 0), (1_000_000, 1920, 1080, 4)])
 def test_tile_size_and_digit_width(ctx, n, W, H, seed, tile_px, bits):
