@@ -302,9 +302,10 @@ def storage_leg(N, L, scenes, steps=5):
     return out
 
 
-def bounce_leg(N, L, scenes, sc, steps=8, warmup=2, half=True):
+def bounce_leg(N, L, scenes, sc, steps=16, warmup=3, half=True):
     """The hill frame with the sky and bounce through probes each frame (Lighting.encodeBounce after the lighting,
-    the frame adding gain x bounce): the bounce's GPU time, its probe gather, and the whole relit frame's rate."""
+    the frame adding gain x bounce): the bounce's GPU time, its probe gather, and the whole relit frame's rate with
+    the frames issued as a render loop issues them."""
     from splatlight_amd import bounce as BN
     from splatlight_amd import tree
     t0 = time.perf_counter()
@@ -331,17 +332,34 @@ def bounce_leg(N, L, scenes, sc, steps=8, warmup=2, half=True):
             bt.append(tm["bounce"])
             gt.append(tm["probe_gather"])
 
-    for _ in range(warmup):
-        frame(False)
+    def frames(n, record=None):
+        # n frames as a render loop issues them: frame i+1's lighting and bounce are encoded before frame i is
+        # finished (the bounce joins frame i's composite on the context's stream; the lighting runs beside it)
+        def lit():
+            scene.light(inp["U"], inp["EP"], inp["emit_nodes"], inp["LS"], inp["ENV"], inp["SM"], inp["desc"])
+            b.encode()
+        if n <= 0:
+            return
+        lit()
+        N.frame_encode(ctx, scene, fb, W, H, out, fmt)
+        t = time.perf_counter()
+        for i in range(n):
+            if i + 1 < n:
+                lit()
+            N.frame_finish(ctx)
+            if record is not None:
+                record.append((time.perf_counter() - t) * 1e3)
+                t = time.perf_counter()
+            if i + 1 < n:
+                N.frame_encode(ctx, scene, fb, W, H, out, fmt)
+
+    frames(warmup)
     ctx.sync()
     t1 = time.perf_counter()
-    for _ in range(steps):
-        t = time.perf_counter()
-        frame(False)
-        ft.append((time.perf_counter() - t) * 1e3)
+    frames(steps, ft)
     ctx.sync()
     total = time.perf_counter() - t1
-    for _ in range(3):
+    for _ in range(3):                     # the bounce's pass timers, from frames issued one after another
         frame(True)
     rays = b.n_probes * 128
     res = {"workload": f"hill {sc.count} Gaussians, tree of {tr.count} nodes, {b.n_probes} probes x 128 rays, "

@@ -79,6 +79,59 @@ def test_pipelined_frames_equal_frames_alone(ctx):
         scene.destroy()
 
 
+def test_pipelined_bounce_frames_equal_frames_alone(ctx):
+    # the next frame's lighting and probe bounce encoded before this frame's finish: the bounce joins the
+    # composite on the context's stream, so every frame equals the same lighting, bounce and frame issued one
+    # after another (each run on a fresh scene and bounce, whose probe rounds then match)
+    from splatlight_amd import bounce as BN
+    from splatlight_amd import tree
+    sc = scenes.hill(n=8_000, seed=4, width=320, height=240, shadow_res=256)
+    W, H = sc.width, sc.height
+    t = tree.scene_tree(sc)
+    eyes = [orbit(sc, k) for k in range(3)]
+    inps = [L.lighting_inputs(sc, eye=e, frame=k) for k, e in enumerate(eyes)]
+    fbs = [sc.frame_bytes(eye=e, bounce_gain=1.0) for e in eyes]
+
+    def run(piped):
+        scene = N.Scene(ctx, sc.gaussians, sc.parts, emitters=sc.emitters, materials=sc.materials,
+                        nodes=t.nodes[:sc.count], vgrid=scenes.vgrid_open(sc.count))
+        b = BN.Bounce(scene, sc, t)
+        outs = [ctx.alloc(W * H * 16) for _ in eyes]
+        imgs = []
+        try:
+            if not piped:
+                for inp, fb, o in zip(inps, fbs, outs):
+                    light(scene, inp)
+                    b.encode()
+                    N.frame_encode(ctx, scene, fb, W, H, o)
+                    N.frame_finish(ctx)
+                    imgs.append(ctx.read(o, W * H * 16, np.float32).copy())
+                return imgs, b.n_probes
+            light(scene, inps[0])
+            b.encode()
+            N.frame_encode(ctx, scene, fbs[0], W, H, outs[0])
+            for k in range(len(eyes)):
+                if k + 1 < len(eyes):
+                    light(scene, inps[k + 1])
+                    b.encode()
+                N.frame_finish(ctx)
+                imgs.append(ctx.read(outs[k], W * H * 16, np.float32).copy())
+                if k + 1 < len(eyes):
+                    N.frame_encode(ctx, scene, fbs[k + 1], W, H, outs[k + 1])
+            return imgs, b.n_probes
+        finally:
+            for o in outs:
+                ctx.free(o)
+            scene.destroy()
+
+    alone, n_probes = run(False)
+    piped, _ = run(True)
+    for k, (a, p) in enumerate(zip(alone, piped)):
+        assert np.array_equal(a, p), f"bounce frame {k} differs from the frame issued alone"
+    assert not np.array_equal(alone[0], alone[1])
+    log(f"  {len(eyes)} bounce frames, {sc.count} Gaussians, {n_probes} probes")
+
+
 def test_pipelined_overflow_is_reported_then_fits():
     # big splats: the first frame's pairs overflow a fresh context's pair buffers
     sc = scenes.cornell(n=2_000, seed=5, width=640, height=480, scale_mu=-1.0)
