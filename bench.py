@@ -378,12 +378,12 @@ def bounce_leg(N, L, scenes, sc, steps=16, warmup=3, half=True):
 def kernel_bytes(passes, P, T, target_bytes, ls):
     """Per-launch algorithmic HBM bytes and HIP-event milliseconds of the heaviest kernel launches of a relit frame.
 
-    sl_composite<256u, 4u> (one launch; 256 splats staged per round over 16x16-pixel tiles, the default tuning): a
+    sl_composite<256u, 4u, 4u> (one launch; 256 splats staged per round over 16x16-pixel tiles, the default tuning): a
     48 B splat fetch and a 4 B pair value per pair, 8 B per tile range, the target.
     sl_map_accumulate<0u> (emitter map) and <1u> (sun maps), one launch each: a 64 B record and a 4 B list entry
     per (record, map tile) pair, 4 B per tile key, 12 B per texel of every map tile that has records.
     """
-    out = [{"kernel": "sl_composite<256u, 4u>", "ms": float(np.mean(passes["splat"])),
+    out = [{"kernel": "sl_composite<256u, 4u, 4u>", "ms": float(np.mean(passes["splat"])),
             "bytes": 48 * P + 4 * P + 8 * T + target_bytes}]
     for kind, m, key in ((0, "lsm", "lsm_sum"), (1, "sun", "sun_sum")):
         if key in passes:
@@ -470,7 +470,7 @@ def pmc_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, out, comma
         k = max(v, key=lambda name: np.mean(v[name]))
         return nbytes / (np.mean(v[k]) * 1024.0), k, float(np.mean(v[k]))
 
-    keep = ("sl_composite<256u, 4u>", "sl_map_accumulate<0u>", "sl_map_accumulate<1u>", "sl_shade_emit_wave",
+    keep = ("sl_composite<256u, 4u, 4u>", "sl_map_accumulate<0u>", "sl_map_accumulate<1u>", "sl_shade_emit_wave",
             "sl_shade_emit<48u>")
     copy_bytes = 1024 << 20
     fs, fk, fv = copy_scale(copy_fetch_dir, "FETCH_SIZE", copy_bytes)

@@ -1,11 +1,14 @@
 // Front-to-back compositing of 16x16 or 32x32 tiles (the blend of shaders/splat.wgsl:22-78 with the "over" state
 // of src/renderer.js:140-142, evaluated per tile instead of by the rasteriser).
 //
-// One workgroup per tile (256 lanes for 16x16, 1,024 for 32x32: a batch staged once serves 4x the pixels),
-// launched heaviest tile first; each wave owns an 8x8 pixel block of it, one lane per pixel. The tile's pair range is walked from its end, which is the reference's back-to-front sequence for
-// every pixel reversed, in batches of CP_BATCH_T splats (64, 128 or 256, tuned): each lane stages one splat into
-// LDS already reduced to what a pixel needs, then each wave culls the batch against its 8x8 block (64 splats per ballot, tight ellipse bounds) and sweeps only the
-// survivors, in order. A pixel stops once its transmittance is below t_eps = 2^-12 / max(1, max|rgb|) (the
+// Workgroups of 256 lanes, each over a 16x16 pixel block: one per tile for 16 px tiles, four per tile (one per
+// quarter, each walking the whole tile's list) for 32 px tiles, whose binning has ~2.3x fewer pairs to emit and
+// sort; a 1,024-lane workgroup per 32 px tile, staging each batch once for 4x the pixels, composited 1.3x slower
+// (its 16 waves wait at every batch's barrier for the slowest block). Tiles are launched heaviest first; each wave
+// owns an 8x8 pixel block, one lane per pixel. The tile's pair range is walked from its end, which is the
+// reference's back-to-front sequence for every pixel reversed, in batches of CP_BATCH_T splats (64, 128 or 256,
+// tuned): each lane stages one splat into LDS already reduced to what a pixel needs, then each wave culls the batch
+// against its 8x8 block (64 splats per ballot, tight ellipse bounds) and sweeps only the survivors, in order. A pixel stops once its transmittance is below t_eps = 2^-12 / max(1, max|rgb|) (the
 // colour it no longer adds is below 2^-12 absolute), a wave stops when all its pixels stopped, and the
 // workgroup leaves when all waves did.
 //
@@ -120,8 +123,8 @@ __device__ __forceinline__ void sweep_batch(uint32_t nb, uint32_t lane, const St
   }
 }
 
-template <uint32_t CP_BATCH_T, uint32_t TSH>
-__global__ void __launch_bounds__((1u << TSH) * (1u << TSH), 8)
+template <uint32_t CP_BATCH_T, uint32_t TSH, uint32_t WSH>
+__global__ void __launch_bounds__((1u << WSH) * (1u << WSH), 8)
 sl_composite(const ViewDev *__restrict__ views, const uint32_t *__restrict__ tile_base, uint32_t n_views,
              const ViewCounters *__restrict__ view_counters, const Splat *__restrict__ splats,
              const uint32_t *__restrict__ vals, const uint2 *__restrict__ ranges,
@@ -132,16 +135,21 @@ sl_composite(const ViewDev *__restrict__ views, const uint32_t *__restrict__ til
   __shared__ SplatExact s_ex[CP_BATCH_T];             // the splat's own words, for the exact re-evaluation
 
   // workgroup b takes the b-th heaviest tile of the batch (sl_tile_order): longest-first over the XCDs
-  const uint32_t gtile = tile_order[blockIdx.x];
+  static_assert(WSH <= TSH, "a workgroup covers at most one tile");
+  constexpr uint32_t SUBS = 1u << (TSH - WSH);           // workgroups per tile side
+  const uint32_t gtile = tile_order[blockIdx.x / (SUBS * SUBS)];
+  const uint32_t sub = blockIdx.x % (SUBS * SUBS);
   const uint32_t v = n_views > 1u ? view_of(tile_base, n_views, gtile) : 0u;
   const FrameDev &f = views[v].f;
   void *__restrict__ out = views[v].out;
   const uint32_t out_format = views[v].out_format;
   const uint32_t tile = gtile - tile_base[v];
   const uint32_t tyi = tile / f.tiles_x, txi = tile - tyi * f.tiles_x;
-  constexpr uint32_t TS = 1u << TSH, WROW = TS / 8u;    // tile side; 8x8 wave blocks per tile row
+  constexpr uint32_t TS = 1u << TSH, WS = 1u << WSH, WROW = WS / 8u;   // tile side, workgroup side; 8x8 wave blocks
+  const uint32_t gx = txi * TS + (sub % SUBS) * WS, gy = tyi * TS + (sub / SUBS) * WS;   // this workgroup's block
+  if (gx >= f.width || gy >= f.height) return;           // a block wholly outside the view (uniform)
   const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
-  const uint32_t bx = txi * TS + (w % WROW) * 8u, by = tyi * TS + (w / WROW) * 8u;   // this wave's 8x8 block
+  const uint32_t bx = gx + (w % WROW) * 8u, by = gy + (w / WROW) * 8u;   // this wave's 8x8 block
   const uint32_t px = bx + (lane & 7u), py = by + (lane >> 3);
   const bool inside = px < f.width && py < f.height;
   const float fw = (float)f.width, fh = (float)f.height;

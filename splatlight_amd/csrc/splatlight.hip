@@ -668,19 +668,19 @@ sl_status enqueue_batch(sl_context *ctx, const Batch &b, bool timers, bool overl
   }
   ctx->timed_overlap = overlap;
   auto composite = [&](auto kernel, uint32_t threads) {
-    kernel<<<b.total_tiles, threads, 0, cs>>>(w.views, w.tile_base, V, w.view_counters, w.splats, vin, w.ranges,
+    kernel<<<b.total_tiles * (b.tile_shift == 5u ? 4u : 1u), threads, 0, cs>>>(w.views, w.tile_base, V, w.view_counters, w.splats, vin, w.ranges,
                                               w.tile_order);
   };
   const bool big = b.tile_shift == 5u;
   switch (tu.composite_batch) {
     case 64u:
-      big ? composite(sl_composite<64, 5>, 1024u) : composite(sl_composite<64, 4>, 256u);
+      big ? composite(sl_composite<64, 5, 4>, 256u) : composite(sl_composite<64, 4, 4>, 256u);
       break;
     case 128u:
-      big ? composite(sl_composite<128, 5>, 1024u) : composite(sl_composite<128, 4>, 256u);
+      big ? composite(sl_composite<128, 5, 4>, 256u) : composite(sl_composite<128, 4, 4>, 256u);
       break;
     default:
-      big ? composite(sl_composite<256, 5>, 1024u) : composite(sl_composite<256, 4>, 256u);
+      big ? composite(sl_composite<256, 5, 4>, 256u) : composite(sl_composite<256, 4, 4>, 256u);
   }
   if ((s = mark(7, cs)) != SL_OK) return s;
   return check_launch(ctx, "frame kernels");
