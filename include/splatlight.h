@@ -213,7 +213,7 @@ typedef struct sl_tuning {
   uint32_t radix_items;        /* keys per lane of an onesweep tile: 8 or 16 */
   uint32_t grid_mul;           /* persistent sort workgroups per CU: 1-16 */
   uint32_t radix_bits;         /* the tile sort's digit width: 8 or 11 (0: 8) */
-  uint32_t tile_px;            /* the composite tile: 16 or 32 pixels square (0: 16) */
+  uint32_t tile_px;            /* the binning tile: 16 or 32 pixels square (0: 16); composited in 16x16 blocks */
   uint32_t pad;
 } sl_tuning;
 sl_status sl_ctx_set_tuning(sl_context *ctx, const sl_tuning *tuning);

@@ -1,6 +1,6 @@
-"""Autotuner for the frame kernels (sl_tuning): the composite's tile size (16 or 32 px) and LDS batch depth, the tile
-sort's radix digit width (8 or 11 bits: 11-bit digits sort a 1080p frame's 32 px tile ids in one pass), keys per lane
-of the onesweep radix passes and persistent radix workgroups per CU.
+"""Autotuner for the frame kernels (sl_tuning): the binning tile size (16 or 32 px), the composite's LDS batch depth,
+the tile sort's radix digit width (8 or 11 bits: 11-bit digits sort a 1080p frame's 32 px tile ids in one pass), keys
+per lane of the onesweep radix passes and persistent radix workgroups per CU.
 
 Every candidate gives bit-identical results, so the tuner only times: it replays the caller's own batch as a
 hipGraph under each candidate and keeps the fastest. Choices are cached per device and workload signature in a JSON

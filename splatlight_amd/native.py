@@ -499,7 +499,8 @@ class Server:
 def set_tuning(ctx: Context, composite_batch: int, radix_items: int, grid_mul: int, radix_bits: int = 8,
                tile_px: int = 16):
     """sl_ctx_set_tuning: composite LDS batch depth, onesweep keys per lane, persistent sort workgroups per CU, the
-    tile sort's digit width (8 or 11 bits) and the composite tile (16 or 32 px)."""
+    tile sort's digit width (8 or 11 bits) and the binning tile (16 or 32 px; either is composited in 16x16-pixel
+    workgroups)."""
     ctx.check(ctx.L.sl_ctx_set_tuning(ctx.h, C.byref(Tuning(composite_batch, radix_items, grid_mul, radix_bits,
                                                              tile_px, 0))))
 
